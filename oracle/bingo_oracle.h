@@ -1,0 +1,130 @@
+/*
+ * TEST INFRASTRUCTURE (oracle): a sequential CPU restatement of Bingo's
+ * radix-based bias-factorized store, its batched updates and its walk
+ * samplers.  Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of
+ * bench.py may load it; the product library (bingo_amd/libbingo.so) never
+ * links, loads or calls it.
+ *
+ * What each part restates (reference file:line):
+ *   store layout ........ include/Bingo.cuh:6-58, 59-94 (edge slots, per-bit
+ *                         member lists group[]/idx[], masses num[])
+ *   build/load .......... include/operation.cuh:12-30, 33-121; Bingo.cuh:159-181
+ *   insert .............. operation.cuh:122-197; Bingo.cuh:182-198
+ *   delete .............. operation.cuh:198-251; Bingo.cuh:199-218
+ *   alias table ......... Bingo.cuh:111-158 (replaced by an integer-exact table)
+ *   sample .............. Bingo.cuh:220-267
+ *   DeepWalk/node2vec/PPR operation.cuh:252-368; sampling 407-430
+ *   generators .......... graph.cuh:16-92 (replaced by seeded R-MAT + update streams)
+ *
+ * Deliberate deviations from the reference (each decided, each tested):
+ *   - all 32 bit groups of a uint32 weight are kept (ref: LOGT/MAXLOG=16 cap);
+ *   - every non-empty bit group keeps a member list (ref RATE>1 mode);
+ *   - members store (slot, target) so a walk step needs no edge-record read;
+ *   - alias table: 32 fixed slots, integer-exact Vose construction with a
+ *     64-bit stay threshold (ref: float/EPS table, Bingo.cuh:111-158);
+ *   - randomness: Philox4x32-10 keyed (seed, walker, step, attempt);
+ *   - node2vec neighbour test is exact "candidate in N(prev)" (ref hash is
+ *     lossy and tests prev in N(cur), operation.cuh:313, Bingo.cuh:39-58);
+ *   - PPR has a per-walker length cap (ref overwrites slot LEN-1);
+ *   - batched delete semantics are order-free (see orc_delete).
+ */
+#ifndef BINGO_ORACLE_H
+#define BINGO_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_GROUPS 32
+#define ORC_AUX_BITS 27
+#define ORC_AUX_ONE (1u << ORC_AUX_BITS)
+#define ORC_MARK 0x80000000u
+
+enum { ORC_DEEPWALK = 0, ORC_NODE2VEC = 1, ORC_PPR = 2, ORC_SAMPLING = 3 };
+enum { ORC_W_CONST1 = 0, ORC_W_UNIFORM16 = 1, ORC_W_LOGUNI32 = 2, ORC_W_ZIPF24 = 3 };
+
+enum {
+    ORC_OK = 0,
+    ORC_EINVAL = -1,
+    ORC_ENOMEM = -2,
+};
+
+typedef struct { int32_t u, v; uint32_t w; } orc_edge;   /* ref Edges (type.h:24-29), w widened to u32 */
+typedef struct { int32_t u, slot; } orc_del;             /* ref Deleted{u,id} (type.h:33-37) */
+
+typedef struct orc_graph orc_graph;
+
+/* Walk parameters after host-side quantisation (shared definition with the
+ * product library's bingo_walk_params -> kernel thresholds). */
+typedef struct {
+    uint64_t seed;
+    uint32_t thr_return;   /* node2vec: accept if aux < thr when cand == prev */
+    uint32_t thr_neighbor; /* node2vec: accept if aux < thr when cand in N(prev) */
+    uint32_t thr_other;    /* node2vec: otherwise */
+    uint32_t thr_stop;     /* PPR: stop if aux < thr */
+    int32_t max_len;       /* columns of the path matrix */
+    int32_t max_attempts;  /* node2vec rejection cap per step */
+} orc_walk_params;
+
+/* graph lifetime */
+orc_graph *orc_create(int32_t n, int neighbor_index);
+void orc_destroy(orc_graph *g);
+int32_t orc_num_nodes(const orc_graph *g);
+
+/* updates: 0 on success or a negative code; on error nothing is changed */
+int orc_insert(orc_graph *g, const orc_edge *e, int64_t k);
+int orc_delete(orc_graph *g, const orc_del *d, int64_t k);
+
+/* walks: paths is [nwalkers][params->max_len] int32, -1 padded.
+ * starts==NULL: walker i starts at node (i % n) + 1.  lens==NULL: max_len.
+ * Returns the number of appended vertices (steps) through steps_out. */
+int orc_walk(const orc_graph *g, int app, const orc_walk_params *p, const int32_t *starts,
+             const int32_t *lens, int64_t nwalkers, int32_t *paths, uint64_t *steps_out,
+             int nthreads);
+
+/* quantise float walk parameters exactly as the product does */
+void orc_walk_thresholds(double p, double q, double tp, orc_walk_params *out);
+
+/* state export for bit-exact diffs (flat, node-major) */
+typedef struct {
+    uint64_t total_slots;    /* out: sum of nslots over nodes 0..n */
+    uint64_t total_members;  /* out: sum of bucket sizes */
+    uint32_t *nslots;        /* [n+1] */
+    uint64_t *mass;          /* [n+1] sum of live weights */
+    uint32_t *bsize;         /* [(n+1)*32] */
+    uint64_t *T;             /* [(n+1)*32] alias stay thresholds */
+    uint8_t *alias;          /* [(n+1)*32] alias partners */
+    uint32_t *slot_v;        /* [total_slots] */
+    uint32_t *slot_w;        /* [total_slots] 0 = tombstone */
+    uint64_t *members;       /* [total_members] (v<<32)|slot, node-major, group-major */
+    uint32_t *backpos;       /* [total_members] live slots in order, set bits ascending */
+} orc_dump;
+/* pass all pointers NULL to learn the totals, then again with buffers */
+int orc_export(const orc_graph *g, orc_dump *d);
+/* multiplicity of live arcs u->v (the node2vec neighbour test) */
+uint32_t orc_edge_count(const orc_graph *g, int32_t u, int32_t v);
+/* alias table of an arbitrary mass vector (tests) */
+void orc_alias_build(const uint64_t mass[32], uint64_t T[32], uint8_t alias[32]);
+/* raw Philox4x32-10 block (tests) */
+void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* one sampling draw on node u with explicit random words (tests) */
+int32_t orc_sample_words(const orc_graph *g, int32_t u, const uint32_t x[4]);
+
+/* generators (workload; mirror of the product's bingo_gen_*) */
+void orc_rmat_thresholds(double a, double b, double c, uint32_t thr[3]);
+void orc_gen_rmat(int scale, int64_t edge_begin, int64_t nedges, const uint32_t thr[3], int wkind,
+                  uint64_t seed, int undirected, orc_edge *out, int nthreads);
+uint32_t orc_gen_weight(int wkind, uint64_t seed, uint64_t e);
+
+typedef struct orc_updgen orc_updgen;
+orc_updgen *orc_updgen_create(int32_t n, int scale, const orc_edge *base, int64_t m,
+                              int64_t edge_begin, const uint32_t thr[3], int wkind, uint64_t seed);
+void orc_updgen_destroy(orc_updgen *s);
+void orc_updgen_inserts(orc_updgen *s, int64_t k, orc_edge *out);
+int orc_updgen_deletes(orc_updgen *s, int64_t k, orc_del *out);
+int64_t orc_updgen_live(const orc_updgen *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
