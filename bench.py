@@ -1,0 +1,196 @@
+"""Benchmark of the bias-factorized walk engine on one MI355X.
+
+A step is one round of the dynamic-graph workload: one insert batch, one
+delete batch, then one walk of every walker, all on HBM-resident inputs.
+
+Default workload (R-MAT 2^22 nodes / 2^26 undirected edges, u32 weights
+log-uniform in [1, 2^32), node2vec p=0.5 q=2, one walker per node of length
+80, 10^6 inserts + 10^6 deletes per step).  `value` is whole-job walk steps/s
+(walk steps / wall time of the K steps, updates included); the line also
+carries walk-only steps/s, updates/s, the walk kernel's roofline and the CPU
+oracle timed on this host's cores on a bounded sample.
+
+    python bench.py [--steps K] [--warmup W] [--config c2|c1] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "walk steps/s + edge updates/s on 1 MI355X (HBM GB/s %peak); p99 query latency"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes (per walk step / draw / probe): node record 8, alias slot
+# 8, group header 8, member 4, target 4, path write 4; a neighbour probe 8
+B_STEP, B_PROBE = 36, 8
+
+CONFIGS = {
+    "c2": dict(workload="rmat22_e26_loguni32_node2vec_upd1e6", scale=22, edges=1 << 26, wkind=2, app=1,
+               max_len=80, batch=1_000_000, p=0.5, q=2.0),
+    "c1": dict(workload="rmat17_e21_uniform16_deepwalk_upd1e4", scale=17, edges=1 << 21, wkind=1, app=0,
+               max_len=80, batch=10_000, p=0.5, q=2.0),
+}
+
+
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(base, n, cfg, seed, budget_s=15.0):
+    """The CPU oracle (a C/OpenMP restatement of the same sampler) on this
+    host's cores: full graph, a bounded sample of walkers."""
+    from oracle import oracle as orc
+
+    threads = len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    o = orc.OracleGraph(n, neighbor_index=True)
+    assert o.load(base) == 0
+    load_s = time.perf_counter() - t0
+    log(f"cpu oracle loaded in {load_s:.1f}s ({threads} threads)")
+    wp = orc.walk_params(seed=seed, p=cfg["p"], q=cfg["q"], max_len=cfg["max_len"])
+    nw = 4096
+    while True:
+        starts = (np.arange(nw, dtype=np.int64) * (n // nw) % n + 1).astype(np.int32)
+        t0 = time.perf_counter()
+        _, steps = o.walk(cfg["app"], wp, starts=starts, nthreads=threads)
+        dt = time.perf_counter() - t0
+        if dt > budget_s / 4 or nw >= n:
+            break
+        nw = min(n, nw * 4)
+    rate = steps / dt
+    o.close()
+    return {"value": rate, "unit": "walk steps/s", "cores": threads, "kind": "port",
+            "sample": f"{nw} {['deepwalk', 'node2vec', 'ppr', 'sampling'][cfg['app']]} walkers of length "
+                      f"{cfg['max_len']} spread over the same graph, {steps} steps in {dt:.2f}s "
+                      f"(load {load_s:.1f}s not counted)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--seed", type=int, default=2026)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the store is built and walked on a single MI355X; "
+              "multi-GPU sharding is a design note only. Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    import bingo_amd as B
+
+    cfg = CONFIGS[args.config]
+    scale, m = cfg["scale"], cfg["edges"]
+    n = 1 << scale
+    K, W, kb = args.steps, args.warmup, cfg["batch"]
+
+    t0 = time.perf_counter()
+    base = B.gen_rmat(scale, m, cfg["wkind"], seed=args.seed)
+    log(f"generated {len(base)} arcs in {time.perf_counter() - t0:.1f}s")
+    g = B.Graph(device=0, neighbor_index=True)
+    d_base = B.DeviceArray(g, base)
+    g.load_device(n, d_base, len(base))
+    g.sync()
+    load_ms = g.stats()["load_ms"]
+    d_base.free()
+    log(f"loaded on GPU in {load_ms:.1f} ms")
+
+    ug = B.UpdateGen(n, scale, base, m, cfg["wkind"], seed=args.seed)
+    batches = []
+    for _ in range(W + K):
+        ins, dels = ug.inserts(kb), ug.deletes(kb)
+        batches.append((B.DeviceArray(g, ins), B.DeviceArray(g, dels)))
+    log(f"staged {W + K} update batches of {kb} + {kb}")
+    d_paths = B.DeviceArray(g, nbytes=n * cfg["max_len"] * 4)
+    d_steps = B.DeviceArray(g, np.zeros(1, np.uint64))
+    wp = B.walk_params(seed=args.seed, p=cfg["p"], q=cfg["q"], max_len=cfg["max_len"], block=args.block)
+
+    def step(i):
+        di, dd = batches[i]
+        g.event(0)
+        g.insert_device(di, kb)
+        g.event(1)
+        g.delete_device(dd, kb)
+        g.event(2)
+        g.walk_device(cfg["app"], wp, n, d_paths, d_steps)
+        g.event(3)
+        st = g.stats()
+        return g.elapsed_ms(0, 1), g.elapsed_ms(1, 2), g.elapsed_ms(2, 3), st
+
+    for i in range(W):
+        step(i)
+        log(f"warmup step {i} done")
+    g.sync()
+    d_steps = B.DeviceArray(g, np.zeros(1, np.uint64))
+    ins_ms = del_ms = walk_ms = kern_ms = 0.0
+    attempts = probes = 0
+    t0 = time.perf_counter()
+    for i in range(K):
+        a, b, c, st = step(W + i)
+        ins_ms += a
+        del_ms += b
+        walk_ms += c
+        kern_ms += st["walk_kernel_ms"]
+        attempts += st["walk_attempts"]
+        probes += st["walk_probes"]
+        log(f"step {i}: insert {a:.2f} ms, delete {b:.2f} ms, walk {c:.2f} ms")
+    g.sync()
+    wall = time.perf_counter() - t0
+    steps = int(d_steps.to_host(np.uint64, (1,))[0])
+    st = g.stats()
+
+    bytes_per_launch = (B_STEP * attempts + B_PROBE * probes) / K
+    kern_s = kern_ms / K / 1e3
+    achieved = bytes_per_launch / kern_s / 1e9
+    out = {
+        "metric": METRIC,
+        "value": steps / wall,
+        "unit": "walk steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": wall * 1e3 / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": cfg["workload"], "nodes": n, "arcs": int(len(base)), "walkers": n,
+                   "walk_len": cfg["max_len"], "app": ["deepwalk", "node2vec", "ppr", "sampling"][cfg["app"]],
+                   "update_batch": kb, "p": cfg["p"], "q": cfg["q"]},
+        "walk_steps_per_s": steps / (walk_ms / 1e3),
+        "updates_per_s": 2 * kb * K / ((ins_ms + del_ms) / 1e3),
+        "insert_ms": ins_ms / K,
+        "delete_ms": del_ms / K,
+        "walk_ms": walk_ms / K,
+        "load_ms": load_ms,
+        "walk_draws_per_step": attempts / max(steps, 1),
+        "store_bytes": st["bytes_reserved"],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": f"k_walk<{cfg['app']}>", "kernel_ms": kern_ms / K,
+                     "bytes_per_launch": bytes_per_launch},
+        "cpu_baseline": None,
+    }
+    for di, dd in batches:
+        di.free()
+        dd.free()
+    d_paths.free()
+    g.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(base, n, cfg, args.seed)
+        out["gpu_vs_cpu_walk"] = out["walk_steps_per_s"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

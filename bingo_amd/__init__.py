@@ -1,0 +1,341 @@
+"""bingo_amd: Python view of the gfx950 bias-factorized random-walk engine.
+
+Everything here calls the C ABI of ``bingo_amd/libbingo.so`` (declared in
+``include/bingo.h``) through ctypes.  There is no CPU fallback: if the HIP
+library is missing or no GPU is present, the calls raise.
+
+Mapping to the reference (Joeyw02/Bingo):
+    Graph.load      build()       include/operation.cuh:33-121
+    Graph.insert    insert()      include/operation.cuh:144-197
+    Graph.delete    deleteE()     include/operation.cuh:215-251
+    Graph.walk      randomWalk()  include/graph.cuh:93-150
+    gen_rmat / UpdateGen          the generators of include/graph.cuh:16-92
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbingo.so")
+
+DEEPWALK, NODE2VEC, PPR, SAMPLING = 0, 1, 2, 3
+APPS = {"deepwalk": DEEPWALK, "node2vec": NODE2VEC, "ppr": PPR, "sampling": SAMPLING}
+W_CONST1, W_UNIFORM16, W_LOGUNI32, W_ZIPF24 = 0, 1, 2, 3
+
+EDGE_DT = np.dtype([("u", np.int32), ("v", np.int32), ("w", np.uint32)])
+DEL_DT = np.dtype([("u", np.int32), ("slot", np.int32)])
+
+ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "ESTATE", -5: "EUNSUPPORTED"}
+
+
+class BingoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("device", C.c_int32),
+        ("neighbor_index", C.c_int32),
+        ("hbm_budget", C.c_uint64),
+        ("pool_slack", C.c_double),
+        ("min_slots", C.c_int32),
+    ]
+
+
+class WalkParams(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64),
+        ("p", C.c_double),
+        ("q", C.c_double),
+        ("tp", C.c_double),
+        ("max_len", C.c_int32),
+        ("max_attempts", C.c_int32),
+        ("block", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("load_ms", C.c_double),
+        ("insert_ms", C.c_double),
+        ("delete_ms", C.c_double),
+        ("walk_ms", C.c_double),
+        ("walk_kernel_ms", C.c_double),
+        ("walk_steps", C.c_uint64),
+        ("walk_attempts", C.c_uint64),
+        ("walk_probes", C.c_uint64),
+        ("arcs", C.c_uint64),
+        ("bytes_reserved", C.c_uint64),
+        ("pool_used", C.c_uint64 * 4),
+    ]
+
+
+class Dump(C.Structure):
+    _fields_ = [
+        ("total_slots", C.c_uint64),
+        ("total_members", C.c_uint64),
+        ("nslots", C.c_void_p),
+        ("mass", C.c_void_p),
+        ("bsize", C.c_void_p),
+        ("T", C.c_void_p),
+        ("alias", C.c_void_p),
+        ("slot_v", C.c_void_p),
+        ("slot_w", C.c_void_p),
+        ("members", C.c_void_p),
+        ("backpos", C.c_void_p),
+    ]
+
+
+# every symbol include/bingo.h declares, with its ctypes signature
+_vp, _i32, _i64, _u64, _int = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64, C.c_int
+SIGNATURES = {
+    "bingo_config_default": (None, [C.POINTER(Config)]),
+    "bingo_walk_params_default": (None, [C.POINTER(WalkParams)]),
+    "bingo_create": (_int, [C.POINTER(Config), C.POINTER(_vp)]),
+    "bingo_destroy": (None, [_vp]),
+    "bingo_last_error": (C.c_char_p, []),
+    "bingo_version": (C.c_char_p, []),
+    "bingo_load": (_int, [_vp, _i32, _vp, _i64]),
+    "bingo_load_device": (_int, [_vp, _i32, _vp, _i64]),
+    "bingo_insert": (_int, [_vp, _vp, _i64]),
+    "bingo_insert_device": (_int, [_vp, _vp, _i64]),
+    "bingo_delete": (_int, [_vp, _vp, _i64]),
+    "bingo_delete_device": (_int, [_vp, _vp, _i64]),
+    "bingo_walk": (_int, [_vp, _int, C.POINTER(WalkParams), _vp, _vp, _i64, _vp, C.POINTER(_u64)]),
+    "bingo_walk_device": (_int, [_vp, _int, C.POINTER(WalkParams), _vp, _vp, _i64, _vp, _vp]),
+    "bingo_num_nodes": (_i32, [_vp]),
+    "bingo_stats_get": (_int, [_vp, C.POINTER(Stats)]),
+    "bingo_sync": (_int, [_vp]),
+    "bingo_stream": (_vp, [_vp]),
+    "bingo_edge_count": (_int, [_vp, _vp, _vp, _i64, _vp]),
+    "bingo_export": (_int, [_vp, C.POINTER(Dump)]),
+    "bingo_dev_alloc": (_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
+    "bingo_dev_free": (_int, [_vp, _vp]),
+    "bingo_memcpy_h2d": (_int, [_vp, _vp, _vp, C.c_size_t]),
+    "bingo_memcpy_d2h": (_int, [_vp, _vp, _vp, C.c_size_t]),
+    "bingo_event_record": (_int, [_vp, _int]),
+    "bingo_event_elapsed": (_int, [_vp, _int, _int, C.POINTER(C.c_float)]),
+    "bingo_gen_rmat": (_int, [_int, _i64, _i64, C.c_double, C.c_double, C.c_double, _int, _u64, _int, _vp]),
+    "bingo_updgen_create": (_int, [_i32, _int, _vp, _i64, _i64, C.c_double, C.c_double, C.c_double, _int, _u64,
+                                   C.POINTER(_vp)]),
+    "bingo_updgen_destroy": (None, [_vp]),
+    "bingo_updgen_inserts": (_int, [_vp, _i64, _vp]),
+    "bingo_updgen_deletes": (_int, [_vp, _i64, _vp]),
+    "bingo_updgen_live": (_i64, [_vp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libbingo.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run `python -m bingo_amd.build` (hipcc, gfx950)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _check(rc):
+    if rc != 0:
+        raise BingoError(rc, lib().bingo_last_error().decode())
+    return rc
+
+
+def walk_params(seed=1, p=0.5, q=2.0, tp=0.0125, max_len=80, max_attempts=4096, block=0) -> WalkParams:
+    wp = WalkParams()
+    lib().bingo_walk_params_default(C.byref(wp))
+    wp.seed, wp.p, wp.q, wp.tp = seed, p, q, tp
+    wp.max_len, wp.max_attempts, wp.block = max_len, max_attempts, block
+    return wp
+
+
+def gen_rmat(scale, nedges, wkind, seed, undirected=True, edge_begin=0, abc=(0.57, 0.19, 0.19)):
+    out = np.zeros(nedges * (2 if undirected else 1), EDGE_DT)
+    _check(lib().bingo_gen_rmat(scale, edge_begin, nedges, *abc, wkind, seed, int(undirected), _ptr(out)))
+    return out
+
+
+class UpdateGen:
+    """Seeded insert/delete stream over a loaded arc list (slots in load order)."""
+
+    def __init__(self, n, scale, base_edges, edge_begin, wkind, seed, abc=(0.57, 0.19, 0.19)):
+        base = np.ascontiguousarray(base_edges, EDGE_DT)
+        h = C.c_void_p()
+        _check(lib().bingo_updgen_create(n, scale, _ptr(base), len(base), edge_begin, *abc, wkind, seed, C.byref(h)))
+        self._h = h
+
+    def inserts(self, k):
+        out = np.zeros(k, EDGE_DT)
+        _check(lib().bingo_updgen_inserts(self._h, k, _ptr(out)))
+        return out
+
+    def deletes(self, k):
+        out = np.zeros(k, DEL_DT)
+        _check(lib().bingo_updgen_deletes(self._h, k, _ptr(out)))
+        return out
+
+    def live(self):
+        return lib().bingo_updgen_live(self._h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().bingo_updgen_destroy(self._h)
+            self._h = None
+
+
+class DeviceArray:
+    """A device allocation owned through a graph handle (HBM-resident inputs)."""
+
+    def __init__(self, graph: "Graph", host: np.ndarray | None = None, nbytes: int | None = None):
+        self.g = graph
+        self.nbytes = int(host.nbytes if host is not None else nbytes)
+        p = C.c_void_p()
+        _check(lib().bingo_dev_alloc(graph._h, self.nbytes, C.byref(p)))
+        self.ptr = p
+        if host is not None and self.nbytes:
+            _check(lib().bingo_memcpy_h2d(graph._h, self.ptr, _ptr(np.ascontiguousarray(host)), self.nbytes))
+
+    def to_host(self, dtype, shape):
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        _check(lib().bingo_memcpy_d2h(self.g._h, _ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().bingo_dev_free(self.g._h, self.ptr)
+            self.ptr = None
+
+
+class Graph:
+    """One HBM-resident dynamic graph on one GPU (ids 1..n)."""
+
+    def __init__(self, device=0, neighbor_index=True, hbm_budget=0, pool_slack=0.25, min_slots=16):
+        cfg = Config()
+        lib().bingo_config_default(C.byref(cfg))
+        cfg.device, cfg.neighbor_index, cfg.hbm_budget = device, int(neighbor_index), hbm_budget
+        cfg.pool_slack, cfg.min_slots = pool_slack, min_slots
+        h = C.c_void_p()
+        _check(lib().bingo_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().bingo_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- updates --
+    def load(self, n, edges):
+        e = np.ascontiguousarray(edges, EDGE_DT)
+        _check(lib().bingo_load(self._h, n, _ptr(e), len(e)))
+        self.n = n
+
+    def load_device(self, n, d_edges: DeviceArray, m):
+        _check(lib().bingo_load_device(self._h, n, d_edges.ptr, m))
+        self.n = n
+
+    def insert(self, edges):
+        e = np.ascontiguousarray(edges, EDGE_DT)
+        _check(lib().bingo_insert(self._h, _ptr(e), len(e)))
+
+    def insert_device(self, d_edges: DeviceArray, k):
+        _check(lib().bingo_insert_device(self._h, d_edges.ptr, k))
+
+    def delete(self, dels):
+        d = np.ascontiguousarray(dels, DEL_DT)
+        _check(lib().bingo_delete(self._h, _ptr(d), len(d)))
+
+    def delete_device(self, d_dels: DeviceArray, k):
+        _check(lib().bingo_delete_device(self._h, d_dels.ptr, k))
+
+    # -- walks --
+    def walk(self, app, params: WalkParams, starts=None, lens=None, nwalkers=None):
+        if starts is not None:
+            starts = np.ascontiguousarray(starts, np.int32)
+            nwalkers = len(starts)
+        if lens is not None:
+            lens = np.ascontiguousarray(lens, np.int32)
+        if nwalkers is None:
+            nwalkers = self.n
+        paths = np.empty((nwalkers, params.max_len), np.int32)
+        steps = C.c_uint64(0)
+        _check(lib().bingo_walk(self._h, app, C.byref(params), _ptr(starts), _ptr(lens), nwalkers, _ptr(paths),
+                                C.byref(steps)))
+        return paths, steps.value
+
+    def walk_device(self, app, params: WalkParams, nwalkers, d_paths: DeviceArray, d_steps: DeviceArray,
+                    d_starts: DeviceArray | None = None, d_lens: DeviceArray | None = None):
+        _check(lib().bingo_walk_device(self._h, app, C.byref(params), d_starts.ptr if d_starts else None,
+                                       d_lens.ptr if d_lens else None, nwalkers, d_paths.ptr, d_steps.ptr))
+
+    # -- inspection --
+    def sync(self):
+        _check(lib().bingo_sync(self._h))
+
+    def stats(self):
+        s = Stats()
+        _check(lib().bingo_stats_get(self._h, C.byref(s)))
+        d = {k: getattr(s, k) for k, _ in Stats._fields_ if k != "pool_used"}
+        d["pool_used"] = list(s.pool_used)
+        return d
+
+    def event(self, slot):
+        _check(lib().bingo_event_record(self._h, slot))
+
+    def elapsed_ms(self, a, b):
+        ms = C.c_float(0)
+        _check(lib().bingo_event_elapsed(self._h, a, b, C.byref(ms)))
+        return ms.value
+
+    def edge_count(self, us, vs):
+        us = np.ascontiguousarray(us, np.int32)
+        vs = np.ascontiguousarray(vs, np.int32)
+        out = np.zeros(len(us), np.uint32)
+        _check(lib().bingo_edge_count(self._h, _ptr(us), _ptr(vs), len(us), _ptr(out)))
+        return out
+
+    def export(self):
+        d = Dump()
+        _check(lib().bingo_export(self._h, C.byref(d)))
+        n1 = self.n + 1
+        out = dict(
+            nslots=np.zeros(n1, np.uint32),
+            mass=np.zeros(n1, np.uint64),
+            bsize=np.zeros((n1, 32), np.uint32),
+            T=np.zeros((n1, 32), np.uint64),
+            alias=np.zeros((n1, 32), np.uint8),
+            slot_v=np.zeros(d.total_slots, np.uint32),
+            slot_w=np.zeros(d.total_slots, np.uint32),
+            members=np.zeros(d.total_members, np.uint64),
+            backpos=np.zeros(d.total_members, np.uint32),
+        )
+        for k, a in out.items():
+            setattr(d, k, a.ctypes.data)
+        _check(lib().bingo_export(self._h, C.byref(d)))
+        return out

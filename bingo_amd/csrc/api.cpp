@@ -1,0 +1,753 @@
+// Host runtime behind include/bingo.h: handle, HBM pools under a budget,
+// batch staging, sort + run encoding of update batches, kernel launches.
+//
+// Reference roles: build/insert/deleteE host drivers (include/operation.cuh:
+// 33-121, 144-197, 215-251), randomWalk (include/graph.cuh:93-150) and the
+// MemPool bump allocator (include/gpuMem.cuh:4-32).  Differences by design:
+// no per-node host round trips (capacity growth is planned on the device and
+// applied in one pass), errors are returned instead of exit(), and all work
+// is ordered on one HIP stream per handle.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bingo.h"
+#include "store.h"
+
+using namespace bingo;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(call)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) return fail(BINGO_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+namespace {
+
+enum Scratch {
+    SC_KEYS,
+    SC_KEYS2,
+    SC_VALS,
+    SC_VALS2,
+    SC_EDGES,
+    SC_UNIQ,
+    SC_COUNTS,
+    SC_STARTS,
+    SC_NRUNS,
+    SC_TMP,
+    SC_U64A,
+    SC_U64B,
+    SC_STAGE,
+    SC_AUX1,
+    SC_AUX2,
+    SC_AUX3,
+    NSCRATCH
+};
+
+constexpr size_t kElem[NPOOL] = {4, 4, 8, 8};
+constexpr int kEvents = 16;
+
+enum Phase { PH_LOAD, PH_INSERT, PH_DELETE, PH_WALK, NPHASE };
+
+}  // namespace
+
+struct bingo_graph {
+    bingo_config cfg{};
+    int device = 0;
+    hipStream_t st = nullptr;
+    DevStore s{};
+    bool loaded = false;
+    int64_t arcs = 0;
+    size_t node_bytes = 0;
+    uint64_t budget = 0;
+    Ctl *ctl_host = nullptr;  // pinned
+    void *sc[NSCRATCH] = {};
+    size_t sc_bytes[NSCRATCH] = {};
+    hipEvent_t ev[kEvents] = {};
+    hipEvent_t ph[NPHASE][2] = {};
+    bool ph_pending[NPHASE] = {};
+    double ph_ms[NPHASE] = {};
+    uint64_t last_walk_steps = 0;
+    unsigned long long *d_steps = nullptr;
+};
+
+static int ensure_scratch(bingo_graph *g, int idx, size_t bytes)
+{
+    if (bytes == 0) bytes = 8;
+    if (g->sc_bytes[idx] >= bytes) return BINGO_OK;
+    if (g->sc[idx]) {
+        HIPCHK(hipStreamSynchronize(g->st));
+        HIPCHK(hipFree(g->sc[idx]));
+        g->sc[idx] = nullptr;
+        g->sc_bytes[idx] = 0;
+    }
+    size_t want = bytes + bytes / 4;
+    if (hipMalloc(&g->sc[idx], want) != hipSuccess) {
+        (void)hipGetLastError();
+        g->sc[idx] = nullptr;
+        return fail(BINGO_ENOMEM, "scratch allocation of " + std::to_string(want) + " bytes failed");
+    }
+    g->sc_bytes[idx] = want;
+    return BINGO_OK;
+}
+
+template <class T>
+static T *scp(bingo_graph *g, int idx)
+{
+    return static_cast<T *>(g->sc[idx]);
+}
+
+static uint64_t store_bytes(const bingo_graph *g)
+{
+    uint64_t b = g->node_bytes;
+    for (int p = 0; p < NPOOL; ++p) b += g->s.cap[p] * kElem[p];
+    return b;
+}
+
+static void phase_begin(bingo_graph *g, int ph) { (void)hipEventRecord(g->ph[ph][0], g->st); }
+static void phase_end(bingo_graph *g, int ph)
+{
+    (void)hipEventRecord(g->ph[ph][1], g->st);
+    g->ph_pending[ph] = true;
+}
+static void phase_collect(bingo_graph *g)
+{
+    for (int p = 0; p < NPHASE; ++p) {
+        if (!g->ph_pending[p]) continue;
+        float ms = 0.f;
+        if (hipEventSynchronize(g->ph[p][1]) == hipSuccess &&
+            hipEventElapsedTime(&ms, g->ph[p][0], g->ph[p][1]) == hipSuccess)
+            g->ph_ms[p] = ms;
+        g->ph_pending[p] = false;
+    }
+}
+
+static int read_ctl(bingo_graph *g)
+{
+    HIPCHK(hipMemcpyAsync(g->ctl_host, g->s.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, g->st));
+    HIPCHK(hipStreamSynchronize(g->st));
+    return BINGO_OK;
+}
+
+// Grow pools so that top + need fits; a grown pool is a new allocation with
+// the old contents copied (region offsets stay valid).  Refuses growth past
+// the HBM budget with BINGO_ENOMEM before anything is modified.
+static int ensure_pools(bingo_graph *g)
+{
+    const Ctl &c = *g->ctl_host;
+    uint64_t newcap[NPOOL];
+    uint64_t total = g->node_bytes;
+    for (int p = 0; p < NPOOL; ++p) {
+        const uint64_t want = c.top[p] + c.need[p];
+        newcap[p] = g->s.cap[p];
+        if (want > g->s.cap[p]) {
+            const uint64_t slack = (uint64_t)((double)want * g->cfg.pool_slack);
+            newcap[p] = std::max<uint64_t>(want + slack + 1024, g->s.cap[p] + g->s.cap[p] / 2);
+        }
+        total += newcap[p] * kElem[p];
+    }
+    if (total > g->budget) {  // retry without slack
+        total = g->node_bytes;
+        for (int p = 0; p < NPOOL; ++p) {
+            const uint64_t want = c.top[p] + c.need[p];
+            if (want > g->s.cap[p]) newcap[p] = want;
+            total += newcap[p] * kElem[p];
+        }
+        if (total > g->budget)
+            return fail(BINGO_ENOMEM, "store would need " + std::to_string(total) + " bytes, budget is " +
+                                          std::to_string(g->budget));
+    }
+    for (int p = 0; p < NPOOL; ++p) {
+        if (newcap[p] == g->s.cap[p]) continue;
+        void *fresh = nullptr;
+        if (hipMalloc(&fresh, newcap[p] * kElem[p]) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(BINGO_ENOMEM, "pool allocation of " + std::to_string(newcap[p] * kElem[p]) + " bytes failed");
+        }
+        void **slotp = p == POOL_SLOT ? (void **)&g->s.slot
+                       : p == POOL_BACK ? (void **)&g->s.back
+                       : p == POOL_MEM  ? (void **)&g->s.mem
+                                        : (void **)&g->s.hash;
+        if (*slotp) {
+            if (c.top[p])
+                HIPCHK(hipMemcpyAsync(fresh, *slotp, c.top[p] * kElem[p], hipMemcpyDeviceToDevice, g->st));
+            HIPCHK(hipStreamSynchronize(g->st));
+            HIPCHK(hipFree(*slotp));
+        }
+        *slotp = fresh;
+        g->s.cap[p] = newcap[p];
+    }
+    return BINGO_OK;
+}
+
+static int bits_for(uint64_t maxval)
+{
+    int b = 1;
+    while (b < 64 && (maxval >> b)) ++b;
+    return b;
+}
+
+// sort by node, encode runs; keys (u32 node ids, unsorted) are in SC_KEYS
+// and values in SC_VALS; outputs SC_KEYS2/SC_VALS2 and the Runs arrays.
+static int encode_runs(bingo_graph *g, const uint32_t *keys_sorted, int64_t k, Runs *r)
+{
+    int rc;
+    size_t tb = run_length_encode_u32(nullptr, 0, keys_sorted, k, scp<uint32_t>(g, SC_UNIQ),
+                                      scp<uint32_t>(g, SC_COUNTS), scp<uint32_t>(g, SC_NRUNS), g->st);
+    tb = std::max(tb, exclusive_scan_u32(nullptr, 0, scp<uint32_t>(g, SC_COUNTS), scp<uint32_t>(g, SC_STARTS), k, g->st));
+    if ((rc = ensure_scratch(g, SC_TMP, tb))) return rc;
+    run_length_encode_u32(g->sc[SC_TMP], g->sc_bytes[SC_TMP], keys_sorted, k, scp<uint32_t>(g, SC_UNIQ),
+                          scp<uint32_t>(g, SC_COUNTS), scp<uint32_t>(g, SC_NRUNS), g->st);
+    exclusive_scan_u32(g->sc[SC_TMP], g->sc_bytes[SC_TMP], scp<uint32_t>(g, SC_COUNTS), scp<uint32_t>(g, SC_STARTS), k,
+                       g->st);
+    HIPCHK(hipGetLastError());
+    r->node = scp<uint32_t>(g, SC_UNIQ);
+    r->start = scp<uint32_t>(g, SC_STARTS);
+    r->count = scp<uint32_t>(g, SC_COUNTS);
+    r->nruns = scp<uint32_t>(g, SC_NRUNS);
+    return BINGO_OK;
+}
+
+static int ensure_batch_scratch(bingo_graph *g, int64_t k)
+{
+    int rc;
+    const size_t k4 = (size_t)k * 4, k8 = (size_t)k * 8;
+    if ((rc = ensure_scratch(g, SC_KEYS, k4)) || (rc = ensure_scratch(g, SC_KEYS2, k4)) ||
+        (rc = ensure_scratch(g, SC_VALS, k4)) || (rc = ensure_scratch(g, SC_VALS2, k4)) ||
+        (rc = ensure_scratch(g, SC_UNIQ, k4)) || (rc = ensure_scratch(g, SC_COUNTS, k4)) ||
+        (rc = ensure_scratch(g, SC_STARTS, k4)) || (rc = ensure_scratch(g, SC_NRUNS, 16)) ||
+        (rc = ensure_scratch(g, SC_U64A, k8)) || (rc = ensure_scratch(g, SC_U64B, k8)) ||
+        (rc = ensure_scratch(g, SC_EDGES, (size_t)k * sizeof(DevEdge))))
+        return rc;
+    return BINGO_OK;
+}
+
+static int do_insert(bingo_graph *g, const DevEdge *d_e, int64_t k)
+{
+    int rc;
+    if (k < 0 || k >= (1ll << 31)) return fail(BINGO_EINVAL, "batch size out of range");
+    if (k == 0) return BINGO_OK;
+    if ((rc = ensure_batch_scratch(g, k))) return rc;
+    launch_reset_ctl(g->s.ctl, g->st);
+    launch_insert_keys(d_e, k, g->s.n, scp<uint32_t>(g, SC_KEYS), scp<uint32_t>(g, SC_VALS), g->s.ctl, g->st);
+    const int bits = bits_for((uint64_t)g->s.n);
+    size_t tb = sort_pairs_u32(nullptr, 0, scp<uint32_t>(g, SC_KEYS), scp<uint32_t>(g, SC_KEYS2),
+                               scp<uint32_t>(g, SC_VALS), scp<uint32_t>(g, SC_VALS2), k, bits, g->st);
+    if ((rc = ensure_scratch(g, SC_TMP, tb))) return rc;
+    sort_pairs_u32(g->sc[SC_TMP], g->sc_bytes[SC_TMP], scp<uint32_t>(g, SC_KEYS), scp<uint32_t>(g, SC_KEYS2),
+                   scp<uint32_t>(g, SC_VALS), scp<uint32_t>(g, SC_VALS2), k, bits, g->st);
+    launch_gather_edges(d_e, scp<uint32_t>(g, SC_VALS2), k, scp<DevEdge>(g, SC_EDGES), g->st);
+    Runs r;
+    if ((rc = encode_runs(g, scp<uint32_t>(g, SC_KEYS2), k, &r))) return rc;
+    launch_insert_plan(g->s, scp<DevEdge>(g, SC_EDGES), r, k, g->st);
+    HIPCHK(hipGetLastError());
+    if ((rc = read_ctl(g))) return rc;
+    if (g->ctl_host->error & ERR_INVAL) return fail(BINGO_EINVAL, "insert batch has an arc with u or v outside 1..n or w == 0");
+    if (g->ctl_host->error & ERR_NOMEM) return fail(BINGO_ENOMEM, "a node would exceed the per-node slot limit");
+    if (g->ctl_host->error) return fail(BINGO_EDEVICE, "store reported an internal error");
+    if ((rc = ensure_pools(g))) return rc;
+    launch_insert_apply(g->s, scp<DevEdge>(g, SC_EDGES), r, k, g->st);
+    HIPCHK(hipGetLastError());
+    g->arcs += k;
+    return BINGO_OK;
+}
+
+static int do_delete(bingo_graph *g, const int32_t *d_pairs, int64_t k)
+{
+    int rc;
+    if (k < 0 || k >= (1ll << 31)) return fail(BINGO_EINVAL, "batch size out of range");
+    if (k == 0) return BINGO_OK;
+    if ((rc = ensure_batch_scratch(g, k))) return rc;
+    launch_reset_ctl(g->s.ctl, g->st);
+    unsigned long long *ka = scp<unsigned long long>(g, SC_U64A), *kb = scp<unsigned long long>(g, SC_U64B);
+    launch_delete_keys(d_pairs, k, ka, g->st);
+    const int bits = 32 + bits_for((uint64_t)g->s.n);
+    size_t tb = sort_keys_u64(nullptr, 0, ka, kb, k, bits, g->st);
+    if ((rc = ensure_scratch(g, SC_TMP, tb))) return rc;
+    sort_keys_u64(g->sc[SC_TMP], g->sc_bytes[SC_TMP], ka, kb, k, bits, g->st);
+    launch_delete_check(g->s, kb, k, scp<uint32_t>(g, SC_KEYS2), scp<uint32_t>(g, SC_VALS2), g->st);
+    Runs r;
+    if ((rc = encode_runs(g, scp<uint32_t>(g, SC_KEYS2), k, &r))) return rc;
+    if ((rc = read_ctl(g))) return rc;
+    if (g->ctl_host->error & ERR_INVAL)
+        return fail(BINGO_EINVAL, "delete batch names a node outside 1..n, a slot that is not live, or a slot twice");
+    if (g->ctl_host->error) return fail(BINGO_EDEVICE, "store reported an internal error");
+    launch_delete_apply(g->s, scp<uint32_t>(g, SC_VALS2), r, k, g->st);
+    HIPCHK(hipGetLastError());
+    g->arcs -= k;
+    return BINGO_OK;
+}
+
+static uint32_t thr27(double prob)
+{
+    if (!(prob > 0)) return 0;
+    if (prob >= 1.0) return 1u << 27;
+    return (uint32_t)std::floor(prob * 134217728.0);
+}
+
+static int fill_walk_args(const bingo_walk_params *p, int app, WalkArgs *a)
+{
+    if (!p) return fail(BINGO_EINVAL, "null walk params");
+    if (app < 0 || app > 3) return fail(BINGO_EINVAL, "unknown app");
+    if (p->max_len < 1) return fail(BINGO_EINVAL, "max_len must be >= 1");
+    if (app == BINGO_NODE2VEC && (p->max_attempts < 1 || !(p->p > 0) || !(p->q > 0)))
+        return fail(BINGO_EINVAL, "node2vec needs p, q > 0 and max_attempts >= 1");
+    if (app == BINGO_PPR && !(p->tp >= 0 && p->tp <= 1)) return fail(BINGO_EINVAL, "tp must be in [0, 1]");
+    // node2vec: accept alpha / max(1/p, 1, 1/q) with alpha = 1/p, 1, 1/q
+    // (ref operation.cuh:291,302-325; lim there is max(1/p, 1/q))
+    const double ip = 1.0 / p->p, iq = 1.0 / p->q;
+    double lim = std::max(1.0, std::max(ip, iq));
+    a->thr_return = thr27(ip / lim);
+    a->thr_neighbor = thr27(1.0 / lim);
+    a->thr_other = thr27(iq / lim);
+    a->thr_stop = thr27(p->tp);
+    a->seed = p->seed;
+    a->max_len = p->max_len;
+    a->max_attempts = p->max_attempts;
+    return BINGO_OK;
+}
+
+// ------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------
+extern "C" {
+
+void bingo_config_default(bingo_config *cfg)
+{
+    if (!cfg) return;
+    cfg->device = 0;
+    cfg->neighbor_index = 1;
+    cfg->hbm_budget = 0;
+    cfg->pool_slack = 0.25;
+    cfg->min_slots = 16;  // ref ORIGINAL (include/api.h:13)
+}
+
+void bingo_walk_params_default(bingo_walk_params *p)
+{
+    if (!p) return;
+    p->seed = 1;
+    p->p = 0.5;          // ref P (include/api.h:8)
+    p->q = 2.0;          // ref Q (include/api.h:9)
+    p->tp = 0.0125;      // ref TP (include/api.h:20)
+    p->max_len = 80;     // ref LEN (include/api.h:12)
+    p->max_attempts = 4096;
+    p->block = 0;
+    p->reserved = 0;
+}
+
+const char *bingo_last_error(void) { return g_err.c_str(); }
+const char *bingo_version(void) { return "bingo-amd 0.1 gfx950"; }
+
+int bingo_create(const bingo_config *cfg, bingo_graph **out)
+{
+    if (!out) return fail(BINGO_EINVAL, "null output handle");
+    *out = nullptr;
+    bingo_graph *g = new bingo_graph();
+    if (cfg) g->cfg = *cfg; else bingo_config_default(&g->cfg);
+    if (g->cfg.min_slots < 1 || g->cfg.min_slots > (1 << 20) || (g->cfg.min_slots & (g->cfg.min_slots - 1))) {
+        delete g;
+        return fail(BINGO_EINVAL, "min_slots must be a power of two in [1, 2^20]");
+    }
+    if (!(g->cfg.pool_slack >= 0) || g->cfg.pool_slack > 16) g->cfg.pool_slack = 0.25;
+    g->device = g->cfg.device;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        delete g;
+        return fail(BINGO_EDEVICE, "no HIP device");
+    }
+    if (g->device < 0 || g->device >= ndev) {
+        delete g;
+        return fail(BINGO_EINVAL, "device ordinal out of range");
+    }
+    if (hipSetDevice(g->device) != hipSuccess || hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking) != hipSuccess) {
+        delete g;
+        return fail(BINGO_EDEVICE, "cannot create HIP stream");
+    }
+    size_t fr = 0, tot = 0;
+    (void)hipMemGetInfo(&fr, &tot);
+    g->budget = g->cfg.hbm_budget ? g->cfg.hbm_budget : (uint64_t)(0.9 * (double)fr);
+    bool ok = hipHostMalloc((void **)&g->ctl_host, sizeof(Ctl), hipHostMallocDefault) == hipSuccess &&
+              hipMalloc((void **)&g->s.ctl, sizeof(Ctl)) == hipSuccess &&
+              hipMalloc((void **)&g->d_steps, sizeof(unsigned long long)) == hipSuccess;
+    for (int i = 0; ok && i < kEvents; ++i) ok = hipEventCreate(&g->ev[i]) == hipSuccess;
+    for (int p = 0; ok && p < NPHASE; ++p)
+        ok = hipEventCreate(&g->ph[p][0]) == hipSuccess && hipEventCreate(&g->ph[p][1]) == hipSuccess;
+    if (!ok) {
+        bingo_destroy(g);
+        return fail(BINGO_EDEVICE, "handle allocation failed");
+    }
+    (void)hipMemset(g->s.ctl, 0, sizeof(Ctl));
+    g->s.min_slots = g->cfg.min_slots;
+    g->s.nbr = g->cfg.neighbor_index ? 1 : 0;
+    *out = g;
+    return BINGO_OK;
+}
+
+void bingo_destroy(bingo_graph *g)
+{
+    if (!g) return;
+    if (g->st) (void)hipStreamSynchronize(g->st);
+    void *bufs[] = {g->s.hdr, g->s.grp, g->s.alias, g->s.hinfo, g->s.slot, g->s.back, g->s.mem, g->s.hash, g->s.ctl,
+                    g->d_steps};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    for (int i = 0; i < NSCRATCH; ++i)
+        if (g->sc[i]) (void)hipFree(g->sc[i]);
+    for (int i = 0; i < kEvents; ++i)
+        if (g->ev[i]) (void)hipEventDestroy(g->ev[i]);
+    for (int p = 0; p < NPHASE; ++p)
+        for (int e = 0; e < 2; ++e)
+            if (g->ph[p][e]) (void)hipEventDestroy(g->ph[p][e]);
+    if (g->ctl_host) (void)hipHostFree(g->ctl_host);
+    if (g->st) (void)hipStreamDestroy(g->st);
+    delete g;
+}
+
+int bingo_load_device(bingo_graph *g, int32_t n, const bingo_edge *d_edges, int64_t m)
+{
+    if (!g) return fail(BINGO_EINVAL, "null handle");
+    if (g->loaded) return fail(BINGO_ESTATE, "graph already loaded; create a new handle");
+    if (n < 1 || n >= (1 << 30)) return fail(BINGO_EINVAL, "n must be in [1, 2^30)");
+    if (m < 0 || (m > 0 && !d_edges)) return fail(BINGO_EINVAL, "bad edge array");
+    HIPCHK(hipSetDevice(g->device));
+    const size_t n1 = (size_t)n + 1;
+    const size_t nb = n1 * sizeof(NodeHdr) + n1 * kGroups * (sizeof(GroupHdr) + sizeof(AliasEnt)) + n1 * 8;
+    if (nb > g->budget) return fail(BINGO_ENOMEM, "node records exceed the HBM budget");
+    bool ok = hipMalloc((void **)&g->s.hdr, n1 * sizeof(NodeHdr)) == hipSuccess &&
+              hipMalloc((void **)&g->s.grp, n1 * kGroups * sizeof(GroupHdr)) == hipSuccess &&
+              hipMalloc((void **)&g->s.alias, n1 * kGroups * sizeof(AliasEnt)) == hipSuccess &&
+              hipMalloc((void **)&g->s.hinfo, n1 * 8) == hipSuccess;
+    if (!ok) {
+        (void)hipGetLastError();
+        return fail(BINGO_ENOMEM, "node record allocation failed");
+    }
+    g->node_bytes = nb;
+    g->s.n = n;
+    phase_begin(g, PH_LOAD);
+    launch_init_nodes(g->s, g->st);
+    HIPCHK(hipGetLastError());
+    g->loaded = true;
+    int rc = do_insert(g, reinterpret_cast<const DevEdge *>(d_edges), m);
+    phase_end(g, PH_LOAD);
+    if (rc) {
+        g->loaded = false;  // leave a handle that can be destroyed, not reused
+        return rc;
+    }
+    return BINGO_OK;
+}
+
+int bingo_load(bingo_graph *g, int32_t n, const bingo_edge *edges, int64_t m)
+{
+    if (!g) return fail(BINGO_EINVAL, "null handle");
+    if (m < 0 || (m > 0 && !edges)) return fail(BINGO_EINVAL, "bad edge array");
+    int rc;
+    if ((rc = ensure_scratch(g, SC_STAGE, (size_t)m * sizeof(bingo_edge)))) return rc;
+    if (m) HIPCHK(hipMemcpyAsync(g->sc[SC_STAGE], edges, (size_t)m * sizeof(bingo_edge), hipMemcpyHostToDevice, g->st));
+    rc = bingo_load_device(g, n, static_cast<const bingo_edge *>(g->sc[SC_STAGE]), m);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(g->st));
+    return BINGO_OK;
+}
+
+int bingo_insert_device(bingo_graph *g, const bingo_edge *d_edges, int64_t k)
+{
+    if (!g || !g->loaded) return fail(BINGO_ESTATE, "graph not loaded");
+    if (k > 0 && !d_edges) return fail(BINGO_EINVAL, "null edges");
+    phase_begin(g, PH_INSERT);
+    int rc = do_insert(g, reinterpret_cast<const DevEdge *>(d_edges), k);
+    phase_end(g, PH_INSERT);
+    return rc;
+}
+
+int bingo_insert(bingo_graph *g, const bingo_edge *edges, int64_t k)
+{
+    if (!g || !g->loaded) return fail(BINGO_ESTATE, "graph not loaded");
+    if (k < 0 || (k > 0 && !edges)) return fail(BINGO_EINVAL, "bad edge array");
+    int rc;
+    if ((rc = ensure_scratch(g, SC_STAGE, (size_t)k * sizeof(bingo_edge)))) return rc;
+    if (k) HIPCHK(hipMemcpyAsync(g->sc[SC_STAGE], edges, (size_t)k * sizeof(bingo_edge), hipMemcpyHostToDevice, g->st));
+    rc = bingo_insert_device(g, static_cast<const bingo_edge *>(g->sc[SC_STAGE]), k);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(g->st));
+    return BINGO_OK;
+}
+
+int bingo_delete_device(bingo_graph *g, const bingo_del *d_dels, int64_t k)
+{
+    if (!g || !g->loaded) return fail(BINGO_ESTATE, "graph not loaded");
+    if (k > 0 && !d_dels) return fail(BINGO_EINVAL, "null deletes");
+    phase_begin(g, PH_DELETE);
+    int rc = do_delete(g, reinterpret_cast<const int32_t *>(d_dels), k);
+    phase_end(g, PH_DELETE);
+    return rc;
+}
+
+int bingo_delete(bingo_graph *g, const bingo_del *dels, int64_t k)
+{
+    if (!g || !g->loaded) return fail(BINGO_ESTATE, "graph not loaded");
+    if (k < 0 || (k > 0 && !dels)) return fail(BINGO_EINVAL, "bad delete array");
+    int rc;
+    if ((rc = ensure_scratch(g, SC_STAGE, (size_t)k * sizeof(bingo_del)))) return rc;
+    if (k) HIPCHK(hipMemcpyAsync(g->sc[SC_STAGE], dels, (size_t)k * sizeof(bingo_del), hipMemcpyHostToDevice, g->st));
+    rc = bingo_delete_device(g, static_cast<const bingo_del *>(g->sc[SC_STAGE]), k);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(g->st));
+    return BINGO_OK;
+}
+
+int bingo_walk_device(bingo_graph *g, int app, const bingo_walk_params *params, const int32_t *d_starts,
+                      const int32_t *d_lens, int64_t nwalkers, int32_t *d_paths, uint64_t *d_steps)
+{
+    if (!g || !g->loaded) return fail(BINGO_ESTATE, "graph not loaded");
+    if (nwalkers < 0 || (nwalkers > 0 && (!d_paths || !d_steps))) return fail(BINGO_EINVAL, "bad walk buffers");
+    if (app == BINGO_NODE2VEC && !g->s.nbr) return fail(BINGO_EUNSUPPORTED, "node2vec needs neighbor_index=1");
+    WalkArgs a{};
+    int rc = fill_walk_args(params, app, &a);
+    if (rc) return rc;
+    if (nwalkers == 0) return BINGO_OK;
+    a.starts = d_starts;
+    a.lens = d_lens;
+    a.paths = d_paths;
+    a.steps = reinterpret_cast<unsigned long long *>(d_steps);
+    a.nwalkers = nwalkers;
+    launch_reset_walk(g->s.ctl, g->st);
+    phase_begin(g, PH_WALK);
+    launch_walk(g->s, app, a, params->block, g->st);
+    phase_end(g, PH_WALK);
+    HIPCHK(hipGetLastError());
+    return BINGO_OK;
+}
+
+int bingo_walk(bingo_graph *g, int app, const bingo_walk_params *params, const int32_t *starts, const int32_t *lens,
+               int64_t nwalkers, int32_t *paths, uint64_t *steps_out)
+{
+    if (!g || !g->loaded) return fail(BINGO_ESTATE, "graph not loaded");
+    if (!params) return fail(BINGO_EINVAL, "null walk params");
+    if (nwalkers < 0 || (nwalkers > 0 && !paths)) return fail(BINGO_EINVAL, "bad path buffer");
+    for (int64_t i = 0; starts && i < nwalkers; ++i)
+        if (starts[i] < 1 || starts[i] > g->s.n) return fail(BINGO_EINVAL, "start node outside 1..n");
+    for (int64_t i = 0; lens && i < nwalkers; ++i)
+        if (lens[i] < 1 || lens[i] > params->max_len) return fail(BINGO_EINVAL, "walk length outside 1..max_len");
+    int rc;
+    const size_t pbytes = (size_t)nwalkers * (size_t)std::max(params->max_len, 1) * 4;
+    if ((rc = ensure_scratch(g, SC_AUX1, pbytes)) || (rc = ensure_scratch(g, SC_AUX2, (size_t)nwalkers * 4)) ||
+        (rc = ensure_scratch(g, SC_AUX3, (size_t)nwalkers * 4)))
+        return rc;
+    const int32_t *ds = nullptr, *dl = nullptr;
+    if (starts && nwalkers) {
+        HIPCHK(hipMemcpyAsync(g->sc[SC_AUX2], starts, (size_t)nwalkers * 4, hipMemcpyHostToDevice, g->st));
+        ds = scp<int32_t>(g, SC_AUX2);
+    }
+    if (lens && nwalkers) {
+        HIPCHK(hipMemcpyAsync(g->sc[SC_AUX3], lens, (size_t)nwalkers * 4, hipMemcpyHostToDevice, g->st));
+        dl = scp<int32_t>(g, SC_AUX3);
+    }
+    HIPCHK(hipMemsetAsync(g->d_steps, 0, 8, g->st));
+    rc = bingo_walk_device(g, app, params, ds, dl, nwalkers, scp<int32_t>(g, SC_AUX1),
+                           reinterpret_cast<uint64_t *>(g->d_steps));
+    if (rc) return rc;
+    if (nwalkers) HIPCHK(hipMemcpyAsync(paths, g->sc[SC_AUX1], pbytes, hipMemcpyDeviceToHost, g->st));
+    unsigned long long steps = 0;
+    HIPCHK(hipMemcpyAsync(&steps, g->d_steps, 8, hipMemcpyDeviceToHost, g->st));
+    HIPCHK(hipStreamSynchronize(g->st));
+    g->last_walk_steps = steps;
+    if (steps_out) *steps_out = steps;
+    return BINGO_OK;
+}
+
+int32_t bingo_num_nodes(const bingo_graph *g) { return g && g->loaded ? g->s.n : 0; }
+
+int bingo_sync(bingo_graph *g)
+{
+    if (!g) return fail(BINGO_EINVAL, "null handle");
+    HIPCHK(hipStreamSynchronize(g->st));
+    if (g->loaded) {
+        int rc = read_ctl(g);
+        if (rc) return rc;
+        if (g->ctl_host->error & ERR_OVERRUN) return fail(BINGO_EDEVICE, "pool overrun detected");
+    }
+    return BINGO_OK;
+}
+
+void *bingo_stream(bingo_graph *g) { return g ? (void *)g->st : nullptr; }
+
+int bingo_stats_get(bingo_graph *g, bingo_stats *out)
+{
+    if (!g || !out) return fail(BINGO_EINVAL, "null argument");
+    HIPCHK(hipStreamSynchronize(g->st));
+    phase_collect(g);
+    std::memset(out, 0, sizeof(*out));
+    out->load_ms = g->ph_ms[PH_LOAD];
+    out->insert_ms = g->ph_ms[PH_INSERT];
+    out->delete_ms = g->ph_ms[PH_DELETE];
+    out->walk_ms = g->ph_ms[PH_WALK];
+    out->walk_kernel_ms = g->ph_ms[PH_WALK];
+    out->walk_steps = g->last_walk_steps;
+    out->arcs = (uint64_t)g->arcs;
+    out->bytes_reserved = store_bytes(g);
+    if (g->loaded) {
+        int rc = read_ctl(g);
+        if (rc) return rc;
+        for (int p = 0; p < NPOOL; ++p) out->pool_used[p] = g->ctl_host->top[p];
+        out->walk_steps = g->ctl_host->steps;
+        out->walk_attempts = g->ctl_host->attempts;
+        out->walk_probes = g->ctl_host->probes;
+    }
+    return BINGO_OK;
+}
+
+int bingo_edge_count(bingo_graph *g, const int32_t *us, const int32_t *vs, int64_t k, uint32_t *counts)
+{
+    if (!g || !g->loaded) return fail(BINGO_ESTATE, "graph not loaded");
+    if (k < 0 || (k > 0 && (!us || !vs || !counts))) return fail(BINGO_EINVAL, "bad query arrays");
+    if (!k) return BINGO_OK;
+    int rc;
+    if ((rc = ensure_scratch(g, SC_AUX1, (size_t)k * 4)) || (rc = ensure_scratch(g, SC_AUX2, (size_t)k * 4)) ||
+        (rc = ensure_scratch(g, SC_AUX3, (size_t)k * 4)))
+        return rc;
+    HIPCHK(hipMemcpyAsync(g->sc[SC_AUX2], us, (size_t)k * 4, hipMemcpyHostToDevice, g->st));
+    HIPCHK(hipMemcpyAsync(g->sc[SC_AUX3], vs, (size_t)k * 4, hipMemcpyHostToDevice, g->st));
+    launch_edge_count(g->s, scp<int32_t>(g, SC_AUX2), scp<int32_t>(g, SC_AUX3), k, scp<uint32_t>(g, SC_AUX1), g->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(counts, g->sc[SC_AUX1], (size_t)k * 4, hipMemcpyDeviceToHost, g->st));
+    HIPCHK(hipStreamSynchronize(g->st));
+    return BINGO_OK;
+}
+
+int bingo_export(bingo_graph *g, bingo_dump *d)
+{
+    if (!g || !g->loaded) return fail(BINGO_ESTATE, "graph not loaded");
+    if (!d) return fail(BINGO_EINVAL, "null dump");
+    const size_t n1 = (size_t)g->s.n + 1, ng = n1 * kGroups;
+    std::vector<uint32_t> nsl(n1), bsz(ng);
+    void *dn = nullptr, *dm = nullptr, *db = nullptr, *dt = nullptr, *da = nullptr;
+    auto cleanup = [&]() {
+        for (void *p : {dn, dm, db, dt, da})
+            if (p) (void)hipFree(p);
+    };
+    if (hipMalloc(&dn, n1 * 4) || hipMalloc(&dm, n1 * 8) || hipMalloc(&db, ng * 4) || hipMalloc(&dt, ng * 8) ||
+        hipMalloc(&da, ng)) {
+        cleanup();
+        return fail(BINGO_ENOMEM, "export buffers");
+    }
+    launch_export_a(g->s, (uint32_t *)dn, (uint64_t *)dm, (uint32_t *)db, (uint64_t *)dt, (uint8_t *)da, g->st);
+    (void)hipMemcpyAsync(nsl.data(), dn, n1 * 4, hipMemcpyDeviceToHost, g->st);
+    (void)hipMemcpyAsync(bsz.data(), db, ng * 4, hipMemcpyDeviceToHost, g->st);
+    if (hipStreamSynchronize(g->st) != hipSuccess) {
+        cleanup();
+        return fail(BINGO_EDEVICE, "export");
+    }
+    std::vector<uint64_t> soff(n1), moff(n1);
+    uint64_t ts = 0, tm = 0;
+    for (size_t u = 0; u < n1; ++u) {
+        soff[u] = ts;
+        moff[u] = tm;
+        ts += nsl[u];
+        for (int j = 0; j < kGroups; ++j) tm += bsz[u * kGroups + j];
+    }
+    d->total_slots = ts;
+    d->total_members = tm;
+    if (!d->nslots) {
+        cleanup();
+        return BINGO_OK;
+    }
+    std::memcpy(d->nslots, nsl.data(), n1 * 4);
+    std::memcpy(d->bsize, bsz.data(), ng * 4);
+    (void)hipMemcpyAsync(d->mass, dm, n1 * 8, hipMemcpyDeviceToHost, g->st);
+    (void)hipMemcpyAsync(d->T, dt, ng * 8, hipMemcpyDeviceToHost, g->st);
+    (void)hipMemcpyAsync(d->alias, da, ng, hipMemcpyDeviceToHost, g->st);
+    void *dso = nullptr, *dmo = nullptr, *dsv = nullptr, *dsw = nullptr, *dmem = nullptr, *dbp = nullptr;
+    auto cleanup2 = [&]() {
+        for (void *p : {dso, dmo, dsv, dsw, dmem, dbp})
+            if (p) (void)hipFree(p);
+    };
+    if (hipMalloc(&dso, n1 * 8) || hipMalloc(&dmo, n1 * 8) || hipMalloc(&dsv, ts * 4 + 4) ||
+        hipMalloc(&dsw, ts * 4 + 4) || hipMalloc(&dmem, tm * 8 + 8) || hipMalloc(&dbp, tm * 4 + 4)) {
+        cleanup();
+        cleanup2();
+        return fail(BINGO_ENOMEM, "export buffers");
+    }
+    (void)hipMemcpyAsync(dso, soff.data(), n1 * 8, hipMemcpyHostToDevice, g->st);
+    (void)hipMemcpyAsync(dmo, moff.data(), n1 * 8, hipMemcpyHostToDevice, g->st);
+    launch_export_b(g->s, (uint64_t *)dso, (uint64_t *)dmo, (uint32_t *)dsv, (uint32_t *)dsw, (uint64_t *)dmem,
+                    (uint32_t *)dbp, g->st);
+    if (ts) {
+        (void)hipMemcpyAsync(d->slot_v, dsv, ts * 4, hipMemcpyDeviceToHost, g->st);
+        (void)hipMemcpyAsync(d->slot_w, dsw, ts * 4, hipMemcpyDeviceToHost, g->st);
+    }
+    if (tm) {
+        (void)hipMemcpyAsync(d->members, dmem, tm * 8, hipMemcpyDeviceToHost, g->st);
+        (void)hipMemcpyAsync(d->backpos, dbp, tm * 4, hipMemcpyDeviceToHost, g->st);
+    }
+    hipError_t e = hipStreamSynchronize(g->st);
+    cleanup();
+    cleanup2();
+    if (e != hipSuccess) return fail(BINGO_EDEVICE, std::string("export: ") + hipGetErrorString(e));
+    return BINGO_OK;
+}
+
+int bingo_dev_alloc(bingo_graph *g, size_t bytes, void **d_ptr)
+{
+    if (!g || !d_ptr) return fail(BINGO_EINVAL, "null argument");
+    HIPCHK(hipSetDevice(g->device));
+    if (hipMalloc(d_ptr, bytes ? bytes : 8) != hipSuccess) {
+        (void)hipGetLastError();
+        *d_ptr = nullptr;
+        return fail(BINGO_ENOMEM, "device allocation failed");
+    }
+    return BINGO_OK;
+}
+
+int bingo_dev_free(bingo_graph *g, void *d_ptr)
+{
+    if (!g) return fail(BINGO_EINVAL, "null handle");
+    HIPCHK(hipStreamSynchronize(g->st));
+    if (d_ptr) HIPCHK(hipFree(d_ptr));
+    return BINGO_OK;
+}
+
+int bingo_memcpy_h2d(bingo_graph *g, void *d_dst, const void *src, size_t bytes)
+{
+    if (!g) return fail(BINGO_EINVAL, "null handle");
+    HIPCHK(hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, g->st));
+    HIPCHK(hipStreamSynchronize(g->st));
+    return BINGO_OK;
+}
+
+int bingo_memcpy_d2h(bingo_graph *g, void *dst, const void *d_src, size_t bytes)
+{
+    if (!g) return fail(BINGO_EINVAL, "null handle");
+    HIPCHK(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, g->st));
+    HIPCHK(hipStreamSynchronize(g->st));
+    return BINGO_OK;
+}
+
+int bingo_event_record(bingo_graph *g, int slot)
+{
+    if (!g || slot < 0 || slot >= kEvents) return fail(BINGO_EINVAL, "bad event slot");
+    HIPCHK(hipEventRecord(g->ev[slot], g->st));
+    return BINGO_OK;
+}
+
+int bingo_event_elapsed(bingo_graph *g, int a, int b, float *ms)
+{
+    if (!g || !ms || a < 0 || b < 0 || a >= kEvents || b >= kEvents) return fail(BINGO_EINVAL, "bad event slot");
+    HIPCHK(hipEventSynchronize(g->ev[b]));
+    HIPCHK(hipEventElapsedTime(ms, g->ev[a], g->ev[b]));
+    return BINGO_OK;
+}
+
+}  // extern "C"

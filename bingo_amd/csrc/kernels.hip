@@ -1,0 +1,935 @@
+// gfx950 kernels of the bias-factorized walk engine.
+//
+// Execution model: one wave64 owns one node of a sorted update batch (lanes
+// 0..31 own bit groups 0..31, all 64 lanes cooperate on arc chunks); one
+// lane owns one walker.  Nothing here is a dense contraction, so there is no
+// MFMA: every kernel is bound by HBM latency/bandwidth of dependent gathers.
+//
+// Reference kernels replaced: buildKernel / insertKernel / deleteKernel
+// (include/operation.cuh:12-30, 122-143, 198-214), NodeData::build / insert /
+// deleteE / buildAliasTable / sample (include/Bingo.cuh:111-267), and the
+// walk kernels deepwalkKernel / node2vecKernel / pprKernel / samplingKernel
+// (operation.cuh:252-368, 407-430).
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include "store.h"
+
+namespace bingo {
+
+// ------------------------------------------------------------------------
+// wave helpers
+// ------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+__device__ __forceinline__ uint32_t rl32(uint32_t v, int l)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+__device__ __forceinline__ uint64_t rl64(uint64_t v, int l)
+{
+    return ((uint64_t)rl32((uint32_t)(v >> 32), l) << 32) | rl32((uint32_t)v, l);
+}
+__device__ __forceinline__ uint32_t uni32(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+
+__device__ __forceinline__ uint32_t wave_sum32(uint32_t v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_or32(uint32_t v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v |= __shfl_xor(v, d, 64);
+    return v;
+}
+// inclusive prefix sum over the wave
+__device__ __forceinline__ uint32_t wave_incl32(uint32_t v)
+{
+    const int ln = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t t = __shfl_up(v, d, 64);
+        if (ln >= d) v += t;
+    }
+    return v;
+}
+__device__ __forceinline__ uint64_t wave_incl64(uint64_t v)
+{
+    const int ln = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint64_t t = __shfl_up(v, d, 64);
+        if (ln >= d) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t pow2_at_least(uint32_t x)
+{
+    return x <= 1u ? 1u : (1u << (32 - __clz(x - 1u)));
+}
+
+// ------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11): counter (walker lo, walker hi,
+// step, attempt), key = seed.  Replaces the reference's xoshiro256++ stream
+// (include/utils.cuh:60-72).
+// ------------------------------------------------------------------------
+__device__ __forceinline__ uint4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                        uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return make_uint4(c0, c1, c2, c3);
+}
+
+// ------------------------------------------------------------------------
+// neighbour multiset (node2vec's exact "x in N(prev)" test)
+// ------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t hslot(uint32_t key, uint32_t cap)
+{
+    return (uint32_t)(((uint64_t)(key * 0x9E3779B1u) * cap) >> 32);
+}
+
+// add `cnt` to key's count; returns 1 if the key was new
+__device__ uint32_t hash_add(uint64_t *tab, uint32_t cap, uint32_t key, uint32_t cnt)
+{
+    uint32_t i = hslot(key, cap);
+    const uint32_t mask = cap - 1u;
+    for (;;) {
+        uint32_t *kp = reinterpret_cast<uint32_t *>(tab + i);
+        uint32_t old = atomicCAS(kp, 0u, key);
+        if (old == 0u || old == key) {
+            atomicAdd(kp + 1, cnt);
+            return old == 0u;
+        }
+        i = (i + 1u) & mask;
+    }
+}
+
+__device__ void hash_dec(uint64_t *tab, uint32_t cap, uint32_t key)
+{
+    uint32_t i = hslot(key, cap);
+    const uint32_t mask = cap - 1u;
+    for (uint32_t probe = 0; probe < cap; ++probe) {
+        uint32_t *kp = reinterpret_cast<uint32_t *>(tab + i);
+        if (*reinterpret_cast<volatile uint32_t *>(kp) == key) {
+            atomicSub(kp + 1, 1u);
+            return;
+        }
+        i = (i + 1u) & mask;
+    }
+}
+
+__device__ __forceinline__ uint64_t make_hinfo(uint64_t off, uint32_t cap)
+{
+    return cap ? ((off << 8) | (uint64_t)(32 - __clz(cap))) : 0ull;  // log2(cap)+1
+}
+
+__device__ __forceinline__ bool is_neighbor(const uint64_t *__restrict__ hash, uint64_t hinfo, uint32_t key)
+{
+    if (!hinfo) return false;
+    const uint32_t cap = 1u << ((uint32_t)(hinfo & 0xFF) - 1u), mask = cap - 1u;
+    const uint64_t *tab = hash + (hinfo >> 8);
+    uint32_t i = hslot(key, cap);
+    for (;;) {
+        const uint64_t e = tab[i];
+        const uint32_t k = (uint32_t)e;
+        if (k == key) return (e >> 32) != 0;
+        if (k == 0u) return false;
+        i = (i + 1u) & mask;
+    }
+}
+
+// ------------------------------------------------------------------------
+// alias table: integer-exact Vose over 32 fixed slots, built by one wave.
+// Lane j holds group j's size and member offset.  Masses M_j = size_j << j,
+// s_j = 32 M_j against slot capacity S = sum M_j; pair the highest small
+// with the highest large until one side is empty; leftovers stay (tau = S).
+// Stay threshold T_j = floor(tau_j 2^64 / S).  Replaces the float/EPS table
+// of NodeData::buildAliasTable (include/Bingo.cuh:111-158).
+// ------------------------------------------------------------------------
+__device__ void alias_rebuild(const DevStore &s, int32_t u, uint32_t size, uint64_t off, int ln)
+{
+    typedef unsigned __int128 u128;
+    const bool own = ln < kGroups;
+    const uint64_t M = own ? ((uint64_t)size << ln) : 0ull;
+    const uint64_t S = wave_sum64(M);  // < 2^63: fewer than 2^31 slots of weight < 2^32
+    u128 sj = (u128)M * 32u;
+    uint32_t small = (uint32_t)__ballot(own && sj < (u128)S);
+    uint32_t large = (uint32_t)__ballot(own && sj >= (u128)S);
+    uint64_t tau = 0;
+    int ali = ln;
+    while (small && large) {
+        const int l = 31 - __clz(small), gi = 31 - __clz(large);
+        const u128 sl = ((u128)rl64((uint64_t)(sj >> 64), l) << 64) | rl64((uint64_t)sj, l);
+        u128 sg = ((u128)rl64((uint64_t)(sj >> 64), gi) << 64) | rl64((uint64_t)sj, gi);
+        if (ln == l) {
+            tau = (uint64_t)sl;
+            ali = gi;
+        }
+        small &= ~(1u << l);
+        sg -= (u128)S - sl;
+        if (ln == gi) sj = sg;
+        if (sg < (u128)S) {
+            large &= ~(1u << gi);
+            small |= 1u << gi;
+        }
+    }
+    const uint32_t rest = small | large;
+    if (own && ((rest >> ln) & 1u)) {
+        tau = S;
+        ali = ln;
+    }
+    uint64_t T;
+    if (tau == S) {
+        T = ~0ull;
+    } else {  // floor(tau * 2^64 / S), tau < S < 2^63: restoring division
+        uint64_t rem = tau, q = 0;
+        for (int i = 0; i < 64; ++i) {
+            rem <<= 1;
+            q <<= 1;
+            if (rem >= S) {
+                rem -= S;
+                q |= 1ull;
+            }
+        }
+        T = q;
+    }
+    const uint32_t size_a = __shfl(size, ali, 64);
+    const uint64_t off_a = __shfl(off, ali, 64);
+    if (own) {
+        AliasEnt e;
+        e.T = T;
+        e.off_stay = off;
+        e.size_stay = size;
+        e.size_alias = size_a;
+        e.off_alias = off_a | ((uint64_t)ali << 56);
+        s.alias[(size_t)u * kGroups + ln] = e;
+    }
+}
+
+// ------------------------------------------------------------------------
+// small kernels
+// ------------------------------------------------------------------------
+__global__ void k_reset_ctl(Ctl *ctl)
+{
+    const int t = threadIdx.x;
+    if (t < NPOOL) ctl->need[t] = 0;
+    if (t == 0) ctl->error = 0;
+}
+
+__global__ void k_reset_walk(Ctl *ctl)
+{
+    if (threadIdx.x == 0) {
+        ctl->steps = 0;
+        ctl->attempts = 0;
+        ctl->probes = 0;
+    }
+}
+
+__global__ void k_init_nodes(DevStore s)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t total = ((int64_t)s.n + 1) * kGroups;
+    if (i >= total) return;
+    const int j = (int)(i & (kGroups - 1));
+    AliasEnt e;
+    e.T = ~0ull;
+    e.off_stay = 0;
+    e.size_stay = 0;
+    e.size_alias = 0;
+    e.off_alias = (uint64_t)j << 56;
+    s.alias[i] = e;
+    GroupHdr g = {0, 0, 0};
+    s.grp[i] = g;
+    if (j == 0) {
+        NodeHdr h = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        s.hdr[i >> 5] = h;
+        s.hinfo[i >> 5] = 0;
+    }
+}
+
+__global__ void k_insert_keys(const DevEdge *__restrict__ e, int64_t k, int32_t n, uint32_t *__restrict__ keys,
+                              uint32_t *__restrict__ vals, Ctl *ctl)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= k) return;
+    const DevEdge x = e[i];
+    const bool bad = x.u < 1 || x.u > n || x.v < 1 || x.v > n || x.w == 0u;
+    if (bad) atomicOr(&ctl->error, (unsigned)ERR_INVAL);
+    keys[i] = bad ? 0u : (uint32_t)x.u;
+    vals[i] = (uint32_t)i;
+}
+
+__global__ void k_gather_edges(const DevEdge *__restrict__ e, const uint32_t *__restrict__ order, int64_t k,
+                               DevEdge *__restrict__ out)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < k) out[i] = e[order[i]];
+}
+
+// ------------------------------------------------------------------------
+// insert: plan (capacity growth per node) and apply
+// ------------------------------------------------------------------------
+struct Growth {
+    uint32_t cnt, new_n, new_back, pcsum;
+    uint32_t cap_slots, cap_back, cap_hash;
+    bool g_slots, g_back, g_hash;
+    // per lane (group j = lane)
+    uint32_t addj, cap_g;
+    bool g_g;
+};
+
+// counts of the run and the growth decisions; identical in plan and apply
+__device__ Growth plan_run(const DevStore &s, const DevEdge *__restrict__ e, uint32_t cnt, const NodeHdr &h,
+                           const GroupHdr &gh, int ln)
+{
+    Growth G;
+    uint32_t addj = 0, pcsum = 0;
+    for (uint32_t c = 0; c < cnt; c += 64) {
+        const uint32_t i = c + (uint32_t)ln;
+        const uint32_t w = i < cnt ? e[i].w : 0u;
+        const uint32_t bits = uni32(wave_or32(w));
+        for (uint32_t b = bits; b;) {
+            const int j = __ffs(b) - 1;
+            b &= b - 1u;
+            const uint64_t m = __ballot((w >> j) & 1u);
+            if (ln == j) addj += (uint32_t)__popcll(m);
+        }
+        pcsum += wave_sum32((uint32_t)__popc(w));
+    }
+    G.cnt = cnt;
+    G.addj = addj;
+    G.pcsum = pcsum;
+    G.new_n = h.nslots + cnt;
+    G.g_slots = G.new_n > h.slot_cap;
+    G.cap_slots = G.g_slots ? pow2_at_least(max(G.new_n, (uint32_t)s.min_slots)) : h.slot_cap;
+    G.new_back = h.back_used + pcsum;
+    G.g_back = G.new_back > h.back_cap;
+    G.cap_back = G.g_back ? pow2_at_least(max(G.new_back, 16u)) : h.back_cap;
+    const uint32_t need_g = gh.size + addj;
+    G.g_g = ln < kGroups && addj && need_g > gh.cap;
+    G.cap_g = G.g_g ? pow2_at_least(max(need_g, 4u)) : gh.cap;
+    const uint32_t hneed = 2u * (h.hash_used + cnt);
+    G.g_hash = s.nbr && hneed > h.hash_cap;
+    G.cap_hash = G.g_hash ? pow2_at_least(max(hneed, 16u)) : h.hash_cap;
+    return G;
+}
+
+__device__ __forceinline__ bool run_too_big(const NodeHdr &h, uint32_t cnt, uint32_t pcsum)
+{
+    return (uint64_t)h.nslots + cnt >= (1ull << 30) || (uint64_t)h.back_used + pcsum >= (1ull << 31) ||
+           2ull * ((uint64_t)h.hash_used + cnt) >= (1ull << 31);
+}
+
+__global__ __launch_bounds__(256) void k_insert_plan(DevStore s, const DevEdge *__restrict__ E, Runs r)
+{
+    const int ln = lane_id();
+    const uint32_t nruns = *r.nruns;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    unsigned long long need[NPOOL] = {0, 0, 0, 0};
+    for (uint32_t ri = wave; ri < nruns; ri += nwaves) {
+        const int32_t u = (int32_t)r.node[ri];
+        if (u < 1 || u > s.n) continue;  // invalid arcs were flagged already
+        const uint32_t cnt = r.count[ri];
+        const NodeHdr h = s.hdr[u];
+        GroupHdr gh = {0, 0, 0};
+        if (ln < kGroups) gh = s.grp[(size_t)u * kGroups + ln];
+        const Growth G = plan_run(s, E + r.start[ri], cnt, h, gh, ln);
+        if (run_too_big(h, cnt, G.pcsum)) {
+            if (ln == 0) atomicOr(&s.ctl->error, (unsigned)ERR_NOMEM);
+            continue;
+        }
+        need[POOL_SLOT] += G.g_slots ? 3ull * G.cap_slots : 0;
+        need[POOL_BACK] += G.g_back ? G.cap_back : 0;
+        need[POOL_HASH] += G.g_hash ? G.cap_hash : 0;
+        need[POOL_MEM] += wave_sum64(G.g_g ? (uint64_t)G.cap_g : 0ull);
+    }
+    if (ln == 0)
+        for (int p = 0; p < NPOOL; ++p)
+            if (need[p]) atomicAdd(&s.ctl->need[p], need[p]);
+}
+
+// Region of n elements from a pool; ~0 if the pool (sized by the host from
+// the plan) would overflow, in which case the caller writes nothing.
+__device__ __forceinline__ uint64_t bump(const DevStore &s, int pool, uint64_t n)
+{
+    const uint64_t at = atomicAdd(&s.ctl->top[pool], (unsigned long long)n);
+    if (at + n > s.cap[pool]) {
+        atomicOr(&s.ctl->error, (unsigned)ERR_OVERRUN);
+        return ~0ull;
+    }
+    return at;
+}
+
+__device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, uint32_t cnt, int32_t u, int ln)
+{
+    NodeHdr h = s.hdr[u];
+    GroupHdr gh = {0, 0, 0};
+    if (ln < kGroups) gh = s.grp[(size_t)u * kGroups + ln];
+    const Growth G = plan_run(s, e, cnt, h, gh, ln);
+    if (run_too_big(h, cnt, G.pcsum)) return;  // rejected by the plan
+
+    // ---- regions (lane 0 bumps node pools; groups bump once per wave) ----
+    uint64_t a_slot = h.slot_off, a_back = h.back_off, a_hash = h.hash_off;
+    const uint64_t gneed = G.g_g ? (uint64_t)G.cap_g : 0ull;
+    const uint64_t gincl = wave_incl64(gneed);
+    const uint64_t gtotal = __shfl(gincl, 63, 64);
+    uint64_t gbase = 0;
+    if (ln == 0) {
+        if (G.g_slots) a_slot = bump(s, POOL_SLOT, 3ull * G.cap_slots);
+        if (G.g_back) a_back = bump(s, POOL_BACK, G.cap_back);
+        if (G.g_hash) a_hash = bump(s, POOL_HASH, G.cap_hash);
+        if (gtotal) gbase = bump(s, POOL_MEM, gtotal);
+    }
+    a_slot = __shfl(a_slot, 0, 64);
+    a_back = __shfl(a_back, 0, 64);
+    a_hash = __shfl(a_hash, 0, 64);
+    gbase = __shfl(gbase, 0, 64);
+    if (a_slot == ~0ull || a_back == ~0ull || a_hash == ~0ull || gbase == ~0ull) return;  // node left as it was
+    const uint64_t goff = G.g_g ? gbase + gincl - gneed : gh.off;
+
+    const uint32_t n0 = h.nslots;
+    uint32_t *nv = s.slot + a_slot;
+    uint32_t *nw = nv + G.cap_slots;
+    uint32_t *nb = nv + 2ull * G.cap_slots;
+    if (G.g_slots && n0) {
+        const uint32_t *ov = s.slot + h.slot_off;
+        const uint32_t *ow = ov + h.slot_cap;
+        const uint32_t *ob = ov + 2ull * h.slot_cap;
+        for (uint32_t i = ln; i < n0; i += 64) {
+            nv[i] = ov[i];
+            nw[i] = ow[i];
+            nb[i] = ob[i];
+        }
+    }
+    uint32_t *back = s.back + a_back;
+    if (G.g_back && h.back_used) {
+        const uint32_t *ob = s.back + h.back_off;
+        for (uint32_t i = ln; i < h.back_used; i += 64) back[i] = ob[i];
+    }
+    for (uint32_t gm = (uint32_t)__ballot(G.g_g && gh.size); gm;) {
+        const int j = __ffs(gm) - 1;
+        gm &= gm - 1u;
+        const uint64_t from = rl64(gh.off, j), to = rl64(goff, j);
+        const uint32_t sz = rl32(gh.size, j);
+        for (uint32_t i = ln; i < sz; i += 64) s.mem[to + i] = s.mem[from + i];
+    }
+    uint32_t hused = h.hash_used;
+    uint64_t *htab = s.hash + a_hash;
+    if (G.g_hash) {
+        for (uint32_t i = ln; i < G.cap_hash; i += 64) htab[i] = 0ull;
+        __threadfence();
+        uint32_t fresh = 0;
+        if (h.hash_cap) {
+            const uint64_t *old = s.hash + h.hash_off;
+            for (uint32_t i = ln; i < h.hash_cap; i += 64) {
+                const uint64_t x = old[i];
+                if ((uint32_t)x && (x >> 32)) fresh += hash_add(htab, G.cap_hash, (uint32_t)x, (uint32_t)(x >> 32));
+            }
+        }
+        hused = wave_sum32(fresh);
+        __threadfence();
+    }
+
+    // ---- append arcs in batch order: ballot + prefix rank per bit group ----
+    uint32_t szj = gh.size;
+    uint32_t boff_run = h.back_used;
+    uint32_t fresh = 0;
+    for (uint32_t c = 0; c < cnt; c += 64) {
+        const uint32_t i = c + (uint32_t)ln;
+        const bool act = i < cnt;
+        DevEdge x = {0, 0, 0u};
+        if (act) x = e[i];
+        const uint32_t slot = n0 + i;
+        const uint32_t pc = (uint32_t)__popc(x.w);
+        const uint32_t pincl = wave_incl32(pc);
+        const uint32_t my_boff = boff_run + pincl - pc;
+        if (act) {
+            nv[slot] = (uint32_t)x.v;
+            nw[slot] = x.w;
+            nb[slot] = my_boff;
+        }
+        const uint64_t lt = (1ull << ln) - 1ull;
+        uint32_t rank = 0;
+        const uint64_t entry = ((uint64_t)(uint32_t)x.v << 32) | slot;
+        for (uint32_t b = uni32(wave_or32(x.w)); b;) {
+            const int j = __ffs(b) - 1;
+            b &= b - 1u;
+            const bool has = (x.w >> j) & 1u;
+            const uint64_t m = __ballot(has);
+            const uint32_t base = rl32(szj, j);
+            const uint64_t go = rl64(goff, j);
+            if (has) {
+                const uint32_t pos = base + (uint32_t)__popcll(m & lt);
+                s.mem[go + pos] = entry;
+                back[my_boff + rank] = pos;
+                ++rank;
+            }
+            if (ln == j) szj += (uint32_t)__popcll(m);
+        }
+        boff_run += __shfl(pincl, 63, 64);
+        if (s.nbr && act) fresh += hash_add(htab, G.cap_hash, (uint32_t)x.v, 1u);
+    }
+    hused += wave_sum32(fresh);
+
+    // ---- headers ----
+    if (ln < kGroups) {
+        GroupHdr ng;
+        ng.off = goff;
+        ng.size = szj;
+        ng.cap = G.cap_g;
+        s.grp[(size_t)u * kGroups + ln] = ng;
+    }
+    if (ln == 0) {
+        h.slot_off = a_slot;
+        h.slot_cap = G.cap_slots;
+        h.nslots = G.new_n;
+        h.back_off = a_back;
+        h.back_cap = G.cap_back;
+        h.back_used = G.new_back;
+        if (s.nbr) {
+            h.hash_off = a_hash;
+            h.hash_cap = G.cap_hash;
+            h.hash_used = hused;
+            s.hinfo[u] = make_hinfo(a_hash, G.cap_hash);
+        }
+        s.hdr[u] = h;
+    }
+    alias_rebuild(s, u, ln < kGroups ? szj : 0u, goff, ln);
+}
+
+__global__ __launch_bounds__(256) void k_insert_apply(DevStore s, const DevEdge *__restrict__ E, Runs r)
+{
+    const int ln = lane_id();
+    const uint32_t nruns = *r.nruns;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t ri = wave; ri < nruns; ri += nwaves) {
+        const int32_t u = (int32_t)r.node[ri];
+        if (u < 1 || u > s.n) continue;
+        insert_run(s, E + r.start[ri], r.count[ri], u, ln);
+    }
+}
+
+// ------------------------------------------------------------------------
+// delete: check (all-or-nothing) and apply
+// ------------------------------------------------------------------------
+__global__ void k_delete_keys(const int32_t *__restrict__ d, int64_t k, unsigned long long *__restrict__ keys)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= k) return;
+    keys[i] = ((unsigned long long)(uint32_t)d[2 * i] << 32) | (uint32_t)d[2 * i + 1];
+}
+
+__global__ void k_delete_check(DevStore s, const unsigned long long *__restrict__ keys, int64_t k,
+                               uint32_t *__restrict__ us, uint32_t *__restrict__ slots)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= k) return;
+    const unsigned long long key = keys[i];
+    const int32_t u = (int32_t)(key >> 32);
+    const uint32_t sl = (uint32_t)key;
+    bool bad = u < 1 || u > s.n || (i > 0 && keys[i - 1] == key);
+    if (!bad) {
+        const NodeHdr h = s.hdr[u];
+        bad = sl >= h.nslots || s.slot[h.slot_off + h.slot_cap + sl] == 0u;
+    }
+    if (bad) atomicOr(&s.ctl->error, (unsigned)ERR_INVAL);
+    us[i] = bad ? 0u : (uint32_t)u;
+    slots[i] = sl;
+}
+
+// Batched delete of one node (slots ascending, distinct, live).  Lane j owns
+// bit group j: deleted members are marked in place, then each hole below the
+// new size (holes in slot order) takes the next unmarked member from the
+// tail, and the moved member's back index is rewritten.  This is the batch
+// form of NodeData::deleteE's swap-with-last (include/Bingo.cuh:199-218).
+__device__ void delete_run(const DevStore &s, const uint32_t *__restrict__ slots, uint32_t cnt, int32_t u, int ln)
+{
+    const NodeHdr h = s.hdr[u];
+    uint32_t *V = s.slot + h.slot_off;
+    uint32_t *W = V + h.slot_cap;
+    const uint32_t *B = V + 2ull * h.slot_cap;
+    uint32_t *back = s.back + h.back_off;
+    GroupHdr gh = {0, 0, 0};
+    if (ln < kGroups) {
+        gh = s.grp[(size_t)u * kGroups + ln];
+        const uint32_t bit = 1u << ln, below = bit - 1u;
+        uint32_t kj = 0;
+        for (uint32_t t = 0; t < cnt; ++t) kj += (W[slots[t]] & bit) != 0u;
+        if (kj) {
+            uint64_t *m = s.mem + gh.off;
+            const uint32_t S2 = gh.size - kj;
+            for (uint32_t t = 0; t < cnt; ++t) {
+                const uint32_t sl = slots[t], w = W[sl];
+                if (w & bit) reinterpret_cast<uint32_t *>(m + back[B[sl] + __popc(w & below)])[0] = sl | kMark;
+            }
+            uint32_t tail = S2;
+            for (uint32_t t = 0; t < cnt; ++t) {
+                const uint32_t sl = slots[t], w = W[sl];
+                if (!(w & bit)) continue;
+                const uint32_t pos = back[B[sl] + __popc(w & below)];
+                if (pos >= S2) continue;
+                uint64_t mv = kMark;
+                while (tail < gh.size && ((uint32_t)(mv = m[tail++]) & kMark)) {
+                }
+                if ((uint32_t)mv & kMark) {  // no survivor left: the store is inconsistent
+                    atomicOr(&s.ctl->error, (unsigned)ERR_OVERRUN);
+                    break;
+                }
+                m[pos] = mv;
+                const uint32_t ms = (uint32_t)mv;
+                back[B[ms] + __popc(W[ms] & below)] = pos;
+            }
+            gh.size = S2;
+            s.grp[(size_t)u * kGroups + ln].size = S2;
+        }
+    }
+    if (s.nbr && h.hash_cap) {
+        uint64_t *htab = s.hash + h.hash_off;
+        for (uint32_t t = ln; t < cnt; t += 64) hash_dec(htab, h.hash_cap, V[slots[t]]);
+    }
+    for (uint32_t t = ln; t < cnt; t += 64) W[slots[t]] = 0u;
+    alias_rebuild(s, u, ln < kGroups ? gh.size : 0u, gh.off, ln);
+}
+
+__global__ __launch_bounds__(256) void k_delete_apply(DevStore s, const uint32_t *__restrict__ slots, Runs r)
+{
+    const int ln = lane_id();
+    const uint32_t nruns = *r.nruns;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t ri = wave; ri < nruns; ri += nwaves) {
+        const int32_t u = (int32_t)r.node[ri];
+        if (u < 1 || u > s.n) continue;
+        delete_run(s, slots + r.start[ri], r.count[ri], u, ln);
+    }
+}
+
+// ------------------------------------------------------------------------
+// walks: one lane per walker, state in registers
+// ------------------------------------------------------------------------
+// One draw on node `cur` (NodeData::sample, include/Bingo.cuh:220-267):
+// slot i = top 5 bits of r.x; the 64-bit coin (r.y:r.z) against T_i picks
+// group i or its partner; member floor(r.w * size / 2^32) of that group.
+__device__ __forceinline__ int32_t sample_step(const AliasEnt *__restrict__ alias, const uint64_t *__restrict__ mem,
+                                               int32_t cur, uint4 r)
+{
+    const uint32_t i = r.x >> 27;
+    const uint4 *p = reinterpret_cast<const uint4 *>(alias + (size_t)cur * kGroups + i);
+    const uint4 q0 = p[0], q1 = p[1];
+    const uint64_t T = ((uint64_t)q0.y << 32) | q0.x;
+    const uint64_t coin = ((uint64_t)r.y << 32) | r.z;
+    const bool stay = coin < T;
+    const uint64_t off = stay ? (((uint64_t)q0.w << 32) | q0.z) : ((((uint64_t)q1.w << 32) | q1.z) & kOffMask);
+    const uint32_t sz = stay ? q1.x : q1.y;
+    if (sz == 0u) return -1;
+    const uint32_t pos = __umulhi(r.w, sz);
+    return (int32_t)reinterpret_cast<const uint32_t *>(mem + off + pos)[1];
+}
+
+template <int APP>
+__global__ __launch_bounds__(256) void k_walk(DevStore s, WalkArgs a)
+{
+    const int64_t wid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    uint32_t steps = 0, draws = 0, probes = 0;
+    if (wid < a.nwalkers) {
+        const int32_t start = a.starts ? a.starts[wid] : (int32_t)(wid % s.n) + 1;
+        int32_t L = a.lens ? a.lens[wid] : a.max_len;
+        if (APP == 3 && L > 2) L = 2;
+        int32_t *row = a.paths + wid * (int64_t)a.max_len;
+        int32_t j = 1;
+        if (start < 1 || start > s.n || L < 1 || L > a.max_len) {
+            j = 0;
+            atomicOr(&s.ctl->error, (unsigned)ERR_INVAL);
+        } else {
+            const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+            const uint32_t w0 = (uint32_t)wid, w1 = (uint32_t)((uint64_t)wid >> 32);
+            row[0] = start;
+            int32_t cur = start, prev = -1;
+            uint64_t hcur = APP == 1 ? s.hinfo[start] : 0ull, hprev = 0ull;
+            const uint32_t tlo = min(a.thr_neighbor, a.thr_other), thi = max(a.thr_neighbor, a.thr_other);
+            for (; j < L; ++j) {
+                uint4 r;
+                int32_t nx;
+                if (APP == 1 && j >= 2) {
+                    for (uint32_t at = 0;; ++at) {
+                        r = philox(w0, w1, (uint32_t)j, at, k0, k1);
+                        nx = sample_step(s.alias, s.mem, cur, r);
+                        ++draws;
+                        if (nx < 0 || (int64_t)at + 1 >= a.max_attempts) break;
+                        const uint32_t aux = r.x & kAuxMask;
+                        bool acc;
+                        if (nx == prev) acc = aux < a.thr_return;
+                        else if (aux < tlo) acc = true;
+                        else if (aux >= thi) acc = false;
+                        else {
+                            ++probes;
+                            acc = aux < (is_neighbor(s.hash, hprev, (uint32_t)nx) ? a.thr_neighbor : a.thr_other);
+                        }
+                        if (acc) break;
+                    }
+                } else {
+                    r = philox(w0, w1, (uint32_t)j, 0u, k0, k1);
+                    nx = sample_step(s.alias, s.mem, cur, r);
+                    ++draws;
+                }
+                if (nx < 0) break;
+                row[j] = nx;
+                ++steps;
+                prev = cur;
+                cur = nx;
+                if (APP == 1) {
+                    hprev = hcur;
+                    hcur = s.hinfo[nx];
+                }
+                if (APP == 2 && (r.x & kAuxMask) < a.thr_stop) {
+                    ++j;
+                    break;
+                }
+            }
+        }
+        for (; j < a.max_len; ++j) row[j] = -1;
+    }
+    const uint32_t tot = wave_sum32(steps), tdraw = wave_sum32(draws), tprobe = wave_sum32(probes);
+    if (lane_id() == 0) {
+        if (tot) {
+            atomicAdd(a.steps, (unsigned long long)tot);
+            atomicAdd(&s.ctl->steps, (unsigned long long)tot);
+        }
+        if (tdraw) atomicAdd(&s.ctl->attempts, (unsigned long long)tdraw);
+        if (tprobe) atomicAdd(&s.ctl->probes, (unsigned long long)tprobe);
+    }
+}
+
+// ------------------------------------------------------------------------
+// export and point queries
+// ------------------------------------------------------------------------
+__global__ void k_export_a(DevStore s, uint32_t *nslots, uint64_t *mass, uint32_t *bsize, uint64_t *T, uint8_t *alias)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t total = ((int64_t)s.n + 1) * kGroups;
+    if (i >= total) return;
+    const int j = (int)(i & (kGroups - 1));
+    const int64_t u = i >> 5;
+    const AliasEnt e = s.alias[i];
+    const uint32_t sz = s.grp[i].size;
+    bsize[i] = sz;
+    T[i] = e.T;
+    alias[i] = (uint8_t)(e.off_alias >> 56);
+    if (j == 0) {
+        nslots[u] = s.hdr[u].nslots;
+        uint64_t acc = 0;
+        for (int g = 0; g < kGroups; ++g) acc += (uint64_t)s.grp[u * kGroups + g].size << g;
+        mass[u] = acc;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_export_b(DevStore s, const uint64_t *slot_out_off, const uint64_t *mem_out_off,
+                                                  uint32_t *slot_v, uint32_t *slot_w, uint64_t *members,
+                                                  uint32_t *backpos)
+{
+    const int ln = lane_id();
+    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    if (wave > s.n) return;
+    const int64_t u = wave;
+    const NodeHdr h = s.hdr[u];
+    const uint32_t *V = s.slot + h.slot_off, *W = V + h.slot_cap, *B = V + 2ull * h.slot_cap;
+    const uint64_t so = slot_out_off[u];
+    for (uint32_t i = ln; i < h.nslots; i += 64) {
+        slot_v[so + i] = V[i];
+        slot_w[so + i] = W[i];
+    }
+    uint64_t mo = mem_out_off[u];
+    for (int j = 0; j < kGroups; ++j) {
+        const GroupHdr g = s.grp[u * kGroups + j];
+        for (uint32_t i = ln; i < g.size; i += 64) members[mo + i] = s.mem[g.off + i];
+        mo += g.size;
+    }
+    uint64_t bo = mem_out_off[u];
+    for (uint32_t c = 0; c < h.nslots; c += 64) {
+        const uint32_t i = c + ln;
+        const uint32_t w = i < h.nslots ? W[i] : 0u;
+        const uint32_t pc = (uint32_t)__popc(w);
+        const uint32_t incl = wave_incl32(pc);
+        for (uint32_t k = 0; k < pc; ++k) backpos[bo + incl - pc + k] = s.back[h.back_off + B[i] + k];
+        bo += __shfl(incl, 63, 64);
+    }
+}
+
+__global__ void k_edge_count(DevStore s, const int32_t *us, const int32_t *vs, int64_t k, uint32_t *counts)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= k) return;
+    const int32_t u = us[i], v = vs[i];
+    uint32_t c = 0;
+    if (u >= 1 && u <= s.n && v >= 1 && v <= s.n) {
+        const NodeHdr h = s.hdr[u];
+        if (s.nbr) {
+            if (h.hash_cap) {
+                const uint64_t *tab = s.hash + h.hash_off;
+                uint32_t p = hslot((uint32_t)v, h.hash_cap);
+                for (uint32_t probe = 0; probe < h.hash_cap; ++probe) {
+                    const uint64_t e = tab[p];
+                    if ((uint32_t)e == (uint32_t)v) {
+                        c = (uint32_t)(e >> 32);
+                        break;
+                    }
+                    if ((uint32_t)e == 0u) break;
+                    p = (p + 1u) & (h.hash_cap - 1u);
+                }
+            }
+        } else {
+            const uint32_t *V = s.slot + h.slot_off, *W = V + h.slot_cap;
+            for (uint32_t t = 0; t < h.nslots; ++t) c += (W[t] != 0u && V[t] == (uint32_t)v);
+        }
+    }
+    counts[i] = c;
+}
+
+// ------------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------------
+static inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+static inline unsigned run_blocks(int64_t k)
+{
+    // one wave per run, 4 waves per block, capped at ~8 resident waves per SIMD
+    const int64_t b = (k + 3) / 4;
+    return (unsigned)(b < 1 ? 1 : (b > 2048 ? 2048 : b));
+}
+
+void launch_reset_ctl(Ctl *ctl, hipStream_t st) { hipLaunchKernelGGL(k_reset_ctl, dim3(1), dim3(64), 0, st, ctl); }
+
+void launch_reset_walk(Ctl *ctl, hipStream_t st) { hipLaunchKernelGGL(k_reset_walk, dim3(1), dim3(64), 0, st, ctl); }
+
+void launch_init_nodes(DevStore s, hipStream_t st)
+{
+    const int64_t total = ((int64_t)s.n + 1) * kGroups;
+    hipLaunchKernelGGL(k_init_nodes, dim3(blocks_for(total, 256)), dim3(256), 0, st, s);
+}
+
+void launch_insert_keys(const DevEdge *e, int64_t k, int32_t n, uint32_t *keys, uint32_t *vals, Ctl *ctl,
+                        hipStream_t st)
+{
+    hipLaunchKernelGGL(k_insert_keys, dim3(blocks_for(k, 256)), dim3(256), 0, st, e, k, n, keys, vals, ctl);
+}
+
+void launch_gather_edges(const DevEdge *e, const uint32_t *order, int64_t k, DevEdge *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_gather_edges, dim3(blocks_for(k, 256)), dim3(256), 0, st, e, order, k, out);
+}
+
+void launch_insert_plan(DevStore s, const DevEdge *sorted, Runs r, int64_t k, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_insert_plan, dim3(run_blocks(k)), dim3(256), 0, st, s, sorted, r);
+}
+
+void launch_insert_apply(DevStore s, const DevEdge *sorted, Runs r, int64_t k, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_insert_apply, dim3(run_blocks(k)), dim3(256), 0, st, s, sorted, r);
+}
+
+void launch_delete_keys(const int32_t *dels, int64_t k, unsigned long long *keys, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_delete_keys, dim3(blocks_for(k, 256)), dim3(256), 0, st, dels, k, keys);
+}
+
+void launch_delete_check(DevStore s, const unsigned long long *keys, int64_t k, uint32_t *us, uint32_t *slots,
+                         hipStream_t st)
+{
+    hipLaunchKernelGGL(k_delete_check, dim3(blocks_for(k, 256)), dim3(256), 0, st, s, keys, k, us, slots);
+}
+
+void launch_delete_apply(DevStore s, const uint32_t *slots, Runs r, int64_t k, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_delete_apply, dim3(run_blocks(k)), dim3(256), 0, st, s, slots, r);
+}
+
+void launch_walk(DevStore s, int app, const WalkArgs &a, int block, hipStream_t st)
+{
+    if (block != 64 && block != 128 && block != 256) block = 256;
+    const dim3 grid(blocks_for(a.nwalkers, block)), blk(block);
+    switch (app) {
+    case 0: hipLaunchKernelGGL(k_walk<0>, grid, blk, 0, st, s, a); break;
+    case 1: hipLaunchKernelGGL(k_walk<1>, grid, blk, 0, st, s, a); break;
+    case 2: hipLaunchKernelGGL(k_walk<2>, grid, blk, 0, st, s, a); break;
+    default: hipLaunchKernelGGL(k_walk<3>, grid, blk, 0, st, s, a); break;
+    }
+}
+
+void launch_export_a(DevStore s, uint32_t *nslots, uint64_t *mass, uint32_t *bsize, uint64_t *T, uint8_t *alias,
+                     hipStream_t st)
+{
+    const int64_t total = ((int64_t)s.n + 1) * kGroups;
+    hipLaunchKernelGGL(k_export_a, dim3(blocks_for(total, 256)), dim3(256), 0, st, s, nslots, mass, bsize, T, alias);
+}
+
+void launch_export_b(DevStore s, const uint64_t *slot_out_off, const uint64_t *mem_out_off, uint32_t *slot_v,
+                     uint32_t *slot_w, uint64_t *members, uint32_t *backpos, hipStream_t st)
+{
+    const int64_t waves = (int64_t)s.n + 1;
+    hipLaunchKernelGGL(k_export_b, dim3(blocks_for(waves * 64, 256)), dim3(256), 0, st, s, slot_out_off, mem_out_off,
+                       slot_v, slot_w, members, backpos);
+}
+
+void launch_edge_count(DevStore s, const int32_t *us, const int32_t *vs, int64_t k, uint32_t *counts, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_edge_count, dim3(blocks_for(k, 256)), dim3(256), 0, st, s, us, vs, k, counts);
+}
+
+// ------------------------------------------------------------------------
+// rocPRIM wrappers
+// ------------------------------------------------------------------------
+size_t sort_pairs_u32(void *tmp, size_t tmp_bytes, const uint32_t *kin, uint32_t *kout, const uint32_t *vin,
+                      uint32_t *vout, int64_t k, int bits, hipStream_t st)
+{
+    size_t b = tmp_bytes;
+    (void)rocprim::radix_sort_pairs(tmp, b, kin, kout, vin, vout, (size_t)k, 0, (unsigned)bits, st);
+    return b;
+}
+
+size_t sort_keys_u64(void *tmp, size_t tmp_bytes, const unsigned long long *kin, unsigned long long *kout, int64_t k,
+                     int bits, hipStream_t st)
+{
+    size_t b = tmp_bytes;
+    (void)rocprim::radix_sort_keys(tmp, b, kin, kout, (size_t)k, 0, (unsigned)bits, st);
+    return b;
+}
+
+size_t run_length_encode_u32(void *tmp, size_t tmp_bytes, const uint32_t *keys, int64_t k, uint32_t *uniq,
+                             uint32_t *counts, uint32_t *nruns, hipStream_t st)
+{
+    size_t b = tmp_bytes;
+    (void)rocprim::run_length_encode(tmp, b, keys, (size_t)k, uniq, counts, nruns, st);
+    return b;
+}
+
+size_t exclusive_scan_u32(void *tmp, size_t tmp_bytes, const uint32_t *in, uint32_t *out, int64_t k, hipStream_t st)
+{
+    size_t b = tmp_bytes;
+    (void)rocprim::exclusive_scan(tmp, b, in, out, 0u, (size_t)k, rocprim::plus<uint32_t>(), st);
+    return b;
+}
+
+}  // namespace bingo

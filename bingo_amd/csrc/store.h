@@ -1,0 +1,141 @@
+// Internal layout of the HBM-resident store and the kernel launchers.
+//
+// Per node u (ids 1..n; record 0 is an always-empty node):
+//   NodeHdr  hdr[u]        slot region, back-index region, neighbour table
+//   GroupHdr grp[u*32 + j] member list of bit group j: pool offset, size, capacity
+//   AliasEnt alias[u*32+i] 32-slot alias table; slot i carries the (offset,
+//                          size) of group i and of its alias partner, so one
+//                          32-byte read decides the group of a walk step
+//   uint64   hinfo[u]      neighbour-table locator for node2vec, (off << 8) | (log2 cap + 1)
+// Pools (one device allocation each, regions handed out by atomic bump):
+//   slot  u32: per node [v x cap | w x cap | back-offset x cap]
+//   back  u32: per slot, the member positions of its set bits, ascending
+//   mem   u64: member entries (v << 32) | slot
+//   hash  u64: neighbour multiset entries key | count << 32 (linear probing)
+//
+// Reference roles: NodeData (include/Bingo.cuh:6-58) held edge[SZ],
+// group[BITS*SZ], idx[BITS*SZ], num[], aliasTable[] per node from a bump pool
+// (include/gpuMem.cuh:4-32); here every bucket has its own capacity and the
+// alias table stores group locators instead of bit indices.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace bingo {
+
+constexpr int kGroups = 32;
+constexpr uint32_t kMark = 0x80000000u;
+constexpr uint32_t kAuxMask = (1u << 27) - 1u;
+constexpr uint64_t kOffMask = (1ull << 56) - 1ull;
+
+enum Pool { POOL_SLOT = 0, POOL_BACK = 1, POOL_MEM = 2, POOL_HASH = 3, NPOOL = 4 };
+enum ErrFlag { ERR_INVAL = 1, ERR_NOMEM = 2, ERR_OVERRUN = 4 };
+
+struct NodeHdr {
+    uint64_t slot_off;
+    uint32_t nslots, slot_cap;
+    uint64_t back_off;
+    uint32_t back_used, back_cap;
+    uint64_t hash_off;
+    uint32_t hash_cap, hash_used;
+};
+
+struct GroupHdr {
+    uint64_t off;
+    uint32_t size, cap;
+};
+
+struct AliasEnt {
+    uint64_t T;          // stay iff coin64 < T
+    uint64_t off_stay;   // group i locator
+    uint32_t size_stay;
+    uint32_t size_alias;
+    uint64_t off_alias;  // partner locator; partner index in bits 56..63
+};
+static_assert(sizeof(AliasEnt) == 32, "alias entry is one 32-byte read");
+static_assert(sizeof(NodeHdr) == 48, "node header layout");
+
+struct Ctl {
+    unsigned long long top[NPOOL];   // pool cursors (elements)
+    unsigned long long need[NPOOL];  // planned growth of the pending batch
+    unsigned int error;              // ErrFlag bits
+    unsigned int pad;
+    unsigned long long steps;        // last walk: appended vertices
+    unsigned long long attempts;     // last walk: sample draws (node2vec rejections included)
+    unsigned long long probes;       // last walk: neighbour-table lookups
+};
+
+struct DevEdge {
+    int32_t u, v;
+    uint32_t w;
+};
+
+struct DevStore {
+    int32_t n;
+    int32_t nbr;
+    int32_t min_slots;
+    int32_t pad;
+    NodeHdr *hdr;
+    GroupHdr *grp;
+    AliasEnt *alias;
+    uint64_t *hinfo;
+    uint32_t *slot;
+    uint32_t *back;
+    uint64_t *mem;
+    uint64_t *hash;
+    unsigned long long cap[NPOOL];
+    Ctl *ctl;
+};
+
+struct WalkArgs {
+    const int32_t *starts;
+    const int32_t *lens;
+    int32_t *paths;
+    unsigned long long *steps;
+    int64_t nwalkers;
+    uint64_t seed;
+    uint32_t thr_return, thr_neighbor, thr_other, thr_stop;
+    int32_t max_len;
+    int32_t max_attempts;
+};
+
+struct Runs {          // per-node runs of a sorted batch
+    const uint32_t *node;
+    const uint32_t *start;
+    const uint32_t *count;
+    const uint32_t *nruns;  // device scalar
+};
+
+// ---- launchers (kernels.hip); all enqueue on `st` ----
+void launch_reset_ctl(Ctl *ctl, hipStream_t st);
+void launch_reset_walk(Ctl *ctl, hipStream_t st);
+void launch_init_nodes(DevStore s, hipStream_t st);
+void launch_insert_keys(const DevEdge *e, int64_t k, int32_t n, uint32_t *keys, uint32_t *vals, Ctl *ctl,
+                        hipStream_t st);
+void launch_gather_edges(const DevEdge *e, const uint32_t *order, int64_t k, DevEdge *out, hipStream_t st);
+void launch_insert_plan(DevStore s, const DevEdge *sorted, Runs r, int64_t k, hipStream_t st);
+void launch_insert_apply(DevStore s, const DevEdge *sorted, Runs r, int64_t k, hipStream_t st);
+void launch_delete_keys(const int32_t *dels, int64_t k, unsigned long long *keys, hipStream_t st);
+void launch_delete_check(DevStore s, const unsigned long long *keys, int64_t k, uint32_t *us, uint32_t *slots,
+                         hipStream_t st);
+void launch_delete_apply(DevStore s, const uint32_t *slots, Runs r, int64_t k, hipStream_t st);
+void launch_walk(DevStore s, int app, const WalkArgs &a, int block, hipStream_t st);
+void launch_export_a(DevStore s, uint32_t *nslots, uint64_t *mass, uint32_t *bsize, uint64_t *T, uint8_t *alias,
+                     hipStream_t st);
+void launch_export_b(DevStore s, const uint64_t *slot_out_off, const uint64_t *mem_out_off, uint32_t *slot_v,
+                     uint32_t *slot_w, uint64_t *members, uint32_t *backpos, hipStream_t st);
+void launch_edge_count(DevStore s, const int32_t *us, const int32_t *vs, int64_t k, uint32_t *counts,
+                       hipStream_t st);
+
+// rocPRIM wrappers (temp storage sized by a null call)
+size_t sort_pairs_u32(void *tmp, size_t tmp_bytes, const uint32_t *kin, uint32_t *kout, const uint32_t *vin,
+                      uint32_t *vout, int64_t k, int bits, hipStream_t st);
+size_t sort_keys_u64(void *tmp, size_t tmp_bytes, const unsigned long long *kin, unsigned long long *kout, int64_t k,
+                     int bits, hipStream_t st);
+size_t run_length_encode_u32(void *tmp, size_t tmp_bytes, const uint32_t *keys, int64_t k, uint32_t *uniq,
+                             uint32_t *counts, uint32_t *nruns, hipStream_t st);
+size_t exclusive_scan_u32(void *tmp, size_t tmp_bytes, const uint32_t *in, uint32_t *out, int64_t k,
+                          hipStream_t st);
+
+}  // namespace bingo

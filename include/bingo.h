@@ -1,0 +1,167 @@
+/*
+ * bingo.h - C ABI of the MI355X (gfx950) bias-factorized random-walk engine.
+ *
+ * Every entry point takes plain pointers and sizes; no C++ or torch types
+ * cross this boundary.  Node ids are 1..n as in the reference, whose loader
+ * shifts 0-based input ids by one (include/graph.cuh:22-27).  All calls
+ * return 0 or a negative BINGO_E* code; bingo_last_error() describes the
+ * last failure of the calling thread.  A handle is single-writer: calls on
+ * one handle must not overlap.  Work is ordered on the handle's own HIP
+ * stream; host-pointer entry points return after the work has finished.
+ *
+ * Reference surface replaced (Joeyw02/Bingo):
+ *   types ........ include/type.h:4-36 (app, utype, Edges, Deleted)
+ *   constants .... include/api.h:5-30 (P, Q, TP, LEN, ORIGINAL, RATE)
+ *   build ........ include/operation.cuh:33 build(Edges*, amount, ...)
+ *   insert ....... include/operation.cuh:144 insert(Edges*, amount, ...)
+ *   deleteE ...... include/operation.cuh:215 deleteE(Deleted*, amount, ...)
+ *   randomWalk ... include/graph.cuh:93 randomWalk(app)
+ *   instance ..... include/apps.h:4 int instance(app, utype, std::string)
+ */
+#ifndef BINGO_H
+#define BINGO_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BINGO_OK 0
+#define BINGO_EINVAL (-1)      /* bad argument or batch entry (whole batch rejected) */
+#define BINGO_ENOMEM (-2)      /* HBM budget or device allocation exhausted */
+#define BINGO_EDEVICE (-3)     /* HIP runtime error */
+#define BINGO_ESTATE (-4)      /* call not valid in the handle's state */
+#define BINGO_EUNSUPPORTED (-5) /* feature disabled for this handle */
+
+/* ref enum class app (type.h:4-10) */
+typedef enum { BINGO_DEEPWALK = 0, BINGO_NODE2VEC = 1, BINGO_PPR = 2, BINGO_SAMPLING = 3 } bingo_app;
+/* ref enum class utype (type.h:16-21): insert-only, delete-only, mixed */
+typedef enum { BINGO_UPD_INSERT = 0, BINGO_UPD_DELETE = 1, BINGO_UPD_MIXED = 2 } bingo_utype;
+/* synthetic weight laws of the workload generators */
+typedef enum { BINGO_W_CONST1 = 0, BINGO_W_UNIFORM16 = 1, BINGO_W_LOGUNI32 = 2, BINGO_W_ZIPF24 = 3 } bingo_wkind;
+
+/* ref struct Edges{int u,v; float weight} (type.h:24-29), weight widened to u32 */
+typedef struct { int32_t u, v; uint32_t w; } bingo_edge;
+/* ref struct Deleted{int u,id} (type.h:33-37): id is the per-node slot ordinal */
+typedef struct { int32_t u, slot; } bingo_del;
+
+typedef struct bingo_graph bingo_graph;
+
+typedef struct {
+    int32_t device;         /* HIP device ordinal */
+    int32_t neighbor_index; /* keep exact neighbour multisets (needed by node2vec; ref NODE2VEC) */
+    uint64_t hbm_budget;    /* bytes of HBM the store may use; 0 = 90% of free memory */
+    double pool_slack;      /* extra pool room reserved at load, as a fraction (0.25) */
+    int32_t min_slots;      /* minimum slot capacity per node (ref ORIGINAL = 16) */
+} bingo_config;
+
+typedef struct {
+    uint64_t seed;          /* Philox key; walker i uses counter (i, step, attempt) */
+    double p, q;            /* node2vec return / in-out parameters (ref P=0.5, Q=2) */
+    double tp;              /* PPR stop probability per step (ref TP=0.0125) */
+    int32_t max_len;        /* path columns, vertices per walk incl. start (ref LEN=80) */
+    int32_t max_attempts;   /* node2vec rejection cap per step; last draw accepted */
+    int32_t block;          /* launch tuning: threads per block (0 = autotuned/default) */
+    int32_t reserved;
+} bingo_walk_params;
+
+typedef struct {
+    double load_ms, insert_ms, delete_ms, walk_ms; /* HIP-event time of the last call */
+    double walk_kernel_ms;  /* device time of the last walk kernel */
+    uint64_t walk_steps;    /* appended vertices of the last walk */
+    uint64_t walk_attempts; /* sample draws of the last walk (node2vec rejections included) */
+    uint64_t walk_probes;   /* neighbour-table lookups of the last walk (node2vec) */
+    uint64_t arcs;          /* live arcs */
+    uint64_t bytes_reserved;/* HBM held by the store */
+    uint64_t pool_used[4];  /* slot, back-index, member, neighbour pools (elements) */
+} bingo_stats;
+
+/* Flat export for bit-exact diffs (node-major; see bingo_export). */
+typedef struct {
+    uint64_t total_slots;
+    uint64_t total_members;
+    uint32_t *nslots;   /* [n+1] */
+    uint64_t *mass;     /* [n+1] sum of live weights */
+    uint32_t *bsize;    /* [(n+1)*32] bucket sizes */
+    uint64_t *T;        /* [(n+1)*32] alias stay thresholds */
+    uint8_t *alias;     /* [(n+1)*32] alias partners */
+    uint32_t *slot_v;   /* [total_slots] */
+    uint32_t *slot_w;   /* [total_slots] 0 = deleted */
+    uint64_t *members;  /* [total_members] (v << 32) | slot, per node, per group */
+    uint32_t *backpos;  /* [total_members] per live slot, its set bits ascending */
+} bingo_dump;
+
+/* ---- lifetime ---- */
+void bingo_config_default(bingo_config *cfg);
+void bingo_walk_params_default(bingo_walk_params *p);
+int bingo_create(const bingo_config *cfg, bingo_graph **out);
+void bingo_destroy(bingo_graph *g);
+const char *bingo_last_error(void);
+const char *bingo_version(void);
+
+/* ---- graph load: ref build() (operation.cuh:33-121) ----
+ * Arcs are taken in array order; the k-th arc of a node gets slot k (as
+ * graph.cuh:38-42 numbers nodeIdu).  u, v in 1..n; w >= 1. */
+int bingo_load(bingo_graph *g, int32_t n, const bingo_edge *edges, int64_t m);
+int bingo_load_device(bingo_graph *g, int32_t n, const bingo_edge *d_edges, int64_t m);
+
+/* ---- batched insert: ref insert() (operation.cuh:144-197) ----
+ * Arcs of one node take its next free slots in batch order. */
+int bingo_insert(bingo_graph *g, const bingo_edge *edges, int64_t k);
+int bingo_insert_device(bingo_graph *g, const bingo_edge *d_edges, int64_t k);
+
+/* ---- batched delete: ref deleteE() (operation.cuh:215-251) ----
+ * Each (u, slot) must be live and appear once; the slot is tombstoned. */
+int bingo_delete(bingo_graph *g, const bingo_del *dels, int64_t k);
+int bingo_delete_device(bingo_graph *g, const bingo_del *d_dels, int64_t k);
+
+/* ---- walks: ref randomWalk() (graph.cuh:93-150) and the kernels of
+ * operation.cuh:252-368 (DeepWalk, node2vec, PPR) and 407-430 (sampling).
+ * paths is [nwalkers][params->max_len] int32, -1 padded.  starts NULL:
+ * walker i starts at (i % n) + 1.  lens NULL: max_len for every walker.
+ * steps_out (may be NULL) receives the number of appended vertices. */
+int bingo_walk(bingo_graph *g, int app, const bingo_walk_params *params, const int32_t *starts,
+               const int32_t *lens, int64_t nwalkers, int32_t *paths, uint64_t *steps_out);
+/* device-pointer form, enqueued on the handle's stream; *d_steps (device
+ * uint64) is incremented by the steps.  Returns without waiting. */
+int bingo_walk_device(bingo_graph *g, int app, const bingo_walk_params *params,
+                      const int32_t *d_starts, const int32_t *d_lens, int64_t nwalkers,
+                      int32_t *d_paths, uint64_t *d_steps);
+
+/* ---- inspection ---- */
+int32_t bingo_num_nodes(const bingo_graph *g);
+int bingo_stats_get(bingo_graph *g, bingo_stats *out);
+int bingo_sync(bingo_graph *g);
+void *bingo_stream(bingo_graph *g);      /* the handle's hipStream_t */
+/* multiplicities of live arcs us[i] -> vs[i] (host arrays) */
+int bingo_edge_count(bingo_graph *g, const int32_t *us, const int32_t *vs, int64_t k, uint32_t *counts);
+/* two calls: first with all pointers NULL to learn the totals */
+int bingo_export(bingo_graph *g, bingo_dump *d);
+
+/* ---- device memory helpers for callers without a HIP runtime binding ---- */
+int bingo_dev_alloc(bingo_graph *g, size_t bytes, void **d_ptr);
+int bingo_dev_free(bingo_graph *g, void *d_ptr);
+int bingo_memcpy_h2d(bingo_graph *g, void *d_dst, const void *src, size_t bytes);
+int bingo_memcpy_d2h(bingo_graph *g, void *dst, const void *d_src, size_t bytes);
+/* HIP events on the handle's stream: record into slot (0..15), elapsed ms */
+int bingo_event_record(bingo_graph *g, int slot);
+int bingo_event_elapsed(bingo_graph *g, int slot_a, int slot_b, float *ms);
+
+/* ---- workload generators (host, deterministic; ref graph.cuh:16-92) ----
+ * R-MAT arcs of edges [edge_begin, edge_begin + nedges) at the given scale,
+ * probabilities a, b, c (d = 1 - a - b - c); undirected emits u->v, v->u. */
+int bingo_gen_rmat(int scale, int64_t edge_begin, int64_t nedges, double a, double b, double c,
+                   int wkind, uint64_t seed, int undirected, bingo_edge *out);
+typedef struct bingo_updgen bingo_updgen;
+int bingo_updgen_create(int32_t n, int scale, const bingo_edge *base, int64_t m, int64_t edge_begin,
+                        double a, double b, double c, int wkind, uint64_t seed, bingo_updgen **out);
+void bingo_updgen_destroy(bingo_updgen *s);
+int bingo_updgen_inserts(bingo_updgen *s, int64_t k, bingo_edge *out);
+int bingo_updgen_deletes(bingo_updgen *s, int64_t k, bingo_del *out);
+int64_t bingo_updgen_live(const bingo_updgen *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

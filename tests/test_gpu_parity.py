@@ -1,0 +1,250 @@
+"""GPU parity: the HIP store, its batched updates and its walks against the
+CPU oracle, bit for bit, on the same seeded inputs (calls go through the C
+ABI in include/bingo.h)."""
+import random
+
+import numpy as np
+import pytest
+
+import bingo_amd as B
+from tests.brute import BruteGraph, check_against_brute
+
+pytestmark = pytest.mark.gpu
+
+APPS = [B.DEEPWALK, B.NODE2VEC, B.PPR, B.SAMPLING]
+
+
+def _pair(orc, n, edges, nbr=True):
+    print(f"[parity] load n={n} m={len(edges)} nbr={nbr}", flush=True)
+    g = B.Graph(neighbor_index=nbr)
+    g.load(n, edges)
+    o = orc.OracleGraph(n, nbr)
+    assert o.load(edges) == 0
+    return g, o
+
+
+def _same_state(g, o, what=""):
+    a, b = g.export(), o.export()
+    for k in b:
+        assert a[k].shape == b[k].shape, (what, k, a[k].shape, b[k].shape)
+        if not np.array_equal(a[k], b[k]):
+            bad = np.argwhere(a[k] != b[k])[:5]
+            raise AssertionError(f"{what}: {k} differs at {bad.tolist()}")
+
+
+def _params(orc, seed, max_len, p=0.5, q=2.0, tp=0.0125, max_attempts=4096):
+    return (B.walk_params(seed=seed, p=p, q=q, tp=tp, max_len=max_len, max_attempts=max_attempts),
+            orc.walk_params(seed=seed, p=p, q=q, tp=tp, max_len=max_len, max_attempts=max_attempts))
+
+
+def _same_walks(g, o, orc, app, seed, max_len, starts=None, lens=None, nwalkers=None, **kw):
+    gp, op = _params(orc, seed, max_len, **kw)
+    pa, sa = g.walk(app, gp, starts=starts, lens=lens, nwalkers=nwalkers)
+    pb, sb = o.walk(app, op, starts=starts, lens=lens, nwalkers=nwalkers)
+    if not np.array_equal(pa, pb):
+        bad = np.argwhere((pa != pb).any(axis=1))[:5, 0]
+        raise AssertionError(f"app {app} seed {seed}: walkers {bad.tolist()} differ\n{pa[bad[0]]}\n{pb[bad[0]]}")
+    assert sa == sb
+    return pa, sa
+
+
+def _rand_edges(rng, n, k, hub=None, wbits=32):
+    e = np.zeros(k, B.EDGE_DT)
+    e["u"] = [hub if (hub and rng.random() < 0.4) else rng.randint(1, n) for _ in range(k)]
+    e["v"] = [rng.randint(1, n) for _ in range(k)]
+    e["w"] = [max(1, rng.getrandbits(rng.randint(1, wbits))) for _ in range(k)]
+    return e
+
+
+def test_load_and_walks_small(orc):
+    rng = random.Random(1)
+    n = 300
+    e = _rand_edges(rng, n, 4000, hub=7)
+    g, o = _pair(orc, n, e)
+    _same_state(g, o, "load")
+    bg = BruteGraph(n)
+    bg.insert(e)
+    check_against_brute(g.export(), bg)
+    starts = np.array([rng.randint(1, n) for _ in range(3000)], np.int32)
+    lens = np.array([rng.randint(1, 40) for _ in range(3000)], np.int32)
+    for app in APPS:
+        _same_walks(g, o, orc, app, seed=11 + app, max_len=40, starts=starts, lens=lens)
+        _same_walks(g, o, orc, app, seed=5, max_len=17, nwalkers=n)
+
+
+@pytest.mark.parametrize("nbr", [True, False])
+def test_update_rounds_match_oracle(orc, nbr):
+    scale = 10
+    n = 1 << scale
+    base = B.gen_rmat(scale, 6000, B.W_LOGUNI32, seed=3, undirected=False)
+    g, o = _pair(orc, n, base, nbr)
+    ug = B.UpdateGen(n, scale, base, 6000, B.W_LOGUNI32, seed=3)
+    bg = BruteGraph(n)
+    bg.insert(base)
+    for rnd, (ki, kd) in enumerate([(1, 1), (7, 3), (1000, 900), (5000, 5000), (0, 2000), (3000, 0)]):
+        print(f"[parity] round {rnd} inserts={ki} deletes={kd}", flush=True)
+        ins, dels = ug.inserts(ki), ug.deletes(kd)
+        g.insert(ins)
+        assert o.insert(ins) == 0
+        g.delete(dels)
+        assert o.delete(dels) == 0
+        bg.insert(ins)
+        bg.delete(dels)
+        _same_state(g, o, f"round {rnd}")
+        app = [B.DEEPWALK, B.PPR, B.SAMPLING][rnd % 3] if not nbr else APPS[rnd % 4]
+        _same_walks(g, o, orc, app, seed=100 + rnd, max_len=30, nwalkers=2 * n)
+    check_against_brute(g.export(), bg)
+
+
+def test_hub_node_long_runs(orc):
+    """One node with tens of thousands of arcs: multi-chunk ballots, bucket
+    growth and neighbour-table rehash inside one batch, thousands of deletes
+    of one node in one batch."""
+    rng = random.Random(7)
+    n = 5000
+    e = _rand_edges(rng, n, 20000, hub=1)
+    g, o = _pair(orc, n, e)
+    for rnd in range(3):
+        ins = _rand_edges(rng, n, 30000, hub=1)
+        g.insert(ins)
+        assert o.insert(ins) == 0
+        d = o.export()
+        so = np.concatenate([[0], np.cumsum(d["nslots"].astype(np.int64))])
+        hub_live = [s for s in range(int(d["nslots"][1])) if d["slot_w"][so[1] + s]]
+        pick = rng.sample(hub_live, len(hub_live) // 3)
+        other = []
+        for u in rng.sample(range(2, n + 1), 500):
+            lw = [s for s in range(int(d["nslots"][u])) if d["slot_w"][so[u] + s]]
+            if lw:
+                other.append((u, rng.choice(lw)))
+        dels = np.array([(1, s) for s in pick] + other, B.DEL_DT)
+        rng.shuffle(dels)
+        g.delete(dels)
+        assert o.delete(dels) == 0
+        _same_state(g, o, f"hub round {rnd}")
+        starts = np.ones(4096, np.int32)
+        for app in APPS:
+            _same_walks(g, o, orc, app, seed=rnd * 10 + app, max_len=12, starts=starts)
+
+
+def test_config1_plain_small(orc):
+    """The plain small workload: undirected R-MAT (0.57, 0.19, 0.19, 0.05),
+    2^17 nodes, 2^21 edges, u32 weights uniform in [1, 2^16); DeepWalk one
+    walker per node, length 80; then 10^4 inserts + 10^4 deletes."""
+    scale, m = 17, 1 << 21
+    n = 1 << scale
+    base = B.gen_rmat(scale, m, B.W_UNIFORM16, seed=2026)
+    g, o = _pair(orc, n, base, nbr=True)
+    _same_state(g, o, "C1 load")
+    pa, steps = _same_walks(g, o, orc, B.DEEPWALK, seed=1, max_len=80, nwalkers=n)
+    assert steps > n * 40
+    ug = B.UpdateGen(n, scale, base, m, B.W_UNIFORM16, seed=2026)
+    ins, dels = ug.inserts(10_000), ug.deletes(10_000)
+    g.insert(ins)
+    o.insert(ins)
+    g.delete(dels)
+    o.delete(dels)
+    _same_state(g, o, "C1 after updates")
+    _same_walks(g, o, orc, B.DEEPWALK, seed=2, max_len=80, nwalkers=n)
+    _same_walks(g, o, orc, B.NODE2VEC, seed=3, max_len=80, nwalkers=n // 4)
+
+
+def test_invalid_batches_rejected_without_change(orc):
+    rng = random.Random(4)
+    n = 50
+    e = _rand_edges(rng, n, 400)
+    g, o = _pair(orc, n, e)
+    bad_ins = [
+        np.array([(0, 1, 1)], B.EDGE_DT),
+        np.array([(1, n + 1, 1)], B.EDGE_DT),
+        np.array([(1, 2, 0)], B.EDGE_DT),
+        np.concatenate([_rand_edges(rng, n, 10), np.array([(-3, 2, 5)], B.EDGE_DT)]),
+    ]
+    for b in bad_ins:
+        with pytest.raises(B.BingoError) as ei:
+            g.insert(b)
+        assert ei.value.code == -1
+    d = o.export()
+    bad_dels = [
+        np.array([(1, int(d["nslots"][1]))], B.DEL_DT),
+        np.array([(n + 1, 0)], B.DEL_DT),
+        np.array([(2, 0), (2, 0)], B.DEL_DT),
+        np.array([(3, -1)], B.DEL_DT),
+    ]
+    for b in bad_dels:
+        with pytest.raises(B.BingoError) as ei:
+            g.delete(b)
+        assert ei.value.code == -1
+    _same_state(g, o, "after rejected batches")
+    g.delete(np.array([(2, 0)], B.DEL_DT))
+    o.delete(np.array([(2, 0)], B.DEL_DT))
+    with pytest.raises(B.BingoError):
+        g.delete(np.array([(2, 0)], B.DEL_DT))  # already dead
+    _same_state(g, o, "after dead-slot delete")
+    with pytest.raises(B.BingoError):
+        g.walk(B.DEEPWALK, B.walk_params(max_len=5), starts=np.array([0], np.int32))
+    with pytest.raises(B.BingoError):
+        g.walk(B.DEEPWALK, B.walk_params(max_len=5), starts=np.array([1], np.int32), lens=np.array([6], np.int32))
+
+
+def test_edge_count_matches_oracle(orc):
+    rng = random.Random(8)
+    n = 80
+    e = _rand_edges(rng, n, 3000)
+    g, o = _pair(orc, n, e)
+    dels = np.array(list({(int(x["u"]), rng.randint(0, 5)) for x in e[:200]}), B.DEL_DT)
+    d = o.export()
+    dels = dels[dels["slot"] < d["nslots"][dels["u"]]]
+    g.delete(dels)
+    o.delete(dels)
+    us = np.array([rng.randint(0, n + 1) for _ in range(5000)], np.int32)
+    vs = np.array([rng.randint(0, n + 1) for _ in range(5000)], np.int32)
+    got = g.edge_count(us, vs)
+    want = [o.edge_count(int(u), int(v)) for u, v in zip(us, vs)]
+    assert got.tolist() == want
+
+
+def test_device_entry_points(orc):
+    rng = random.Random(9)
+    n = 500
+    e = _rand_edges(rng, n, 8000)
+    o = orc.OracleGraph(n)
+    o.load(e)
+    g = B.Graph()
+    de = B.DeviceArray(g, e)
+    g.load_device(n, de, len(e))
+    ins = _rand_edges(rng, n, 3000)
+    di = B.DeviceArray(g, ins)
+    g.insert_device(di, len(ins))
+    o.insert(ins)
+    nw, L = 2000, 25
+    dp = B.DeviceArray(g, nbytes=nw * L * 4)
+    ds = B.DeviceArray(g, np.zeros(1, np.uint64))
+    gp, op = _params(orc, 77, L)
+    g.walk_device(B.NODE2VEC, gp, nw, dp, ds)
+    g.sync()
+    pa = dp.to_host(np.int32, (nw, L))
+    sa = int(ds.to_host(np.uint64, (1,))[0])
+    pb, sb = o.walk(B.NODE2VEC, op, nwalkers=nw)
+    assert np.array_equal(pa, pb) and sa == sb
+    st = g.stats()
+    assert st["arcs"] == len(e) + len(ins) and st["walk_ms"] > 0
+    for a in (de, di, dp, ds):
+        a.free()
+
+
+def test_gpu_transition_law_chi_square(orc):
+    from scipy import stats
+
+    rng = random.Random(12)
+    w = [max(1, rng.getrandbits(rng.randint(1, 32))) for _ in range(40)]
+    n = 41
+    e = np.array([(1, i + 2, x) for i, x in enumerate(w)], B.EDGE_DT)
+    g = B.Graph()
+    g.load(n, e)
+    N = 1_000_000
+    paths, steps = g.walk(B.SAMPLING, B.walk_params(seed=5, max_len=2), starts=np.ones(N, np.int32))
+    cnt = np.bincount(paths[:, 1], minlength=n + 1)[2:]
+    exp = np.array(w, float) / sum(w) * N
+    keep = exp > 5
+    assert stats.chisquare(cnt[keep], exp[keep] * cnt[keep].sum() / exp[keep].sum()).pvalue > 0.01
