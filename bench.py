@@ -81,12 +81,29 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the library's plain copy kernel over MIB MiB (traffic-counter reference)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the store is built and walked on a single MI355X; "
               "multi-GPU sharding is a design note only. Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     import bingo_amd as B
+
+    if args.copy_calibration:
+        nbytes = args.copy_calibration << 20
+        g = B.Graph(device=0)
+        src = B.DeviceArray(g, np.arange(nbytes // 8, dtype=np.uint64))
+        dst = B.DeviceArray(g, nbytes=nbytes)
+        for i in range(3):
+            g.event(0)
+            B._check(B.lib().bingo_memcpy_d2d(g._h, dst.ptr, src.ptr, nbytes))
+            g.event(1)
+            ms = g.elapsed_ms(0, 1)
+            log(f"copy {i}: {nbytes} bytes in {ms:.3f} ms = {2 * nbytes / ms / 1e6:.1f} GB/s (read+write)")
+        ok = bool((dst.to_host(np.uint64, (nbytes // 8,)) == np.arange(nbytes // 8, dtype=np.uint64)).all())
+        print(json.dumps({"copy_bytes": nbytes, "kernel": "k_copy16", "ms": ms, "correct": ok}), flush=True)
+        return
 
     cfg = CONFIGS[args.config]
     scale, m = cfg["scale"], cfg["edges"]

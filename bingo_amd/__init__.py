@@ -55,7 +55,11 @@ class WalkParams(C.Structure):
         ("tp", C.c_double),
         ("max_len", C.c_int32),
         ("max_attempts", C.c_int32),
+        ("form", C.c_int32),
         ("block", C.c_int32),
+        ("walkers_per_lane", C.c_int32),
+        ("lds_depth", C.c_int32),
+        ("blocks_per_cu", C.c_int32),
         ("reserved", C.c_int32),
     ]
 
@@ -109,6 +113,7 @@ SIGNATURES = {
     "bingo_delete_device": (_int, [_vp, _vp, _i64]),
     "bingo_walk": (_int, [_vp, _int, C.POINTER(WalkParams), _vp, _vp, _i64, _vp, C.POINTER(_u64)]),
     "bingo_walk_device": (_int, [_vp, _int, C.POINTER(WalkParams), _vp, _vp, _i64, _vp, _vp]),
+    "bingo_walk_tune": (_int, [_vp, _int, C.POINTER(WalkParams), _i64, C.POINTER(WalkParams)]),
     "bingo_num_nodes": (_i32, [_vp]),
     "bingo_stats_get": (_int, [_vp, C.POINTER(Stats)]),
     "bingo_sync": (_int, [_vp]),
@@ -119,6 +124,7 @@ SIGNATURES = {
     "bingo_dev_free": (_int, [_vp, _vp]),
     "bingo_memcpy_h2d": (_int, [_vp, _vp, _vp, C.c_size_t]),
     "bingo_memcpy_d2h": (_int, [_vp, _vp, _vp, C.c_size_t]),
+    "bingo_memcpy_d2d": (_int, [_vp, _vp, _vp, C.c_size_t]),
     "bingo_event_record": (_int, [_vp, _int]),
     "bingo_event_elapsed": (_int, [_vp, _int, _int, C.POINTER(C.c_float)]),
     "bingo_gen_rmat": (_int, [_int, _i64, _i64, C.c_double, C.c_double, C.c_double, _int, _u64, _int, _vp]),
@@ -158,11 +164,16 @@ def _check(rc):
     return rc
 
 
-def walk_params(seed=1, p=0.5, q=2.0, tp=0.0125, max_len=80, max_attempts=4096, block=0) -> WalkParams:
+FORM_AUTO, FORM_PLAIN, FORM_PERSISTENT = 0, 1, 2
+
+
+def walk_params(seed=1, p=0.5, q=2.0, tp=0.0125, max_len=80, max_attempts=4096, form=FORM_AUTO, block=0,
+                walkers_per_lane=0, lds_depth=-1, blocks_per_cu=0) -> WalkParams:
     wp = WalkParams()
     lib().bingo_walk_params_default(C.byref(wp))
     wp.seed, wp.p, wp.q, wp.tp = seed, p, q, tp
-    wp.max_len, wp.max_attempts, wp.block = max_len, max_attempts, block
+    wp.max_len, wp.max_attempts, wp.form, wp.block = max_len, max_attempts, form, block
+    wp.walkers_per_lane, wp.lds_depth, wp.blocks_per_cu = walkers_per_lane, lds_depth, blocks_per_cu
     return wp
 
 
@@ -293,6 +304,12 @@ class Graph:
                     d_starts: DeviceArray | None = None, d_lens: DeviceArray | None = None):
         _check(lib().bingo_walk_device(self._h, app, C.byref(params), d_starts.ptr if d_starts else None,
                                        d_lens.ptr if d_lens else None, nwalkers, d_paths.ptr, d_steps.ptr))
+
+    def tune(self, app, params: WalkParams, sample):
+        """Autotune the walk schedule for `app`; returns the chosen knobs."""
+        out = WalkParams()
+        _check(lib().bingo_walk_tune(self._h, app, C.byref(params), sample, C.byref(out)))
+        return dict(form=out.form, block=out.block, walkers_per_lane=out.walkers_per_lane, lds_depth=out.lds_depth)
 
     # -- inspection --
     def sync(self):

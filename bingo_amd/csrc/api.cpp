@@ -82,6 +82,12 @@ struct bingo_graph {
     double ph_ms[NPHASE] = {};
     uint64_t last_walk_steps = 0;
     unsigned long long *d_steps = nullptr;
+    int num_cu = 0;
+    struct Tuned {
+        bool valid = false;
+        int32_t max_len = 0;
+        int form = BINGO_FORM_PERSISTENT, block = 256, wpl = 2, depth = 0;
+    } tuned[4];
 };
 
 static int ensure_scratch(bingo_graph *g, int idx, size_t bytes)
@@ -291,6 +297,57 @@ static int do_delete(bingo_graph *g, const int32_t *d_pairs, int64_t k)
     return BINGO_OK;
 }
 
+// How a walk call is scheduled: the plain form (one thread per walker) or
+// the persistent form with explicit, tuned or default knobs.
+struct WalkChoice {
+    int form, block, wpl, depth;
+};
+
+static bool choice_ok(const WalkChoice &c, int32_t max_len)
+{
+    const bool blk = c.block == 64 || c.block == 128 || c.block == 256 || c.block == 512;
+    if (c.form == BINGO_FORM_PLAIN) return blk;
+    return blk && (c.wpl == 1 || c.wpl == 2 || c.wpl == 4) && (c.depth == 0 || c.depth == 4 || c.depth == 16) &&
+           (c.depth == 0 || max_len % c.depth == 0);
+}
+
+static int resolve_walk(bingo_graph *g, int app, const bingo_walk_params *p, WalkChoice *c)
+{
+    if (p->form == BINGO_FORM_PLAIN) {
+        *c = WalkChoice{BINGO_FORM_PLAIN, p->block ? p->block : 256, 1, 0};
+    } else if (p->form == BINGO_FORM_AUTO && g->tuned[app].valid && g->tuned[app].max_len == p->max_len) {
+        const auto &t = g->tuned[app];
+        *c = WalkChoice{t.form, t.block, t.wpl, t.depth};
+    } else if (p->form == BINGO_FORM_AUTO || p->form == BINGO_FORM_PERSISTENT) {
+        *c = WalkChoice{BINGO_FORM_PERSISTENT, p->block ? p->block : 256, p->walkers_per_lane ? p->walkers_per_lane : 2,
+                        p->lds_depth >= 0 ? p->lds_depth : 0};
+    } else {
+        return fail(BINGO_EINVAL, "unknown walk form");
+    }
+    if (!choice_ok(*c, p->max_len))
+        return fail(BINGO_EINVAL, "walk knobs: block 64..512 (power of two), walkers_per_lane 1/2/4, lds_depth "
+                                  "0/4/16 dividing max_len");
+    return BINGO_OK;
+}
+
+static int launch_choice(bingo_graph *g, int app, const WalkChoice &c, const WalkArgs &a, int32_t blocks_per_cu)
+{
+    if (c.form == BINGO_FORM_PLAIN) {
+        launch_walk(g->s, app, a, c.block, g->st);
+    } else {
+        int nb = persist_max_blocks_per_cu(app, c.wpl, c.depth, c.block);
+        if (nb < 1) return fail(BINGO_EINVAL, "persistent walk configuration does not fit a CU");
+        if (blocks_per_cu > 0 && blocks_per_cu < nb) nb = blocks_per_cu;
+        const int64_t per_block = (int64_t)c.block * c.wpl;
+        int64_t blocks = (int64_t)g->num_cu * nb;
+        blocks = std::min<int64_t>(blocks, (a.nwalkers + per_block - 1) / per_block);
+        PersistCfg pc{c.block, c.wpl, c.depth, (int)std::max<int64_t>(blocks, 1)};
+        launch_walk_persist(g->s, app, a, pc, g->st);
+    }
+    HIPCHK(hipGetLastError());
+    return BINGO_OK;
+}
+
 static uint32_t thr27(double prob)
 {
     if (!(prob > 0)) return 0;
@@ -344,7 +401,11 @@ void bingo_walk_params_default(bingo_walk_params *p)
     p->tp = 0.0125;      // ref TP (include/api.h:20)
     p->max_len = 80;     // ref LEN (include/api.h:12)
     p->max_attempts = 4096;
+    p->form = BINGO_FORM_AUTO;
     p->block = 0;
+    p->walkers_per_lane = 0;
+    p->lds_depth = -1;
+    p->blocks_per_cu = 0;
     p->reserved = 0;
 }
 
@@ -378,6 +439,8 @@ int bingo_create(const bingo_config *cfg, bingo_graph **out)
     }
     size_t fr = 0, tot = 0;
     (void)hipMemGetInfo(&fr, &tot);
+    (void)hipDeviceGetAttribute(&g->num_cu, hipDeviceAttributeMultiprocessorCount, g->device);
+    if (g->num_cu < 1) g->num_cu = 1;
     g->budget = g->cfg.hbm_budget ? g->cfg.hbm_budget : (uint64_t)(0.9 * (double)fr);
     bool ok = hipHostMalloc((void **)&g->ctl_host, sizeof(Ctl), hipHostMallocDefault) == hipSuccess &&
               hipMalloc((void **)&g->s.ctl, sizeof(Ctl)) == hipSuccess &&
@@ -522,12 +585,95 @@ int bingo_walk_device(bingo_graph *g, int app, const bingo_walk_params *params, 
     a.lens = d_lens;
     a.paths = d_paths;
     a.steps = reinterpret_cast<unsigned long long *>(d_steps);
+    a.cursor = &g->s.ctl->cursor;
     a.nwalkers = nwalkers;
+    WalkChoice c;
+    if ((rc = resolve_walk(g, app, params, &c))) return rc;
     launch_reset_walk(g->s.ctl, g->st);
     phase_begin(g, PH_WALK);
-    launch_walk(g->s, app, a, params->block, g->st);
+    rc = launch_choice(g, app, c, a, params->blocks_per_cu);
     phase_end(g, PH_WALK);
-    HIPCHK(hipGetLastError());
+    return rc;
+}
+
+int bingo_walk_tune(bingo_graph *g, int app, const bingo_walk_params *params, int64_t sample,
+                    bingo_walk_params *chosen)
+{
+    if (!g || !g->loaded) return fail(BINGO_ESTATE, "graph not loaded");
+    if (app == BINGO_NODE2VEC && !g->s.nbr) return fail(BINGO_EUNSUPPORTED, "node2vec needs neighbor_index=1");
+    WalkArgs a{};
+    int rc = fill_walk_args(params, app, &a);
+    if (rc) return rc;
+    if (sample < 1) return fail(BINGO_EINVAL, "sample must be >= 1");
+    const int32_t n = g->s.n;
+    std::vector<int32_t> starts((size_t)sample);
+    const int64_t stride = std::max<int64_t>(1, n / sample);
+    for (int64_t i = 0; i < sample; ++i) starts[(size_t)i] = (int32_t)((i * stride) % n) + 1;
+    void *dstarts = nullptr, *dpaths = nullptr, *dsteps = nullptr;
+    auto cleanup = [&]() {
+        for (void *p : {dstarts, dpaths, dsteps})
+            if (p) (void)hipFree(p);
+    };
+    if (hipMalloc(&dstarts, (size_t)sample * 4) || hipMalloc(&dpaths, (size_t)sample * params->max_len * 4) ||
+        hipMalloc(&dsteps, 8)) {
+        (void)hipGetLastError();
+        cleanup();
+        return fail(BINGO_ENOMEM, "tuner buffers");
+    }
+    (void)hipMemcpyAsync(dstarts, starts.data(), (size_t)sample * 4, hipMemcpyHostToDevice, g->st);
+    a.starts = static_cast<int32_t *>(dstarts);
+    a.lens = nullptr;
+    a.paths = static_cast<int32_t *>(dpaths);
+    a.steps = static_cast<unsigned long long *>(dsteps);
+    a.cursor = &g->s.ctl->cursor;
+    a.nwalkers = sample;
+    std::vector<WalkChoice> cand;
+    for (int b : {256, 512, 128}) cand.push_back(WalkChoice{BINGO_FORM_PLAIN, b, 1, 0});
+    for (int b : {128, 256, 512})
+        for (int w : {1, 2, 4})
+            for (int d : {0, 4, 16}) {
+                WalkChoice c{BINGO_FORM_PERSISTENT, b, w, d};
+                if (choice_ok(c, params->max_len) && persist_max_blocks_per_cu(app, w, d, b) > 0) cand.push_back(c);
+            }
+    float best = 1e30f;
+    WalkChoice pick = cand[0];
+    for (const WalkChoice &c : cand) {
+        float ms = 0;
+        for (int rep = 0; rep < 2; ++rep) {
+            launch_reset_walk(g->s.ctl, g->st);
+            (void)hipEventRecord(g->ev[kEvents - 2], g->st);
+            if ((rc = launch_choice(g, app, c, a, 0))) {
+                cleanup();
+                return rc;
+            }
+            (void)hipEventRecord(g->ev[kEvents - 1], g->st);
+            if (hipEventSynchronize(g->ev[kEvents - 1]) != hipSuccess ||
+                hipEventElapsedTime(&ms, g->ev[kEvents - 2], g->ev[kEvents - 1]) != hipSuccess) {
+                cleanup();
+                return fail(BINGO_EDEVICE, "tuner timing failed");
+            }
+        }
+        if (ms < best) {
+            best = ms;
+            pick = c;
+        }
+    }
+    cleanup();
+    auto &t = g->tuned[app];
+    t.valid = true;
+    t.max_len = params->max_len;
+    t.form = pick.form;
+    t.block = pick.block;
+    t.wpl = pick.wpl;
+    t.depth = pick.depth;
+    if (chosen) {
+        *chosen = *params;
+        chosen->form = pick.form;
+        chosen->block = pick.block;
+        chosen->walkers_per_lane = pick.wpl;
+        chosen->lds_depth = pick.depth;
+        chosen->blocks_per_cu = 0;
+    }
     return BINGO_OK;
 }
 
@@ -732,6 +878,15 @@ int bingo_memcpy_d2h(bingo_graph *g, void *dst, const void *d_src, size_t bytes)
     if (!g) return fail(BINGO_EINVAL, "null handle");
     HIPCHK(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, g->st));
     HIPCHK(hipStreamSynchronize(g->st));
+    return BINGO_OK;
+}
+
+int bingo_memcpy_d2d(bingo_graph *g, void *d_dst, const void *d_src, size_t bytes)
+{
+    if (!g || (bytes && (!d_dst || !d_src))) return fail(BINGO_EINVAL, "bad copy arguments");
+    if (((uintptr_t)d_dst | (uintptr_t)d_src) & 15) return fail(BINGO_EINVAL, "copy needs 16-byte aligned pointers");
+    if (bytes) launch_copy(d_dst, d_src, bytes, g->st);
+    HIPCHK(hipGetLastError());
     return BINGO_OK;
 }
 

@@ -10,6 +10,7 @@
 // deleteE / buildAliasTable / sample (include/Bingo.cuh:111-267), and the
 // walk kernels deepwalkKernel / node2vecKernel / pprKernel / samplingKernel
 // (operation.cuh:252-368, 407-430).
+#include <algorithm>
 #include <cstring>
 
 #include <rocprim/rocprim.hpp>
@@ -243,6 +244,7 @@ __global__ void k_reset_walk(Ctl *ctl)
         ctl->steps = 0;
         ctl->attempts = 0;
         ctl->probes = 0;
+        ctl->cursor = 0;
     }
 }
 
@@ -724,6 +726,189 @@ __global__ __launch_bounds__(256) void k_walk(DevStore s, WalkArgs a)
 }
 
 // ------------------------------------------------------------------------
+// persistent walks (batched form): each lane carries WPL walker slots; a
+// slot whose walker finished is refilled from a global cursor with one
+// atomic per wave (ballot of empty slots, popcount ranks), so walkers of
+// uneven length (PPR, per-walker caps, dead ends, node2vec rejections) never
+// leave lanes idle.  Each loop iteration makes one draw per active slot; the
+// WPL slots of a lane are independent chains, so their gathers overlap.
+// With D > 0 the path is staged D vertices at a time in LDS (odd row pitch,
+// no bank conflicts on the per-step writes) and written out in 16-byte
+// vectors.  Walker i's path depends only on (seed, i, start, len), so any
+// schedule gives the same paths as k_walk and the oracle.
+// ------------------------------------------------------------------------
+template <int WPL, int D>
+struct PathOut {
+    int32_t *stage;  // this lane's WPL rows of D + 1 ints (LDS)
+    __device__ __forceinline__ void put(const WalkArgs &a, int k, int64_t wid, int32_t j, int32_t v) const
+    {
+        if (D == 0) {
+            a.paths[wid * (int64_t)a.max_len + j] = v;
+        } else {
+            int32_t *st = stage + k * (D + 1);
+            st[j % D] = v;
+            if (j % D == D - 1) flush(a, k, wid, j - (D - 1), D);
+        }
+    }
+    __device__ __forceinline__ void flush(const WalkArgs &a, int k, int64_t wid, int32_t j0, int cnt) const
+    {
+        const int32_t *st = stage + k * (D + 1);
+        int4 *dst = reinterpret_cast<int4 *>(a.paths + wid * (int64_t)a.max_len + j0);
+#pragma unroll
+        for (int c = 0; c < D / 4; ++c)
+            if (4 * c < cnt) dst[c] = make_int4(st[4 * c], st[4 * c + 1], st[4 * c + 2], st[4 * c + 3]);
+    }
+    // pad positions j..max_len-1 with -1
+    __device__ __forceinline__ void finish(const WalkArgs &a, int k, int64_t wid, int32_t j) const
+    {
+        int32_t *row = a.paths + wid * (int64_t)a.max_len;
+        if (D == 0) {
+            for (; j < a.max_len; ++j) row[j] = -1;
+            return;
+        }
+        const int32_t r = j % D;
+        if (r) {
+            int32_t *st = stage + k * (D + 1);
+            for (int t = r; t < D; ++t) st[t] = -1;
+            flush(a, k, wid, j - r, D);
+            j += D - r;
+        }
+        int4 *dst = reinterpret_cast<int4 *>(row + j);
+        for (int32_t c = 0; c < (a.max_len - j) / 4; ++c) dst[c] = make_int4(-1, -1, -1, -1);
+    }
+};
+
+template <int APP, int WPL, int D>
+__global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
+{
+    extern __shared__ int32_t lds_stage[];
+    const int ln = lane_id();
+    const uint64_t lt = (1ull << ln) - 1ull;
+    PathOut<WPL, D> out{lds_stage + (size_t)threadIdx.x * WPL * (D + 1)};
+    const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+    const uint32_t tlo = min(a.thr_neighbor, a.thr_other), thi = max(a.thr_neighbor, a.thr_other);
+    int64_t wid[WPL];
+    int32_t cur[WPL], prev[WPL], jj[WPL], len[WPL];
+    uint32_t att[WPL];
+    uint64_t hc[WPL], hp[WPL];
+#pragma unroll
+    for (int k = 0; k < WPL; ++k) {
+        wid[k] = -1;
+        cur[k] = prev[k] = jj[k] = len[k] = 0;
+        att[k] = 0;
+        hc[k] = hp[k] = 0;
+    }
+    uint32_t steps = 0, draws = 0, probes = 0;
+    bool more = true;
+    for (;;) {
+        if (more) {
+            uint64_t emp[WPL];
+            uint32_t need = 0;
+#pragma unroll
+            for (int k = 0; k < WPL; ++k) {
+                emp[k] = __ballot(wid[k] < 0);
+                need += (uint32_t)__popcll(emp[k]);
+            }
+            if (need) {
+                unsigned long long base = 0;
+                if (ln == 0) base = atomicAdd(a.cursor, (unsigned long long)need);
+                base = __shfl(base, 0, 64);
+                uint32_t off = 0;
+#pragma unroll
+                for (int k = 0; k < WPL; ++k) {
+                    if (wid[k] < 0) {
+                        const uint64_t c = base + off + (uint32_t)__popcll(emp[k] & lt);
+                        if (c < (uint64_t)a.nwalkers) {
+                            const int64_t w = (int64_t)c;
+                            const int32_t st = a.starts ? a.starts[w] : (int32_t)(w % s.n) + 1;
+                            int32_t L = a.lens ? a.lens[w] : a.max_len;
+                            if (APP == 3 && L > 2) L = 2;
+                            if (st < 1 || st > s.n || L < 1 || L > a.max_len) {
+                                atomicOr(&s.ctl->error, (unsigned)ERR_INVAL);
+                                out.finish(a, k, w, 0);
+                            } else {
+                                wid[k] = w;
+                                cur[k] = st;
+                                prev[k] = -1;
+                                jj[k] = 1;
+                                len[k] = L;
+                                att[k] = 0;
+                                if (APP == 1) hc[k] = s.hinfo[st];
+                                out.put(a, k, w, 0, st);
+                                if (L == 1) {
+                                    out.finish(a, k, w, 1);
+                                    wid[k] = -1;
+                                }
+                            }
+                        }
+                    }
+                    off += (uint32_t)__popcll(emp[k]);
+                }
+                if (base + need >= (uint64_t)a.nwalkers) more = false;
+            }
+        }
+        bool mine = false;
+#pragma unroll
+        for (int k = 0; k < WPL; ++k) mine |= wid[k] >= 0;
+        if (!__any(mine)) {
+            if (!more) break;
+            continue;
+        }
+#pragma unroll
+        for (int k = 0; k < WPL; ++k) {
+            if (wid[k] < 0) continue;
+            const int64_t w = wid[k];
+            const uint4 r = philox((uint32_t)w, (uint32_t)((uint64_t)w >> 32), (uint32_t)jj[k], att[k], k0, k1);
+            const int32_t nx = sample_step(s.alias, s.mem, cur[k], r);
+            ++draws;
+            if (nx < 0) {
+                out.finish(a, k, w, jj[k]);
+                wid[k] = -1;
+                continue;
+            }
+            const uint32_t aux = r.x & kAuxMask;
+            if (APP == 1 && jj[k] >= 2 && (int64_t)att[k] + 1 < a.max_attempts) {
+                bool acc;
+                if (nx == prev[k]) acc = aux < a.thr_return;
+                else if (aux < tlo) acc = true;
+                else if (aux >= thi) acc = false;
+                else {
+                    ++probes;
+                    acc = aux < (is_neighbor(s.hash, hp[k], (uint32_t)nx) ? a.thr_neighbor : a.thr_other);
+                }
+                if (!acc) {
+                    ++att[k];
+                    continue;
+                }
+            }
+            out.put(a, k, w, jj[k], nx);
+            ++steps;
+            prev[k] = cur[k];
+            cur[k] = nx;
+            ++jj[k];
+            att[k] = 0;
+            if (APP == 1) {
+                hp[k] = hc[k];
+                hc[k] = s.hinfo[nx];
+            }
+            if ((APP == 2 && aux < a.thr_stop) || jj[k] >= len[k]) {
+                out.finish(a, k, w, jj[k]);
+                wid[k] = -1;
+            }
+        }
+    }
+    const uint32_t tot = wave_sum32(steps), tdraw = wave_sum32(draws), tprobe = wave_sum32(probes);
+    if (ln == 0) {
+        if (tot) {
+            atomicAdd(a.steps, (unsigned long long)tot);
+            atomicAdd(&s.ctl->steps, (unsigned long long)tot);
+        }
+        if (tdraw) atomicAdd(&s.ctl->attempts, (unsigned long long)tdraw);
+        if (tprobe) atomicAdd(&s.ctl->probes, (unsigned long long)tprobe);
+    }
+}
+
+// ------------------------------------------------------------------------
 // export and point queries
 // ------------------------------------------------------------------------
 __global__ void k_export_a(DevStore s, uint32_t *nslots, uint64_t *mass, uint32_t *bsize, uint64_t *T, uint8_t *alias)
@@ -808,6 +993,16 @@ __global__ void k_edge_count(DevStore s, const int32_t *us, const int32_t *vs, i
     counts[i] = c;
 }
 
+// plain streaming copy: 16-byte vectors, grid-stride (tail bytes by lane 0)
+__global__ __launch_bounds__(256) void k_copy16(uint4 *__restrict__ dst, const uint4 *__restrict__ src, size_t n16,
+                                                uint8_t *__restrict__ dtail, const uint8_t *__restrict__ stail,
+                                                uint32_t ntail)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+    if (blockIdx.x == 0 && threadIdx.x < ntail) dtail[threadIdx.x] = stail[threadIdx.x];
+}
+
 // ------------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------------
@@ -820,6 +1015,15 @@ static inline unsigned run_blocks(int64_t k)
 }
 
 void launch_reset_ctl(Ctl *ctl, hipStream_t st) { hipLaunchKernelGGL(k_reset_ctl, dim3(1), dim3(64), 0, st, ctl); }
+
+void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st)
+{
+    const size_t n16 = bytes / 16, done = n16 * 16;
+    const unsigned blocks = (unsigned)std::min<size_t>(std::max<size_t>((n16 + 255) / 256, 1), 4096);
+    hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, st, static_cast<uint4 *>(dst),
+                       static_cast<const uint4 *>(src), n16, static_cast<uint8_t *>(dst) + done,
+                       static_cast<const uint8_t *>(src) + done, (uint32_t)(bytes - done));
+}
 
 void launch_reset_walk(Ctl *ctl, hipStream_t st) { hipLaunchKernelGGL(k_reset_walk, dim3(1), dim3(64), 0, st, ctl); }
 
@@ -876,6 +1080,61 @@ void launch_walk(DevStore s, int app, const WalkArgs &a, int block, hipStream_t 
     case 2: hipLaunchKernelGGL(k_walk<2>, grid, blk, 0, st, s, a); break;
     default: hipLaunchKernelGGL(k_walk<3>, grid, blk, 0, st, s, a); break;
     }
+}
+
+size_t persist_lds_bytes(int block, int wpl, int depth)
+{
+    return depth ? (size_t)block * wpl * (depth + 1) * sizeof(int32_t) : 0;
+}
+
+template <int APP>
+static const void *persist_fn(int wpl, int depth)
+{
+#define BINGO_PF(W, D) reinterpret_cast<const void *>(&k_walk_persist<APP, W, D>)
+    switch (wpl * 100 + depth) {
+    case 100: return BINGO_PF(1, 0);
+    case 104: return BINGO_PF(1, 4);
+    case 116: return BINGO_PF(1, 16);
+    case 200: return BINGO_PF(2, 0);
+    case 204: return BINGO_PF(2, 4);
+    case 216: return BINGO_PF(2, 16);
+    case 400: return BINGO_PF(4, 0);
+    case 404: return BINGO_PF(4, 4);
+    case 416: return BINGO_PF(4, 16);
+    default: return nullptr;
+    }
+#undef BINGO_PF
+}
+
+static const void *persist_kernel(int app, int wpl, int depth)
+{
+    switch (app) {
+    case 0: return persist_fn<0>(wpl, depth);
+    case 1: return persist_fn<1>(wpl, depth);
+    case 2: return persist_fn<2>(wpl, depth);
+    default: return persist_fn<3>(wpl, depth);
+    }
+}
+
+int persist_max_blocks_per_cu(int app, int wpl, int depth, int block)
+{
+    const void *fn = persist_kernel(app, wpl, depth);
+    if (!fn) return 0;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, persist_lds_bytes(block, wpl, depth)) !=
+        hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return nb;
+}
+
+void launch_walk_persist(DevStore s, int app, const WalkArgs &a, const PersistCfg &c, hipStream_t st)
+{
+    const void *fn = persist_kernel(app, c.wpl, c.depth);
+    if (!fn) return;
+    void *args[] = {&s, const_cast<WalkArgs *>(&a)};
+    (void)hipLaunchKernel(fn, dim3(c.blocks), dim3(c.block), args, persist_lds_bytes(c.block, c.wpl, c.depth), st);
 }
 
 void launch_export_a(DevStore s, uint32_t *nslots, uint64_t *mass, uint32_t *bsize, uint64_t *T, uint8_t *alias,

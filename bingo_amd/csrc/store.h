@@ -64,6 +64,7 @@ struct Ctl {
     unsigned long long steps;        // last walk: appended vertices
     unsigned long long attempts;     // last walk: sample draws (node2vec rejections included)
     unsigned long long probes;       // last walk: neighbour-table lookups
+    unsigned long long cursor;       // persistent walk: next unclaimed walker
 };
 
 struct DevEdge {
@@ -93,6 +94,7 @@ struct WalkArgs {
     const int32_t *lens;
     int32_t *paths;
     unsigned long long *steps;
+    unsigned long long *cursor;  // persistent form: next unclaimed walker
     int64_t nwalkers;
     uint64_t seed;
     uint32_t thr_return, thr_neighbor, thr_other, thr_stop;
@@ -121,12 +123,20 @@ void launch_delete_check(DevStore s, const unsigned long long *keys, int64_t k, 
                          hipStream_t st);
 void launch_delete_apply(DevStore s, const uint32_t *slots, Runs r, int64_t k, hipStream_t st);
 void launch_walk(DevStore s, int app, const WalkArgs &a, int block, hipStream_t st);
+struct PersistCfg {
+    int block, wpl, depth, blocks;  // blocks = grid size
+};
+size_t persist_lds_bytes(int block, int wpl, int depth);
+int persist_max_blocks_per_cu(int app, int wpl, int depth, int block);
+void launch_walk_persist(DevStore s, int app, const WalkArgs &a, const PersistCfg &c, hipStream_t st);
 void launch_export_a(DevStore s, uint32_t *nslots, uint64_t *mass, uint32_t *bsize, uint64_t *T, uint8_t *alias,
                      hipStream_t st);
 void launch_export_b(DevStore s, const uint64_t *slot_out_off, const uint64_t *mem_out_off, uint32_t *slot_v,
                      uint32_t *slot_w, uint64_t *members, uint32_t *backpos, hipStream_t st);
 void launch_edge_count(DevStore s, const int32_t *us, const int32_t *vs, int64_t k, uint32_t *counts,
                        hipStream_t st);
+
+void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st);
 
 // rocPRIM wrappers (temp storage sized by a null call)
 size_t sort_pairs_u32(void *tmp, size_t tmp_bytes, const uint32_t *kin, uint32_t *kout, const uint32_t *vin,

@@ -62,9 +62,18 @@ typedef struct {
     double tp;              /* PPR stop probability per step (ref TP=0.0125) */
     int32_t max_len;        /* path columns, vertices per walk incl. start (ref LEN=80) */
     int32_t max_attempts;   /* node2vec rejection cap per step; last draw accepted */
-    int32_t block;          /* launch tuning: threads per block (0 = autotuned/default) */
+    int32_t form;           /* BINGO_FORM_*: how walkers are scheduled (paths are the same) */
+    int32_t block;          /* threads per block, 64..512; 0 = tuned or default */
+    int32_t walkers_per_lane; /* persistent form: 1, 2 or 4 walker slots per lane; 0 = tuned */
+    int32_t lds_depth;      /* persistent form: path staging in LDS, 0, 4 or 16; -1 = tuned */
+    int32_t blocks_per_cu;  /* persistent form: resident blocks per CU; 0 = occupancy limit */
     int32_t reserved;
 } bingo_walk_params;
+
+/* walk scheduling forms */
+#define BINGO_FORM_AUTO 0       /* persistent with the handle's tuned knobs (defaults if untuned) */
+#define BINGO_FORM_PLAIN 1      /* one thread per walker, one launch (ref deepwalkKernel style) */
+#define BINGO_FORM_PERSISTENT 2 /* persistent lanes with walker refill, knobs as given */
 
 typedef struct {
     double load_ms, insert_ms, delete_ms, walk_ms; /* HIP-event time of the last call */
@@ -129,6 +138,14 @@ int bingo_walk_device(bingo_graph *g, int app, const bingo_walk_params *params,
                       const int32_t *d_starts, const int32_t *d_lens, int64_t nwalkers,
                       int32_t *d_paths, uint64_t *d_steps);
 
+/* Autotuner: times the persistent form over block size, walkers per lane
+ * and LDS staging depth (and the plain form) on up to `sample` walkers with
+ * strided starts, keeps the fastest for `app` on this handle (used by
+ * BINGO_FORM_AUTO) and returns it in *chosen (may be NULL).  The graph is
+ * not modified. */
+int bingo_walk_tune(bingo_graph *g, int app, const bingo_walk_params *params, int64_t sample,
+                    bingo_walk_params *chosen);
+
 /* ---- inspection ---- */
 int32_t bingo_num_nodes(const bingo_graph *g);
 int bingo_stats_get(bingo_graph *g, bingo_stats *out);
@@ -144,6 +161,9 @@ int bingo_dev_alloc(bingo_graph *g, size_t bytes, void **d_ptr);
 int bingo_dev_free(bingo_graph *g, void *d_ptr);
 int bingo_memcpy_h2d(bingo_graph *g, void *d_dst, const void *src, size_t bytes);
 int bingo_memcpy_d2h(bingo_graph *g, void *dst, const void *d_src, size_t bytes);
+/* device-to-device copy by a plain streaming kernel (k_copy16) on the
+ * handle's stream; also the known-size reference for HBM traffic counters */
+int bingo_memcpy_d2d(bingo_graph *g, void *d_dst, const void *d_src, size_t bytes);
 /* HIP events on the handle's stream: record into slot (0..15), elapsed ms */
 int bingo_event_record(bingo_graph *g, int slot);
 int bingo_event_elapsed(bingo_graph *g, int slot_a, int slot_b, float *ms);

@@ -118,7 +118,7 @@ def test_hub_node_long_runs(orc):
             if lw:
                 other.append((u, rng.choice(lw)))
         dels = np.array([(1, s) for s in pick] + other, B.DEL_DT)
-        rng.shuffle(dels)
+        dels = dels[np.random.RandomState(rnd).permutation(len(dels))]  # batch order is irrelevant
         g.delete(dels)
         assert o.delete(dels) == 0
         _same_state(g, o, f"hub round {rnd}")
