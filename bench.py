@@ -79,7 +79,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--schedule", default="", metavar="BLOCK,WPL,DEPTH",
+                    help="fixed persistent walk knobs instead of autotuning (e.g. 512,2,16)")
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the library's plain copy kernel over MIB MiB (traffic-counter reference)")
@@ -129,7 +130,12 @@ def main():
     log(f"staged {W + K} update batches of {kb} + {kb}")
     d_paths = B.DeviceArray(g, nbytes=n * cfg["max_len"] * 4)
     d_steps = B.DeviceArray(g, np.zeros(1, np.uint64))
-    wp = B.walk_params(seed=args.seed, p=cfg["p"], q=cfg["q"], max_len=cfg["max_len"], block=args.block)
+    if args.schedule:
+        blk, wpl, dep = (int(x) for x in args.schedule.split(","))
+        wp = B.walk_params(seed=args.seed, p=cfg["p"], q=cfg["q"], max_len=cfg["max_len"], form=B.FORM_PERSISTENT,
+                           block=blk, walkers_per_lane=wpl, lds_depth=dep)
+    else:
+        wp = B.walk_params(seed=args.seed, p=cfg["p"], q=cfg["q"], max_len=cfg["max_len"])
 
     def step(i):
         di, dd = batches[i]
@@ -143,6 +149,8 @@ def main():
         st = g.stats()
         return g.elapsed_ms(0, 1), g.elapsed_ms(1, 2), g.elapsed_ms(2, 3), st
 
+    tuned = g.tune(cfg["app"], wp, sample=min(n, 1 << 20)) if not args.schedule else args.schedule
+    log(f"autotuned walk schedule: {tuned}")
     for i in range(W):
         step(i)
         log(f"warmup step {i} done")
@@ -164,6 +172,18 @@ def main():
     wall = time.perf_counter() - t0
     steps = int(d_steps.to_host(np.uint64, (1,))[0])
     st = g.stats()
+
+    # the plain per-launch form (one thread per walker) on the same graph
+    wplain = B.walk_params(seed=args.seed, p=cfg["p"], q=cfg["q"], max_len=cfg["max_len"], form=B.FORM_PLAIN)
+    d_s2 = B.DeviceArray(g, np.zeros(1, np.uint64))
+    plain_ms = []
+    for _ in range(2):
+        g.event(4)
+        g.walk_device(cfg["app"], wplain, n, d_paths, d_s2)
+        g.event(5)
+        plain_ms.append(g.elapsed_ms(4, 5))
+    plain_steps = int(d_s2.to_host(np.uint64, (1,))[0]) // 2
+    log(f"plain form walk: {min(plain_ms):.2f} ms for {plain_steps} steps")
 
     bytes_per_launch = (B_STEP * attempts + B_PROBE * probes) / K
     kern_s = kern_ms / K / 1e3
@@ -191,6 +211,9 @@ def main():
         "walk_ms": walk_ms / K,
         "load_ms": load_ms,
         "walk_draws_per_step": attempts / max(steps, 1),
+        "walk_schedule": tuned,
+        "plain_form_walk_steps_per_s": plain_steps / (min(plain_ms) / 1e3),
+        "batched_vs_plain": (steps / (walk_ms / 1e3)) / (plain_steps / (min(plain_ms) / 1e3)),
         "store_bytes": st["bytes_reserved"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

@@ -96,6 +96,28 @@ class Dump(C.Structure):
     ]
 
 
+class InstanceOpts(C.Structure):
+    _fields_ = [
+        ("rounds", C.c_int32),
+        ("batch", C.c_int64),
+        ("seed", C.c_uint64),
+        ("walk_len", C.c_int32),
+        ("device", C.c_int32),
+        ("print", C.c_int32),
+    ]
+
+
+class InstanceResult(C.Structure):
+    _fields_ = [
+        ("walk_s", C.c_double),
+        ("eval_s", C.c_double),
+        ("memory_gb", C.c_double),
+        ("walk_steps", C.c_uint64),
+        ("arcs", C.c_int64),
+        ("n", C.c_int32),
+    ]
+
+
 # every symbol include/bingo.h declares, with its ctypes signature
 _vp, _i32, _i64, _u64, _int = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64, C.c_int
 SIGNATURES = {
@@ -127,6 +149,9 @@ SIGNATURES = {
     "bingo_memcpy_d2d": (_int, [_vp, _vp, _vp, C.c_size_t]),
     "bingo_event_record": (_int, [_vp, _int]),
     "bingo_event_elapsed": (_int, [_vp, _int, _int, C.POINTER(C.c_float)]),
+    "bingo_instance_opts_default": (None, [C.POINTER(InstanceOpts)]),
+    "bingo_read_edgelist": (_int, [C.c_char_p, C.POINTER(_i32), _vp, _i64, C.POINTER(_i64)]),
+    "bingo_instance": (_int, [_int, _int, C.c_char_p, C.POINTER(InstanceOpts), C.POINTER(InstanceResult)]),
     "bingo_gen_rmat": (_int, [_int, _i64, _i64, C.c_double, C.c_double, C.c_double, _int, _u64, _int, _vp]),
     "bingo_updgen_create": (_int, [_i32, _int, _vp, _i64, _i64, C.c_double, C.c_double, C.c_double, _int, _u64,
                                    C.POINTER(_vp)]),
@@ -181,6 +206,28 @@ def gen_rmat(scale, nedges, wkind, seed, undirected=True, edge_begin=0, abc=(0.5
     out = np.zeros(nedges * (2 if undirected else 1), EDGE_DT)
     _check(lib().bingo_gen_rmat(scale, edge_begin, nedges, *abc, wkind, seed, int(undirected), _ptr(out)))
     return out
+
+
+def read_edgelist(path):
+    """Parse and weight a 0-based "u v" edge list the way the reference's loader does."""
+    n, m = C.c_int32(0), C.c_int64(0)
+    _check(lib().bingo_read_edgelist(path.encode(), C.byref(n), None, 0, C.byref(m)))
+    out = np.zeros(m.value, EDGE_DT)
+    _check(lib().bingo_read_edgelist(path.encode(), C.byref(n), _ptr(out), m.value, C.byref(m)))
+    return n.value, out
+
+
+def instance(app, utype, dataset, rounds=10, batch=100000, seed=1, walk_len=80, device=0, verbose=True):
+    """The reference's instance(app, utype, dataset) (include/apps.h:4)."""
+    o = InstanceOpts()
+    lib().bingo_instance_opts_default(C.byref(o))
+    o.rounds, o.batch, o.seed, o.walk_len, o.device, o.print = rounds, batch, seed, walk_len, device, int(verbose)
+    r = InstanceResult()
+    _check(lib().bingo_instance(app, utype, dataset.encode(), C.byref(o), C.byref(r)))
+    return {k: getattr(r, k) for k, _ in InstanceResult._fields_}
+
+
+UPD_INSERT, UPD_DELETE, UPD_MIXED = 0, 1, 2
 
 
 class UpdateGen:

@@ -18,6 +18,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbingo.so")
+CLI = os.path.join(HERE, "bingo_cli")
 ARCH = os.environ.get("BINGO_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -50,7 +51,7 @@ def _run(cmd, verbose):
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     objs = []
-    for src in ("kernels.hip", "api.cpp"):
+    for src in ("kernels.hip", "api.cpp", "apps.cpp"):
         obj = os.path.join(BUILD, src.rsplit(".", 1)[0] + ".o")
         _run([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj], verbose)
         objs.append(obj)
@@ -61,6 +62,11 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp, "-lgomp"], verbose)
     os.replace(tmp, LIB)
+    # the reference's command line (src/main.cpp) over the C ABI
+    cli_tmp = CLI + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}", os.path.join(CSRC, "cli.cpp"),
+          "-o", cli_tmp, f"-L{HERE}", "-lbingo", "-Wl,-rpath,$ORIGIN"], verbose)
+    os.replace(cli_tmp, CLI)
     return LIB
 
 

@@ -165,6 +165,10 @@ static int ensure_pools(bingo_graph *g)
         }
         total += newcap[p] * kElem[p];
     }
+    for (int p = 0; p < NPOOL; ++p)  // alias slots address members and tables with 36 bits
+        if (c.top[p] + c.need[p] > (1ull << kOffBits))
+            return fail(BINGO_ENOMEM, "a pool would exceed 2^36 elements");
+    for (int p = 0; p < NPOOL; ++p) newcap[p] = std::min<uint64_t>(newcap[p], 1ull << kOffBits);
     if (total > g->budget) {  // retry without slack
         total = g->node_bytes;
         for (int p = 0; p < NPOOL; ++p) {
@@ -463,7 +467,7 @@ void bingo_destroy(bingo_graph *g)
 {
     if (!g) return;
     if (g->st) (void)hipStreamSynchronize(g->st);
-    void *bufs[] = {g->s.hdr, g->s.grp, g->s.alias, g->s.hinfo, g->s.slot, g->s.back, g->s.mem, g->s.hash, g->s.ctl,
+    void *bufs[] = {g->s.hdr, g->s.grp, g->s.alias, g->s.slot, g->s.back, g->s.mem, g->s.hash, g->s.ctl,
                     g->d_steps};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -487,12 +491,11 @@ int bingo_load_device(bingo_graph *g, int32_t n, const bingo_edge *d_edges, int6
     if (m < 0 || (m > 0 && !d_edges)) return fail(BINGO_EINVAL, "bad edge array");
     HIPCHK(hipSetDevice(g->device));
     const size_t n1 = (size_t)n + 1;
-    const size_t nb = n1 * sizeof(NodeHdr) + n1 * kGroups * (sizeof(GroupHdr) + sizeof(AliasEnt)) + n1 * 8;
+    const size_t nb = n1 * sizeof(NodeHdr) + n1 * kGroups * (sizeof(GroupHdr) + sizeof(AliasEnt));
     if (nb > g->budget) return fail(BINGO_ENOMEM, "node records exceed the HBM budget");
     bool ok = hipMalloc((void **)&g->s.hdr, n1 * sizeof(NodeHdr)) == hipSuccess &&
               hipMalloc((void **)&g->s.grp, n1 * kGroups * sizeof(GroupHdr)) == hipSuccess &&
-              hipMalloc((void **)&g->s.alias, n1 * kGroups * sizeof(AliasEnt)) == hipSuccess &&
-              hipMalloc((void **)&g->s.hinfo, n1 * 8) == hipSuccess;
+              hipMalloc((void **)&g->s.alias, n1 * kGroups * sizeof(AliasEnt)) == hipSuccess;
     if (!ok) {
         (void)hipGetLastError();
         return fail(BINGO_ENOMEM, "node record allocation failed");

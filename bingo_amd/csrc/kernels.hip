@@ -140,16 +140,19 @@ __device__ void hash_dec(uint64_t *tab, uint32_t cap, uint32_t key)
     }
 }
 
-__device__ __forceinline__ uint64_t make_hinfo(uint64_t off, uint32_t cap)
+// neighbour-table locator kept in every alias slot of its node:
+// (log2 cap + 1) | offset << 6, 0 when the node has no table
+__device__ __forceinline__ uint64_t make_hmeta(uint64_t off, uint32_t cap)
 {
-    return cap ? ((off << 8) | (uint64_t)(32 - __clz(cap))) : 0ull;  // log2(cap)+1
+    return cap ? ((uint64_t)(32 - __clz(cap)) | (off << 6)) : 0ull;
 }
 
-__device__ __forceinline__ bool is_neighbor(const uint64_t *__restrict__ hash, uint64_t hinfo, uint32_t key)
+__device__ __forceinline__ bool is_neighbor(const uint64_t *__restrict__ hash, uint64_t hm, uint32_t key)
 {
-    if (!hinfo) return false;
-    const uint32_t cap = 1u << ((uint32_t)(hinfo & 0xFF) - 1u), mask = cap - 1u;
-    const uint64_t *tab = hash + (hinfo >> 8);
+    const uint32_t lg = (uint32_t)(hm & 63);
+    if (!lg) return false;
+    const uint32_t cap = 1u << (lg - 1u), mask = cap - 1u;
+    const uint64_t *tab = hash + (hm >> 6);
     uint32_t i = hslot(key, cap);
     for (;;) {
         const uint64_t e = tab[i];
@@ -168,7 +171,7 @@ __device__ __forceinline__ bool is_neighbor(const uint64_t *__restrict__ hash, u
 // Stay threshold T_j = floor(tau_j 2^64 / S).  Replaces the float/EPS table
 // of NodeData::buildAliasTable (include/Bingo.cuh:111-158).
 // ------------------------------------------------------------------------
-__device__ void alias_rebuild(const DevStore &s, int32_t u, uint32_t size, uint64_t off, int ln)
+__device__ void alias_rebuild(const DevStore &s, int32_t u, uint32_t size, uint64_t off, uint64_t hmeta, int ln)
 {
     typedef unsigned __int128 u128;
     const bool own = ln < kGroups;
@@ -220,10 +223,9 @@ __device__ void alias_rebuild(const DevStore &s, int32_t u, uint32_t size, uint6
     if (own) {
         AliasEnt e;
         e.T = T;
-        e.off_stay = off;
-        e.size_stay = size;
-        e.size_alias = size_a;
-        e.off_alias = off_a | ((uint64_t)ali << 56);
+        e.stay = off | ((uint64_t)size << kOffBits);
+        e.alias = off_a | ((uint64_t)size_a << kOffBits);
+        e.meta = (uint64_t)ali | (hmeta << 5);
         s.alias[(size_t)u * kGroups + ln] = e;
     }
 }
@@ -256,17 +258,15 @@ __global__ void k_init_nodes(DevStore s)
     const int j = (int)(i & (kGroups - 1));
     AliasEnt e;
     e.T = ~0ull;
-    e.off_stay = 0;
-    e.size_stay = 0;
-    e.size_alias = 0;
-    e.off_alias = (uint64_t)j << 56;
+    e.stay = 0;
+    e.alias = 0;
+    e.meta = (uint64_t)j;
     s.alias[i] = e;
     GroupHdr g = {0, 0, 0};
     s.grp[i] = g;
     if (j == 0) {
         NodeHdr h = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         s.hdr[i >> 5] = h;
-        s.hinfo[i >> 5] = 0;
     }
 }
 
@@ -339,7 +339,8 @@ __device__ Growth plan_run(const DevStore &s, const DevEdge *__restrict__ e, uin
 
 __device__ __forceinline__ bool run_too_big(const NodeHdr &h, uint32_t cnt, uint32_t pcsum)
 {
-    return (uint64_t)h.nslots + cnt >= (1ull << 30) || (uint64_t)h.back_used + pcsum >= (1ull << 31) ||
+    // group sizes must fit the 28-bit size field of an alias slot
+    return (uint64_t)h.nslots + cnt >= kMaxGroupSize || (uint64_t)h.back_used + pcsum >= (1ull << 31) ||
            2ull * ((uint64_t)h.hash_used + cnt) >= (1ull << 31);
 }
 
@@ -514,11 +515,10 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, uin
             h.hash_off = a_hash;
             h.hash_cap = G.cap_hash;
             h.hash_used = hused;
-            s.hinfo[u] = make_hinfo(a_hash, G.cap_hash);
         }
         s.hdr[u] = h;
     }
-    alias_rebuild(s, u, ln < kGroups ? szj : 0u, goff, ln);
+    alias_rebuild(s, u, ln < kGroups ? szj : 0u, goff, s.nbr ? make_hmeta(a_hash, G.cap_hash) : 0ull, ln);
 }
 
 __global__ __launch_bounds__(256) void k_insert_apply(DevStore s, const DevEdge *__restrict__ E, Runs r)
@@ -613,7 +613,7 @@ __device__ void delete_run(const DevStore &s, const uint32_t *__restrict__ slots
         for (uint32_t t = ln; t < cnt; t += 64) hash_dec(htab, h.hash_cap, V[slots[t]]);
     }
     for (uint32_t t = ln; t < cnt; t += 64) W[slots[t]] = 0u;
-    alias_rebuild(s, u, ln < kGroups ? gh.size : 0u, gh.off, ln);
+    alias_rebuild(s, u, ln < kGroups ? gh.size : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, ln);
 }
 
 __global__ __launch_bounds__(256) void k_delete_apply(DevStore s, const uint32_t *__restrict__ slots, Runs r)
@@ -635,20 +635,21 @@ __global__ __launch_bounds__(256) void k_delete_apply(DevStore s, const uint32_t
 // One draw on node `cur` (NodeData::sample, include/Bingo.cuh:220-267):
 // slot i = top 5 bits of r.x; the 64-bit coin (r.y:r.z) against T_i picks
 // group i or its partner; member floor(r.w * size / 2^32) of that group.
+// `hm` receives cur's neighbour-table locator carried by the slot.
 __device__ __forceinline__ int32_t sample_step(const AliasEnt *__restrict__ alias, const uint64_t *__restrict__ mem,
-                                               int32_t cur, uint4 r)
+                                               int32_t cur, uint4 r, uint64_t &hm)
 {
     const uint32_t i = r.x >> 27;
     const uint4 *p = reinterpret_cast<const uint4 *>(alias + (size_t)cur * kGroups + i);
     const uint4 q0 = p[0], q1 = p[1];
     const uint64_t T = ((uint64_t)q0.y << 32) | q0.x;
     const uint64_t coin = ((uint64_t)r.y << 32) | r.z;
-    const bool stay = coin < T;
-    const uint64_t off = stay ? (((uint64_t)q0.w << 32) | q0.z) : ((((uint64_t)q1.w << 32) | q1.z) & kOffMask);
-    const uint32_t sz = stay ? q1.x : q1.y;
+    const uint64_t loc = coin < T ? (((uint64_t)q0.w << 32) | q0.z) : (((uint64_t)q1.y << 32) | q1.x);
+    hm = (((uint64_t)q1.w << 32) | q1.z) >> 5;
+    const uint32_t sz = (uint32_t)(loc >> kOffBits);
     if (sz == 0u) return -1;
     const uint32_t pos = __umulhi(r.w, sz);
-    return (int32_t)reinterpret_cast<const uint32_t *>(mem + off + pos)[1];
+    return (int32_t)reinterpret_cast<const uint32_t *>(mem + (loc & kOffMask36) + pos)[1];
 }
 
 template <int APP>
@@ -670,7 +671,7 @@ __global__ __launch_bounds__(256) void k_walk(DevStore s, WalkArgs a)
             const uint32_t w0 = (uint32_t)wid, w1 = (uint32_t)((uint64_t)wid >> 32);
             row[0] = start;
             int32_t cur = start, prev = -1;
-            uint64_t hcur = APP == 1 ? s.hinfo[start] : 0ull, hprev = 0ull;
+            uint64_t hcur = 0ull, hprev = 0ull;  // neighbour-table locators of cur and prev
             const uint32_t tlo = min(a.thr_neighbor, a.thr_other), thi = max(a.thr_neighbor, a.thr_other);
             for (; j < L; ++j) {
                 uint4 r;
@@ -678,7 +679,7 @@ __global__ __launch_bounds__(256) void k_walk(DevStore s, WalkArgs a)
                 if (APP == 1 && j >= 2) {
                     for (uint32_t at = 0;; ++at) {
                         r = philox(w0, w1, (uint32_t)j, at, k0, k1);
-                        nx = sample_step(s.alias, s.mem, cur, r);
+                        nx = sample_step(s.alias, s.mem, cur, r, hcur);
                         ++draws;
                         if (nx < 0 || (int64_t)at + 1 >= a.max_attempts) break;
                         const uint32_t aux = r.x & kAuxMask;
@@ -694,7 +695,7 @@ __global__ __launch_bounds__(256) void k_walk(DevStore s, WalkArgs a)
                     }
                 } else {
                     r = philox(w0, w1, (uint32_t)j, 0u, k0, k1);
-                    nx = sample_step(s.alias, s.mem, cur, r);
+                    nx = sample_step(s.alias, s.mem, cur, r, hcur);
                     ++draws;
                 }
                 if (nx < 0) break;
@@ -702,10 +703,7 @@ __global__ __launch_bounds__(256) void k_walk(DevStore s, WalkArgs a)
                 ++steps;
                 prev = cur;
                 cur = nx;
-                if (APP == 1) {
-                    hprev = hcur;
-                    hcur = s.hinfo[nx];
-                }
+                hprev = hcur;
                 if (APP == 2 && (r.x & kAuxMask) < a.thr_stop) {
                     ++j;
                     break;
@@ -790,13 +788,13 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
     int64_t wid[WPL];
     int32_t cur[WPL], prev[WPL], jj[WPL], len[WPL];
     uint32_t att[WPL];
-    uint64_t hc[WPL], hp[WPL];
+    uint64_t hp[WPL];  // neighbour-table locator of each slot's previous node
 #pragma unroll
     for (int k = 0; k < WPL; ++k) {
         wid[k] = -1;
         cur[k] = prev[k] = jj[k] = len[k] = 0;
         att[k] = 0;
-        hc[k] = hp[k] = 0;
+        hp[k] = 0;
     }
     uint32_t steps = 0, draws = 0, probes = 0;
     bool more = true;
@@ -833,7 +831,6 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
                                 jj[k] = 1;
                                 len[k] = L;
                                 att[k] = 0;
-                                if (APP == 1) hc[k] = s.hinfo[st];
                                 out.put(a, k, w, 0, st);
                                 if (L == 1) {
                                     out.finish(a, k, w, 1);
@@ -859,7 +856,8 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
             if (wid[k] < 0) continue;
             const int64_t w = wid[k];
             const uint4 r = philox((uint32_t)w, (uint32_t)((uint64_t)w >> 32), (uint32_t)jj[k], att[k], k0, k1);
-            const int32_t nx = sample_step(s.alias, s.mem, cur[k], r);
+            uint64_t hcur;
+            const int32_t nx = sample_step(s.alias, s.mem, cur[k], r, hcur);
             ++draws;
             if (nx < 0) {
                 out.finish(a, k, w, jj[k]);
@@ -888,8 +886,7 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
             ++jj[k];
             att[k] = 0;
             if (APP == 1) {
-                hp[k] = hc[k];
-                hc[k] = s.hinfo[nx];
+                hp[k] = hcur;
             }
             if ((APP == 2 && aux < a.thr_stop) || jj[k] >= len[k]) {
                 out.finish(a, k, w, jj[k]);
@@ -922,7 +919,7 @@ __global__ void k_export_a(DevStore s, uint32_t *nslots, uint64_t *mass, uint32_
     const uint32_t sz = s.grp[i].size;
     bsize[i] = sz;
     T[i] = e.T;
-    alias[i] = (uint8_t)(e.off_alias >> 56);
+    alias[i] = (uint8_t)(e.meta & 31u);
     if (j == 0) {
         nslots[u] = s.hdr[u].nslots;
         uint64_t acc = 0;

@@ -6,7 +6,6 @@
 //   AliasEnt alias[u*32+i] 32-slot alias table; slot i carries the (offset,
 //                          size) of group i and of its alias partner, so one
 //                          32-byte read decides the group of a walk step
-//   uint64   hinfo[u]      neighbour-table locator for node2vec, (off << 8) | (log2 cap + 1)
 // Pools (one device allocation each, regions handed out by atomic bump):
 //   slot  u32: per node [v x cap | w x cap | back-offset x cap]
 //   back  u32: per slot, the member positions of its set bits, ascending
@@ -27,7 +26,6 @@ namespace bingo {
 constexpr int kGroups = 32;
 constexpr uint32_t kMark = 0x80000000u;
 constexpr uint32_t kAuxMask = (1u << 27) - 1u;
-constexpr uint64_t kOffMask = (1ull << 56) - 1ull;
 
 enum Pool { POOL_SLOT = 0, POOL_BACK = 1, POOL_MEM = 2, POOL_HASH = 3, NPOOL = 4 };
 enum ErrFlag { ERR_INVAL = 1, ERR_NOMEM = 2, ERR_OVERRUN = 4 };
@@ -46,13 +44,23 @@ struct GroupHdr {
     uint32_t size, cap;
 };
 
+// One 32-byte alias slot i of a node:
+//   T     stay iff coin64 < T
+//   stay  locator of group i:        member offset (36 bits) | size << 36
+//   alias locator of the partner:    member offset (36 bits) | size << 36
+//   meta  partner index (5 bits) | the node's neighbour-table locator << 5,
+//         locator = (log2 cap + 1) | table offset << 6 (0: no table)
+// so a walk step learns the group, its members and (for node2vec) where
+// the node's neighbour table lives from a single read.
 struct AliasEnt {
-    uint64_t T;          // stay iff coin64 < T
-    uint64_t off_stay;   // group i locator
-    uint32_t size_stay;
-    uint32_t size_alias;
-    uint64_t off_alias;  // partner locator; partner index in bits 56..63
+    uint64_t T;
+    uint64_t stay;
+    uint64_t alias;
+    uint64_t meta;
 };
+constexpr int kOffBits = 36;
+constexpr uint64_t kOffMask36 = (1ull << kOffBits) - 1ull;
+constexpr uint32_t kMaxGroupSize = 1u << 28;  // sizes live in 28 bits
 static_assert(sizeof(AliasEnt) == 32, "alias entry is one 32-byte read");
 static_assert(sizeof(NodeHdr) == 48, "node header layout");
 
@@ -80,7 +88,6 @@ struct DevStore {
     NodeHdr *hdr;
     GroupHdr *grp;
     AliasEnt *alias;
-    uint64_t *hinfo;
     uint32_t *slot;
     uint32_t *back;
     uint64_t *mem;

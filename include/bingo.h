@@ -168,6 +168,33 @@ int bingo_memcpy_d2d(bingo_graph *g, void *d_dst, const void *d_src, size_t byte
 int bingo_event_record(bingo_graph *g, int slot);
 int bingo_event_elapsed(bingo_graph *g, int slot_a, int slot_b, float *ms);
 
+/* ---- the reference driver: instance() (include/apps.h:4, src/apps.cu:10-42) ----
+ * Reads a "u v" edge list (0-based ids, one arc per pair, as graph.cuh:22-27),
+ * weights arcs bias(outdeg(v)) + 1, loads, then runs `rounds` x {insert batch,
+ * delete batch, one walk per node} for utype I, D or M (M halves the batch),
+ * printing the reference's lines to stderr when opts->print is set. */
+typedef struct {
+    int32_t rounds;     /* ref: 10 */
+    int64_t batch;      /* ref BATCHSIZE: 100000 */
+    uint64_t seed;
+    int32_t walk_len;   /* ref LEN: 80 */
+    int32_t device;
+    int32_t print;
+} bingo_instance_opts;
+typedef struct {
+    double walk_s;      /* "Random walk in ... s." */
+    double eval_s;      /* "Evaluation time: ... s." */
+    double memory_gb;   /* "Memory Consumption: ... GB." */
+    uint64_t walk_steps;
+    int64_t arcs;       /* live arcs at the end */
+    int32_t n;
+} bingo_instance_result;
+void bingo_instance_opts_default(bingo_instance_opts *o);
+/* parse + weight an edge list; edges NULL: only *n_out and *m_out are set */
+int bingo_read_edgelist(const char *path, int32_t *n_out, bingo_edge *edges, int64_t cap, int64_t *m_out);
+int bingo_instance(int app, int utype, const char *dataset, const bingo_instance_opts *opts,
+                   bingo_instance_result *res);
+
 /* ---- workload generators (host, deterministic; ref graph.cuh:16-92) ----
  * R-MAT arcs of edges [edge_begin, edge_begin + nedges) at the given scale,
  * probabilities a, b, c (d = 1 - a - b - c); undirected emits u->v, v->u. */

@@ -248,3 +248,49 @@ def test_gpu_transition_law_chi_square(orc):
     exp = np.array(w, float) / sum(w) * N
     keep = exp > 5
     assert stats.chisquare(cnt[keep], exp[keep] * cnt[keep].sum() / exp[keep].sum()).pvalue > 0.01
+
+
+@pytest.mark.parametrize("knobs", [
+    dict(form=B.FORM_PLAIN, block=64),
+    dict(form=B.FORM_PLAIN, block=512),
+    dict(form=B.FORM_PERSISTENT, block=256, walkers_per_lane=1, lds_depth=0),
+    dict(form=B.FORM_PERSISTENT, block=128, walkers_per_lane=2, lds_depth=4),
+    dict(form=B.FORM_PERSISTENT, block=512, walkers_per_lane=4, lds_depth=16),
+    dict(form=B.FORM_PERSISTENT, block=64, walkers_per_lane=4, lds_depth=0, blocks_per_cu=1),
+])
+def test_walk_forms_give_identical_paths(orc, knobs):
+    """Every scheduling form and knob setting yields the oracle's paths:
+    uneven lengths, PPR stops and dead ends included."""
+    rng = random.Random(21)
+    scale = 11
+    n = 1 << scale
+    base = B.gen_rmat(scale, 9000, B.W_ZIPF24, seed=21, undirected=False)  # directed: dead ends
+    g, o = _pair(orc, n, base)
+    nw = 20000
+    starts = np.array([rng.randint(1, n) for _ in range(nw)], np.int32)
+    lens = np.array([rng.randint(1, 48) for _ in range(nw)], np.int32)
+    for app in APPS:
+        gp = B.walk_params(seed=31 + app, tp=0.15, max_len=48, **knobs)
+        op = orc.walk_params(seed=31 + app, tp=0.15, max_len=48)
+        pa, sa = g.walk(app, gp, starts=starts, lens=lens)
+        pb, sb = o.walk(app, op, starts=starts, lens=lens)
+        assert np.array_equal(pa, pb), (app, knobs)
+        assert sa == sb
+        pa, sa = g.walk(app, gp, nwalkers=n)  # default starts, max_len everywhere
+        pb, sb = o.walk(app, op, nwalkers=n)
+        assert np.array_equal(pa, pb) and sa == sb, (app, knobs)
+
+
+def test_tuner_picks_a_valid_schedule(orc):
+    scale = 12
+    n = 1 << scale
+    base = B.gen_rmat(scale, 30000, B.W_LOGUNI32, seed=5)
+    g, o = _pair(orc, n, base)
+    wp = B.walk_params(seed=3, max_len=40)
+    chosen = g.tune(B.NODE2VEC, wp, sample=n)
+    assert chosen["form"] in (B.FORM_PLAIN, B.FORM_PERSISTENT)
+    pa, sa = g.walk(B.NODE2VEC, wp, nwalkers=n)  # FORM_AUTO now uses the tuned knobs
+    pb, sb = o.walk(B.NODE2VEC, orc.walk_params(seed=3, max_len=40), nwalkers=n)
+    assert np.array_equal(pa, pb) and sa == sb
+    with pytest.raises(B.BingoError):
+        g.walk(B.DEEPWALK, B.walk_params(max_len=40, form=B.FORM_PERSISTENT, lds_depth=16), nwalkers=4)
