@@ -43,6 +43,20 @@ def log(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
+def measured_traffic(workload):
+    """HBM bytes per walk launch from the rocprofv3 FETCH_SIZE + WRITE_SIZE
+    runs summarised under profiles/ (copy-calibrated), or None."""
+    import glob
+    import re
+
+    for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        for line in open(f):
+            m = re.match(r"traffic_bytes_per_launch\[(.+)\] = (\S+)", line.strip())
+            if m and m.group(1) == workload:
+                return float(m.group(2))
+    return None
+
+
 def cpu_baseline(base, n, cfg, seed, budget_s=15.0):
     """The CPU oracle (a C/OpenMP restatement of the same sampler) on this
     host's cores: full graph, a bounded sample of walkers."""
@@ -188,6 +202,9 @@ def main():
     bytes_per_launch = (B_STEP * attempts + B_PROBE * probes) / K
     kern_s = kern_ms / K / 1e3
     achieved = bytes_per_launch / kern_s / 1e9
+    traffic = measured_traffic(cfg["workload"])
+    if isinstance(tuned, str):
+        tuned = dict(zip(("block", "walkers_per_lane", "lds_depth"), map(int, tuned.split(","))), form=2)
     out = {
         "metric": METRIC,
         "value": steps / wall,
@@ -216,9 +233,10 @@ def main():
         "batched_vs_plain": (steps / (walk_ms / 1e3)) / (plain_steps / (min(plain_ms) / 1e3)),
         "store_bytes": st["bytes_reserved"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": f"k_walk<{cfg['app']}>", "kernel_ms": kern_ms / K,
-                     "bytes_per_launch": bytes_per_launch},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_GBps": traffic / kern_s / 1e9 if traffic else None,
+                     "kernel": "k_walk_persist" if (tuned or {}).get("form", 2) == 2 else "k_walk",
+                     "kernel_ms": kern_ms / K, "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": None,
     }
     for di, dd in batches:

@@ -578,32 +578,59 @@ __device__ void delete_run(const DevStore &s, const uint32_t *__restrict__ slots
     if (ln < kGroups) {
         gh = s.grp[(size_t)u * kGroups + ln];
         const uint32_t bit = 1u << ln, below = bit - 1u;
+        uint64_t *m = s.mem + gh.off;
+        // The deletes of a node are walked kPF at a time: their slot, weight,
+        // back-offset and member-position gathers are all issued before any is
+        // used, so a hub's thousands of deletes are not one dependent chain.
+        constexpr int kPF = 8;
+        uint32_t sl[kPF], w[kPF], pos[kPF];
+        auto gather = [&](uint32_t t0) {
+#pragma unroll
+            for (int q = 0; q < kPF; ++q) sl[q] = t0 + q < cnt ? slots[t0 + q] : 0u;
+            uint32_t bo[kPF];
+#pragma unroll
+            for (int q = 0; q < kPF; ++q) {
+                w[q] = t0 + q < cnt ? W[sl[q]] : 0u;
+                bo[q] = t0 + q < cnt ? B[sl[q]] : 0u;
+            }
+#pragma unroll
+            for (int q = 0; q < kPF; ++q) pos[q] = (w[q] & bit) ? back[bo[q] + __popc(w[q] & below)] : 0u;
+        };
+        // pass 1: count this group's deletions and mark their members
         uint32_t kj = 0;
-        for (uint32_t t = 0; t < cnt; ++t) kj += (W[slots[t]] & bit) != 0u;
+        for (uint32_t t0 = 0; t0 < cnt; t0 += kPF) {
+            gather(t0);
+#pragma unroll
+            for (int q = 0; q < kPF; ++q)
+                if (w[q] & bit) {
+                    ++kj;
+                    reinterpret_cast<uint32_t *>(m + pos[q])[0] = sl[q] | kMark;
+                }
+        }
+        // pass 2: holes below the new size, in slot order, take the tail's
+        // unmarked members in position order
         if (kj) {
-            uint64_t *m = s.mem + gh.off;
             const uint32_t S2 = gh.size - kj;
-            for (uint32_t t = 0; t < cnt; ++t) {
-                const uint32_t sl = slots[t], w = W[sl];
-                if (w & bit) reinterpret_cast<uint32_t *>(m + back[B[sl] + __popc(w & below)])[0] = sl | kMark;
-            }
             uint32_t tail = S2;
-            for (uint32_t t = 0; t < cnt; ++t) {
-                const uint32_t sl = slots[t], w = W[sl];
-                if (!(w & bit)) continue;
-                const uint32_t pos = back[B[sl] + __popc(w & below)];
-                if (pos >= S2) continue;
-                uint64_t mv = kMark;
-                while (tail < gh.size && ((uint32_t)(mv = m[tail++]) & kMark)) {
+            bool bad = false;
+            for (uint32_t t0 = 0; t0 < cnt && !bad; t0 += kPF) {
+                gather(t0);
+#pragma unroll
+                for (int q = 0; q < kPF; ++q) {
+                    if (bad || !(w[q] & bit) || pos[q] >= S2) continue;
+                    uint64_t mv = kMark;
+                    while (tail < gh.size && ((uint32_t)(mv = m[tail++]) & kMark)) {
+                    }
+                    if ((uint32_t)mv & kMark) {  // no survivor left: the store is inconsistent
+                        bad = true;
+                        continue;
+                    }
+                    m[pos[q]] = mv;
+                    const uint32_t ms = (uint32_t)mv;
+                    back[B[ms] + __popc(W[ms] & below)] = pos[q];
                 }
-                if ((uint32_t)mv & kMark) {  // no survivor left: the store is inconsistent
-                    atomicOr(&s.ctl->error, (unsigned)ERR_OVERRUN);
-                    break;
-                }
-                m[pos] = mv;
-                const uint32_t ms = (uint32_t)mv;
-                back[B[ms] + __popc(W[ms] & below)] = pos;
             }
+            if (bad) atomicOr(&s.ctl->error, (unsigned)ERR_OVERRUN);
             gh.size = S2;
             s.grp[(size_t)u * kGroups + ln].size = S2;
         }
