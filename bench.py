@@ -3,14 +3,17 @@
 A step is one round of the dynamic-graph workload: one insert batch, one
 delete batch, then one walk of every walker, all on HBM-resident inputs.
 
-Default workload (R-MAT 2^22 nodes / 2^26 undirected edges, u32 weights
+Default workload `c2` (R-MAT 2^22 nodes / 2^26 undirected edges, u32 weights
 log-uniform in [1, 2^32), node2vec p=0.5 q=2, one walker per node of length
 80, 10^6 inserts + 10^6 deletes per step).  `value` is whole-job walk steps/s
 (walk steps / wall time of the K steps, updates included); the line also
-carries walk-only steps/s, updates/s, the walk kernel's roofline and the CPU
-oracle timed on this host's cores on a bounded sample.
+carries walk-only steps/s, updates/s, the walk kernel's roofline, the plain
+per-launch form on the same graph, and the CPU oracle timed on this host's
+cores on a bounded sample.  `c3` is the batched workload (R-MAT 2^23 / 2^27,
+Zipf weights, 10^7 PPR walkers with lengths in [5, 200], update batches of
+log-uniform size), replayed from a captured HIP graph; `c1` the small one.
 
-    python bench.py [--steps K] [--warmup W] [--config c2|c1] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--config c1|c2|c3] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -30,12 +33,16 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes (per walk step / draw / probe): node record 8, alias slot
 # 8, group header 8, member 4, target 4, path write 4; a neighbour probe 8
 B_STEP, B_PROBE = 36, 8
+APP_NAMES = ["deepwalk", "node2vec", "ppr", "sampling"]
 
 CONFIGS = {
     "c2": dict(workload="rmat22_e26_loguni32_node2vec_upd1e6", scale=22, edges=1 << 26, wkind=2, app=1,
-               max_len=80, batch=1_000_000, p=0.5, q=2.0),
+               max_len=80, batch=1_000_000, p=0.5, q=2.0, tp=0.0125, walkers=None, lens=None, graph=False),
     "c1": dict(workload="rmat17_e21_uniform16_deepwalk_upd1e4", scale=17, edges=1 << 21, wkind=1, app=0,
-               max_len=80, batch=10_000, p=0.5, q=2.0),
+               max_len=80, batch=10_000, p=0.5, q=2.0, tp=0.0125, walkers=None, lens=None, graph=False),
+    "c3": dict(workload="rmat23_e27_zipf24_ppr015_len5to200_w1e7", scale=23, edges=1 << 27, wkind=3, app=2,
+               max_len=200, batch=(1_000, 1_000_000), p=0.5, q=2.0, tp=0.15, walkers=10_000_000, lens=(5, 200),
+               graph=True),
 }
 
 
@@ -57,6 +64,21 @@ def measured_traffic(workload):
     return None
 
 
+def walker_lens(cfg, nw, seed):
+    if not cfg["lens"]:
+        return None
+    lo, hi = cfg["lens"]
+    return np.random.default_rng(seed).integers(lo, hi + 1, size=nw, dtype=np.int32)
+
+
+def batch_sizes(cfg, k, seed):
+    b = cfg["batch"]
+    if isinstance(b, int):
+        return [b] * k
+    lo, hi = np.log(b[0]), np.log(b[1])
+    return [int(round(np.exp(x))) for x in np.random.default_rng(seed + 1).uniform(lo, hi, size=k)]
+
+
 def cpu_baseline(base, n, cfg, seed, budget_s=15.0):
     """The CPU oracle (a C/OpenMP restatement of the same sampler) on this
     host's cores: full graph, a bounded sample of walkers."""
@@ -64,26 +86,40 @@ def cpu_baseline(base, n, cfg, seed, budget_s=15.0):
 
     threads = len(os.sched_getaffinity(0))
     t0 = time.perf_counter()
-    o = orc.OracleGraph(n, neighbor_index=True)
+    o = orc.OracleGraph(n, neighbor_index=cfg["app"] == 1)
     assert o.load(base) == 0
     load_s = time.perf_counter() - t0
     log(f"cpu oracle loaded in {load_s:.1f}s ({threads} threads)")
-    wp = orc.walk_params(seed=seed, p=cfg["p"], q=cfg["q"], max_len=cfg["max_len"])
+    wp = orc.walk_params(seed=seed, p=cfg["p"], q=cfg["q"], tp=cfg["tp"], max_len=cfg["max_len"])
     nw = 4096
     while True:
         starts = (np.arange(nw, dtype=np.int64) * (n // nw) % n + 1).astype(np.int32)
         t0 = time.perf_counter()
-        _, steps = o.walk(cfg["app"], wp, starts=starts, nthreads=threads)
+        _, steps = o.walk(cfg["app"], wp, starts=starts, lens=walker_lens(cfg, nw, seed), nthreads=threads)
         dt = time.perf_counter() - t0
         if dt > budget_s / 4 or nw >= n:
             break
         nw = min(n, nw * 4)
-    rate = steps / dt
     o.close()
-    return {"value": rate, "unit": "walk steps/s", "cores": threads, "kind": "port",
-            "sample": f"{nw} {['deepwalk', 'node2vec', 'ppr', 'sampling'][cfg['app']]} walkers of length "
-                      f"{cfg['max_len']} spread over the same graph, {steps} steps in {dt:.2f}s "
-                      f"(load {load_s:.1f}s not counted)"}
+    return {"value": steps / dt, "unit": "walk steps/s", "cores": threads, "kind": "port",
+            "sample": f"{nw} {APP_NAMES[cfg['app']]} walkers (max length {cfg['max_len']}) spread over the same "
+                      f"graph, {steps} steps in {dt:.2f}s on {threads} threads (load {load_s:.1f}s not counted)"}
+
+
+def copy_calibration(B, mib):
+    nbytes = mib << 20
+    g = B.Graph(device=0)
+    src = B.DeviceArray(g, np.arange(nbytes // 8, dtype=np.uint64))
+    dst = B.DeviceArray(g, nbytes=nbytes)
+    ms = 0.0
+    for i in range(3):
+        g.event(0)
+        B._check(B.lib().bingo_memcpy_d2d(g._h, dst.ptr, src.ptr, nbytes))
+        g.event(1)
+        ms = g.elapsed_ms(0, 1)
+        log(f"copy {i}: {nbytes} bytes in {ms:.3f} ms = {2 * nbytes / ms / 1e6:.1f} GB/s (read+write)")
+    ok = bool((dst.to_host(np.uint64, (nbytes // 8,)) == np.arange(nbytes // 8, dtype=np.uint64)).all())
+    print(json.dumps({"copy_bytes": nbytes, "kernel": "k_copy16", "ms": ms, "correct": ok}), flush=True)
 
 
 def main():
@@ -106,29 +142,20 @@ def main():
     import bingo_amd as B
 
     if args.copy_calibration:
-        nbytes = args.copy_calibration << 20
-        g = B.Graph(device=0)
-        src = B.DeviceArray(g, np.arange(nbytes // 8, dtype=np.uint64))
-        dst = B.DeviceArray(g, nbytes=nbytes)
-        for i in range(3):
-            g.event(0)
-            B._check(B.lib().bingo_memcpy_d2d(g._h, dst.ptr, src.ptr, nbytes))
-            g.event(1)
-            ms = g.elapsed_ms(0, 1)
-            log(f"copy {i}: {nbytes} bytes in {ms:.3f} ms = {2 * nbytes / ms / 1e6:.1f} GB/s (read+write)")
-        ok = bool((dst.to_host(np.uint64, (nbytes // 8,)) == np.arange(nbytes // 8, dtype=np.uint64)).all())
-        print(json.dumps({"copy_bytes": nbytes, "kernel": "k_copy16", "ms": ms, "correct": ok}), flush=True)
+        copy_calibration(B, args.copy_calibration)
         return
 
     cfg = CONFIGS[args.config]
     scale, m = cfg["scale"], cfg["edges"]
     n = 1 << scale
-    K, W, kb = args.steps, args.warmup, cfg["batch"]
+    K, W = args.steps, args.warmup
+    nw = cfg["walkers"] or n
+    app = cfg["app"]
 
     t0 = time.perf_counter()
     base = B.gen_rmat(scale, m, cfg["wkind"], seed=args.seed)
     log(f"generated {len(base)} arcs in {time.perf_counter() - t0:.1f}s")
-    g = B.Graph(device=0, neighbor_index=True)
+    g = B.Graph(device=0, neighbor_index=app == B.NODE2VEC)
     d_base = B.DeviceArray(g, base)
     g.load_device(n, d_base, len(base))
     g.sync()
@@ -136,40 +163,49 @@ def main():
     d_base.free()
     log(f"loaded on GPU in {load_ms:.1f} ms")
 
+    sizes = batch_sizes(cfg, W + K, args.seed)
     ug = B.UpdateGen(n, scale, base, m, cfg["wkind"], seed=args.seed)
     batches = []
-    for _ in range(W + K):
+    for kb in sizes:
         ins, dels = ug.inserts(kb), ug.deletes(kb)
-        batches.append((B.DeviceArray(g, ins), B.DeviceArray(g, dels)))
-    log(f"staged {W + K} update batches of {kb} + {kb}")
-    d_paths = B.DeviceArray(g, nbytes=n * cfg["max_len"] * 4)
+        batches.append((B.DeviceArray(g, ins), B.DeviceArray(g, dels), kb))
+    del ug
+    log(f"staged {W + K} update batches, sizes {sizes}")
+    lens = walker_lens(cfg, nw, args.seed)
+    d_lens = B.DeviceArray(g, lens) if lens is not None else None
+    d_paths = B.DeviceArray(g, nbytes=nw * cfg["max_len"] * 4)
     d_steps = B.DeviceArray(g, np.zeros(1, np.uint64))
+    kw = dict(seed=args.seed, p=cfg["p"], q=cfg["q"], tp=cfg["tp"], max_len=cfg["max_len"])
     if args.schedule:
         blk, wpl, dep = (int(x) for x in args.schedule.split(","))
-        wp = B.walk_params(seed=args.seed, p=cfg["p"], q=cfg["q"], max_len=cfg["max_len"], form=B.FORM_PERSISTENT,
-                           block=blk, walkers_per_lane=wpl, lds_depth=dep)
+        wp = B.walk_params(form=B.FORM_PERSISTENT, block=blk, walkers_per_lane=wpl, lds_depth=dep, **kw)
+        tuned = dict(form=B.FORM_PERSISTENT, block=blk, walkers_per_lane=wpl, lds_depth=dep)
     else:
-        wp = B.walk_params(seed=args.seed, p=cfg["p"], q=cfg["q"], max_len=cfg["max_len"])
+        wp = B.walk_params(**kw)
+        tuned = g.tune(app, wp, sample=min(nw, 1 << 20))
+    log(f"walk schedule: {tuned}")
+    wgraph = g.walk_graph(app, wp, nw, d_paths, d_steps, d_lens=d_lens) if cfg["graph"] else None
 
     def step(i):
-        di, dd = batches[i]
+        di, dd, kb = batches[i]
         g.event(0)
         g.insert_device(di, kb)
         g.event(1)
         g.delete_device(dd, kb)
         g.event(2)
-        g.walk_device(cfg["app"], wp, n, d_paths, d_steps)
+        if wgraph:
+            wgraph.launch()
+        else:
+            g.walk_device(app, wp, nw, d_paths, d_steps, d_lens=d_lens)
         g.event(3)
         st = g.stats()
         return g.elapsed_ms(0, 1), g.elapsed_ms(1, 2), g.elapsed_ms(2, 3), st
 
-    tuned = g.tune(cfg["app"], wp, sample=min(n, 1 << 20)) if not args.schedule else args.schedule
-    log(f"autotuned walk schedule: {tuned}")
     for i in range(W):
         step(i)
         log(f"warmup step {i} done")
     g.sync()
-    d_steps = B.DeviceArray(g, np.zeros(1, np.uint64))
+    B._check(B.lib().bingo_memcpy_h2d(g._h, d_steps.ptr, np.zeros(1, np.uint64).ctypes.data, 8))
     ins_ms = del_ms = walk_ms = kern_ms = 0.0
     attempts = probes = 0
     t0 = time.perf_counter()
@@ -181,19 +217,21 @@ def main():
         kern_ms += st["walk_kernel_ms"]
         attempts += st["walk_attempts"]
         probes += st["walk_probes"]
-        log(f"step {i}: insert {a:.2f} ms, delete {b:.2f} ms, walk {c:.2f} ms")
+        log(f"step {i}: insert {batches[W + i][2]} in {a:.2f} ms, delete {batches[W + i][2]} in {b:.2f} ms, "
+            f"walk {c:.2f} ms")
     g.sync()
     wall = time.perf_counter() - t0
     steps = int(d_steps.to_host(np.uint64, (1,))[0])
     st = g.stats()
+    nupd = 2 * sum(b[2] for b in batches[W:])
 
     # the plain per-launch form (one thread per walker) on the same graph
-    wplain = B.walk_params(seed=args.seed, p=cfg["p"], q=cfg["q"], max_len=cfg["max_len"], form=B.FORM_PLAIN)
+    wplain = B.walk_params(form=B.FORM_PLAIN, **kw)
     d_s2 = B.DeviceArray(g, np.zeros(1, np.uint64))
     plain_ms = []
     for _ in range(2):
         g.event(4)
-        g.walk_device(cfg["app"], wplain, n, d_paths, d_s2)
+        g.walk_device(app, wplain, nw, d_paths, d_s2, d_lens=d_lens)
         g.event(5)
         plain_ms.append(g.elapsed_ms(4, 5))
     plain_steps = int(d_s2.to_host(np.uint64, (1,))[0]) // 2
@@ -203,8 +241,8 @@ def main():
     kern_s = kern_ms / K / 1e3
     achieved = bytes_per_launch / kern_s / 1e9
     traffic = measured_traffic(cfg["workload"])
-    if isinstance(tuned, str):
-        tuned = dict(zip(("block", "walkers_per_lane", "lds_depth"), map(int, tuned.split(","))), form=2)
+    walk_rate = steps / (walk_ms / 1e3)
+    plain_rate = plain_steps / (min(plain_ms) / 1e3)
     out = {
         "metric": METRIC,
         "value": steps / wall,
@@ -218,28 +256,31 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic",
-        "config": {"workload": cfg["workload"], "nodes": n, "arcs": int(len(base)), "walkers": n,
-                   "walk_len": cfg["max_len"], "app": ["deepwalk", "node2vec", "ppr", "sampling"][cfg["app"]],
-                   "update_batch": kb, "p": cfg["p"], "q": cfg["q"]},
-        "walk_steps_per_s": steps / (walk_ms / 1e3),
-        "updates_per_s": 2 * kb * K / ((ins_ms + del_ms) / 1e3),
+        "config": {"workload": cfg["workload"], "nodes": n, "arcs": int(len(base)), "walkers": nw,
+                   "walk_len": cfg["lens"] or cfg["max_len"], "app": APP_NAMES[app],
+                   "update_batch": cfg["batch"], "p": cfg["p"], "q": cfg["q"], "tp": cfg["tp"],
+                   "hip_graph": cfg["graph"]},
+        "walk_steps_per_s": walk_rate,
+        "updates_per_s": nupd / ((ins_ms + del_ms) / 1e3),
         "insert_ms": ins_ms / K,
         "delete_ms": del_ms / K,
         "walk_ms": walk_ms / K,
         "load_ms": load_ms,
         "walk_draws_per_step": attempts / max(steps, 1),
         "walk_schedule": tuned,
-        "plain_form_walk_steps_per_s": plain_steps / (min(plain_ms) / 1e3),
-        "batched_vs_plain": (steps / (walk_ms / 1e3)) / (plain_steps / (min(plain_ms) / 1e3)),
+        "plain_form_walk_steps_per_s": plain_rate,
+        "batched_vs_plain": walk_rate / plain_rate,
         "store_bytes": st["bytes_reserved"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_GBps": traffic / kern_s / 1e9 if traffic else None,
-                     "kernel": "k_walk_persist" if (tuned or {}).get("form", 2) == 2 else "k_walk",
+                     "kernel": "k_walk_persist" if tuned.get("form") == B.FORM_PERSISTENT else "k_walk",
                      "kernel_ms": kern_ms / K, "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": None,
     }
-    for di, dd in batches:
+    if wgraph:
+        wgraph.close()
+    for di, dd, _ in batches:
         di.free()
         dd.free()
     d_paths.free()

@@ -136,6 +136,9 @@ SIGNATURES = {
     "bingo_walk": (_int, [_vp, _int, C.POINTER(WalkParams), _vp, _vp, _i64, _vp, C.POINTER(_u64)]),
     "bingo_walk_device": (_int, [_vp, _int, C.POINTER(WalkParams), _vp, _vp, _i64, _vp, _vp]),
     "bingo_walk_tune": (_int, [_vp, _int, C.POINTER(WalkParams), _i64, C.POINTER(WalkParams)]),
+    "bingo_walk_graph_create": (_int, [_vp, _int, C.POINTER(WalkParams), _vp, _vp, _i64, _vp, _vp, C.POINTER(_vp)]),
+    "bingo_walk_graph_launch": (_int, [_vp]),
+    "bingo_walk_graph_destroy": (None, [_vp]),
     "bingo_num_nodes": (_i32, [_vp]),
     "bingo_stats_get": (_int, [_vp, C.POINTER(Stats)]),
     "bingo_sync": (_int, [_vp]),
@@ -282,6 +285,28 @@ class DeviceArray:
             self.ptr = None
 
 
+class WalkGraph:
+    """A captured walk launch (reset + walk kernel) replayed with hipGraphLaunch."""
+
+    def __init__(self, graph, app, params, nwalkers, d_paths, d_steps, d_starts=None, d_lens=None):
+        h = C.c_void_p()
+        _check(lib().bingo_walk_graph_create(graph._h, app, C.byref(params), d_starts.ptr if d_starts else None,
+                                             d_lens.ptr if d_lens else None, nwalkers, d_paths.ptr, d_steps.ptr,
+                                             C.byref(h)))
+        self._h = h
+        self.g = graph  # the store must outlive its captured walks
+
+    def launch(self):
+        _check(lib().bingo_walk_graph_launch(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().bingo_walk_graph_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+
 class Graph:
     """One HBM-resident dynamic graph on one GPU (ids 1..n)."""
 
@@ -351,6 +376,11 @@ class Graph:
                     d_starts: DeviceArray | None = None, d_lens: DeviceArray | None = None):
         _check(lib().bingo_walk_device(self._h, app, C.byref(params), d_starts.ptr if d_starts else None,
                                        d_lens.ptr if d_lens else None, nwalkers, d_paths.ptr, d_steps.ptr))
+
+    def walk_graph(self, app, params: WalkParams, nwalkers, d_paths: DeviceArray, d_steps: DeviceArray,
+                   d_starts: DeviceArray | None = None, d_lens: DeviceArray | None = None):
+        """Capture one walk launch as a HIP graph; call .launch() to replay it."""
+        return WalkGraph(self, app, params, nwalkers, d_paths, d_steps, d_starts, d_lens)
 
     def tune(self, app, params: WalkParams, sample):
         """Autotune the walk schedule for `app`; returns the chosen knobs."""

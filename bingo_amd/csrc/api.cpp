@@ -83,6 +83,7 @@ struct bingo_graph {
     uint64_t last_walk_steps = 0;
     unsigned long long *d_steps = nullptr;
     int num_cu = 0;
+    uint64_t store_gen = 0;  // bumped whenever a pool moves
     struct Tuned {
         bool valid = false;
         int32_t max_len = 0;
@@ -199,6 +200,7 @@ static int ensure_pools(bingo_graph *g)
         }
         *slotp = fresh;
         g->s.cap[p] = newcap[p];
+        ++g->store_gen;  // captured walk graphs hold the old pool addresses
     }
     return BINGO_OK;
 }
@@ -597,6 +599,110 @@ int bingo_walk_device(bingo_graph *g, int app, const bingo_walk_params *params, 
     rc = launch_choice(g, app, c, a, params->blocks_per_cu);
     phase_end(g, PH_WALK);
     return rc;
+}
+
+}  // extern "C"
+
+struct bingo_walk_graph {
+    bingo_graph *g = nullptr;
+    int app = 0;
+    bingo_walk_params params{};
+    WalkArgs args{};
+    WalkChoice choice{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t gen = 0;
+};
+
+static void walk_graph_release(bingo_walk_graph *wg)
+{
+    if (wg->exec) (void)hipGraphExecDestroy(wg->exec);
+    if (wg->graph) (void)hipGraphDestroy(wg->graph);
+    wg->exec = nullptr;
+    wg->graph = nullptr;
+}
+
+// Capture reset + walk on the handle's stream.  Nothing captured allocates
+// or issues a memset: the counters are cleared by a kernel.
+static int walk_graph_capture(bingo_walk_graph *wg)
+{
+    bingo_graph *g = wg->g;
+    walk_graph_release(wg);
+    HIPCHK(hipStreamBeginCapture(g->st, hipStreamCaptureModeThreadLocal));
+    launch_reset_walk(g->s.ctl, g->st);
+    int rc = launch_choice(g, wg->app, wg->choice, wg->args, wg->params.blocks_per_cu);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(g->st, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(BINGO_EDEVICE, std::string("walk graph capture: ") + hipGetErrorString(e));
+    wg->graph = graph;
+    if (hipGraphInstantiate(&wg->exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        walk_graph_release(wg);
+        return fail(BINGO_EDEVICE, "walk graph instantiate failed");
+    }
+    wg->gen = g->store_gen;
+    return BINGO_OK;
+}
+
+extern "C" {
+
+int bingo_walk_graph_create(bingo_graph *g, int app, const bingo_walk_params *params, const int32_t *d_starts,
+                            const int32_t *d_lens, int64_t nwalkers, int32_t *d_paths, uint64_t *d_steps,
+                            bingo_walk_graph **out)
+{
+    if (!out) return fail(BINGO_EINVAL, "null output");
+    *out = nullptr;
+    if (!g || !g->loaded) return fail(BINGO_ESTATE, "graph not loaded");
+    if (nwalkers < 1 || !d_paths || !d_steps) return fail(BINGO_EINVAL, "bad walk buffers");
+    if (app == BINGO_NODE2VEC && !g->s.nbr) return fail(BINGO_EUNSUPPORTED, "node2vec needs neighbor_index=1");
+    bingo_walk_graph *wg = new bingo_walk_graph();
+    wg->g = g;
+    wg->app = app;
+    int rc = fill_walk_args(params, app, &wg->args);
+    if (!rc) rc = resolve_walk(g, app, params, &wg->choice);
+    if (rc) {
+        delete wg;
+        return rc;
+    }
+    wg->params = *params;
+    wg->args.starts = d_starts;
+    wg->args.lens = d_lens;
+    wg->args.paths = d_paths;
+    wg->args.steps = reinterpret_cast<unsigned long long *>(d_steps);
+    wg->args.cursor = &g->s.ctl->cursor;
+    wg->args.nwalkers = nwalkers;
+    if ((rc = walk_graph_capture(wg))) {
+        delete wg;
+        return rc;
+    }
+    *out = wg;
+    return BINGO_OK;
+}
+
+int bingo_walk_graph_launch(bingo_walk_graph *wg)
+{
+    if (!wg || !wg->g || !wg->g->loaded) return fail(BINGO_ESTATE, "no walk graph");
+    bingo_graph *g = wg->g;
+    if (wg->gen != g->store_gen) {  // same walk, new pool addresses
+        int rc = walk_graph_capture(wg);
+        if (rc) return rc;
+    }
+    phase_begin(g, PH_WALK);
+    HIPCHK(hipGraphLaunch(wg->exec, g->st));
+    phase_end(g, PH_WALK);
+    return BINGO_OK;
+}
+
+void bingo_walk_graph_destroy(bingo_walk_graph *wg)
+{
+    if (!wg) return;
+    if (wg->g && wg->g->st) (void)hipStreamSynchronize(wg->g->st);
+    walk_graph_release(wg);
+    delete wg;
 }
 
 int bingo_walk_tune(bingo_graph *g, int app, const bingo_walk_params *params, int64_t sample,

@@ -138,6 +138,18 @@ int bingo_walk_device(bingo_graph *g, int app, const bingo_walk_params *params,
                       const int32_t *d_starts, const int32_t *d_lens, int64_t nwalkers,
                       int32_t *d_paths, uint64_t *d_steps);
 
+/* A walk launch captured once as a HIP graph and replayed (batched form):
+ * the reset of the walk counters and the walk kernel, with the device
+ * buffers given here.  Replays see the store as it is at replay time; if
+ * update batches grew the HBM pools since capture, the launch re-captures
+ * transparently.  *d_steps accumulates over replays. */
+typedef struct bingo_walk_graph bingo_walk_graph;
+int bingo_walk_graph_create(bingo_graph *g, int app, const bingo_walk_params *params, const int32_t *d_starts,
+                            const int32_t *d_lens, int64_t nwalkers, int32_t *d_paths, uint64_t *d_steps,
+                            bingo_walk_graph **out);
+int bingo_walk_graph_launch(bingo_walk_graph *wg);
+void bingo_walk_graph_destroy(bingo_walk_graph *wg);
+
 /* Autotuner: times the persistent form over block size, walkers per lane
  * and LDS staging depth (and the plain form) on up to `sample` walkers with
  * strided starts, keeps the fastest for `app` on this handle (used by

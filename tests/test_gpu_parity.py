@@ -294,3 +294,37 @@ def test_tuner_picks_a_valid_schedule(orc):
     assert np.array_equal(pa, pb) and sa == sb
     with pytest.raises(B.BingoError):
         g.walk(B.DEEPWALK, B.walk_params(max_len=40, form=B.FORM_PERSISTENT, lds_depth=16), nwalkers=4)
+
+
+@pytest.mark.parametrize("form", [B.FORM_PLAIN, B.FORM_PERSISTENT])
+def test_walk_graph_replays_match_eager(orc, form):
+    """A captured walk replayed several times, with eager walks, syncs and an
+    update batch that grows the pools (forcing a re-capture) in between:
+    every replay equals the eager walk and the oracle."""
+    scale = 11
+    n = 1 << scale
+    base = B.gen_rmat(scale, 6000, B.W_LOGUNI32, seed=2)
+    g, o = _pair(orc, n, base)
+    nw, L = 5000, 30
+    dp = B.DeviceArray(g, nbytes=nw * L * 4)
+    ds = B.DeviceArray(g, np.zeros(1, np.uint64))
+    wp = B.walk_params(seed=4, max_len=L, form=form, block=256, walkers_per_lane=2, lds_depth=0)
+    op = orc.walk_params(seed=4, max_len=L)
+    wgr = g.walk_graph(B.NODE2VEC, wp, nw, dp, ds)
+    ug = B.UpdateGen(n, scale, base, 6000, B.W_LOGUNI32, seed=2)
+    total = 0
+    for rep in range(4):
+        if rep == 2:
+            ins = ug.inserts(40000)  # several times the loaded arcs: pools must grow
+            g.insert(ins)
+            assert o.insert(ins) == 0
+        print(f"[parity] walk graph replay {rep}", flush=True)
+        wgr.launch()
+        g.sync()
+        pa = dp.to_host(np.int32, (nw, L))
+        eager, se = g.walk(B.NODE2VEC, wp, nwalkers=nw)
+        pb, sb = o.walk(B.NODE2VEC, op, nwalkers=nw)
+        assert np.array_equal(pa, eager) and np.array_equal(pa, pb), rep
+        total += sb
+    assert int(ds.to_host(np.uint64, (1,))[0]) == total
+    wgr.close()
