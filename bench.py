@@ -13,7 +13,10 @@ cores on a bounded sample.  `c3` is the batched workload (R-MAT 2^23 / 2^27,
 Zipf weights, 10^7 PPR walkers with lengths in [5, 200], update batches of
 log-uniform size), replayed from a captured HIP graph; `c1` the small one.
 
-    python bench.py [--steps K] [--warmup W] [--config c1|c2|c3] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--config c1|c2|c3|c4] [--no-cpu]
+
+`c4` is the served form: a Poisson stream of walk queries and update
+batches through the serving loop, with p50/p99 latency.
 """
 from __future__ import annotations
 
@@ -106,6 +109,113 @@ def cpu_baseline(base, n, cfg, seed, budget_s=15.0):
                       f"graph, {steps} steps in {dt:.2f}s on {threads} threads (load {load_s:.1f}s not counted)"}
 
 
+def served(B, args):
+    """Served form on the c2 graph: a Poisson stream of walk queries (10^3 to
+    10^6 walkers, DeepWalk/node2vec mixed, length 80) with update batches
+    (10^3 to 10^5 arcs) among them, submitted to the serving loop at
+    `--rate` requests/s; outputs stay in HBM.  Reports latency percentiles
+    (submit -> completion) and throughput, next to the same requests run one
+    at a time as plain per-launch walks."""
+    cfg = CONFIGS["c2"]
+    scale, m = cfg["scale"], cfg["edges"]
+    n = 1 << scale
+    rng = np.random.default_rng(args.seed + 7)
+    base = B.gen_rmat(scale, m, cfg["wkind"], seed=args.seed)
+    ug = B.UpdateGen(n, scale, base, m, cfg["wkind"], seed=args.seed)
+    nreq = args.requests
+    reqs = []
+    for i in range(nreq):
+        if rng.random() < 0.1:
+            k = int(round(np.exp(rng.uniform(np.log(1e3), np.log(1e5)))))
+            reqs.append(("update", ug.inserts(k // 2), ug.deletes(k - k // 2)))
+        else:
+            nw = int(round(np.exp(rng.uniform(np.log(1e3), np.log(1e6)))))
+            app = B.DEEPWALK if rng.random() < 0.5 else B.NODE2VEC
+            starts = rng.integers(1, n + 1, size=nw, dtype=np.int32)
+            reqs.append(("walk", app, starts, int(rng.integers(1, 2 ** 31))))
+    gaps = rng.exponential(1.0 / args.rate, size=nreq)
+
+    def fresh_graph():
+        g = B.Graph(device=0, neighbor_index=True)
+        d_base = B.DeviceArray(g, base)
+        g.load_device(n, d_base, len(base))
+        g.sync()
+        d_base.free()
+        return g
+
+    # served form
+    g = fresh_graph()
+    outs = [B.DeviceArray(g, nbytes=len(r[2]) * 80 * 4) if r[0] == "walk" else None for r in reqs]
+    srv = B.Server(g, arena_bytes=1 << 30, max_launch_walkers=1 << 22)
+    log(f"served: {nreq} requests at {args.rate} req/s")
+    tickets, t_sub = [], []
+    t0 = time.perf_counter()
+    due = t0
+    for r, gap, out in zip(reqs, gaps, outs):
+        due += gap
+        while time.perf_counter() < due:
+            pass
+        if r[0] == "walk":
+            tickets.append(srv.submit_walk(r[1], B.walk_params(seed=r[3], max_len=80), starts=r[2], out=out))
+        else:
+            tickets.append(srv.submit_update(r[1], r[2]))
+    submit_span = time.perf_counter() - t0
+    lat_per_mstep, steps_total = [], 0
+    for r, t in zip(reqs, tickets):
+        _, steps, lat = srv.wait(t)
+        if r[0] == "walk":
+            steps_total += steps
+            lat_per_mstep.append(lat / max(steps, 1) * 1e6)
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    for o in outs:
+        if o:
+            o.free()
+    g.close()
+
+    # plain form: the same requests one at a time, one plain launch each
+    g = fresh_graph()
+    dpaths = B.DeviceArray(g, nbytes=max(len(r[2]) for r in reqs if r[0] == "walk") * 80 * 4)
+    dsteps = B.DeviceArray(g, np.zeros(1, np.uint64))
+    t1 = time.perf_counter()
+    for r in reqs:
+        if r[0] == "walk":
+            ds = B.DeviceArray(g, r[2])
+            g.walk_device(r[1], B.walk_params(seed=r[3], max_len=80, form=B.FORM_PLAIN), len(r[2]), dpaths, dsteps,
+                          d_starts=ds)
+            g.sync()
+            ds.free()
+        else:
+            g.insert(r[1])
+            g.delete(r[2])
+    plain_wall = time.perf_counter() - t1
+    plain_steps = int(dsteps.to_host(np.uint64, (1,))[0])
+    g.close()
+    lp = np.percentile(lat_per_mstep, [50, 99]) if lat_per_mstep else [None, None]
+    nupd = sum(len(r[1]) + len(r[2]) for r in reqs if r[0] == "update")
+    out = {
+        "metric": METRIC,
+        "value": steps_total / wall,
+        "unit": "walk steps/s",
+        "n_gpus": 1, "steps": nreq, "warmup": 0, "ms_per_step": wall * 1e3 / nreq,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": "served_rmat22_e26_loguni32_mixed_poisson", "nodes": n, "arcs": int(len(base)),
+                   "requests": nreq, "offered_rate_per_s": args.rate, "walkers_per_query": [1000, 1000000],
+                   "update_batch": [1000, 100000], "walk_len": 80, "apps": ["deepwalk", "node2vec"]},
+        "achieved_rate_per_s": nreq / wall,
+        "submit_span_s": submit_span,
+        "p50_ms": st["p50_ms"], "p90_ms": st["p90_ms"], "p99_ms": st["p99_ms"], "max_ms": st["max_ms"],
+        "p50_ms_per_1M_steps": float(lp[0]), "p99_ms_per_1M_steps": float(lp[1]),
+        "launches": st["launches"], "walk_requests": st["walk_requests"], "update_requests": st["update_requests"],
+        "updates_applied": nupd,
+        "plain_form_walk_steps_per_s": plain_steps / plain_wall,
+        "served_vs_plain": (steps_total / wall) / (plain_steps / plain_wall),
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def copy_calibration(B, mib):
     nbytes = mib << 20
     g = B.Graph(device=0)
@@ -127,7 +237,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c4"])
+    ap.add_argument("--rate", type=float, default=2000.0, help="c4: offered requests per second")
+    ap.add_argument("--requests", type=int, default=300, help="c4: number of requests")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--schedule", default="", metavar="BLOCK,WPL,DEPTH",
                     help="fixed persistent walk knobs instead of autotuning (e.g. 512,2,16)")
@@ -143,6 +255,9 @@ def main():
 
     if args.copy_calibration:
         copy_calibration(B, args.copy_calibration)
+        return
+    if args.config == "c4":
+        served(B, args)
         return
 
     cfg = CONFIGS[args.config]

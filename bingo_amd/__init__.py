@@ -96,6 +96,32 @@ class Dump(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("arena_bytes", C.c_uint64),
+        ("max_launch_walkers", C.c_int64),
+        ("blocks_per_cu", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("requests", C.c_uint64),
+        ("walk_requests", C.c_uint64),
+        ("update_requests", C.c_uint64),
+        ("rejected", C.c_uint64),
+        ("walk_steps", C.c_uint64),
+        ("launches", C.c_uint64),
+        ("p50_ms", C.c_double),
+        ("p90_ms", C.c_double),
+        ("p99_ms", C.c_double),
+        ("max_ms", C.c_double),
+        ("mean_ms", C.c_double),
+        ("busy_s", C.c_double),
+    ]
+
+
 class InstanceOpts(C.Structure):
     _fields_ = [
         ("rounds", C.c_int32),
@@ -152,6 +178,13 @@ SIGNATURES = {
     "bingo_memcpy_d2d": (_int, [_vp, _vp, _vp, C.c_size_t]),
     "bingo_event_record": (_int, [_vp, _int]),
     "bingo_event_elapsed": (_int, [_vp, _int, _int, C.POINTER(C.c_float)]),
+    "bingo_server_config_default": (None, [C.POINTER(ServerConfig)]),
+    "bingo_server_create": (_int, [_vp, C.POINTER(ServerConfig), C.POINTER(_vp)]),
+    "bingo_server_submit_walk": (_int, [_vp, _int, C.POINTER(WalkParams), _vp, _vp, _i64, _vp, _int, C.POINTER(_u64)]),
+    "bingo_server_submit_update": (_int, [_vp, _vp, _i64, _vp, _i64, C.POINTER(_u64)]),
+    "bingo_server_wait": (_int, [_vp, _u64, C.POINTER(_u64), C.POINTER(C.c_double)]),
+    "bingo_server_stats_get": (_int, [_vp, C.POINTER(ServerStats)]),
+    "bingo_server_destroy": (None, [_vp]),
     "bingo_instance_opts_default": (None, [C.POINTER(InstanceOpts)]),
     "bingo_read_edgelist": (_int, [C.c_char_p, C.POINTER(_i32), _vp, _i64, C.POINTER(_i64)]),
     "bingo_instance": (_int, [_int, _int, C.c_char_p, C.POINTER(InstanceOpts), C.POINTER(InstanceResult)]),
@@ -302,6 +335,67 @@ class WalkGraph:
     def close(self):
         if getattr(self, "_h", None):
             lib().bingo_walk_graph_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+
+class Server:
+    """Serving loop on a loaded graph: submit walks and update batches from any
+    thread, collect results with wait().  The graph must not be used directly
+    while the server runs."""
+
+    def __init__(self, graph, arena_bytes=4 << 30, max_launch_walkers=1 << 23, blocks_per_cu=0):
+        cfg = ServerConfig()
+        lib().bingo_server_config_default(C.byref(cfg))
+        cfg.arena_bytes, cfg.max_launch_walkers, cfg.blocks_per_cu = arena_bytes, max_launch_walkers, blocks_per_cu
+        h = C.c_void_p()
+        _check(lib().bingo_server_create(graph._h, C.byref(cfg), C.byref(h)))
+        self._h = h
+        self.g = graph
+        self._keep = {}  # host output buffers of pending requests
+
+    def submit_walk(self, app, params: WalkParams, nwalkers=None, starts=None, lens=None, out=None):
+        """out: a numpy int32 [nwalkers][max_len] array (filled before
+        completion) or a DeviceArray (written in HBM).  Returns a ticket."""
+        if starts is not None:
+            starts = np.ascontiguousarray(starts, np.int32)
+            nwalkers = len(starts)
+        if lens is not None:
+            lens = np.ascontiguousarray(lens, np.int32)
+        on_dev = isinstance(out, DeviceArray)
+        if out is None:
+            out = np.empty((nwalkers, params.max_len), np.int32)
+        ptr = out.ptr if on_dev else _ptr(out)
+        t = C.c_uint64(0)
+        _check(lib().bingo_server_submit_walk(self._h, app, C.byref(params), _ptr(starts), _ptr(lens), nwalkers, ptr,
+                                              int(on_dev), C.byref(t)))
+        self._keep[t.value] = out
+        return t.value
+
+    def submit_update(self, inserts=None, deletes=None):
+        ins = np.ascontiguousarray(inserts if inserts is not None else np.zeros(0, EDGE_DT), EDGE_DT)
+        dels = np.ascontiguousarray(deletes if deletes is not None else np.zeros(0, DEL_DT), DEL_DT)
+        t = C.c_uint64(0)
+        _check(lib().bingo_server_submit_update(self._h, _ptr(ins), len(ins), _ptr(dels), len(dels), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket):
+        """Returns (output, steps, latency_ms); raises on a failed request."""
+        steps, lat = C.c_uint64(0), C.c_double(0)
+        rc = lib().bingo_server_wait(self._h, ticket, C.byref(steps), C.byref(lat))
+        out = self._keep.pop(ticket, None)
+        _check(rc)
+        return out, steps.value, lat.value
+
+    def stats(self):
+        s = ServerStats()
+        _check(lib().bingo_server_stats_get(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().bingo_server_destroy(self._h)
             self._h = None
 
     __del__ = close

@@ -51,7 +51,7 @@ def _run(cmd, verbose):
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     objs = []
-    for src in ("kernels.hip", "api.cpp", "apps.cpp"):
+    for src in ("kernels.hip", "api.cpp", "apps.cpp", "server.cpp"):
         obj = os.path.join(BUILD, src.rsplit(".", 1)[0] + ".o")
         _run([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj], verbose)
         objs.append(obj)

@@ -17,79 +17,22 @@
 #include <vector>
 
 #include "../../include/bingo.h"
+#include "handle.h"
 #include "store.h"
 
 using namespace bingo;
 
+namespace bingo {
+
 static thread_local std::string g_err;
 
-static int fail(int code, const std::string &msg)
+int fail(int code, const std::string &msg)
 {
     g_err = msg;
     return code;
 }
 
-#define HIPCHK(call)                                                                              \
-    do {                                                                                          \
-        hipError_t e_ = (call);                                                                   \
-        if (e_ != hipSuccess) return fail(BINGO_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-namespace {
-
-enum Scratch {
-    SC_KEYS,
-    SC_KEYS2,
-    SC_VALS,
-    SC_VALS2,
-    SC_EDGES,
-    SC_UNIQ,
-    SC_COUNTS,
-    SC_STARTS,
-    SC_NRUNS,
-    SC_TMP,
-    SC_U64A,
-    SC_U64B,
-    SC_STAGE,
-    SC_AUX1,
-    SC_AUX2,
-    SC_AUX3,
-    NSCRATCH
-};
-
-constexpr size_t kElem[NPOOL] = {4, 4, 8, 8};
-constexpr int kEvents = 16;
-
-enum Phase { PH_LOAD, PH_INSERT, PH_DELETE, PH_WALK, NPHASE };
-
-}  // namespace
-
-struct bingo_graph {
-    bingo_config cfg{};
-    int device = 0;
-    hipStream_t st = nullptr;
-    DevStore s{};
-    bool loaded = false;
-    int64_t arcs = 0;
-    size_t node_bytes = 0;
-    uint64_t budget = 0;
-    Ctl *ctl_host = nullptr;  // pinned
-    void *sc[NSCRATCH] = {};
-    size_t sc_bytes[NSCRATCH] = {};
-    hipEvent_t ev[kEvents] = {};
-    hipEvent_t ph[NPHASE][2] = {};
-    bool ph_pending[NPHASE] = {};
-    double ph_ms[NPHASE] = {};
-    uint64_t last_walk_steps = 0;
-    unsigned long long *d_steps = nullptr;
-    int num_cu = 0;
-    uint64_t store_gen = 0;  // bumped whenever a pool moves
-    struct Tuned {
-        bool valid = false;
-        int32_t max_len = 0;
-        int form = BINGO_FORM_PERSISTENT, block = 256, wpl = 2, depth = 0;
-    } tuned[4];
-};
+}  // namespace bingo
 
 static int ensure_scratch(bingo_graph *g, int idx, size_t bytes)
 {
@@ -361,7 +304,7 @@ static uint32_t thr27(double prob)
     return (uint32_t)std::floor(prob * 134217728.0);
 }
 
-static int fill_walk_args(const bingo_walk_params *p, int app, WalkArgs *a)
+int bingo::fill_walk_args(const bingo_walk_params *p, int app, WalkArgs *a)
 {
     if (!p) return fail(BINGO_EINVAL, "null walk params");
     if (app < 0 || app > 3) return fail(BINGO_EINVAL, "unknown app");
@@ -415,7 +358,7 @@ void bingo_walk_params_default(bingo_walk_params *p)
     p->reserved = 0;
 }
 
-const char *bingo_last_error(void) { return g_err.c_str(); }
+const char *bingo_last_error(void) { return bingo::g_err.c_str(); }
 const char *bingo_version(void) { return "bingo-amd 0.1 gfx950"; }
 
 int bingo_create(const bingo_config *cfg, bingo_graph **out)

@@ -933,6 +933,151 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
 }
 
 // ------------------------------------------------------------------------
+// serving: one persistent launch runs the walkers of many requests.  The
+// walkers of all admitted requests form one index space (request r owns
+// [base_r, base_r + nwalkers_r)); lanes claim indices with the same
+// wave-aggregated refill as k_walk_persist, map them to (request, local
+// walker) by binary search over the request table held in LDS, and run the
+// request's own app, parameters and seed.  Requests of different apps and
+// sizes therefore share every wave, and a request's paths do not depend on
+// what it was batched with.
+// ------------------------------------------------------------------------
+constexpr int kServeWPL = 2;
+
+__global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *__restrict__ reqs, int nreq,
+                                                    unsigned long long total, unsigned long long *cursor)
+{
+    __shared__ ServeReq rq[kMaxServeReqs];
+    for (int i = threadIdx.x; i < nreq; i += blockDim.x) rq[i] = reqs[i];
+    __syncthreads();
+    const int ln = lane_id();
+    const uint64_t lt = (1ull << ln) - 1ull;
+    int32_t rid[kServeWPL], cur[kServeWPL], prev[kServeWPL], jj[kServeWPL], len[kServeWPL];
+    int64_t wid[kServeWPL];
+    uint32_t att[kServeWPL], acc[kServeWPL];
+    int32_t acc_req[kServeWPL];
+    uint64_t hp[kServeWPL];
+#pragma unroll
+    for (int k = 0; k < kServeWPL; ++k) {
+        rid[k] = acc_req[k] = -1;
+        cur[k] = prev[k] = jj[k] = len[k] = 0;
+        wid[k] = 0;
+        att[k] = acc[k] = 0;
+        hp[k] = 0;
+    }
+    bool more = true;
+    for (;;) {
+        if (more) {
+            uint64_t emp[kServeWPL];
+            uint32_t need = 0;
+#pragma unroll
+            for (int k = 0; k < kServeWPL; ++k) {
+                emp[k] = __ballot(rid[k] < 0);
+                need += (uint32_t)__popcll(emp[k]);
+            }
+            if (need) {
+                unsigned long long base = 0;
+                if (ln == 0) base = atomicAdd(cursor, (unsigned long long)need);
+                base = __shfl(base, 0, 64);
+                uint32_t off = 0;
+#pragma unroll
+                for (int k = 0; k < kServeWPL; ++k) {
+                    if (rid[k] < 0) {
+                        const unsigned long long c = base + off + (uint32_t)__popcll(emp[k] & lt);
+                        if (c < total) {
+                            int lo = 0, hi = nreq - 1;  // last request with base <= c
+                            while (lo < hi) {
+                                const int mid = (lo + hi + 1) >> 1;
+                                if (rq[mid].base <= c) lo = mid; else hi = mid - 1;
+                            }
+                            const ServeReq &q = rq[lo];
+                            const int64_t w = (int64_t)(c - q.base);
+                            if (acc_req[k] != lo) {
+                                if (acc_req[k] >= 0 && acc[k]) atomicAdd(rq[acc_req[k]].steps, (unsigned long long)acc[k]);
+                                acc_req[k] = lo;
+                                acc[k] = 0;
+                            }
+                            const int32_t st = q.starts ? q.starts[w] : (int32_t)(w % s.n) + 1;
+                            int32_t L = q.lens ? q.lens[w] : q.max_len;
+                            if (q.app == 3 && L > 2) L = 2;
+                            int32_t *row = q.paths + w * (int64_t)q.max_len;
+                            if (st < 1 || st > s.n || L < 1 || L > q.max_len) {
+                                atomicOr(&s.ctl->error, (unsigned)ERR_INVAL);
+                                for (int32_t t = 0; t < q.max_len; ++t) row[t] = -1;
+                            } else {
+                                row[0] = st;
+                                rid[k] = lo;
+                                wid[k] = w;
+                                cur[k] = st;
+                                prev[k] = -1;
+                                jj[k] = 1;
+                                len[k] = L;
+                                att[k] = 0;
+                                if (L == 1) {
+                                    for (int32_t t = 1; t < q.max_len; ++t) row[t] = -1;
+                                    rid[k] = -1;
+                                }
+                            }
+                        }
+                    }
+                    off += (uint32_t)__popcll(emp[k]);
+                }
+                if (base + need >= total) more = false;
+            }
+        }
+        bool mine = false;
+#pragma unroll
+        for (int k = 0; k < kServeWPL; ++k) mine |= rid[k] >= 0;
+        if (!__any(mine)) {
+            if (!more) break;
+            continue;
+        }
+#pragma unroll
+        for (int k = 0; k < kServeWPL; ++k) {
+            if (rid[k] < 0) continue;
+            const ServeReq &q = rq[rid[k]];
+            const int64_t w = wid[k];
+            int32_t *row = q.paths + w * (int64_t)q.max_len;
+            const uint4 r = philox((uint32_t)w, (uint32_t)((uint64_t)w >> 32), (uint32_t)jj[k], att[k],
+                                   (uint32_t)q.seed, (uint32_t)(q.seed >> 32));
+            uint64_t hcur;
+            const int32_t nx = sample_step(s.alias, s.mem, cur[k], r, hcur);
+            bool done = nx < 0;
+            if (!done) {
+                const uint32_t aux = r.x & kAuxMask;
+                if (q.app == 1 && jj[k] >= 2 && (int64_t)att[k] + 1 < q.max_attempts) {
+                    const uint32_t tlo = min(q.thr_neighbor, q.thr_other), thi = max(q.thr_neighbor, q.thr_other);
+                    bool ok;
+                    if (nx == prev[k]) ok = aux < q.thr_return;
+                    else if (aux < tlo) ok = true;
+                    else if (aux >= thi) ok = false;
+                    else ok = aux < (is_neighbor(s.hash, hp[k], (uint32_t)nx) ? q.thr_neighbor : q.thr_other);
+                    if (!ok) {
+                        ++att[k];
+                        continue;
+                    }
+                }
+                row[jj[k]] = nx;
+                ++acc[k];
+                prev[k] = cur[k];
+                cur[k] = nx;
+                hp[k] = hcur;
+                ++jj[k];
+                att[k] = 0;
+                done = (q.app == 2 && aux < q.thr_stop) || jj[k] >= len[k];
+            }
+            if (done) {
+                for (int32_t t = jj[k]; t < q.max_len; ++t) row[t] = -1;
+                rid[k] = -1;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kServeWPL; ++k)
+        if (acc_req[k] >= 0 && acc[k]) atomicAdd(rq[acc_req[k]].steps, (unsigned long long)acc[k]);
+}
+
+// ------------------------------------------------------------------------
 // export and point queries
 // ------------------------------------------------------------------------
 __global__ void k_export_a(DevStore s, uint32_t *nslots, uint64_t *mass, uint32_t *bsize, uint64_t *T, uint8_t *alias)
@@ -1104,6 +1249,23 @@ void launch_walk(DevStore s, int app, const WalkArgs &a, int block, hipStream_t 
     case 2: hipLaunchKernelGGL(k_walk<2>, grid, blk, 0, st, s, a); break;
     default: hipLaunchKernelGGL(k_walk<3>, grid, blk, 0, st, s, a); break;
     }
+}
+
+int serve_max_blocks_per_cu()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void *>(&k_walk_serve), 256, 0) !=
+        hipSuccess) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    return nb > 0 ? nb : 1;
+}
+
+void launch_walk_serve(DevStore s, const ServeReq *reqs, int nreq, unsigned long long total,
+                       unsigned long long *cursor, int blocks, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_walk_serve, dim3(blocks), dim3(256), 0, st, s, reqs, nreq, total, cursor);
 }
 
 size_t persist_lds_bytes(int block, int wpl, int depth)

@@ -109,6 +109,24 @@ struct WalkArgs {
     int32_t max_attempts;
 };
 
+// One admitted walk request of the serving loop.  Its walkers occupy the
+// global index range [base, base + nwalkers) of a serving launch; walker i
+// of the request draws from Philox(seed; i, step, attempt), exactly as the
+// request would alone.
+struct ServeReq {
+    const int32_t *starts;
+    const int32_t *lens;
+    int32_t *paths;
+    unsigned long long *steps;
+    unsigned long long base;
+    int64_t nwalkers;
+    uint64_t seed;
+    uint32_t thr_return, thr_neighbor, thr_other, thr_stop;
+    int32_t max_len, max_attempts;
+    int32_t app, pad;
+};
+constexpr int kMaxServeReqs = 128;
+
 struct Runs {          // per-node runs of a sorted batch
     const uint32_t *node;
     const uint32_t *start;
@@ -136,6 +154,9 @@ struct PersistCfg {
 size_t persist_lds_bytes(int block, int wpl, int depth);
 int persist_max_blocks_per_cu(int app, int wpl, int depth, int block);
 void launch_walk_persist(DevStore s, int app, const WalkArgs &a, const PersistCfg &c, hipStream_t st);
+void launch_walk_serve(DevStore s, const ServeReq *reqs, int nreq, unsigned long long total,
+                       unsigned long long *cursor, int blocks, hipStream_t st);
+int serve_max_blocks_per_cu();
 void launch_export_a(DevStore s, uint32_t *nslots, uint64_t *mass, uint32_t *bsize, uint64_t *T, uint8_t *alias,
                      hipStream_t st);
 void launch_export_b(DevStore s, const uint64_t *slot_out_off, const uint64_t *mem_out_off, uint32_t *slot_v,

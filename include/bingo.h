@@ -180,6 +180,45 @@ int bingo_memcpy_d2d(bingo_graph *g, void *d_dst, const void *d_src, size_t byte
 int bingo_event_record(bingo_graph *g, int slot);
 int bingo_event_elapsed(bingo_graph *g, int slot_a, int slot_b, float *ms);
 
+/* ---- serving loop (continuous batching of walk queries and update batches) ----
+ * A server owns a loaded graph handle (no other calls on that handle while it
+ * runs) and a worker thread.  Requests take effect in submission order: a walk
+ * sees exactly the update batches submitted before it.  Consecutive walk
+ * requests are admitted together into one persistent launch while the launch's
+ * walker budget and the device arena (the HBM budget for request inputs and
+ * host-bound outputs) allow; the rest wait for the next launch, which starts as
+ * soon as the previous one drains.  A walk's paths are those it would get alone
+ * (walker i of a request draws from Philox(seed; i, step, attempt)).  There is
+ * no reference counterpart: the reference runs one walk kernel at a time
+ * (include/graph.cuh:93-150). */
+typedef struct bingo_server bingo_server;
+typedef struct {
+    uint64_t arena_bytes;      /* device arena for request staging (HBM admission budget) */
+    int64_t max_launch_walkers;/* walker budget of one launch (a larger request runs alone) */
+    int32_t blocks_per_cu;     /* persistent grid per CU; 0 = occupancy limit */
+    int32_t reserved;
+} bingo_server_config;
+typedef struct {
+    uint64_t requests, walk_requests, update_requests, rejected;
+    uint64_t walk_steps, launches;
+    double p50_ms, p90_ms, p99_ms, max_ms, mean_ms; /* submit -> completion, walk requests */
+    double busy_s;             /* time the worker spent executing launches and updates */
+} bingo_server_stats;
+void bingo_server_config_default(bingo_server_config *c);
+int bingo_server_create(bingo_graph *g, const bingo_server_config *c, bingo_server **out);
+/* starts/lens (host, may be NULL) are copied at submit.  paths is
+ * [nwalkers][max_len]; paths_on_device != 0: a device pointer the kernel
+ * writes directly, else a host pointer filled before completion. */
+int bingo_server_submit_walk(bingo_server *s, int app, const bingo_walk_params *params, const int32_t *starts,
+                             const int32_t *lens, int64_t nwalkers, int32_t *paths, int paths_on_device,
+                             uint64_t *ticket);
+int bingo_server_submit_update(bingo_server *s, const bingo_edge *ins, int64_t ni, const bingo_del *dels, int64_t nd,
+                               uint64_t *ticket);
+/* blocks until the request completed; returns its status (0 or BINGO_E*) */
+int bingo_server_wait(bingo_server *s, uint64_t ticket, uint64_t *steps_out, double *latency_ms);
+int bingo_server_stats_get(bingo_server *s, bingo_server_stats *out);
+void bingo_server_destroy(bingo_server *s); /* completes every submitted request first */
+
 /* ---- the reference driver: instance() (include/apps.h:4, src/apps.cu:10-42) ----
  * Reads a "u v" edge list (0-based ids, one arc per pair, as graph.cuh:22-27),
  * weights arcs bias(outdeg(v)) + 1, loads, then runs `rounds` x {insert batch,

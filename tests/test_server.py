@@ -1,0 +1,121 @@
+"""Serving loop: walk queries and update batches submitted together are
+batched into shared launches, yet every walk returns exactly what it returns
+alone on the same graph state (and what the oracle returns)."""
+import random
+
+import numpy as np
+import pytest
+
+import bingo_amd as B
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests(n, scale, base, m, rng):
+    ug = B.UpdateGen(n, scale, base, m, B.W_LOGUNI32, seed=77)
+    reqs = []
+    for i in range(40):
+        if i % 9 == 8:
+            reqs.append(("update", ug.inserts(rng.randint(10, 3000)), ug.deletes(rng.randint(10, 2000))))
+            continue
+        app = [B.DEEPWALK, B.NODE2VEC, B.PPR, B.SAMPLING][i % 4]
+        L = rng.choice([8, 20, 33])
+        p = B.walk_params(seed=1000 + i, max_len=L, tp=0.1, p=rng.choice([0.5, 2.0]), q=rng.choice([0.5, 2.0]))
+        nw = rng.choice([1, 7, 500, 4000])
+        starts = np.array([rng.randint(1, n) for _ in range(nw)], np.int32) if i % 3 else None
+        lens = np.array([rng.randint(1, L) for _ in range(nw)], np.int32) if i % 5 == 1 else None
+        reqs.append(("walk", app, p, nw, starts, lens))
+    return reqs
+
+
+def _load(n, base):
+    g = B.Graph(neighbor_index=True)
+    g.load(n, base)
+    return g
+
+
+@pytest.mark.parametrize("arena", [1 << 30, 600_000])
+def test_served_results_equal_alone(orc, arena):
+    rng = random.Random(5)
+    scale = 12
+    n = 1 << scale
+    base = B.gen_rmat(scale, 20000, B.W_LOGUNI32, seed=77)
+    reqs = _requests(n, scale, base, 20000, rng)
+
+    alone = []  # each request run by itself, in submission order
+    g1 = _load(n, base)
+    o = orc.OracleGraph(n)
+    o.load(base)
+    for i, r in enumerate(reqs):
+        if r[0] == "update":
+            g1.insert(r[1])
+            g1.delete(r[2])
+            o.insert(r[1])
+            o.delete(r[2])
+            alone.append(None)
+        else:
+            _, app, p, nw, starts, lens = r
+            paths, steps = g1.walk(app, p, starts=starts, lens=lens, nwalkers=nw)
+            if i % 4 == 0:
+                op = orc.walk_params(seed=p.seed, max_len=p.max_len, tp=p.tp, p=p.p, q=p.q)
+                pb, sb = o.walk(app, op, starts=starts, lens=lens, nwalkers=nw)
+                assert np.array_equal(paths, pb) and steps == sb
+            alone.append((paths, steps))
+    g1.close()
+
+    g2 = _load(n, base)
+    srv = B.Server(g2, arena_bytes=arena, max_launch_walkers=6000)
+    tickets = []
+    for r in reqs:
+        if r[0] == "update":
+            tickets.append(srv.submit_update(r[1], r[2]))
+        else:
+            _, app, p, nw, starts, lens = r
+            tickets.append(srv.submit_walk(app, p, nwalkers=nw, starts=starts, lens=lens))
+    for i, t in enumerate(tickets):
+        out, steps, lat = srv.wait(t)
+        assert lat >= 0
+        if alone[i] is not None:
+            assert np.array_equal(out, alone[i][0]), i
+            assert steps == alone[i][1], i
+    st = srv.stats()
+    assert st["walk_requests"] == sum(r[0] == "walk" for r in reqs)
+    assert st["update_requests"] == sum(r[0] == "update" for r in reqs)
+    assert st["launches"] < st["walk_requests"]  # requests shared launches
+    assert 0 <= st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
+    srv.close()
+    g2.close()
+
+
+def test_server_device_outputs_and_admission(orc):
+    scale = 11
+    n = 1 << scale
+    base = B.gen_rmat(scale, 8000, B.W_UNIFORM16, seed=3)
+    g = _load(n, base)
+    srv = B.Server(g, arena_bytes=1 << 16, max_launch_walkers=1 << 20)
+    p = B.walk_params(seed=9, max_len=16)
+    # a host-bound output larger than the arena is refused at submit
+    with pytest.raises(B.BingoError) as ei:
+        srv.submit_walk(B.DEEPWALK, p, nwalkers=5000)
+    assert ei.value.code == -2
+    # device-resident outputs need no arena
+    nw = 5000
+    dp = B.DeviceArray(g, nbytes=nw * 16 * 4)
+    t = srv.submit_walk(B.DEEPWALK, p, nwalkers=nw, out=dp)
+    _, steps, _ = srv.wait(t)
+    got = dp.to_host(np.int32, (nw, 16))
+    o = orc.OracleGraph(n)
+    o.load(base)
+    want, ws = o.walk(B.DEEPWALK, orc.walk_params(seed=9, max_len=16), nwalkers=nw)
+    assert np.array_equal(got, want) and steps == ws
+    # an invalid update fails its own ticket only
+    bad = srv.submit_update(deletes=np.array([(1, 10 ** 6)], B.DEL_DT))
+    with pytest.raises(B.BingoError):
+        srv.wait(bad)
+    t2 = srv.submit_walk(B.PPR, p, nwalkers=100)
+    out, _, _ = srv.wait(t2)
+    assert out.shape == (100, 16)
+    assert srv.stats()["rejected"] == 1
+    srv.close()
+    dp.free()
+    g.close()
