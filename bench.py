@@ -160,12 +160,14 @@ def served(B, args):
         else:
             tickets.append(srv.submit_update(r[1], r[2]))
     submit_span = time.perf_counter() - t0
-    lat_per_mstep, steps_total = [], 0
+    lat_per_mstep, steps_total, lat_1m = [], 0, []
     for r, t in zip(reqs, tickets):
         _, steps, lat = srv.wait(t)
         if r[0] == "walk":
             steps_total += steps
             lat_per_mstep.append(lat / max(steps, 1) * 1e6)
+            if 0.5e6 <= steps <= 2e6:  # queries of about one million steps
+                lat_1m.append(lat)
     wall = time.perf_counter() - t0
     st = srv.stats()
     srv.close()
@@ -207,6 +209,9 @@ def served(B, args):
         "submit_span_s": submit_span,
         "p50_ms": st["p50_ms"], "p90_ms": st["p90_ms"], "p99_ms": st["p99_ms"], "max_ms": st["max_ms"],
         "p50_ms_per_1M_steps": float(lp[0]), "p99_ms_per_1M_steps": float(lp[1]),
+        "queries_0p5M_to_2M_steps": len(lat_1m),
+        "p50_ms_1M_step_queries": float(np.percentile(lat_1m, 50)) if lat_1m else None,
+        "p99_ms_1M_step_queries": float(np.percentile(lat_1m, 99)) if lat_1m else None,
         "launches": st["launches"], "walk_requests": st["walk_requests"], "update_requests": st["update_requests"],
         "updates_applied": nupd,
         "plain_form_walk_steps_per_s": plain_steps / plain_wall,

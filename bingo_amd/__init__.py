@@ -508,6 +508,17 @@ class Graph:
         _check(lib().bingo_edge_count(self._h, _ptr(us), _ptr(vs), len(us), _ptr(out)))
         return out
 
+    def export_nodes(self):
+        """Per-node slot counts, masses and bucket sizes only (cheap at any size)."""
+        d = Dump()
+        n1 = self.n + 1
+        out = dict(nslots=np.zeros(n1, np.uint32), mass=np.zeros(n1, np.uint64), bsize=np.zeros((n1, 32), np.uint32))
+        for k, a in out.items():
+            setattr(d, k, a.ctypes.data)
+        _check(lib().bingo_export(self._h, C.byref(d)))
+        out["total_slots"], out["total_members"] = d.total_slots, d.total_members
+        return out
+
     def export(self):
         d = Dump()
         _check(lib().bingo_export(self._h, C.byref(d)))

@@ -858,15 +858,18 @@ int bingo_export(bingo_graph *g, bingo_dump *d)
     }
     d->total_slots = ts;
     d->total_members = tm;
-    if (!d->nslots) {
+    // per-node arrays are copied when their pointer is given; the slot and
+    // member arrays only when all four of them are given
+    if (d->nslots) std::memcpy(d->nslots, nsl.data(), n1 * 4);
+    if (d->bsize) std::memcpy(d->bsize, bsz.data(), ng * 4);
+    if (d->mass) (void)hipMemcpyAsync(d->mass, dm, n1 * 8, hipMemcpyDeviceToHost, g->st);
+    if (d->T) (void)hipMemcpyAsync(d->T, dt, ng * 8, hipMemcpyDeviceToHost, g->st);
+    if (d->alias) (void)hipMemcpyAsync(d->alias, da, ng, hipMemcpyDeviceToHost, g->st);
+    if (!d->slot_v || !d->slot_w || !d->members || !d->backpos) {
+        const hipError_t e0 = hipStreamSynchronize(g->st);
         cleanup();
-        return BINGO_OK;
+        return e0 == hipSuccess ? BINGO_OK : fail(BINGO_EDEVICE, "export");
     }
-    std::memcpy(d->nslots, nsl.data(), n1 * 4);
-    std::memcpy(d->bsize, bsz.data(), ng * 4);
-    (void)hipMemcpyAsync(d->mass, dm, n1 * 8, hipMemcpyDeviceToHost, g->st);
-    (void)hipMemcpyAsync(d->T, dt, ng * 8, hipMemcpyDeviceToHost, g->st);
-    (void)hipMemcpyAsync(d->alias, da, ng, hipMemcpyDeviceToHost, g->st);
     void *dso = nullptr, *dmo = nullptr, *dsv = nullptr, *dsw = nullptr, *dmem = nullptr, *dbp = nullptr;
     auto cleanup2 = [&]() {
         for (void *p : {dso, dmo, dsv, dsw, dmem, dbp})

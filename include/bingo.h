@@ -165,7 +165,9 @@ int bingo_sync(bingo_graph *g);
 void *bingo_stream(bingo_graph *g);      /* the handle's hipStream_t */
 /* multiplicities of live arcs us[i] -> vs[i] (host arrays) */
 int bingo_edge_count(bingo_graph *g, const int32_t *us, const int32_t *vs, int64_t k, uint32_t *counts);
-/* two calls: first with all pointers NULL to learn the totals */
+/* two calls: first with all pointers NULL to learn the totals.  Per-node
+ * arrays (nslots, mass, bsize, T, alias) are filled when non-NULL; the slot
+ * and member arrays only when slot_v, slot_w, members and backpos all are. */
 int bingo_export(bingo_graph *g, bingo_dump *d);
 
 /* ---- device memory helpers for callers without a HIP runtime binding ---- */
