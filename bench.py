@@ -13,10 +13,14 @@ cores on a bounded sample.  `c3` is the batched workload (R-MAT 2^23 / 2^27,
 Zipf weights, 10^7 PPR walkers with lengths in [5, 200], update batches of
 log-uniform size), replayed from a captured HIP graph; `c1` the small one.
 
-    python bench.py [--steps K] [--warmup W] [--config c1|c2|c3|c4] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--config c1|c2|c3|c4|c5] [--no-cpu]
 
 `c4` is the served form: a Poisson stream of walk queries and update
-batches through the serving loop, with p50/p99 latency.
+batches through the serving loop, with p50/p99 latency.  `c5` is the scale
+workload: 2^30 R-MAT arcs over 2^25 nodes (log-uniform u32 weights) in a
+store fitted to the graph at load, 3.2e7 DeepWalk walkers of length 80, and
+a sliding window of 10^7 inserts + 10^7 expiries of the oldest arcs per step;
+its CPU baseline runs on the first 2^26 arcs of the same stream.
 """
 from __future__ import annotations
 
@@ -46,6 +50,11 @@ CONFIGS = {
     "c3": dict(workload="rmat23_e27_zipf24_ppr015_len5to200_w1e7", scale=23, edges=1 << 27, wkind=3, app=2,
                max_len=200, batch=(1_000, 1_000_000), p=0.5, q=2.0, tp=0.15, walkers=10_000_000, lens=(5, 200),
                graph=True),
+    # 2^30 R-MAT arcs (one arc per generated edge), store fitted to the graph
+    # at load; sliding-window churn: 10^7 new arcs in, the 10^7 oldest out
+    "c5": dict(workload="rmat25_a30_loguni32_deepwalk_w32e6_churn1e7", scale=25, edges=1 << 30, wkind=2, app=0,
+               max_len=80, batch=10_000_000, p=0.5, q=2.0, tp=0.0125, walkers=32_000_000, lens=None, graph=False,
+               directed=True, fit=True, churn="expire", cpu_sample_edges=1 << 26),
 }
 
 
@@ -84,10 +93,19 @@ def batch_sizes(cfg, k, seed):
 
 def cpu_baseline(base, n, cfg, seed, budget_s=15.0):
     """The CPU oracle (a C/OpenMP restatement of the same sampler) on this
-    host's cores: full graph, a bounded sample of walkers."""
+    host's cores: full graph (or, where the configuration names a sample,
+    the first arcs of the same generator stream), a bounded sample of
+    walkers."""
     from oracle import oracle as orc
 
     threads = len(os.sched_getaffinity(0))
+    graph_note = "the same graph"
+    if cfg.get("cpu_sample_edges"):
+        k = cfg["cpu_sample_edges"]
+        base = orc.gen_rmat(cfg["scale"], k, cfg["wkind"], seed=seed, undirected=not cfg.get("directed"),
+                            nthreads=threads)
+        graph_note = (f"the first {len(base)} arcs of the same R-MAT stream and node space (the full graph's "
+                      f"CPU store would not fit host memory)")
     t0 = time.perf_counter()
     o = orc.OracleGraph(n, neighbor_index=cfg["app"] == 1)
     assert o.load(base) == 0
@@ -105,8 +123,8 @@ def cpu_baseline(base, n, cfg, seed, budget_s=15.0):
         nw = min(n, nw * 4)
     o.close()
     return {"value": steps / dt, "unit": "walk steps/s", "cores": threads, "kind": "port",
-            "sample": f"{nw} {APP_NAMES[cfg['app']]} walkers (max length {cfg['max_len']}) spread over the same "
-                      f"graph, {steps} steps in {dt:.2f}s on {threads} threads (load {load_s:.1f}s not counted)"}
+            "sample": f"{nw} {APP_NAMES[cfg['app']]} walkers (max length {cfg['max_len']}) spread over "
+                      f"{graph_note}, {steps} steps in {dt:.2f}s on {threads} threads (load {load_s:.1f}s not counted)"}
 
 
 def served(B, args):
@@ -273,21 +291,27 @@ def main():
     app = cfg["app"]
 
     t0 = time.perf_counter()
-    base = B.gen_rmat(scale, m, cfg["wkind"], seed=args.seed)
+    base = B.gen_rmat(scale, m, cfg["wkind"], seed=args.seed, undirected=not cfg.get("directed"))
     log(f"generated {len(base)} arcs in {time.perf_counter() - t0:.1f}s")
-    g = B.Graph(device=0, neighbor_index=app == B.NODE2VEC)
+    fit = dict(fit_load=True, fit_headroom=1.0 / 16, pool_slack=0.03) if cfg.get("fit") else {}
+    g = B.Graph(device=0, neighbor_index=app == B.NODE2VEC, **fit)
     d_base = B.DeviceArray(g, base)
     g.load_device(n, d_base, len(base))
     g.sync()
     load_ms = g.stats()["load_ms"]
     d_base.free()
-    log(f"loaded on GPU in {load_ms:.1f} ms")
+    narcs = len(base)
+    log(f"loaded on GPU in {load_ms:.1f} ms; store {g.stats()['bytes_reserved'] / 1e9:.1f} GB")
 
     sizes = batch_sizes(cfg, W + K, args.seed)
     ug = B.UpdateGen(n, scale, base, m, cfg["wkind"], seed=args.seed)
+    if cfg.get("cpu_sample_edges"):
+        del base  # the CPU baseline regenerates its own sample
+        base = None
     batches = []
     for kb in sizes:
-        ins, dels = ug.inserts(kb), ug.deletes(kb)
+        ins = ug.inserts(kb)
+        dels = ug.expire(kb) if cfg.get("churn") == "expire" else ug.deletes(kb)
         batches.append((B.DeviceArray(g, ins), B.DeviceArray(g, dels), kb))
     del ug
     log(f"staged {W + K} update batches, sizes {sizes}")
@@ -376,10 +400,12 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic",
-        "config": {"workload": cfg["workload"], "nodes": n, "arcs": int(len(base)), "walkers": nw,
+        "config": {"workload": cfg["workload"], "nodes": n, "arcs": int(narcs), "walkers": nw,
                    "walk_len": cfg["lens"] or cfg["max_len"], "app": APP_NAMES[app],
                    "update_batch": cfg["batch"], "p": cfg["p"], "q": cfg["q"], "tp": cfg["tp"],
-                   "hip_graph": cfg["graph"]},
+                   "hip_graph": cfg["graph"], "directed": bool(cfg.get("directed")),
+                   "deletes": "oldest arcs (sliding window)" if cfg.get("churn") == "expire" else "uniform live arcs",
+                   "store": "fitted at load" if cfg.get("fit") else "power-of-two regions"},
         "walk_steps_per_s": walk_rate,
         "updates_per_s": nupd / ((ins_ms + del_ms) / 1e3),
         "insert_ms": ins_ms / K,

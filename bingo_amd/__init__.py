@@ -44,6 +44,8 @@ class Config(C.Structure):
         ("hbm_budget", C.c_uint64),
         ("pool_slack", C.c_double),
         ("min_slots", C.c_int32),
+        ("fit_load", C.c_int32),
+        ("fit_headroom", C.c_double),
     ]
 
 
@@ -194,6 +196,7 @@ SIGNATURES = {
     "bingo_updgen_destroy": (None, [_vp]),
     "bingo_updgen_inserts": (_int, [_vp, _i64, _vp]),
     "bingo_updgen_deletes": (_int, [_vp, _i64, _vp]),
+    "bingo_updgen_expire": (_int, [_vp, _i64, _vp]),
     "bingo_updgen_live": (_i64, [_vp]),
 }
 
@@ -283,6 +286,12 @@ class UpdateGen:
     def deletes(self, k):
         out = np.zeros(k, DEL_DT)
         _check(lib().bingo_updgen_deletes(self._h, k, _ptr(out)))
+        return out
+
+    def expire(self, k):
+        """The k oldest live arcs (sliding-window deletes)."""
+        out = np.zeros(k, DEL_DT)
+        _check(lib().bingo_updgen_expire(self._h, k, _ptr(out)))
         return out
 
     def live(self):
@@ -404,11 +413,13 @@ class Server:
 class Graph:
     """One HBM-resident dynamic graph on one GPU (ids 1..n)."""
 
-    def __init__(self, device=0, neighbor_index=True, hbm_budget=0, pool_slack=0.25, min_slots=16):
+    def __init__(self, device=0, neighbor_index=True, hbm_budget=0, pool_slack=0.25, min_slots=16, fit_load=False,
+                 fit_headroom=1.0 / 16):
         cfg = Config()
         lib().bingo_config_default(C.byref(cfg))
         cfg.device, cfg.neighbor_index, cfg.hbm_budget = device, int(neighbor_index), hbm_budget
         cfg.pool_slack, cfg.min_slots = pool_slack, min_slots
+        cfg.fit_load, cfg.fit_headroom = int(fit_load), fit_headroom
         h = C.c_void_p()
         _check(lib().bingo_create(C.byref(cfg), C.byref(h)))
         self._h = h

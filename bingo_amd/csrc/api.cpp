@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <string>
 #include <vector>
 
@@ -44,13 +45,26 @@ static int ensure_scratch(bingo_graph *g, int idx, size_t bytes)
         g->sc[idx] = nullptr;
         g->sc_bytes[idx] = 0;
     }
-    size_t want = bytes + bytes / 4;
+    // room to spare for batches of varying size; a large buffer is taken exact
+    const size_t want = bytes < (256ull << 20) ? bytes + bytes / 4 : bytes;
     if (hipMalloc(&g->sc[idx], want) != hipSuccess) {
         (void)hipGetLastError();
         g->sc[idx] = nullptr;
         return fail(BINGO_ENOMEM, "scratch allocation of " + std::to_string(want) + " bytes failed");
     }
     g->sc_bytes[idx] = want;
+    return BINGO_OK;
+}
+
+static int release_scratch(bingo_graph *g, std::initializer_list<int> which)
+{
+    HIPCHK(hipStreamSynchronize(g->st));
+    for (int idx : which)
+        if (g->sc[idx]) {
+            HIPCHK(hipFree(g->sc[idx]));
+            g->sc[idx] = nullptr;
+            g->sc_bytes[idx] = 0;
+        }
     return BINGO_OK;
 }
 
@@ -155,18 +169,23 @@ static int bits_for(uint64_t maxval)
     return b;
 }
 
-// sort by node, encode runs; keys (u32 node ids, unsorted) are in SC_KEYS
-// and values in SC_VALS; outputs SC_KEYS2/SC_VALS2 and the Runs arrays.
+// Keys are node ids in 0..n (0 for a rejected entry), so a batch has at most
+// min(k, n + 1) runs.
+static int64_t max_runs(const bingo_graph *g, int64_t k) { return std::min<int64_t>(k, (int64_t)g->s.n + 1); }
+
+// run-length encode sorted node keys into the Runs arrays (node, start, count)
+
 static int encode_runs(bingo_graph *g, const uint32_t *keys_sorted, int64_t k, Runs *r)
 {
     int rc;
+    const int64_t kr = max_runs(g, k);
     size_t tb = run_length_encode_u32(nullptr, 0, keys_sorted, k, scp<uint32_t>(g, SC_UNIQ),
                                       scp<uint32_t>(g, SC_COUNTS), scp<uint32_t>(g, SC_NRUNS), g->st);
-    tb = std::max(tb, exclusive_scan_u32(nullptr, 0, scp<uint32_t>(g, SC_COUNTS), scp<uint32_t>(g, SC_STARTS), k, g->st));
+    tb = std::max(tb, exclusive_scan_u32(nullptr, 0, scp<uint32_t>(g, SC_COUNTS), scp<uint32_t>(g, SC_STARTS), kr, g->st));
     if ((rc = ensure_scratch(g, SC_TMP, tb))) return rc;
     run_length_encode_u32(g->sc[SC_TMP], g->sc_bytes[SC_TMP], keys_sorted, k, scp<uint32_t>(g, SC_UNIQ),
                           scp<uint32_t>(g, SC_COUNTS), scp<uint32_t>(g, SC_NRUNS), g->st);
-    exclusive_scan_u32(g->sc[SC_TMP], g->sc_bytes[SC_TMP], scp<uint32_t>(g, SC_COUNTS), scp<uint32_t>(g, SC_STARTS), k,
+    exclusive_scan_u32(g->sc[SC_TMP], g->sc_bytes[SC_TMP], scp<uint32_t>(g, SC_COUNTS), scp<uint32_t>(g, SC_STARTS), kr,
                        g->st);
     HIPCHK(hipGetLastError());
     r->node = scp<uint32_t>(g, SC_UNIQ);
@@ -176,17 +195,22 @@ static int encode_runs(bingo_graph *g, const uint32_t *keys_sorted, int64_t k, R
     return BINGO_OK;
 }
 
-static int ensure_batch_scratch(bingo_graph *g, int64_t k)
+// scratch of one insert (del false) or delete batch of k entries
+static int ensure_batch_scratch(bingo_graph *g, int64_t k, bool del)
 {
     int rc;
-    const size_t k4 = (size_t)k * 4, k8 = (size_t)k * 8;
-    if ((rc = ensure_scratch(g, SC_KEYS, k4)) || (rc = ensure_scratch(g, SC_KEYS2, k4)) ||
-        (rc = ensure_scratch(g, SC_VALS, k4)) || (rc = ensure_scratch(g, SC_VALS2, k4)) ||
-        (rc = ensure_scratch(g, SC_UNIQ, k4)) || (rc = ensure_scratch(g, SC_COUNTS, k4)) ||
-        (rc = ensure_scratch(g, SC_STARTS, k4)) || (rc = ensure_scratch(g, SC_NRUNS, 16)) ||
-        (rc = ensure_scratch(g, SC_U64A, k8)) || (rc = ensure_scratch(g, SC_U64B, k8)) ||
-        (rc = ensure_scratch(g, SC_EDGES, (size_t)k * sizeof(DevEdge))))
+    const size_t k4 = (size_t)k * 4, k8 = (size_t)k * 8, r4 = (size_t)max_runs(g, k) * 4;
+    if ((rc = ensure_scratch(g, SC_KEYS2, k4)) || (rc = ensure_scratch(g, SC_VALS2, k4)) ||
+        (rc = ensure_scratch(g, SC_UNIQ, r4)) || (rc = ensure_scratch(g, SC_COUNTS, r4)) ||
+        (rc = ensure_scratch(g, SC_STARTS, r4)) || (rc = ensure_scratch(g, SC_NRUNS, 16)))
         return rc;
+    if (del) {
+        if ((rc = ensure_scratch(g, SC_U64A, k8)) || (rc = ensure_scratch(g, SC_U64B, k8))) return rc;
+    } else {
+        if ((rc = ensure_scratch(g, SC_KEYS, k4)) || (rc = ensure_scratch(g, SC_VALS, k4)) ||
+            (rc = ensure_scratch(g, SC_EDGES, (size_t)k * sizeof(DevEdge))))
+            return rc;
+    }
     return BINGO_OK;
 }
 
@@ -195,18 +219,19 @@ static int do_insert(bingo_graph *g, const DevEdge *d_e, int64_t k)
     int rc;
     if (k < 0 || k >= (1ll << 31)) return fail(BINGO_EINVAL, "batch size out of range");
     if (k == 0) return BINGO_OK;
-    if ((rc = ensure_batch_scratch(g, k))) return rc;
+    if ((rc = ensure_batch_scratch(g, k, false))) return rc;
     launch_reset_ctl(g->s.ctl, g->st);
     launch_insert_keys(d_e, k, g->s.n, scp<uint32_t>(g, SC_KEYS), scp<uint32_t>(g, SC_VALS), g->s.ctl, g->st);
     const int bits = bits_for((uint64_t)g->s.n);
-    size_t tb = sort_pairs_u32(nullptr, 0, scp<uint32_t>(g, SC_KEYS), scp<uint32_t>(g, SC_KEYS2),
-                               scp<uint32_t>(g, SC_VALS), scp<uint32_t>(g, SC_VALS2), k, bits, g->st);
+    uint32_t *k0 = scp<uint32_t>(g, SC_KEYS), *k1 = scp<uint32_t>(g, SC_KEYS2);
+    uint32_t *v0 = scp<uint32_t>(g, SC_VALS), *v1 = scp<uint32_t>(g, SC_VALS2);
+    size_t tb = sort_pairs_u32(nullptr, 0, k0, k1, v0, v1, k, bits, g->st, nullptr);
     if ((rc = ensure_scratch(g, SC_TMP, tb))) return rc;
-    sort_pairs_u32(g->sc[SC_TMP], g->sc_bytes[SC_TMP], scp<uint32_t>(g, SC_KEYS), scp<uint32_t>(g, SC_KEYS2),
-                   scp<uint32_t>(g, SC_VALS), scp<uint32_t>(g, SC_VALS2), k, bits, g->st);
-    launch_gather_edges(d_e, scp<uint32_t>(g, SC_VALS2), k, scp<DevEdge>(g, SC_EDGES), g->st);
+    bool second = false;
+    sort_pairs_u32(g->sc[SC_TMP], g->sc_bytes[SC_TMP], k0, k1, v0, v1, k, bits, g->st, &second);
+    launch_gather_edges(d_e, second ? v1 : v0, k, scp<DevEdge>(g, SC_EDGES), g->st);
     Runs r;
-    if ((rc = encode_runs(g, scp<uint32_t>(g, SC_KEYS2), k, &r))) return rc;
+    if ((rc = encode_runs(g, second ? k1 : k0, k, &r))) return rc;
     launch_insert_plan(g->s, scp<DevEdge>(g, SC_EDGES), r, k, g->st);
     HIPCHK(hipGetLastError());
     if ((rc = read_ctl(g))) return rc;
@@ -225,7 +250,7 @@ static int do_delete(bingo_graph *g, const int32_t *d_pairs, int64_t k)
     int rc;
     if (k < 0 || k >= (1ll << 31)) return fail(BINGO_EINVAL, "batch size out of range");
     if (k == 0) return BINGO_OK;
-    if ((rc = ensure_batch_scratch(g, k))) return rc;
+    if ((rc = ensure_batch_scratch(g, k, true))) return rc;
     launch_reset_ctl(g->s.ctl, g->st);
     unsigned long long *ka = scp<unsigned long long>(g, SC_U64A), *kb = scp<unsigned long long>(g, SC_U64B);
     launch_delete_keys(d_pairs, k, ka, g->st);
@@ -339,6 +364,8 @@ void bingo_config_default(bingo_config *cfg)
     cfg->hbm_budget = 0;
     cfg->pool_slack = 0.25;
     cfg->min_slots = 16;  // ref ORIGINAL (include/api.h:13)
+    cfg->fit_load = 0;
+    cfg->fit_headroom = 1.0 / 16;
 }
 
 void bingo_walk_params_default(bingo_walk_params *p)
@@ -372,6 +399,10 @@ int bingo_create(const bingo_config *cfg, bingo_graph **out)
         return fail(BINGO_EINVAL, "min_slots must be a power of two in [1, 2^20]");
     }
     if (!(g->cfg.pool_slack >= 0) || g->cfg.pool_slack > 16) g->cfg.pool_slack = 0.25;
+    if (g->cfg.fit_load && !(g->cfg.fit_headroom >= 0 && g->cfg.fit_headroom <= 1)) {
+        delete g;
+        return fail(BINGO_EINVAL, "fit_headroom must be in [0, 1]");
+    }
     g->device = g->cfg.device;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
@@ -451,13 +482,16 @@ int bingo_load_device(bingo_graph *g, int32_t n, const bingo_edge *d_edges, int6
     launch_init_nodes(g->s, g->st);
     HIPCHK(hipGetLastError());
     g->loaded = true;
+    if (g->cfg.fit_load) g->s.fit = 1u + (uint32_t)std::llround(g->cfg.fit_headroom * 65536.0);
     int rc = do_insert(g, reinterpret_cast<const DevEdge *>(d_edges), m);
+    g->s.fit = 0;  // later inserts grow regions by doubling
     phase_end(g, PH_LOAD);
     if (rc) {
         g->loaded = false;  // leave a handle that can be destroyed, not reused
         return rc;
     }
-    return BINGO_OK;
+    // a load's sort buffers are sized for the whole graph: give them back
+    return release_scratch(g, {SC_KEYS, SC_KEYS2, SC_VALS, SC_VALS2, SC_EDGES, SC_UNIQ, SC_COUNTS, SC_STARTS, SC_TMP});
 }
 
 int bingo_load(bingo_graph *g, int32_t n, const bingo_edge *edges, int64_t m)
@@ -469,8 +503,7 @@ int bingo_load(bingo_graph *g, int32_t n, const bingo_edge *edges, int64_t m)
     if (m) HIPCHK(hipMemcpyAsync(g->sc[SC_STAGE], edges, (size_t)m * sizeof(bingo_edge), hipMemcpyHostToDevice, g->st));
     rc = bingo_load_device(g, n, static_cast<const bingo_edge *>(g->sc[SC_STAGE]), m);
     if (rc) return rc;
-    HIPCHK(hipStreamSynchronize(g->st));
-    return BINGO_OK;
+    return release_scratch(g, {SC_STAGE});
 }
 
 int bingo_insert_device(bingo_graph *g, const bingo_edge *d_edges, int64_t k)

@@ -106,7 +106,9 @@ struct bingo_updgen {
     int32_t n;
     uint64_t next_edge, draws;
     std::vector<uint32_t> deg;
-    std::vector<uint64_t> live;  // (u << 32) | slot
+    std::vector<uint64_t> live;  // (u << 32) | slot; live[head..] in arrival order until a random delete
+    size_t head = 0;             // arcs expired so far (sliding window)
+    int mode = 0;                // 0 no deletes yet, 1 random deletes, 2 expiry
 };
 
 extern "C" {
@@ -178,7 +180,8 @@ int bingo_updgen_inserts(bingo_updgen *s, int64_t k, bingo_edge *out)
 // retries dead positions; swap-remove gives the same law).
 int bingo_updgen_deletes(bingo_updgen *s, int64_t k, bingo_del *out)
 {
-    if (!s || k < 0 || (k && !out) || k > (int64_t)s->live.size()) return BINGO_EINVAL;
+    if (!s || k < 0 || (k && !out) || k > (int64_t)s->live.size() || s->mode == 2) return BINGO_EINVAL;
+    if (k) s->mode = 1;
     for (int64_t i = 0; i < k; ++i) {
         uint32_t x[4];
         Philox::block(s->sp.seed, (uint32_t)s->draws, (uint32_t)(s->draws >> 32), 0, kTagDel, x);
@@ -194,6 +197,19 @@ int bingo_updgen_deletes(bingo_updgen *s, int64_t k, bingo_del *out)
     return BINGO_OK;
 }
 
-int64_t bingo_updgen_live(const bingo_updgen *s) { return s ? (int64_t)s->live.size() : 0; }
+// Expiry: the k oldest live arcs in arrival order (load order, then inserts),
+// the deletes of a sliding window.  Not mixed with random deletes.
+int bingo_updgen_expire(bingo_updgen *s, int64_t k, bingo_del *out)
+{
+    if (!s || k < 0 || (k && !out) || s->mode == 1 || k > (int64_t)(s->live.size() - s->head)) return BINGO_EINVAL;
+    if (k) s->mode = 2;
+    for (int64_t i = 0; i < k; ++i) {
+        const uint64_t pick = s->live[s->head++];
+        out[i] = bingo_del{(int32_t)(pick >> 32), (int32_t)(uint32_t)pick};
+    }
+    return BINGO_OK;
+}
+
+int64_t bingo_updgen_live(const bingo_updgen *s) { return s ? (int64_t)(s->live.size() - s->head) : 0; }
 
 }  // extern "C"

@@ -74,6 +74,22 @@ __device__ __forceinline__ uint64_t wave_incl64(uint64_t v)
     return v;
 }
 
+// dst[0, n) = src[0, n) by one wave, kCU loads in flight per lane
+template <typename T>
+__device__ __forceinline__ void wave_copy(T *__restrict__ dst, const T *__restrict__ src, uint32_t n, int ln)
+{
+    constexpr uint32_t kCU = 8;
+    uint32_t i = (uint32_t)ln;
+    for (; i + (kCU - 1) * 64 < n; i += kCU * 64) {
+        T x[kCU];
+#pragma unroll
+        for (uint32_t q = 0; q < kCU; ++q) x[q] = src[i + q * 64];
+#pragma unroll
+        for (uint32_t q = 0; q < kCU; ++q) dst[i + q * 64] = x[q];
+    }
+    for (; i < n; i += 64) dst[i] = src[i];
+}
+
 __device__ __forceinline__ uint32_t pow2_at_least(uint32_t x)
 {
     return x <= 1u ? 1u : (1u << (32 - __clz(x - 1u)));
@@ -110,17 +126,20 @@ __device__ __forceinline__ uint32_t hslot(uint32_t key, uint32_t cap)
     return (uint32_t)(((uint64_t)(key * 0x9E3779B1u) * cap) >> 32);
 }
 
-// add `cnt` to key's count; returns 1 if the key was new
+// add `cnt` to key's count; returns 1 if the key was new.  A new key is one
+// 64-bit CAS (key and count together); a present key adds to the count half.
 __device__ uint32_t hash_add(uint64_t *tab, uint32_t cap, uint32_t key, uint32_t cnt)
 {
     uint32_t i = hslot(key, cap);
     const uint32_t mask = cap - 1u;
+    const unsigned long long add = (unsigned long long)cnt << 32;
     for (;;) {
-        uint32_t *kp = reinterpret_cast<uint32_t *>(tab + i);
-        uint32_t old = atomicCAS(kp, 0u, key);
-        if (old == 0u || old == key) {
-            atomicAdd(kp + 1, cnt);
-            return old == 0u;
+        unsigned long long *p = reinterpret_cast<unsigned long long *>(tab + i);
+        const unsigned long long old = atomicCAS(p, 0ull, add | key);
+        if (old == 0ull) return 1u;
+        if ((uint32_t)old == key) {
+            atomicAdd(p, add);
+            return 0u;
         }
         i = (i + 1u) & mask;
     }
@@ -301,6 +320,13 @@ struct Growth {
     bool g_g;
 };
 
+// capacity of a region that has to grow to hold `need` elements (DevStore::fit)
+__device__ __forceinline__ uint32_t grow_cap(const DevStore &s, uint32_t need, uint32_t least)
+{
+    if (s.fit) return need + (uint32_t)(((uint64_t)need * (s.fit - 1u) + 0xFFFFu) >> 16);
+    return pow2_at_least(max(need, least));
+}
+
 // counts of the run and the growth decisions; identical in plan and apply
 __device__ Growth plan_run(const DevStore &s, const DevEdge *__restrict__ e, uint32_t cnt, const NodeHdr &h,
                            const GroupHdr &gh, int ln)
@@ -324,13 +350,13 @@ __device__ Growth plan_run(const DevStore &s, const DevEdge *__restrict__ e, uin
     G.pcsum = pcsum;
     G.new_n = h.nslots + cnt;
     G.g_slots = G.new_n > h.slot_cap;
-    G.cap_slots = G.g_slots ? pow2_at_least(max(G.new_n, (uint32_t)s.min_slots)) : h.slot_cap;
+    G.cap_slots = G.g_slots ? grow_cap(s, G.new_n, (uint32_t)s.min_slots) : h.slot_cap;
     G.new_back = h.back_used + pcsum;
     G.g_back = G.new_back > h.back_cap;
-    G.cap_back = G.g_back ? pow2_at_least(max(G.new_back, 16u)) : h.back_cap;
+    G.cap_back = G.g_back ? grow_cap(s, G.new_back, 16u) : h.back_cap;
     const uint32_t need_g = gh.size + addj;
     G.g_g = ln < kGroups && addj && need_g > gh.cap;
-    G.cap_g = G.g_g ? pow2_at_least(max(need_g, 4u)) : gh.cap;
+    G.cap_g = G.g_g ? grow_cap(s, need_g, 4u) : gh.cap;
     const uint32_t hneed = 2u * (h.hash_used + cnt);
     G.g_hash = s.nbr && hneed > h.hash_cap;
     G.cap_hash = G.g_hash ? pow2_at_least(max(hneed, 16u)) : h.hash_cap;
@@ -418,25 +444,16 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, uin
     uint32_t *nb = nv + 2ull * G.cap_slots;
     if (G.g_slots && n0) {
         const uint32_t *ov = s.slot + h.slot_off;
-        const uint32_t *ow = ov + h.slot_cap;
-        const uint32_t *ob = ov + 2ull * h.slot_cap;
-        for (uint32_t i = ln; i < n0; i += 64) {
-            nv[i] = ov[i];
-            nw[i] = ow[i];
-            nb[i] = ob[i];
-        }
+        wave_copy(nv, ov, n0, ln);
+        wave_copy(nw, ov + h.slot_cap, n0, ln);
+        wave_copy(nb, ov + 2ull * h.slot_cap, n0, ln);
     }
     uint32_t *back = s.back + a_back;
-    if (G.g_back && h.back_used) {
-        const uint32_t *ob = s.back + h.back_off;
-        for (uint32_t i = ln; i < h.back_used; i += 64) back[i] = ob[i];
-    }
+    if (G.g_back && h.back_used) wave_copy(back, s.back + h.back_off, h.back_used, ln);
     for (uint32_t gm = (uint32_t)__ballot(G.g_g && gh.size); gm;) {
         const int j = __ffs(gm) - 1;
         gm &= gm - 1u;
-        const uint64_t from = rl64(gh.off, j), to = rl64(goff, j);
-        const uint32_t sz = rl32(gh.size, j);
-        for (uint32_t i = ln; i < sz; i += 64) s.mem[to + i] = s.mem[from + i];
+        wave_copy(s.mem + rl64(goff, j), s.mem + rl64(gh.off, j), rl32(gh.size, j), ln);
     }
     uint32_t hused = h.hash_used;
     uint64_t *htab = s.hash + a_hash;
@@ -562,11 +579,22 @@ __global__ void k_delete_check(DevStore s, const unsigned long long *__restrict_
     slots[i] = sl;
 }
 
-// Batched delete of one node (slots ascending, distinct, live).  Lane j owns
-// bit group j: deleted members are marked in place, then each hole below the
-// new size (holes in slot order) takes the next unmarked member from the
-// tail, and the moved member's back index is rewritten.  This is the batch
-// form of NodeData::deleteE's swap-with-last (include/Bingo.cuh:199-218).
+// Batched delete of one node (slots ascending, distinct, live).  Per bit
+// group j, with k_j deleted members and new size S2 = size - k_j: the deleted
+// members below S2 are holes, and the k-th hole (holes in slot order) takes
+// the k-th surviving member of the tail [S2, size) in position order; the
+// moved member's back index is rewritten.  This is the batch form of
+// NodeData::deleteE's swap-with-last (include/Bingo.cuh:199-218).
+//
+// Work split: the deletes are staged 64 at a time, lane t holding delete t's
+// slot, weight and back offset; lane j < 32 owns group j and reads a staged
+// delete with readlane.  A group lane issues its back-index gathers, tail
+// reads and moved-member gathers kDB at a time, so a hub's thousands of
+// deletes cost a few dependent round trips per kDB deletes rather than a
+// chain per delete.  The tail is compacted in place first (survivors to
+// [S2, S2 + holes)), which lets the k-th hole read its survivor at S2 + k.
+constexpr int kDB = 8;
+
 __device__ void delete_run(const DevStore &s, const uint32_t *__restrict__ slots, uint32_t cnt, int32_t u, int ln)
 {
     const NodeHdr h = s.hdr[u];
@@ -574,73 +602,108 @@ __device__ void delete_run(const DevStore &s, const uint32_t *__restrict__ slots
     uint32_t *W = V + h.slot_cap;
     const uint32_t *B = V + 2ull * h.slot_cap;
     uint32_t *back = s.back + h.back_off;
+    const bool own = ln < kGroups;
+    const uint32_t bit = own ? 1u << ln : 0u, below = bit - 1u;
     GroupHdr gh = {0, 0, 0};
-    if (ln < kGroups) {
-        gh = s.grp[(size_t)u * kGroups + ln];
-        const uint32_t bit = 1u << ln, below = bit - 1u;
-        uint64_t *m = s.mem + gh.off;
-        // The deletes of a node are walked kPF at a time: their slot, weight,
-        // back-offset and member-position gathers are all issued before any is
-        // used, so a hub's thousands of deletes are not one dependent chain.
-        constexpr int kPF = 8;
-        uint32_t sl[kPF], w[kPF], pos[kPF];
-        auto gather = [&](uint32_t t0) {
+    if (own) gh = s.grp[(size_t)u * kGroups + ln];
+    uint64_t *m = s.mem + gh.off;
+    const bool hashed = s.nbr && h.hash_cap;
+    const bool single = cnt <= 64u;  // one staged chunk serves both passes
+
+    uint32_t csl = 0, cw = 0, cbo = 0, cv = 0;
+    auto stage = [&](uint32_t c0) {
+        const uint32_t t = c0 + (uint32_t)ln;
+        const bool in = t < cnt;
+        csl = in ? slots[t] : 0u;
+        cw = in ? W[csl] : 0u;  // 0 past the end: such a delete holds no group
+        cbo = in ? B[csl] : 0u;
+        cv = in && hashed ? V[csl] : 0u;
+    };
+    // back-index positions in this lane's group of staged deletes q0..q0+kDB-1
+    auto positions = [&](uint32_t q0, uint32_t(&pos)[kDB]) {
 #pragma unroll
-            for (int q = 0; q < kPF; ++q) sl[q] = t0 + q < cnt ? slots[t0 + q] : 0u;
-            uint32_t bo[kPF];
+        for (int q = 0; q < kDB; ++q) {
+            const uint32_t w = rl32(cw, (int)q0 + q), bo = rl32(cbo, (int)q0 + q);
+            pos[q] = (w & bit) ? back[bo + __popc(w & below)] : 0u;
+        }
+    };
+
+    // pass 1: count this group's deletes and mark their members
+    uint32_t kj = 0;
+    uint32_t keep[kDB];  // positions of a run of at most kDB deletes, reused by pass 2
+    for (uint32_t c0 = 0; c0 < cnt; c0 += 64) {
+        stage(c0);
+        const uint32_t nq = min(64u, cnt - c0);
+        for (uint32_t q0 = 0; q0 < nq; q0 += kDB) {
+            uint32_t pos[kDB];
+            positions(q0, pos);
 #pragma unroll
-            for (int q = 0; q < kPF; ++q) {
-                w[q] = t0 + q < cnt ? W[sl[q]] : 0u;
-                bo[q] = t0 + q < cnt ? B[sl[q]] : 0u;
-            }
-#pragma unroll
-            for (int q = 0; q < kPF; ++q) pos[q] = (w[q] & bit) ? back[bo[q] + __popc(w[q] & below)] : 0u;
-        };
-        // pass 1: count this group's deletions and mark their members
-        uint32_t kj = 0;
-        for (uint32_t t0 = 0; t0 < cnt; t0 += kPF) {
-            gather(t0);
-#pragma unroll
-            for (int q = 0; q < kPF; ++q)
-                if (w[q] & bit) {
+            for (int q = 0; q < kDB; ++q) {
+                keep[q] = pos[q];
+                if (rl32(cw, (int)q0 + q) & bit) {
                     ++kj;
-                    reinterpret_cast<uint32_t *>(m + pos[q])[0] = sl[q] | kMark;
-                }
-        }
-        // pass 2: holes below the new size, in slot order, take the tail's
-        // unmarked members in position order
-        if (kj) {
-            const uint32_t S2 = gh.size - kj;
-            uint32_t tail = S2;
-            bool bad = false;
-            for (uint32_t t0 = 0; t0 < cnt && !bad; t0 += kPF) {
-                gather(t0);
-#pragma unroll
-                for (int q = 0; q < kPF; ++q) {
-                    if (bad || !(w[q] & bit) || pos[q] >= S2) continue;
-                    uint64_t mv = kMark;
-                    while (tail < gh.size && ((uint32_t)(mv = m[tail++]) & kMark)) {
-                    }
-                    if ((uint32_t)mv & kMark) {  // no survivor left: the store is inconsistent
-                        bad = true;
-                        continue;
-                    }
-                    m[pos[q]] = mv;
-                    const uint32_t ms = (uint32_t)mv;
-                    back[B[ms] + __popc(W[ms] & below)] = pos[q];
+                    reinterpret_cast<uint32_t *>(m + pos[q])[0] = rl32(csl, (int)q0 + q) | kMark;
                 }
             }
-            if (bad) atomicOr(&s.ctl->error, (unsigned)ERR_OVERRUN);
-            gh.size = S2;
-            s.grp[(size_t)u * kGroups + ln].size = S2;
         }
     }
-    if (s.nbr && h.hash_cap) {
-        uint64_t *htab = s.hash + h.hash_off;
-        for (uint32_t t = ln; t < cnt; t += 64) hash_dec(htab, h.hash_cap, V[slots[t]]);
+
+    // compact the tail's survivors to [S2, S2 + nsurv), in position order
+    const uint32_t S = gh.size, S2 = S - kj;
+    uint32_t nsurv = 0;
+    for (uint32_t p0 = S2; p0 < S; p0 += kDB) {
+        uint64_t e[kDB];
+#pragma unroll
+        for (int q = 0; q < kDB; ++q) e[q] = p0 + q < S ? m[p0 + q] : (uint64_t)kMark;
+#pragma unroll
+        for (int q = 0; q < kDB; ++q)
+            if (!((uint32_t)e[q] & kMark)) m[S2 + nsurv++] = e[q];
     }
-    for (uint32_t t = ln; t < cnt; t += 64) W[slots[t]] = 0u;
-    alias_rebuild(s, u, ln < kGroups ? gh.size : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, ln);
+
+    // pass 2: fill holes; then the slots' neighbour counts and tombstones
+    uint32_t hr = 0;
+    bool bad = false;
+    for (uint32_t c0 = 0; c0 < cnt; c0 += 64) {
+        if (!single) stage(c0);
+        const uint32_t nq = min(64u, cnt - c0);
+        for (uint32_t q0 = 0; q0 < nq; q0 += kDB) {
+            uint32_t pos[kDB];
+            if (cnt <= (uint32_t)kDB) {
+#pragma unroll
+                for (int q = 0; q < kDB; ++q) pos[q] = keep[q];
+            } else {
+                positions(q0, pos);
+            }
+            uint32_t rk[kDB];
+            uint64_t mv[kDB];
+#pragma unroll
+            for (int q = 0; q < kDB; ++q) {
+                const bool hole = (rl32(cw, (int)q0 + q) & bit) && pos[q] < S2;
+                rk[q] = hole ? hr : ~0u;
+                hr += hole;
+                bad |= hole && rk[q] >= nsurv;  // no survivor left: the store is inconsistent
+                mv[q] = rk[q] < nsurv ? m[S2 + rk[q]] : 0ull;
+            }
+            uint32_t mb[kDB], mw[kDB];
+#pragma unroll
+            for (int q = 0; q < kDB; ++q) {
+                const uint32_t ms = (uint32_t)mv[q];
+                mb[q] = rk[q] < nsurv ? B[ms] : 0u;
+                mw[q] = rk[q] < nsurv ? W[ms] : 0u;
+                if (rk[q] < nsurv) m[pos[q]] = mv[q];
+            }
+#pragma unroll
+            for (int q = 0; q < kDB; ++q)
+                if (rk[q] < nsurv) back[mb[q] + __popc(mw[q] & below)] = pos[q];
+        }
+        if (c0 + (uint32_t)ln < cnt) {
+            if (hashed) hash_dec(s.hash + h.hash_off, h.hash_cap, cv);
+            W[csl] = 0u;
+        }
+    }
+    if (bad) atomicOr(&s.ctl->error, (unsigned)ERR_OVERRUN);
+    if (kj) s.grp[(size_t)u * kGroups + ln].size = S2;
+    alias_rebuild(s, u, own ? S2 : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, ln);
 }
 
 __global__ __launch_bounds__(256) void k_delete_apply(DevStore s, const uint32_t *__restrict__ slots, Runs r)
@@ -1346,11 +1409,13 @@ void launch_edge_count(DevStore s, const int32_t *us, const int32_t *vs, int64_t
 // ------------------------------------------------------------------------
 // rocPRIM wrappers
 // ------------------------------------------------------------------------
-size_t sort_pairs_u32(void *tmp, size_t tmp_bytes, const uint32_t *kin, uint32_t *kout, const uint32_t *vin,
-                      uint32_t *vout, int64_t k, int bits, hipStream_t st)
+size_t sort_pairs_u32(void *tmp, size_t tmp_bytes, uint32_t *k0, uint32_t *k1, uint32_t *v0, uint32_t *v1, int64_t k,
+                      int bits, hipStream_t st, bool *second)
 {
     size_t b = tmp_bytes;
-    (void)rocprim::radix_sort_pairs(tmp, b, kin, kout, vin, vout, (size_t)k, 0, (unsigned)bits, st);
+    rocprim::double_buffer<uint32_t> keys(k0, k1), vals(v0, v1);
+    (void)rocprim::radix_sort_pairs(tmp, b, keys, vals, (size_t)k, 0, (unsigned)bits, st);
+    if (second) *second = keys.current() == k1;
     return b;
 }
 

@@ -84,7 +84,11 @@ struct DevStore {
     int32_t n;
     int32_t nbr;
     int32_t min_slots;
-    int32_t pad;
+    // Growth rule of the call in flight.  0: a region that must grow gets the
+    // next power of two (slots: at least min_slots).  Otherwise regions are
+    // sized to their contents plus ceil(size * fit / 2^16) (a load that fits
+    // the store to the graph); neighbour tables stay powers of two.
+    uint32_t fit;
     NodeHdr *hdr;
     GroupHdr *grp;
     AliasEnt *alias;
@@ -167,8 +171,10 @@ void launch_edge_count(DevStore s, const int32_t *us, const int32_t *vs, int64_t
 void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st);
 
 // rocPRIM wrappers (temp storage sized by a null call)
-size_t sort_pairs_u32(void *tmp, size_t tmp_bytes, const uint32_t *kin, uint32_t *kout, const uint32_t *vin,
-                      uint32_t *vout, int64_t k, int bits, hipStream_t st);
+// Pairs sort over double buffers: input in (k0, v0), (k1, v1) is the other
+// half; *second says whether the sorted output ended in (k1, v1).
+size_t sort_pairs_u32(void *tmp, size_t tmp_bytes, uint32_t *k0, uint32_t *k1, uint32_t *v0, uint32_t *v1, int64_t k,
+                      int bits, hipStream_t st, bool *second);
 size_t sort_keys_u64(void *tmp, size_t tmp_bytes, const unsigned long long *kin, unsigned long long *kout, int64_t k,
                      int bits, hipStream_t st);
 size_t run_length_encode_u32(void *tmp, size_t tmp_bytes, const uint32_t *keys, int64_t k, uint32_t *uniq,

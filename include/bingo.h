@@ -54,6 +54,9 @@ typedef struct {
     uint64_t hbm_budget;    /* bytes of HBM the store may use; 0 = 90% of free memory */
     double pool_slack;      /* extra pool room reserved at load, as a fraction (0.25) */
     int32_t min_slots;      /* minimum slot capacity per node (ref ORIGINAL = 16) */
+    int32_t fit_load;       /* 1: load sizes every region to its contents plus fit_headroom
+                               instead of the next power of two (at least min_slots) */
+    double fit_headroom;    /* fit_load's spare room per region, as a fraction (1/16) */
 } bingo_config;
 
 typedef struct {
@@ -259,6 +262,9 @@ int bingo_updgen_create(int32_t n, int scale, const bingo_edge *base, int64_t m,
 void bingo_updgen_destroy(bingo_updgen *s);
 int bingo_updgen_inserts(bingo_updgen *s, int64_t k, bingo_edge *out);
 int bingo_updgen_deletes(bingo_updgen *s, int64_t k, bingo_del *out);
+/* sliding window: the k oldest live arcs (load order, then inserts); a
+ * generator uses either this or bingo_updgen_deletes */
+int bingo_updgen_expire(bingo_updgen *s, int64_t k, bingo_del *out);
 int64_t bingo_updgen_live(const bingo_updgen *s);
 
 #ifdef __cplusplus

@@ -53,6 +53,28 @@ def test_update_generator_matches_oracle(orc):
         assert ga.live() == gb.live()
 
 
+def test_update_generator_expiry_is_arrival_order():
+    """Sliding-window deletes name the oldest live arcs: load order, then the
+    inserts, each by its slot ordinal within its node."""
+    n, m = 1 << 8, 1000
+    base = B.gen_rmat(8, m, B.W_LOGUNI32, seed=3, undirected=False)
+    ug = B.UpdateGen(n, 8, base, m, B.W_LOGUNI32, seed=3)
+    ins = ug.inserts(50)
+    d = np.concatenate([ug.expire(600), ug.expire(420)])
+    arcs = np.concatenate([base, ins])
+    slot = np.zeros(len(arcs), np.int64)
+    seen = np.zeros(n + 1, np.int64)
+    for i, u in enumerate(arcs["u"]):
+        slot[i] = seen[u]
+        seen[u] += 1
+    assert (d["u"] == arcs["u"][:1020]).all() and (d["slot"] == slot[:1020]).all()
+    assert ug.live() == 30
+    with pytest.raises(B.BingoError):
+        ug.deletes(1)  # random deletes do not mix with a sliding window
+    with pytest.raises(B.BingoError):
+        ug.expire(31)
+
+
 def test_generator_rejects_bad_arguments():
     out = np.zeros(4, B.EDGE_DT)
     assert B.lib().bingo_gen_rmat(0, 0, 2, 0.57, 0.19, 0.19, 1, 1, 1, out.ctypes.data) == -1

@@ -14,9 +14,9 @@ pytestmark = pytest.mark.gpu
 APPS = [B.DEEPWALK, B.NODE2VEC, B.PPR, B.SAMPLING]
 
 
-def _pair(orc, n, edges, nbr=True):
-    print(f"[parity] load n={n} m={len(edges)} nbr={nbr}", flush=True)
-    g = B.Graph(neighbor_index=nbr)
+def _pair(orc, n, edges, nbr=True, fit=False):
+    print(f"[parity] load n={n} m={len(edges)} nbr={nbr} fit={fit}", flush=True)
+    g = B.Graph(neighbor_index=nbr, fit_load=fit)
     g.load(n, edges)
     o = orc.OracleGraph(n, nbr)
     assert o.load(edges) == 0
@@ -72,12 +72,12 @@ def test_load_and_walks_small(orc):
         _same_walks(g, o, orc, app, seed=5, max_len=17, nwalkers=n)
 
 
-@pytest.mark.parametrize("nbr", [True, False])
-def test_update_rounds_match_oracle(orc, nbr):
+@pytest.mark.parametrize("nbr,fit", [(True, False), (False, False), (True, True)])
+def test_update_rounds_match_oracle(orc, nbr, fit):
     scale = 10
     n = 1 << scale
     base = B.gen_rmat(scale, 6000, B.W_LOGUNI32, seed=3, undirected=False)
-    g, o = _pair(orc, n, base, nbr)
+    g, o = _pair(orc, n, base, nbr, fit)
     ug = B.UpdateGen(n, scale, base, 6000, B.W_LOGUNI32, seed=3)
     bg = BruteGraph(n)
     bg.insert(base)
@@ -94,6 +94,32 @@ def test_update_rounds_match_oracle(orc, nbr):
         app = [B.DEEPWALK, B.PPR, B.SAMPLING][rnd % 3] if not nbr else APPS[rnd % 4]
         _same_walks(g, o, orc, app, seed=100 + rnd, max_len=30, nwalkers=2 * n)
     check_against_brute(g.export(), bg)
+
+
+def test_fitted_store_under_sliding_window(orc):
+    """A store fitted to the graph at load (regions sized to their contents
+    plus 1/16) holds the same state as the oracle, is smaller than the
+    power-of-two store of the same graph, and stays exact while a sliding
+    window inserts new arcs and expires the oldest ones (regions outgrow
+    their headroom and move)."""
+    scale, m = 12, 40000
+    n = 1 << scale
+    base = B.gen_rmat(scale, m, B.W_LOGUNI32, seed=8, undirected=False)
+    g, o = _pair(orc, n, base, nbr=False, fit=True)
+    _same_state(g, o, "fitted load")
+    with B.Graph(neighbor_index=False) as gp:
+        gp.load(n, base)
+        assert g.stats()["bytes_reserved"] < gp.stats()["bytes_reserved"]
+    ug = B.UpdateGen(n, scale, base, m, B.W_LOGUNI32, seed=8)
+    for rnd, k in enumerate([100, 5000, 20000, 20000]):
+        print(f"[parity] window round {rnd} k={k}", flush=True)
+        ins, dels = ug.inserts(k), ug.expire(k)
+        g.insert(ins)
+        assert o.insert(ins) == 0
+        g.delete(dels)
+        assert o.delete(dels) == 0
+        _same_state(g, o, f"window round {rnd}")
+        _same_walks(g, o, orc, B.DEEPWALK, seed=40 + rnd, max_len=25, nwalkers=n)
 
 
 def test_hub_node_long_runs(orc):
