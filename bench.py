@@ -133,7 +133,9 @@ def served(B, args):
     (10^3 to 10^5 arcs) among them, submitted to the serving loop at
     `--rate` requests/s; outputs stay in HBM.  Reports latency percentiles
     (submit -> completion) and throughput, next to the same requests run one
-    at a time as plain per-launch walks."""
+    at a time as plain per-launch walks, and the latencies a server running
+    them that way (one at a time, arrival order) would give under the same
+    arrivals."""
     cfg = CONFIGS["c2"]
     scale, m = cfg["scale"], cfg["edges"]
     n = 1 << scale
@@ -164,7 +166,7 @@ def served(B, args):
     # served form
     g = fresh_graph()
     outs = [B.DeviceArray(g, nbytes=len(r[2]) * 80 * 4) if r[0] == "walk" else None for r in reqs]
-    srv = B.Server(g, arena_bytes=1 << 30, max_launch_walkers=1 << 22)
+    srv = B.Server(g, arena_bytes=1 << 30, max_launch_walkers=1 << 22, streams=2)
     log(f"served: {nreq} requests at {args.rate} req/s")
     tickets, t_sub = [], []
     t0 = time.perf_counter()
@@ -179,9 +181,11 @@ def served(B, args):
             tickets.append(srv.submit_update(r[1], r[2]))
     submit_span = time.perf_counter() - t0
     lat_per_mstep, steps_total, lat_1m = [], 0, []
-    for r, t in zip(reqs, tickets):
+    req_steps = [0] * nreq
+    for i, (r, t) in enumerate(zip(reqs, tickets)):
         _, steps, lat = srv.wait(t)
         if r[0] == "walk":
+            req_steps[i] = steps
             steps_total += steps
             lat_per_mstep.append(lat / max(steps, 1) * 1e6)
             if 0.5e6 <= steps <= 2e6:  # queries of about one million steps
@@ -198,8 +202,10 @@ def served(B, args):
     g = fresh_graph()
     dpaths = B.DeviceArray(g, nbytes=max(len(r[2]) for r in reqs if r[0] == "walk") * 80 * 4)
     dsteps = B.DeviceArray(g, np.zeros(1, np.uint64))
+    svc = []  # service time of each request run alone (ms)
     t1 = time.perf_counter()
     for r in reqs:
+        ta = time.perf_counter()
         if r[0] == "walk":
             ds = B.DeviceArray(g, r[2])
             g.walk_device(r[1], B.walk_params(seed=r[3], max_len=80, form=B.FORM_PLAIN), len(r[2]), dpaths, dsteps,
@@ -209,7 +215,18 @@ def served(B, args):
         else:
             g.insert(r[1])
             g.delete(r[2])
+        svc.append((time.perf_counter() - ta) * 1e3)
     plain_wall = time.perf_counter() - t1
+    # the plain form as a server: one request at a time in arrival order, each
+    # taking its measured service time, under the same Poisson arrivals
+    arrive = np.cumsum(gaps) * 1e3
+    free_at, plain_lat, plain_1m = 0.0, [], []
+    for i, r in enumerate(reqs):
+        free_at = max(free_at, arrive[i]) + svc[i]
+        if r[0] == "walk":
+            plain_lat.append(free_at - arrive[i])
+            if 0.5e6 <= req_steps[i] <= 2e6:
+                plain_1m.append(free_at - arrive[i])
     plain_steps = int(dsteps.to_host(np.uint64, (1,))[0])
     g.close()
     lp = np.percentile(lat_per_mstep, [50, 99]) if lat_per_mstep else [None, None]
@@ -234,6 +251,9 @@ def served(B, args):
         "updates_applied": nupd,
         "plain_form_walk_steps_per_s": plain_steps / plain_wall,
         "served_vs_plain": (steps_total / wall) / (plain_steps / plain_wall),
+        "plain_server_p50_ms": float(np.percentile(plain_lat, 50)),
+        "plain_server_p99_ms": float(np.percentile(plain_lat, 99)),
+        "plain_server_p99_ms_1M_step_queries": float(np.percentile(plain_1m, 99)) if plain_1m else None,
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

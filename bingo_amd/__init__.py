@@ -103,7 +103,7 @@ class ServerConfig(C.Structure):
         ("arena_bytes", C.c_uint64),
         ("max_launch_walkers", C.c_int64),
         ("blocks_per_cu", C.c_int32),
-        ("reserved", C.c_int32),
+        ("streams", C.c_int32),
     ]
 
 
@@ -354,10 +354,11 @@ class Server:
     thread, collect results with wait().  The graph must not be used directly
     while the server runs."""
 
-    def __init__(self, graph, arena_bytes=4 << 30, max_launch_walkers=1 << 23, blocks_per_cu=0):
+    def __init__(self, graph, arena_bytes=4 << 30, max_launch_walkers=1 << 22, blocks_per_cu=0, streams=2):
         cfg = ServerConfig()
         lib().bingo_server_config_default(C.byref(cfg))
         cfg.arena_bytes, cfg.max_launch_walkers, cfg.blocks_per_cu = arena_bytes, max_launch_walkers, blocks_per_cu
+        cfg.streams = streams
         h = C.c_void_p()
         _check(lib().bingo_server_create(graph._h, C.byref(cfg), C.byref(h)))
         self._h = h

@@ -1054,7 +1054,7 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
                                 if (rq[mid].base <= c) lo = mid; else hi = mid - 1;
                             }
                             const ServeReq &q = rq[lo];
-                            const int64_t w = (int64_t)(c - q.base);
+                            const int64_t w = q.first + (int64_t)(c - q.base);
                             if (acc_req[k] != lo) {
                                 if (acc_req[k] >= 0 && acc[k]) atomicAdd(rq[acc_req[k]].steps, (unsigned long long)acc[k]);
                                 acc_req[k] = lo;

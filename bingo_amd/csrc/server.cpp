@@ -1,12 +1,20 @@
 // Serving loop: continuous batching of walk queries and update batches on one
 // graph handle.
 //
-// A worker thread takes requests in submission order.  An update batch runs
-// alone through the normal insert/delete pipeline.  A run of consecutive walk
-// requests is admitted into one persistent launch (k_walk_serve) while the
-// launch's walker budget and the device arena (HBM for request inputs and
-// host-bound outputs) allow; the next launch takes whatever queued up
-// meanwhile.  Every request records submit -> completion latency.
+// A scheduler thread owns the device.  Requests take effect in submission
+// order; an update batch is a barrier: it runs once every walk submitted
+// before it has finished, and walks submitted after it start only once it is
+// applied.  Between barriers, walk requests are admitted as the device arena
+// frees up (their starts/lens are staged there, and host-bound outputs are
+// written there) and packed in submission order into persistent launches
+// (k_walk_serve) of at most max_launch_walkers walkers; a larger request is
+// cut into chunks.  Up to `streams` launches are in flight, each on its own
+// HIP stream with its own request table and counters, and a launch's
+// completion is observed through its event: the next launch is packed from
+// whatever arrived and enqueued while the current one runs, so the device
+// goes from one to the next without a gap and a request that arrives while
+// others run starts in the next launch.  Every request records submit ->
+// completion latency.
 //
 // There is no counterpart in the reference, which launches one walk kernel at
 // a time and synchronises after it (include/graph.cuh:93-150).
@@ -16,6 +24,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -35,6 +44,8 @@ double now_ms()
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+constexpr int kMaxStreams = 4;
+
 struct Req {
     uint64_t ticket = 0;
     bool walk = true;
@@ -42,15 +53,62 @@ struct Req {
     WalkArgs args{};
     std::vector<int32_t> starts, lens;
     int64_t nwalkers = 0;
-    int32_t *paths = nullptr;
+    int32_t *paths = nullptr;  // caller's output, host or device
     bool paths_dev = false;
     std::vector<bingo_edge> ins;
     std::vector<bingo_del> dels;
     size_t arena_need = 0;
+    // scheduler state (scheduler thread only)
+    bool admitted = false;
+    bool has_arena = false;  // holds [arena_off, arena_off + arena_need)
+    size_t arena_off = 0;
+    const int32_t *d_starts = nullptr, *d_lens = nullptr;
+    int32_t *d_paths = nullptr;
+    int64_t next = 0;      // first walker not yet given to a launch
+    int64_t finished = 0;  // walkers whose launch completed
+    uint64_t acc_steps = 0;
+    // completion (guarded by the server mutex)
     double t_submit = 0, t_done = 0;
     bool done = false;
     int status = BINGO_OK;
     uint64_t steps = 0;
+};
+
+struct Part {
+    std::shared_ptr<Req> r;
+    int64_t first, take;  // walkers [first, first + take) of r
+    int64_t base;         // at launch indices [base, base + take)
+};
+
+struct Launch {
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    ServeReq *d_table = nullptr, *h_table = nullptr;             // h_table: pinned
+    unsigned long long *d_counts = nullptr, *h_counts = nullptr;  // [kMaxServeReqs] steps, then the cursor
+    std::vector<Part> parts;
+    int64_t total = 0;
+    bool busy = false;
+    uint64_t seq = 0;  // launch order
+};
+
+// first-fit placement of request staging in the device arena
+struct Arena {
+    char *base = nullptr;
+    size_t bytes = 0;
+    std::map<size_t, size_t> used;  // offset -> size
+    bool alloc(size_t need, size_t *off)
+    {
+        size_t at = 0;
+        for (const auto &u : used) {
+            if (u.first - at >= need) break;
+            at = u.first + u.second;
+        }
+        if (at + need > bytes) return false;
+        used[at] = need;
+        *off = at;
+        return true;
+    }
+    void release(size_t off) { used.erase(off); }
 };
 
 }  // namespace
@@ -61,130 +119,269 @@ struct bingo_server {
     std::thread worker;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
-    std::deque<std::shared_ptr<Req>> queue;
+    std::deque<std::shared_ptr<Req>> inbox;  // submitted, not yet seen by the scheduler
     std::unordered_map<uint64_t, std::shared_ptr<Req>> live;
     uint64_t next_ticket = 1;
     bool stopping = false;
-    char *arena = nullptr;
-    ServeReq *d_table = nullptr;
-    unsigned long long *d_counts = nullptr;
+    Arena arena;
+    hipStream_t copy_st = nullptr;
+    Launch ln[kMaxStreams];
+    int nstreams = 2;
     std::vector<double> lat;
     bingo_server_stats st{};
     int blocks_per_cu = 1;
+    // scheduler-local
+    std::deque<std::shared_ptr<Req>> order;  // unfinished requests, submission order
+    double busy_since = -1;
+    uint64_t launch_seq = 0;
 };
 
-static int run_launch(bingo_server *sv, std::vector<std::shared_ptr<Req>> &batch)
+static void complete(bingo_server *sv, const std::shared_ptr<Req> &r, int rc)
 {
-    bingo_graph *g = sv->g;
-    std::vector<ServeReq> table(batch.size());
-    size_t off = 0;
-    unsigned long long base = 0;
-    for (size_t i = 0; i < batch.size(); ++i) {
-        Req &r = *batch[i];
-        ServeReq q{};
-        q.seed = r.args.seed;
-        q.thr_return = r.args.thr_return;
-        q.thr_neighbor = r.args.thr_neighbor;
-        q.thr_other = r.args.thr_other;
-        q.thr_stop = r.args.thr_stop;
-        q.max_len = r.args.max_len;
-        q.max_attempts = r.args.max_attempts;
-        q.app = r.app;
-        q.nwalkers = r.nwalkers;
-        q.base = base;
-        q.steps = sv->d_counts + i;
-        if (!r.starts.empty()) {
-            q.starts = reinterpret_cast<const int32_t *>(sv->arena + off);
-            HIPCHK(hipMemcpyAsync(sv->arena + off, r.starts.data(), r.starts.size() * 4, hipMemcpyHostToDevice, g->st));
-            off += align256(r.starts.size() * 4);
-        }
-        if (!r.lens.empty()) {
-            q.lens = reinterpret_cast<const int32_t *>(sv->arena + off);
-            HIPCHK(hipMemcpyAsync(sv->arena + off, r.lens.data(), r.lens.size() * 4, hipMemcpyHostToDevice, g->st));
-            off += align256(r.lens.size() * 4);
-        }
-        if (r.paths_dev) {
-            q.paths = r.paths;
-        } else {
-            q.paths = reinterpret_cast<int32_t *>(sv->arena + off);
-            off += align256((size_t)r.nwalkers * r.args.max_len * 4);
-        }
-        base += (unsigned long long)r.nwalkers;
-        table[i] = q;
+    const double t = now_ms();
+    std::lock_guard<std::mutex> lk(sv->mu);
+    r->status = rc;
+    r->t_done = t;
+    r->steps = r->acc_steps;
+    r->done = true;
+    sv->st.requests++;
+    if (r->walk) {
+        sv->st.walk_requests++;
+        sv->st.walk_steps += r->acc_steps;
+        sv->lat.push_back(r->t_done - r->t_submit);
+    } else {
+        sv->st.update_requests++;
     }
-    std::vector<unsigned long long> counts(batch.size(), 0);
-    HIPCHK(hipMemcpyAsync(sv->d_table, table.data(), table.size() * sizeof(ServeReq), hipMemcpyHostToDevice, g->st));
-    HIPCHK(hipMemcpyAsync(sv->d_counts, counts.data(), counts.size() * 8, hipMemcpyHostToDevice, g->st));
-    launch_reset_walk(g->s.ctl, g->st);
+    sv->cv_done.notify_all();
+}
+
+static void retire(bingo_server *sv, const std::shared_ptr<Req> &r)
+{
+    auto it = std::find(sv->order.begin(), sv->order.end(), r);
+    if (it != sv->order.end()) sv->order.erase(it);
+}
+
+static int busy_launches(const bingo_server *sv)
+{
+    int b = 0;
+    for (int i = 0; i < sv->nstreams; ++i) b += sv->ln[i].busy;
+    return b;
+}
+
+static void busy_clock(bingo_server *sv)
+{
+    const bool busy = busy_launches(sv) > 0;
+    const double t = now_ms();
+    if (busy && sv->busy_since < 0) sv->busy_since = t;
+    if (!busy && sv->busy_since >= 0) {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->st.busy_s += (t - sv->busy_since) / 1e3;
+        sv->busy_since = -1;
+    }
+}
+
+// a walk whose last chunk completed: host-bound paths come back, arena frees
+static void finish_walk(bingo_server *sv, const std::shared_ptr<Req> &r)
+{
+    int rc = BINGO_OK;
+    if (!r->paths_dev) {
+        const size_t bytes = (size_t)r->nwalkers * r->args.max_len * 4;
+        if (hipMemcpyAsync(r->paths, r->d_paths, bytes, hipMemcpyDeviceToHost, sv->copy_st) != hipSuccess ||
+            hipStreamSynchronize(sv->copy_st) != hipSuccess)
+            rc = fail(BINGO_EDEVICE, "copying walk output to the host failed");
+    }
+    if (r->has_arena) sv->arena.release(r->arena_off);
+    r->has_arena = false;
+    retire(sv, r);
+    complete(sv, r, rc);
+}
+
+// A launch ended: its parts' walkers are done; finished requests complete.
+static void harvest(bingo_server *sv, Launch &L, hipError_t e)
+{
+    for (size_t i = 0; i < L.parts.size(); ++i) {
+        Part &pt = L.parts[i];
+        auto &r = pt.r;
+        r->finished += pt.take;
+        r->acc_steps += e == hipSuccess ? L.h_counts[i] : 0;
+        if (e != hipSuccess) r->status = BINGO_EDEVICE;
+        if (r->finished == r->nwalkers) {
+            if (r->status) {
+                if (r->has_arena) sv->arena.release(r->arena_off);
+                r->has_arena = false;
+                retire(sv, r);
+                complete(sv, r, fail(BINGO_EDEVICE, "walk launch failed"));
+            } else {
+                finish_walk(sv, r);
+            }
+        }
+    }
+    L.parts.clear();
+    L.busy = false;
+}
+
+// stage a request's inputs in the arena (on the copy stream, which the caller
+// synchronises before any launch uses them); false if the arena is full for now
+static bool admit(bingo_server *sv, Req &r)
+{
+    const hipStream_t st = sv->copy_st;
+    size_t off = 0;
+    if (r.arena_need) {
+        if (!sv->arena.alloc(r.arena_need, &off)) return false;
+        r.has_arena = true;
+    }
+    r.arena_off = off;
+    r.admitted = true;
+    char *p = sv->arena.base + off;
+    if (!r.starts.empty()) {
+        (void)hipMemcpyAsync(p, r.starts.data(), r.starts.size() * 4, hipMemcpyHostToDevice, st);
+        r.d_starts = reinterpret_cast<const int32_t *>(p);
+        p += align256(r.starts.size() * 4);
+    }
+    if (!r.lens.empty()) {
+        (void)hipMemcpyAsync(p, r.lens.data(), r.lens.size() * 4, hipMemcpyHostToDevice, st);
+        r.d_lens = reinterpret_cast<const int32_t *>(p);
+        p += align256(r.lens.size() * 4);
+    }
+    r.d_paths = r.paths_dev ? r.paths : reinterpret_cast<int32_t *>(p);
+    return true;
+}
+
+// Pack the schedulable walks (those before the first pending update) into
+// launch L in submission order: whole requests while the launch's walker
+// budget (max_launch_walkers) and the arena allow, a request larger than the
+// budget in chunks.  The launch is enqueued while the previous one still
+// runs, so under load launches are large and back to back, and a request
+// that arrives while the device is busy goes into the next one.  False if
+// there was nothing to run.
+static bool schedule(bingo_server *sv, Launch &L)
+{
+    const int64_t budget = sv->cfg.max_launch_walkers;
+    int64_t total = 0;
+    bool staged = false;
+    auto add = [&](const std::shared_ptr<Req> &r, int64_t first, int64_t take) {
+        ServeReq q{};
+        q.starts = r->d_starts;
+        q.lens = r->d_lens;
+        q.paths = r->d_paths;
+        q.steps = L.d_counts + L.parts.size();
+        q.base = (unsigned long long)total;
+        q.first = first;
+        q.nwalkers = take;
+        q.seed = r->args.seed;
+        q.thr_return = r->args.thr_return;
+        q.thr_neighbor = r->args.thr_neighbor;
+        q.thr_other = r->args.thr_other;
+        q.thr_stop = r->args.thr_stop;
+        q.max_len = r->args.max_len;
+        q.max_attempts = r->args.max_attempts;
+        q.app = r->app;
+        L.h_table[L.parts.size()] = q;
+        L.parts.push_back(Part{r, first, take, total});
+        total += take;
+    };
+    for (auto &r : sv->order) {
+        if (!r->walk || total >= budget || (int)L.parts.size() >= kMaxServeReqs) break;
+        if (r->next >= r->nwalkers) continue;  // already in flight
+        if (!r->admitted) {
+            if (!admit(sv, *r)) break;  // arena full: later requests wait their turn
+            staged = true;
+        }
+        const int64_t take = std::min(r->nwalkers - r->next, budget - total);
+        add(r, r->next, take);
+        r->next += take;
+    }
+    if (L.parts.empty()) return false;
+    L.total = total;
+    // inputs staged here may be read by chunks on any stream from now on
+    hipError_t e = staged ? hipStreamSynchronize(sv->copy_st) : hipSuccess;
+    bingo_graph *g = sv->g;
+    const int n = (int)L.parts.size();
     const int64_t per_block = 256 * 2;
-    const int64_t blocks =
-        std::max<int64_t>(1, std::min<int64_t>((int64_t)g->num_cu * sv->blocks_per_cu, ((int64_t)base + per_block - 1) / per_block));
-    launch_walk_serve(g->s, sv->d_table, (int)batch.size(), base, &g->s.ctl->cursor, (int)blocks, g->st);
-    HIPCHK(hipGetLastError());
-    for (size_t i = 0; i < batch.size(); ++i)
-        if (!batch[i]->paths_dev)
-            HIPCHK(hipMemcpyAsync(batch[i]->paths, table[i].paths, (size_t)batch[i]->nwalkers * table[i].max_len * 4,
-                                  hipMemcpyDeviceToHost, g->st));
-    HIPCHK(hipMemcpyAsync(counts.data(), sv->d_counts, counts.size() * 8, hipMemcpyDeviceToHost, g->st));
-    HIPCHK(hipStreamSynchronize(g->st));
-    for (size_t i = 0; i < batch.size(); ++i) batch[i]->steps = counts[i];
-    return BINGO_OK;
+    const int64_t full = (int64_t)g->num_cu * sv->blocks_per_cu;
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(full, (total + per_block - 1) / per_block));
+    if (e == hipSuccess) e = hipMemcpyAsync(L.d_table, L.h_table, n * sizeof(ServeReq), hipMemcpyHostToDevice, L.st);
+    if (e == hipSuccess) e = hipMemsetAsync(L.d_counts, 0, (kMaxServeReqs + 1) * sizeof(unsigned long long), L.st);
+    if (e == hipSuccess) {
+        launch_walk_serve(g->s, L.d_table, n, (unsigned long long)total, L.d_counts + kMaxServeReqs, (int)blocks,
+                          L.st);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(L.h_counts, L.d_counts, (kMaxServeReqs + 1) * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, L.st);
+    if (e == hipSuccess) e = hipEventRecord(L.ev, L.st);
+    L.busy = true;
+    L.seq = ++sv->launch_seq;
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(L.st);
+        harvest(sv, L, e);
+        return true;
+    }
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->st.launches++;
+    return true;
+}
+
+static void run_update(bingo_server *sv, const std::shared_ptr<Req> &u)
+{
+    const double t0 = now_ms();
+    int rc = bingo_insert(sv->g, u->ins.data(), (int64_t)u->ins.size());
+    if (!rc) rc = bingo_delete(sv->g, u->dels.data(), (int64_t)u->dels.size());
+    if (hipStreamSynchronize(sv->g->st) != hipSuccess && !rc) rc = fail(BINGO_EDEVICE, "update failed on the device");
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->st.busy_s += (now_ms() - t0) / 1e3;
+    }
+    retire(sv, u);
+    complete(sv, u, rc);
 }
 
 static void worker_main(bingo_server *sv)
 {
     (void)hipSetDevice(sv->g->device);
     for (;;) {
-        std::vector<std::shared_ptr<Req>> batch;
         {
-            std::unique_lock<std::mutex> lk(sv->mu);
-            sv->cv_work.wait(lk, [&] { return sv->stopping || !sv->queue.empty(); });
-            if (sv->queue.empty()) return;  // stopping and drained
-            if (!sv->queue.front()->walk) {
-                batch.push_back(sv->queue.front());
-                sv->queue.pop_front();
-            } else {  // admit consecutive walks while slots and arena last
-                size_t used = 0;
-                int64_t walkers = 0;
-                while (!sv->queue.empty() && sv->queue.front()->walk && batch.size() < (size_t)kMaxServeReqs) {
-                    const auto &r = sv->queue.front();
-                    if (!batch.empty() && (used + r->arena_need > sv->cfg.arena_bytes ||
-                                           walkers + r->nwalkers > sv->cfg.max_launch_walkers))
-                        break;
-                    used += r->arena_need;
-                    walkers += r->nwalkers;
-                    batch.push_back(r);
-                    sv->queue.pop_front();
-                }
+            std::lock_guard<std::mutex> lk(sv->mu);
+            while (!sv->inbox.empty()) {
+                sv->order.push_back(sv->inbox.front());
+                sv->inbox.pop_front();
             }
+            if (sv->stopping && sv->order.empty() && !busy_launches(sv)) return;
         }
-        const double t0 = now_ms();
-        int rc;
-        if (!batch[0]->walk) {
-            Req &u = *batch[0];
-            rc = bingo_insert(sv->g, u.ins.data(), (int64_t)u.ins.size());
-            if (!rc) rc = bingo_delete(sv->g, u.dels.data(), (int64_t)u.dels.size());
+        bool did = false;
+        for (int i = 0; i < sv->nstreams; ++i) {  // completed launches
+            Launch &L = sv->ln[i];
+            if (!L.busy) continue;
+            const hipError_t e = hipEventQuery(L.ev);
+            if (e == hipErrorNotReady) continue;
+            harvest(sv, L, e);
+            did = true;
+        }
+        if (!sv->order.empty() && !sv->order.front()->walk) {
+            if (!busy_launches(sv)) {  // every earlier walk has finished
+                run_update(sv, sv->order.front());
+                did = true;
+            }
         } else {
-            rc = run_launch(sv, batch);
+            for (int i = 0; i < sv->nstreams; ++i)
+                if (!sv->ln[i].busy && schedule(sv, sv->ln[i])) did = true;
         }
-        const double t1 = now_ms();
-        std::lock_guard<std::mutex> lk(sv->mu);
-        sv->st.busy_s += (t1 - t0) / 1e3;
-        if (batch[0]->walk) sv->st.launches++;
-        for (auto &r : batch) {
-            r->status = rc;
-            r->t_done = t1;
-            r->done = true;
-            sv->st.requests++;
-            if (r->walk) {
-                sv->st.walk_requests++;
-                sv->st.walk_steps += r->steps;
-                sv->lat.push_back(r->t_done - r->t_submit);
-            } else {
-                sv->st.update_requests++;
-            }
+        busy_clock(sv);
+        if (did) continue;
+        const int busy = busy_launches(sv);
+        if (busy == sv->nstreams) {  // nothing can start: sleep on the oldest launch
+            int old = 0;
+            for (int i = 1; i < sv->nstreams; ++i)
+                if (sv->ln[i].seq < sv->ln[old].seq) old = i;
+            (void)hipEventSynchronize(sv->ln[old].ev);
+            continue;
         }
-        sv->cv_done.notify_all();
+        std::unique_lock<std::mutex> lk(sv->mu);
+        if (busy)  // poll completions while waiting for new requests
+            sv->cv_work.wait_for(lk, std::chrono::microseconds(50), [&] { return !sv->inbox.empty(); });
+        else
+            sv->cv_work.wait(lk, [&] { return sv->stopping || !sv->inbox.empty(); });
     }
 }
 
@@ -195,10 +392,26 @@ static int submit(bingo_server *sv, std::shared_ptr<Req> r, uint64_t *ticket)
     r->ticket = sv->next_ticket++;
     r->t_submit = now_ms();
     sv->live[r->ticket] = r;
-    sv->queue.push_back(r);
+    sv->inbox.push_back(r);
     if (ticket) *ticket = r->ticket;
     sv->cv_work.notify_one();
     return BINGO_OK;
+}
+
+static void release_server(bingo_server *sv)
+{
+    for (auto &L : sv->ln) {
+        if (L.st) (void)hipStreamSynchronize(L.st);
+        if (L.ev) (void)hipEventDestroy(L.ev);
+        if (L.st) (void)hipStreamDestroy(L.st);
+        if (L.d_table) (void)hipFree(L.d_table);
+        if (L.d_counts) (void)hipFree(L.d_counts);
+        if (L.h_table) (void)hipHostFree(L.h_table);
+        if (L.h_counts) (void)hipHostFree(L.h_counts);
+    }
+    if (sv->copy_st) (void)hipStreamDestroy(sv->copy_st);
+    if (sv->arena.base) (void)hipFree(sv->arena.base);
+    delete sv;
 }
 
 extern "C" {
@@ -207,9 +420,9 @@ void bingo_server_config_default(bingo_server_config *c)
 {
     if (!c) return;
     c->arena_bytes = 4ull << 30;
-    c->max_launch_walkers = 1ll << 23;
+    c->max_launch_walkers = 1ll << 22;
     c->blocks_per_cu = 0;
-    c->reserved = 0;
+    c->streams = 2;
 }
 
 int bingo_server_create(bingo_graph *g, const bingo_server_config *c, bingo_server **out)
@@ -220,20 +433,32 @@ int bingo_server_create(bingo_graph *g, const bingo_server_config *c, bingo_serv
     bingo_server *sv = new bingo_server();
     sv->g = g;
     if (c) sv->cfg = *c; else bingo_server_config_default(&sv->cfg);
-    if (sv->cfg.arena_bytes < 4096 || sv->cfg.max_launch_walkers < 1) {
+    if (sv->cfg.arena_bytes < 4096 || sv->cfg.max_launch_walkers < 1 || sv->cfg.streams < 0 ||
+        sv->cfg.streams > kMaxStreams) {
         delete sv;
-        return fail(BINGO_EINVAL, "server arena or walker budget too small");
+        return fail(BINGO_EINVAL, "server arena, walker budget or stream count out of range");
     }
+    sv->nstreams = sv->cfg.streams ? sv->cfg.streams : 2;
     const int occ = serve_max_blocks_per_cu();
     sv->blocks_per_cu = sv->cfg.blocks_per_cu > 0 ? std::min(sv->cfg.blocks_per_cu, occ) : occ;
-    if (hipMalloc((void **)&sv->arena, sv->cfg.arena_bytes) != hipSuccess ||
-        hipMalloc((void **)&sv->d_table, kMaxServeReqs * sizeof(ServeReq)) != hipSuccess ||
-        hipMalloc((void **)&sv->d_counts, kMaxServeReqs * 8) != hipSuccess) {
+    (void)hipSetDevice(g->device);
+    bool ok = hipMalloc((void **)&sv->arena.base, sv->cfg.arena_bytes) == hipSuccess &&
+              hipStreamCreateWithFlags(&sv->copy_st, hipStreamNonBlocking) == hipSuccess;
+    sv->arena.bytes = sv->cfg.arena_bytes;
+    for (int i = 0; ok && i < sv->nstreams; ++i) {
+        Launch &L = sv->ln[i];
+        ok = hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&L.ev, hipEventDisableTiming) == hipSuccess &&
+             hipMalloc((void **)&L.d_table, kMaxServeReqs * sizeof(ServeReq)) == hipSuccess &&
+             hipMalloc((void **)&L.d_counts, (kMaxServeReqs + 1) * sizeof(unsigned long long)) == hipSuccess &&
+             hipHostMalloc((void **)&L.h_table, kMaxServeReqs * sizeof(ServeReq), hipHostMallocDefault) == hipSuccess &&
+             hipHostMalloc((void **)&L.h_counts, (kMaxServeReqs + 1) * sizeof(unsigned long long),
+                           hipHostMallocDefault) == hipSuccess;
+    }
+    if (!ok) {
         (void)hipGetLastError();
-        if (sv->arena) (void)hipFree(sv->arena);
-        if (sv->d_table) (void)hipFree(sv->d_table);
-        delete sv;
-        return fail(BINGO_ENOMEM, "server arena allocation failed");
+        release_server(sv);
+        return fail(BINGO_ENOMEM, "server arena, stream or table allocation failed");
     }
     sv->worker = std::thread(worker_main, sv);
     *out = sv;
@@ -329,10 +554,7 @@ void bingo_server_destroy(bingo_server *sv)
     }
     if (sv->worker.joinable()) sv->worker.join();
     (void)hipStreamSynchronize(sv->g->st);
-    (void)hipFree(sv->arena);
-    (void)hipFree(sv->d_table);
-    (void)hipFree(sv->d_counts);
-    delete sv;
+    release_server(sv);
 }
 
 }  // extern "C"

@@ -117,12 +117,16 @@ struct WalkArgs {
 // global index range [base, base + nwalkers) of a serving launch; walker i
 // of the request draws from Philox(seed; i, step, attempt), exactly as the
 // request would alone.
+// One entry of a serving launch: walkers [first, first + nwalkers) of a
+// request, at launch indices [base, base + nwalkers).  starts/lens/paths are
+// the request's whole arrays, indexed by the request's own walker id.
 struct ServeReq {
     const int32_t *starts;
     const int32_t *lens;
     int32_t *paths;
     unsigned long long *steps;
     unsigned long long base;
+    int64_t first;
     int64_t nwalkers;
     uint64_t seed;
     uint32_t thr_return, thr_neighbor, thr_other, thr_stop;

@@ -187,21 +187,25 @@ int bingo_event_elapsed(bingo_graph *g, int slot_a, int slot_b, float *ms);
 
 /* ---- serving loop (continuous batching of walk queries and update batches) ----
  * A server owns a loaded graph handle (no other calls on that handle while it
- * runs) and a worker thread.  Requests take effect in submission order: a walk
- * sees exactly the update batches submitted before it.  Consecutive walk
- * requests are admitted together into one persistent launch while the launch's
- * walker budget and the device arena (the HBM budget for request inputs and
- * host-bound outputs) allow; the rest wait for the next launch, which starts as
- * soon as the previous one drains.  A walk's paths are those it would get alone
- * (walker i of a request draws from Philox(seed; i, step, attempt)).  There is
- * no reference counterpart: the reference runs one walk kernel at a time
- * (include/graph.cuh:93-150). */
+ * runs) and a scheduler thread.  Requests take effect in submission order: a
+ * walk sees exactly the update batches submitted before it, and an update runs
+ * once every walk submitted before it has finished.  Between two updates,
+ * walk requests are admitted as device arena (the HBM budget for request
+ * inputs and host-bound outputs) frees up and packed in submission order into
+ * persistent launches of at most max_launch_walkers walkers (a larger request
+ * is cut into chunks).  Up to `streams` launches are in flight on their own HIP
+ * streams: the next launch is packed from whatever arrived and enqueued while
+ * the current one runs, so work that arrives while others run starts in the
+ * next launch and the device never idles between launches.  A walk's paths are those it
+ * would get alone (walker i of a request draws from Philox(seed; i, step,
+ * attempt), whatever chunk runs it).  There is no reference counterpart: the
+ * reference runs one walk kernel at a time (include/graph.cuh:93-150). */
 typedef struct bingo_server bingo_server;
 typedef struct {
     uint64_t arena_bytes;      /* device arena for request staging (HBM admission budget) */
-    int64_t max_launch_walkers;/* walker budget of one launch (a larger request runs alone) */
+    int64_t max_launch_walkers;/* walkers per launch; a larger request is split into chunks */
     int32_t blocks_per_cu;     /* persistent grid per CU; 0 = occupancy limit */
-    int32_t reserved;
+    int32_t streams;           /* launches in flight at once, 1..4; 0 = 2 */
 } bingo_server_config;
 typedef struct {
     uint64_t requests, walk_requests, update_requests, rejected;
