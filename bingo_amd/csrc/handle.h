@@ -43,6 +43,7 @@ enum Scratch {
     SC_AUX1,
     SC_AUX2,
     SC_AUX3,
+    SC_BIG,   // delete batch: long runs and their finished group tasks
     NSCRATCH
 };
 

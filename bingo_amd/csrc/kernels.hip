@@ -189,30 +189,61 @@ __device__ __forceinline__ bool is_neighbor(const uint64_t *__restrict__ hash, u
 // with the highest large until one side is empty; leftovers stay (tau = S).
 // Stay threshold T_j = floor(tau_j 2^64 / S).  Replaces the float/EPS table
 // of NodeData::buildAliasTable (include/Bingo.cuh:111-158).
+//
+// Bounds from the store's limits (group size < 2^28, weight < 2^32):
+// M_j < 2^59, so s_j = 32 M_j < 2^64, and S < 2^60; 64-bit words suffice.
 // ------------------------------------------------------------------------
+
+// floor(tau 2^64 / S) for tau < S < 2^60: a double-precision estimate (off by
+// at most a few thousand), one refinement from the exact 128-bit remainder,
+// then at most a step onto the exact quotient (the loops only finish the job;
+// 2*10^7 random and edge-case pairs checked against 128-bit division needed
+// one step at most).
+__device__ __forceinline__ uint64_t frac64(uint64_t tau, uint64_t S)
+{
+    typedef __int128 i128;
+    const double two64 = 18446744073709551616.0;
+    const double dS = (double)S;
+    const double est = (double)tau / dS * two64;
+    uint64_t q = est >= two64 ? ~0ull : (uint64_t)est;
+    i128 r = ((i128)tau << 64) - (i128)((unsigned __int128)q * S);
+    // |r| < 2^73: high part exact, low 32 bits added once (no cancellation)
+    const double dr = (double)(int64_t)(r >> 32) * 4294967296.0 + (double)(uint32_t)r;
+    const int64_t d = (int64_t)floor(dr / dS);
+    q += (uint64_t)d;
+    r -= (i128)d * (i128)S;
+    while (r < 0) {
+        --q;
+        r += S;
+    }
+    while (r >= (i128)S) {
+        ++q;
+        r -= S;
+    }
+    return q;
+}
+
 __device__ void alias_rebuild(const DevStore &s, int32_t u, uint32_t size, uint64_t off, uint64_t hmeta, int ln)
 {
-    typedef unsigned __int128 u128;
     const bool own = ln < kGroups;
     const uint64_t M = own ? ((uint64_t)size << ln) : 0ull;
-    const uint64_t S = wave_sum64(M);  // < 2^63: fewer than 2^31 slots of weight < 2^32
-    u128 sj = (u128)M * 32u;
-    uint32_t small = (uint32_t)__ballot(own && sj < (u128)S);
-    uint32_t large = (uint32_t)__ballot(own && sj >= (u128)S);
+    const uint64_t S = wave_sum64(M);
+    uint64_t sj = M << 5;
+    uint32_t small = (uint32_t)__ballot(own && sj < S);
+    uint32_t large = (uint32_t)__ballot(own && sj >= S);
     uint64_t tau = 0;
     int ali = ln;
     while (small && large) {
         const int l = 31 - __clz(small), gi = 31 - __clz(large);
-        const u128 sl = ((u128)rl64((uint64_t)(sj >> 64), l) << 64) | rl64((uint64_t)sj, l);
-        u128 sg = ((u128)rl64((uint64_t)(sj >> 64), gi) << 64) | rl64((uint64_t)sj, gi);
+        const uint64_t sl = rl64(sj, l);
+        const uint64_t sg = rl64(sj, gi) - (S - sl);
         if (ln == l) {
-            tau = (uint64_t)sl;
+            tau = sl;
             ali = gi;
         }
         small &= ~(1u << l);
-        sg -= (u128)S - sl;
         if (ln == gi) sj = sg;
-        if (sg < (u128)S) {
+        if (sg < S) {
             large &= ~(1u << gi);
             small |= 1u << gi;
         }
@@ -222,21 +253,8 @@ __device__ void alias_rebuild(const DevStore &s, int32_t u, uint32_t size, uint6
         tau = S;
         ali = ln;
     }
-    uint64_t T;
-    if (tau == S) {
-        T = ~0ull;
-    } else {  // floor(tau * 2^64 / S), tau < S < 2^63: restoring division
-        uint64_t rem = tau, q = 0;
-        for (int i = 0; i < 64; ++i) {
-            rem <<= 1;
-            q <<= 1;
-            if (rem >= S) {
-                rem -= S;
-                q |= 1ull;
-            }
-        }
-        T = q;
-    }
+    uint64_t T = ~0ull;
+    if (own && tau != S) T = frac64(tau, S);
     const uint32_t size_a = __shfl(size, ali, 64);
     const uint64_t off_a = __shfl(off, ali, 64);
     if (own) {
@@ -256,7 +274,10 @@ __global__ void k_reset_ctl(Ctl *ctl)
 {
     const int t = threadIdx.x;
     if (t < NPOOL) ctl->need[t] = 0;
-    if (t == 0) ctl->error = 0;
+    if (t == 0) {
+        ctl->error = 0;
+        ctl->nbig = 0;
+    }
 }
 
 __global__ void k_reset_walk(Ctl *ctl)
@@ -706,16 +727,224 @@ __device__ void delete_run(const DevStore &s, const uint32_t *__restrict__ slots
     alias_rebuild(s, u, own ? S2 : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, ln);
 }
 
-__global__ __launch_bounds__(256) void k_delete_apply(DevStore s, const uint32_t *__restrict__ slots, Runs r)
+// A run of at most kDB deletes: the same result as delete_run in fewer
+// dependent round trips.  Each group lane prefetches its list's last
+// min(cnt, size) members together with the deletes' slot records, tells the
+// deleted tail members from the survivors by comparing slots in registers,
+// pairs the k-th hole with the k-th survivor there, and only gathers the
+// moved members' back offsets afterwards.
+__device__ void delete_small(const DevStore &s, const uint32_t *__restrict__ slots, uint32_t cnt, int32_t u, int ln)
+{
+    const NodeHdr h = s.hdr[u];
+    const bool own = ln < kGroups;
+    GroupHdr gh = {0, 0, 0};
+    if (own) gh = s.grp[(size_t)u * kGroups + ln];
+    const bool in = (uint32_t)ln < cnt;
+    const uint32_t t_sl = in ? slots[ln] : 0u;
+    uint32_t *V = s.slot + h.slot_off;
+    uint32_t *W = V + h.slot_cap;
+    const uint32_t *B = V + 2ull * h.slot_cap;
+    uint32_t *back = s.back + h.back_off;
+    const uint32_t bit = own ? 1u << ln : 0u, below = bit - 1u;
+    const bool hashed = s.nbr && h.hash_cap;
+    uint64_t *m = s.mem + gh.off;
+
+    // the deletes' slot records; this group's last c members
+    const uint32_t cw = in ? W[t_sl] : 0u, cbo = in ? B[t_sl] : 0u, cv = in && hashed ? V[t_sl] : 0u;
+    const uint32_t S = gh.size, c = min(cnt, S);
+    uint64_t e[kDB];
+#pragma unroll
+    for (int i = 0; i < kDB; ++i) e[i] = (uint32_t)i < c ? m[S - c + i] : 0ull;
+
+    // member positions of this group's deletes
+    uint32_t pos[kDB], wq[kDB], sq[kDB];
+    uint32_t kj = 0;
+#pragma unroll
+    for (int q = 0; q < kDB; ++q) {
+        wq[q] = rl32(cw, q);  // 0 for q >= cnt
+        sq[q] = rl32(t_sl, q);
+        pos[q] = (wq[q] & bit) ? back[rl32(cbo, q) + __popc(wq[q] & below)] : 0u;
+        kj += (wq[q] & bit) ? 1u : 0u;
+    }
+    const uint32_t S2 = S - kj;  // kj <= c: the tail [S2, S) is prefetched
+
+    // survivors: tail members whose slot is not one of this group's deletes
+    bool surv[kDB];
+    uint32_t srank[kDB], ns = 0;
+#pragma unroll
+    for (int i = 0; i < kDB; ++i) {
+        bool del = false;
+#pragma unroll
+        for (int q = 0; q < kDB; ++q) del |= (wq[q] & bit) && (uint32_t)e[i] == sq[q];
+        surv[i] = (uint32_t)i < c && (uint32_t)i + kj >= c && !del;
+        srank[i] = ns;
+        ns += surv[i] ? 1u : 0u;
+    }
+    // the k-th hole (deletes in slot order, position below S2) takes the
+    // k-th survivor
+    bool hole[kDB], bad = false;
+    uint64_t mv[kDB];
+    uint32_t hr = 0;
+#pragma unroll
+    for (int q = 0; q < kDB; ++q) {
+        const bool h0 = (wq[q] & bit) && pos[q] < S2;
+        uint64_t x = 0;
+        bool found = false;
+#pragma unroll
+        for (int i = 0; i < kDB; ++i)
+            if (surv[i] && srank[i] == hr) {
+                x = e[i];
+                found = true;
+            }
+        bad |= h0 && !found;  // no survivor left: the store is inconsistent
+        hole[q] = h0 && found;
+        mv[q] = x;
+        hr += h0 ? 1u : 0u;
+    }
+    // the moved members' back offsets
+    uint32_t mb[kDB], mw[kDB];
+#pragma unroll
+    for (int q = 0; q < kDB; ++q) {
+        const uint32_t ms = (uint32_t)mv[q];
+        mb[q] = hole[q] ? B[ms] : 0u;
+        mw[q] = hole[q] ? W[ms] : 0u;
+        if (hole[q]) m[pos[q]] = mv[q];
+    }
+#pragma unroll
+    for (int q = 0; q < kDB; ++q)
+        if (hole[q]) back[mb[q] + __popc(mw[q] & below)] = pos[q];
+    if (bad) atomicOr(&s.ctl->error, (unsigned)ERR_OVERRUN);
+    if (kj) s.grp[(size_t)u * kGroups + ln].size = S2;
+    if (in) {
+        if (hashed) hash_dec(s.hash + h.hash_off, h.hash_cap, cv);
+        W[t_sl] = 0u;
+    }
+    alias_rebuild(s, u, own ? S2 : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, ln);
+}
+
+// One bit group j of a long delete run, by one wave with lanes across the
+// deletes: mark the group's deleted members and count them (k_j), compact the
+// tail [S2, size) in place, then fill the holes (deletes in slot order with a
+// member below S2) with the compacted survivors in order.  Groups of one node
+// touch disjoint members and back entries and only read the slot arrays, so
+// the 32 tasks of a run run concurrently; tombstones wait for the last one.
+__device__ void delete_group(const DevStore &s, const uint32_t *__restrict__ slots, uint32_t cnt, int32_t u, int j,
+                             int ln)
+{
+    const NodeHdr h = s.hdr[u];
+    const uint32_t *W = s.slot + h.slot_off + h.slot_cap;
+    const uint32_t *B = W + h.slot_cap;
+    uint32_t *back = s.back + h.back_off;
+    const GroupHdr gh = s.grp[(size_t)u * kGroups + j];
+    const uint32_t bit = 1u << j, below = bit - 1u;
+    uint64_t *m = s.mem + gh.off;
+    const uint64_t lt = (1ull << ln) - 1ull;
+    uint32_t kj = 0;
+    for (uint32_t c0 = 0; c0 < cnt; c0 += 64) {  // mark
+        const uint32_t t = c0 + (uint32_t)ln;
+        const uint32_t sl = t < cnt ? slots[t] : 0u;
+        const uint32_t w = t < cnt ? W[sl] : 0u, bo = t < cnt ? B[sl] : 0u;
+        const bool has = (w & bit) != 0u;
+        if (has) reinterpret_cast<uint32_t *>(m + back[bo + __popc(w & below)])[0] = sl | kMark;
+        kj += (uint32_t)__popcll(__ballot(has));
+    }
+    __threadfence_block();  // marks by any lane before the tail is read
+    const uint32_t S = gh.size, S2 = S - kj;
+    uint32_t ns = 0;
+    for (uint32_t p0 = S2; p0 < S; p0 += 64) {  // survivors of the tail, in order, to [S2, S2 + ns)
+        const uint32_t p = p0 + (uint32_t)ln;
+        const uint64_t e = p < S ? m[p] : (uint64_t)kMark;
+        const bool surv = !((uint32_t)e & kMark);
+        const uint64_t b = __ballot(surv);
+        if (surv) m[S2 + ns + (uint32_t)__popcll(b & lt)] = e;  // never above p
+        ns += (uint32_t)__popcll(b);
+    }
+    __threadfence_block();  // compacted survivors before they are read
+    uint32_t hr = 0;
+    bool bad = false;
+    for (uint32_t c0 = 0; c0 < cnt; c0 += 64) {  // holes take the survivors
+        const uint32_t t = c0 + (uint32_t)ln;
+        const uint32_t sl = t < cnt ? slots[t] : 0u;
+        const uint32_t w = t < cnt ? W[sl] : 0u, bo = t < cnt ? B[sl] : 0u;
+        const bool has = (w & bit) != 0u;
+        const uint32_t pos = has ? back[bo + __popc(w & below)] : 0u;
+        const bool hole = has && pos < S2;
+        const uint64_t hb = __ballot(hole);
+        const uint32_t rank = hr + (uint32_t)__popcll(hb & lt);
+        const bool ok = hole && rank < ns;
+        bad |= hole && !ok;
+        const uint64_t mv = ok ? m[S2 + rank] : 0ull;
+        if (ok) m[pos] = mv;
+        const uint32_t ms = (uint32_t)mv;
+        const uint32_t mb = ok ? B[ms] : 0u, mw = ok ? W[ms] : 0u;
+        if (ok) back[mb + __popc(mw & below)] = pos;
+        hr += (uint32_t)__popcll(hb);
+    }
+    if (__any(bad) && ln == 0) atomicOr(&s.ctl->error, (unsigned)ERR_OVERRUN);
+    if (kj && ln == 0) s.grp[(size_t)u * kGroups + j].size = S2;
+}
+
+// After the last group task of a long run: neighbour counts, tombstones and
+// the node's alias table.
+__device__ void delete_finish(const DevStore &s, const uint32_t *__restrict__ slots, uint32_t cnt, int32_t u, int ln)
+{
+    const NodeHdr h = s.hdr[u];
+    uint32_t *V = s.slot + h.slot_off;
+    uint32_t *W = V + h.slot_cap;
+    const bool hashed = s.nbr && h.hash_cap;
+    for (uint32_t t = ln; t < cnt; t += 64) {
+        const uint32_t sl = slots[t];
+        if (hashed) hash_dec(s.hash + h.hash_off, h.hash_cap, V[sl]);
+        W[sl] = 0u;
+    }
+    GroupHdr gh = {0, 0, 0};
+    if (ln < kGroups) gh = s.grp[(size_t)u * kGroups + ln];
+    alias_rebuild(s, u, ln < kGroups ? gh.size : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, ln);
+}
+
+__global__ void k_delete_big_list(Runs r, uint32_t *__restrict__ big, Ctl *ctl)
+{
+    const uint32_t nruns = *r.nruns;
+    const uint32_t ri = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ri < nruns && r.count[ri] > kBigDelete) big[atomicAdd(&ctl->nbig, 1u)] = ri;
+}
+
+// Tasks: first one per (long run, bit group), then every other run.  A wave
+// takes tasks by grid stride; long runs' group tasks start first.
+__global__ __launch_bounds__(256) void k_delete_apply(DevStore s, const uint32_t *__restrict__ slots, Runs r,
+                                                      const uint32_t *__restrict__ big, uint32_t *__restrict__ done)
 {
     const int ln = lane_id();
     const uint32_t nruns = *r.nruns;
+    const uint32_t nbig = s.ctl->nbig;
+    const uint32_t ntasks = nbig * kGroups + nruns;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (uint32_t ri = wave; ri < nruns; ri += nwaves) {
+    for (uint32_t task = wave; task < ntasks; task += nwaves) {
+        if (task < nbig * kGroups) {
+            const uint32_t bi = task / kGroups, ri = big[bi];
+            const int32_t u = (int32_t)r.node[ri];
+            if (u < 1 || u > s.n) continue;
+            const uint32_t *sl = slots + r.start[ri];
+            delete_group(s, sl, r.count[ri], u, (int)(task % kGroups), ln);
+            __threadfence();  // this group's writes before the count
+            uint32_t prev = 0;
+            if (ln == 0) prev = atomicAdd(&done[bi], 1u);
+            prev = __shfl(prev, 0, 64);
+            if (prev == kGroups - 1) {
+                __threadfence();  // every group's writes before the finish
+                delete_finish(s, sl, r.count[ri], u, ln);
+            }
+            continue;
+        }
+        const uint32_t ri = task - nbig * kGroups;
         const int32_t u = (int32_t)r.node[ri];
         if (u < 1 || u > s.n) continue;
-        delete_run(s, slots + r.start[ri], r.count[ri], u, ln);
+        const uint32_t cnt = r.count[ri];
+        if (cnt <= (uint32_t)kDB)
+            delete_small(s, slots + r.start[ri], cnt, u, ln);
+        else if (cnt <= kBigDelete)
+            delete_run(s, slots + r.start[ri], cnt, u, ln);
     }
 }
 
@@ -1297,9 +1526,15 @@ void launch_delete_check(DevStore s, const unsigned long long *keys, int64_t k, 
     hipLaunchKernelGGL(k_delete_check, dim3(blocks_for(k, 256)), dim3(256), 0, st, s, keys, k, us, slots);
 }
 
-void launch_delete_apply(DevStore s, const uint32_t *slots, Runs r, int64_t k, hipStream_t st)
+void launch_delete_big_list(Runs r, int64_t k, uint32_t *big, Ctl *ctl, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_delete_apply, dim3(run_blocks(k)), dim3(256), 0, st, s, slots, r);
+    hipLaunchKernelGGL(k_delete_big_list, dim3(blocks_for(k, 256)), dim3(256), 0, st, r, big, ctl);
+}
+
+void launch_delete_apply(DevStore s, const uint32_t *slots, Runs r, const uint32_t *big, uint32_t *done, int64_t k,
+                         hipStream_t st)
+{
+    hipLaunchKernelGGL(k_delete_apply, dim3(run_blocks(k)), dim3(256), 0, st, s, slots, r, big, done);
 }
 
 void launch_walk(DevStore s, int app, const WalkArgs &a, int block, hipStream_t st)

@@ -68,7 +68,7 @@ struct Ctl {
     unsigned long long top[NPOOL];   // pool cursors (elements)
     unsigned long long need[NPOOL];  // planned growth of the pending batch
     unsigned int error;              // ErrFlag bits
-    unsigned int pad;
+    unsigned int nbig;               // delete batch: runs split into per-group tasks
     unsigned long long steps;        // last walk: appended vertices
     unsigned long long attempts;     // last walk: sample draws (node2vec rejections included)
     unsigned long long probes;       // last walk: neighbour-table lookups
@@ -154,7 +154,13 @@ void launch_insert_apply(DevStore s, const DevEdge *sorted, Runs r, int64_t k, h
 void launch_delete_keys(const int32_t *dels, int64_t k, unsigned long long *keys, hipStream_t st);
 void launch_delete_check(DevStore s, const unsigned long long *keys, int64_t k, uint32_t *us, uint32_t *slots,
                          hipStream_t st);
-void launch_delete_apply(DevStore s, const uint32_t *slots, Runs r, int64_t k, hipStream_t st);
+// Delete runs longer than kBigDelete are listed (big: run index, done: group
+// tasks finished, zeroed by the caller; room for k / kBigDelete + 1 entries)
+// and handled by one wave per bit group.
+constexpr uint32_t kBigDelete = 128;
+void launch_delete_big_list(Runs r, int64_t k, uint32_t *big, Ctl *ctl, hipStream_t st);
+void launch_delete_apply(DevStore s, const uint32_t *slots, Runs r, const uint32_t *big, uint32_t *done, int64_t k,
+                         hipStream_t st);
 void launch_walk(DevStore s, int app, const WalkArgs &a, int block, hipStream_t st);
 struct PersistCfg {
     int block, wpl, depth, blocks;  // blocks = grid size

@@ -122,6 +122,40 @@ def test_fitted_store_under_sliding_window(orc):
         _same_walks(g, o, orc, B.DEEPWALK, seed=40 + rnd, max_len=25, nwalkers=n)
 
 
+def test_many_long_delete_runs(orc):
+    """Twenty nodes with hundreds of deletes each in one batch (the per-group
+    tasks of long runs run side by side), among short runs of other nodes,
+    with and without neighbour tables."""
+    scale = 12
+    n = 1 << scale
+    base = B.gen_rmat(scale, 150_000, B.W_LOGUNI32, seed=21, undirected=False)
+    for nbr in (True, False):
+        g, o = _pair(orc, n, base, nbr)
+        d = o.export()
+        so = np.concatenate([[0], np.cumsum(d["nslots"].astype(np.int64))])
+        rng = np.random.default_rng(4)
+        hubs = np.argsort(d["nslots"])[::-1][:20]
+        dels = []
+        for u in range(1, n + 1):
+            ns = int(d["nslots"][u])
+            if not ns:
+                continue
+            frac = 0.35 if u in hubs else 0.05
+            pick = np.flatnonzero(rng.random(ns) < frac)
+            dels += [(u, int(s)) for s in pick]
+        dels = np.array(dels, B.DEL_DT)
+        dels = dels[rng.permutation(len(dels))]
+        counts = np.bincount(dels["u"], minlength=n + 1)
+        assert (counts > 128).sum() >= 10 and ((counts > 0) & (counts <= 8)).sum() > 100
+        g.delete(dels)
+        assert o.delete(dels) == 0
+        _same_state(g, o, f"long runs nbr={nbr}")
+        app = B.NODE2VEC if nbr else B.DEEPWALK
+        _same_walks(g, o, orc, app, seed=77, max_len=20, nwalkers=2 * n)
+        g.close()
+        o.close()
+
+
 def test_hub_node_long_runs(orc):
     """One node with tens of thousands of arcs: multi-chunk ballots, bucket
     growth and neighbour-table rehash inside one batch, thousands of deletes
