@@ -32,24 +32,13 @@ __device__ __forceinline__ uint64_t rl64(uint64_t v, int l)
 {
     return ((uint64_t)rl32((uint32_t)(v >> 32), l) << 32) | rl32((uint32_t)v, l);
 }
-__device__ __forceinline__ uint32_t uni32(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 
+// LDS-permute forms, correct whatever lanes are active (walk kernel tails,
+// export)
 __device__ __forceinline__ uint32_t wave_sum32(uint32_t v)
 {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
-}
-__device__ __forceinline__ uint64_t wave_sum64(uint64_t v)
-{
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
-}
-__device__ __forceinline__ uint32_t wave_or32(uint32_t v)
-{
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v |= __shfl_xor(v, d, 64);
     return v;
 }
 // inclusive prefix sum over the wave
@@ -59,16 +48,6 @@ __device__ __forceinline__ uint32_t wave_incl32(uint32_t v)
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
         uint32_t t = __shfl_up(v, d, 64);
-        if (ln >= d) v += t;
-    }
-    return v;
-}
-__device__ __forceinline__ uint64_t wave_incl64(uint64_t v)
-{
-    const int ln = lane_id();
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint64_t t = __shfl_up(v, d, 64);
         if (ln >= d) v += t;
     }
     return v;
@@ -88,6 +67,53 @@ __device__ __forceinline__ void wave_copy(T *__restrict__ dst, const T *__restri
         for (uint32_t q = 0; q < kCU; ++q) dst[i + q * 64] = x[q];
     }
     for (; i < n; i += 64) dst[i] = src[i];
+}
+
+// DPP forms of the wave scans for the update kernels, where all 64 lanes are
+// active: row shifts 1, 2, 4, 8 scan each 16-lane row, then row broadcasts of
+// lanes 15 and 31 carry the row totals; a few VALU steps instead of a chain of
+// LDS permutes.  Lane 63 ends with the total.
+#define BINGO_DPP(v, ctrl, rows) ((uint32_t)__builtin_amdgcn_update_dpp(0, (int)(v), (ctrl), (rows), 0xf, false))
+__device__ __forceinline__ uint32_t dpp_incl_add(uint32_t v)
+{
+    v += BINGO_DPP(v, 0x111, 0xf);
+    v += BINGO_DPP(v, 0x112, 0xf);
+    v += BINGO_DPP(v, 0x114, 0xf);
+    v += BINGO_DPP(v, 0x118, 0xf);
+    v += BINGO_DPP(v, 0x142, 0xa);
+    v += BINGO_DPP(v, 0x143, 0xc);
+    return v;
+}
+__device__ __forceinline__ uint32_t dpp_incl_or(uint32_t v)
+{
+    v |= BINGO_DPP(v, 0x111, 0xf);
+    v |= BINGO_DPP(v, 0x112, 0xf);
+    v |= BINGO_DPP(v, 0x114, 0xf);
+    v |= BINGO_DPP(v, 0x118, 0xf);
+    v |= BINGO_DPP(v, 0x142, 0xa);
+    v |= BINGO_DPP(v, 0x143, 0xc);
+    return v;
+}
+#undef BINGO_DPP
+__device__ __forceinline__ uint32_t dpp_sum32(uint32_t v) { return rl32(dpp_incl_add(v), 63); }
+__device__ __forceinline__ uint32_t dpp_or32(uint32_t v) { return rl32(dpp_incl_or(v), 63); }
+// 64-bit sum / inclusive prefix over the lanes whose value is non-zero (few
+// in the update kernels: one per bit group at most)
+__device__ __forceinline__ uint64_t sparse_sum64(uint64_t v)
+{
+    uint64_t acc = 0;
+    for (uint64_t nz = __ballot(v != 0ull); nz; nz &= nz - 1ull) acc += rl64(v, __ffsll((unsigned long long)nz) - 1);
+    return acc;
+}
+__device__ __forceinline__ uint64_t sparse_incl64(uint64_t v, int ln)
+{
+    uint64_t acc = 0;
+    for (uint64_t nz = __ballot(v != 0ull); nz; nz &= nz - 1ull) {
+        const int l = __ffsll((unsigned long long)nz) - 1;
+        const uint64_t x = rl64(v, l);
+        if (ln >= l) acc += x;
+    }
+    return acc;
 }
 
 __device__ __forceinline__ uint32_t pow2_at_least(uint32_t x)
@@ -227,7 +253,7 @@ __device__ void alias_rebuild(const DevStore &s, int32_t u, uint32_t size, uint6
 {
     const bool own = ln < kGroups;
     const uint64_t M = own ? ((uint64_t)size << ln) : 0ull;
-    const uint64_t S = wave_sum64(M);
+    const uint64_t S = sparse_sum64(M);
     uint64_t sj = M << 5;
     uint32_t small = (uint32_t)__ballot(own && sj < S);
     uint32_t large = (uint32_t)__ballot(own && sj >= S);
@@ -357,14 +383,14 @@ __device__ Growth plan_run(const DevStore &s, const DevEdge *__restrict__ e, uin
     for (uint32_t c = 0; c < cnt; c += 64) {
         const uint32_t i = c + (uint32_t)ln;
         const uint32_t w = i < cnt ? e[i].w : 0u;
-        const uint32_t bits = uni32(wave_or32(w));
+        const uint32_t bits = dpp_or32(w);
         for (uint32_t b = bits; b;) {
             const int j = __ffs(b) - 1;
             b &= b - 1u;
             const uint64_t m = __ballot((w >> j) & 1u);
             if (ln == j) addj += (uint32_t)__popcll(m);
         }
-        pcsum += wave_sum32((uint32_t)__popc(w));
+        pcsum += dpp_sum32((uint32_t)__popc(w));
     }
     G.cnt = cnt;
     G.addj = addj;
@@ -413,7 +439,7 @@ __global__ __launch_bounds__(256) void k_insert_plan(DevStore s, const DevEdge *
         need[POOL_SLOT] += G.g_slots ? 3ull * G.cap_slots : 0;
         need[POOL_BACK] += G.g_back ? G.cap_back : 0;
         need[POOL_HASH] += G.g_hash ? G.cap_hash : 0;
-        need[POOL_MEM] += wave_sum64(G.g_g ? (uint64_t)G.cap_g : 0ull);
+        need[POOL_MEM] += sparse_sum64(G.g_g ? (uint64_t)G.cap_g : 0ull);
     }
     if (ln == 0)
         for (int p = 0; p < NPOOL; ++p)
@@ -443,8 +469,8 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, uin
     // ---- regions (lane 0 bumps node pools; groups bump once per wave) ----
     uint64_t a_slot = h.slot_off, a_back = h.back_off, a_hash = h.hash_off;
     const uint64_t gneed = G.g_g ? (uint64_t)G.cap_g : 0ull;
-    const uint64_t gincl = wave_incl64(gneed);
-    const uint64_t gtotal = __shfl(gincl, 63, 64);
+    const uint64_t gincl = sparse_incl64(gneed, ln);
+    const uint64_t gtotal = rl64(gincl, 63);
     uint64_t gbase = 0;
     if (ln == 0) {
         if (G.g_slots) a_slot = bump(s, POOL_SLOT, 3ull * G.cap_slots);
@@ -452,10 +478,10 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, uin
         if (G.g_hash) a_hash = bump(s, POOL_HASH, G.cap_hash);
         if (gtotal) gbase = bump(s, POOL_MEM, gtotal);
     }
-    a_slot = __shfl(a_slot, 0, 64);
-    a_back = __shfl(a_back, 0, 64);
-    a_hash = __shfl(a_hash, 0, 64);
-    gbase = __shfl(gbase, 0, 64);
+    a_slot = rl64(a_slot, 0);
+    a_back = rl64(a_back, 0);
+    a_hash = rl64(a_hash, 0);
+    gbase = rl64(gbase, 0);
     if (a_slot == ~0ull || a_back == ~0ull || a_hash == ~0ull || gbase == ~0ull) return;  // node left as it was
     const uint64_t goff = G.g_g ? gbase + gincl - gneed : gh.off;
 
@@ -489,7 +515,7 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, uin
                 if ((uint32_t)x && (x >> 32)) fresh += hash_add(htab, G.cap_hash, (uint32_t)x, (uint32_t)(x >> 32));
             }
         }
-        hused = wave_sum32(fresh);
+        hused = dpp_sum32(fresh);
         __threadfence();
     }
 
@@ -504,7 +530,7 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, uin
         if (act) x = e[i];
         const uint32_t slot = n0 + i;
         const uint32_t pc = (uint32_t)__popc(x.w);
-        const uint32_t pincl = wave_incl32(pc);
+        const uint32_t pincl = dpp_incl_add(pc);
         const uint32_t my_boff = boff_run + pincl - pc;
         if (act) {
             nv[slot] = (uint32_t)x.v;
@@ -514,7 +540,7 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, uin
         const uint64_t lt = (1ull << ln) - 1ull;
         uint32_t rank = 0;
         const uint64_t entry = ((uint64_t)(uint32_t)x.v << 32) | slot;
-        for (uint32_t b = uni32(wave_or32(x.w)); b;) {
+        for (uint32_t b = dpp_or32(x.w); b;) {
             const int j = __ffs(b) - 1;
             b &= b - 1u;
             const bool has = (x.w >> j) & 1u;
@@ -529,10 +555,10 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, uin
             }
             if (ln == j) szj += (uint32_t)__popcll(m);
         }
-        boff_run += __shfl(pincl, 63, 64);
+        boff_run += rl32(pincl, 63);
         if (s.nbr && act) fresh += hash_add(htab, G.cap_hash, (uint32_t)x.v, 1u);
     }
-    hused += wave_sum32(fresh);
+    hused += dpp_sum32(fresh);
 
     // ---- headers ----
     if (ln < kGroups) {
@@ -559,6 +585,8 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, uin
     alias_rebuild(s, u, ln < kGroups ? szj : 0u, goff, s.nbr ? make_hmeta(a_hash, G.cap_hash) : 0ull, ln);
 }
 
+// Runs by grid stride in node order (neighbouring runs touch neighbouring
+// node records; a shared work counter measured slower).
 __global__ __launch_bounds__(256) void k_insert_apply(DevStore s, const DevEdge *__restrict__ E, Runs r)
 {
     const int ln = lane_id();
@@ -909,8 +937,8 @@ __global__ void k_delete_big_list(Runs r, uint32_t *__restrict__ big, Ctl *ctl)
     if (ri < nruns && r.count[ri] > kBigDelete) big[atomicAdd(&ctl->nbig, 1u)] = ri;
 }
 
-// Tasks: first one per (long run, bit group), then every other run.  A wave
-// takes tasks by grid stride; long runs' group tasks start first.
+// Tasks: first one per (long run, bit group), then every other run; a wave
+// takes tasks by grid stride, so long runs' group tasks start first.
 __global__ __launch_bounds__(256) void k_delete_apply(DevStore s, const uint32_t *__restrict__ slots, Runs r,
                                                       const uint32_t *__restrict__ big, uint32_t *__restrict__ done)
 {
