@@ -15,6 +15,9 @@
  *   node2vec ......... closed-form second-order law by brute force
  *   PPR .............. geometric length law with cap
  *   generators ....... R-MAT quadrant frequencies, weight ranges
+ *   float weights .... exact law (1 - F/(S+F)) I/S + (F/(S+F)) f/F = w/sum(w)
+ *                      from the exported state, chi-square of draws,
+ *                      threshold against 128-bit division, float splitting
  */
 #include "bingo_oracle.h"
 #include "philox.h"
@@ -41,11 +44,14 @@ typedef struct {
     uint64_t mass;
     uint64_t T[ORC_GROUPS];
     uint8_t alias[ORC_GROUPS];
+    uint32_t *fr;                    /* float weights: per slot fraction x 2^32 */
+    uint64_t fmass, fthr;            /* sum of fractions; fractional-part threshold */
 } node_t;
 
 struct orc_graph {
     int32_t n;
     int nbr;
+    int fw;                          /* weights are float bits */
     node_t *nodes;
 };
 
@@ -160,22 +166,53 @@ void orc_alias_build(const uint64_t M[32], uint64_t T[32], uint8_t alias[32])
         T[j] = (tau[j] == S) ? UINT64_MAX : (uint64_t)((tau[j] << 64) / S);
 }
 
+/* Fractional part of a draw: the 64-bit fraction floor(F 2^64 / D) of
+ * F / D, D = S 2^32 + F, F in units of 2^-32 (S < 2^60, F < 2^60: D < 2^93,
+ * F 2^64 < 2^124).  S = 0: always the fractional part. */
+uint64_t orc_frac_threshold(uint64_t S, uint64_t F)
+{
+    if (!F) return 0;
+    if (!S) return UINT64_MAX;
+    u128 D = ((u128)S << 32) + F;
+    return (uint64_t)(((u128)F << 64) / D);
+}
+
+/* Float weight bits -> integer part and fraction x 2^32 (floor).  Valid:
+ * finite, 0 < w < 2^32, and at least 2^-32 after the floor. */
+int orc_split_float(uint32_t bits, uint32_t *ipart, uint32_t *frac)
+{
+    float x;
+    memcpy(&x, &bits, 4);
+    double d = (double)x;
+    if (!(d > 0.0) || !(d < 4294967296.0)) return 0;
+    double fl = floor(d);
+    uint32_t ip = (uint32_t)fl, fr = (uint32_t)floor((d - fl) * 4294967296.0);
+    if (!ip && !fr) return 0;
+    *ipart = ip;
+    *frac = fr;
+    return 1;
+}
+
 static void node_alias(node_t *x)
 {
     uint64_t M[32], S = 0;
     for (int j = 0; j < 32; ++j) { M[j] = (uint64_t)x->size[j] << j; S += M[j]; }
     x->mass = S;
     orc_alias_build(M, x->T, x->alias);
+    x->fthr = orc_frac_threshold(S, x->fmass);
 }
 
 /* ---------------- graph ---------------- */
 
-orc_graph *orc_create(int32_t n, int neighbor_index)
+orc_graph *orc_create(int32_t n, int neighbor_index) { return orc_create_ex(n, neighbor_index, 0); }
+
+orc_graph *orc_create_ex(int32_t n, int neighbor_index, int float_weights)
 {
     if (n < 1) return NULL;
     orc_graph *g = (orc_graph *)calloc(1, sizeof(orc_graph));
     g->n = n;
     g->nbr = neighbor_index;
+    g->fw = float_weights != 0;
     g->nodes = (node_t *)calloc((size_t)n + 1, sizeof(node_t));
     if (!g->nodes) abort();
     for (int32_t u = 0; u <= n; ++u) node_alias(&g->nodes[u]);
@@ -187,7 +224,7 @@ void orc_destroy(orc_graph *g)
     if (!g) return;
     for (int32_t u = 0; u <= g->n; ++u) {
         node_t *x = &g->nodes[u];
-        free(x->v); free(x->w); free(x->boff); free(x->back); free(x->hash);
+        free(x->v); free(x->w); free(x->boff); free(x->back); free(x->hash); free(x->fr);
         for (int j = 0; j < 32; ++j) free(x->mem[j]);
     }
     free(g->nodes);
@@ -199,7 +236,7 @@ int32_t orc_num_nodes(const orc_graph *g) { return g->n; }
 /* Append one arc as a new slot: slot = nslots++, and for each set bit j in
  * ascending order the pair (v, slot) is appended to bucket j
  * (Bingo.cuh:182-198 insert; Bingo.cuh:159-181 build uses the same order). */
-static void node_append(node_t *x, uint32_t v, uint32_t w, int nbr)
+static void node_append(node_t *x, uint32_t v, uint32_t w, uint32_t f, int fw, int nbr)
 {
     uint32_t s = x->nslots;
     if (s == x->slot_cap) {
@@ -207,9 +244,14 @@ static void node_append(node_t *x, uint32_t v, uint32_t w, int nbr)
         x->v = (uint32_t *)xrealloc(x->v, (size_t)x->slot_cap * 4);
         x->w = (uint32_t *)xrealloc(x->w, (size_t)x->slot_cap * 4);
         x->boff = (uint32_t *)xrealloc(x->boff, (size_t)x->slot_cap * 4);
+        if (fw) x->fr = (uint32_t *)xrealloc(x->fr, (size_t)x->slot_cap * 4);
     }
     x->v[s] = v;
     x->w[s] = w;
+    if (fw) {
+        x->fr[s] = f;
+        x->fmass += f;
+    }
     x->boff[s] = x->back_used;
     uint32_t pc = (uint32_t)popc32(w);
     if (x->back_used + pc > x->back_cap) {
@@ -270,9 +312,11 @@ static void grouping_free(grouping *G) { free(G->order); free(G->runs); free(G->
 int orc_insert(orc_graph *g, const orc_edge *e, int64_t k)
 {
     if (k < 0 || (k > 0 && !e)) return ORC_EINVAL;
-    for (int64_t i = 0; i < k; ++i)
-        if (e[i].u < 1 || e[i].u > g->n || e[i].v < 1 || e[i].v > g->n || e[i].w == 0)
-            return ORC_EINVAL;
+    for (int64_t i = 0; i < k; ++i) {
+        uint32_t ip, fr;
+        if (e[i].u < 1 || e[i].u > g->n || e[i].v < 1 || e[i].v > g->n) return ORC_EINVAL;
+        if (g->fw ? !orc_split_float(e[i].w, &ip, &fr) : e[i].w == 0) return ORC_EINVAL;
+    }
     if (!k) return ORC_OK;
     grouping G = group_by_node(g->n, k, &e[0].u, sizeof(orc_edge));
     for (int64_t r = 0; r < G.nruns; ++r) {
@@ -287,7 +331,9 @@ int orc_insert(orc_graph *g, const orc_edge *e, int64_t k)
         node_t *x = &g->nodes[G.run_node[r]];
         for (int64_t t = G.runs[r]; t < G.runs[r + 1]; ++t) {
             const orc_edge *a = &e[G.order[t]];
-            node_append(x, (uint32_t)a->v, a->w, g->nbr);
+            uint32_t w = a->w, f = 0;
+            if (g->fw) orc_split_float(a->w, &w, &f);
+            node_append(x, (uint32_t)a->v, w, f, g->fw, g->nbr);
         }
         node_alias(x);
     }
@@ -333,6 +379,10 @@ static void node_delete_run(node_t *x, const uint32_t *slots, int64_t cnt, int n
     for (int64_t t = 0; t < cnt; ++t) {
         if (nbr) hash_put(x, x->v[slots[t]], -1);
         x->w[slots[t]] = 0;
+        if (x->fr) {
+            x->fmass -= x->fr[slots[t]];
+            x->fr[slots[t]] = 0;
+        }
     }
 }
 
@@ -351,7 +401,8 @@ int orc_delete(orc_graph *g, const orc_del *d, int64_t k)
     for (int64_t i = 0; i < k; ++i) {
         if (d[i].u < 1 || d[i].u > g->n || d[i].slot < 0) return ORC_EINVAL;
         node_t *x = &g->nodes[d[i].u];
-        if ((uint32_t)d[i].slot >= x->nslots || x->w[d[i].slot] == 0) return ORC_EINVAL;
+        if ((uint32_t)d[i].slot >= x->nslots) return ORC_EINVAL;
+        if (x->w[d[i].slot] == 0 && !(g->fw && x->fr[d[i].slot])) return ORC_EINVAL;
     }
     if (!k) return ORC_OK;
     uint64_t *key = (uint64_t *)malloc((size_t)k * 8);
@@ -384,7 +435,7 @@ uint32_t orc_edge_count(const orc_graph *g, int32_t u, int32_t v)
     const node_t *x = &g->nodes[u];
     if (g->nbr) return hash_get(x, (uint32_t)v);
     uint32_t c = 0;
-    for (uint32_t s = 0; s < x->nslots; ++s) c += (x->w[s] && x->v[s] == (uint32_t)v);
+    for (uint32_t s = 0; s < x->nslots; ++s) c += ((x->w[s] || (g->fw && x->fr[s])) && x->v[s] == (uint32_t)v);
     return c;
 }
 
@@ -408,6 +459,32 @@ static inline int32_t node_sample(const node_t *x, const uint32_t r[4])
 void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
 {
     orc_philox4x32_10(ctr, key, out);
+}
+
+/* One draw with float weights: with probability F / (S + F) (coin64 of the
+ * fractional stream below the threshold, or no integer mass) the fractional
+ * part: a uniform slot, accepted with probability fraction / 2^32, up to
+ * ORC_FRAC_ITERS rounds from stream blocks (wid; step, FRAC_TAG | it << 16 |
+ * attempt); otherwise (or when every round rejects) the integer draw. */
+static int32_t node_draw(const orc_graph *g, const node_t *x, const uint32_t r[4], const uint32_t key[2],
+                         uint64_t wid, uint32_t step, uint32_t att)
+{
+    if (g->fw && x->fthr) {
+        uint32_t b[4], ctr[4] = {(uint32_t)wid, (uint32_t)(wid >> 32), step, ORC_FRAC_TAG | att};
+        orc_philox4x32_10(ctr, key, b);
+        uint64_t coin = ((uint64_t)b[0] << 32) | b[1];
+        if (!x->mass || coin < x->fthr) {
+            for (uint32_t it = 0; it < ORC_FRAC_ITERS; ++it) {
+                if (it) {
+                    ctr[3] = ORC_FRAC_TAG | (it << 16) | att;
+                    orc_philox4x32_10(ctr, key, b);
+                }
+                uint32_t sl = (uint32_t)(((uint64_t)b[2] * x->nslots) >> 32);
+                if (b[3] < x->fr[sl]) return (int32_t)x->v[sl];
+            }
+        }
+    }
+    return node_sample(x, r);
 }
 
 int32_t orc_sample_words(const orc_graph *g, int32_t u, const uint32_t x[4])
@@ -454,7 +531,7 @@ static uint64_t walk_one(const orc_graph *g, int app, const orc_walk_params *p, 
             for (uint32_t a = 0;; ++a) {
                 ctr[3] = a;
                 orc_philox4x32_10(ctr, key, r);
-                nx = node_sample(x, r);
+                nx = node_draw(g, x, r, key, wid, (uint32_t)j, a);
                 if (nx < 0) break;
                 if ((int64_t)a + 1 >= p->max_attempts) break;
                 uint32_t aux = r[0] & (ORC_AUX_ONE - 1u), thr;
@@ -465,7 +542,7 @@ static uint64_t walk_one(const orc_graph *g, int app, const orc_walk_params *p, 
             }
         } else {
             orc_philox4x32_10(ctr, key, r);
-            nx = node_sample(x, r);
+            nx = node_draw(g, x, r, key, wid, (uint32_t)j, 0);
         }
         if (nx < 0) break;
         row[j] = nx;
@@ -485,6 +562,7 @@ int orc_walk(const orc_graph *g, int app, const orc_walk_params *p, const int32_
     if (!p || p->max_len < 1 || nwalkers < 0 || (nwalkers && !paths) || app < 0 || app > 3)
         return ORC_EINVAL;
     if (app == ORC_NODE2VEC && p->max_attempts < 1) return ORC_EINVAL;
+    if (g->fw && p->max_attempts > ORC_MAX_ATTEMPTS_FLOAT) return ORC_EINVAL;
     for (int64_t i = 0; starts && i < nwalkers; ++i)
         if (starts[i] < 1 || starts[i] > g->n) return ORC_EINVAL;
     for (int64_t i = 0; lens && i < nwalkers; ++i)
@@ -526,9 +604,12 @@ int orc_export(const orc_graph *g, orc_dump *d)
             d->alias[(size_t)u * 32 + j] = x->alias[j];
             for (uint32_t t = 0; t < x->size[j]; ++t) d->members[mo++] = x->mem[j][t];
         }
+        if (d->fmass) d->fmass[u] = x->fmass;
+        if (d->fthr) d->fthr[u] = x->fthr;
         for (uint32_t s = 0; s < x->nslots; ++s) {
             d->slot_v[so] = x->v[s];
             d->slot_w[so] = x->w[s];
+            if (d->slot_f) d->slot_f[so] = x->fr ? x->fr[s] : 0u;
             so++;
             uint32_t w = x->w[s];
             for (uint32_t k = 0; k < (uint32_t)popc32(w); ++k) d->backpos[bo++] = x->back[x->boff[s] + k];

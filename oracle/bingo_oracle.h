@@ -26,7 +26,13 @@
  *   - node2vec neighbour test is exact "candidate in N(prev)" (ref hash is
  *     lossy and tests prev in N(cur), operation.cuh:313, Bingo.cuh:39-58);
  *   - PPR has a per-walker length cap (ref overwrites slot LEN-1);
- *   - batched delete semantics are order-free (see orc_delete).
+ *   - batched delete semantics are order-free (see orc_delete);
+ *   - float weights (ref wtype::Float, Bingo.cuh:65-69,176-177,193-194,
+ *     216-217,254-264): the integer part goes through the 32 bit groups, the
+ *     fraction is kept exactly as a 32-bit fixed-point value per slot; a draw
+ *     first picks the fractional part with probability F / (S + F) from an
+ *     exact 64-bit threshold, then a uniform slot accepted with probability
+ *     equal to its fraction (the reference's rejection on w - floor(w)).
  */
 #ifndef BINGO_ORACLE_H
 #define BINGO_ORACLE_H
@@ -39,6 +45,9 @@ extern "C" {
 #define ORC_AUX_BITS 27
 #define ORC_AUX_ONE (1u << ORC_AUX_BITS)
 #define ORC_MARK 0x80000000u
+#define ORC_FRAC_TAG 0x40000000u   /* Philox counter word 3 of the fractional stream */
+#define ORC_FRAC_ITERS 4096u       /* fractional rejection rounds per draw */
+#define ORC_MAX_ATTEMPTS_FLOAT 65536 /* node2vec cap with float weights (attempt fits 16 bits) */
 
 enum { ORC_DEEPWALK = 0, ORC_NODE2VEC = 1, ORC_PPR = 2, ORC_SAMPLING = 3 };
 enum { ORC_W_CONST1 = 0, ORC_W_UNIFORM16 = 1, ORC_W_LOGUNI32 = 2, ORC_W_ZIPF24 = 3 };
@@ -49,7 +58,9 @@ enum {
     ORC_ENOMEM = -2,
 };
 
-typedef struct { int32_t u, v; uint32_t w; } orc_edge;   /* ref Edges (type.h:24-29), w widened to u32 */
+/* ref Edges (type.h:24-29): w is a u32 weight, or with float weights the
+ * IEEE-754 bits of a float in [2^-32, 2^32) taken to 2^-32 resolution */
+typedef struct { int32_t u, v; uint32_t w; } orc_edge;
 typedef struct { int32_t u, slot; } orc_del;             /* ref Deleted{u,id} (type.h:33-37) */
 
 typedef struct orc_graph orc_graph;
@@ -66,8 +77,9 @@ typedef struct {
     int32_t max_attempts;  /* node2vec rejection cap per step */
 } orc_walk_params;
 
-/* graph lifetime */
+/* graph lifetime; float_weights != 0: edge weights are float bits */
 orc_graph *orc_create(int32_t n, int neighbor_index);
+orc_graph *orc_create_ex(int32_t n, int neighbor_index, int float_weights);
 void orc_destroy(orc_graph *g);
 int32_t orc_num_nodes(const orc_graph *g);
 
@@ -98,6 +110,10 @@ typedef struct {
     uint32_t *slot_w;        /* [total_slots] 0 = tombstone */
     uint64_t *members;       /* [total_members] (v<<32)|slot, node-major, group-major */
     uint32_t *backpos;       /* [total_members] live slots in order, set bits ascending */
+    /* float weights only (may be NULL): */
+    uint32_t *slot_f;        /* [total_slots] fraction of each slot's weight, x 2^32 */
+    uint64_t *fmass;         /* [n+1] sum of the fractions, x 2^32 */
+    uint64_t *fthr;          /* [n+1] fractional-part threshold against a 64-bit coin */
 } orc_dump;
 /* pass all pointers NULL to learn the totals, then again with buffers */
 int orc_export(const orc_graph *g, orc_dump *d);
@@ -105,6 +121,11 @@ int orc_export(const orc_graph *g, orc_dump *d);
 uint32_t orc_edge_count(const orc_graph *g, int32_t u, int32_t v);
 /* alias table of an arbitrary mass vector (tests) */
 void orc_alias_build(const uint64_t mass[32], uint64_t T[32], uint8_t alias[32]);
+/* fractional-part threshold of a node with integer mass S and fraction mass
+ * F (units of 2^-32): floor(2^64 F / (S 2^32 + F)) (tests) */
+uint64_t orc_frac_threshold(uint64_t S, uint64_t F);
+/* integer part and 32-bit fraction of float weight bits; 0 if invalid */
+int orc_split_float(uint32_t bits, uint32_t *ipart, uint32_t *frac);
 /* raw Philox4x32-10 block (tests) */
 void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 /* one sampling draw on node u with explicit random words (tests) */

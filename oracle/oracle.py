@@ -47,6 +47,9 @@ class Dump(C.Structure):
         ("slot_w", C.c_void_p),
         ("members", C.c_void_p),
         ("backpos", C.c_void_p),
+        ("slot_f", C.c_void_p),
+        ("fmass", C.c_void_p),
+        ("fthr", C.c_void_p),
     ]
 
 
@@ -66,6 +69,12 @@ def lib():
         vp, i32, i64, u64 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64
         L.orc_create.restype = vp
         L.orc_create.argtypes = [i32, C.c_int]
+        L.orc_create_ex.restype = vp
+        L.orc_create_ex.argtypes = [i32, C.c_int, C.c_int]
+        L.orc_frac_threshold.restype = u64
+        L.orc_frac_threshold.argtypes = [u64, u64]
+        L.orc_split_float.restype = C.c_int
+        L.orc_split_float.argtypes = [C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.orc_destroy.argtypes = [vp]
         L.orc_insert.argtypes = [vp, vp, i64]
         L.orc_delete.argtypes = [vp, vp, i64]
@@ -163,12 +172,26 @@ class UpdateGen:
             self._h = None
 
 
-class OracleGraph:
-    """Sequential CPU model of the store (n nodes, ids 1..n)."""
+def frac_threshold(S, F):
+    return lib().orc_frac_threshold(S, F)
 
-    def __init__(self, n, neighbor_index=True):
+
+def split_float(bits):
+    """(integer part, fraction x 2^32) of float weight bits, or None."""
+    ip, fr = C.c_uint32(0), C.c_uint32(0)
+    if not lib().orc_split_float(int(bits), C.byref(ip), C.byref(fr)):
+        return None
+    return ip.value, fr.value
+
+
+class OracleGraph:
+    """Sequential CPU model of the store (n nodes, ids 1..n).  With
+    float_weights, the `w` field of edges carries IEEE-754 float bits."""
+
+    def __init__(self, n, neighbor_index=True, float_weights=False):
         self.n = n
-        self._h = lib().orc_create(n, int(neighbor_index))
+        self.float_weights = float_weights
+        self._h = lib().orc_create_ex(n, int(neighbor_index), int(float_weights))
         if not self._h:
             raise ValueError("bad n")
 
@@ -227,6 +250,9 @@ class OracleGraph:
             members=np.zeros(d.total_members, np.uint64),
             backpos=np.zeros(d.total_members, np.uint32),
         )
+        if self.float_weights:
+            out.update(slot_f=np.zeros(d.total_slots, np.uint32), fmass=np.zeros(n1, np.uint64),
+                       fthr=np.zeros(n1, np.uint64))
         for k, a in out.items():
             setattr(d, k, a.ctypes.data)
         lib().orc_export(self._h, C.byref(d))
