@@ -46,6 +46,7 @@ class Config(C.Structure):
         ("min_slots", C.c_int32),
         ("fit_load", C.c_int32),
         ("fit_headroom", C.c_double),
+        ("float_weights", C.c_int32),
     ]
 
 
@@ -95,6 +96,9 @@ class Dump(C.Structure):
         ("slot_w", C.c_void_p),
         ("members", C.c_void_p),
         ("backpos", C.c_void_p),
+        ("slot_f", C.c_void_p),
+        ("fmass", C.c_void_p),
+        ("fthr", C.c_void_p),
     ]
 
 
@@ -229,6 +233,11 @@ def _check(rc):
 
 
 FORM_AUTO, FORM_PLAIN, FORM_PERSISTENT = 0, 1, 2
+
+
+def float_bits(w):
+    """float32 weights -> the uint32 bit patterns a float_weights graph takes as bingo_edge.w"""
+    return np.ascontiguousarray(w, np.float32).view(np.uint32)
 
 
 def walk_params(seed=1, p=0.5, q=2.0, tp=0.0125, max_len=80, max_attempts=4096, form=FORM_AUTO, block=0,
@@ -412,19 +421,24 @@ class Server:
 
 
 class Graph:
-    """One HBM-resident dynamic graph on one GPU (ids 1..n)."""
+    """One HBM-resident dynamic graph on one GPU (ids 1..n).
+
+    float_weights=True: edge weights are the IEEE-754 bits of float32 values
+    (the reference's wtype::Float), see float_bits()."""
 
     def __init__(self, device=0, neighbor_index=True, hbm_budget=0, pool_slack=0.25, min_slots=16, fit_load=False,
-                 fit_headroom=1.0 / 16):
+                 fit_headroom=1.0 / 16, float_weights=False):
         cfg = Config()
         lib().bingo_config_default(C.byref(cfg))
         cfg.device, cfg.neighbor_index, cfg.hbm_budget = device, int(neighbor_index), hbm_budget
         cfg.pool_slack, cfg.min_slots = pool_slack, min_slots
         cfg.fit_load, cfg.fit_headroom = int(fit_load), fit_headroom
+        cfg.float_weights = int(float_weights)
         h = C.c_void_p()
         _check(lib().bingo_create(C.byref(cfg), C.byref(h)))
         self._h = h
         self.n = 0
+        self.float_weights = bool(float_weights)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -546,6 +560,9 @@ class Graph:
             members=np.zeros(d.total_members, np.uint64),
             backpos=np.zeros(d.total_members, np.uint32),
         )
+        if self.float_weights:
+            out.update(slot_f=np.zeros(d.total_slots, np.uint32), fmass=np.zeros(n1, np.uint64),
+                       fthr=np.zeros(n1, np.uint64))
         for k, a in out.items():
             setattr(d, k, a.ctypes.data)
         _check(lib().bingo_export(self._h, C.byref(d)))

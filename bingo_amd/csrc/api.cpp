@@ -207,7 +207,8 @@ static int ensure_batch_scratch(bingo_graph *g, int64_t k, bool del)
         if ((rc = ensure_scratch(g, SC_U64A, k8)) || (rc = ensure_scratch(g, SC_U64B, k8))) return rc;
     } else {
         if ((rc = ensure_scratch(g, SC_KEYS, k4)) || (rc = ensure_scratch(g, SC_VALS, k4)) ||
-            (rc = ensure_scratch(g, SC_EDGES, (size_t)k * sizeof(DevEdge))))
+            (rc = ensure_scratch(g, SC_EDGES, (size_t)k * sizeof(DevEdge))) ||
+            (g->s.fw && (rc = ensure_scratch(g, SC_FRAC, k4))))
             return rc;
     }
     return BINGO_OK;
@@ -220,7 +221,7 @@ static int do_insert(bingo_graph *g, const DevEdge *d_e, int64_t k)
     if (k == 0) return BINGO_OK;
     if ((rc = ensure_batch_scratch(g, k, false))) return rc;
     launch_reset_ctl(g->s.ctl, g->st);
-    launch_insert_keys(d_e, k, g->s.n, scp<uint32_t>(g, SC_KEYS), scp<uint32_t>(g, SC_VALS), g->s.ctl, g->st);
+    launch_insert_keys(d_e, k, g->s.n, g->s.fw, scp<uint32_t>(g, SC_KEYS), scp<uint32_t>(g, SC_VALS), g->s.ctl, g->st);
     const int bits = bits_for((uint64_t)g->s.n);
     uint32_t *k0 = scp<uint32_t>(g, SC_KEYS), *k1 = scp<uint32_t>(g, SC_KEYS2);
     uint32_t *v0 = scp<uint32_t>(g, SC_VALS), *v1 = scp<uint32_t>(g, SC_VALS2);
@@ -228,17 +229,21 @@ static int do_insert(bingo_graph *g, const DevEdge *d_e, int64_t k)
     if ((rc = ensure_scratch(g, SC_TMP, tb))) return rc;
     bool second = false;
     sort_pairs_u32(g->sc[SC_TMP], g->sc_bytes[SC_TMP], k0, k1, v0, v1, k, bits, g->st, &second);
-    launch_gather_edges(d_e, second ? v1 : v0, k, scp<DevEdge>(g, SC_EDGES), g->st);
+    uint32_t *frac = g->s.fw ? scp<uint32_t>(g, SC_FRAC) : nullptr;
+    launch_gather_edges(d_e, second ? v1 : v0, k, scp<DevEdge>(g, SC_EDGES), frac, g->st);
     Runs r;
     if ((rc = encode_runs(g, second ? k1 : k0, k, &r))) return rc;
     launch_insert_plan(g->s, scp<DevEdge>(g, SC_EDGES), r, k, g->st);
     HIPCHK(hipGetLastError());
     if ((rc = read_ctl(g))) return rc;
-    if (g->ctl_host->error & ERR_INVAL) return fail(BINGO_EINVAL, "insert batch has an arc with u or v outside 1..n or w == 0");
+    if (g->ctl_host->error & ERR_INVAL)
+        return fail(BINGO_EINVAL, g->s.fw ? "insert batch has an arc with u or v outside 1..n or a float weight that is "
+                                            "not finite, not in (0, 2^32) or below 2^-32"
+                                          : "insert batch has an arc with u or v outside 1..n or w == 0");
     if (g->ctl_host->error & ERR_NOMEM) return fail(BINGO_ENOMEM, "a node would exceed the per-node slot limit");
     if (g->ctl_host->error) return fail(BINGO_EDEVICE, "store reported an internal error");
     if ((rc = ensure_pools(g))) return rc;
-    launch_insert_apply(g->s, scp<DevEdge>(g, SC_EDGES), r, k, g->st);
+    launch_insert_apply(g->s, scp<DevEdge>(g, SC_EDGES), frac, r, k, g->st);
     HIPCHK(hipGetLastError());
     g->arcs += k;
     return BINGO_OK;
@@ -314,7 +319,7 @@ static int launch_choice(bingo_graph *g, int app, const WalkChoice &c, const Wal
     if (c.form == BINGO_FORM_PLAIN) {
         launch_walk(g->s, app, a, c.block, g->st);
     } else {
-        int nb = persist_max_blocks_per_cu(app, c.wpl, c.depth, c.block);
+        int nb = persist_max_blocks_per_cu(app, c.wpl, c.depth, c.block, g->s.fw);
         if (nb < 1) return fail(BINGO_EINVAL, "persistent walk configuration does not fit a CU");
         if (blocks_per_cu > 0 && blocks_per_cu < nb) nb = blocks_per_cu;
         const int64_t per_block = (int64_t)c.block * c.wpl;
@@ -334,13 +339,16 @@ static uint32_t thr27(double prob)
     return (uint32_t)std::floor(prob * 134217728.0);
 }
 
-int bingo::fill_walk_args(const bingo_walk_params *p, int app, WalkArgs *a)
+int bingo::fill_walk_args(const bingo_walk_params *p, int app, WalkArgs *a, int fw)
 {
     if (!p) return fail(BINGO_EINVAL, "null walk params");
     if (app < 0 || app > 3) return fail(BINGO_EINVAL, "unknown app");
     if (p->max_len < 1) return fail(BINGO_EINVAL, "max_len must be >= 1");
     if (app == BINGO_NODE2VEC && (p->max_attempts < 1 || !(p->p > 0) || !(p->q > 0)))
         return fail(BINGO_EINVAL, "node2vec needs p, q > 0 and max_attempts >= 1");
+    // the fractional stream's counter holds the attempt in 16 bits
+    if (fw && p->max_attempts > kMaxAttemptsFloat)
+        return fail(BINGO_EINVAL, "max_attempts above 65536 with float weights");
     if (app == BINGO_PPR && !(p->tp >= 0 && p->tp <= 1)) return fail(BINGO_EINVAL, "tp must be in [0, 1]");
     // node2vec: accept alpha / max(1/p, 1, 1/q) with alpha = 1/p, 1, 1/q
     // (ref operation.cuh:291,302-325; lim there is max(1/p, 1/q))
@@ -371,6 +379,7 @@ void bingo_config_default(bingo_config *cfg)
     cfg->min_slots = 16;  // ref ORIGINAL (include/api.h:13)
     cfg->fit_load = 0;
     cfg->fit_headroom = 1.0 / 16;
+    cfg->float_weights = 0;
 }
 
 void bingo_walk_params_default(bingo_walk_params *p)
@@ -440,6 +449,7 @@ int bingo_create(const bingo_config *cfg, bingo_graph **out)
     (void)hipMemset(g->s.ctl, 0, sizeof(Ctl));
     g->s.min_slots = g->cfg.min_slots;
     g->s.nbr = g->cfg.neighbor_index ? 1 : 0;
+    g->s.fw = g->cfg.float_weights ? 1 : 0;
     *out = g;
     return BINGO_OK;
 }
@@ -448,7 +458,7 @@ void bingo_destroy(bingo_graph *g)
 {
     if (!g) return;
     if (g->st) (void)hipStreamSynchronize(g->st);
-    void *bufs[] = {g->s.hdr, g->s.grp, g->s.alias, g->s.slot, g->s.back, g->s.mem, g->s.hash, g->s.ctl,
+    void *bufs[] = {g->s.hdr, g->s.grp, g->s.alias, g->s.fnode, g->s.slot, g->s.back, g->s.mem, g->s.hash, g->s.ctl,
                     g->d_steps};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -472,11 +482,13 @@ int bingo_load_device(bingo_graph *g, int32_t n, const bingo_edge *d_edges, int6
     if (m < 0 || (m > 0 && !d_edges)) return fail(BINGO_EINVAL, "bad edge array");
     HIPCHK(hipSetDevice(g->device));
     const size_t n1 = (size_t)n + 1;
-    const size_t nb = n1 * sizeof(NodeHdr) + n1 * kGroups * (sizeof(GroupHdr) + sizeof(AliasEnt));
+    const size_t nb = n1 * sizeof(NodeHdr) + n1 * kGroups * (sizeof(GroupHdr) + sizeof(AliasEnt)) +
+                      (g->s.fw ? n1 * sizeof(FNode) : 0);
     if (nb > g->budget) return fail(BINGO_ENOMEM, "node records exceed the HBM budget");
     bool ok = hipMalloc((void **)&g->s.hdr, n1 * sizeof(NodeHdr)) == hipSuccess &&
               hipMalloc((void **)&g->s.grp, n1 * kGroups * sizeof(GroupHdr)) == hipSuccess &&
-              hipMalloc((void **)&g->s.alias, n1 * kGroups * sizeof(AliasEnt)) == hipSuccess;
+              hipMalloc((void **)&g->s.alias, n1 * kGroups * sizeof(AliasEnt)) == hipSuccess &&
+              (!g->s.fw || hipMalloc((void **)&g->s.fnode, n1 * sizeof(FNode)) == hipSuccess);
     if (!ok) {
         (void)hipGetLastError();
         return fail(BINGO_ENOMEM, "node record allocation failed");
@@ -496,7 +508,8 @@ int bingo_load_device(bingo_graph *g, int32_t n, const bingo_edge *d_edges, int6
         return rc;
     }
     // a load's sort buffers are sized for the whole graph: give them back
-    return release_scratch(g, {SC_KEYS, SC_KEYS2, SC_VALS, SC_VALS2, SC_EDGES, SC_UNIQ, SC_COUNTS, SC_STARTS, SC_TMP});
+    return release_scratch(
+        g, {SC_KEYS, SC_KEYS2, SC_VALS, SC_VALS2, SC_EDGES, SC_UNIQ, SC_COUNTS, SC_STARTS, SC_TMP, SC_FRAC});
 }
 
 int bingo_load(bingo_graph *g, int32_t n, const bingo_edge *edges, int64_t m)
@@ -564,7 +577,7 @@ int bingo_walk_device(bingo_graph *g, int app, const bingo_walk_params *params, 
     if (nwalkers < 0 || (nwalkers > 0 && (!d_paths || !d_steps))) return fail(BINGO_EINVAL, "bad walk buffers");
     if (app == BINGO_NODE2VEC && !g->s.nbr) return fail(BINGO_EUNSUPPORTED, "node2vec needs neighbor_index=1");
     WalkArgs a{};
-    int rc = fill_walk_args(params, app, &a);
+    int rc = fill_walk_args(params, app, &a, g->s.fw);
     if (rc) return rc;
     if (nwalkers == 0) return BINGO_OK;
     a.starts = d_starts;
@@ -643,7 +656,7 @@ int bingo_walk_graph_create(bingo_graph *g, int app, const bingo_walk_params *pa
     bingo_walk_graph *wg = new bingo_walk_graph();
     wg->g = g;
     wg->app = app;
-    int rc = fill_walk_args(params, app, &wg->args);
+    int rc = fill_walk_args(params, app, &wg->args, g->s.fw);
     if (!rc) rc = resolve_walk(g, app, params, &wg->choice);
     if (rc) {
         delete wg;
@@ -692,7 +705,7 @@ int bingo_walk_tune(bingo_graph *g, int app, const bingo_walk_params *params, in
     if (!g || !g->loaded) return fail(BINGO_ESTATE, "graph not loaded");
     if (app == BINGO_NODE2VEC && !g->s.nbr) return fail(BINGO_EUNSUPPORTED, "node2vec needs neighbor_index=1");
     WalkArgs a{};
-    int rc = fill_walk_args(params, app, &a);
+    int rc = fill_walk_args(params, app, &a, g->s.fw);
     if (rc) return rc;
     if (sample < 1) return fail(BINGO_EINVAL, "sample must be >= 1");
     const int32_t n = g->s.n;
@@ -723,7 +736,8 @@ int bingo_walk_tune(bingo_graph *g, int app, const bingo_walk_params *params, in
         for (int w : {1, 2, 4})
             for (int d : {0, 4, 16}) {
                 WalkChoice c{BINGO_FORM_PERSISTENT, b, w, d};
-                if (choice_ok(c, params->max_len) && persist_max_blocks_per_cu(app, w, d, b) > 0) cand.push_back(c);
+                if (choice_ok(c, params->max_len) && persist_max_blocks_per_cu(app, w, d, b, g->s.fw) > 0)
+                    cand.push_back(c);
             }
     float best = 1e30f;
     WalkChoice pick = cand[0];
@@ -870,8 +884,9 @@ int bingo_export(bingo_graph *g, bingo_dump *d)
     const size_t n1 = (size_t)g->s.n + 1, ng = n1 * kGroups;
     std::vector<uint32_t> nsl(n1), bsz(ng);
     void *dn = nullptr, *dm = nullptr, *db = nullptr, *dt = nullptr, *da = nullptr;
+    void *dfm = nullptr, *dft = nullptr;
     auto cleanup = [&]() {
-        for (void *p : {dn, dm, db, dt, da})
+        for (void *p : {dn, dm, db, dt, da, dfm, dft})
             if (p) (void)hipFree(p);
     };
     if (hipMalloc(&dn, n1 * 4) || hipMalloc(&dm, n1 * 8) || hipMalloc(&db, ng * 4) || hipMalloc(&dt, ng * 8) ||
@@ -879,7 +894,12 @@ int bingo_export(bingo_graph *g, bingo_dump *d)
         cleanup();
         return fail(BINGO_ENOMEM, "export buffers");
     }
-    launch_export_a(g->s, (uint32_t *)dn, (uint64_t *)dm, (uint32_t *)db, (uint64_t *)dt, (uint8_t *)da, g->st);
+    if (g->s.fw && (hipMalloc(&dfm, n1 * 8) || hipMalloc(&dft, n1 * 8))) {  // fraction masses, thresholds
+        cleanup();
+        return fail(BINGO_ENOMEM, "export buffers");
+    }
+    launch_export_a(g->s, (uint32_t *)dn, (uint64_t *)dm, (uint32_t *)db, (uint64_t *)dt, (uint8_t *)da,
+                    (uint64_t *)dfm, (uint64_t *)dft, g->st);
     (void)hipMemcpyAsync(nsl.data(), dn, n1 * 4, hipMemcpyDeviceToHost, g->st);
     (void)hipMemcpyAsync(bsz.data(), db, ng * 4, hipMemcpyDeviceToHost, g->st);
     if (hipStreamSynchronize(g->st) != hipSuccess) {
@@ -903,18 +923,23 @@ int bingo_export(bingo_graph *g, bingo_dump *d)
     if (d->mass) (void)hipMemcpyAsync(d->mass, dm, n1 * 8, hipMemcpyDeviceToHost, g->st);
     if (d->T) (void)hipMemcpyAsync(d->T, dt, ng * 8, hipMemcpyDeviceToHost, g->st);
     if (d->alias) (void)hipMemcpyAsync(d->alias, da, ng, hipMemcpyDeviceToHost, g->st);
+    if (g->s.fw && d->fmass) (void)hipMemcpyAsync(d->fmass, dfm, n1 * 8, hipMemcpyDeviceToHost, g->st);
+    if (g->s.fw && d->fthr) (void)hipMemcpyAsync(d->fthr, dft, n1 * 8, hipMemcpyDeviceToHost, g->st);
     if (!d->slot_v || !d->slot_w || !d->members || !d->backpos) {
         const hipError_t e0 = hipStreamSynchronize(g->st);
         cleanup();
         return e0 == hipSuccess ? BINGO_OK : fail(BINGO_EDEVICE, "export");
     }
-    void *dso = nullptr, *dmo = nullptr, *dsv = nullptr, *dsw = nullptr, *dmem = nullptr, *dbp = nullptr;
+    void *dso = nullptr, *dmo = nullptr, *dsv = nullptr, *dsw = nullptr, *dmem = nullptr, *dbp = nullptr,
+         *dsf = nullptr;
     auto cleanup2 = [&]() {
-        for (void *p : {dso, dmo, dsv, dsw, dmem, dbp})
+        for (void *p : {dso, dmo, dsv, dsw, dmem, dbp, dsf})
             if (p) (void)hipFree(p);
     };
+    const bool want_f = g->s.fw && d->slot_f;
     if (hipMalloc(&dso, n1 * 8) || hipMalloc(&dmo, n1 * 8) || hipMalloc(&dsv, ts * 4 + 4) ||
-        hipMalloc(&dsw, ts * 4 + 4) || hipMalloc(&dmem, tm * 8 + 8) || hipMalloc(&dbp, tm * 4 + 4)) {
+        hipMalloc(&dsw, ts * 4 + 4) || hipMalloc(&dmem, tm * 8 + 8) || hipMalloc(&dbp, tm * 4 + 4) ||
+        (want_f && hipMalloc(&dsf, ts * 4 + 4))) {
         cleanup();
         cleanup2();
         return fail(BINGO_ENOMEM, "export buffers");
@@ -922,10 +947,11 @@ int bingo_export(bingo_graph *g, bingo_dump *d)
     (void)hipMemcpyAsync(dso, soff.data(), n1 * 8, hipMemcpyHostToDevice, g->st);
     (void)hipMemcpyAsync(dmo, moff.data(), n1 * 8, hipMemcpyHostToDevice, g->st);
     launch_export_b(g->s, (uint64_t *)dso, (uint64_t *)dmo, (uint32_t *)dsv, (uint32_t *)dsw, (uint64_t *)dmem,
-                    (uint32_t *)dbp, g->st);
+                    (uint32_t *)dbp, (uint32_t *)dsf, g->st);
     if (ts) {
         (void)hipMemcpyAsync(d->slot_v, dsv, ts * 4, hipMemcpyDeviceToHost, g->st);
         (void)hipMemcpyAsync(d->slot_w, dsw, ts * 4, hipMemcpyDeviceToHost, g->st);
+        if (want_f) (void)hipMemcpyAsync(d->slot_f, dsf, ts * 4, hipMemcpyDeviceToHost, g->st);
     }
     if (tm) {
         (void)hipMemcpyAsync(d->members, dmem, tm * 8, hipMemcpyDeviceToHost, g->st);

@@ -13,7 +13,7 @@ namespace bingo {
 // records msg as the calling thread's last error and returns code
 int fail(int code, const std::string &msg);
 // host-side quantisation of walk parameters into kernel thresholds
-int fill_walk_args(const bingo_walk_params *p, int app, WalkArgs *a);
+int fill_walk_args(const bingo_walk_params *p, int app, WalkArgs *a, int fw);
 
 }  // namespace bingo
 
@@ -44,6 +44,7 @@ enum Scratch {
     SC_AUX2,
     SC_AUX3,
     SC_BIG,   // delete batch: long runs and their finished group tasks
+    SC_FRAC,  // insert batch with float weights: fractions in node order
     NSCRATCH
 };
 

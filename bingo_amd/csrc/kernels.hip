@@ -116,6 +116,15 @@ __device__ __forceinline__ uint64_t sparse_incl64(uint64_t v, int ln)
     return acc;
 }
 
+// 64-bit wave sum of values below 2^58 (fraction sums): three DPP sums of
+// 24-, 24- and 10-bit pieces, none of which can overflow 32 bits
+__device__ __forceinline__ uint64_t dpp_sum64(uint64_t v)
+{
+    const uint64_t a = dpp_sum32((uint32_t)v & 0xFFFFFFu), b = dpp_sum32((uint32_t)(v >> 24) & 0xFFFFFFu),
+                   c = dpp_sum32((uint32_t)(v >> 48));
+    return a + (b << 24) + (c << 48);
+}
+
 __device__ __forceinline__ uint32_t pow2_at_least(uint32_t x)
 {
     return x <= 1u ? 1u : (1u << (32 - __clz(x - 1u)));
@@ -209,6 +218,48 @@ __device__ __forceinline__ bool is_neighbor(const uint64_t *__restrict__ hash, u
 }
 
 // ------------------------------------------------------------------------
+// float weights (ref wtype::Float, include/Bingo.cuh:65-69,176-177,193-194,
+// 216-217,254-264): an arc's weight w (IEEE-754 single, 0 < w < 2^32) is
+// split into its integer part, which goes through the bit groups like a u32
+// weight, and its fraction to 2^-32 resolution, kept per slot.  Valid iff
+// finite, in range and not 0 after the floor.  Every step is exact in double
+// precision, so the split is the same on any IEEE-754 machine.
+// ------------------------------------------------------------------------
+__device__ __forceinline__ bool split_float(uint32_t bits, uint32_t &ip, uint32_t &fr)
+{
+    const double d = (double)__uint_as_float(bits);
+    ip = fr = 0u;
+    if (!(d > 0.0) || !(d < 4294967296.0)) return false;
+    const double fl = floor(d);
+    ip = (uint32_t)fl;
+    fr = (uint32_t)floor((d - fl) * 4294967296.0);
+    return ip || fr;
+}
+
+// Fractional-part threshold floor(2^64 F / (S 2^32 + F)) for integer mass
+// S < 2^60 and fraction mass F < 2^60 (units of 2^-32): restoring division,
+// one quotient bit per step, remainder below the divisor (< 2^93).  All-ones
+// for S = 0 (every draw takes the fractional part), 0 for F = 0.
+__device__ uint64_t frac_threshold(uint64_t S, uint64_t F)
+{
+    typedef unsigned __int128 u128;
+    if (!F) return 0ull;
+    if (!S) return ~0ull;
+    const u128 D = ((u128)S << 32) + F;
+    u128 r = F;
+    uint64_t q = 0;
+    for (int i = 0; i < 64; ++i) {
+        r <<= 1;
+        q <<= 1;
+        if (r >= D) {
+            r -= D;
+            q |= 1ull;
+        }
+    }
+    return q;
+}
+
+// ------------------------------------------------------------------------
 // alias table: integer-exact Vose over 32 fixed slots, built by one wave.
 // Lane j holds group j's size and member offset.  Masses M_j = size_j << j,
 // s_j = 32 M_j against slot capacity S = sum M_j; pair the highest small
@@ -249,7 +300,8 @@ __device__ __forceinline__ uint64_t frac64(uint64_t tau, uint64_t S)
     return q;
 }
 
-__device__ void alias_rebuild(const DevStore &s, int32_t u, uint32_t size, uint64_t off, uint64_t hmeta, int ln)
+// Returns the node's integer mass S (all lanes).
+__device__ uint64_t alias_rebuild(const DevStore &s, int32_t u, uint32_t size, uint64_t off, uint64_t hmeta, int ln)
 {
     const bool own = ln < kGroups;
     const uint64_t M = own ? ((uint64_t)size << ln) : 0ull;
@@ -290,6 +342,18 @@ __device__ void alias_rebuild(const DevStore &s, int32_t u, uint32_t size, uint6
         e.alias = off_a | ((uint64_t)size_a << kOffBits);
         e.meta = (uint64_t)ali | (hmeta << 5);
         s.alias[(size_t)u * kGroups + ln] = e;
+    }
+    return S;
+}
+
+// float weights: a node's fraction mass and threshold after an update
+__device__ __forceinline__ void store_fnode(const DevStore &s, int32_t u, uint64_t S, uint64_t F, int ln)
+{
+    if (ln == 0) {
+        FNode f;
+        f.F = F;
+        f.thr = frac_threshold(S, F);
+        s.fnode[u] = f;
     }
 }
 
@@ -333,26 +397,42 @@ __global__ void k_init_nodes(DevStore s)
     if (j == 0) {
         NodeHdr h = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         s.hdr[i >> 5] = h;
+        if (s.fw) {
+            FNode f = {0, 0};
+            s.fnode[i >> 5] = f;
+        }
     }
 }
 
-__global__ void k_insert_keys(const DevEdge *__restrict__ e, int64_t k, int32_t n, uint32_t *__restrict__ keys,
+__global__ void k_insert_keys(const DevEdge *__restrict__ e, int64_t k, int32_t n, int fw, uint32_t *__restrict__ keys,
                               uint32_t *__restrict__ vals, Ctl *ctl)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= k) return;
     const DevEdge x = e[i];
-    const bool bad = x.u < 1 || x.u > n || x.v < 1 || x.v > n || x.w == 0u;
+    uint32_t ip, fr;
+    const bool wbad = fw ? !split_float(x.w, ip, fr) : x.w == 0u;
+    const bool bad = x.u < 1 || x.u > n || x.v < 1 || x.v > n || wbad;
     if (bad) atomicOr(&ctl->error, (unsigned)ERR_INVAL);
     keys[i] = bad ? 0u : (uint32_t)x.u;
     vals[i] = (uint32_t)i;
 }
 
+// Arcs in node order.  With float weights (frac != null) the gathered arc
+// carries the integer part as its weight and frac[i] its fraction.
 __global__ void k_gather_edges(const DevEdge *__restrict__ e, const uint32_t *__restrict__ order, int64_t k,
-                               DevEdge *__restrict__ out)
+                               DevEdge *__restrict__ out, uint32_t *__restrict__ frac)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i < k) out[i] = e[order[i]];
+    if (i >= k) return;
+    DevEdge x = e[order[i]];
+    if (frac) {
+        uint32_t ip, fr;
+        split_float(x.w, ip, fr);
+        x.w = ip;
+        frac[i] = fr;
+    }
+    out[i] = x;
 }
 
 // ------------------------------------------------------------------------
@@ -436,7 +516,7 @@ __global__ __launch_bounds__(256) void k_insert_plan(DevStore s, const DevEdge *
             if (ln == 0) atomicOr(&s.ctl->error, (unsigned)ERR_NOMEM);
             continue;
         }
-        need[POOL_SLOT] += G.g_slots ? 3ull * G.cap_slots : 0;
+        need[POOL_SLOT] += G.g_slots ? (uint64_t)slot_arrays(s.fw) * G.cap_slots : 0;
         need[POOL_BACK] += G.g_back ? G.cap_back : 0;
         need[POOL_HASH] += G.g_hash ? G.cap_hash : 0;
         need[POOL_MEM] += sparse_sum64(G.g_g ? (uint64_t)G.cap_g : 0ull);
@@ -458,7 +538,9 @@ __device__ __forceinline__ uint64_t bump(const DevStore &s, int pool, uint64_t n
     return at;
 }
 
-__device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, uint32_t cnt, int32_t u, int ln)
+// `fr` (float weights only) holds the run's fractions, arc for arc.
+__device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, const uint32_t *__restrict__ fr,
+                           uint32_t cnt, int32_t u, int ln)
 {
     NodeHdr h = s.hdr[u];
     GroupHdr gh = {0, 0, 0};
@@ -473,7 +555,7 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, uin
     const uint64_t gtotal = rl64(gincl, 63);
     uint64_t gbase = 0;
     if (ln == 0) {
-        if (G.g_slots) a_slot = bump(s, POOL_SLOT, 3ull * G.cap_slots);
+        if (G.g_slots) a_slot = bump(s, POOL_SLOT, (uint64_t)slot_arrays(s.fw) * G.cap_slots);
         if (G.g_back) a_back = bump(s, POOL_BACK, G.cap_back);
         if (G.g_hash) a_hash = bump(s, POOL_HASH, G.cap_hash);
         if (gtotal) gbase = bump(s, POOL_MEM, gtotal);
@@ -489,11 +571,13 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, uin
     uint32_t *nv = s.slot + a_slot;
     uint32_t *nw = nv + G.cap_slots;
     uint32_t *nb = nv + 2ull * G.cap_slots;
+    uint32_t *nf = nv + 3ull * G.cap_slots;  // float weights only
     if (G.g_slots && n0) {
         const uint32_t *ov = s.slot + h.slot_off;
         wave_copy(nv, ov, n0, ln);
         wave_copy(nw, ov + h.slot_cap, n0, ln);
         wave_copy(nb, ov + 2ull * h.slot_cap, n0, ln);
+        if (s.fw) wave_copy(nf, ov + 3ull * h.slot_cap, n0, ln);
     }
     uint32_t *back = s.back + a_back;
     if (G.g_back && h.back_used) wave_copy(back, s.back + h.back_off, h.back_used, ln);
@@ -523,6 +607,7 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, uin
     uint32_t szj = gh.size;
     uint32_t boff_run = h.back_used;
     uint32_t fresh = 0;
+    uint64_t fsum = 0;
     for (uint32_t c = 0; c < cnt; c += 64) {
         const uint32_t i = c + (uint32_t)ln;
         const bool act = i < cnt;
@@ -536,6 +621,11 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, uin
             nv[slot] = (uint32_t)x.v;
             nw[slot] = x.w;
             nb[slot] = my_boff;
+            if (s.fw) {
+                const uint32_t f = fr[i];
+                nf[slot] = f;
+                fsum += f;
+            }
         }
         const uint64_t lt = (1ull << ln) - 1ull;
         uint32_t rank = 0;
@@ -582,12 +672,14 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, uin
         }
         s.hdr[u] = h;
     }
-    alias_rebuild(s, u, ln < kGroups ? szj : 0u, goff, s.nbr ? make_hmeta(a_hash, G.cap_hash) : 0ull, ln);
+    const uint64_t S = alias_rebuild(s, u, ln < kGroups ? szj : 0u, goff, s.nbr ? make_hmeta(a_hash, G.cap_hash) : 0ull, ln);
+    if (s.fw) store_fnode(s, u, S, s.fnode[u].F + dpp_sum64(fsum), ln);
 }
 
 // Runs by grid stride in node order (neighbouring runs touch neighbouring
 // node records; a shared work counter measured slower).
-__global__ __launch_bounds__(256) void k_insert_apply(DevStore s, const DevEdge *__restrict__ E, Runs r)
+__global__ __launch_bounds__(256) void k_insert_apply(DevStore s, const DevEdge *__restrict__ E,
+                                                      const uint32_t *__restrict__ FR, Runs r)
 {
     const int ln = lane_id();
     const uint32_t nruns = *r.nruns;
@@ -596,7 +688,7 @@ __global__ __launch_bounds__(256) void k_insert_apply(DevStore s, const DevEdge 
     for (uint32_t ri = wave; ri < nruns; ri += nwaves) {
         const int32_t u = (int32_t)r.node[ri];
         if (u < 1 || u > s.n) continue;
-        insert_run(s, E + r.start[ri], r.count[ri], u, ln);
+        insert_run(s, E + r.start[ri], FR ? FR + r.start[ri] : nullptr, r.count[ri], u, ln);
     }
 }
 
@@ -621,7 +713,9 @@ __global__ void k_delete_check(DevStore s, const unsigned long long *__restrict_
     bool bad = u < 1 || u > s.n || (i > 0 && keys[i - 1] == key);
     if (!bad) {
         const NodeHdr h = s.hdr[u];
-        bad = sl >= h.nslots || s.slot[h.slot_off + h.slot_cap + sl] == 0u;
+        // live: a weight, or with float weights a weight or a fraction
+        bad = sl >= h.nslots ||
+              (s.slot[h.slot_off + h.slot_cap + sl] == 0u && !(s.fw && s.slot[h.slot_off + 3ull * h.slot_cap + sl]));
     }
     if (bad) atomicOr(&s.ctl->error, (unsigned)ERR_INVAL);
     us[i] = bad ? 0u : (uint32_t)u;
@@ -711,6 +805,7 @@ __device__ void delete_run(const DevStore &s, const uint32_t *__restrict__ slots
 
     // pass 2: fill holes; then the slots' neighbour counts and tombstones
     uint32_t hr = 0;
+    uint64_t fsub = 0;
     bool bad = false;
     for (uint32_t c0 = 0; c0 < cnt; c0 += 64) {
         if (!single) stage(c0);
@@ -748,11 +843,17 @@ __device__ void delete_run(const DevStore &s, const uint32_t *__restrict__ slots
         if (c0 + (uint32_t)ln < cnt) {
             if (hashed) hash_dec(s.hash + h.hash_off, h.hash_cap, cv);
             W[csl] = 0u;
+            if (s.fw) {
+                uint32_t *FR = V + 3ull * h.slot_cap;
+                fsub += FR[csl];
+                FR[csl] = 0u;
+            }
         }
     }
     if (bad) atomicOr(&s.ctl->error, (unsigned)ERR_OVERRUN);
     if (kj) s.grp[(size_t)u * kGroups + ln].size = S2;
-    alias_rebuild(s, u, own ? S2 : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, ln);
+    const uint64_t Sm = alias_rebuild(s, u, own ? S2 : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, ln);
+    if (s.fw) store_fnode(s, u, Sm, s.fnode[u].F - dpp_sum64(fsub), ln);
 }
 
 // A run of at most kDB deletes: the same result as delete_run in fewer
@@ -843,11 +944,18 @@ __device__ void delete_small(const DevStore &s, const uint32_t *__restrict__ slo
         if (hole[q]) back[mb[q] + __popc(mw[q] & below)] = pos[q];
     if (bad) atomicOr(&s.ctl->error, (unsigned)ERR_OVERRUN);
     if (kj) s.grp[(size_t)u * kGroups + ln].size = S2;
+    uint64_t fsub = 0;
     if (in) {
         if (hashed) hash_dec(s.hash + h.hash_off, h.hash_cap, cv);
         W[t_sl] = 0u;
+        if (s.fw) {
+            uint32_t *FR = V + 3ull * h.slot_cap;
+            fsub = FR[t_sl];
+            FR[t_sl] = 0u;
+        }
     }
-    alias_rebuild(s, u, own ? S2 : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, ln);
+    const uint64_t Sm = alias_rebuild(s, u, own ? S2 : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, ln);
+    if (s.fw) store_fnode(s, u, Sm, s.fnode[u].F - dpp_sum64(fsub), ln);
 }
 
 // One bit group j of a long delete run, by one wave with lanes across the
@@ -919,15 +1027,23 @@ __device__ void delete_finish(const DevStore &s, const uint32_t *__restrict__ sl
     const NodeHdr h = s.hdr[u];
     uint32_t *V = s.slot + h.slot_off;
     uint32_t *W = V + h.slot_cap;
+    uint32_t *FR = V + 3ull * h.slot_cap;  // float weights only
     const bool hashed = s.nbr && h.hash_cap;
+    uint64_t fsub = 0;
     for (uint32_t t = ln; t < cnt; t += 64) {
         const uint32_t sl = slots[t];
         if (hashed) hash_dec(s.hash + h.hash_off, h.hash_cap, V[sl]);
         W[sl] = 0u;
+        if (s.fw) {
+            fsub += FR[sl];
+            FR[sl] = 0u;
+        }
     }
     GroupHdr gh = {0, 0, 0};
     if (ln < kGroups) gh = s.grp[(size_t)u * kGroups + ln];
-    alias_rebuild(s, u, ln < kGroups ? gh.size : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, ln);
+    const uint64_t Sm =
+        alias_rebuild(s, u, ln < kGroups ? gh.size : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, ln);
+    if (s.fw) store_fnode(s, u, Sm, s.fnode[u].F - dpp_sum64(fsub), ln);
 }
 
 __global__ void k_delete_big_list(Runs r, uint32_t *__restrict__ big, Ctl *ctl)
@@ -999,7 +1115,41 @@ __device__ __forceinline__ int32_t sample_step(const AliasEnt *__restrict__ alia
     return (int32_t)reinterpret_cast<const uint32_t *>(mem + (loc & kOffMask36) + pos)[1];
 }
 
-template <int APP>
+// One draw on `cur` with float weights (NodeData::sample's wtype::Float
+// branch, include/Bingo.cuh:254-264).  With probability F / (S + F) -- the
+// 64-bit coin of the fractional stream (counter word 3 = kFracTag | attempt)
+// below the node's threshold, or no integer mass -- the fractional part: a
+// uniform slot accepted when a 32-bit uniform is below its fraction, round
+// `it` drawing from counter word 3 = kFracTag | it << 16 | attempt, up to
+// kFracIters rounds.  Otherwise, or when every round rejects, the integer
+// draw from `r`.  FW false compiles to sample_step alone.
+template <bool FW>
+__device__ __forceinline__ int32_t draw(const DevStore &s, int32_t cur, uint4 r, uint64_t &hm, uint32_t w0, uint32_t w1,
+                                        uint32_t step, uint32_t att, uint32_t k0, uint32_t k1)
+{
+    if (FW) {
+        const uint64_t thr = s.fnode[cur].thr;
+        if (thr) {
+            uint4 b = philox(w0, w1, step, kFracTag | att, k0, k1);
+            const uint64_t coin = ((uint64_t)b.x << 32) | b.y;
+            if (thr == ~0ull || coin < thr) {
+                const NodeHdr h = s.hdr[cur];
+                const uint32_t *V = s.slot + h.slot_off, *FR = V + 3ull * h.slot_cap;
+                for (uint32_t it = 0; it < kFracIters; ++it) {
+                    if (it) b = philox(w0, w1, step, kFracTag | (it << 16) | att, k0, k1);
+                    const uint32_t sl = __umulhi(b.z, h.nslots);
+                    if (b.w < FR[sl]) {
+                        hm = make_hmeta(h.hash_off, h.hash_cap);
+                        return (int32_t)V[sl];
+                    }
+                }
+            }
+        }
+    }
+    return sample_step(s.alias, s.mem, cur, r, hm);
+}
+
+template <int APP, bool FW>
 __global__ __launch_bounds__(256) void k_walk(DevStore s, WalkArgs a)
 {
     const int64_t wid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -1026,7 +1176,7 @@ __global__ __launch_bounds__(256) void k_walk(DevStore s, WalkArgs a)
                 if (APP == 1 && j >= 2) {
                     for (uint32_t at = 0;; ++at) {
                         r = philox(w0, w1, (uint32_t)j, at, k0, k1);
-                        nx = sample_step(s.alias, s.mem, cur, r, hcur);
+                        nx = draw<FW>(s, cur, r, hcur, w0, w1, (uint32_t)j, at, k0, k1);
                         ++draws;
                         if (nx < 0 || (int64_t)at + 1 >= a.max_attempts) break;
                         const uint32_t aux = r.x & kAuxMask;
@@ -1042,7 +1192,7 @@ __global__ __launch_bounds__(256) void k_walk(DevStore s, WalkArgs a)
                     }
                 } else {
                     r = philox(w0, w1, (uint32_t)j, 0u, k0, k1);
-                    nx = sample_step(s.alias, s.mem, cur, r, hcur);
+                    nx = draw<FW>(s, cur, r, hcur, w0, w1, (uint32_t)j, 0u, k0, k1);
                     ++draws;
                 }
                 if (nx < 0) break;
@@ -1123,7 +1273,7 @@ struct PathOut {
     }
 };
 
-template <int APP, int WPL, int D>
+template <int APP, int WPL, int D, bool FW>
 __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
 {
     extern __shared__ int32_t lds_stage[];
@@ -1202,9 +1352,10 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
         for (int k = 0; k < WPL; ++k) {
             if (wid[k] < 0) continue;
             const int64_t w = wid[k];
-            const uint4 r = philox((uint32_t)w, (uint32_t)((uint64_t)w >> 32), (uint32_t)jj[k], att[k], k0, k1);
+            const uint32_t w0 = (uint32_t)w, w1 = (uint32_t)((uint64_t)w >> 32);
+            const uint4 r = philox(w0, w1, (uint32_t)jj[k], att[k], k0, k1);
             uint64_t hcur;
-            const int32_t nx = sample_step(s.alias, s.mem, cur[k], r, hcur);
+            const int32_t nx = draw<FW>(s, cur[k], r, hcur, w0, w1, (uint32_t)jj[k], att[k], k0, k1);
             ++draws;
             if (nx < 0) {
                 out.finish(a, k, w, jj[k]);
@@ -1264,6 +1415,7 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
 // ------------------------------------------------------------------------
 constexpr int kServeWPL = 2;
 
+template <bool FW>
 __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *__restrict__ reqs, int nreq,
                                                     unsigned long long total, unsigned long long *cursor)
 {
@@ -1358,10 +1510,11 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
             const ServeReq &q = rq[rid[k]];
             const int64_t w = wid[k];
             int32_t *row = q.paths + w * (int64_t)q.max_len;
-            const uint4 r = philox((uint32_t)w, (uint32_t)((uint64_t)w >> 32), (uint32_t)jj[k], att[k],
-                                   (uint32_t)q.seed, (uint32_t)(q.seed >> 32));
+            const uint32_t w0 = (uint32_t)w, w1 = (uint32_t)((uint64_t)w >> 32);
+            const uint32_t k0 = (uint32_t)q.seed, k1 = (uint32_t)(q.seed >> 32);
+            const uint4 r = philox(w0, w1, (uint32_t)jj[k], att[k], k0, k1);
             uint64_t hcur;
-            const int32_t nx = sample_step(s.alias, s.mem, cur[k], r, hcur);
+            const int32_t nx = draw<FW>(s, cur[k], r, hcur, w0, w1, (uint32_t)jj[k], att[k], k0, k1);
             bool done = nx < 0;
             if (!done) {
                 const uint32_t aux = r.x & kAuxMask;
@@ -1400,7 +1553,8 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
 // ------------------------------------------------------------------------
 // export and point queries
 // ------------------------------------------------------------------------
-__global__ void k_export_a(DevStore s, uint32_t *nslots, uint64_t *mass, uint32_t *bsize, uint64_t *T, uint8_t *alias)
+__global__ void k_export_a(DevStore s, uint32_t *nslots, uint64_t *mass, uint32_t *bsize, uint64_t *T, uint8_t *alias,
+                           uint64_t *fmass, uint64_t *fthr)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int64_t total = ((int64_t)s.n + 1) * kGroups;
@@ -1417,12 +1571,17 @@ __global__ void k_export_a(DevStore s, uint32_t *nslots, uint64_t *mass, uint32_
         uint64_t acc = 0;
         for (int g = 0; g < kGroups; ++g) acc += (uint64_t)s.grp[u * kGroups + g].size << g;
         mass[u] = acc;
+        if (fmass) {
+            const FNode f = s.fnode[u];
+            fmass[u] = f.F;
+            fthr[u] = f.thr;
+        }
     }
 }
 
 __global__ __launch_bounds__(256) void k_export_b(DevStore s, const uint64_t *slot_out_off, const uint64_t *mem_out_off,
                                                   uint32_t *slot_v, uint32_t *slot_w, uint64_t *members,
-                                                  uint32_t *backpos)
+                                                  uint32_t *backpos, uint32_t *slot_f)
 {
     const int ln = lane_id();
     const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
@@ -1434,6 +1593,7 @@ __global__ __launch_bounds__(256) void k_export_b(DevStore s, const uint64_t *sl
     for (uint32_t i = ln; i < h.nslots; i += 64) {
         slot_v[so + i] = V[i];
         slot_w[so + i] = W[i];
+        if (slot_f) slot_f[so + i] = V[3ull * h.slot_cap + i];
     }
     uint64_t mo = mem_out_off[u];
     for (int j = 0; j < kGroups; ++j) {
@@ -1475,8 +1635,9 @@ __global__ void k_edge_count(DevStore s, const int32_t *us, const int32_t *vs, i
                 }
             }
         } else {
-            const uint32_t *V = s.slot + h.slot_off, *W = V + h.slot_cap;
-            for (uint32_t t = 0; t < h.nslots; ++t) c += (W[t] != 0u && V[t] == (uint32_t)v);
+            const uint32_t *V = s.slot + h.slot_off, *W = V + h.slot_cap, *FR = V + 3ull * h.slot_cap;
+            for (uint32_t t = 0; t < h.nslots; ++t)
+                c += ((W[t] != 0u || (s.fw && FR[t] != 0u)) && V[t] == (uint32_t)v);
         }
     }
     counts[i] = c;
@@ -1522,15 +1683,16 @@ void launch_init_nodes(DevStore s, hipStream_t st)
     hipLaunchKernelGGL(k_init_nodes, dim3(blocks_for(total, 256)), dim3(256), 0, st, s);
 }
 
-void launch_insert_keys(const DevEdge *e, int64_t k, int32_t n, uint32_t *keys, uint32_t *vals, Ctl *ctl,
+void launch_insert_keys(const DevEdge *e, int64_t k, int32_t n, int fw, uint32_t *keys, uint32_t *vals, Ctl *ctl,
                         hipStream_t st)
 {
-    hipLaunchKernelGGL(k_insert_keys, dim3(blocks_for(k, 256)), dim3(256), 0, st, e, k, n, keys, vals, ctl);
+    hipLaunchKernelGGL(k_insert_keys, dim3(blocks_for(k, 256)), dim3(256), 0, st, e, k, n, fw, keys, vals, ctl);
 }
 
-void launch_gather_edges(const DevEdge *e, const uint32_t *order, int64_t k, DevEdge *out, hipStream_t st)
+void launch_gather_edges(const DevEdge *e, const uint32_t *order, int64_t k, DevEdge *out, uint32_t *frac,
+                         hipStream_t st)
 {
-    hipLaunchKernelGGL(k_gather_edges, dim3(blocks_for(k, 256)), dim3(256), 0, st, e, order, k, out);
+    hipLaunchKernelGGL(k_gather_edges, dim3(blocks_for(k, 256)), dim3(256), 0, st, e, order, k, out, frac);
 }
 
 void launch_insert_plan(DevStore s, const DevEdge *sorted, Runs r, int64_t k, hipStream_t st)
@@ -1538,9 +1700,9 @@ void launch_insert_plan(DevStore s, const DevEdge *sorted, Runs r, int64_t k, hi
     hipLaunchKernelGGL(k_insert_plan, dim3(run_blocks(k)), dim3(256), 0, st, s, sorted, r);
 }
 
-void launch_insert_apply(DevStore s, const DevEdge *sorted, Runs r, int64_t k, hipStream_t st)
+void launch_insert_apply(DevStore s, const DevEdge *sorted, const uint32_t *frac, Runs r, int64_t k, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_insert_apply, dim3(run_blocks(k)), dim3(256), 0, st, s, sorted, r);
+    hipLaunchKernelGGL(k_insert_apply, dim3(run_blocks(k)), dim3(256), 0, st, s, sorted, frac, r);
 }
 
 void launch_delete_keys(const int32_t *dels, int64_t k, unsigned long long *keys, hipStream_t st)
@@ -1569,19 +1731,29 @@ void launch_walk(DevStore s, int app, const WalkArgs &a, int block, hipStream_t 
 {
     if (block != 64 && block != 128 && block != 256) block = 256;
     const dim3 grid(blocks_for(a.nwalkers, block)), blk(block);
+    if (s.fw) {
+        switch (app) {
+        case 0: hipLaunchKernelGGL((k_walk<0, true>), grid, blk, 0, st, s, a); break;
+        case 1: hipLaunchKernelGGL((k_walk<1, true>), grid, blk, 0, st, s, a); break;
+        case 2: hipLaunchKernelGGL((k_walk<2, true>), grid, blk, 0, st, s, a); break;
+        default: hipLaunchKernelGGL((k_walk<3, true>), grid, blk, 0, st, s, a); break;
+        }
+        return;
+    }
     switch (app) {
-    case 0: hipLaunchKernelGGL(k_walk<0>, grid, blk, 0, st, s, a); break;
-    case 1: hipLaunchKernelGGL(k_walk<1>, grid, blk, 0, st, s, a); break;
-    case 2: hipLaunchKernelGGL(k_walk<2>, grid, blk, 0, st, s, a); break;
-    default: hipLaunchKernelGGL(k_walk<3>, grid, blk, 0, st, s, a); break;
+    case 0: hipLaunchKernelGGL((k_walk<0, false>), grid, blk, 0, st, s, a); break;
+    case 1: hipLaunchKernelGGL((k_walk<1, false>), grid, blk, 0, st, s, a); break;
+    case 2: hipLaunchKernelGGL((k_walk<2, false>), grid, blk, 0, st, s, a); break;
+    default: hipLaunchKernelGGL((k_walk<3, false>), grid, blk, 0, st, s, a); break;
     }
 }
 
-int serve_max_blocks_per_cu()
+int serve_max_blocks_per_cu(int fw)
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void *>(&k_walk_serve), 256, 0) !=
-        hipSuccess) {
+    const void *fn = fw ? reinterpret_cast<const void *>(&k_walk_serve<true>)
+                        : reinterpret_cast<const void *>(&k_walk_serve<false>);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, 0) != hipSuccess) {
         (void)hipGetLastError();
         return 1;
     }
@@ -1591,7 +1763,10 @@ int serve_max_blocks_per_cu()
 void launch_walk_serve(DevStore s, const ServeReq *reqs, int nreq, unsigned long long total,
                        unsigned long long *cursor, int blocks, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_walk_serve, dim3(blocks), dim3(256), 0, st, s, reqs, nreq, total, cursor);
+    if (s.fw)
+        hipLaunchKernelGGL(k_walk_serve<true>, dim3(blocks), dim3(256), 0, st, s, reqs, nreq, total, cursor);
+    else
+        hipLaunchKernelGGL(k_walk_serve<false>, dim3(blocks), dim3(256), 0, st, s, reqs, nreq, total, cursor);
 }
 
 size_t persist_lds_bytes(int block, int wpl, int depth)
@@ -1599,10 +1774,10 @@ size_t persist_lds_bytes(int block, int wpl, int depth)
     return depth ? (size_t)block * wpl * (depth + 1) * sizeof(int32_t) : 0;
 }
 
-template <int APP>
+template <int APP, bool FW>
 static const void *persist_fn(int wpl, int depth)
 {
-#define BINGO_PF(W, D) reinterpret_cast<const void *>(&k_walk_persist<APP, W, D>)
+#define BINGO_PF(W, D) reinterpret_cast<const void *>(&k_walk_persist<APP, W, D, FW>)
     switch (wpl * 100 + depth) {
     case 100: return BINGO_PF(1, 0);
     case 104: return BINGO_PF(1, 4);
@@ -1618,19 +1793,27 @@ static const void *persist_fn(int wpl, int depth)
 #undef BINGO_PF
 }
 
-static const void *persist_kernel(int app, int wpl, int depth)
+static const void *persist_kernel(int app, int wpl, int depth, int fw)
 {
+    if (fw) {
+        switch (app) {
+        case 0: return persist_fn<0, true>(wpl, depth);
+        case 1: return persist_fn<1, true>(wpl, depth);
+        case 2: return persist_fn<2, true>(wpl, depth);
+        default: return persist_fn<3, true>(wpl, depth);
+        }
+    }
     switch (app) {
-    case 0: return persist_fn<0>(wpl, depth);
-    case 1: return persist_fn<1>(wpl, depth);
-    case 2: return persist_fn<2>(wpl, depth);
-    default: return persist_fn<3>(wpl, depth);
+    case 0: return persist_fn<0, false>(wpl, depth);
+    case 1: return persist_fn<1, false>(wpl, depth);
+    case 2: return persist_fn<2, false>(wpl, depth);
+    default: return persist_fn<3, false>(wpl, depth);
     }
 }
 
-int persist_max_blocks_per_cu(int app, int wpl, int depth, int block)
+int persist_max_blocks_per_cu(int app, int wpl, int depth, int block, int fw)
 {
-    const void *fn = persist_kernel(app, wpl, depth);
+    const void *fn = persist_kernel(app, wpl, depth, fw);
     if (!fn) return 0;
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, persist_lds_bytes(block, wpl, depth)) !=
@@ -1643,25 +1826,26 @@ int persist_max_blocks_per_cu(int app, int wpl, int depth, int block)
 
 void launch_walk_persist(DevStore s, int app, const WalkArgs &a, const PersistCfg &c, hipStream_t st)
 {
-    const void *fn = persist_kernel(app, c.wpl, c.depth);
+    const void *fn = persist_kernel(app, c.wpl, c.depth, s.fw);
     if (!fn) return;
     void *args[] = {&s, const_cast<WalkArgs *>(&a)};
     (void)hipLaunchKernel(fn, dim3(c.blocks), dim3(c.block), args, persist_lds_bytes(c.block, c.wpl, c.depth), st);
 }
 
 void launch_export_a(DevStore s, uint32_t *nslots, uint64_t *mass, uint32_t *bsize, uint64_t *T, uint8_t *alias,
-                     hipStream_t st)
+                     uint64_t *fmass, uint64_t *fthr, hipStream_t st)
 {
     const int64_t total = ((int64_t)s.n + 1) * kGroups;
-    hipLaunchKernelGGL(k_export_a, dim3(blocks_for(total, 256)), dim3(256), 0, st, s, nslots, mass, bsize, T, alias);
+    hipLaunchKernelGGL(k_export_a, dim3(blocks_for(total, 256)), dim3(256), 0, st, s, nslots, mass, bsize, T, alias,
+                       fmass, fthr);
 }
 
 void launch_export_b(DevStore s, const uint64_t *slot_out_off, const uint64_t *mem_out_off, uint32_t *slot_v,
-                     uint32_t *slot_w, uint64_t *members, uint32_t *backpos, hipStream_t st)
+                     uint32_t *slot_w, uint64_t *members, uint32_t *backpos, uint32_t *slot_f, hipStream_t st)
 {
     const int64_t waves = (int64_t)s.n + 1;
     hipLaunchKernelGGL(k_export_b, dim3(blocks_for(waves * 64, 256)), dim3(256), 0, st, s, slot_out_off, mem_out_off,
-                       slot_v, slot_w, members, backpos);
+                       slot_v, slot_w, members, backpos, slot_f);
 }
 
 void launch_edge_count(DevStore s, const int32_t *us, const int32_t *vs, int64_t k, uint32_t *counts, hipStream_t st)

@@ -439,7 +439,7 @@ int bingo_server_create(bingo_graph *g, const bingo_server_config *c, bingo_serv
         return fail(BINGO_EINVAL, "server arena, walker budget or stream count out of range");
     }
     sv->nstreams = sv->cfg.streams ? sv->cfg.streams : 2;
-    const int occ = serve_max_blocks_per_cu();
+    const int occ = serve_max_blocks_per_cu(g->s.fw);
     sv->blocks_per_cu = sv->cfg.blocks_per_cu > 0 ? std::min(sv->cfg.blocks_per_cu, occ) : occ;
     (void)hipSetDevice(g->device);
     bool ok = hipMalloc((void **)&sv->arena.base, sv->cfg.arena_bytes) == hipSuccess &&
@@ -472,7 +472,7 @@ int bingo_server_submit_walk(bingo_server *sv, int app, const bingo_walk_params 
     if (!sv) return fail(BINGO_EINVAL, "null server");
     bingo_graph *g = sv->g;
     auto r = std::make_shared<Req>();
-    int rc = fill_walk_args(params, app, &r->args);
+    int rc = fill_walk_args(params, app, &r->args, g->s.fw);
     if (rc) return rc;
     if (nwalkers < 1 || !paths) return fail(BINGO_EINVAL, "bad walk request");
     if (app == BINGO_NODE2VEC && !g->s.nbr) return fail(BINGO_EUNSUPPORTED, "node2vec needs neighbor_index=1");

@@ -7,7 +7,8 @@
 //                          size) of group i and of its alias partner, so one
 //                          32-byte read decides the group of a walk step
 // Pools (one device allocation each, regions handed out by atomic bump):
-//   slot  u32: per node [v x cap | w x cap | back-offset x cap]
+//   slot  u32: per node [v x cap | w x cap | back-offset x cap], with float
+//              weights also [fraction x cap] (w holds the integer part)
 //   back  u32: per slot, the member positions of its set bits, ascending
 //   mem   u64: member entries (v << 32) | slot
 //   hash  u64: neighbour multiset entries key | count << 32 (linear probing)
@@ -44,6 +45,18 @@ struct GroupHdr {
     uint32_t size, cap;
 };
 
+// Float weights only: a node's fraction mass F (sum of the slots' fractions,
+// units of 2^-32) and the threshold thr = floor(2^64 F / (S 2^32 + F)) a draw
+// compares its fractional-stream coin with (all-ones when the integer mass S
+// is 0: every draw takes the fractional part).
+struct FNode {
+    uint64_t F;
+    uint64_t thr;
+};
+constexpr uint32_t kFracTag = 0x40000000u;     // Philox counter word 3 of the fractional stream
+constexpr uint32_t kFracIters = 4096u;         // fractional rejection rounds per draw
+constexpr int32_t kMaxAttemptsFloat = 65536;   // node2vec cap with float weights (attempt in 16 bits)
+
 // One 32-byte alias slot i of a node:
 //   T     stay iff coin64 < T
 //   stay  locator of group i:        member offset (36 bits) | size << 36
@@ -63,6 +76,9 @@ constexpr uint64_t kOffMask36 = (1ull << kOffBits) - 1ull;
 constexpr uint32_t kMaxGroupSize = 1u << 28;  // sizes live in 28 bits
 static_assert(sizeof(AliasEnt) == 32, "alias entry is one 32-byte read");
 static_assert(sizeof(NodeHdr) == 48, "node header layout");
+
+// slot arrays per node region: v, w, back offset (and fraction)
+__host__ __device__ inline uint32_t slot_arrays(int32_t fw) { return fw ? 4u : 3u; }
 
 struct Ctl {
     unsigned long long top[NPOOL];   // pool cursors (elements)
@@ -89,6 +105,9 @@ struct DevStore {
     // sized to their contents plus ceil(size * fit / 2^16) (a load that fits
     // the store to the graph); neighbour tables stay powers of two.
     uint32_t fit;
+    int32_t fw;      // weights are float bits: slot regions carry a fraction array
+    int32_t pad;
+    FNode *fnode;    // [n+1] when fw
     NodeHdr *hdr;
     GroupHdr *grp;
     AliasEnt *alias;
@@ -146,11 +165,14 @@ struct Runs {          // per-node runs of a sorted batch
 void launch_reset_ctl(Ctl *ctl, hipStream_t st);
 void launch_reset_walk(Ctl *ctl, hipStream_t st);
 void launch_init_nodes(DevStore s, hipStream_t st);
-void launch_insert_keys(const DevEdge *e, int64_t k, int32_t n, uint32_t *keys, uint32_t *vals, Ctl *ctl,
+// fw: weights are float bits (validated as such; the gather then writes the
+// integer part as the arc's weight and the fraction to frac[i])
+void launch_insert_keys(const DevEdge *e, int64_t k, int32_t n, int fw, uint32_t *keys, uint32_t *vals, Ctl *ctl,
                         hipStream_t st);
-void launch_gather_edges(const DevEdge *e, const uint32_t *order, int64_t k, DevEdge *out, hipStream_t st);
+void launch_gather_edges(const DevEdge *e, const uint32_t *order, int64_t k, DevEdge *out, uint32_t *frac,
+                         hipStream_t st);
 void launch_insert_plan(DevStore s, const DevEdge *sorted, Runs r, int64_t k, hipStream_t st);
-void launch_insert_apply(DevStore s, const DevEdge *sorted, Runs r, int64_t k, hipStream_t st);
+void launch_insert_apply(DevStore s, const DevEdge *sorted, const uint32_t *frac, Runs r, int64_t k, hipStream_t st);
 void launch_delete_keys(const int32_t *dels, int64_t k, unsigned long long *keys, hipStream_t st);
 void launch_delete_check(DevStore s, const unsigned long long *keys, int64_t k, uint32_t *us, uint32_t *slots,
                          hipStream_t st);
@@ -166,15 +188,16 @@ struct PersistCfg {
     int block, wpl, depth, blocks;  // blocks = grid size
 };
 size_t persist_lds_bytes(int block, int wpl, int depth);
-int persist_max_blocks_per_cu(int app, int wpl, int depth, int block);
+int persist_max_blocks_per_cu(int app, int wpl, int depth, int block, int fw);
 void launch_walk_persist(DevStore s, int app, const WalkArgs &a, const PersistCfg &c, hipStream_t st);
 void launch_walk_serve(DevStore s, const ServeReq *reqs, int nreq, unsigned long long total,
                        unsigned long long *cursor, int blocks, hipStream_t st);
-int serve_max_blocks_per_cu();
+int serve_max_blocks_per_cu(int fw);
+// fmass/fthr (export_a) and slot_f (export_b) are null unless fw
 void launch_export_a(DevStore s, uint32_t *nslots, uint64_t *mass, uint32_t *bsize, uint64_t *T, uint8_t *alias,
-                     hipStream_t st);
+                     uint64_t *fmass, uint64_t *fthr, hipStream_t st);
 void launch_export_b(DevStore s, const uint64_t *slot_out_off, const uint64_t *mem_out_off, uint32_t *slot_v,
-                     uint32_t *slot_w, uint64_t *members, uint32_t *backpos, hipStream_t st);
+                     uint32_t *slot_w, uint64_t *members, uint32_t *backpos, uint32_t *slot_f, hipStream_t st);
 void launch_edge_count(DevStore s, const int32_t *us, const int32_t *vs, int64_t k, uint32_t *counts,
                        hipStream_t st);
 

@@ -41,7 +41,10 @@ typedef enum { BINGO_UPD_INSERT = 0, BINGO_UPD_DELETE = 1, BINGO_UPD_MIXED = 2 }
 /* synthetic weight laws of the workload generators */
 typedef enum { BINGO_W_CONST1 = 0, BINGO_W_UNIFORM16 = 1, BINGO_W_LOGUNI32 = 2, BINGO_W_ZIPF24 = 3 } bingo_wkind;
 
-/* ref struct Edges{int u,v; float weight} (type.h:24-29), weight widened to u32 */
+/* ref struct Edges{int u,v; float weight} (type.h:24-29): w is a u32 weight,
+ * or, on a handle created with float_weights, the IEEE-754 bits of a float
+ * weight in (0, 2^32), taken to 2^-32 resolution (ref wtype::Float,
+ * include/Bingo.cuh:65-69) */
 typedef struct { int32_t u, v; uint32_t w; } bingo_edge;
 /* ref struct Deleted{int u,id} (type.h:33-37): id is the per-node slot ordinal */
 typedef struct { int32_t u, slot; } bingo_del;
@@ -57,6 +60,8 @@ typedef struct {
     int32_t fit_load;       /* 1: load sizes every region to its contents plus fit_headroom
                                instead of the next power of two (at least min_slots) */
     double fit_headroom;    /* fit_load's spare room per region, as a fraction (1/16) */
+    int32_t float_weights;  /* 1: edge weights are float bits (ref wtype::Float, Bingo.cuh:65-69,
+                               254-264); integer part through the bit groups, fraction per slot */
 } bingo_config;
 
 typedef struct {
@@ -64,7 +69,8 @@ typedef struct {
     double p, q;            /* node2vec return / in-out parameters (ref P=0.5, Q=2) */
     double tp;              /* PPR stop probability per step (ref TP=0.0125) */
     int32_t max_len;        /* path columns, vertices per walk incl. start (ref LEN=80) */
-    int32_t max_attempts;   /* node2vec rejection cap per step; last draw accepted */
+    int32_t max_attempts;   /* node2vec rejection cap per step; last draw accepted (<= 65536
+                               with float weights) */
     int32_t form;           /* BINGO_FORM_*: how walkers are scheduled (paths are the same) */
     int32_t block;          /* threads per block, 64..512; 0 = tuned or default */
     int32_t walkers_per_lane; /* persistent form: 1, 2 or 4 walker slots per lane; 0 = tuned */
@@ -102,6 +108,10 @@ typedef struct {
     uint32_t *slot_w;   /* [total_slots] 0 = deleted */
     uint64_t *members;  /* [total_members] (v << 32) | slot, per node, per group */
     uint32_t *backpos;  /* [total_members] per live slot, its set bits ascending */
+    /* float_weights handles only (NULL to skip; slot_f with the slot arrays): */
+    uint32_t *slot_f;   /* [total_slots] fraction of each slot's weight, x 2^32 (slot_w: integer part) */
+    uint64_t *fmass;    /* [n+1] sum of the live fractions, x 2^32 */
+    uint64_t *fthr;     /* [n+1] fractional-part threshold floor(2^64 F / (S 2^32 + F)) */
 } bingo_dump;
 
 /* ---- lifetime ---- */
@@ -114,7 +124,8 @@ const char *bingo_version(void);
 
 /* ---- graph load: ref build() (operation.cuh:33-121) ----
  * Arcs are taken in array order; the k-th arc of a node gets slot k (as
- * graph.cuh:38-42 numbers nodeIdu).  u, v in 1..n; w >= 1. */
+ * graph.cuh:38-42 numbers nodeIdu).  u, v in 1..n; w >= 1 (float weights:
+ * finite, 2^-32 <= w < 2^32). */
 int bingo_load(bingo_graph *g, int32_t n, const bingo_edge *edges, int64_t m);
 int bingo_load_device(bingo_graph *g, int32_t n, const bingo_edge *d_edges, int64_t m);
 

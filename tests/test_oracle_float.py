@@ -3,24 +3,15 @@ float into integer part and 32-bit fraction, the fractional-part threshold
 against the exact ratio, store state after updates against a brute-force
 recomputation, and the one-step law of a draw — integer part through the
 alias table, fractional part by rejection — against w / sum(w)."""
-import struct
 from fractions import Fraction
 
 import numpy as np
 from scipy import stats
 
 from tests.brute import BruteGraph, check_against_brute, member_law, split_dump, transition_law
+from tests.floatw import fbits, fexact, float_edges
 
 M32, M64 = 1 << 32, 1 << 64
-
-
-def fbits(x):
-    return struct.unpack("<I", struct.pack("<f", x))[0]
-
-
-def fexact(bits):
-    """Exact value of float32 bits (a Python float holds any float32 exactly)."""
-    return Fraction(struct.unpack("<f", struct.pack("<I", bits))[0])
 
 
 def test_split_float(orc):
@@ -47,20 +38,6 @@ def test_frac_threshold_matches_ratio(orc):
     assert orc.frac_threshold(0, 7) == M64 - 1
 
 
-def _float_edges(rng, n, k, hub=None):
-    e = np.zeros(k, np.dtype([("u", np.int32), ("v", np.int32), ("w", np.uint32)]))
-    e["u"] = np.where(rng.random(k) < 0.3, hub, rng.integers(1, n + 1, k)) if hub else rng.integers(1, n + 1, k)
-    e["v"] = rng.integers(1, n + 1, k)
-    kind = rng.integers(0, 5, k)
-    vals = np.where(kind == 0, rng.integers(1, 50, k).astype(np.float64),               # integers
-           np.where(kind == 1, rng.integers(1, 50, k) + rng.integers(0, 100, k) * 0.01,  # d + rd(100)*.01
-           np.where(kind == 2, rng.random(k) * 0.99 + 0.005,                            # below 1
-           np.where(kind == 3, rng.random(k) * 2e6 + 1.0,                                # large
-                    rng.random(k) * 7.0 + 0.125))))
-    e["w"] = [fbits(float(x)) for x in vals]
-    return e
-
-
 def _split(e, orc):
     ip = np.zeros(len(e), np.int64)
     fr = np.zeros(len(e), np.int64)
@@ -72,11 +49,11 @@ def _split(e, orc):
 def test_float_store_after_updates(orc):
     rng = np.random.default_rng(11)
     n = 200
-    base = _float_edges(rng, n, 3000, hub=7)
+    base = float_edges(rng, n, 3000, hub=7)
     for nbr in (True, False):
         o = orc.OracleGraph(n, nbr, float_weights=True)
         assert o.load(base) == 0
-        ins = _float_edges(rng, n, 800, hub=7)
+        ins = float_edges(rng, n, 800, hub=7)
         assert o.insert(ins) == 0
         arcs = np.concatenate([base, ins])
         ip, fr = _split(arcs, orc)
@@ -127,7 +104,7 @@ def test_float_store_after_updates(orc):
         o.close()
 
 
-def _draw_law(d, u, n):
+def draw_law(d, u, n):
     """Exact law of one draw on node u from the exported state."""
     nodes = split_dump(d, n)
     sv, sw, groups = nodes[u]
@@ -155,7 +132,7 @@ def _draw_law(d, u, n):
 def test_float_one_step_law(orc):
     rng = np.random.default_rng(5)
     n = 60
-    e = _float_edges(rng, n, 1500, hub=3)
+    e = float_edges(rng, n, 1500, hub=3)
     o = orc.OracleGraph(n, False, float_weights=True)
     assert o.load(e) == 0
     d = o.export()
@@ -168,14 +145,14 @@ def test_float_one_step_law(orc):
         for v, a, b in zip(e["v"][sel], ip[sel], fr[sel]):
             want[int(v)] = want.get(int(v), 0) + Fraction(int(a)) + Fraction(int(b), M32)
         tot = sum(want.values())
-        got = _draw_law(d, u, n)
+        got = draw_law(d, u, n)
         assert set(got) == set(want), u
         for v in want:
             assert abs(float(got[v] / (want[v] / tot)) - 1.0) < 1e-6, (u, v)
     # the drawn frequencies follow the law (fractional rejection included)
     N = 200_000
     paths, _ = o.walk(orc.SAMPLING, orc.walk_params(seed=9, max_len=2), starts=np.full(N, 3, np.int32))
-    law = _draw_law(d, 3, n)
+    law = draw_law(d, 3, n)
     vs = sorted(law)
     cnt = np.bincount(paths[:, 1], minlength=n + 1)[vs]
     exp = np.array([float(law[v]) for v in vs]) * N
