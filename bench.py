@@ -401,6 +401,17 @@ def main():
     plain_steps = int(d_s2.to_host(np.uint64, (1,))[0]) // 2
     log(f"plain form walk: {min(plain_ms):.2f} ms for {plain_steps} steps")
 
+    # random-read ceiling on this store: independent 32-byte (alias slot) and
+    # 8-byte (member, neighbour probe) reads over the member pool; a walk
+    # launch's draws and probes at these rates take ceiling_ms
+    rates = {}
+    for width in (32, 8, 32, 8):  # second pass counts
+        nr, ms = g.gather_probe(width)
+        rates[width] = nr / (ms / 1e3)
+    ceiling_s = attempts / K * (1 / rates[32] + 1 / rates[8]) + probes / K / rates[8]
+    log(f"random reads: {rates[32] / 1e9:.1f} G/s (32 B), {rates[8] / 1e9:.1f} G/s (8 B); "
+        f"walk at {ceiling_s / (kern_ms / K / 1e3):.2f} of that ceiling")
+
     bytes_per_launch = (B_STEP * attempts + B_PROBE * probes) / K
     kern_s = kern_ms / K / 1e3
     achieved = bytes_per_launch / kern_s / 1e9
@@ -442,6 +453,8 @@ def main():
                      "traffic_GBps": traffic / kern_s / 1e9 if traffic else None,
                      "kernel": "k_walk_persist" if tuned.get("form") == B.FORM_PERSISTENT else "k_walk",
                      "kernel_ms": kern_ms / K, "algorithmic_bytes_per_launch": bytes_per_launch},
+        "gather_ceiling": {"reads32_per_s": rates[32], "reads8_per_s": rates[8],
+                           "kernel_ms_at_ceiling": ceiling_s * 1e3, "frac": ceiling_s / kern_s},
         "cpu_baseline": None,
     }
     if wgraph:

@@ -182,6 +182,7 @@ SIGNATURES = {
     "bingo_memcpy_h2d": (_int, [_vp, _vp, _vp, C.c_size_t]),
     "bingo_memcpy_d2h": (_int, [_vp, _vp, _vp, C.c_size_t]),
     "bingo_memcpy_d2d": (_int, [_vp, _vp, _vp, C.c_size_t]),
+    "bingo_gather_probe": (_int, [_vp, C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_float)]),
     "bingo_event_record": (_int, [_vp, _int]),
     "bingo_event_elapsed": (_int, [_vp, _int, _int, C.POINTER(C.c_float)]),
     "bingo_server_config_default": (None, [C.POINTER(ServerConfig)]),
@@ -526,6 +527,12 @@ class Graph:
         ms = C.c_float(0)
         _check(lib().bingo_event_elapsed(self._h, a, b, C.byref(ms)))
         return ms.value
+
+    def gather_probe(self, width):
+        """(reads, kernel ms) of independent random `width`-byte reads over the member pool"""
+        n, ms = C.c_uint64(0), C.c_float(0)
+        _check(lib().bingo_gather_probe(self._h, width, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
 
     def edge_count(self, us, vs):
         us = np.ascontiguousarray(us, np.int32)

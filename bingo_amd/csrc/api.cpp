@@ -1009,6 +1009,26 @@ int bingo_memcpy_d2d(bingo_graph *g, void *d_dst, const void *d_src, size_t byte
     return BINGO_OK;
 }
 
+int bingo_gather_probe(bingo_graph *g, int width, uint64_t *reads, float *ms)
+{
+    if (!g || !g->loaded) return fail(BINGO_ESTATE, "graph not loaded");
+    if ((width != 8 && width != 32) || !reads || !ms) return fail(BINGO_EINVAL, "width must be 8 or 32");
+    int rc;
+    if ((rc = read_ctl(g))) return rc;
+    const uint64_t bytes = g->ctl_host->top[POOL_MEM] * kElem[POOL_MEM];
+    const uint64_t units = bytes / (uint64_t)width;
+    if (units < 1) return fail(BINGO_EINVAL, "member pool is empty");
+    if ((rc = ensure_scratch(g, SC_AUX1, 64))) return rc;
+    hipEvent_t e0 = g->ev[kEvents - 2], e1 = g->ev[kEvents - 1];
+    HIPCHK(hipEventRecord(e0, g->st));
+    *reads = launch_gather_probe(g->s.mem, units, width, scp<unsigned long long>(g, SC_AUX1), g->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e1, g->st));
+    HIPCHK(hipEventSynchronize(e1));
+    HIPCHK(hipEventElapsedTime(ms, e0, e1));
+    return BINGO_OK;
+}
+
 int bingo_event_record(bingo_graph *g, int slot)
 {
     if (!g || slot < 0 || slot >= kEvents) return fail(BINGO_EINVAL, "bad event slot");

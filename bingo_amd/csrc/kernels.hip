@@ -1653,6 +1653,45 @@ __global__ __launch_bounds__(256) void k_copy16(uint4 *__restrict__ dst, const u
     if (blockIdx.x == 0 && threadIdx.x < ntail) dtail[threadIdx.x] = stail[threadIdx.x];
 }
 
+// Random-gather probe (the ceiling a walk's dependent gathers are judged
+// against): each thread makes `iters` rounds of 8 independent W-byte reads
+// at hashed positions in [base, base + units * W).  W = 32 reads like an
+// alias slot (two 16-byte loads), W = 8 like a member.
+__device__ __forceinline__ uint64_t probe_mix(uint64_t x)
+{
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ull;
+    x ^= x >> 33;
+    return x;
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void k_gather_probe(const uint8_t *__restrict__ base, uint64_t units, uint32_t iters,
+                                                      unsigned long long *sink)
+{
+    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    uint64_t acc = 0;
+    for (uint32_t it = 0; it < iters; ++it) {
+        uint64_t v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint64_t i = probe_mix(t * 0x9E3779B97F4A7C15ull + (uint64_t)it * 8 + k) % units;
+            if (W == 8) {
+                v[k] = reinterpret_cast<const uint64_t *>(base)[i];
+            } else {
+                const uint4 *p = reinterpret_cast<const uint4 *>(base + i * 32);
+                const uint4 a = p[0], b = p[1];
+                v[k] = (uint64_t)a.x + a.y + a.z + a.w + b.x + b.y + b.z + b.w;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc += v[k];
+    }
+    if (acc == 0x5eed5eed5eedull) *sink = acc;  // keeps the loads; practically never taken
+}
+
 // ------------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------------
@@ -1665,6 +1704,19 @@ static inline unsigned run_blocks(int64_t k)
 }
 
 void launch_reset_ctl(Ctl *ctl, hipStream_t st) { hipLaunchKernelGGL(k_reset_ctl, dim3(1), dim3(64), 0, st, ctl); }
+
+uint64_t launch_gather_probe(const void *base, uint64_t units, int width, unsigned long long *sink, hipStream_t st)
+{
+    const unsigned blocks = 4096, threads = 256;
+    const uint32_t iters = 64;
+    if (width == 32)
+        hipLaunchKernelGGL(k_gather_probe<32>, dim3(blocks), dim3(threads), 0, st,
+                           static_cast<const uint8_t *>(base), units, iters, sink);
+    else
+        hipLaunchKernelGGL(k_gather_probe<8>, dim3(blocks), dim3(threads), 0, st, static_cast<const uint8_t *>(base),
+                           units, iters, sink);
+    return (uint64_t)blocks * threads * iters * 8;
+}
 
 void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st)
 {

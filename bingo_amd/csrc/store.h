@@ -202,6 +202,9 @@ void launch_edge_count(DevStore s, const int32_t *us, const int32_t *vs, int64_t
                        hipStream_t st);
 
 void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st);
+// random reads of `width` (8 or 32) bytes over units * width bytes at base;
+// returns the number of reads
+uint64_t launch_gather_probe(const void *base, uint64_t units, int width, unsigned long long *sink, hipStream_t st);
 
 // rocPRIM wrappers (temp storage sized by a null call)
 // Pairs sort over double buffers: input in (k0, v0), (k1, v1) is the other

@@ -192,7 +192,14 @@ int bingo_memcpy_d2h(bingo_graph *g, void *dst, const void *d_src, size_t bytes)
 /* device-to-device copy by a plain streaming kernel (k_copy16) on the
  * handle's stream; also the known-size reference for HBM traffic counters */
 int bingo_memcpy_d2d(bingo_graph *g, void *d_dst, const void *d_src, size_t bytes);
-/* HIP events on the handle's stream: record into slot (0..15), elapsed ms */
+/* Measurement probe (no reference counterpart): random reads of `width`
+ * bytes (8 = a member, 32 = an alias slot) at hashed positions over the
+ * graph's member pool, independent of each other; *reads receives their
+ * number and *ms the kernel time.  The rate is the ceiling for a walk's
+ * dependent gathers on this store.  Overwrites event slots 14 and 15. */
+int bingo_gather_probe(bingo_graph *g, int width, uint64_t *reads, float *ms);
+/* HIP events on the handle's stream: record into slot (0..15), elapsed ms
+ * (bingo_walk_tune and bingo_gather_probe overwrite slots 14 and 15) */
 int bingo_event_record(bingo_graph *g, int slot);
 int bingo_event_elapsed(bingo_graph *g, int slot_a, int slot_b, float *ms);
 

@@ -388,3 +388,14 @@ def test_walk_graph_replays_match_eager(orc, form):
         total += sb
     assert int(ds.to_host(np.uint64, (1,))[0]) == total
     wgr.close()
+
+
+def test_gather_probe_reports_rates():
+    g = B.Graph(neighbor_index=False)
+    g.load(1 << 12, B.gen_rmat(12, 50_000, B.W_LOGUNI32, seed=1))
+    for width in (8, 32):
+        n, ms = g.gather_probe(width)
+        assert n > 0 and ms > 0
+    with pytest.raises(B.BingoError):
+        g.gather_probe(16)
+    g.close()
