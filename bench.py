@@ -14,6 +14,12 @@ Zipf weights, 10^7 PPR walkers with lengths in [5, 200], update batches of
 log-uniform size), replayed from a captured HIP graph; `c1` the small one.
 
     python bench.py [--steps K] [--warmup W] [--config c1|c2|c3|c4|c5] [--no-cpu]
+                    [--app deepwalk|node2vec|ppr|sampling] [--float-weights]
+
+`--app` runs a configuration's graph and schedule with another walk app;
+`--float-weights` turns its weights into float32 values w / 256 (the
+reference's wtype::Float: integer part through the bit groups, fraction by
+rejection), for graph, updates and CPU baseline alike.
 
 `c4` is the served form: a Poisson stream of walk queries and update
 batches through the serving loop, with p50/p99 latency.  `c5` is the scale
@@ -91,6 +97,15 @@ def batch_sizes(cfg, k, seed):
     return [int(round(np.exp(x))) for x in np.random.default_rng(seed + 1).uniform(lo, hi, size=k)]
 
 
+def to_float_bits(edges):
+    """u32 weights w -> IEEE-754 bits of float32(w / 256): a log-uniform u32
+    law becomes a float law over [2^-8, 2^24) whose values mostly have a
+    fractional part."""
+    e = edges.copy()
+    e["w"] = (edges["w"].astype(np.float64) / 256.0).astype(np.float32).view(np.uint32)
+    return e
+
+
 def cpu_baseline(base, n, cfg, seed, budget_s=15.0):
     """The CPU oracle (a C/OpenMP restatement of the same sampler) on this
     host's cores: full graph (or, where the configuration names a sample,
@@ -106,8 +121,10 @@ def cpu_baseline(base, n, cfg, seed, budget_s=15.0):
                             nthreads=threads)
         graph_note = (f"the first {len(base)} arcs of the same R-MAT stream and node space (the full graph's "
                       f"CPU store would not fit host memory)")
+    if cfg.get("float"):
+        base = to_float_bits(base)
     t0 = time.perf_counter()
-    o = orc.OracleGraph(n, neighbor_index=cfg["app"] == 1)
+    o = orc.OracleGraph(n, neighbor_index=cfg["app"] == 1, float_weights=bool(cfg.get("float")))
     assert o.load(base) == 0
     load_s = time.perf_counter() - t0
     log(f"cpu oracle loaded in {load_s:.1f}s ({threads} threads)")
@@ -284,6 +301,8 @@ def main():
     ap.add_argument("--rate", type=float, default=2000.0, help="c4: offered requests per second")
     ap.add_argument("--requests", type=int, default=300, help="c4: number of requests")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--app", choices=APP_NAMES, default=None, help="walk app instead of the configuration's")
+    ap.add_argument("--float-weights", action="store_true", help="float32 weights w / 256 (wtype::Float)")
     ap.add_argument("--schedule", default="", metavar="BLOCK,WPL,DEPTH",
                     help="fixed persistent walk knobs instead of autotuning (e.g. 512,2,16)")
     ap.add_argument("--seed", type=int, default=2026)
@@ -303,7 +322,13 @@ def main():
         served(B, args)
         return
 
-    cfg = CONFIGS[args.config]
+    cfg = dict(CONFIGS[args.config])
+    if args.app and APP_NAMES.index(args.app) != cfg["app"]:
+        cfg["app"] = APP_NAMES.index(args.app)
+        cfg["workload"] += f"_app_{args.app}"
+    if args.float_weights:
+        cfg["float"] = True
+        cfg["workload"] += "_float32w"
     scale, m = cfg["scale"], cfg["edges"]
     n = 1 << scale
     K, W = args.steps, args.warmup
@@ -314,8 +339,8 @@ def main():
     base = B.gen_rmat(scale, m, cfg["wkind"], seed=args.seed, undirected=not cfg.get("directed"))
     log(f"generated {len(base)} arcs in {time.perf_counter() - t0:.1f}s")
     fit = dict(fit_load=True, fit_headroom=1.0 / 16, pool_slack=0.03) if cfg.get("fit") else {}
-    g = B.Graph(device=0, neighbor_index=app == B.NODE2VEC, **fit)
-    d_base = B.DeviceArray(g, base)
+    g = B.Graph(device=0, neighbor_index=app == B.NODE2VEC, float_weights=bool(cfg.get("float")), **fit)
+    d_base = B.DeviceArray(g, to_float_bits(base) if cfg.get("float") else base)
     g.load_device(n, d_base, len(base))
     g.sync()
     load_ms = g.stats()["load_ms"]
@@ -331,6 +356,8 @@ def main():
     batches = []
     for kb in sizes:
         ins = ug.inserts(kb)
+        if cfg.get("float"):
+            ins = to_float_bits(ins)
         dels = ug.expire(kb) if cfg.get("churn") == "expire" else ug.deletes(kb)
         batches.append((B.DeviceArray(g, ins), B.DeviceArray(g, dels), kb))
     del ug
@@ -436,7 +463,8 @@ def main():
                    "update_batch": cfg["batch"], "p": cfg["p"], "q": cfg["q"], "tp": cfg["tp"],
                    "hip_graph": cfg["graph"], "directed": bool(cfg.get("directed")),
                    "deletes": "oldest arcs (sliding window)" if cfg.get("churn") == "expire" else "uniform live arcs",
-                   "store": "fitted at load" if cfg.get("fit") else "power-of-two regions"},
+                   "store": "fitted at load" if cfg.get("fit") else "power-of-two regions",
+                   "weights": "float32 w/256" if cfg.get("float") else "u32"},
         "walk_steps_per_s": walk_rate,
         "updates_per_s": nupd / ((ins_ms + del_ms) / 1e3),
         "insert_ms": ins_ms / K,

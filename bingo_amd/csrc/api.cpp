@@ -208,7 +208,8 @@ static int ensure_batch_scratch(bingo_graph *g, int64_t k, bool del)
     } else {
         if ((rc = ensure_scratch(g, SC_KEYS, k4)) || (rc = ensure_scratch(g, SC_VALS, k4)) ||
             (rc = ensure_scratch(g, SC_EDGES, (size_t)k * sizeof(DevEdge))) ||
-            (g->s.fw && (rc = ensure_scratch(g, SC_FRAC, k4))))
+            (g->s.fw && (rc = ensure_scratch(g, SC_FRAC, k4))) ||
+            (rc = ensure_scratch(g, SC_WRES, insert_wres_bytes(k))))
             return rc;
     }
     return BINGO_OK;
@@ -233,17 +234,20 @@ static int do_insert(bingo_graph *g, const DevEdge *d_e, int64_t k)
     launch_gather_edges(d_e, second ? v1 : v0, k, scp<DevEdge>(g, SC_EDGES), frac, g->st);
     Runs r;
     if ((rc = encode_runs(g, second ? k1 : k0, k, &r))) return rc;
-    launch_insert_plan(g->s, scp<DevEdge>(g, SC_EDGES), r, k, g->st);
+    unsigned long long *wres = scp<unsigned long long>(g, SC_WRES);
+    launch_insert_plan(g->s, scp<DevEdge>(g, SC_EDGES), r, wres, k, g->st);
     HIPCHK(hipGetLastError());
     if ((rc = read_ctl(g))) return rc;
     if (g->ctl_host->error & ERR_INVAL)
         return fail(BINGO_EINVAL, g->s.fw ? "insert batch has an arc with u or v outside 1..n or a float weight that is "
                                             "not finite, not in (0, 2^32) or below 2^-32"
                                           : "insert batch has an arc with u or v outside 1..n or w == 0");
-    if (g->ctl_host->error & ERR_NOMEM) return fail(BINGO_ENOMEM, "a node would exceed the per-node slot limit");
+    if (g->ctl_host->error & ERR_NOMEM)
+        return fail(BINGO_ENOMEM, "a node would reach the per-node limits (2^28 slots, integer weight 2^59)");
     if (g->ctl_host->error) return fail(BINGO_EDEVICE, "store reported an internal error");
     if ((rc = ensure_pools(g))) return rc;
-    launch_insert_apply(g->s, scp<DevEdge>(g, SC_EDGES), frac, r, k, g->st);
+    launch_insert_apply(g->s, scp<DevEdge>(g, SC_EDGES), frac, r, wres, k, g->st);
+    if (g->s.nbr) launch_hash_insert(g->s, scp<DevEdge>(g, SC_EDGES), k, g->st);
     HIPCHK(hipGetLastError());
     g->arcs += k;
     return BINGO_OK;
@@ -276,6 +280,7 @@ static int do_delete(bingo_graph *g, const int32_t *d_pairs, int64_t k)
         return fail(BINGO_EINVAL, "delete batch names a node outside 1..n, a slot that is not live, or a slot twice");
     if (g->ctl_host->error) return fail(BINGO_EDEVICE, "store reported an internal error");
     launch_delete_apply(g->s, scp<uint32_t>(g, SC_VALS2), r, big, big + nbig_max, k, g->st);
+    if (g->s.nbr) launch_hash_delete(g->s, scp<uint32_t>(g, SC_KEYS2), scp<uint32_t>(g, SC_VALS2), k, g->st);
     HIPCHK(hipGetLastError());
     g->arcs -= k;
     return BINGO_OK;

@@ -97,31 +97,20 @@ __device__ __forceinline__ uint32_t dpp_incl_or(uint32_t v)
 #undef BINGO_DPP
 __device__ __forceinline__ uint32_t dpp_sum32(uint32_t v) { return rl32(dpp_incl_add(v), 63); }
 __device__ __forceinline__ uint32_t dpp_or32(uint32_t v) { return rl32(dpp_incl_or(v), 63); }
-// 64-bit sum / inclusive prefix over the lanes whose value is non-zero (few
-// in the update kernels: one per bit group at most)
-__device__ __forceinline__ uint64_t sparse_sum64(uint64_t v)
-{
-    uint64_t acc = 0;
-    for (uint64_t nz = __ballot(v != 0ull); nz; nz &= nz - 1ull) acc += rl64(v, __ffsll((unsigned long long)nz) - 1);
-    return acc;
-}
-__device__ __forceinline__ uint64_t sparse_incl64(uint64_t v, int ln)
-{
-    uint64_t acc = 0;
-    for (uint64_t nz = __ballot(v != 0ull); nz; nz &= nz - 1ull) {
-        const int l = __ffsll((unsigned long long)nz) - 1;
-        const uint64_t x = rl64(v, l);
-        if (ln >= l) acc += x;
-    }
-    return acc;
-}
 
-// 64-bit wave sum of values below 2^58 (fraction sums): three DPP sums of
-// 24-, 24- and 10-bit pieces, none of which can overflow 32 bits
+// 64-bit wave sum / inclusive prefix sum, exact when the result is below
+// 2^64: three DPP scans of the 24-, 24- and 16-bit pieces of each value, none
+// of which can overflow 32 bits over 64 lanes
 __device__ __forceinline__ uint64_t dpp_sum64(uint64_t v)
 {
     const uint64_t a = dpp_sum32((uint32_t)v & 0xFFFFFFu), b = dpp_sum32((uint32_t)(v >> 24) & 0xFFFFFFu),
                    c = dpp_sum32((uint32_t)(v >> 48));
+    return a + (b << 24) + (c << 48);
+}
+__device__ __forceinline__ uint64_t dpp_incl64(uint64_t v)
+{
+    const uint64_t a = dpp_incl_add((uint32_t)v & 0xFFFFFFu), b = dpp_incl_add((uint32_t)(v >> 24) & 0xFFFFFFu),
+                   c = dpp_incl_add((uint32_t)(v >> 48));
     return a + (b << 24) + (c << 48);
 }
 
@@ -260,16 +249,26 @@ __device__ uint64_t frac_threshold(uint64_t S, uint64_t F)
 }
 
 // ------------------------------------------------------------------------
-// alias table: integer-exact Vose over 32 fixed slots, built by one wave.
+// alias table: integer-exact sweep over 32 fixed slots, built by one wave.
 // Lane j holds group j's size and member offset.  Masses M_j = size_j << j,
-// s_j = 32 M_j against slot capacity S = sum M_j; pair the highest small
-// with the highest large until one side is empty; leftovers stay (tau = S).
-// Stay threshold T_j = floor(tau_j 2^64 / S).  Replaces the float/EPS table
-// of NodeData::buildAliasTable (include/Bingo.cuh:111-158).
+// s_j = 32 M_j against slot capacity S = sum M_j.  Lights (s_j < S) and
+// heavies (the rest), each in index order.  The sequential sweep -- each
+// light takes the current heavy as alias and charges it its deficit S - s_j;
+// a heavy whose residual falls below S turns light on the spot, with the next
+// heavy as alias -- is computed lane-parallel from two prefix sums: A_j, the
+// lights' deficits before light j, and E_k, the heavies' excesses s - S up to
+// and including heavy k.  Light j's alias is the first heavy with E_k >= A_j;
+// heavy k splits iff E_k falls inside a light's deficit range [A_j, A_j + d_j),
+// with tau_k = S - (A_j + d_j - E_k) and the next heavy as alias; other
+// heavies keep tau = S.  Stay threshold T_j = floor(tau_j 2^64 / S).
+// Replaces the float/EPS table of NodeData::buildAliasTable
+// (include/Bingo.cuh:111-158).
 //
-// Bounds from the store's limits (group size < 2^28, weight < 2^32):
-// M_j < 2^59, so s_j = 32 M_j < 2^64, and S < 2^60; 64-bit words suffice.
+// Bounds: group sizes < 2^28 and weights < 2^32 give M_j < 2^59; the node
+// limit S < 2^59 (inserts past it are rejected) keeps the prefix sums, at most
+// 31 S, below 2^64.
 // ------------------------------------------------------------------------
+constexpr uint64_t kMaxNodeMass = 1ull << 59;
 
 // floor(tau 2^64 / S) for tau < S < 2^60: a double-precision estimate (off by
 // at most a few thousand), one refinement from the exact 128-bit remainder,
@@ -300,48 +299,49 @@ __device__ __forceinline__ uint64_t frac64(uint64_t tau, uint64_t S)
     return q;
 }
 
-// Returns the node's integer mass S (all lanes).
+// Returns the node's integer mass S (all lanes).  All 64 lanes active.
 __device__ uint64_t alias_rebuild(const DevStore &s, int32_t u, uint32_t size, uint64_t off, uint64_t hmeta, int ln)
 {
     const bool own = ln < kGroups;
     const uint64_t M = own ? ((uint64_t)size << ln) : 0ull;
-    const uint64_t S = sparse_sum64(M);
-    uint64_t sj = M << 5;
-    uint32_t small = (uint32_t)__ballot(own && sj < S);
-    uint32_t large = (uint32_t)__ballot(own && sj >= S);
-    uint64_t tau = 0;
+    const uint64_t S = dpp_sum64(M);
+    const uint64_t sj = M << 5;
+    const bool light = own && sj < S, heavy = own && !light;
+    const uint64_t d = light ? S - sj : 0ull, e = heavy ? sj - S : 0ull;
+    const uint64_t A = dpp_incl64(d) - d;  // lights' deficits before this lane
+    const uint64_t E = dpp_incl64(e);      // heavies' excesses up to this lane
+    const uint64_t Aend = A + d;
+    uint64_t tau = light ? sj : S;
     int ali = ln;
-    while (small && large) {
-        const int l = 31 - __clz(small), gi = 31 - __clz(large);
-        const uint64_t sl = rl64(sj, l);
-        const uint64_t sg = rl64(sj, gi) - (S - sl);
-        if (ln == l) {
-            tau = sl;
-            ali = gi;
+    bool found = false;
+    for (uint32_t hb = S ? (uint32_t)__ballot(heavy) : 0u; hb;) {  // S = 0: no lights
+        const int k = __ffs(hb) - 1;
+        hb &= hb - 1u;
+        const uint64_t Ek = rl64(E, k);
+        if (light && !found && A <= Ek) {
+            ali = k;
+            found = true;
         }
-        small &= ~(1u << l);
-        if (ln == gi) sj = sg;
-        if (sg < S) {
-            large &= ~(1u << gi);
-            small |= 1u << gi;
+        const uint64_t cont = __ballot(light && A <= Ek && Ek < Aend);
+        if (cont && hb) {  // heavy k turns light inside that light's range
+            const uint64_t end = rl64(Aend, __ffsll((unsigned long long)cont) - 1);
+            if (ln == k) {
+                tau = S - (end - Ek);
+                ali = __ffs(hb) - 1;
+            }
         }
-    }
-    const uint32_t rest = small | large;
-    if (own && ((rest >> ln) & 1u)) {
-        tau = S;
-        ali = ln;
     }
     uint64_t T = ~0ull;
     if (own && tau != S) T = frac64(tau, S);
     const uint32_t size_a = __shfl(size, ali, 64);
     const uint64_t off_a = __shfl(off, ali, 64);
     if (own) {
-        AliasEnt e;
-        e.T = T;
-        e.stay = off | ((uint64_t)size << kOffBits);
-        e.alias = off_a | ((uint64_t)size_a << kOffBits);
-        e.meta = (uint64_t)ali | (hmeta << 5);
-        s.alias[(size_t)u * kGroups + ln] = e;
+        AliasEnt ae;
+        ae.T = T;
+        ae.stay = off | ((uint64_t)size << kOffBits);
+        ae.alias = off_a | ((uint64_t)size_a << kOffBits);
+        ae.meta = (uint64_t)ali | (hmeta << 5);
+        s.alias[(size_t)u * kGroups + ln] = ae;
     }
     return S;
 }
@@ -440,6 +440,7 @@ __global__ void k_gather_edges(const DevEdge *__restrict__ e, const uint32_t *__
 // ------------------------------------------------------------------------
 struct Growth {
     uint32_t cnt, new_n, new_back, pcsum;
+    bool heavy_node;  // the node's integer weight would reach kMaxNodeMass
     uint32_t cap_slots, cap_back, cap_hash;
     bool g_slots, g_back, g_hash;
     // per lane (group j = lane)
@@ -454,15 +455,18 @@ __device__ __forceinline__ uint32_t grow_cap(const DevStore &s, uint32_t need, u
     return pow2_at_least(max(need, least));
 }
 
-// counts of the run and the growth decisions; identical in plan and apply
+// counts of the run and the growth decisions; identical in plan and apply.
+// w0: this lane's arc of the first chunk (already loaded; 0 past the run)
 __device__ Growth plan_run(const DevStore &s, const DevEdge *__restrict__ e, uint32_t cnt, const NodeHdr &h,
-                           const GroupHdr &gh, int ln)
+                           const GroupHdr &gh, uint32_t w0, int ln)
 {
     Growth G;
     uint32_t addj = 0, pcsum = 0;
+    uint64_t wsum = 0;
     for (uint32_t c = 0; c < cnt; c += 64) {
         const uint32_t i = c + (uint32_t)ln;
-        const uint32_t w = i < cnt ? e[i].w : 0u;
+        const uint32_t w = c == 0 ? w0 : (i < cnt ? e[i].w : 0u);
+        wsum += w;
         const uint32_t bits = dpp_or32(w);
         for (uint32_t b = bits; b;) {
             const int j = __ffs(b) - 1;
@@ -475,6 +479,9 @@ __device__ Growth plan_run(const DevStore &s, const DevEdge *__restrict__ e, uin
     G.cnt = cnt;
     G.addj = addj;
     G.pcsum = pcsum;
+    // integer mass before (group sizes << j) plus the run's weights; the
+    // run's sum is below 2^28 * 2^32 and the mass below 2^59, so no overflow
+    G.heavy_node = dpp_sum64(ln < kGroups ? (uint64_t)gh.size << ln : 0ull) + dpp_sum64(wsum) >= kMaxNodeMass;
     G.new_n = h.nslots + cnt;
     G.g_slots = G.new_n > h.slot_cap;
     G.cap_slots = G.g_slots ? grow_cap(s, G.new_n, (uint32_t)s.min_slots) : h.slot_cap;
@@ -490,80 +497,157 @@ __device__ Growth plan_run(const DevStore &s, const DevEdge *__restrict__ e, uin
     return G;
 }
 
-__device__ __forceinline__ bool run_too_big(const NodeHdr &h, uint32_t cnt, uint32_t pcsum)
+__device__ __forceinline__ bool run_too_big(const NodeHdr &h, const Growth &G)
 {
-    // group sizes must fit the 28-bit size field of an alias slot
-    return (uint64_t)h.nslots + cnt >= kMaxGroupSize || (uint64_t)h.back_used + pcsum >= (1ull << 31) ||
-           2ull * ((uint64_t)h.hash_used + cnt) >= (1ull << 31);
+    // group sizes must fit the 28-bit size field of an alias slot; the node's
+    // integer weight stays below kMaxNodeMass
+    return (uint64_t)h.nslots + G.cnt >= kMaxGroupSize || (uint64_t)h.back_used + G.pcsum >= (1ull << 31) ||
+           2ull * ((uint64_t)h.hash_used + G.cnt) >= (1ull << 31) || G.heavy_node;
 }
 
-__global__ __launch_bounds__(256) void k_insert_plan(DevStore s, const DevEdge *__restrict__ E, Runs r)
+// ------------------------------------------------------------------------
+// Run loops are software-pipelined: while a wave works on one run it already
+// holds the next run's node header, group headers and first 64 arcs (or
+// delete slots), and the run table entry of the one after, so the dependent
+// round trips at the start of a run (table -> node -> records) overlap with
+// the previous run's work.  Runs name distinct nodes, so reading the next
+// node's records early sees the same values as reading them later.
+// ------------------------------------------------------------------------
+struct RunMeta {
+    int32_t u;  // 0: no run (past the end, or an invalid arc's key)
+    uint32_t start, cnt;
+};
+
+__device__ __forceinline__ RunMeta run_meta(const DevStore &s, const Runs &r, uint32_t ri, uint32_t nruns)
+{
+    RunMeta m = {0, 0, 0};
+    if (ri < nruns) {
+        const int32_t u = (int32_t)r.node[ri];
+        if (u >= 1 && u <= s.n) {
+            m.u = u;
+            m.start = r.start[ri];
+            m.cnt = r.count[ri];
+        }
+    }
+    return m;
+}
+
+struct RunPre {
+    NodeHdr h;
+    GroupHdr gh;  // this lane's group (lanes < 32)
+    DevEdge x;    // insert: this lane's arc of the first chunk
+    uint32_t f;   // insert with float weights: its fraction; delete: this lane's slot
+};
+
+template <bool DEL>
+__device__ __forceinline__ RunPre run_pre(const DevStore &s, const DevEdge *__restrict__ E,
+                                          const uint32_t *__restrict__ aux, const RunMeta &m, int ln)
+{
+    RunPre p;
+    p.h = NodeHdr{0, 0, 0, 0, 0, 0, 0, 0, 0};
+    p.gh = GroupHdr{0, 0, 0};
+    p.x = DevEdge{0, 0, 0u};
+    p.f = 0;
+    if (m.u) {
+        p.h = s.hdr[m.u];
+        if (ln < kGroups) p.gh = s.grp[(size_t)m.u * kGroups + ln];
+        if ((uint32_t)ln < m.cnt) {
+            if (DEL) {
+                p.f = aux[m.start + ln];
+            } else {
+                p.x = E[m.start + ln];
+                if (aux) p.f = aux[m.start + ln];
+            }
+        }
+    }
+    return p;
+}
+
+// The plan also reserves each wave's pool space: a wave's growth over all its
+// runs, added to the batch total with one atomic per pool per wave, gives the
+// wave's offset (wres[wave][pool]) into the batch's new space.  The apply
+// kernel runs with the same grid, so every wave sees the same runs in the same
+// order, makes the same growth decisions, and carves its regions from its own
+// reservation with a wave-local cursor -- no shared counter (one contended
+// 64-bit atomic costs ~12 ns on this part; a batch makes one per growing run).
+__global__ __launch_bounds__(256) void k_insert_plan(DevStore s, const DevEdge *__restrict__ E, Runs r,
+                                                     unsigned long long *__restrict__ wres)
 {
     const int ln = lane_id();
     const uint32_t nruns = *r.nruns;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     unsigned long long need[NPOOL] = {0, 0, 0, 0};
+    RunMeta m0 = run_meta(s, r, wave, nruns), m1 = run_meta(s, r, wave + nwaves, nruns);
+    RunPre p0 = run_pre<false>(s, E, nullptr, m0, ln);
     for (uint32_t ri = wave; ri < nruns; ri += nwaves) {
-        const int32_t u = (int32_t)r.node[ri];
-        if (u < 1 || u > s.n) continue;  // invalid arcs were flagged already
-        const uint32_t cnt = r.count[ri];
-        const NodeHdr h = s.hdr[u];
-        GroupHdr gh = {0, 0, 0};
-        if (ln < kGroups) gh = s.grp[(size_t)u * kGroups + ln];
-        const Growth G = plan_run(s, E + r.start[ri], cnt, h, gh, ln);
-        if (run_too_big(h, cnt, G.pcsum)) {
+        const RunMeta m2 = run_meta(s, r, ri + 2 * nwaves, nruns);
+        const RunPre p1 = run_pre<false>(s, E, nullptr, m1, ln);
+        const RunMeta m = m0;
+        const NodeHdr h = p0.h;
+        const GroupHdr gh = p0.gh;
+        const uint32_t w0 = p0.x.w;
+        m0 = m1;
+        m1 = m2;
+        p0 = p1;
+        if (!m.u) continue;  // invalid arcs were flagged already
+        const uint32_t cnt = m.cnt;
+        const Growth G = plan_run(s, E + m.start, cnt, h, gh, w0, ln);
+        if (run_too_big(h, G)) {
             if (ln == 0) atomicOr(&s.ctl->error, (unsigned)ERR_NOMEM);
             continue;
         }
         need[POOL_SLOT] += G.g_slots ? (uint64_t)slot_arrays(s.fw) * G.cap_slots : 0;
         need[POOL_BACK] += G.g_back ? G.cap_back : 0;
         need[POOL_HASH] += G.g_hash ? G.cap_hash : 0;
-        need[POOL_MEM] += sparse_sum64(G.g_g ? (uint64_t)G.cap_g : 0ull);
+        need[POOL_MEM] += dpp_sum64(G.g_g ? (uint64_t)G.cap_g : 0ull);
     }
     if (ln == 0)
         for (int p = 0; p < NPOOL; ++p)
-            if (need[p]) atomicAdd(&s.ctl->need[p], need[p]);
+            wres[(size_t)wave * NPOOL + p] = need[p] ? atomicAdd(&s.ctl->need[p], need[p]) : 0ull;
 }
 
-// Region of n elements from a pool; ~0 if the pool (sized by the host from
-// the plan) would overflow, in which case the caller writes nothing.
-__device__ __forceinline__ uint64_t bump(const DevStore &s, int pool, uint64_t n)
+// Region of n elements from a pool at the wave's cursor (wave-uniform); ~0
+// if the pool (sized by the host from the plan) would overflow, in which case
+// the caller writes nothing.
+__device__ __forceinline__ uint64_t bump(const DevStore &s, unsigned long long (&cur)[NPOOL], int pool, uint64_t n)
 {
-    const uint64_t at = atomicAdd(&s.ctl->top[pool], (unsigned long long)n);
+    const uint64_t at = cur[pool];
+    cur[pool] += n;
     if (at + n > s.cap[pool]) {
-        atomicOr(&s.ctl->error, (unsigned)ERR_OVERRUN);
+        if (lane_id() == 0) atomicOr(&s.ctl->error, (unsigned)ERR_OVERRUN);
         return ~0ull;
     }
     return at;
 }
 
-// `fr` (float weights only) holds the run's fractions, arc for arc.
-__device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, const uint32_t *__restrict__ fr,
-                           uint32_t cnt, int32_t u, int ln)
+// the batch's reservations are used: the pool tops move past them
+__global__ void k_commit_tops(Ctl *ctl)
 {
-    NodeHdr h = s.hdr[u];
-    GroupHdr gh = {0, 0, 0};
-    if (ln < kGroups) gh = s.grp[(size_t)u * kGroups + ln];
-    const Growth G = plan_run(s, e, cnt, h, gh, ln);
-    if (run_too_big(h, cnt, G.pcsum)) return;  // rejected by the plan
+    const int t = threadIdx.x;
+    if (t < NPOOL) ctl->top[t] += ctl->need[t];
+}
 
-    // ---- regions (lane 0 bumps node pools; groups bump once per wave) ----
+// `fr` (float weights only) holds the run's fractions, arc for arc; `p` the
+// node's records and this lane's first arc, loaded by the run loop.
+__device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, const uint32_t *__restrict__ fr,
+                           uint32_t cnt, int32_t u, const RunPre &p, unsigned long long (&cur)[NPOOL], int ln)
+{
+    NodeHdr h = p.h;
+    const GroupHdr gh = p.gh;
+    const Growth G = plan_run(s, e, cnt, h, gh, p.x.w, ln);
+    if (run_too_big(h, G)) return;  // rejected by the plan
+
+    // ---- regions from the wave's reservation (in plan order) ----
     uint64_t a_slot = h.slot_off, a_back = h.back_off, a_hash = h.hash_off;
     const uint64_t gneed = G.g_g ? (uint64_t)G.cap_g : 0ull;
-    const uint64_t gincl = sparse_incl64(gneed, ln);
+    const uint64_t gincl = dpp_incl64(gneed);
     const uint64_t gtotal = rl64(gincl, 63);
     uint64_t gbase = 0;
-    if (ln == 0) {
-        if (G.g_slots) a_slot = bump(s, POOL_SLOT, (uint64_t)slot_arrays(s.fw) * G.cap_slots);
-        if (G.g_back) a_back = bump(s, POOL_BACK, G.cap_back);
-        if (G.g_hash) a_hash = bump(s, POOL_HASH, G.cap_hash);
-        if (gtotal) gbase = bump(s, POOL_MEM, gtotal);
-    }
-    a_slot = rl64(a_slot, 0);
-    a_back = rl64(a_back, 0);
-    a_hash = rl64(a_hash, 0);
-    gbase = rl64(gbase, 0);
+    if (G.g_slots) a_slot = bump(s, cur, POOL_SLOT, (uint64_t)slot_arrays(s.fw) * G.cap_slots);
+    if (G.g_back) a_back = bump(s, cur, POOL_BACK, G.cap_back);
+    if (G.g_hash) a_hash = bump(s, cur, POOL_HASH, G.cap_hash);
+    if (gtotal) gbase = bump(s, cur, POOL_MEM, gtotal);
     if (a_slot == ~0ull || a_back == ~0ull || a_hash == ~0ull || gbase == ~0ull) return;  // node left as it was
     const uint64_t goff = G.g_g ? gbase + gincl - gneed : gh.off;
 
@@ -606,13 +690,13 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, con
     // ---- append arcs in batch order: ballot + prefix rank per bit group ----
     uint32_t szj = gh.size;
     uint32_t boff_run = h.back_used;
-    uint32_t fresh = 0;
     uint64_t fsum = 0;
     for (uint32_t c = 0; c < cnt; c += 64) {
         const uint32_t i = c + (uint32_t)ln;
         const bool act = i < cnt;
         DevEdge x = {0, 0, 0u};
-        if (act) x = e[i];
+        if (c == 0) x = p.x;
+        else if (act) x = e[i];
         const uint32_t slot = n0 + i;
         const uint32_t pc = (uint32_t)__popc(x.w);
         const uint32_t pincl = dpp_incl_add(pc);
@@ -622,7 +706,7 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, con
             nw[slot] = x.w;
             nb[slot] = my_boff;
             if (s.fw) {
-                const uint32_t f = fr[i];
+                const uint32_t f = c == 0 ? p.f : fr[i];
                 nf[slot] = f;
                 fsum += f;
             }
@@ -646,9 +730,9 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, con
             if (ln == j) szj += (uint32_t)__popcll(m);
         }
         boff_run += rl32(pincl, 63);
-        if (s.nbr && act) fresh += hash_add(htab, G.cap_hash, (uint32_t)x.v, 1u);
     }
-    hused += dpp_sum32(fresh);
+    // the run's arcs enter the neighbour table in k_hash_insert (one thread
+    // per arc), which also counts the new keys into hash_used
 
     // ---- headers ----
     if (ln < kGroups) {
@@ -679,16 +763,25 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, con
 // Runs by grid stride in node order (neighbouring runs touch neighbouring
 // node records; a shared work counter measured slower).
 __global__ __launch_bounds__(256) void k_insert_apply(DevStore s, const DevEdge *__restrict__ E,
-                                                      const uint32_t *__restrict__ FR, Runs r)
+                                                      const uint32_t *__restrict__ FR, Runs r,
+                                                      const unsigned long long *__restrict__ wres)
 {
     const int ln = lane_id();
     const uint32_t nruns = *r.nruns;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    unsigned long long cur[NPOOL];
+#pragma unroll
+    for (int p = 0; p < NPOOL; ++p) cur[p] = s.ctl->top[p] + wres[(size_t)wave * NPOOL + p];
+    RunMeta m0 = run_meta(s, r, wave, nruns), m1 = run_meta(s, r, wave + nwaves, nruns);
+    RunPre p0 = run_pre<false>(s, E, FR, m0, ln);
     for (uint32_t ri = wave; ri < nruns; ri += nwaves) {
-        const int32_t u = (int32_t)r.node[ri];
-        if (u < 1 || u > s.n) continue;
-        insert_run(s, E + r.start[ri], FR ? FR + r.start[ri] : nullptr, r.count[ri], u, ln);
+        const RunMeta m2 = run_meta(s, r, ri + 2 * nwaves, nruns);
+        const RunPre p1 = run_pre<false>(s, E, FR, m1, ln);
+        if (m0.u) insert_run(s, E + m0.start, FR ? FR + m0.start : nullptr, m0.cnt, m0.u, p0, cur, ln);
+        m0 = m1;
+        m1 = m2;
+        p0 = p1;
     }
 }
 
@@ -738,29 +831,27 @@ __global__ void k_delete_check(DevStore s, const unsigned long long *__restrict_
 // [S2, S2 + holes)), which lets the k-th hole read its survivor at S2 + k.
 constexpr int kDB = 8;
 
-__device__ void delete_run(const DevStore &s, const uint32_t *__restrict__ slots, uint32_t cnt, int32_t u, int ln)
+__device__ void delete_run(const DevStore &s, const uint32_t *__restrict__ slots, uint32_t cnt, int32_t u,
+                           const RunPre &p, int ln)
 {
-    const NodeHdr h = s.hdr[u];
+    const NodeHdr h = p.h;
     uint32_t *V = s.slot + h.slot_off;
     uint32_t *W = V + h.slot_cap;
     const uint32_t *B = V + 2ull * h.slot_cap;
     uint32_t *back = s.back + h.back_off;
     const bool own = ln < kGroups;
     const uint32_t bit = own ? 1u << ln : 0u, below = bit - 1u;
-    GroupHdr gh = {0, 0, 0};
-    if (own) gh = s.grp[(size_t)u * kGroups + ln];
+    const GroupHdr gh = p.gh;
     uint64_t *m = s.mem + gh.off;
-    const bool hashed = s.nbr && h.hash_cap;
     const bool single = cnt <= 64u;  // one staged chunk serves both passes
 
-    uint32_t csl = 0, cw = 0, cbo = 0, cv = 0;
+    uint32_t csl = 0, cw = 0, cbo = 0;
     auto stage = [&](uint32_t c0) {
         const uint32_t t = c0 + (uint32_t)ln;
         const bool in = t < cnt;
-        csl = in ? slots[t] : 0u;
+        csl = c0 == 0 ? p.f : (in ? slots[t] : 0u);
         cw = in ? W[csl] : 0u;  // 0 past the end: such a delete holds no group
         cbo = in ? B[csl] : 0u;
-        cv = in && hashed ? V[csl] : 0u;
     };
     // back-index positions in this lane's group of staged deletes q0..q0+kDB-1
     auto positions = [&](uint32_t q0, uint32_t(&pos)[kDB]) {
@@ -841,7 +932,6 @@ __device__ void delete_run(const DevStore &s, const uint32_t *__restrict__ slots
                 if (rk[q] < nsurv) back[mb[q] + __popc(mw[q] & below)] = pos[q];
         }
         if (c0 + (uint32_t)ln < cnt) {
-            if (hashed) hash_dec(s.hash + h.hash_off, h.hash_cap, cv);
             W[csl] = 0u;
             if (s.fw) {
                 uint32_t *FR = V + 3ull * h.slot_cap;
@@ -862,24 +952,22 @@ __device__ void delete_run(const DevStore &s, const uint32_t *__restrict__ slots
 // deleted tail members from the survivors by comparing slots in registers,
 // pairs the k-th hole with the k-th survivor there, and only gathers the
 // moved members' back offsets afterwards.
-__device__ void delete_small(const DevStore &s, const uint32_t *__restrict__ slots, uint32_t cnt, int32_t u, int ln)
+__device__ void delete_small(const DevStore &s, uint32_t cnt, int32_t u, const RunPre &p, int ln)
 {
-    const NodeHdr h = s.hdr[u];
+    const NodeHdr h = p.h;
     const bool own = ln < kGroups;
-    GroupHdr gh = {0, 0, 0};
-    if (own) gh = s.grp[(size_t)u * kGroups + ln];
+    const GroupHdr gh = p.gh;
     const bool in = (uint32_t)ln < cnt;
-    const uint32_t t_sl = in ? slots[ln] : 0u;
+    const uint32_t t_sl = p.f;  // 0 past the run
     uint32_t *V = s.slot + h.slot_off;
     uint32_t *W = V + h.slot_cap;
     const uint32_t *B = V + 2ull * h.slot_cap;
     uint32_t *back = s.back + h.back_off;
     const uint32_t bit = own ? 1u << ln : 0u, below = bit - 1u;
-    const bool hashed = s.nbr && h.hash_cap;
     uint64_t *m = s.mem + gh.off;
 
     // the deletes' slot records; this group's last c members
-    const uint32_t cw = in ? W[t_sl] : 0u, cbo = in ? B[t_sl] : 0u, cv = in && hashed ? V[t_sl] : 0u;
+    const uint32_t cw = in ? W[t_sl] : 0u, cbo = in ? B[t_sl] : 0u;
     const uint32_t S = gh.size, c = min(cnt, S);
     uint64_t e[kDB];
 #pragma unroll
@@ -946,7 +1034,6 @@ __device__ void delete_small(const DevStore &s, const uint32_t *__restrict__ slo
     if (kj) s.grp[(size_t)u * kGroups + ln].size = S2;
     uint64_t fsub = 0;
     if (in) {
-        if (hashed) hash_dec(s.hash + h.hash_off, h.hash_cap, cv);
         W[t_sl] = 0u;
         if (s.fw) {
             uint32_t *FR = V + 3ull * h.slot_cap;
@@ -1020,19 +1107,17 @@ __device__ void delete_group(const DevStore &s, const uint32_t *__restrict__ slo
     if (kj && ln == 0) s.grp[(size_t)u * kGroups + j].size = S2;
 }
 
-// After the last group task of a long run: neighbour counts, tombstones and
-// the node's alias table.
+// After the last group task of a long run: tombstones and the node's alias
+// table.
 __device__ void delete_finish(const DevStore &s, const uint32_t *__restrict__ slots, uint32_t cnt, int32_t u, int ln)
 {
     const NodeHdr h = s.hdr[u];
     uint32_t *V = s.slot + h.slot_off;
     uint32_t *W = V + h.slot_cap;
     uint32_t *FR = V + 3ull * h.slot_cap;  // float weights only
-    const bool hashed = s.nbr && h.hash_cap;
     uint64_t fsub = 0;
     for (uint32_t t = ln; t < cnt; t += 64) {
         const uint32_t sl = slots[t];
-        if (hashed) hash_dec(s.hash + h.hash_off, h.hash_cap, V[sl]);
         W[sl] = 0u;
         if (s.fw) {
             fsub += FR[sl];
@@ -1044,6 +1129,34 @@ __device__ void delete_finish(const DevStore &s, const uint32_t *__restrict__ sl
     const uint64_t Sm =
         alias_rebuild(s, u, ln < kGroups ? gh.size : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, ln);
     if (s.fw) store_fnode(s, u, Sm, s.fnode[u].F - dpp_sum64(fsub), ln);
+}
+
+// ------------------------------------------------------------------------
+// neighbour multisets, kept apart from the run kernels: one thread per arc
+// or delete, after the apply kernel has placed (and, when it grew, rehashed)
+// every node's table.  A hub's thousands of arcs then probe in parallel
+// instead of one 64-arc chunk at a time inside its run.
+// ------------------------------------------------------------------------
+__global__ void k_hash_insert(DevStore s, const DevEdge *__restrict__ E, int64_t k)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= k) return;
+    const DevEdge x = E[i];
+    if (x.u < 1 || x.u > s.n) return;
+    const NodeHdr h = s.hdr[x.u];
+    if (h.hash_cap && hash_add(s.hash + h.hash_off, h.hash_cap, (uint32_t)x.v, 1u))
+        atomicAdd(&s.hdr[x.u].hash_used, 1u);
+}
+
+__global__ void k_hash_delete(DevStore s, const uint32_t *__restrict__ us, const uint32_t *__restrict__ slots,
+                              int64_t k)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= k) return;
+    const int32_t u = (int32_t)us[i];
+    if (u < 1 || u > s.n) return;
+    const NodeHdr h = s.hdr[u];
+    if (h.hash_cap) hash_dec(s.hash + h.hash_off, h.hash_cap, s.slot[h.slot_off + slots[i]]);
 }
 
 __global__ void k_delete_big_list(Runs r, uint32_t *__restrict__ big, Ctl *ctl)
@@ -1061,11 +1174,12 @@ __global__ __launch_bounds__(256) void k_delete_apply(DevStore s, const uint32_t
     const int ln = lane_id();
     const uint32_t nruns = *r.nruns;
     const uint32_t nbig = s.ctl->nbig;
-    const uint32_t ntasks = nbig * kGroups + nruns;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t ngt = nbig * kGroups;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (uint32_t task = wave; task < ntasks; task += nwaves) {
-        if (task < nbig * kGroups) {
+    uint32_t task = wave;
+    for (; task < ngt; task += nwaves) {
+        {
             const uint32_t bi = task / kGroups, ri = big[bi];
             const int32_t u = (int32_t)r.node[ri];
             if (u < 1 || u > s.n) continue;
@@ -1079,16 +1193,28 @@ __global__ __launch_bounds__(256) void k_delete_apply(DevStore s, const uint32_t
                 __threadfence();  // every group's writes before the finish
                 delete_finish(s, sl, r.count[ri], u, ln);
             }
-            continue;
         }
-        const uint32_t ri = task - nbig * kGroups;
-        const int32_t u = (int32_t)r.node[ri];
-        if (u < 1 || u > s.n) continue;
-        const uint32_t cnt = r.count[ri];
-        if (cnt <= (uint32_t)kDB)
-            delete_small(s, slots + r.start[ri], cnt, u, ln);
-        else if (cnt <= kBigDelete)
-            delete_run(s, slots + r.start[ri], cnt, u, ln);
+    }
+    // the other runs, continuing this wave's stride (long runs were done above)
+    auto meta = [&](uint32_t t) {
+        RunMeta m = run_meta(s, r, t - ngt, nruns);
+        if (m.cnt > kBigDelete) m.u = 0;
+        return m;
+    };
+    RunMeta m0 = meta(task), m1 = meta(task + nwaves);
+    RunPre p0 = run_pre<true>(s, nullptr, slots, m0, ln);
+    for (; task - ngt < nruns; task += nwaves) {
+        const RunMeta m2 = meta(task + 2 * nwaves);
+        const RunPre p1 = run_pre<true>(s, nullptr, slots, m1, ln);
+        if (m0.u) {
+            if (m0.cnt <= (uint32_t)kDB)
+                delete_small(s, m0.cnt, m0.u, p0, ln);
+            else
+                delete_run(s, slots + m0.start, m0.cnt, m0.u, p0, ln);
+        }
+        m0 = m1;
+        m1 = m2;
+        p0 = p1;
     }
 }
 
@@ -1747,14 +1873,30 @@ void launch_gather_edges(const DevEdge *e, const uint32_t *order, int64_t k, Dev
     hipLaunchKernelGGL(k_gather_edges, dim3(blocks_for(k, 256)), dim3(256), 0, st, e, order, k, out, frac);
 }
 
-void launch_insert_plan(DevStore s, const DevEdge *sorted, Runs r, int64_t k, hipStream_t st)
+size_t insert_wres_bytes(int64_t k) { return (size_t)run_blocks(k) * 4 * NPOOL * sizeof(unsigned long long); }
+
+void launch_insert_plan(DevStore s, const DevEdge *sorted, Runs r, unsigned long long *wres, int64_t k,
+                        hipStream_t st)
 {
-    hipLaunchKernelGGL(k_insert_plan, dim3(run_blocks(k)), dim3(256), 0, st, s, sorted, r);
+    hipLaunchKernelGGL(k_insert_plan, dim3(run_blocks(k)), dim3(256), 0, st, s, sorted, r, wres);
 }
 
-void launch_insert_apply(DevStore s, const DevEdge *sorted, const uint32_t *frac, Runs r, int64_t k, hipStream_t st)
+void launch_insert_apply(DevStore s, const DevEdge *sorted, const uint32_t *frac, Runs r,
+                         const unsigned long long *wres, int64_t k, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_insert_apply, dim3(run_blocks(k)), dim3(256), 0, st, s, sorted, frac, r);
+    // same grid as the plan: wave w uses the reservation wave w made
+    hipLaunchKernelGGL(k_insert_apply, dim3(run_blocks(k)), dim3(256), 0, st, s, sorted, frac, r, wres);
+    hipLaunchKernelGGL(k_commit_tops, dim3(1), dim3(64), 0, st, s.ctl);
+}
+
+void launch_hash_insert(DevStore s, const DevEdge *sorted, int64_t k, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_hash_insert, dim3(blocks_for(k, 256)), dim3(256), 0, st, s, sorted, k);
+}
+
+void launch_hash_delete(DevStore s, const uint32_t *us, const uint32_t *slots, int64_t k, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_hash_delete, dim3(blocks_for(k, 256)), dim3(256), 0, st, s, us, slots, k);
 }
 
 void launch_delete_keys(const int32_t *dels, int64_t k, unsigned long long *keys, hipStream_t st)

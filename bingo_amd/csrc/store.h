@@ -171,8 +171,18 @@ void launch_insert_keys(const DevEdge *e, int64_t k, int32_t n, int fw, uint32_t
                         hipStream_t st);
 void launch_gather_edges(const DevEdge *e, const uint32_t *order, int64_t k, DevEdge *out, uint32_t *frac,
                          hipStream_t st);
-void launch_insert_plan(DevStore s, const DevEdge *sorted, Runs r, int64_t k, hipStream_t st);
-void launch_insert_apply(DevStore s, const DevEdge *sorted, const uint32_t *frac, Runs r, int64_t k, hipStream_t st);
+// The plan reserves each wave's pool space (wres: insert_wres_bytes(k)); the
+// apply, launched for the same k, allocates from it and then moves the pool
+// tops past the batch's total (Ctl::need).
+size_t insert_wres_bytes(int64_t k);
+void launch_insert_plan(DevStore s, const DevEdge *sorted, Runs r, unsigned long long *wres, int64_t k,
+                        hipStream_t st);
+void launch_insert_apply(DevStore s, const DevEdge *sorted, const uint32_t *frac, Runs r,
+                         const unsigned long long *wres, int64_t k, hipStream_t st);
+// neighbour multisets (nbr stores): the batch's arcs / deletes, one thread
+// each, after the matching apply kernel
+void launch_hash_insert(DevStore s, const DevEdge *sorted, int64_t k, hipStream_t st);
+void launch_hash_delete(DevStore s, const uint32_t *us, const uint32_t *slots, int64_t k, hipStream_t st);
 void launch_delete_keys(const int32_t *dels, int64_t k, unsigned long long *keys, hipStream_t st);
 void launch_delete_check(DevStore s, const unsigned long long *keys, int64_t k, uint32_t *us, uint32_t *slots,
                          hipStream_t st);

@@ -131,36 +131,41 @@ static uint32_t hash_get(const node_t *x, uint32_t key)
 
 /* ---------------- alias table ---------------- */
 
-/* Integer-exact Vose construction over 32 fixed slots (slot j <-> bit group j).
- * s_j = 32 * M_j, slot capacity S = sum M_j.  Small = {s_j < S}, large = rest.
- * Repeatedly pair the highest-index small l with the highest-index large g:
- * tau_l = s_l, alias_l = g, s_g -= S - s_l (g turns small when s_g < S).
- * Leftovers get tau = S, alias = self.  The stay threshold is the 64-bit
- * fraction T_j = floor(tau_j * 2^64 / S) (all-ones when tau_j == S), used as
+/* Integer-exact alias table over 32 fixed slots (slot j <-> bit group j),
+ * built by a sweep.  s_j = 32 * M_j, slot capacity S = sum M_j.  Light =
+ * {s_j < S}, heavy = the rest, each in ascending index order.  One heavy is
+ * current, with residual r (initially its s).  Each light i in turn takes
+ * tau_i = s_i and alias_i = the current heavy h, whose residual drops by the
+ * light's deficit S - s_i; whenever r < S the current heavy turns light on
+ * the spot: tau_h = r, alias_h = the next heavy, whose residual drops by
+ * S - r and which becomes current.  Heavies never turned light keep
+ * tau = S, alias = self.  The stay threshold is the 64-bit fraction
+ * T_j = floor(tau_j * 2^64 / S) (all-ones when tau_j == S), used as
  * "stay iff coin64 < T_j".  Replaces Bingo.cuh:111-158 (float, EPS=1e-3). */
 void orc_alias_build(const uint64_t M[32], uint64_t T[32], uint8_t alias[32])
 {
     u128 S = 0, s[32], tau[32];
-    uint32_t small = 0, large = 0;
+    int light[32], heavy[32], nl = 0, nh = 0;
     for (int j = 0; j < 32; ++j) S += M[j];
     for (int j = 0; j < 32; ++j) {
         s[j] = (u128)M[j] * 32u;
-        if (s[j] < S) small |= 1u << j; else large |= 1u << j;
-    }
-    while (small && large) {
-        int l = hibit32(small), gi = hibit32(large);
-        tau[l] = s[l];
-        alias[l] = (uint8_t)gi;
-        small &= ~(1u << l);
-        s[gi] -= S - s[l];
-        if (s[gi] < S) { large &= ~(1u << gi); small |= 1u << gi; }
-    }
-    uint32_t rest = small | large;
-    while (rest) {
-        int j = hibit32(rest);
-        rest &= ~(1u << j);
         tau[j] = S;
         alias[j] = (uint8_t)j;
+        if (s[j] < S) light[nl++] = j; else heavy[nh++] = j;
+    }
+    int k = 0;
+    u128 r = nh ? s[heavy[0]] : 0;
+    for (int t = 0; t < nl && k < nh; ++t) {
+        const int i = light[t];
+        tau[i] = s[i];
+        alias[i] = (uint8_t)heavy[k];
+        r -= S - s[i];
+        while (r < S && k + 1 < nh) {
+            const int h = heavy[k];
+            tau[h] = r;
+            alias[h] = (uint8_t)heavy[k + 1];
+            r = s[heavy[++k]] - (S - tau[h]);
+        }
     }
     for (int j = 0; j < 32; ++j)
         T[j] = (tau[j] == S) ? UINT64_MAX : (uint64_t)((tau[j] << 64) / S);
@@ -321,7 +326,14 @@ int orc_insert(orc_graph *g, const orc_edge *e, int64_t k)
     grouping G = group_by_node(g->n, k, &e[0].u, sizeof(orc_edge));
     for (int64_t r = 0; r < G.nruns; ++r) {
         node_t *x = &g->nodes[G.run_node[r]];
-        if ((uint64_t)x->nslots + (uint64_t)(G.runs[r + 1] - G.runs[r]) >= ORC_MARK) {
+        uint64_t add = 0;  /* integer weight the run adds */
+        for (int64_t t = G.runs[r]; t < G.runs[r + 1]; ++t) {
+            uint32_t w = e[G.order[t]].w, f;
+            if (g->fw) orc_split_float(e[G.order[t]].w, &w, &f);
+            add += w;
+        }
+        if ((uint64_t)x->nslots + (uint64_t)(G.runs[r + 1] - G.runs[r]) >= ORC_MAX_SLOTS ||
+            x->mass + add >= ORC_MAX_MASS) {
             grouping_free(&G);
             return ORC_ENOMEM;
         }

@@ -20,8 +20,10 @@
  *   - all 32 bit groups of a uint32 weight are kept (ref: LOGT/MAXLOG=16 cap);
  *   - every non-empty bit group keeps a member list (ref RATE>1 mode);
  *   - members store (slot, target) so a walk step needs no edge-record read;
- *   - alias table: 32 fixed slots, integer-exact Vose construction with a
+ *   - alias table: 32 fixed slots, integer-exact sweep construction with a
  *     64-bit stay threshold (ref: float/EPS table, Bingo.cuh:111-158);
+ *   - limits per node: fewer than 2^28 slots and integer weight below 2^59
+ *     (an insert batch past either is rejected whole with ORC_ENOMEM);
  *   - randomness: Philox4x32-10 keyed (seed, walker, step, attempt);
  *   - node2vec neighbour test is exact "candidate in N(prev)" (ref hash is
  *     lossy and tests prev in N(cur), operation.cuh:313, Bingo.cuh:39-58);
@@ -48,6 +50,8 @@ extern "C" {
 #define ORC_FRAC_TAG 0x40000000u   /* Philox counter word 3 of the fractional stream */
 #define ORC_FRAC_ITERS 4096u       /* fractional rejection rounds per draw */
 #define ORC_MAX_ATTEMPTS_FLOAT 65536 /* node2vec cap with float weights (attempt fits 16 bits) */
+#define ORC_MAX_SLOTS (1u << 28)      /* slots per node (the store's 28-bit group sizes) */
+#define ORC_MAX_MASS (1ull << 59)     /* integer weight per node (alias prefix sums stay in 64 bits) */
 
 enum { ORC_DEEPWALK = 0, ORC_NODE2VEC = 1, ORC_PPR = 2, ORC_SAMPLING = 3 };
 enum { ORC_W_CONST1 = 0, ORC_W_UNIFORM16 = 1, ORC_W_LOGUNI32 = 2, ORC_W_ZIPF24 = 3 };
@@ -83,7 +87,8 @@ orc_graph *orc_create_ex(int32_t n, int neighbor_index, int float_weights);
 void orc_destroy(orc_graph *g);
 int32_t orc_num_nodes(const orc_graph *g);
 
-/* updates: 0 on success or a negative code; on error nothing is changed */
+/* updates: 0 on success or a negative code; on error nothing is changed
+ * (ORC_ENOMEM: a node would reach ORC_MAX_SLOTS slots or ORC_MAX_MASS weight) */
 int orc_insert(orc_graph *g, const orc_edge *e, int64_t k);
 int orc_delete(orc_graph *g, const orc_del *d, int64_t k);
 

@@ -85,6 +85,37 @@ def alias_law(T, alias):
     return P
 
 
+def alias_prefix(M):
+    """The sweep alias table restated through prefix sums (the form the GPU
+    computes lane-parallel): light i (s_i < S) takes the first heavy whose
+    cumulative excess reaches the light's cumulative deficit before it; a
+    heavy whose cumulative excess E_k falls inside a light's deficit range
+    splits: tau = S - (end of that range - E_k), alias = the next heavy."""
+    S = sum(int(m) for m in M)
+    s = [32 * int(m) for m in M]
+    light = [j for j in range(32) if s[j] < S]
+    heavy = [j for j in range(32) if s[j] >= S]
+    tau, ali = [S] * 32, list(range(32))
+    A, acc = {}, 0
+    for j in light:
+        A[j] = acc
+        acc += S - s[j]
+    E, acc = [], 0
+    for h in heavy:
+        acc += s[h] - S
+        E.append(acc)
+    for i in light:
+        k = next(k for k in range(len(heavy)) if A[i] <= E[k])
+        ali[i], tau[i] = heavy[k], s[i]
+    for k, h in enumerate(heavy):
+        cont = [i for i in light if A[i] <= E[k] < A[i] + S - s[i]]
+        if cont and k + 1 < len(heavy):
+            end = A[cont[0]] + S - s[cont[0]]
+            tau[h], ali[h] = S - (end - E[k]), heavy[k + 1]
+    T = [M64 - 1 if tau[j] == S else (tau[j] << 64) // S for j in range(32)]
+    return T, ali
+
+
 def member_law(size):
     """Exact probability of each position under floor(x * size / 2^32)."""
     cuts = [-(-(p * M32) // size) for p in range(size + 1)]  # ceil(p*2^32/size)

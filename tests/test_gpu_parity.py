@@ -294,6 +294,10 @@ def test_device_entry_points(orc):
 
 
 def test_gpu_transition_law_chi_square(orc):
+    """Empirical one-step frequencies of 1.2e7 GPU draws (three walk seeds of
+    4e6 draws, pooled) from a 40-arc node with log-uniform weights against
+    w / sum(w), bins with more than 20 expected draws (a seeded sample: the
+    test's false-failure rate is the 1% of its threshold)."""
     from scipy import stats
 
     rng = random.Random(12)
@@ -302,12 +306,15 @@ def test_gpu_transition_law_chi_square(orc):
     e = np.array([(1, i + 2, x) for i, x in enumerate(w)], B.EDGE_DT)
     g = B.Graph()
     g.load(n, e)
-    N = 1_000_000
-    paths, steps = g.walk(B.SAMPLING, B.walk_params(seed=5, max_len=2), starts=np.ones(N, np.int32))
-    cnt = np.bincount(paths[:, 1], minlength=n + 1)[2:]
-    exp = np.array(w, float) / sum(w) * N
-    keep = exp > 5
-    assert stats.chisquare(cnt[keep], exp[keep] * cnt[keep].sum() / exp[keep].sum()).pvalue > 0.01
+    N = 4_000_000
+    cnt = np.zeros(40)
+    for seed in (1, 2, 3):
+        paths, steps = g.walk(B.SAMPLING, B.walk_params(seed=seed, max_len=2), starts=np.ones(N, np.int32))
+        assert steps == N
+        cnt += np.bincount(paths[:, 1], minlength=n + 1)[2:]
+    exp = np.array(w, float) / sum(w)
+    keep = exp * cnt.sum() > 20
+    assert stats.chisquare(cnt[keep], exp[keep] / exp[keep].sum() * cnt[keep].sum()).pvalue > 0.01
 
 
 @pytest.mark.parametrize("knobs", [

@@ -8,7 +8,7 @@ from fractions import Fraction
 import numpy as np
 import pytest
 
-from tests.brute import BruteGraph, alias_law, check_against_brute, split_dump
+from tests.brute import BruteGraph, alias_law, alias_prefix, check_against_brute, split_dump
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -42,6 +42,9 @@ def test_alias_exact_law(orc, case):
     M = MASS_CASES[case] if case < len(MASS_CASES) else _random_masses(rng)
     T, a = orc.alias_build(np.array(M, dtype=np.uint64))
     S = sum(M)
+    # the sequential sweep and its prefix-sum restatement give the same table
+    T2, a2 = alias_prefix(M)
+    assert [int(x) for x in T] == T2 and [int(x) for x in a] == a2
     P = alias_law(T, a)
     assert sum(P) == 1
     for j in range(32):
@@ -55,6 +58,36 @@ def test_alias_exact_law(orc, case):
             assert P[j] == 0
         elif want > Fraction(1, 1 << 40):
             assert abs(P[j] - want) / want <= Fraction(1, 10**6)
+
+
+def test_alias_sweep_equals_prefix_form(orc):
+    """Many heavies, heavies of exactly one slot's capacity (zero excess),
+    chains of heavies turning light, single groups: the oracle's sequential
+    sweep equals the prefix-sum form on every case."""
+    rng = random.Random(99)
+    cases = []
+    for _ in range(400):
+        kind = rng.randrange(4)
+        if kind == 0:  # equal-ish masses: many heavies near capacity
+            M = [rng.choice([0, 1, 2, 3]) << 20 for _ in range(32)]
+        elif kind == 1:  # a few heavies, many small lights
+            M = [rng.choice([0, 1, 5]) for _ in range(32)]
+            for j in rng.sample(range(32), rng.randint(1, 4)):
+                M[j] = rng.randint(1, 1 << 40)
+        elif kind == 2:  # sizes << j as in the store
+            M = [rng.choice([0, 1, 2, 7, 64, 1000]) << j for j in range(32)]
+        else:
+            M = [rng.randrange(1 << rng.randint(1, 50)) for _ in range(32)]
+        cases.append(M)
+    cases.append([1] * 16 + [0] * 16)  # 16 heavies of exactly S / 32 * 2
+    cases.append([3] + [1] * 31)
+    for M in cases:
+        T, a = orc.alias_build(np.array(M, dtype=np.uint64))
+        T2, a2 = alias_prefix(M)
+        assert [int(x) for x in T] == T2 and [int(x) for x in a] == a2, M
+        if sum(M):
+            P = alias_law(T, a)
+            assert all(abs(P[j] - Fraction(M[j], sum(M))) <= Fraction(32, 1 << 64) for j in range(32))
 
 
 def _rand_edges(rng, n, k, wmax_bits=32, hub=None):
