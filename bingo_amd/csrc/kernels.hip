@@ -299,8 +299,11 @@ __device__ __forceinline__ uint64_t frac64(uint64_t tau, uint64_t S)
     return q;
 }
 
-// Returns the node's integer mass S (all lanes).  All 64 lanes active.
-__device__ uint64_t alias_rebuild(const DevStore &s, int32_t u, uint32_t size, uint64_t off, uint64_t hmeta, int ln)
+// Returns the node's integer mass S (all lanes).  All 64 lanes active.  With
+// float weights, F is the node's new fraction mass: the node's FNode is
+// written and every slot carries the threshold's 17-bit ceiling.
+__device__ uint64_t alias_rebuild(const DevStore &s, int32_t u, uint32_t size, uint64_t off, uint64_t hmeta,
+                                  uint64_t F, int ln)
 {
     const bool own = ln < kGroups;
     const uint64_t M = own ? ((uint64_t)size << ln) : 0ull;
@@ -333,6 +336,20 @@ __device__ uint64_t alias_rebuild(const DevStore &s, int32_t u, uint32_t size, u
     }
     uint64_t T = ~0ull;
     if (own && tau != S) T = frac64(tau, S);
+    uint64_t thr_hi = 0;
+    if (s.fw) {
+        uint64_t thr = 0;
+        if (ln == 0) {
+            thr = frac_threshold(S, F);
+            FNode f;
+            f.F = F;
+            f.thr = thr;
+            s.fnode[u] = f;
+        }
+        thr = rl64(thr, 0);
+        thr_hi = (thr >> 47) + ((thr & ((1ull << 47) - 1)) != 0);
+        thr_hi = thr_hi > kThrHiMax ? kThrHiMax : thr_hi;
+    }
     const uint32_t size_a = __shfl(size, ali, 64);
     const uint64_t off_a = __shfl(off, ali, 64);
     if (own) {
@@ -340,21 +357,10 @@ __device__ uint64_t alias_rebuild(const DevStore &s, int32_t u, uint32_t size, u
         ae.T = T;
         ae.stay = off | ((uint64_t)size << kOffBits);
         ae.alias = off_a | ((uint64_t)size_a << kOffBits);
-        ae.meta = (uint64_t)ali | (hmeta << 5);
+        ae.meta = (uint64_t)ali | (hmeta << 5) | (thr_hi << 47);
         s.alias[(size_t)u * kGroups + ln] = ae;
     }
     return S;
-}
-
-// float weights: a node's fraction mass and threshold after an update
-__device__ __forceinline__ void store_fnode(const DevStore &s, int32_t u, uint64_t S, uint64_t F, int ln)
-{
-    if (ln == 0) {
-        FNode f;
-        f.F = F;
-        f.thr = frac_threshold(S, F);
-        s.fnode[u] = f;
-    }
 }
 
 // ------------------------------------------------------------------------
@@ -756,8 +762,8 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, con
         }
         s.hdr[u] = h;
     }
-    const uint64_t S = alias_rebuild(s, u, ln < kGroups ? szj : 0u, goff, s.nbr ? make_hmeta(a_hash, G.cap_hash) : 0ull, ln);
-    if (s.fw) store_fnode(s, u, S, s.fnode[u].F + dpp_sum64(fsum), ln);
+    const uint64_t F = s.fw ? s.fnode[u].F + dpp_sum64(fsum) : 0ull;
+    alias_rebuild(s, u, ln < kGroups ? szj : 0u, goff, s.nbr ? make_hmeta(a_hash, G.cap_hash) : 0ull, F, ln);
 }
 
 // Runs by grid stride in node order (neighbouring runs touch neighbouring
@@ -942,8 +948,8 @@ __device__ void delete_run(const DevStore &s, const uint32_t *__restrict__ slots
     }
     if (bad) atomicOr(&s.ctl->error, (unsigned)ERR_OVERRUN);
     if (kj) s.grp[(size_t)u * kGroups + ln].size = S2;
-    const uint64_t Sm = alias_rebuild(s, u, own ? S2 : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, ln);
-    if (s.fw) store_fnode(s, u, Sm, s.fnode[u].F - dpp_sum64(fsub), ln);
+    const uint64_t F = s.fw ? s.fnode[u].F - dpp_sum64(fsub) : 0ull;
+    alias_rebuild(s, u, own ? S2 : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, F, ln);
 }
 
 // A run of at most kDB deletes: the same result as delete_run in fewer
@@ -1041,8 +1047,8 @@ __device__ void delete_small(const DevStore &s, uint32_t cnt, int32_t u, const R
             FR[t_sl] = 0u;
         }
     }
-    const uint64_t Sm = alias_rebuild(s, u, own ? S2 : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, ln);
-    if (s.fw) store_fnode(s, u, Sm, s.fnode[u].F - dpp_sum64(fsub), ln);
+    const uint64_t F = s.fw ? s.fnode[u].F - dpp_sum64(fsub) : 0ull;
+    alias_rebuild(s, u, own ? S2 : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, F, ln);
 }
 
 // One bit group j of a long delete run, by one wave with lanes across the
@@ -1126,9 +1132,9 @@ __device__ void delete_finish(const DevStore &s, const uint32_t *__restrict__ sl
     }
     GroupHdr gh = {0, 0, 0};
     if (ln < kGroups) gh = s.grp[(size_t)u * kGroups + ln];
-    const uint64_t Sm =
-        alias_rebuild(s, u, ln < kGroups ? gh.size : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, ln);
-    if (s.fw) store_fnode(s, u, Sm, s.fnode[u].F - dpp_sum64(fsub), ln);
+    const uint64_t F = s.fw ? s.fnode[u].F - dpp_sum64(fsub) : 0ull;
+    alias_rebuild(s, u, ln < kGroups ? gh.size : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, F,
+                  ln);
 }
 
 // ------------------------------------------------------------------------
@@ -1224,21 +1230,33 @@ __global__ __launch_bounds__(256) void k_delete_apply(DevStore s, const uint32_t
 // One draw on node `cur` (NodeData::sample, include/Bingo.cuh:220-267):
 // slot i = top 5 bits of r.x; the 64-bit coin (r.y:r.z) against T_i picks
 // group i or its partner; member floor(r.w * size / 2^32) of that group.
-// `hm` receives cur's neighbour-table locator carried by the slot.
-__device__ __forceinline__ int32_t sample_step(const AliasEnt *__restrict__ alias, const uint64_t *__restrict__ mem,
-                                               int32_t cur, uint4 r, uint64_t &hm)
+// `hm` receives cur's neighbour-table locator carried by the slot.  Split in
+// two so that callers can issue the slot read together with other reads.
+__device__ __forceinline__ void alias_slot(const AliasEnt *__restrict__ alias, int32_t cur, uint4 r, uint4 &q0,
+                                           uint4 &q1)
 {
-    const uint32_t i = r.x >> 27;
-    const uint4 *p = reinterpret_cast<const uint4 *>(alias + (size_t)cur * kGroups + i);
-    const uint4 q0 = p[0], q1 = p[1];
+    const uint4 *p = reinterpret_cast<const uint4 *>(alias + (size_t)cur * kGroups + (r.x >> 27));
+    q0 = p[0];
+    q1 = p[1];
+}
+__device__ __forceinline__ int32_t slot_member(const uint64_t *__restrict__ mem, uint4 r, uint4 q0, uint4 q1,
+                                               uint64_t &hm)
+{
     const uint64_t T = ((uint64_t)q0.y << 32) | q0.x;
     const uint64_t coin = ((uint64_t)r.y << 32) | r.z;
     const uint64_t loc = coin < T ? (((uint64_t)q0.w << 32) | q0.z) : (((uint64_t)q1.y << 32) | q1.x);
-    hm = (((uint64_t)q1.w << 32) | q1.z) >> 5;
+    hm = ((((uint64_t)q1.w << 32) | q1.z) >> 5) & kHmetaMask;
     const uint32_t sz = (uint32_t)(loc >> kOffBits);
     if (sz == 0u) return -1;
     const uint32_t pos = __umulhi(r.w, sz);
     return (int32_t)reinterpret_cast<const uint32_t *>(mem + (loc & kOffMask36) + pos)[1];
+}
+__device__ __forceinline__ int32_t sample_step(const AliasEnt *__restrict__ alias, const uint64_t *__restrict__ mem,
+                                               int32_t cur, uint4 r, uint64_t &hm)
+{
+    uint4 q0, q1;
+    alias_slot(alias, cur, r, q0, q1);
+    return slot_member(mem, r, q0, q1, hm);
 }
 
 // One draw on `cur` with float weights (NodeData::sample's wtype::Float
@@ -1248,31 +1266,45 @@ __device__ __forceinline__ int32_t sample_step(const AliasEnt *__restrict__ alia
 // uniform slot accepted when a 32-bit uniform is below its fraction, round
 // `it` drawing from counter word 3 = kFracTag | it << 16 | attempt, up to
 // kFracIters rounds.  Otherwise, or when every round rejects, the integer
-// draw from `r`.  FW false compiles to sample_step alone.
+// draw from `r`.  The alias slot and the threshold are read together.  FW
+// false compiles to sample_step alone.
 template <bool FW>
 __device__ __forceinline__ int32_t draw(const DevStore &s, int32_t cur, uint4 r, uint64_t &hm, uint32_t w0, uint32_t w1,
                                         uint32_t step, uint32_t att, uint32_t k0, uint32_t k1)
 {
-    if (FW) {
-        const uint64_t thr = s.fnode[cur].thr;
-        if (thr) {
-            uint4 b = philox(w0, w1, step, kFracTag | att, k0, k1);
-            const uint64_t coin = ((uint64_t)b.x << 32) | b.y;
-            if (thr == ~0ull || coin < thr) {
-                const NodeHdr h = s.hdr[cur];
-                const uint32_t *V = s.slot + h.slot_off, *FR = V + 3ull * h.slot_cap;
-                for (uint32_t it = 0; it < kFracIters; ++it) {
-                    if (it) b = philox(w0, w1, step, kFracTag | (it << 16) | att, k0, k1);
-                    const uint32_t sl = __umulhi(b.z, h.nslots);
-                    if (b.w < FR[sl]) {
-                        hm = make_hmeta(h.hash_off, h.hash_cap);
-                        return (int32_t)V[sl];
-                    }
+    if (!FW) return sample_step(s.alias, s.mem, cur, r, hm);
+    uint4 q0, q1;
+    alias_slot(s.alias, cur, r, q0, q1);
+    const uint32_t th = q1.w >> 15;  // ceil(thr / 2^47), saturated; 0 iff thr = 0
+    if (th) {
+        uint4 b = philox(w0, w1, step, kFracTag | att, k0, k1);
+        const uint64_t coin = ((uint64_t)b.x << 32) | b.y;
+        const uint32_t c = (uint32_t)(coin >> 47);
+        // (th - 1) 2^47 < thr <= th 2^47 (no upper bound when saturated):
+        // the top 17 bits decide unless they fall on the boundary
+        bool frac;
+        if (c + 1u < th)
+            frac = true;
+        else if (c >= th && th < kThrHiMax)
+            frac = false;
+        else {
+            const uint64_t thr = s.fnode[cur].thr;
+            frac = thr == ~0ull || coin < thr;
+        }
+        if (frac) {
+            const NodeHdr h = s.hdr[cur];
+            const uint32_t *V = s.slot + h.slot_off, *FR = V + 3ull * h.slot_cap;
+            for (uint32_t it = 0; it < kFracIters; ++it) {
+                if (it) b = philox(w0, w1, step, kFracTag | (it << 16) | att, k0, k1);
+                const uint32_t sl = __umulhi(b.z, h.nslots);
+                if (b.w < FR[sl]) {
+                    hm = make_hmeta(h.hash_off, h.hash_cap);
+                    return (int32_t)V[sl];
                 }
             }
         }
     }
-    return sample_step(s.alias, s.mem, cur, r, hm);
+    return slot_member(s.mem, r, q0, q1, hm);
 }
 
 template <int APP, bool FW>

@@ -56,15 +56,21 @@ struct FNode {
 constexpr uint32_t kFracTag = 0x40000000u;     // Philox counter word 3 of the fractional stream
 constexpr uint32_t kFracIters = 4096u;         // fractional rejection rounds per draw
 constexpr int32_t kMaxAttemptsFloat = 65536;   // node2vec cap with float weights (attempt in 16 bits)
+constexpr uint32_t kThrHiMax = (1u << 17) - 1;  // the alias slot's 17-bit copy of thr saturates here
+constexpr uint64_t kHmetaMask = (1ull << 42) - 1;
 
 // One 32-byte alias slot i of a node:
 //   T     stay iff coin64 < T
 //   stay  locator of group i:        member offset (36 bits) | size << 36
 //   alias locator of the partner:    member offset (36 bits) | size << 36
-//   meta  partner index (5 bits) | the node's neighbour-table locator << 5,
-//         locator = (log2 cap + 1) | table offset << 6 (0: no table)
-// so a walk step learns the group, its members and (for node2vec) where
-// the node's neighbour table lives from a single read.
+//   meta  partner index (5 bits) | the node's neighbour-table locator << 5
+//         (42 bits: (log2 cap + 1) | table offset << 6, 0: no table)
+//         | with float weights, the node's fractional-part threshold
+//         rounded up to 17 bits << 47 (ceil(thr / 2^47), at most 2^17 - 1;
+//         0 iff thr = 0)
+// so a walk step learns the group, its members, (for node2vec) where the
+// node's neighbour table lives and (for float weights) almost always whether
+// to take the fractional part, from a single read.
 struct AliasEnt {
     uint64_t T;
     uint64_t stay;
