@@ -269,17 +269,16 @@ static int do_delete(bingo_graph *g, const int32_t *d_pairs, int64_t k)
     launch_delete_check(g->s, kb, k, scp<uint32_t>(g, SC_KEYS2), scp<uint32_t>(g, SC_VALS2), g->st);
     Runs r;
     if ((rc = encode_runs(g, scp<uint32_t>(g, SC_KEYS2), k, &r))) return rc;
-    // long runs (more than kBigDelete deletes) and a zeroed count per run
+    // long runs (more than kBigDelete deletes)
     const size_t nbig_max = (size_t)k / kBigDelete + 1;
-    if ((rc = ensure_scratch(g, SC_BIG, 2 * nbig_max * sizeof(uint32_t)))) return rc;
+    if ((rc = ensure_scratch(g, SC_BIG, nbig_max * sizeof(uint32_t)))) return rc;
     uint32_t *big = scp<uint32_t>(g, SC_BIG);
     launch_delete_big_list(r, k, big, g->s.ctl, g->st);
-    HIPCHK(hipMemsetAsync(big + nbig_max, 0, nbig_max * sizeof(uint32_t), g->st));
     if ((rc = read_ctl(g))) return rc;
     if (g->ctl_host->error & ERR_INVAL)
         return fail(BINGO_EINVAL, "delete batch names a node outside 1..n, a slot that is not live, or a slot twice");
     if (g->ctl_host->error) return fail(BINGO_EDEVICE, "store reported an internal error");
-    launch_delete_apply(g->s, scp<uint32_t>(g, SC_VALS2), r, big, big + nbig_max, k, g->st);
+    launch_delete_apply(g->s, scp<uint32_t>(g, SC_VALS2), r, big, k, g->st);
     if (g->s.nbr) launch_hash_delete(g->s, scp<uint32_t>(g, SC_KEYS2), scp<uint32_t>(g, SC_VALS2), k, g->st);
     HIPCHK(hipGetLastError());
     g->arcs -= k;

@@ -43,7 +43,7 @@ enum Scratch {
     SC_AUX1,
     SC_AUX2,
     SC_AUX3,
-    SC_BIG,   // delete batch: long runs and their finished group tasks
+    SC_BIG,   // delete batch: long runs
     SC_FRAC,  // insert batch with float weights: fractions in node order
     SC_WRES,  // insert batch: each plan wave's pool reservation
     NSCRATCH

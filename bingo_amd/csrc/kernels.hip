@@ -461,6 +461,12 @@ __device__ __forceinline__ uint32_t grow_cap(const DevStore &s, uint32_t need, u
     return pow2_at_least(max(need, least));
 }
 
+// Runs of at most kSmallRun arcs (most runs of a batch spread over many
+// nodes) take a lane-per-group path: lane j walks the arcs and tests bit j,
+// instead of one ballot per set bit.  Both paths give the same counts, and the
+// same member and back-index order (arcs in run order within each group).
+constexpr uint32_t kSmallRun = 4;
+
 // counts of the run and the growth decisions; identical in plan and apply.
 // w0: this lane's arc of the first chunk (already loaded; 0 past the run)
 __device__ Growth plan_run(const DevStore &s, const DevEdge *__restrict__ e, uint32_t cnt, const NodeHdr &h,
@@ -468,26 +474,37 @@ __device__ Growth plan_run(const DevStore &s, const DevEdge *__restrict__ e, uin
 {
     Growth G;
     uint32_t addj = 0, pcsum = 0;
-    uint64_t wsum = 0;
-    for (uint32_t c = 0; c < cnt; c += 64) {
-        const uint32_t i = c + (uint32_t)ln;
-        const uint32_t w = c == 0 ? w0 : (i < cnt ? e[i].w : 0u);
-        wsum += w;
-        const uint32_t bits = dpp_or32(w);
-        for (uint32_t b = bits; b;) {
-            const int j = __ffs(b) - 1;
-            b &= b - 1u;
-            const uint64_t m = __ballot((w >> j) & 1u);
-            if (ln == j) addj += (uint32_t)__popcll(m);
+    uint64_t wtot = 0;
+    if (cnt <= kSmallRun) {  // lane j tests bit j of each arc in turn
+        for (uint32_t i = 0; i < cnt; ++i) {
+            const uint32_t wi = rl32(w0, (int)i);
+            if (ln < kGroups) addj += (wi >> ln) & 1u;
+            pcsum += (uint32_t)__popc(wi);
+            wtot += wi;
         }
-        pcsum += dpp_sum32((uint32_t)__popc(w));
+    } else {  // 64 arcs at a time, one ballot per set bit
+        uint64_t wsum = 0;
+        for (uint32_t c = 0; c < cnt; c += 64) {
+            const uint32_t i = c + (uint32_t)ln;
+            const uint32_t w = c == 0 ? w0 : (i < cnt ? e[i].w : 0u);
+            wsum += w;
+            const uint32_t bits = dpp_or32(w);
+            for (uint32_t b = bits; b;) {
+                const int j = __ffs(b) - 1;
+                b &= b - 1u;
+                const uint64_t m = __ballot((w >> j) & 1u);
+                if (ln == j) addj += (uint32_t)__popcll(m);
+            }
+            pcsum += dpp_sum32((uint32_t)__popc(w));
+        }
+        wtot = dpp_sum64(wsum);
     }
     G.cnt = cnt;
     G.addj = addj;
     G.pcsum = pcsum;
     // integer mass before (group sizes << j) plus the run's weights; the
     // run's sum is below 2^28 * 2^32 and the mass below 2^59, so no overflow
-    G.heavy_node = dpp_sum64(ln < kGroups ? (uint64_t)gh.size << ln : 0ull) + dpp_sum64(wsum) >= kMaxNodeMass;
+    G.heavy_node = dpp_sum64(ln < kGroups ? (uint64_t)gh.size << ln : 0ull) + wtot >= kMaxNodeMass;
     G.new_n = h.nslots + cnt;
     G.g_slots = G.new_n > h.slot_cap;
     G.cap_slots = G.g_slots ? grow_cap(s, G.new_n, (uint32_t)s.min_slots) : h.slot_cap;
@@ -679,8 +696,12 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, con
     uint32_t hused = h.hash_used;
     uint64_t *htab = s.hash + a_hash;
     if (G.g_hash) {
+        // Only this wave touches the new table until the kernel ends, so the
+        // zeros need only be complete before its own CAS: a workgroup-scope
+        // fence (a wait on the stores).  A device-scope fence here would write
+        // back and invalidate this XCD's L2 once per growing table.
         for (uint32_t i = ln; i < G.cap_hash; i += 64) htab[i] = 0ull;
-        __threadfence();
+        __threadfence_block();
         uint32_t fresh = 0;
         if (h.hash_cap) {
             const uint64_t *old = s.hash + h.hash_off;
@@ -690,52 +711,77 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, con
             }
         }
         hused = dpp_sum32(fresh);
-        __threadfence();
     }
 
     // ---- append arcs in batch order: ballot + prefix rank per bit group ----
     uint32_t szj = gh.size;
     uint32_t boff_run = h.back_used;
     uint64_t fsum = 0;
-    for (uint32_t c = 0; c < cnt; c += 64) {
-        const uint32_t i = c + (uint32_t)ln;
-        const bool act = i < cnt;
-        DevEdge x = {0, 0, 0u};
-        if (c == 0) x = p.x;
-        else if (act) x = e[i];
-        const uint32_t slot = n0 + i;
+    if (cnt <= kSmallRun) {
+        // slot records by lanes < cnt; then lane j appends group j's members
+        // arc by arc (arc i's back entries start at boff_i, bits ascending)
+        const DevEdge x = p.x;  // zero past the run
+        const bool act = (uint32_t)ln < cnt;
         const uint32_t pc = (uint32_t)__popc(x.w);
-        const uint32_t pincl = dpp_incl_add(pc);
-        const uint32_t my_boff = boff_run + pincl - pc;
+        const uint32_t my_boff = boff_run + dpp_incl_add(pc) - pc;
         if (act) {
-            nv[slot] = (uint32_t)x.v;
-            nw[slot] = x.w;
-            nb[slot] = my_boff;
+            nv[n0 + ln] = (uint32_t)x.v;
+            nw[n0 + ln] = x.w;
+            nb[n0 + ln] = my_boff;
             if (s.fw) {
-                const uint32_t f = c == 0 ? p.f : fr[i];
-                nf[slot] = f;
-                fsum += f;
+                nf[n0 + ln] = p.f;
+                fsum = p.f;
             }
         }
-        const uint64_t lt = (1ull << ln) - 1ull;
-        uint32_t rank = 0;
-        const uint64_t entry = ((uint64_t)(uint32_t)x.v << 32) | slot;
-        for (uint32_t b = dpp_or32(x.w); b;) {
-            const int j = __ffs(b) - 1;
-            b &= b - 1u;
-            const bool has = (x.w >> j) & 1u;
-            const uint64_t m = __ballot(has);
-            const uint32_t base = rl32(szj, j);
-            const uint64_t go = rl64(goff, j);
-            if (has) {
-                const uint32_t pos = base + (uint32_t)__popcll(m & lt);
-                s.mem[go + pos] = entry;
-                back[my_boff + rank] = pos;
-                ++rank;
+        for (uint32_t i = 0; i < cnt; ++i) {
+            const uint32_t wi = rl32(x.w, (int)i), vi = rl32((uint32_t)x.v, (int)i), bi = rl32(my_boff, (int)i);
+            if (ln < kGroups && ((wi >> ln) & 1u)) {
+                s.mem[goff + szj] = ((uint64_t)vi << 32) | (n0 + i);
+                back[bi + __popc(wi & ((1u << ln) - 1u))] = szj;
+                ++szj;
             }
-            if (ln == j) szj += (uint32_t)__popcll(m);
         }
-        boff_run += rl32(pincl, 63);
+    } else {
+        for (uint32_t c = 0; c < cnt; c += 64) {
+            const uint32_t i = c + (uint32_t)ln;
+            const bool act = i < cnt;
+            DevEdge x = {0, 0, 0u};
+            if (c == 0) x = p.x;
+            else if (act) x = e[i];
+            const uint32_t slot = n0 + i;
+            const uint32_t pc = (uint32_t)__popc(x.w);
+            const uint32_t pincl = dpp_incl_add(pc);
+            const uint32_t my_boff = boff_run + pincl - pc;
+            if (act) {
+                nv[slot] = (uint32_t)x.v;
+                nw[slot] = x.w;
+                nb[slot] = my_boff;
+                if (s.fw) {
+                    const uint32_t f = c == 0 ? p.f : fr[i];
+                    nf[slot] = f;
+                    fsum += f;
+                }
+            }
+            const uint64_t lt = (1ull << ln) - 1ull;
+            uint32_t rank = 0;
+            const uint64_t entry = ((uint64_t)(uint32_t)x.v << 32) | slot;
+            for (uint32_t b = dpp_or32(x.w); b;) {
+                const int j = __ffs(b) - 1;
+                b &= b - 1u;
+                const bool has = (x.w >> j) & 1u;
+                const uint64_t m = __ballot(has);
+                const uint32_t base = rl32(szj, j);
+                const uint64_t go = rl64(goff, j);
+                if (has) {
+                    const uint32_t pos = base + (uint32_t)__popcll(m & lt);
+                    s.mem[go + pos] = entry;
+                    back[my_boff + rank] = pos;
+                    ++rank;
+                }
+                if (ln == j) szj += (uint32_t)__popcll(m);
+            }
+            boff_run += rl32(pincl, 63);
+        }
     }
     // the run's arcs enter the neighbour table in k_hash_insert (one thread
     // per arc), which also counts the new keys into hash_used
@@ -952,12 +998,14 @@ __device__ void delete_run(const DevStore &s, const uint32_t *__restrict__ slots
     alias_rebuild(s, u, own ? S2 : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, F, ln);
 }
 
-// A run of at most kDB deletes: the same result as delete_run in fewer
+// A run of at most K (<= kDB) deletes: the same result as delete_run in fewer
 // dependent round trips.  Each group lane prefetches its list's last
 // min(cnt, size) members together with the deletes' slot records, tells the
 // deleted tail members from the survivors by comparing slots in registers,
 // pairs the k-th hole with the k-th survivor there, and only gathers the
-// moved members' back offsets afterwards.
+// moved members' back offsets afterwards.  Instantiated for K = 1, 2, 4, 8
+// (the loops are unrolled over K; most runs of a batch delete one arc).
+template <int K>
 __device__ void delete_small(const DevStore &s, uint32_t cnt, int32_t u, const RunPre &p, int ln)
 {
     const NodeHdr h = p.h;
@@ -975,15 +1023,15 @@ __device__ void delete_small(const DevStore &s, uint32_t cnt, int32_t u, const R
     // the deletes' slot records; this group's last c members
     const uint32_t cw = in ? W[t_sl] : 0u, cbo = in ? B[t_sl] : 0u;
     const uint32_t S = gh.size, c = min(cnt, S);
-    uint64_t e[kDB];
+    uint64_t e[K];
 #pragma unroll
-    for (int i = 0; i < kDB; ++i) e[i] = (uint32_t)i < c ? m[S - c + i] : 0ull;
+    for (int i = 0; i < K; ++i) e[i] = (uint32_t)i < c ? m[S - c + i] : 0ull;
 
     // member positions of this group's deletes
-    uint32_t pos[kDB], wq[kDB], sq[kDB];
+    uint32_t pos[K], wq[K], sq[K];
     uint32_t kj = 0;
 #pragma unroll
-    for (int q = 0; q < kDB; ++q) {
+    for (int q = 0; q < K; ++q) {
         wq[q] = rl32(cw, q);  // 0 for q >= cnt
         sq[q] = rl32(t_sl, q);
         pos[q] = (wq[q] & bit) ? back[rl32(cbo, q) + __popc(wq[q] & below)] : 0u;
@@ -992,29 +1040,29 @@ __device__ void delete_small(const DevStore &s, uint32_t cnt, int32_t u, const R
     const uint32_t S2 = S - kj;  // kj <= c: the tail [S2, S) is prefetched
 
     // survivors: tail members whose slot is not one of this group's deletes
-    bool surv[kDB];
-    uint32_t srank[kDB], ns = 0;
+    bool surv[K];
+    uint32_t srank[K], ns = 0;
 #pragma unroll
-    for (int i = 0; i < kDB; ++i) {
+    for (int i = 0; i < K; ++i) {
         bool del = false;
 #pragma unroll
-        for (int q = 0; q < kDB; ++q) del |= (wq[q] & bit) && (uint32_t)e[i] == sq[q];
+        for (int q = 0; q < K; ++q) del |= (wq[q] & bit) && (uint32_t)e[i] == sq[q];
         surv[i] = (uint32_t)i < c && (uint32_t)i + kj >= c && !del;
         srank[i] = ns;
         ns += surv[i] ? 1u : 0u;
     }
     // the k-th hole (deletes in slot order, position below S2) takes the
     // k-th survivor
-    bool hole[kDB], bad = false;
-    uint64_t mv[kDB];
+    bool hole[K], bad = false;
+    uint64_t mv[K];
     uint32_t hr = 0;
 #pragma unroll
-    for (int q = 0; q < kDB; ++q) {
+    for (int q = 0; q < K; ++q) {
         const bool h0 = (wq[q] & bit) && pos[q] < S2;
         uint64_t x = 0;
         bool found = false;
 #pragma unroll
-        for (int i = 0; i < kDB; ++i)
+        for (int i = 0; i < K; ++i)
             if (surv[i] && srank[i] == hr) {
                 x = e[i];
                 found = true;
@@ -1025,16 +1073,16 @@ __device__ void delete_small(const DevStore &s, uint32_t cnt, int32_t u, const R
         hr += h0 ? 1u : 0u;
     }
     // the moved members' back offsets
-    uint32_t mb[kDB], mw[kDB];
+    uint32_t mb[K], mw[K];
 #pragma unroll
-    for (int q = 0; q < kDB; ++q) {
+    for (int q = 0; q < K; ++q) {
         const uint32_t ms = (uint32_t)mv[q];
         mb[q] = hole[q] ? B[ms] : 0u;
         mw[q] = hole[q] ? W[ms] : 0u;
         if (hole[q]) m[pos[q]] = mv[q];
     }
 #pragma unroll
-    for (int q = 0; q < kDB; ++q)
+    for (int q = 0; q < K; ++q)
         if (hole[q]) back[mb[q] + __popc(mw[q] & below)] = pos[q];
     if (bad) atomicOr(&s.ctl->error, (unsigned)ERR_OVERRUN);
     if (kj) s.grp[(size_t)u * kGroups + ln].size = S2;
@@ -1165,6 +1213,21 @@ __global__ void k_hash_delete(DevStore s, const uint32_t *__restrict__ us, const
     if (h.hash_cap) hash_dec(s.hash + h.hash_off, h.hash_cap, s.slot[h.slot_off + slots[i]]);
 }
 
+// one wave per long delete run: tombstones, fractions and alias table
+__global__ __launch_bounds__(256) void k_delete_finish(DevStore s, const uint32_t *__restrict__ slots, Runs r,
+                                                       const uint32_t *__restrict__ big)
+{
+    const int ln = lane_id();
+    const uint32_t nbig = s.ctl->nbig;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t bi = wave; bi < nbig; bi += nwaves) {
+        const uint32_t ri = big[bi];
+        const int32_t u = (int32_t)r.node[ri];
+        if (u >= 1 && u <= s.n) delete_finish(s, slots + r.start[ri], r.count[ri], u, ln);
+    }
+}
+
 __global__ void k_delete_big_list(Runs r, uint32_t *__restrict__ big, Ctl *ctl)
 {
     const uint32_t nruns = *r.nruns;
@@ -1173,9 +1236,12 @@ __global__ void k_delete_big_list(Runs r, uint32_t *__restrict__ big, Ctl *ctl)
 }
 
 // Tasks: first one per (long run, bit group), then every other run; a wave
-// takes tasks by grid stride, so long runs' group tasks start first.
+// takes tasks by grid stride, so long runs' group tasks start first.  Long
+// runs are finished by k_delete_finish after this kernel: the kernel boundary
+// orders every group task's writes before the finish, where a device-scope
+// fence inside the kernel would write back and invalidate an XCD's L2.
 __global__ __launch_bounds__(256) void k_delete_apply(DevStore s, const uint32_t *__restrict__ slots, Runs r,
-                                                      const uint32_t *__restrict__ big, uint32_t *__restrict__ done)
+                                                      const uint32_t *__restrict__ big)
 {
     const int ln = lane_id();
     const uint32_t nruns = *r.nruns;
@@ -1189,16 +1255,7 @@ __global__ __launch_bounds__(256) void k_delete_apply(DevStore s, const uint32_t
             const uint32_t bi = task / kGroups, ri = big[bi];
             const int32_t u = (int32_t)r.node[ri];
             if (u < 1 || u > s.n) continue;
-            const uint32_t *sl = slots + r.start[ri];
-            delete_group(s, sl, r.count[ri], u, (int)(task % kGroups), ln);
-            __threadfence();  // this group's writes before the count
-            uint32_t prev = 0;
-            if (ln == 0) prev = atomicAdd(&done[bi], 1u);
-            prev = __shfl(prev, 0, 64);
-            if (prev == kGroups - 1) {
-                __threadfence();  // every group's writes before the finish
-                delete_finish(s, sl, r.count[ri], u, ln);
-            }
+            delete_group(s, slots + r.start[ri], r.count[ri], u, (int)(task % kGroups), ln);
         }
     }
     // the other runs, continuing this wave's stride (long runs were done above)
@@ -1213,8 +1270,14 @@ __global__ __launch_bounds__(256) void k_delete_apply(DevStore s, const uint32_t
         const RunMeta m2 = meta(task + 2 * nwaves);
         const RunPre p1 = run_pre<true>(s, nullptr, slots, m1, ln);
         if (m0.u) {
-            if (m0.cnt <= (uint32_t)kDB)
-                delete_small(s, m0.cnt, m0.u, p0, ln);
+            if (m0.cnt == 1)
+                delete_small<1>(s, m0.cnt, m0.u, p0, ln);
+            else if (m0.cnt == 2)
+                delete_small<2>(s, m0.cnt, m0.u, p0, ln);
+            else if (m0.cnt <= 4)
+                delete_small<4>(s, m0.cnt, m0.u, p0, ln);
+            else if (m0.cnt <= (uint32_t)kDB)
+                delete_small<kDB>(s, m0.cnt, m0.u, p0, ln);
             else
                 delete_run(s, slots + m0.start, m0.cnt, m0.u, p0, ln);
         }
@@ -1947,10 +2010,12 @@ void launch_delete_big_list(Runs r, int64_t k, uint32_t *big, Ctl *ctl, hipStrea
     hipLaunchKernelGGL(k_delete_big_list, dim3(blocks_for(k, 256)), dim3(256), 0, st, r, big, ctl);
 }
 
-void launch_delete_apply(DevStore s, const uint32_t *slots, Runs r, const uint32_t *big, uint32_t *done, int64_t k,
-                         hipStream_t st)
+void launch_delete_apply(DevStore s, const uint32_t *slots, Runs r, const uint32_t *big, int64_t k, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_delete_apply, dim3(run_blocks(k)), dim3(256), 0, st, s, slots, r, big, done);
+    hipLaunchKernelGGL(k_delete_apply, dim3(run_blocks(k)), dim3(256), 0, st, s, slots, r, big);
+    const int64_t nbig_max = k / kBigDelete + 1;  // waves loop over the list: at most 1024 blocks
+    hipLaunchKernelGGL(k_delete_finish, dim3(std::min<unsigned>(blocks_for(nbig_max * 64, 256), 1024u)), dim3(256), 0,
+                       st, s, slots, r, big);
 }
 
 void launch_walk(DevStore s, int app, const WalkArgs &a, int block, hipStream_t st)

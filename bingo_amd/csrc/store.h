@@ -192,13 +192,12 @@ void launch_hash_delete(DevStore s, const uint32_t *us, const uint32_t *slots, i
 void launch_delete_keys(const int32_t *dels, int64_t k, unsigned long long *keys, hipStream_t st);
 void launch_delete_check(DevStore s, const unsigned long long *keys, int64_t k, uint32_t *us, uint32_t *slots,
                          hipStream_t st);
-// Delete runs longer than kBigDelete are listed (big: run index, done: group
-// tasks finished, zeroed by the caller; room for k / kBigDelete + 1 entries)
-// and handled by one wave per bit group.
+// Delete runs longer than kBigDelete are listed (big: run indices, room for
+// k / kBigDelete + 1 entries), handled by one wave per bit group and then
+// finished (tombstones, alias table) by a second kernel.
 constexpr uint32_t kBigDelete = 128;
 void launch_delete_big_list(Runs r, int64_t k, uint32_t *big, Ctl *ctl, hipStream_t st);
-void launch_delete_apply(DevStore s, const uint32_t *slots, Runs r, const uint32_t *big, uint32_t *done, int64_t k,
-                         hipStream_t st);
+void launch_delete_apply(DevStore s, const uint32_t *slots, Runs r, const uint32_t *big, int64_t k, hipStream_t st);
 void launch_walk(DevStore s, int app, const WalkArgs &a, int block, hipStream_t st);
 struct PersistCfg {
     int block, wpl, depth, blocks;  // blocks = grid size
