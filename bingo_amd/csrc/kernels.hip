@@ -1295,12 +1295,20 @@ __global__ __launch_bounds__(256) void k_delete_apply(DevStore s, const uint32_t
 // group i or its partner; member floor(r.w * size / 2^32) of that group.
 // `hm` receives cur's neighbour-table locator carried by the slot.  Split in
 // two so that callers can issue the slot read together with other reads.
+// The slot is read as two 16-byte loads.  The empty asm takes both as
+// operands, so the compiler can neither narrow them to the fields a caller
+// happens to use nor turn them into a load of T followed by a dependent load
+// of the chosen locator: random gathers here are bound by the number of
+// requests, and two independent requests per slot is the fewest.
 __device__ __forceinline__ void alias_slot(const AliasEnt *__restrict__ alias, int32_t cur, uint4 r, uint4 &q0,
                                            uint4 &q1)
 {
-    const uint4 *p = reinterpret_cast<const uint4 *>(alias + (size_t)cur * kGroups + (r.x >> 27));
-    q0 = p[0];
-    q1 = p[1];
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 *p = reinterpret_cast<const u32x4 *>(alias + (size_t)cur * kGroups + (r.x >> 27));
+    u32x4 a = p[0], b = p[1];
+    asm volatile("" : "+v"(a), "+v"(b));
+    q0 = make_uint4(a.x, a.y, a.z, a.w);
+    q1 = make_uint4(b.x, b.y, b.z, b.w);
 }
 __device__ __forceinline__ int32_t slot_member(const uint64_t *__restrict__ mem, uint4 r, uint4 q0, uint4 q1,
                                                uint64_t &hm)
