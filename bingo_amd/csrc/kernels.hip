@@ -1300,15 +1300,27 @@ __global__ __launch_bounds__(256) void k_delete_apply(DevStore s, const uint32_t
 // happens to use nor turn them into a load of T followed by a dependent load
 // of the chosen locator: random gathers here are bound by the number of
 // requests, and two independent requests per slot is the fewest.
-__device__ __forceinline__ void alias_slot(const AliasEnt *__restrict__ alias, int32_t cur, uint4 r, uint4 &q0,
-                                           uint4 &q1)
+// Split in two so that a caller can do other work between issuing the loads
+// and pinning them (the pin waits for them).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void slot_load(const AliasEnt *__restrict__ alias, int32_t cur, uint4 r, u32x4 &a, u32x4 &b)
 {
-    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
     const u32x4 *p = reinterpret_cast<const u32x4 *>(alias + (size_t)cur * kGroups + (r.x >> 27));
-    u32x4 a = p[0], b = p[1];
+    a = p[0];
+    b = p[1];
+}
+__device__ __forceinline__ void slot_pin(u32x4 a, u32x4 b, uint4 &q0, uint4 &q1)
+{
     asm volatile("" : "+v"(a), "+v"(b));
     q0 = make_uint4(a.x, a.y, a.z, a.w);
     q1 = make_uint4(b.x, b.y, b.z, b.w);
+}
+__device__ __forceinline__ void alias_slot(const AliasEnt *__restrict__ alias, int32_t cur, uint4 r, uint4 &q0,
+                                           uint4 &q1)
+{
+    u32x4 a, b;
+    slot_load(alias, cur, r, a, b);
+    slot_pin(a, b, q0, q1);
 }
 __device__ __forceinline__ int32_t slot_member(const uint64_t *__restrict__ mem, uint4 r, uint4 q0, uint4 q1,
                                                uint64_t &hm)
@@ -1344,11 +1356,17 @@ __device__ __forceinline__ int32_t draw(const DevStore &s, int32_t cur, uint4 r,
                                         uint32_t step, uint32_t att, uint32_t k0, uint32_t k1)
 {
     if (!FW) return sample_step(s.alias, s.mem, cur, r, hm);
-    uint4 q0, q1;
-    alias_slot(s.alias, cur, r, q0, q1);
+    // the fractional stream's first block is computed while the slot loads
+    // (pinned with the slot, so it is not sunk past the wait)
+    u32x4 sa, sb;
+    slot_load(s.alias, cur, r, sa, sb);
+    const uint4 b0 = philox(w0, w1, step, kFracTag | att, k0, k1);
+    u32x4 pb = {b0.x, b0.y, b0.z, b0.w};
+    asm volatile("" : "+v"(sa), "+v"(sb), "+v"(pb));
+    const uint4 q0 = make_uint4(sa.x, sa.y, sa.z, sa.w), q1 = make_uint4(sb.x, sb.y, sb.z, sb.w);
+    uint4 b = make_uint4(pb.x, pb.y, pb.z, pb.w);
     const uint32_t th = q1.w >> 15;  // ceil(thr / 2^47), saturated; 0 iff thr = 0
     if (th) {
-        uint4 b = philox(w0, w1, step, kFracTag | att, k0, k1);
         const uint64_t coin = ((uint64_t)b.x << 32) | b.y;
         const uint32_t c = (uint32_t)(coin >> 47);
         // (th - 1) 2^47 < thr <= th 2^47 (no upper bound when saturated):
