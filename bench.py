@@ -44,8 +44,10 @@ sys.path.insert(0, ROOT)
 METRIC = "walk steps/s + edge updates/s on 1 MI355X (HBM GB/s %peak); p99 query latency"
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes (per walk step / draw / probe): node record 8, alias slot
-# 8, group header 8, member 4, target 4, path write 4; a neighbour probe 8
-B_STEP, B_PROBE = 36, 8
+# 8, group header 8, member 4, target 4, path write 4; a neighbour probe 8; a
+# node2vec fold lookup 24 (the node's mass 8, the return arc's 16-byte
+# neighbour entry)
+B_STEP, B_PROBE, B_FOLD = 36, 8, 24
 APP_NAMES = ["deepwalk", "node2vec", "ppr", "sampling"]
 
 CONFIGS = {
@@ -128,7 +130,8 @@ def cpu_baseline(base, n, cfg, seed, budget_s=15.0):
     assert o.load(base) == 0
     load_s = time.perf_counter() - t0
     log(f"cpu oracle loaded in {load_s:.1f}s ({threads} threads)")
-    wp = orc.walk_params(seed=seed, p=cfg["p"], q=cfg["q"], tp=cfg["tp"], max_len=cfg["max_len"])
+    wp = orc.walk_params(seed=seed, p=cfg["p"], q=cfg["q"], tp=cfg["tp"], max_len=cfg["max_len"],
+                         float_weights=bool(cfg.get("float")))
     nw = 4096
     while True:
         starts = (np.arange(nw, dtype=np.int64) * (n // nw) % n + 1).astype(np.int32)
@@ -398,7 +401,7 @@ def main():
     g.sync()
     B._check(B.lib().bingo_memcpy_h2d(g._h, d_steps.ptr, np.zeros(1, np.uint64).ctypes.data, 8))
     ins_ms = del_ms = walk_ms = kern_ms = 0.0
-    attempts = probes = 0
+    attempts = probes = folds = 0
     t0 = time.perf_counter()
     for i in range(K):
         a, b, c, st = step(W + i)
@@ -408,6 +411,7 @@ def main():
         kern_ms += st["walk_kernel_ms"]
         attempts += st["walk_attempts"]
         probes += st["walk_probes"]
+        folds += st["walk_folds"]
         log(f"step {i}: insert {batches[W + i][2]} in {a:.2f} ms, delete {batches[W + i][2]} in {b:.2f} ms, "
             f"walk {c:.2f} ms")
     g.sync()
@@ -435,11 +439,11 @@ def main():
     for width in (32, 8, 32, 8):  # second pass counts
         nr, ms = g.gather_probe(width)
         rates[width] = nr / (ms / 1e3)
-    ceiling_s = attempts / K * (1 / rates[32] + 1 / rates[8]) + probes / K / rates[8]
+    ceiling_s = attempts / K * (1 / rates[32] + 1 / rates[8]) + (probes + 2 * folds) / K / rates[8]
     log(f"random reads: {rates[32] / 1e9:.1f} G/s (32 B), {rates[8] / 1e9:.1f} G/s (8 B); "
         f"walk at {ceiling_s / (kern_ms / K / 1e3):.2f} of that ceiling")
 
-    bytes_per_launch = (B_STEP * attempts + B_PROBE * probes) / K
+    bytes_per_launch = (B_STEP * attempts + B_PROBE * probes + B_FOLD * folds) / K
     kern_s = kern_ms / K / 1e3
     achieved = bytes_per_launch / kern_s / 1e9
     traffic = measured_traffic(cfg["workload"])
@@ -472,6 +476,7 @@ def main():
         "walk_ms": walk_ms / K,
         "load_ms": load_ms,
         "walk_draws_per_step": attempts / max(steps, 1),
+        "walk_fold_lookups_per_step": folds / max(steps, 1),
         "walk_schedule": tuned,
         "plain_form_walk_steps_per_s": plain_rate,
         "batched_vs_plain": walk_rate / plain_rate,

@@ -77,6 +77,7 @@ class Stats(C.Structure):
         ("walk_steps", C.c_uint64),
         ("walk_attempts", C.c_uint64),
         ("walk_probes", C.c_uint64),
+        ("walk_folds", C.c_uint64),
         ("arcs", C.c_uint64),
         ("bytes_reserved", C.c_uint64),
         ("pool_used", C.c_uint64 * 4),

@@ -278,8 +278,10 @@ static int do_delete(bingo_graph *g, const int32_t *d_pairs, int64_t k)
     if (g->ctl_host->error & ERR_INVAL)
         return fail(BINGO_EINVAL, "delete batch names a node outside 1..n, a slot that is not live, or a slot twice");
     if (g->ctl_host->error) return fail(BINGO_EDEVICE, "store reported an internal error");
-    launch_delete_apply(g->s, scp<uint32_t>(g, SC_VALS2), r, big, k, g->st);
+    // neighbour entries first: they subtract each deleted slot's weight,
+    // which the apply kernel then tombstones
     if (g->s.nbr) launch_hash_delete(g->s, scp<uint32_t>(g, SC_KEYS2), scp<uint32_t>(g, SC_VALS2), k, g->st);
+    launch_delete_apply(g->s, scp<uint32_t>(g, SC_VALS2), r, big, k, g->st);
     HIPCHK(hipGetLastError());
     g->arcs -= k;
     return BINGO_OK;
@@ -354,13 +356,27 @@ int bingo::fill_walk_args(const bingo_walk_params *p, int app, WalkArgs *a, int 
     if (fw && p->max_attempts > kMaxAttemptsFloat)
         return fail(BINGO_EINVAL, "max_attempts above 65536 with float weights");
     if (app == BINGO_PPR && !(p->tp >= 0 && p->tp <= 1)) return fail(BINGO_EINVAL, "tp must be in [0, 1]");
-    // node2vec: accept alpha / max(1/p, 1, 1/q) with alpha = 1/p, 1, 1/q
-    // (ref operation.cuh:291,302-325; lim there is max(1/p, 1/q))
-    const double ip = 1.0 / p->p, iq = 1.0 / p->q;
-    double lim = std::max(1.0, std::max(ip, iq));
-    a->thr_return = thr27(ip / lim);
-    a->thr_neighbor = thr27(1.0 / lim);
-    a->thr_other = thr27(iq / lim);
+    // node2vec: alpha = 1/p (return), 1 (distance 1), 1/q (distance 2)
+    // (ref operation.cuh:291,302-325; lim there is max(1/p, 1/q)).  A proposal
+    // is accepted with alpha / env, env = max(1, 1/q); when 1/p > env the
+    // return arc's excess is folded into a region of its own (fold_r, x 2^27)
+    // that each attempt takes outright with probability w_ret fold_r /
+    // (S 2^27 + w_ret fold_r).  Float-weight stores, and factors
+    // (1/p - env) / env of 2^32 or more, keep the plain envelope
+    // max(1/p, 1, 1/q): folding needs integer weight sums.
+    const double ip = 1.0 / p->p, iq = 1.0 / p->q, env = std::max(1.0, iq);
+    if (!fw && ip > env && (ip - env) / env < 4294967296.0) {
+        a->thr_return = 1u << 27;
+        a->thr_neighbor = thr27(1.0 / env);
+        a->thr_other = thr27(iq / env);
+        a->fold_r = (uint64_t)std::floor((ip - env) / env * 134217728.0);
+    } else {
+        const double lim = std::max(ip, env);
+        a->thr_return = thr27(ip / lim);
+        a->thr_neighbor = thr27(1.0 / lim);
+        a->thr_other = thr27(iq / lim);
+        a->fold_r = 0;
+    }
     a->thr_stop = thr27(p->tp);
     a->seed = p->seed;
     a->max_len = p->max_len;
@@ -462,8 +478,8 @@ void bingo_destroy(bingo_graph *g)
 {
     if (!g) return;
     if (g->st) (void)hipStreamSynchronize(g->st);
-    void *bufs[] = {g->s.hdr, g->s.grp, g->s.alias, g->s.fnode, g->s.slot, g->s.back, g->s.mem, g->s.hash, g->s.ctl,
-                    g->d_steps};
+    void *bufs[] = {g->s.hdr, g->s.grp, g->s.alias, g->s.fnode, g->s.nmass, g->s.slot, g->s.back, g->s.mem,
+                    g->s.hash, g->s.ctl, g->d_steps};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (int i = 0; i < NSCRATCH; ++i)
@@ -487,12 +503,13 @@ int bingo_load_device(bingo_graph *g, int32_t n, const bingo_edge *d_edges, int6
     HIPCHK(hipSetDevice(g->device));
     const size_t n1 = (size_t)n + 1;
     const size_t nb = n1 * sizeof(NodeHdr) + n1 * kGroups * (sizeof(GroupHdr) + sizeof(AliasEnt)) +
-                      (g->s.fw ? n1 * sizeof(FNode) : 0);
+                      (g->s.fw ? n1 * sizeof(FNode) : 0) + (g->s.nbr ? n1 * sizeof(uint64_t) : 0);
     if (nb > g->budget) return fail(BINGO_ENOMEM, "node records exceed the HBM budget");
     bool ok = hipMalloc((void **)&g->s.hdr, n1 * sizeof(NodeHdr)) == hipSuccess &&
               hipMalloc((void **)&g->s.grp, n1 * kGroups * sizeof(GroupHdr)) == hipSuccess &&
               hipMalloc((void **)&g->s.alias, n1 * kGroups * sizeof(AliasEnt)) == hipSuccess &&
-              (!g->s.fw || hipMalloc((void **)&g->s.fnode, n1 * sizeof(FNode)) == hipSuccess);
+              (!g->s.fw || hipMalloc((void **)&g->s.fnode, n1 * sizeof(FNode)) == hipSuccess) &&
+              (!g->s.nbr || hipMalloc((void **)&g->s.nmass, n1 * sizeof(uint64_t)) == hipSuccess);
     if (!ok) {
         (void)hipGetLastError();
         return fail(BINGO_ENOMEM, "node record allocation failed");
@@ -859,6 +876,7 @@ int bingo_stats_get(bingo_graph *g, bingo_stats *out)
         out->walk_steps = g->ctl_host->steps;
         out->walk_attempts = g->ctl_host->attempts;
         out->walk_probes = g->ctl_host->probes;
+        out->walk_folds = g->ctl_host->folds;
     }
     return BINGO_OK;
 }

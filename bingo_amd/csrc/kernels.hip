@@ -150,33 +150,37 @@ __device__ __forceinline__ uint32_t hslot(uint32_t key, uint32_t cap)
     return (uint32_t)(((uint64_t)(key * 0x9E3779B1u) * cap) >> 32);
 }
 
-// add `cnt` to key's count; returns 1 if the key was new.  A new key is one
-// 64-bit CAS (key and count together); a present key adds to the count half.
-__device__ uint32_t hash_add(uint64_t *tab, uint32_t cap, uint32_t key, uint32_t cnt)
+// A neighbour entry is 16 bytes: key | arc count << 32, then the arcs'
+// weight sum (integer parts); a table of cap entries takes 2 cap pool words.
+// hash_add adds `cnt` arcs of total weight `w` to key; returns 1 if the key
+// was new.  A new key is one 64-bit CAS (key and count together); a present
+// key adds to the count half; the weight sum is one 64-bit atomic add.
+__device__ uint32_t hash_add(uint64_t *tab, uint32_t cap, uint32_t key, uint32_t cnt, uint64_t w)
 {
     uint32_t i = hslot(key, cap);
     const uint32_t mask = cap - 1u;
     const unsigned long long add = (unsigned long long)cnt << 32;
     for (;;) {
-        unsigned long long *p = reinterpret_cast<unsigned long long *>(tab + i);
+        unsigned long long *p = reinterpret_cast<unsigned long long *>(tab + 2ull * i);
         const unsigned long long old = atomicCAS(p, 0ull, add | key);
-        if (old == 0ull) return 1u;
-        if ((uint32_t)old == key) {
-            atomicAdd(p, add);
-            return 0u;
+        if (old == 0ull || (uint32_t)old == key) {
+            if (old) atomicAdd(p, add);
+            if (w) atomicAdd(p + 1, (unsigned long long)w);
+            return old == 0ull;
         }
         i = (i + 1u) & mask;
     }
 }
 
-__device__ void hash_dec(uint64_t *tab, uint32_t cap, uint32_t key)
+__device__ void hash_dec(uint64_t *tab, uint32_t cap, uint32_t key, uint64_t w)
 {
     uint32_t i = hslot(key, cap);
     const uint32_t mask = cap - 1u;
     for (uint32_t probe = 0; probe < cap; ++probe) {
-        uint32_t *kp = reinterpret_cast<uint32_t *>(tab + i);
+        uint32_t *kp = reinterpret_cast<uint32_t *>(tab + 2ull * i);
         if (*reinterpret_cast<volatile uint32_t *>(kp) == key) {
             atomicSub(kp + 1, 1u);
+            if (w) atomicSub(reinterpret_cast<unsigned long long *>(tab + 2ull * i + 1), (unsigned long long)w);
             return;
         }
         i = (i + 1u) & mask;
@@ -198,10 +202,26 @@ __device__ __forceinline__ bool is_neighbor(const uint64_t *__restrict__ hash, u
     const uint64_t *tab = hash + (hm >> 6);
     uint32_t i = hslot(key, cap);
     for (;;) {
-        const uint64_t e = tab[i];
+        const uint64_t e = tab[2ull * i];
         const uint32_t k = (uint32_t)e;
         if (k == key) return (e >> 32) != 0;
         if (k == 0u) return false;
+        i = (i + 1u) & mask;
+    }
+}
+
+// weight sum of the live arcs to `key` in the table located by hm (0 if none)
+__device__ __forceinline__ uint64_t neighbor_weight(const uint64_t *__restrict__ hash, uint64_t hm, uint32_t key)
+{
+    const uint32_t lg = (uint32_t)(hm & 63);
+    if (!lg) return 0ull;
+    const uint32_t cap = 1u << (lg - 1u), mask = cap - 1u;
+    const uint4 *tab = reinterpret_cast<const uint4 *>(hash + (hm >> 6));
+    uint32_t i = hslot(key, cap);
+    for (;;) {
+        const uint4 e = tab[i];  // one 16-byte entry
+        if (e.x == key) return ((uint64_t)e.w << 32) | e.z;
+        if (e.x == 0u) return 0ull;
         i = (i + 1u) & mask;
     }
 }
@@ -299,6 +319,35 @@ __device__ __forceinline__ uint64_t frac64(uint64_t tau, uint64_t S)
     return q;
 }
 
+// node2vec's folded return region as a 64-bit threshold: floor(2^64 num /
+// den), num = w_ret fold_r, den = S 2^27 + num, both cut to their top 60 bits
+// by the same shift; 0 when num = 0, all-ones when num >= den after the cut.
+// S < 2^59 and fold_r < 2^59 keep num and den below 2^119.
+__device__ __forceinline__ uint64_t fold_threshold(uint64_t S, uint64_t wret, uint64_t fold_r)
+{
+    typedef unsigned __int128 u128;
+    u128 num = (u128)wret * fold_r;
+    if (!num) return 0ull;
+    u128 den = ((u128)S << 27) + num;
+    const uint64_t hi = (uint64_t)(den >> 64), lo = (uint64_t)den;
+    const int bits = hi ? 128 - __clzll(hi) : 64 - __clzll(lo);
+    if (bits > 60) {
+        num >>= bits - 60;
+        den >>= bits - 60;
+    }
+    if (num >= den) return ~0ull;
+    return frac64((uint64_t)num, (uint64_t)den);
+}
+
+// node2vec's per-attempt fold coin: the 64-bit uniform of stream
+// (walker; step | kMixBit, attempt) below tout takes the return arc
+__device__ __forceinline__ bool fold_coin(uint32_t w0, uint32_t w1, uint32_t step, uint32_t att, uint32_t k0,
+                                          uint32_t k1, uint64_t tout)
+{
+    const uint4 m = philox(w0, w1, step | kMixBit, att, k0, k1);
+    return (((uint64_t)m.x << 32) | m.y) < tout;
+}
+
 // Returns the node's integer mass S (all lanes).  All 64 lanes active.  With
 // float weights, F is the node's new fraction mass: the node's FNode is
 // written and every slot carries the threshold's 17-bit ceiling.
@@ -350,6 +399,7 @@ __device__ uint64_t alias_rebuild(const DevStore &s, int32_t u, uint32_t size, u
         thr_hi = (thr >> 47) + ((thr & ((1ull << 47) - 1)) != 0);
         thr_hi = thr_hi > kThrHiMax ? kThrHiMax : thr_hi;
     }
+    if (s.nmass && ln == 0) s.nmass[u] = S;
     const uint32_t size_a = __shfl(size, ali, 64);
     const uint64_t off_a = __shfl(off, ali, 64);
     if (own) {
@@ -382,6 +432,7 @@ __global__ void k_reset_walk(Ctl *ctl)
         ctl->steps = 0;
         ctl->attempts = 0;
         ctl->probes = 0;
+        ctl->folds = 0;
         ctl->cursor = 0;
     }
 }
@@ -407,6 +458,7 @@ __global__ void k_init_nodes(DevStore s)
             FNode f = {0, 0};
             s.fnode[i >> 5] = f;
         }
+        if (s.nmass) s.nmass[i >> 5] = 0ull;
     }
 }
 
@@ -622,7 +674,7 @@ __global__ __launch_bounds__(256) void k_insert_plan(DevStore s, const DevEdge *
         }
         need[POOL_SLOT] += G.g_slots ? (uint64_t)slot_arrays(s.fw) * G.cap_slots : 0;
         need[POOL_BACK] += G.g_back ? G.cap_back : 0;
-        need[POOL_HASH] += G.g_hash ? G.cap_hash : 0;
+        need[POOL_HASH] += G.g_hash ? 2ull * G.cap_hash : 0;  // 16-byte entries
         need[POOL_MEM] += dpp_sum64(G.g_g ? (uint64_t)G.cap_g : 0ull);
     }
     if (ln == 0)
@@ -669,7 +721,7 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, con
     uint64_t gbase = 0;
     if (G.g_slots) a_slot = bump(s, cur, POOL_SLOT, (uint64_t)slot_arrays(s.fw) * G.cap_slots);
     if (G.g_back) a_back = bump(s, cur, POOL_BACK, G.cap_back);
-    if (G.g_hash) a_hash = bump(s, cur, POOL_HASH, G.cap_hash);
+    if (G.g_hash) a_hash = bump(s, cur, POOL_HASH, 2ull * G.cap_hash);
     if (gtotal) gbase = bump(s, cur, POOL_MEM, gtotal);
     if (a_slot == ~0ull || a_back == ~0ull || a_hash == ~0ull || gbase == ~0ull) return;  // node left as it was
     const uint64_t goff = G.g_g ? gbase + gincl - gneed : gh.off;
@@ -700,14 +752,14 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, con
         // zeros need only be complete before its own CAS: a workgroup-scope
         // fence (a wait on the stores).  A device-scope fence here would write
         // back and invalidate this XCD's L2 once per growing table.
-        for (uint32_t i = ln; i < G.cap_hash; i += 64) htab[i] = 0ull;
+        for (uint32_t i = ln; i < 2u * G.cap_hash; i += 64) htab[i] = 0ull;
         __threadfence_block();
         uint32_t fresh = 0;
         if (h.hash_cap) {
             const uint64_t *old = s.hash + h.hash_off;
             for (uint32_t i = ln; i < h.hash_cap; i += 64) {
-                const uint64_t x = old[i];
-                if ((uint32_t)x && (x >> 32)) fresh += hash_add(htab, G.cap_hash, (uint32_t)x, (uint32_t)(x >> 32));
+                const uint64_t x = old[2ull * i], w = old[2ull * i + 1];
+                if ((uint32_t)x && (x >> 32)) fresh += hash_add(htab, G.cap_hash, (uint32_t)x, (uint32_t)(x >> 32), w);
             }
         }
         hused = dpp_sum32(fresh);
@@ -1198,7 +1250,7 @@ __global__ void k_hash_insert(DevStore s, const DevEdge *__restrict__ E, int64_t
     const DevEdge x = E[i];
     if (x.u < 1 || x.u > s.n) return;
     const NodeHdr h = s.hdr[x.u];
-    if (h.hash_cap && hash_add(s.hash + h.hash_off, h.hash_cap, (uint32_t)x.v, 1u))
+    if (h.hash_cap && hash_add(s.hash + h.hash_off, h.hash_cap, (uint32_t)x.v, 1u, x.w))
         atomicAdd(&s.hdr[x.u].hash_used, 1u);
 }
 
@@ -1210,7 +1262,9 @@ __global__ void k_hash_delete(DevStore s, const uint32_t *__restrict__ us, const
     const int32_t u = (int32_t)us[i];
     if (u < 1 || u > s.n) return;
     const NodeHdr h = s.hdr[u];
-    if (h.hash_cap) hash_dec(s.hash + h.hash_off, h.hash_cap, s.slot[h.slot_off + slots[i]]);
+    // before the apply kernel tombstones the slot: its weight is still there
+    if (h.hash_cap)
+        hash_dec(s.hash + h.hash_off, h.hash_cap, s.slot[h.slot_off + slots[i]], s.slot[h.slot_off + h.slot_cap + slots[i]]);
 }
 
 // one wave per long delete run: tombstones, fractions and alias table
@@ -1400,7 +1454,7 @@ template <int APP, bool FW>
 __global__ __launch_bounds__(256) void k_walk(DevStore s, WalkArgs a)
 {
     const int64_t wid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    uint32_t steps = 0, draws = 0, probes = 0;
+    uint32_t steps = 0, draws = 0, probes = 0, folds = 0;
     if (wid < a.nwalkers) {
         const int32_t start = a.starts ? a.starts[wid] : (int32_t)(wid % s.n) + 1;
         int32_t L = a.lens ? a.lens[wid] : a.max_len;
@@ -1421,10 +1475,26 @@ __global__ __launch_bounds__(256) void k_walk(DevStore s, WalkArgs a)
                 uint4 r;
                 int32_t nx;
                 if (APP == 1 && j >= 2) {
+                    // folded return region (integer weights only): cur's mass
+                    // is read with the first draw, the return arc's weight sum
+                    // from cur's neighbour table once the slot gave its locator
+                    const bool fold = !FW && a.fold_r;
+                    const uint64_t Sc = fold ? s.nmass[cur] : 0ull;
+                    uint64_t tout = 0;
                     for (uint32_t at = 0;; ++at) {
                         r = philox(w0, w1, (uint32_t)j, at, k0, k1);
                         nx = draw<FW>(s, cur, r, hcur, w0, w1, (uint32_t)j, at, k0, k1);
                         ++draws;
+                        if (fold) {
+                            if (at == 0) {
+                                ++folds;
+                                tout = fold_threshold(Sc, neighbor_weight(s.hash, hcur, (uint32_t)prev), a.fold_r);
+                            }
+                            if (tout && fold_coin(w0, w1, (uint32_t)j, at, k0, k1, tout)) {
+                                nx = prev;
+                                break;
+                            }
+                        }
                         if (nx < 0 || (int64_t)at + 1 >= a.max_attempts) break;
                         const uint32_t aux = r.x & kAuxMask;
                         bool acc;
@@ -1456,7 +1526,8 @@ __global__ __launch_bounds__(256) void k_walk(DevStore s, WalkArgs a)
         }
         for (; j < a.max_len; ++j) row[j] = -1;
     }
-    const uint32_t tot = wave_sum32(steps), tdraw = wave_sum32(draws), tprobe = wave_sum32(probes);
+    const uint32_t tot = wave_sum32(steps), tdraw = wave_sum32(draws), tprobe = wave_sum32(probes),
+                   tfold = wave_sum32(folds);
     if (lane_id() == 0) {
         if (tot) {
             atomicAdd(a.steps, (unsigned long long)tot);
@@ -1464,6 +1535,7 @@ __global__ __launch_bounds__(256) void k_walk(DevStore s, WalkArgs a)
         }
         if (tdraw) atomicAdd(&s.ctl->attempts, (unsigned long long)tdraw);
         if (tprobe) atomicAdd(&s.ctl->probes, (unsigned long long)tprobe);
+        if (tfold) atomicAdd(&s.ctl->folds, (unsigned long long)tfold);
     }
 }
 
@@ -1532,15 +1604,18 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
     int64_t wid[WPL];
     int32_t cur[WPL], prev[WPL], jj[WPL], len[WPL];
     uint32_t att[WPL];
-    uint64_t hp[WPL];  // neighbour-table locator of each slot's previous node
+    uint64_t hp[WPL];    // neighbour-table locator of each slot's previous node
+    uint64_t tout[WPL];  // node2vec: the step's folded return threshold (attempt 0 sets it)
+    const bool fold = !FW && APP == 1 && a.fold_r;
 #pragma unroll
     for (int k = 0; k < WPL; ++k) {
         wid[k] = -1;
         cur[k] = prev[k] = jj[k] = len[k] = 0;
         att[k] = 0;
         hp[k] = 0;
+        tout[k] = 0;
     }
-    uint32_t steps = 0, draws = 0, probes = 0;
+    uint32_t steps = 0, draws = 0, probes = 0, folds = 0;
     bool more = true;
     for (;;) {
         if (more) {
@@ -1601,16 +1676,29 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
             const int64_t w = wid[k];
             const uint32_t w0 = (uint32_t)w, w1 = (uint32_t)((uint64_t)w >> 32);
             const uint4 r = philox(w0, w1, (uint32_t)jj[k], att[k], k0, k1);
+            const bool f0 = fold && jj[k] >= 2 && att[k] == 0;
+            const uint64_t Sc = f0 ? s.nmass[cur[k]] : 0ull;
             uint64_t hcur;
-            const int32_t nx = draw<FW>(s, cur[k], r, hcur, w0, w1, (uint32_t)jj[k], att[k], k0, k1);
+            int32_t nx = draw<FW>(s, cur[k], r, hcur, w0, w1, (uint32_t)jj[k], att[k], k0, k1);
             ++draws;
+            bool taken = false;  // the fold coin took the return arc
+            if (fold && jj[k] >= 2) {
+                if (f0) {
+                    ++folds;
+                    tout[k] = fold_threshold(Sc, neighbor_weight(s.hash, hcur, (uint32_t)prev[k]), a.fold_r);
+                }
+                if (tout[k] && fold_coin(w0, w1, (uint32_t)jj[k], att[k], k0, k1, tout[k])) {
+                    nx = prev[k];
+                    taken = true;
+                }
+            }
             if (nx < 0) {
                 out.finish(a, k, w, jj[k]);
                 wid[k] = -1;
                 continue;
             }
             const uint32_t aux = r.x & kAuxMask;
-            if (APP == 1 && jj[k] >= 2 && (int64_t)att[k] + 1 < a.max_attempts) {
+            if (APP == 1 && !taken && jj[k] >= 2 && (int64_t)att[k] + 1 < a.max_attempts) {
                 bool acc;
                 if (nx == prev[k]) acc = aux < a.thr_return;
                 else if (aux < tlo) acc = true;
@@ -1639,7 +1727,8 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
             }
         }
     }
-    const uint32_t tot = wave_sum32(steps), tdraw = wave_sum32(draws), tprobe = wave_sum32(probes);
+    const uint32_t tot = wave_sum32(steps), tdraw = wave_sum32(draws), tprobe = wave_sum32(probes),
+                   tfold = wave_sum32(folds);
     if (ln == 0) {
         if (tot) {
             atomicAdd(a.steps, (unsigned long long)tot);
@@ -1647,6 +1736,7 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
         }
         if (tdraw) atomicAdd(&s.ctl->attempts, (unsigned long long)tdraw);
         if (tprobe) atomicAdd(&s.ctl->probes, (unsigned long long)tprobe);
+        if (tfold) atomicAdd(&s.ctl->folds, (unsigned long long)tfold);
     }
 }
 
@@ -1675,14 +1765,14 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
     int64_t wid[kServeWPL];
     uint32_t att[kServeWPL], acc[kServeWPL];
     int32_t acc_req[kServeWPL];
-    uint64_t hp[kServeWPL];
+    uint64_t hp[kServeWPL], tout[kServeWPL];
 #pragma unroll
     for (int k = 0; k < kServeWPL; ++k) {
         rid[k] = acc_req[k] = -1;
         cur[k] = prev[k] = jj[k] = len[k] = 0;
         wid[k] = 0;
         att[k] = acc[k] = 0;
-        hp[k] = 0;
+        hp[k] = tout[k] = 0;
     }
     bool more = true;
     for (;;) {
@@ -1760,12 +1850,23 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
             const uint32_t w0 = (uint32_t)w, w1 = (uint32_t)((uint64_t)w >> 32);
             const uint32_t k0 = (uint32_t)q.seed, k1 = (uint32_t)(q.seed >> 32);
             const uint4 r = philox(w0, w1, (uint32_t)jj[k], att[k], k0, k1);
+            const bool fold = !FW && q.app == 1 && q.fold_r && jj[k] >= 2;
+            const uint64_t Sc = fold && att[k] == 0 ? s.nmass[cur[k]] : 0ull;
             uint64_t hcur;
-            const int32_t nx = draw<FW>(s, cur[k], r, hcur, w0, w1, (uint32_t)jj[k], att[k], k0, k1);
+            int32_t nx = draw<FW>(s, cur[k], r, hcur, w0, w1, (uint32_t)jj[k], att[k], k0, k1);
+            bool taken = false;  // the fold coin took the return arc
+            if (fold) {
+                if (att[k] == 0)
+                    tout[k] = fold_threshold(Sc, neighbor_weight(s.hash, hcur, (uint32_t)prev[k]), q.fold_r);
+                if (tout[k] && fold_coin(w0, w1, (uint32_t)jj[k], att[k], k0, k1, tout[k])) {
+                    nx = prev[k];
+                    taken = true;
+                }
+            }
             bool done = nx < 0;
             if (!done) {
                 const uint32_t aux = r.x & kAuxMask;
-                if (q.app == 1 && jj[k] >= 2 && (int64_t)att[k] + 1 < q.max_attempts) {
+                if (q.app == 1 && !taken && jj[k] >= 2 && (int64_t)att[k] + 1 < q.max_attempts) {
                     const uint32_t tlo = min(q.thr_neighbor, q.thr_other), thi = max(q.thr_neighbor, q.thr_other);
                     bool ok;
                     if (nx == prev[k]) ok = aux < q.thr_return;
@@ -1872,7 +1973,7 @@ __global__ void k_edge_count(DevStore s, const int32_t *us, const int32_t *vs, i
                 const uint64_t *tab = s.hash + h.hash_off;
                 uint32_t p = hslot((uint32_t)v, h.hash_cap);
                 for (uint32_t probe = 0; probe < h.hash_cap; ++probe) {
-                    const uint64_t e = tab[p];
+                    const uint64_t e = tab[2ull * p];
                     if ((uint32_t)e == (uint32_t)v) {
                         c = (uint32_t)(e >> 32);
                         break;

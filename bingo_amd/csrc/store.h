@@ -11,7 +11,9 @@
 //              weights also [fraction x cap] (w holds the integer part)
 //   back  u32: per slot, the member positions of its set bits, ascending
 //   mem   u64: member entries (v << 32) | slot
-//   hash  u64: neighbour multiset entries key | count << 32 (linear probing)
+//   hash  u64: neighbour multiset, 16-byte entries {key | count << 32, weight
+//              sum of those arcs} (linear probing); a table of cap entries
+//              takes 2 cap words
 //
 // Reference roles: NodeData (include/Bingo.cuh:6-58) held edge[SZ],
 // group[BITS*SZ], idx[BITS*SZ], num[], aliasTable[] per node from a bump pool
@@ -54,6 +56,7 @@ struct FNode {
     uint64_t thr;
 };
 constexpr uint32_t kFracTag = 0x40000000u;     // Philox counter word 3 of the fractional stream
+constexpr uint32_t kMixBit = 0x80000000u;      // Philox counter word 2 (step) of node2vec's fold stream
 constexpr uint32_t kFracIters = 4096u;         // fractional rejection rounds per draw
 constexpr int32_t kMaxAttemptsFloat = 65536;   // node2vec cap with float weights (attempt in 16 bits)
 constexpr uint32_t kThrHiMax = (1u << 17) - 1;  // the alias slot's 17-bit copy of thr saturates here
@@ -93,7 +96,8 @@ struct Ctl {
     unsigned int nbig;               // delete batch: runs split into per-group tasks
     unsigned long long steps;        // last walk: appended vertices
     unsigned long long attempts;     // last walk: sample draws (node2vec rejections included)
-    unsigned long long probes;       // last walk: neighbour-table lookups
+    unsigned long long probes;       // last walk: neighbour-table membership lookups
+    unsigned long long folds;        // last walk: node2vec return-region lookups (mass + entry)
     unsigned long long cursor;       // persistent walk: next unclaimed walker
 };
 
@@ -114,6 +118,7 @@ struct DevStore {
     int32_t fw;      // weights are float bits: slot regions carry a fraction array
     int32_t pad;
     FNode *fnode;    // [n+1] when fw
+    uint64_t *nmass; // [n+1] when nbr: each node's integer weight S (node2vec's folded return region)
     NodeHdr *hdr;
     GroupHdr *grp;
     AliasEnt *alias;
@@ -136,6 +141,7 @@ struct WalkArgs {
     uint32_t thr_return, thr_neighbor, thr_other, thr_stop;
     int32_t max_len;
     int32_t max_attempts;
+    uint64_t fold_r;  // node2vec: folded return region factor x 2^27 (0: plain envelope)
 };
 
 // One admitted walk request of the serving loop.  Its walkers occupy the
@@ -157,6 +163,7 @@ struct ServeReq {
     uint32_t thr_return, thr_neighbor, thr_other, thr_stop;
     int32_t max_len, max_attempts;
     int32_t app, pad;
+    uint64_t fold_r;
 };
 constexpr int kMaxServeReqs = 128;
 
@@ -185,8 +192,9 @@ void launch_insert_plan(DevStore s, const DevEdge *sorted, Runs r, unsigned long
                         hipStream_t st);
 void launch_insert_apply(DevStore s, const DevEdge *sorted, const uint32_t *frac, Runs r,
                          const unsigned long long *wres, int64_t k, hipStream_t st);
-// neighbour multisets (nbr stores): the batch's arcs / deletes, one thread
-// each, after the matching apply kernel
+// neighbour multisets (nbr stores): the batch's arcs, one thread each, after
+// the insert apply kernel; the batch's deletes before the delete apply kernel
+// (their slot weights are read before the tombstones)
 void launch_hash_insert(DevStore s, const DevEdge *sorted, int64_t k, hipStream_t st);
 void launch_hash_delete(DevStore s, const uint32_t *us, const uint32_t *slots, int64_t k, hipStream_t st);
 void launch_delete_keys(const int32_t *dels, int64_t k, unsigned long long *keys, hipStream_t st);

@@ -89,7 +89,9 @@ typedef struct {
     double walk_kernel_ms;  /* device time of the last walk kernel */
     uint64_t walk_steps;    /* appended vertices of the last walk */
     uint64_t walk_attempts; /* sample draws of the last walk (node2vec rejections included) */
-    uint64_t walk_probes;   /* neighbour-table lookups of the last walk (node2vec) */
+    uint64_t walk_probes;   /* neighbour-table membership lookups of the last walk (node2vec) */
+    uint64_t walk_folds;    /* node2vec steps that read the node's mass and the return arc's
+                               weight sum (folded return region) */
     uint64_t arcs;          /* live arcs */
     uint64_t bytes_reserved;/* HBM held by the store */
     uint64_t pool_used[4];  /* slot, back-index, member, neighbour pools (elements) */

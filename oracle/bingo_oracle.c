@@ -30,7 +30,7 @@
 
 typedef unsigned __int128 u128;
 
-typedef struct { uint32_t key, count; } hent;
+typedef struct { uint32_t key, count; uint64_t wsum; } hent;  /* target, arcs, their weight */
 
 typedef struct {
     uint32_t nslots, slot_cap;
@@ -85,7 +85,7 @@ static inline uint32_t hslot(uint32_t key, uint32_t cap)
     return (uint32_t)(((uint64_t)(key * 0x9E3779B1u) * cap) >> 32);
 }
 
-static void hash_put(node_t *x, uint32_t key, int32_t delta);
+static void hash_put(node_t *x, uint32_t key, int32_t delta, int64_t wdelta);
 
 static void hash_rehash(node_t *x, uint32_t newcap)
 {
@@ -96,21 +96,27 @@ static void hash_rehash(node_t *x, uint32_t newcap)
     x->hcap = newcap;
     x->hused = 0;
     for (uint32_t i = 0; i < oldcap; ++i)
-        if (old[i].key && old[i].count) hash_put(x, old[i].key, (int32_t)old[i].count);
+        if (old[i].key && old[i].count) hash_put(x, old[i].key, (int32_t)old[i].count, (int64_t)old[i].wsum);
     free(old);
 }
 
-static void hash_put(node_t *x, uint32_t key, int32_t delta)
+/* key's arc count += delta and weight sum += wdelta (integer parts) */
+static void hash_put(node_t *x, uint32_t key, int32_t delta, int64_t wdelta)
 {
     if (delta > 0 && (uint64_t)(x->hused + 1) * 2 > x->hcap)
         hash_rehash(x, x->hcap ? x->hcap * 2 : 16);
     uint32_t mask = x->hcap - 1, i = hslot(key, x->hcap);
     for (;;) {
-        if (x->hash[i].key == key) { x->hash[i].count += (uint32_t)delta; return; }
+        if (x->hash[i].key == key) {
+            x->hash[i].count += (uint32_t)delta;
+            x->hash[i].wsum += (uint64_t)wdelta;
+            return;
+        }
         if (x->hash[i].key == 0) {
             if (delta < 0) abort(); /* removing an absent key: store corrupted */
             x->hash[i].key = key;
             x->hash[i].count = (uint32_t)delta;
+            x->hash[i].wsum = (uint64_t)wdelta;
             x->hused++;
             return;
         }
@@ -118,15 +124,25 @@ static void hash_put(node_t *x, uint32_t key, int32_t delta)
     }
 }
 
-static uint32_t hash_get(const node_t *x, uint32_t key)
+static const hent *hash_find(const node_t *x, uint32_t key)
 {
-    if (!x->hcap) return 0;
+    if (!x->hcap) return NULL;
     uint32_t mask = x->hcap - 1, i = hslot(key, x->hcap);
     for (;;) {
-        if (x->hash[i].key == key) return x->hash[i].count;
-        if (x->hash[i].key == 0) return 0;
+        if (x->hash[i].key == key) return &x->hash[i];
+        if (x->hash[i].key == 0) return NULL;
         i = (i + 1) & mask;
     }
+}
+static uint32_t hash_get(const node_t *x, uint32_t key)
+{
+    const hent *h = hash_find(x, key);
+    return h ? h->count : 0;
+}
+static uint64_t hash_wsum(const node_t *x, uint32_t key)
+{
+    const hent *h = hash_find(x, key);
+    return h ? h->wsum : 0;
 }
 
 /* ---------------- alias table ---------------- */
@@ -274,7 +290,7 @@ static void node_append(node_t *x, uint32_t v, uint32_t w, uint32_t f, int fw, i
         x->back[x->back_used++] = pos;
     }
     x->nslots = s + 1;
-    if (nbr) hash_put(x, v, 1);
+    if (nbr) hash_put(x, v, 1, (int64_t)w);
 }
 
 /* Stable grouping of k items by node id: order[] lists item indices grouped
@@ -389,7 +405,7 @@ static void node_delete_run(node_t *x, const uint32_t *slots, int64_t cnt, int n
         x->size[j] = S2;
     }
     for (int64_t t = 0; t < cnt; ++t) {
-        if (nbr) hash_put(x, x->v[slots[t]], -1);
+        if (nbr) hash_put(x, x->v[slots[t]], -1, -(int64_t)x->w[slots[t]]);
         x->w[slots[t]] = 0;
         if (x->fr) {
             x->fmass -= x->fr[slots[t]];
@@ -512,19 +528,56 @@ static uint32_t thr27(double prob)
     return (uint32_t)floor(prob * (double)ORC_AUX_ONE);
 }
 
-/* node2vec bias alpha = 1/p (return), 1 (distance 1), 1/q (distance 2),
- * accepted with alpha / max(1/p, 1, 1/q) (operation.cuh:291,302-325 uses
- * lim = max(1/p, 1/q)); PPR stops with probability tp after each step
- * (operation.cuh:359). */
+/* node2vec bias alpha = 1/p (return), 1 (distance 1), 1/q (distance 2)
+ * (operation.cuh:291,302-325; lim there is max(1/p, 1/q)).  A proposal
+ * x ~ w is accepted with alpha(x) / env, env = max(1, 1/q).  When 1/p > env
+ * the return arc's excess is folded out of the envelope (outlier folding,
+ * Yang et al., KnightKing, SOSP'19): the proposal accepts a return
+ * candidate always, and each attempt first takes the return arc outright
+ * with the probability of an extra region of mass w_ret (1/p - env) / env
+ * next to the proposal mass S (fold_r = that factor in units of 2^-27).
+ * Without folding (fold = 0, 1/p <= env, or a factor (1/p - env) / env of
+ * 2^32 or more) the envelope is max(1/p, 1, 1/q).
+ * PPR stops with probability tp after each step (operation.cuh:359). */
+void orc_walk_thresholds_ex(double p, double q, double tp, int fold, orc_walk_params *out)
+{
+    double ip = 1.0 / p, iq = 1.0 / q, env = iq > 1.0 ? iq : 1.0;
+    if (fold && ip > env && (ip - env) / env < 4294967296.0) {
+        out->thr_return = ORC_AUX_ONE;
+        out->thr_neighbor = thr27(1.0 / env);
+        out->thr_other = thr27(iq / env);
+        out->fold_r = (uint64_t)floor((ip - env) / env * (double)ORC_AUX_ONE);
+    } else {
+        double lim = ip > env ? ip : env;
+        out->thr_return = thr27(ip / lim);
+        out->thr_neighbor = thr27(1.0 / lim);
+        out->thr_other = thr27(iq / lim);
+        out->fold_r = 0;
+    }
+    out->thr_stop = thr27(tp);
+}
+
 void orc_walk_thresholds(double p, double q, double tp, orc_walk_params *out)
 {
-    double ip = 1.0 / p, iq = 1.0 / q, lim = ip;
-    if (lim < 1.0) lim = 1.0;
-    if (lim < iq) lim = iq;
-    out->thr_return = thr27(ip / lim);
-    out->thr_neighbor = thr27(1.0 / lim);
-    out->thr_other = thr27(iq / lim);
-    out->thr_stop = thr27(tp);
+    orc_walk_thresholds_ex(p, q, tp, 1, out);
+}
+
+/* Probability of the folded return region as a 64-bit threshold:
+ * floor(2^64 num / den), num = w_ret fold_r, den = S 2^27 + num, both cut to
+ * their top 60 bits by the same shift (relative error < 2^-59).  0 when
+ * num = 0; all-ones when num >= den after the cut. */
+uint64_t orc_fold_threshold(uint64_t S, uint64_t wret, uint64_t fold_r)
+{
+    u128 num = (u128)wret * fold_r, den = ((u128)S << 27) + num;
+    if (!num) return 0;
+    int bits = 0;
+    for (u128 t = den; t; t >>= 1) ++bits;
+    if (bits > 60) {
+        num >>= bits - 60;
+        den >>= bits - 60;
+    }
+    if (num >= den) return UINT64_MAX;
+    return (uint64_t)((num << 64) / den);
 }
 
 static uint64_t walk_one(const orc_graph *g, int app, const orc_walk_params *p, uint64_t wid,
@@ -540,10 +593,22 @@ static uint64_t walk_one(const orc_graph *g, int app, const orc_walk_params *p, 
         int32_t nx;
         if (app == ORC_NODE2VEC && j >= 2) {
             const node_t *px = &g->nodes[prev];
+            /* folded return region: per attempt a 64-bit coin from stream
+             * (wid; step | 2^31, attempt) below tout takes the return arc */
+            const uint64_t tout =
+                p->fold_r ? orc_fold_threshold(x->mass, g->nbr ? hash_wsum(x, (uint32_t)prev) : 0, p->fold_r) : 0;
             for (uint32_t a = 0;; ++a) {
                 ctr[3] = a;
                 orc_philox4x32_10(ctr, key, r);
                 nx = node_draw(g, x, r, key, wid, (uint32_t)j, a);
+                if (tout) {
+                    uint32_t m[4], mc[4] = {(uint32_t)wid, (uint32_t)(wid >> 32), (uint32_t)j | ORC_MIX_BIT, a};
+                    orc_philox4x32_10(mc, key, m);
+                    if ((((uint64_t)m[0] << 32) | m[1]) < tout) {
+                        nx = prev;
+                        break;
+                    }
+                }
                 if (nx < 0) break;
                 if ((int64_t)a + 1 >= p->max_attempts) break;
                 uint32_t aux = r[0] & (ORC_AUX_ONE - 1u), thr;
@@ -575,6 +640,7 @@ int orc_walk(const orc_graph *g, int app, const orc_walk_params *p, const int32_
         return ORC_EINVAL;
     if (app == ORC_NODE2VEC && p->max_attempts < 1) return ORC_EINVAL;
     if (g->fw && p->max_attempts > ORC_MAX_ATTEMPTS_FLOAT) return ORC_EINVAL;
+    if (app == ORC_NODE2VEC && p->fold_r && (g->fw || !g->nbr)) return ORC_EINVAL;  /* folding needs weight sums */
     for (int64_t i = 0; starts && i < nwalkers; ++i)
         if (starts[i] < 1 || starts[i] > g->n) return ORC_EINVAL;
     for (int64_t i = 0; lens && i < nwalkers; ++i)

@@ -27,6 +27,8 @@
  *   - randomness: Philox4x32-10 keyed (seed, walker, step, attempt);
  *   - node2vec neighbour test is exact "candidate in N(prev)" (ref hash is
  *     lossy and tests prev in N(cur), operation.cuh:313, Bingo.cuh:39-58);
+ *   - node2vec folds the return arc's excess out of the rejection envelope
+ *     (integer graphs; see orc_walk_thresholds_ex), same law, fewer draws;
  *   - PPR has a per-walker length cap (ref overwrites slot LEN-1);
  *   - batched delete semantics are order-free (see orc_delete);
  *   - float weights (ref wtype::Float, Bingo.cuh:65-69,176-177,193-194,
@@ -52,6 +54,7 @@ extern "C" {
 #define ORC_MAX_ATTEMPTS_FLOAT 65536 /* node2vec cap with float weights (attempt fits 16 bits) */
 #define ORC_MAX_SLOTS (1u << 28)      /* slots per node (the store's 28-bit group sizes) */
 #define ORC_MAX_MASS (1ull << 59)     /* integer weight per node (alias prefix sums stay in 64 bits) */
+#define ORC_MIX_BIT 0x80000000u       /* Philox counter word 2 (step) of the node2vec fold stream */
 
 enum { ORC_DEEPWALK = 0, ORC_NODE2VEC = 1, ORC_PPR = 2, ORC_SAMPLING = 3 };
 enum { ORC_W_CONST1 = 0, ORC_W_UNIFORM16 = 1, ORC_W_LOGUNI32 = 2, ORC_W_ZIPF24 = 3 };
@@ -79,6 +82,7 @@ typedef struct {
     uint32_t thr_stop;     /* PPR: stop if aux < thr */
     int32_t max_len;       /* columns of the path matrix */
     int32_t max_attempts;  /* node2vec rejection cap per step */
+    uint64_t fold_r;       /* node2vec: folded return region factor x 2^27 (0: no folding) */
 } orc_walk_params;
 
 /* graph lifetime; float_weights != 0: edge weights are float bits */
@@ -99,8 +103,12 @@ int orc_walk(const orc_graph *g, int app, const orc_walk_params *p, const int32_
              const int32_t *lens, int64_t nwalkers, int32_t *paths, uint64_t *steps_out,
              int nthreads);
 
-/* quantise float walk parameters exactly as the product does */
+/* quantise float walk parameters exactly as the product does; fold = 0
+ * keeps the plain envelope max(1/p, 1, 1/q) (float-weight graphs) */
 void orc_walk_thresholds(double p, double q, double tp, orc_walk_params *out);
+void orc_walk_thresholds_ex(double p, double q, double tp, int fold, orc_walk_params *out);
+/* folded return region threshold (tests) */
+uint64_t orc_fold_threshold(uint64_t S, uint64_t wret, uint64_t fold_r);
 
 /* state export for bit-exact diffs (flat, node-major) */
 typedef struct {

@@ -31,6 +31,7 @@ class WalkParams(C.Structure):
         ("thr_stop", C.c_uint32),
         ("max_len", C.c_int32),
         ("max_attempts", C.c_int32),
+        ("fold_r", C.c_uint64),
     ]
 
 
@@ -80,6 +81,9 @@ def lib():
         L.orc_delete.argtypes = [vp, vp, i64]
         L.orc_walk.argtypes = [vp, C.c_int, C.POINTER(WalkParams), vp, vp, i64, vp, C.POINTER(u64), C.c_int]
         L.orc_walk_thresholds.argtypes = [C.c_double, C.c_double, C.c_double, C.POINTER(WalkParams)]
+        L.orc_walk_thresholds_ex.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, C.POINTER(WalkParams)]
+        L.orc_fold_threshold.restype = C.c_uint64
+        L.orc_fold_threshold.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
         L.orc_export.argtypes = [vp, C.POINTER(Dump)]
         L.orc_edge_count.restype = C.c_uint32
         L.orc_edge_count.argtypes = [vp, i32, i32]
@@ -106,13 +110,19 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
 
 
-def walk_params(seed=1, p=0.5, q=2.0, tp=0.0125, max_len=80, max_attempts=4096) -> WalkParams:
+def walk_params(seed=1, p=0.5, q=2.0, tp=0.0125, max_len=80, max_attempts=4096, float_weights=False) -> WalkParams:
+    """float_weights: parameters for a float-weight graph (node2vec keeps the
+    plain envelope there; folding needs integer weight sums)"""
     wp = WalkParams()
-    lib().orc_walk_thresholds(p, q, tp, C.byref(wp))
+    lib().orc_walk_thresholds_ex(p, q, tp, 0 if float_weights else 1, C.byref(wp))
     wp.seed = seed
     wp.max_len = max_len
     wp.max_attempts = max_attempts
     return wp
+
+
+def fold_threshold(S, wret, fold_r):
+    return int(lib().orc_fold_threshold(S, wret, fold_r))
 
 
 def philox(ctr, key):

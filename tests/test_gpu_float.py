@@ -35,7 +35,7 @@ def _same_state(g, o, what=""):
 
 def _same_walks(g, o, orc, app, seed, max_len, starts=None, nwalkers=None, form=B.FORM_AUTO, **kw):
     gp = B.walk_params(seed=seed, max_len=max_len, form=form, **kw)
-    op = orc.walk_params(seed=seed, max_len=max_len, **kw)
+    op = orc.walk_params(seed=seed, max_len=max_len, float_weights=True, **kw)
     pa, sa = g.walk(app, gp, starts=starts, nwalkers=nwalkers)
     pb, sb = o.walk(app, op, starts=starts, nwalkers=nwalkers)
     if not np.array_equal(pa, pb):
@@ -174,7 +174,7 @@ def test_float_walk_forms_and_server_agree(orc):
     dp = B.DeviceArray(g, nbytes=nw * L * 4)
     ds = B.DeviceArray(g, np.zeros(1, np.uint64))
     wg = g.walk_graph(B.PPR, B.walk_params(seed=4, max_len=L), nw, dp, ds)
-    want, steps = o.walk(B.PPR, orc.walk_params(seed=4, max_len=L), nwalkers=nw)
+    want, steps = o.walk(B.PPR, orc.walk_params(seed=4, max_len=L, float_weights=True), nwalkers=nw)
     for rep in range(3):
         wg.launch()
         g.sync()
@@ -187,6 +187,6 @@ def test_float_walk_forms_and_server_agree(orc):
     tickets = [srv.submit_walk(app, B.walk_params(seed=s, max_len=20), nwalkers=s) for app, s in reqs]
     for (app, s), t in zip(reqs, tickets):
         out, st, _ = srv.wait(t)
-        pb, sb = o.walk(app, orc.walk_params(seed=s, max_len=20), nwalkers=s)
+        pb, sb = o.walk(app, orc.walk_params(seed=s, max_len=20, float_weights=True), nwalkers=s)
         assert np.array_equal(out, pb) and st == sb, (app, s)
     srv.close()

@@ -105,21 +105,26 @@ def test_walks_are_keyed_by_walker(orc):
     assert (a[:10] == c).all()
 
 
-def test_node2vec_second_order_law(orc):
-    # undirected little graph with a triangle (1,2,3) and pendant 4,5 on 2
+@pytest.mark.parametrize("P,Q,fold", [(0.5, 2.0, True), (0.25, 1.0, True), (0.1, 4.0, True), (2.0, 0.5, True),
+                                       (1.0, 1.0, True), (0.5, 2.0, False)])
+def test_node2vec_second_order_law(orc, P, Q, fold):
+    """The second-order law w * alpha on a graph with a triangle (1,2,3),
+    pendants 4,5 on 2 and a doubled return arc 2->1 (weight sums), with the
+    return arc folded out of the envelope (1/p > max(1, 1/q)) or not."""
     und = [(1, 2, 3), (2, 3, 5), (1, 3, 2), (2, 4, 7), (2, 5, 1), (4, 5, 2)]
     e = []
     for u, v, w in und:
         e += [(u, v, w), (v, u, w)]
+    e.append((2, 1, 4))  # a second arc 2 -> 1: the return mass is 3 + 4
     g = orc.OracleGraph(5)
     g.load(np.array(e, dtype=orc.EDGE_DT))
-    P, Q = 0.5, 2.0
-    p = orc.walk_params(seed=9, p=P, q=Q, max_len=3)
+    p = orc.walk_params(seed=9, p=P, q=Q, max_len=3, max_attempts=1 << 20, float_weights=not fold)
+    assert (p.fold_r > 0) == (fold and 1 / P > max(1.0, 1 / Q))
     N = 400_000
     paths, _ = g.walk(orc.NODE2VEC, p, starts=np.ones(N, np.int32))
     sel = paths[:, 1] == 2  # condition on 1 -> 2
     third = paths[sel, 2]
-    nbr2 = {1: 3, 3: 5, 4: 7, 5: 1}
+    nbr2 = {1: 3 + 4, 3: 5, 4: 7, 5: 1}
     nbr1 = {2, 3}
     alpha = {x: (1 / P if x == 1 else (1.0 if x in nbr1 else 1 / Q)) for x in nbr2}
     xs = sorted(nbr2)
@@ -128,6 +133,45 @@ def test_node2vec_second_order_law(orc):
     cnt = [(third == x).sum() for x in xs]
     assert sum(cnt) == sel.sum()
     assert _chi2_p(cnt, probs) > 0.01
+
+
+def test_fold_threshold_matches_ratio(orc):
+    """The folded region's threshold against the exact ratio w_ret R /
+    (S 2^27 + w_ret R), including the cut to 60 bits."""
+    from fractions import Fraction
+    rng = np.random.default_rng(6)
+    cases = [(1, 1, 1 << 27), (7, 3, 1 << 27), ((1 << 59) - 1, (1 << 59) - 1, 99 << 27), (1 << 40, 1, 1),
+             (5, 0, 1 << 27), (1, 1, (1 << 63) - 1)]
+    cases += [(int(rng.integers(1, 1 << 59)), 0, int(rng.integers(1, 1 << 40))) for _ in range(50)]
+    cases += [(S, int(rng.integers(0, S + 1)), int(rng.integers(1, 1 << 40)))
+              for S in (int(x) for x in rng.integers(1, 1 << 59, 2000))]
+    cases += [(S, int(rng.integers(0, S + 1)), int(rng.integers(1, 1 << 30)))
+              for S in (int(x) for x in rng.integers(1, 1 << 20, 1000))]
+    for S, w, R in cases:
+        t = orc.fold_threshold(S, w, R)
+        if w == 0:
+            assert t == 0
+            continue
+        exact = Fraction(w * R, (S << 27) + w * R)
+        assert abs(exact - Fraction(t, 1 << 64)) < Fraction(1, 1 << 58) * exact + Fraction(1, 1 << 63), (S, w, R)
+
+
+def test_walk_thresholds_fold_rules(orc):
+    """Folding applies when 1/p exceeds env = max(1, 1/q) by a factor below
+    2^32 on integer weights; otherwise the plain envelope max(1/p, 1, 1/q)."""
+    one = 1 << 27
+    p = orc.walk_params(p=0.5, q=2.0)
+    assert (p.thr_return, p.thr_neighbor, p.thr_other, p.fold_r) == (one, one, one // 2, one)
+    p = orc.walk_params(p=0.25, q=0.5)  # env 2, 1/p = 4: factor 1
+    assert (p.thr_return, p.thr_neighbor, p.thr_other, p.fold_r) == (one, one // 2, one, one)
+    p = orc.walk_params(p=2.0, q=0.5)  # 1/p below env: plain, lim 2
+    assert (p.thr_return, p.thr_neighbor, p.thr_other, p.fold_r) == (one // 4, one // 2, one, 0)
+    p = orc.walk_params(p=0.5, q=2.0, float_weights=True)  # plain, lim 2
+    assert (p.thr_return, p.thr_neighbor, p.thr_other, p.fold_r) == (one, one // 2, one // 4, 0)
+    p = orc.walk_params(p=1e-12, q=1.0)  # factor ~1e12 >= 2^32: plain
+    assert p.fold_r == 0 and p.thr_return == one and p.thr_neighbor == 0
+    p = orc.walk_params(p=1.0 / (1 + 2.0 ** 31), q=1.0)  # factor 2^31: folded
+    assert p.fold_r == 1 << 58
 
 
 def test_ppr_length_law(orc):
