@@ -1551,6 +1551,15 @@ __global__ __launch_bounds__(256) void k_walk(DevStore s, WalkArgs a)
 // vectors.  Walker i's path depends only on (seed, i, start, len), so any
 // schedule gives the same paths as k_walk and the oracle.
 // ------------------------------------------------------------------------
+// Path rows are written once and not read back by the walk: 16-byte
+// non-temporal stores keep them from displacing the store's lines in L2.
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void store_row4(int32_t *p, int32_t a, int32_t b, int32_t c, int32_t d)
+{
+    const i32x4 v = {a, b, c, d};
+    __builtin_nontemporal_store(v, reinterpret_cast<i32x4 *>(p));
+}
+
 template <int WPL, int D>
 struct PathOut {
     int32_t *stage;  // this lane's WPL rows of D + 1 ints (LDS)
@@ -1567,12 +1576,26 @@ struct PathOut {
     __device__ __forceinline__ void flush(const WalkArgs &a, int k, int64_t wid, int32_t j0, int cnt) const
     {
         const int32_t *st = stage + k * (D + 1);
-        int4 *dst = reinterpret_cast<int4 *>(a.paths + wid * (int64_t)a.max_len + j0);
+        int32_t *dst = a.paths + wid * (int64_t)a.max_len + j0;
 #pragma unroll
         for (int c = 0; c < D / 4; ++c)
-            if (4 * c < cnt) dst[c] = make_int4(st[4 * c], st[4 * c + 1], st[4 * c + 2], st[4 * c + 3]);
+            if (4 * c < cnt) store_row4(dst + 4 * c, st[4 * c], st[4 * c + 1], st[4 * c + 2], st[4 * c + 3]);
     }
-    // pad positions j..max_len-1 with -1
+    // write out the partial stage (its tail -1) and return the first position
+    // not yet written (a multiple of D when D > 0)
+    __device__ __forceinline__ int32_t close(const WalkArgs &a, int k, int64_t wid, int32_t j) const
+    {
+        if (D == 0) return j;
+        const int32_t r = j % D;
+        if (r) {
+            int32_t *st = stage + k * (D + 1);
+            for (int t = r; t < D; ++t) st[t] = -1;
+            flush(a, k, wid, j - r, D);
+            j += D - r;
+        }
+        return j;
+    }
+    // pad positions j..max_len-1 with -1 (this lane alone)
     __device__ __forceinline__ void finish(const WalkArgs &a, int k, int64_t wid, int32_t j) const
     {
         int32_t *row = a.paths + wid * (int64_t)a.max_len;
@@ -1606,6 +1629,8 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
     uint32_t att[WPL];
     uint64_t hp[WPL];    // neighbour-table locator of each slot's previous node
     uint64_t tout[WPL];  // node2vec: the step's folded return threshold (attempt 0 sets it)
+    int64_t padw[WPL];   // walker whose row tail the wave pads at the end of this iteration (-1: none)
+    int32_t padj[WPL];   // ... from this position on
     const bool fold = !FW && APP == 1 && a.fold_r;
 #pragma unroll
     for (int k = 0; k < WPL; ++k) {
@@ -1614,6 +1639,8 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
         att[k] = 0;
         hp[k] = 0;
         tout[k] = 0;
+        padw[k] = -1;
+        padj[k] = 0;
     }
     uint32_t steps = 0, draws = 0, probes = 0, folds = 0;
     bool more = true;
@@ -1693,7 +1720,8 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
                 }
             }
             if (nx < 0) {
-                out.finish(a, k, w, jj[k]);
+                padj[k] = out.close(a, k, w, jj[k]);
+                padw[k] = w;
                 wid[k] = -1;
                 continue;
             }
@@ -1722,9 +1750,31 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
                 hp[k] = hcur;
             }
             if ((APP == 2 && aux < a.thr_stop) || jj[k] >= len[k]) {
-                out.finish(a, k, w, jj[k]);
+                padj[k] = out.close(a, k, w, jj[k]);
+                padw[k] = w;
                 wid[k] = -1;
             }
+        }
+        // Row tails of the walkers that finished in this iteration, padded
+        // with -1 by the whole wave one row at a time: consecutive lanes
+        // write consecutive 16-byte (D > 0: rows are 16-byte aligned from
+        // position j, a multiple of D) or 4-byte pieces of the row.
+#pragma unroll
+        for (int k = 0; k < WPL; ++k) {
+            for (uint64_t m = __ballot(padw[k] >= 0); m; m &= m - 1) {
+                const int src = __ffsll((unsigned long long)m) - 1;
+                const uint32_t wlo = __shfl((uint32_t)padw[k], src, 64);
+                const uint32_t whi = __shfl((uint32_t)((uint64_t)padw[k] >> 32), src, 64);
+                const int32_t j0 = __shfl(padj[k], src, 64);
+                int32_t *row = a.paths + (int64_t)(((uint64_t)whi << 32) | wlo) * a.max_len;
+                if (D > 0) {
+                    for (int32_t t = j0 + 4 * ln; t < a.max_len; t += 256)
+                        store_row4(row + t, -1, -1, -1, -1);
+                } else {
+                    for (int32_t t = j0 + ln; t < a.max_len; t += 64) row[t] = -1;
+                }
+            }
+            padw[k] = -1;
         }
     }
     const uint32_t tot = wave_sum32(steps), tdraw = wave_sum32(draws), tprobe = wave_sum32(probes),
