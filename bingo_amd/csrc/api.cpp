@@ -45,8 +45,9 @@ static int ensure_scratch(bingo_graph *g, int idx, size_t bytes)
         g->sc[idx] = nullptr;
         g->sc_bytes[idx] = 0;
     }
-    // room to spare for batches of varying size; a large buffer is taken exact
-    const size_t want = bytes < (256ull << 20) ? bytes + bytes / 4 : bytes;
+    // room to spare for batches of varying size (at least 4 MiB, so batches
+    // up to a few 10^5 arcs never reallocate); a large buffer is taken exact
+    const size_t want = bytes < (256ull << 20) ? std::max<size_t>(bytes + bytes / 4, 4ull << 20) : bytes;
     if (hipMalloc(&g->sc[idx], want) != hipSuccess) {
         (void)hipGetLastError();
         g->sc[idx] = nullptr;
@@ -247,6 +248,9 @@ static int do_insert(bingo_graph *g, const DevEdge *d_e, int64_t k)
     if (g->ctl_host->error) return fail(BINGO_EDEVICE, "store reported an internal error");
     if ((rc = ensure_pools(g))) return rc;
     launch_insert_apply(g->s, scp<DevEdge>(g, SC_EDGES), frac, r, wres, k, g->st);
+    // moves of large regions (hubs) by the whole grid, before the neighbour
+    // tables take the batch's arcs
+    if (g->ctl_host->nbigcopy) launch_copy_tasks(g->s, g->st);
     if (g->s.nbr) launch_hash_insert(g->s, scp<DevEdge>(g, SC_EDGES), k, g->st);
     HIPCHK(hipGetLastError());
     g->arcs += k;
@@ -458,6 +462,7 @@ int bingo_create(const bingo_config *cfg, bingo_graph **out)
     g->budget = g->cfg.hbm_budget ? g->cfg.hbm_budget : (uint64_t)(0.9 * (double)fr);
     bool ok = hipHostMalloc((void **)&g->ctl_host, sizeof(Ctl), hipHostMallocDefault) == hipSuccess &&
               hipMalloc((void **)&g->s.ctl, sizeof(Ctl)) == hipSuccess &&
+              hipMalloc((void **)&g->s.tasks, kMaxCopyTasks * sizeof(CopyTask)) == hipSuccess &&
               hipMalloc((void **)&g->d_steps, sizeof(unsigned long long)) == hipSuccess;
     for (int i = 0; ok && i < kEvents; ++i) ok = hipEventCreate(&g->ev[i]) == hipSuccess;
     for (int p = 0; ok && p < NPHASE; ++p)
@@ -479,7 +484,7 @@ void bingo_destroy(bingo_graph *g)
     if (!g) return;
     if (g->st) (void)hipStreamSynchronize(g->st);
     void *bufs[] = {g->s.hdr, g->s.grp, g->s.alias, g->s.fnode, g->s.nmass, g->s.slot, g->s.back, g->s.mem,
-                    g->s.hash, g->s.ctl, g->d_steps};
+                    g->s.hash, g->s.ctl, g->s.tasks, g->d_steps};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (int i = 0; i < NSCRATCH; ++i)

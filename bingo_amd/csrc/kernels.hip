@@ -423,6 +423,8 @@ __global__ void k_reset_ctl(Ctl *ctl)
     if (t == 0) {
         ctl->error = 0;
         ctl->nbig = 0;
+        ctl->nbigcopy = 0;
+        ctl->taskctr = 0;
     }
 }
 
@@ -572,6 +574,15 @@ __device__ Growth plan_run(const DevStore &s, const DevEdge *__restrict__ e, uin
     return G;
 }
 
+// whether the run moves a region of kBigCopyBytes or more (wave-uniform)
+__device__ __forceinline__ bool run_moves_big(const DevStore &s, const NodeHdr &h, const GroupHdr &gh, const Growth &G)
+{
+    const bool grp = __ballot(G.g_g && (uint64_t)gh.size * 8u >= kBigCopyBytes) != 0ull;
+    return grp || (G.g_slots && (uint64_t)h.nslots * 4u >= kBigCopyBytes) ||
+           (G.g_back && (uint64_t)h.back_used * 4u >= kBigCopyBytes) ||
+           (G.g_hash && (uint64_t)h.hash_cap * 16u >= kBigCopyBytes);
+}
+
 __device__ __forceinline__ bool run_too_big(const NodeHdr &h, const Growth &G)
 {
     // group sizes must fit the 28-bit size field of an alias slot; the node's
@@ -653,6 +664,7 @@ __global__ __launch_bounds__(256) void k_insert_plan(DevStore s, const DevEdge *
     const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     unsigned long long need[NPOOL] = {0, 0, 0, 0};
+    uint32_t nbigcopy = 0;
     RunMeta m0 = run_meta(s, r, wave, nruns), m1 = run_meta(s, r, wave + nwaves, nruns);
     RunPre p0 = run_pre<false>(s, E, nullptr, m0, ln);
     for (uint32_t ri = wave; ri < nruns; ri += nwaves) {
@@ -676,10 +688,13 @@ __global__ __launch_bounds__(256) void k_insert_plan(DevStore s, const DevEdge *
         need[POOL_BACK] += G.g_back ? G.cap_back : 0;
         need[POOL_HASH] += G.g_hash ? 2ull * G.cap_hash : 0;  // 16-byte entries
         need[POOL_MEM] += dpp_sum64(G.g_g ? (uint64_t)G.cap_g : 0ull);
+        nbigcopy += run_moves_big(s, h, gh, G) ? 1u : 0u;
     }
-    if (ln == 0)
+    if (ln == 0) {
         for (int p = 0; p < NPOOL; ++p)
             wres[(size_t)wave * NPOOL + p] = need[p] ? atomicAdd(&s.ctl->need[p], need[p]) : 0ull;
+        if (nbigcopy) atomicAdd(&s.ctl->nbigcopy, nbigcopy);
+    }
 }
 
 // Region of n elements from a pool at the wave's cursor (wave-uniform); ~0
@@ -701,6 +716,33 @@ __global__ void k_commit_tops(Ctl *ctl)
 {
     const int t = threadIdx.x;
     if (t < NPOOL) ctl->top[t] += ctl->need[t];
+}
+
+// Hands a region move of kBigCopyBytes or more to k_copy_tasks (true), unless
+// the task table is full (false: the caller's wave moves it).  Wave-uniform.
+__device__ __forceinline__ bool defer_task(const DevStore &s, CopyTask t, int ln)
+{
+    const uint64_t bytes = t.kind == TASK_COPY ? t.n : 16u * t.n;
+    t.nch = (uint32_t)((bytes + kCopyChunk - 1) / kCopyChunk);
+    unsigned long long at = 0;
+    if (ln == 0) at = atomicAdd(&s.ctl->taskctr, (1ull << 32) | t.nch);
+    at = __shfl(at, 0, 64);
+    const uint32_t ti = (uint32_t)(at >> 32);
+    if (ti >= kMaxCopyTasks) return false;
+    t.cbase = (uint32_t)at;
+    if (ln == 0) s.tasks[ti] = t;
+    return true;
+}
+
+template <typename T>
+__device__ __forceinline__ void region_copy(const DevStore &s, T *dst, const T *src, uint32_t n, int ln)
+{
+    const uint64_t bytes = (uint64_t)n * sizeof(T);
+    if (bytes >= kBigCopyBytes) {
+        const CopyTask t = {(uint64_t)dst, (uint64_t)src, bytes, TASK_COPY, 0, 0u, 0u, 0u, 0u};
+        if (defer_task(s, t, ln)) return;
+    }
+    wave_copy(dst, src, n, ln);
 }
 
 // `fr` (float weights only) holds the run's fractions, arc for arc; `p` the
@@ -733,21 +775,32 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, con
     uint32_t *nf = nv + 3ull * G.cap_slots;  // float weights only
     if (G.g_slots && n0) {
         const uint32_t *ov = s.slot + h.slot_off;
-        wave_copy(nv, ov, n0, ln);
-        wave_copy(nw, ov + h.slot_cap, n0, ln);
-        wave_copy(nb, ov + 2ull * h.slot_cap, n0, ln);
-        if (s.fw) wave_copy(nf, ov + 3ull * h.slot_cap, n0, ln);
+        region_copy(s, nv, ov, n0, ln);
+        region_copy(s, nw, ov + h.slot_cap, n0, ln);
+        region_copy(s, nb, ov + 2ull * h.slot_cap, n0, ln);
+        if (s.fw) region_copy(s, nf, ov + 3ull * h.slot_cap, n0, ln);
     }
     uint32_t *back = s.back + a_back;
-    if (G.g_back && h.back_used) wave_copy(back, s.back + h.back_off, h.back_used, ln);
+    if (G.g_back && h.back_used) region_copy(s, back, s.back + h.back_off, h.back_used, ln);
     for (uint32_t gm = (uint32_t)__ballot(G.g_g && gh.size); gm;) {
         const int j = __ffs(gm) - 1;
         gm &= gm - 1u;
-        wave_copy(s.mem + rl64(goff, j), s.mem + rl64(gh.off, j), rl32(gh.size, j), ln);
+        region_copy(s, s.mem + rl64(goff, j), s.mem + rl64(gh.off, j), rl32(gh.size, j), ln);
     }
     uint32_t hused = h.hash_used;
     uint64_t *htab = s.hash + a_hash;
-    if (G.g_hash) {
+    bool rehash = G.g_hash;
+    if (rehash && (uint64_t)h.hash_cap * 16u >= kBigCopyBytes) {
+        // zeroing and re-inserting by k_copy_tasks, which counts the keys
+        // into the header's hash_used
+        const CopyTask t = {(uint64_t)htab, (uint64_t)(s.hash + h.hash_off), G.cap_hash, TASK_REHASH, u,
+                            h.hash_cap, 0u, 0u, 0u};
+        if (defer_task(s, t, ln)) {
+            rehash = false;
+            hused = 0;
+        }
+    }
+    if (rehash) {
         // Only this wave touches the new table until the kernel ends, so the
         // zeros need only be complete before its own CAS: a workgroup-scope
         // fence (a wait on the stores).  A device-scope fence here would write
@@ -2091,6 +2144,61 @@ __global__ __launch_bounds__(256) void k_gather_probe(const uint8_t *__restrict_
 }
 
 // ------------------------------------------------------------------------
+// deferred region moves of an insert batch (see CopyTask): phase 0 copies
+// regions and zeroes new neighbour tables, phase 1 re-inserts the old tables'
+// live keys.  A workgroup per chunk; the chunk's task is found by binary
+// search over the tasks' chunk bases, held in LDS.
+// ------------------------------------------------------------------------
+template <int PHASE>
+__global__ __launch_bounds__(256) void k_copy_tasks(DevStore s)
+{
+    __shared__ uint32_t cb[kMaxCopyTasks];
+    const uint32_t nt = min((uint32_t)(s.ctl->taskctr >> 32), kMaxCopyTasks);
+    if (nt == 0) return;
+    for (uint32_t i = threadIdx.x; i < nt; i += blockDim.x) cb[i] = s.tasks[i].cbase;
+    __syncthreads();
+    const uint32_t nchunks = cb[nt - 1] + s.tasks[nt - 1].nch;
+    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        uint32_t lo = 0, hi = nt - 1;  // last task with cbase <= c
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (cb[mid] <= c) lo = mid; else hi = mid - 1;
+        }
+        const CopyTask t = s.tasks[lo];
+        const uint32_t k = c - t.cbase;
+        if (PHASE == 0) {
+            const uint64_t total = t.kind == TASK_COPY ? t.n : 16u * t.n;
+            const uint64_t b0 = (uint64_t)k * kCopyChunk, b1 = min(b0 + kCopyChunk, total);
+            if (t.kind == TASK_COPY && ((t.dst | t.src | t.n) & 7u) == 0u) {
+                uint64_t *d = reinterpret_cast<uint64_t *>(t.dst);
+                const uint64_t *q = reinterpret_cast<const uint64_t *>(t.src);
+                for (uint64_t i = b0 / 8u + threadIdx.x; i < b1 / 8u; i += blockDim.x) d[i] = q[i];
+            } else if (t.kind == TASK_COPY) {
+                uint32_t *d = reinterpret_cast<uint32_t *>(t.dst);
+                const uint32_t *q = reinterpret_cast<const uint32_t *>(t.src);
+                for (uint64_t i = b0 / 4u + threadIdx.x; i < b1 / 4u; i += blockDim.x) d[i] = q[i];
+            } else {
+                uint64_t *d = reinterpret_cast<uint64_t *>(t.dst);
+                for (uint64_t i = b0 / 8u + threadIdx.x; i < b1 / 8u; i += blockDim.x) d[i] = 0ull;
+            }
+        } else if (t.kind == TASK_REHASH) {
+            // this chunk's share of the old table's entries
+            const uint64_t e0 = (uint64_t)t.old_cap * k / t.nch, e1 = (uint64_t)t.old_cap * (k + 1) / t.nch;
+            const uint64_t *old = reinterpret_cast<const uint64_t *>(t.src);
+            uint32_t fresh = 0;
+            for (uint64_t i = e0 + threadIdx.x; i < e1; i += blockDim.x) {
+                const uint64_t x = old[2u * i], w = old[2u * i + 1u];
+                if ((uint32_t)x && (x >> 32))
+                    fresh += hash_add(reinterpret_cast<uint64_t *>(t.dst), (uint32_t)t.n, (uint32_t)x,
+                                      (uint32_t)(x >> 32), w);
+            }
+            fresh = wave_sum32(fresh);
+            if (lane_id() == 0 && fresh) atomicAdd(&s.hdr[t.u].hash_used, fresh);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------------
 static inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -2123,6 +2231,12 @@ void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st)
     hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, st, static_cast<uint4 *>(dst),
                        static_cast<const uint4 *>(src), n16, static_cast<uint8_t *>(dst) + done,
                        static_cast<const uint8_t *>(src) + done, (uint32_t)(bytes - done));
+}
+
+void launch_copy_tasks(const DevStore &s, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_copy_tasks<0>, dim3(2048), dim3(256), 0, st, s);
+    hipLaunchKernelGGL(k_copy_tasks<1>, dim3(2048), dim3(256), 0, st, s);
 }
 
 void launch_reset_walk(Ctl *ctl, hipStream_t st) { hipLaunchKernelGGL(k_reset_walk, dim3(1), dim3(64), 0, st, ctl); }

@@ -99,7 +99,32 @@ struct Ctl {
     unsigned long long probes;       // last walk: neighbour-table membership lookups
     unsigned long long folds;        // last walk: node2vec return-region lookups (mass + entry)
     unsigned long long cursor;       // persistent walk: next unclaimed walker
+    unsigned int nbigcopy;           // insert plan: runs that move a region of kBigCopyBytes or more
+    unsigned int pad;
+    unsigned long long taskctr;      // insert apply: region moves left to k_copy_tasks << 32 | their chunks
 };
+
+// A region move of an insert batch that is too large for the run's own wave:
+// the apply kernel records it and k_copy_tasks carries it out with the whole
+// grid before the batch's neighbour-table kernel.  Nothing reads a moved
+// region's old part within the apply kernel (a run only appends past it), so
+// the move may finish later.  A task is cut into chunks of kCopyChunk bytes
+// of its first phase (copy, or zeroing the new table), one workgroup each;
+// the tasks' chunk ranges [cbase, cbase + nch) follow each other in task
+// order (one 64-bit atomic hands out both).
+enum CopyKind : uint32_t { TASK_COPY = 0, TASK_REHASH = 1 };
+struct CopyTask {
+    uint64_t dst, src;  // device addresses
+    uint64_t n;         // TASK_COPY: bytes; TASK_REHASH: the new table's capacity (entries)
+    uint32_t kind;
+    int32_t u;          // TASK_REHASH: the node (its header counts the re-inserted keys)
+    uint32_t old_cap;   // TASK_REHASH: the old table's capacity
+    uint32_t cbase, nch;
+    uint32_t pad;
+};
+constexpr uint64_t kBigCopyBytes = 64ull << 10;
+constexpr uint64_t kCopyChunk = 16ull << 10;
+constexpr uint32_t kMaxCopyTasks = 4096;
 
 struct DevEdge {
     int32_t u, v;
@@ -128,6 +153,7 @@ struct DevStore {
     uint64_t *hash;
     unsigned long long cap[NPOOL];
     Ctl *ctl;
+    CopyTask *tasks;  // [kMaxCopyTasks]
 };
 
 struct WalkArgs {
@@ -225,6 +251,7 @@ void launch_edge_count(DevStore s, const int32_t *us, const int32_t *vs, int64_t
                        hipStream_t st);
 
 void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st);
+void launch_copy_tasks(const DevStore &s, hipStream_t st);
 // random reads of `width` (8 or 32) bytes over units * width bytes at base;
 // returns the number of reads
 uint64_t launch_gather_probe(const void *base, uint64_t units, int width, unsigned long long *sink, hipStream_t st);
