@@ -1604,13 +1604,19 @@ __global__ __launch_bounds__(256) void k_walk(DevStore s, WalkArgs a)
 // vectors.  Walker i's path depends only on (seed, i, start, len), so any
 // schedule gives the same paths as k_walk and the oracle.
 // ------------------------------------------------------------------------
-// Path rows are written once and not read back by the walk: 16-byte
-// non-temporal stores keep them from displacing the store's lines in L2.
+// Path rows are written once and not read back by the walk.  Whole lines
+// written by a wave together go out as non-temporal stores (kept from
+// displacing the store's lines in L2); a lane's own 16-byte pieces of its
+// row stay ordinary stores, so that L2 merges them into lines.
 typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ void store_row4(int32_t *p, int32_t a, int32_t b, int32_t c, int32_t d)
 {
     const i32x4 v = {a, b, c, d};
     __builtin_nontemporal_store(v, reinterpret_cast<i32x4 *>(p));
+}
+__device__ __forceinline__ void store4(int32_t *p, int32_t a, int32_t b, int32_t c, int32_t d)
+{
+    *reinterpret_cast<int4 *>(p) = make_int4(a, b, c, d);
 }
 
 template <int WPL, int D>
@@ -1632,7 +1638,7 @@ struct PathOut {
         int32_t *dst = a.paths + wid * (int64_t)a.max_len + j0;
 #pragma unroll
         for (int c = 0; c < D / 4; ++c)
-            if (4 * c < cnt) store_row4(dst + 4 * c, st[4 * c], st[4 * c + 1], st[4 * c + 2], st[4 * c + 3]);
+            if (4 * c < cnt) store4(dst + 4 * c, st[4 * c], st[4 * c + 1], st[4 * c + 2], st[4 * c + 3]);
     }
     // write out the partial stage (its tail -1) and return the first position
     // not yet written (a multiple of D when D > 0)
@@ -1697,6 +1703,12 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
     }
     uint32_t steps = 0, draws = 0, probes = 0, folds = 0;
     bool more = true;
+    // walker ids are claimed from the global cursor kClaim at a time into a
+    // wave-local range [lp, le) and handed to empty slots from there: one
+    // atomic per kClaim walkers instead of one per refill (short walkers
+    // refill every few iterations, and the cursor is one contended address)
+    constexpr uint32_t kClaim = 4u * 64u * WPL;
+    uint64_t lp = 0, le = 0;
     for (;;) {
         if (more) {
             uint64_t emp[WPL];
@@ -1707,14 +1719,24 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
                 need += (uint32_t)__popcll(emp[k]);
             }
             if (need) {
-                unsigned long long base = 0;
-                if (ln == 0) base = atomicAdd(a.cursor, (unsigned long long)need);
-                base = __shfl(base, 0, 64);
+                // ranks below n0 take the rest of the local range, the others
+                // a fresh claim (need <= 64 WPL <= kClaim: one claim suffices)
+                const uint64_t r0 = lp, n0 = le - lp;
+                unsigned long long b = 0;
+                if (n0 < need) {
+                    if (ln == 0) b = atomicAdd(a.cursor, (unsigned long long)kClaim);
+                    b = __shfl(b, 0, 64);
+                    lp = b + (need - n0);
+                    le = b + kClaim;
+                } else {
+                    lp += need;
+                }
                 uint32_t off = 0;
 #pragma unroll
                 for (int k = 0; k < WPL; ++k) {
                     if (wid[k] < 0) {
-                        const uint64_t c = base + off + (uint32_t)__popcll(emp[k] & lt);
+                        const uint64_t rk = off + (uint32_t)__popcll(emp[k] & lt);
+                        const uint64_t c = rk < n0 ? r0 + rk : b + (rk - n0);
                         if (c < (uint64_t)a.nwalkers) {
                             const int64_t w = (int64_t)c;
                             const int32_t st = a.starts ? a.starts[w] : (int32_t)(w % s.n) + 1;
@@ -1740,7 +1762,8 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
                     }
                     off += (uint32_t)__popcll(emp[k]);
                 }
-                if (base + need >= (uint64_t)a.nwalkers) more = false;
+                // nothing valid left locally; the cursor is past the end too
+                if (lp >= (uint64_t)a.nwalkers) more = false;
             }
         }
         bool mine = false;
@@ -1809,12 +1832,29 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
             }
         }
         // Row tails of the walkers that finished in this iteration, padded
-        // with -1 by the whole wave one row at a time: consecutive lanes
-        // write consecutive 16-byte (D > 0: rows are 16-byte aligned from
-        // position j, a multiple of D) or 4-byte pieces of the row.
+        // with -1.  Few rows: by the whole wave one row at a time, consecutive
+        // lanes writing consecutive 16-byte (D > 0: rows are 16-byte aligned
+        // from position j, a multiple of D) or 4-byte pieces.  Many rows (a
+        // wave's worth of one-step walkers): each lane its own row, which
+        // then takes fewer store instructions.
+        constexpr int32_t per = D > 0 ? 4 : 1;
 #pragma unroll
         for (int k = 0; k < WPL; ++k) {
-            for (uint64_t m = __ballot(padw[k] >= 0); m; m &= m - 1) {
+            const uint64_t fin = __ballot(padw[k] >= 0);
+            const int32_t rows = __popcll(fin);
+            if ((int64_t)rows * ((a.max_len + 64 * per - 1) / (64 * per)) > (a.max_len + per - 1) / per) {
+                if (padw[k] >= 0) {
+                    int32_t *row = a.paths + padw[k] * (int64_t)a.max_len;
+                    if (D > 0) {
+                        for (int32_t t = padj[k]; t < a.max_len; t += 4) store4(row + t, -1, -1, -1, -1);
+                    } else {
+                        for (int32_t t = padj[k]; t < a.max_len; ++t) row[t] = -1;
+                    }
+                }
+                padw[k] = -1;
+                continue;
+            }
+            for (uint64_t m = fin; m; m &= m - 1) {
                 const int src = __ffsll((unsigned long long)m) - 1;
                 const uint32_t wlo = __shfl((uint32_t)padw[k], src, 64);
                 const uint32_t whi = __shfl((uint32_t)((uint64_t)padw[k] >> 32), src, 64);
