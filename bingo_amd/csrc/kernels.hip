@@ -194,8 +194,10 @@ __device__ __forceinline__ uint64_t make_hmeta(uint64_t off, uint32_t cap)
     return cap ? ((uint64_t)(32 - __clz(cap)) | (off << 6)) : 0ull;
 }
 
+// hm: a table locator, possibly with the alias slot's bits above it (masked)
 __device__ __forceinline__ bool is_neighbor(const uint64_t *__restrict__ hash, uint64_t hm, uint32_t key)
 {
+    hm &= kHmetaMask;
     const uint32_t lg = (uint32_t)(hm & 63);
     if (!lg) return false;
     const uint32_t cap = 1u << (lg - 1u), mask = cap - 1u;
@@ -213,6 +215,7 @@ __device__ __forceinline__ bool is_neighbor(const uint64_t *__restrict__ hash, u
 // weight sum of the live arcs to `key` in the table located by hm (0 if none)
 __device__ __forceinline__ uint64_t neighbor_weight(const uint64_t *__restrict__ hash, uint64_t hm, uint32_t key)
 {
+    hm &= kHmetaMask;
     const uint32_t lg = (uint32_t)(hm & 63);
     if (!lg) return 0ull;
     const uint32_t cap = 1u << (lg - 1u), mask = cap - 1u;
@@ -339,13 +342,50 @@ __device__ __forceinline__ uint64_t fold_threshold(uint64_t S, uint64_t wret, ui
     return frac64((uint64_t)num, (uint64_t)den);
 }
 
-// node2vec's per-attempt fold coin: the 64-bit uniform of stream
-// (walker; step | kMixBit, attempt) below tout takes the return arc
-__device__ __forceinline__ bool fold_coin(uint32_t w0, uint32_t w1, uint32_t step, uint32_t att, uint32_t k0,
-                                          uint32_t k1, uint64_t tout)
+// A node's integer weight S in 17 bits (kept in every alias slot of an
+// integer-weight store): exact below 2^11, else exponent e << 11 | the top 11
+// bits m, with m 2^e <= S < (m + 1) 2^e.
+__device__ __forceinline__ uint32_t mass_code(uint64_t S)
 {
+    const int bl = S ? 64 - __clzll((long long)S) : 0;
+    if (bl <= 11) return (uint32_t)S;
+    const int e = bl - 11;
+    return ((uint32_t)e << 11) | (uint32_t)(S >> e);
+}
+
+// Bracket [lo, hi) around the fold threshold tout for a node whose mass
+// code is `code` and whose return arc weighs w: a coin below lo is below
+// tout, a coin at or above hi is not.  tout is within 33 of 2^64 num / den
+// (the 60-bit cut moves the ratio by at most 2^-59), which falls as S grows,
+// so the thresholds at the range's ends widened by 256 bracket every S in
+// it.  lo = hi = tout when the code holds S exactly.
+__device__ __forceinline__ void fold_bracket(uint32_t code, uint64_t w, uint64_t fold_r, uint64_t &lo, uint64_t &hi)
+{
+    const uint32_t e = code >> 11;
+    const uint64_t slo = (uint64_t)(code & 2047u) << e, shi = slo + ((1ull << e) - 1ull);
+    if (slo == shi) {
+        lo = hi = fold_threshold(slo, w, fold_r);
+        return;
+    }
+    const uint64_t a = fold_threshold(shi, w, fold_r), b = fold_threshold(slo, w, fold_r);
+    lo = a > 256u ? a - 256u : 0ull;
+    hi = b < ~0ull - 256u ? b + 256u : ~0ull;
+}
+
+// Whether the fold coin of (walker, step, attempt) takes the return arc;
+// the node's exact mass is read only when the coin falls inside the
+// bracket, which then collapses to the exact threshold.
+__device__ __forceinline__ bool fold_taken(const DevStore &s, int32_t cur, uint64_t w, uint64_t fold_r, uint64_t &lo,
+                                           uint64_t &hi, uint32_t w0, uint32_t w1, uint32_t step, uint32_t att,
+                                           uint32_t k0, uint32_t k1)
+{
+    if (!hi) return false;  // tout = 0
     const uint4 m = philox(w0, w1, step | kMixBit, att, k0, k1);
-    return (((uint64_t)m.x << 32) | m.y) < tout;
+    const uint64_t c = ((uint64_t)m.x << 32) | m.y;
+    if (c < lo) return true;
+    if (c >= hi) return false;
+    lo = hi = fold_threshold(s.nmass[cur], w, fold_r);
+    return c < lo;
 }
 
 // Returns the node's integer mass S (all lanes).  All 64 lanes active.  With
@@ -399,6 +439,7 @@ __device__ uint64_t alias_rebuild(const DevStore &s, int32_t u, uint32_t size, u
         thr_hi = (thr >> 47) + ((thr & ((1ull << 47) - 1)) != 0);
         thr_hi = thr_hi > kThrHiMax ? kThrHiMax : thr_hi;
     }
+    if (!s.fw) thr_hi = mass_code(S);
     if (s.nmass && ln == 0) s.nmass[u] = S;
     const uint32_t size_a = __shfl(size, ali, 64);
     const uint64_t off_a = __shfl(off, ali, 64);
@@ -1400,7 +1441,9 @@ __global__ __launch_bounds__(256) void k_delete_apply(DevStore s, const uint32_t
 // One draw on node `cur` (NodeData::sample, include/Bingo.cuh:220-267):
 // slot i = top 5 bits of r.x; the 64-bit coin (r.y:r.z) against T_i picks
 // group i or its partner; member floor(r.w * size / 2^32) of that group.
-// `hm` receives cur's neighbour-table locator carried by the slot.  Split in
+// `hm` receives the slot's meta word without the partner index: cur's
+// neighbour-table locator (kHmetaMask bits) and the 17-bit field above it
+// (mass code or float threshold ceiling).  Split in
 // two so that callers can issue the slot read together with other reads.
 // The slot is read as two 16-byte loads.  The empty asm takes both as
 // operands, so the compiler can neither narrow them to the fields a caller
@@ -1435,7 +1478,7 @@ __device__ __forceinline__ int32_t slot_member(const uint64_t *__restrict__ mem,
     const uint64_t T = ((uint64_t)q0.y << 32) | q0.x;
     const uint64_t coin = ((uint64_t)r.y << 32) | r.z;
     const uint64_t loc = coin < T ? (((uint64_t)q0.w << 32) | q0.z) : (((uint64_t)q1.y << 32) | q1.x);
-    hm = ((((uint64_t)q1.w << 32) | q1.z) >> 5) & kHmetaMask;
+    hm = (((uint64_t)q1.w << 32) | q1.z) >> 5;  // locator | the 17-bit field << 42
     const uint32_t sz = (uint32_t)(loc >> kOffBits);
     if (sz == 0u) return -1;
     const uint32_t pos = __umulhi(r.w, sz);
@@ -1528,12 +1571,11 @@ __global__ __launch_bounds__(256) void k_walk(DevStore s, WalkArgs a)
                 uint4 r;
                 int32_t nx;
                 if (APP == 1 && j >= 2) {
-                    // folded return region (integer weights only): cur's mass
-                    // is read with the first draw, the return arc's weight sum
-                    // from cur's neighbour table once the slot gave its locator
+                    // folded return region (integer weights only): the return
+                    // arc's weight sum from cur's neighbour table once the
+                    // first draw's slot gave its locator and mass code
                     const bool fold = !FW && a.fold_r;
-                    const uint64_t Sc = fold ? s.nmass[cur] : 0ull;
-                    uint64_t tout = 0;
+                    uint64_t wret = 0, flo = 0, fhi = 0;
                     for (uint32_t at = 0;; ++at) {
                         r = philox(w0, w1, (uint32_t)j, at, k0, k1);
                         nx = draw<FW>(s, cur, r, hcur, w0, w1, (uint32_t)j, at, k0, k1);
@@ -1541,9 +1583,10 @@ __global__ __launch_bounds__(256) void k_walk(DevStore s, WalkArgs a)
                         if (fold) {
                             if (at == 0) {
                                 ++folds;
-                                tout = fold_threshold(Sc, neighbor_weight(s.hash, hcur, (uint32_t)prev), a.fold_r);
+                                wret = neighbor_weight(s.hash, hcur, (uint32_t)prev);
+                                fold_bracket((uint32_t)(hcur >> 42), wret, a.fold_r, flo, fhi);
                             }
-                            if (tout && fold_coin(w0, w1, (uint32_t)j, at, k0, k1, tout)) {
+                            if (fold_taken(s, cur, wret, a.fold_r, flo, fhi, w0, w1, (uint32_t)j, at, k0, k1)) {
                                 nx = prev;
                                 break;
                             }
@@ -1687,7 +1730,8 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
     int32_t cur[WPL], prev[WPL], jj[WPL], len[WPL];
     uint32_t att[WPL];
     uint64_t hp[WPL];    // neighbour-table locator of each slot's previous node
-    uint64_t tout[WPL];  // node2vec: the step's folded return threshold (attempt 0 sets it)
+    uint64_t flo[WPL], fhi[WPL];  // node2vec: the step's fold bracket (attempt 0 sets it)
+    uint64_t wret[WPL];           // ... and the return arc's weight
     int64_t padw[WPL];   // walker whose row tail the wave pads at the end of this iteration (-1: none)
     int32_t padj[WPL];   // ... from this position on
     const bool fold = !FW && APP == 1 && a.fold_r;
@@ -1697,7 +1741,7 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
         cur[k] = prev[k] = jj[k] = len[k] = 0;
         att[k] = 0;
         hp[k] = 0;
-        tout[k] = 0;
+        flo[k] = fhi[k] = wret[k] = 0;
         padw[k] = -1;
         padj[k] = 0;
     }
@@ -1779,18 +1823,18 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
             const int64_t w = wid[k];
             const uint32_t w0 = (uint32_t)w, w1 = (uint32_t)((uint64_t)w >> 32);
             const uint4 r = philox(w0, w1, (uint32_t)jj[k], att[k], k0, k1);
-            const bool f0 = fold && jj[k] >= 2 && att[k] == 0;
-            const uint64_t Sc = f0 ? s.nmass[cur[k]] : 0ull;
             uint64_t hcur;
             int32_t nx = draw<FW>(s, cur[k], r, hcur, w0, w1, (uint32_t)jj[k], att[k], k0, k1);
             ++draws;
             bool taken = false;  // the fold coin took the return arc
             if (fold && jj[k] >= 2) {
-                if (f0) {
+                if (att[k] == 0) {
                     ++folds;
-                    tout[k] = fold_threshold(Sc, neighbor_weight(s.hash, hcur, (uint32_t)prev[k]), a.fold_r);
+                    wret[k] = neighbor_weight(s.hash, hcur, (uint32_t)prev[k]);
+                    fold_bracket((uint32_t)(hcur >> 42), wret[k], a.fold_r, flo[k], fhi[k]);
                 }
-                if (tout[k] && fold_coin(w0, w1, (uint32_t)jj[k], att[k], k0, k1, tout[k])) {
+                if (fold_taken(s, cur[k], wret[k], a.fold_r, flo[k], fhi[k], w0, w1, (uint32_t)jj[k], att[k], k0,
+                               k1)) {
                     nx = prev[k];
                     taken = true;
                 }
@@ -1908,14 +1952,14 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
     int64_t wid[kServeWPL];
     uint32_t att[kServeWPL], acc[kServeWPL];
     int32_t acc_req[kServeWPL];
-    uint64_t hp[kServeWPL], tout[kServeWPL];
+    uint64_t hp[kServeWPL], flo[kServeWPL], fhi[kServeWPL], wret[kServeWPL];
 #pragma unroll
     for (int k = 0; k < kServeWPL; ++k) {
         rid[k] = acc_req[k] = -1;
         cur[k] = prev[k] = jj[k] = len[k] = 0;
         wid[k] = 0;
         att[k] = acc[k] = 0;
-        hp[k] = tout[k] = 0;
+        hp[k] = flo[k] = fhi[k] = wret[k] = 0;
     }
     bool more = true;
     for (;;) {
@@ -1994,14 +2038,16 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
             const uint32_t k0 = (uint32_t)q.seed, k1 = (uint32_t)(q.seed >> 32);
             const uint4 r = philox(w0, w1, (uint32_t)jj[k], att[k], k0, k1);
             const bool fold = !FW && q.app == 1 && q.fold_r && jj[k] >= 2;
-            const uint64_t Sc = fold && att[k] == 0 ? s.nmass[cur[k]] : 0ull;
             uint64_t hcur;
             int32_t nx = draw<FW>(s, cur[k], r, hcur, w0, w1, (uint32_t)jj[k], att[k], k0, k1);
             bool taken = false;  // the fold coin took the return arc
             if (fold) {
-                if (att[k] == 0)
-                    tout[k] = fold_threshold(Sc, neighbor_weight(s.hash, hcur, (uint32_t)prev[k]), q.fold_r);
-                if (tout[k] && fold_coin(w0, w1, (uint32_t)jj[k], att[k], k0, k1, tout[k])) {
+                if (att[k] == 0) {
+                    wret[k] = neighbor_weight(s.hash, hcur, (uint32_t)prev[k]);
+                    fold_bracket((uint32_t)(hcur >> 42), wret[k], q.fold_r, flo[k], fhi[k]);
+                }
+                if (fold_taken(s, cur[k], wret[k], q.fold_r, flo[k], fhi[k], w0, w1, (uint32_t)jj[k], att[k], k0,
+                               k1)) {
                     nx = prev[k];
                     taken = true;
                 }

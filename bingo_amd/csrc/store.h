@@ -68,12 +68,15 @@ constexpr uint64_t kHmetaMask = (1ull << 42) - 1;
 //   alias locator of the partner:    member offset (36 bits) | size << 36
 //   meta  partner index (5 bits) | the node's neighbour-table locator << 5
 //         (42 bits: (log2 cap + 1) | table offset << 6, 0: no table)
-//         | with float weights, the node's fractional-part threshold
-//         rounded up to 17 bits << 47 (ceil(thr / 2^47), at most 2^17 - 1;
-//         0 iff thr = 0)
+//         | << 47, 17 bits: with float weights, the node's fractional-part
+//         threshold rounded up (ceil(thr / 2^47), at most 2^17 - 1; 0 iff
+//         thr = 0); with integer weights, the node's integer weight S coded
+//         as exponent e (6 bits) << 11 | the top 11 bits m of S (S exact
+//         below 2^11; else m 2^e <= S < (m + 1) 2^e)
 // so a walk step learns the group, its members, (for node2vec) where the
-// node's neighbour table lives and (for float weights) almost always whether
-// to take the fractional part, from a single read.
+// node's neighbour table lives and almost always whether to take node2vec's
+// folded return region or (float weights) the fractional part, from a
+// single read.
 struct AliasEnt {
     uint64_t T;
     uint64_t stay;
