@@ -564,7 +564,7 @@ void orc_walk_thresholds(double p, double q, double tp, orc_walk_params *out)
 
 /* Probability of the folded return region as a 64-bit threshold:
  * floor(2^64 num / den), num = w_ret fold_r, den = S 2^27 + num, both cut to
- * their top 60 bits by the same shift (relative error < 2^-59).  0 when
+ * their top 60 bits by the same shift (the ratio moves by at most 2^-59).  0 when
  * num = 0; all-ones when num >= den after the cut. */
 uint64_t orc_fold_threshold(uint64_t S, uint64_t wret, uint64_t fold_r)
 {
