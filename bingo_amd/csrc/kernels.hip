@@ -1938,16 +1938,60 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
 // what it was batched with.
 // ------------------------------------------------------------------------
 constexpr int kServeWPL = 2;
+constexpr int kServeD = 8;  // path positions staged per slot in LDS
+
+// A served walker's row: staged kServeD positions at a time in LDS and
+// written as 16-byte stores when the request's rows allow it (max_len a
+// multiple of kServeD, 16-byte aligned paths), else position by position.
+struct ServeRow {
+    int32_t *st;  // this slot's kServeD + 1 staging ints
+    __device__ __forceinline__ void flush(int32_t *row, int32_t j0) const
+    {
+#pragma unroll
+        for (int c = 0; c < kServeD / 4; ++c) store4(row + j0 + 4 * c, st[4 * c], st[4 * c + 1], st[4 * c + 2], st[4 * c + 3]);
+    }
+    __device__ __forceinline__ void put(bool stg, int32_t *row, int32_t j, int32_t v) const
+    {
+        if (!stg) {
+            row[j] = v;
+            return;
+        }
+        st[j % kServeD] = v;
+        if (j % kServeD == kServeD - 1) flush(row, j - (kServeD - 1));
+    }
+    __device__ __forceinline__ void finish(bool stg, int32_t *row, int32_t j, int32_t max_len) const
+    {
+        if (!stg) {
+            for (; j < max_len; ++j) row[j] = -1;
+            return;
+        }
+        const int32_t r = j % kServeD;
+        if (r) {
+            for (int t = r; t < kServeD; ++t) st[t] = -1;
+            flush(row, j - r);
+            j += kServeD - r;
+        }
+        for (; j < max_len; j += 4) store4(row + j, -1, -1, -1, -1);
+    }
+};
 
 template <bool FW>
 __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *__restrict__ reqs, int nreq,
                                                     unsigned long long total, unsigned long long *cursor)
 {
     __shared__ ServeReq rq[kMaxServeReqs];
+    __shared__ int32_t stage[256 * kServeWPL * (kServeD + 1)];  // odd pitch: no bank conflicts
     for (int i = threadIdx.x; i < nreq; i += blockDim.x) rq[i] = reqs[i];
     __syncthreads();
     const int ln = lane_id();
     const uint64_t lt = (1ull << ln) - 1ull;
+    ServeRow out[kServeWPL];
+    bool stg[kServeWPL];
+#pragma unroll
+    for (int k = 0; k < kServeWPL; ++k) {
+        out[k].st = stage + ((size_t)threadIdx.x * kServeWPL + k) * (kServeD + 1);
+        stg[k] = false;
+    }
     int32_t rid[kServeWPL], cur[kServeWPL], prev[kServeWPL], jj[kServeWPL], len[kServeWPL];
     int64_t wid[kServeWPL];
     uint32_t att[kServeWPL], acc[kServeWPL];
@@ -2001,7 +2045,8 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
                                 atomicOr(&s.ctl->error, (unsigned)ERR_INVAL);
                                 for (int32_t t = 0; t < q.max_len; ++t) row[t] = -1;
                             } else {
-                                row[0] = st;
+                                stg[k] = q.max_len % kServeD == 0 && ((uintptr_t)q.paths & 15u) == 0;
+                                out[k].put(stg[k], row, 0, st);
                                 rid[k] = lo;
                                 wid[k] = w;
                                 cur[k] = st;
@@ -2010,7 +2055,7 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
                                 len[k] = L;
                                 att[k] = 0;
                                 if (L == 1) {
-                                    for (int32_t t = 1; t < q.max_len; ++t) row[t] = -1;
+                                    out[k].finish(stg[k], row, 1, q.max_len);
                                     rid[k] = -1;
                                 }
                             }
@@ -2067,7 +2112,7 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
                         continue;
                     }
                 }
-                row[jj[k]] = nx;
+                out[k].put(stg[k], row, jj[k], nx);
                 ++acc[k];
                 prev[k] = cur[k];
                 cur[k] = nx;
@@ -2077,7 +2122,7 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
                 done = (q.app == 2 && aux < q.thr_stop) || jj[k] >= len[k];
             }
             if (done) {
-                for (int32_t t = jj[k]; t < q.max_len; ++t) row[t] = -1;
+                out[k].finish(stg[k], row, jj[k], q.max_len);
                 rid[k] = -1;
             }
         }
