@@ -376,7 +376,7 @@ def main():
         tuned = dict(form=B.FORM_PERSISTENT, block=blk, walkers_per_lane=wpl, lds_depth=dep)
     else:
         wp = B.walk_params(**kw)
-        tuned = g.tune(app, wp, sample=min(nw, 1 << 20))
+        tuned = g.tune(app, wp, sample=nw)  # at the workload's own walker count
     log(f"walk schedule: {tuned}")
     wgraph = g.walk_graph(app, wp, nw, d_paths, d_steps, d_lens=d_lens) if cfg["graph"] else None
 

@@ -2569,12 +2569,19 @@ void launch_edge_count(DevStore s, const int32_t *us, const int32_t *vs, int64_t
 // ------------------------------------------------------------------------
 // rocPRIM wrappers
 // ------------------------------------------------------------------------
+// Update batches sort node ids (23-30 significant bits; deletes also the
+// slot).  rocPRIM's default takes a block merge sort up to 2^20 items, which
+// for a 10^6-arc batch costs ~20 launches and 200-250 us; past 2^16 items
+// the onesweep radix sort (a few passes of 8-bit digits) is used instead.
+using BatchSortCfg = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                                rocprim::default_config, (size_t)1 << 16>;
+
 size_t sort_pairs_u32(void *tmp, size_t tmp_bytes, uint32_t *k0, uint32_t *k1, uint32_t *v0, uint32_t *v1, int64_t k,
                       int bits, hipStream_t st, bool *second)
 {
     size_t b = tmp_bytes;
     rocprim::double_buffer<uint32_t> keys(k0, k1), vals(v0, v1);
-    (void)rocprim::radix_sort_pairs(tmp, b, keys, vals, (size_t)k, 0, (unsigned)bits, st);
+    (void)rocprim::radix_sort_pairs<BatchSortCfg>(tmp, b, keys, vals, (size_t)k, 0, (unsigned)bits, st);
     if (second) *second = keys.current() == k1;
     return b;
 }
@@ -2583,7 +2590,7 @@ size_t sort_keys_u64(void *tmp, size_t tmp_bytes, const unsigned long long *kin,
                      int bits, hipStream_t st)
 {
     size_t b = tmp_bytes;
-    (void)rocprim::radix_sort_keys(tmp, b, kin, kout, (size_t)k, 0, (unsigned)bits, st);
+    (void)rocprim::radix_sort_keys<BatchSortCfg>(tmp, b, kin, kout, (size_t)k, 0, (unsigned)bits, st);
     return b;
 }
 
