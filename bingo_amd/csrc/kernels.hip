@@ -1747,11 +1747,13 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
     }
     uint32_t steps = 0, draws = 0, probes = 0, folds = 0;
     bool more = true;
-    // walker ids are claimed from the global cursor kClaim at a time into a
+    // walker ids are claimed from the global cursor a chunk at a time into a
     // wave-local range [lp, le) and handed to empty slots from there: one
-    // atomic per kClaim walkers instead of one per refill (short walkers
-    // refill every few iterations, and the cursor is one contended address)
-    constexpr uint32_t kClaim = 4u * 64u * WPL;
+    // atomic per chunk instead of one per refill (short walkers refill every
+    // few iterations, and the cursor is one contended address).  Chunks stay
+    // small enough that every wave of the grid gets about 8 of them.
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint32_t chunk = (uint32_t)min<uint64_t>(4u * 64u * WPL, (uint64_t)a.nwalkers / (8u * nwaves));
     uint64_t lp = 0, le = 0;
     for (;;) {
         if (more) {
@@ -1764,14 +1766,15 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
             }
             if (need) {
                 // ranks below n0 take the rest of the local range, the others
-                // a fresh claim (need <= 64 WPL <= kClaim: one claim suffices)
+                // a fresh claim of at least `need` ids
                 const uint64_t r0 = lp, n0 = le - lp;
                 unsigned long long b = 0;
                 if (n0 < need) {
-                    if (ln == 0) b = atomicAdd(a.cursor, (unsigned long long)kClaim);
+                    const uint32_t claim = max(chunk, need);
+                    if (ln == 0) b = atomicAdd(a.cursor, (unsigned long long)claim);
                     b = __shfl(b, 0, 64);
                     lp = b + (need - n0);
-                    le = b + kClaim;
+                    le = b + claim;
                 } else {
                     lp += need;
                 }
