@@ -1184,6 +1184,16 @@ __device__ void delete_small(const DevStore &s, uint32_t cnt, int32_t u, const R
         kj += (wq[q] & bit) ? 1u : 0u;
     }
     const uint32_t S2 = S - kj;  // kj <= c: the tail [S2, S) is prefetched
+    // slot records of the tail [S2, S), read together with the positions
+    // above: the survivors among them are the members that move
+    uint32_t eb[K], ew[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        const bool cand = (uint32_t)i < c && (uint32_t)i + kj >= c;
+        const uint32_t ms = (uint32_t)e[i];
+        eb[i] = cand ? B[ms] : 0u;
+        ew[i] = cand ? W[ms] : 0u;
+    }
 
     // survivors: tail members whose slot is not one of this group's deletes
     bool surv[K];
@@ -1201,32 +1211,32 @@ __device__ void delete_small(const DevStore &s, uint32_t cnt, int32_t u, const R
     // k-th survivor
     bool hole[K], bad = false;
     uint64_t mv[K];
+    uint32_t mb[K], mw[K];  // the moved members' back offsets and weights
     uint32_t hr = 0;
 #pragma unroll
     for (int q = 0; q < K; ++q) {
         const bool h0 = (wq[q] & bit) && pos[q] < S2;
         uint64_t x = 0;
+        uint32_t xb = 0, xw = 0;
         bool found = false;
 #pragma unroll
         for (int i = 0; i < K; ++i)
             if (surv[i] && srank[i] == hr) {
                 x = e[i];
+                xb = eb[i];
+                xw = ew[i];
                 found = true;
             }
         bad |= h0 && !found;  // no survivor left: the store is inconsistent
         hole[q] = h0 && found;
         mv[q] = x;
+        mb[q] = xb;
+        mw[q] = xw;
         hr += h0 ? 1u : 0u;
     }
-    // the moved members' back offsets
-    uint32_t mb[K], mw[K];
 #pragma unroll
-    for (int q = 0; q < K; ++q) {
-        const uint32_t ms = (uint32_t)mv[q];
-        mb[q] = hole[q] ? B[ms] : 0u;
-        mw[q] = hole[q] ? W[ms] : 0u;
+    for (int q = 0; q < K; ++q)
         if (hole[q]) m[pos[q]] = mv[q];
-    }
 #pragma unroll
     for (int q = 0; q < K; ++q)
         if (hole[q]) back[mb[q] + __popc(mw[q] & below)] = pos[q];
