@@ -374,12 +374,19 @@ int bingo::fill_walk_args(const bingo_walk_params *p, int app, WalkArgs *a, int 
         a->thr_neighbor = thr27(1.0 / env);
         a->thr_other = thr27(iq / env);
         a->fold_r = (uint64_t)std::floor((ip - env) / env * 134217728.0);
+        // a return weight is at most its node's weight, so every threshold
+        // is at most 2^64 R / (2^27 + R) (+ 33 for the 60-bit cut); the
+        // kernels skip the weight lookup for coins at or above this bound
+        typedef unsigned __int128 u128;
+        const u128 q = ((u128)a->fold_r << 64) / (((u128)1 << 27) + a->fold_r);
+        a->fold_cap = q >= ((u128)1 << 64) - 128 ? ~0ull : (uint64_t)q + 128u;
     } else {
         const double lim = std::max(ip, env);
         a->thr_return = thr27(ip / lim);
         a->thr_neighbor = thr27(1.0 / lim);
         a->thr_other = thr27(iq / lim);
         a->fold_r = 0;
+        a->fold_cap = 0;
     }
     a->thr_stop = thr27(p->tp);
     a->seed = p->seed;

@@ -372,20 +372,39 @@ __device__ __forceinline__ void fold_bracket(uint32_t code, uint64_t w, uint64_t
     hi = b < ~0ull - 256u ? b + 256u : ~0ull;
 }
 
-// Whether the fold coin of (walker, step, attempt) takes the return arc;
-// the node's exact mass is read only when the coin falls inside the
-// bracket, which then collapses to the exact threshold.
-__device__ __forceinline__ bool fold_taken(const DevStore &s, int32_t cur, uint64_t w, uint64_t fold_r, uint64_t &lo,
-                                           uint64_t &hi, uint32_t w0, uint32_t w1, uint32_t step, uint32_t att,
-                                           uint32_t k0, uint32_t k1)
+// A walker's fold state for one step.  Until the return arc's weight is
+// looked up (known false), a coin at or above fold_cap -- a bound on every
+// node's threshold, since a return weight never exceeds its node's weight
+// -- cannot take the return arc, and no lookup is made for it.
+struct FoldState {
+    uint64_t lo, hi, w;
+    bool known;
+};
+
+// Whether the fold coin of (walker, step, attempt) takes the return arc.
+// The first coin below fold_cap looks up the return arc's weight in cur's
+// neighbour table and brackets the threshold from the slot's mass code; the
+// node's exact mass is read only when a coin falls inside the bracket,
+// which then collapses to the exact threshold.
+__device__ __forceinline__ bool fold_step(const DevStore &s, int32_t cur, int32_t prev, uint64_t hcur,
+                                          uint64_t fold_r, uint64_t fold_cap, FoldState &f, uint32_t &lookups,
+                                          uint32_t w0, uint32_t w1, uint32_t step, uint32_t att, uint32_t k0,
+                                          uint32_t k1)
 {
-    if (!hi) return false;  // tout = 0
     const uint4 m = philox(w0, w1, step | kMixBit, att, k0, k1);
     const uint64_t c = ((uint64_t)m.x << 32) | m.y;
-    if (c < lo) return true;
-    if (c >= hi) return false;
-    lo = hi = fold_threshold(s.nmass[cur], w, fold_r);
-    return c < lo;
+    if (!f.known) {
+        if (c >= fold_cap) return false;
+        f.known = true;
+        ++lookups;
+        f.w = neighbor_weight(s.hash, hcur, (uint32_t)prev);
+        fold_bracket((uint32_t)(hcur >> 42), f.w, fold_r, f.lo, f.hi);
+    }
+    if (!f.hi) return false;  // tout = 0
+    if (c < f.lo) return true;
+    if (c >= f.hi) return false;
+    f.lo = f.hi = fold_threshold(s.nmass[cur], f.w, fold_r);
+    return c < f.lo;
 }
 
 // Returns the node's integer mass S (all lanes).  All 64 lanes active.  With
@@ -1581,25 +1600,17 @@ __global__ __launch_bounds__(256) void k_walk(DevStore s, WalkArgs a)
                 uint4 r;
                 int32_t nx;
                 if (APP == 1 && j >= 2) {
-                    // folded return region (integer weights only): the return
-                    // arc's weight sum from cur's neighbour table once the
-                    // first draw's slot gave its locator and mass code
+                    // folded return region (integer weights only): see fold_step
                     const bool fold = !FW && a.fold_r;
-                    uint64_t wret = 0, flo = 0, fhi = 0;
+                    FoldState fs = {0, 0, 0, false};
                     for (uint32_t at = 0;; ++at) {
                         r = philox(w0, w1, (uint32_t)j, at, k0, k1);
                         nx = draw<FW>(s, cur, r, hcur, w0, w1, (uint32_t)j, at, k0, k1);
                         ++draws;
-                        if (fold) {
-                            if (at == 0) {
-                                ++folds;
-                                wret = neighbor_weight(s.hash, hcur, (uint32_t)prev);
-                                fold_bracket((uint32_t)(hcur >> 42), wret, a.fold_r, flo, fhi);
-                            }
-                            if (fold_taken(s, cur, wret, a.fold_r, flo, fhi, w0, w1, (uint32_t)j, at, k0, k1)) {
-                                nx = prev;
-                                break;
-                            }
+                        if (fold && fold_step(s, cur, prev, hcur, a.fold_r, a.fold_cap, fs, folds, w0, w1,
+                                              (uint32_t)j, at, k0, k1)) {
+                            nx = prev;
+                            break;
                         }
                         if (nx < 0 || (int64_t)at + 1 >= a.max_attempts) break;
                         const uint32_t aux = r.x & kAuxMask;
@@ -1740,8 +1751,7 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
     int32_t cur[WPL], prev[WPL], jj[WPL], len[WPL];
     uint32_t att[WPL];
     uint64_t hp[WPL];    // neighbour-table locator of each slot's previous node
-    uint64_t flo[WPL], fhi[WPL];  // node2vec: the step's fold bracket (attempt 0 sets it)
-    uint64_t wret[WPL];           // ... and the return arc's weight
+    FoldState fst[WPL];  // node2vec: each slot's fold state for its current step
     int64_t padw[WPL];   // walker whose row tail the wave pads at the end of this iteration (-1: none)
     int32_t padj[WPL];   // ... from this position on
     const bool fold = !FW && APP == 1 && a.fold_r;
@@ -1751,7 +1761,7 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
         cur[k] = prev[k] = jj[k] = len[k] = 0;
         att[k] = 0;
         hp[k] = 0;
-        flo[k] = fhi[k] = wret[k] = 0;
+        fst[k] = FoldState{0, 0, 0, false};
         padw[k] = -1;
         padj[k] = 0;
     }
@@ -1841,13 +1851,9 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
             ++draws;
             bool taken = false;  // the fold coin took the return arc
             if (fold && jj[k] >= 2) {
-                if (att[k] == 0) {
-                    ++folds;
-                    wret[k] = neighbor_weight(s.hash, hcur, (uint32_t)prev[k]);
-                    fold_bracket((uint32_t)(hcur >> 42), wret[k], a.fold_r, flo[k], fhi[k]);
-                }
-                if (fold_taken(s, cur[k], wret[k], a.fold_r, flo[k], fhi[k], w0, w1, (uint32_t)jj[k], att[k], k0,
-                               k1)) {
+                if (att[k] == 0) fst[k].known = false;
+                if (fold_step(s, cur[k], prev[k], hcur, a.fold_r, a.fold_cap, fst[k], folds, w0, w1, (uint32_t)jj[k],
+                              att[k], k0, k1)) {
                     nx = prev[k];
                     taken = true;
                 }
@@ -2009,14 +2015,17 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
     int64_t wid[kServeWPL];
     uint32_t att[kServeWPL], acc[kServeWPL];
     int32_t acc_req[kServeWPL];
-    uint64_t hp[kServeWPL], flo[kServeWPL], fhi[kServeWPL], wret[kServeWPL];
+    uint64_t hp[kServeWPL];
+    FoldState fst[kServeWPL];
+    uint32_t lookups = 0;
 #pragma unroll
     for (int k = 0; k < kServeWPL; ++k) {
         rid[k] = acc_req[k] = -1;
         cur[k] = prev[k] = jj[k] = len[k] = 0;
         wid[k] = 0;
         att[k] = acc[k] = 0;
-        hp[k] = flo[k] = fhi[k] = wret[k] = 0;
+        hp[k] = 0;
+        fst[k] = FoldState{0, 0, 0, false};
     }
     bool more = true;
     for (;;) {
@@ -2100,12 +2109,9 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
             int32_t nx = draw<FW>(s, cur[k], r, hcur, w0, w1, (uint32_t)jj[k], att[k], k0, k1);
             bool taken = false;  // the fold coin took the return arc
             if (fold) {
-                if (att[k] == 0) {
-                    wret[k] = neighbor_weight(s.hash, hcur, (uint32_t)prev[k]);
-                    fold_bracket((uint32_t)(hcur >> 42), wret[k], q.fold_r, flo[k], fhi[k]);
-                }
-                if (fold_taken(s, cur[k], wret[k], q.fold_r, flo[k], fhi[k], w0, w1, (uint32_t)jj[k], att[k], k0,
-                               k1)) {
+                if (att[k] == 0) fst[k].known = false;
+                if (fold_step(s, cur[k], prev[k], hcur, q.fold_r, q.fold_cap, fst[k], lookups, w0, w1, (uint32_t)jj[k],
+                              att[k], k0, k1)) {
                     nx = prev[k];
                     taken = true;
                 }

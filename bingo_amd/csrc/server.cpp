@@ -276,6 +276,7 @@ static bool schedule(bingo_server *sv, Launch &L)
         q.max_len = r->args.max_len;
         q.max_attempts = r->args.max_attempts;
         q.fold_r = r->args.fold_r;
+        q.fold_cap = r->args.fold_cap;
         q.app = r->app;
         L.h_table[L.parts.size()] = q;
         L.parts.push_back(Part{r, first, take, total});

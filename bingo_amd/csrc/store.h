@@ -170,7 +170,8 @@ struct WalkArgs {
     uint32_t thr_return, thr_neighbor, thr_other, thr_stop;
     int32_t max_len;
     int32_t max_attempts;
-    uint64_t fold_r;  // node2vec: folded return region factor x 2^27 (0: plain envelope)
+    uint64_t fold_r;    // node2vec: folded return region factor x 2^27 (0: plain envelope)
+    uint64_t fold_cap;  // an upper bound of every node's fold threshold (return weight <= node weight)
 };
 
 // One admitted walk request of the serving loop.  Its walkers occupy the
@@ -192,7 +193,7 @@ struct ServeReq {
     uint32_t thr_return, thr_neighbor, thr_other, thr_stop;
     int32_t max_len, max_attempts;
     int32_t app, pad;
-    uint64_t fold_r;
+    uint64_t fold_r, fold_cap;
 };
 constexpr int kMaxServeReqs = 128;
 
