@@ -635,7 +635,7 @@ __device__ Growth plan_run(const DevStore &s, const DevEdge *__restrict__ e, uin
 }
 
 // whether the run moves a region of kBigCopyBytes or more (wave-uniform)
-__device__ __forceinline__ bool run_moves_big(const DevStore &s, const NodeHdr &h, const GroupHdr &gh, const Growth &G)
+__device__ __forceinline__ bool run_moves_big(const NodeHdr &h, const GroupHdr &gh, const Growth &G)
 {
     const bool grp = __ballot(G.g_g && (uint64_t)gh.size * 8u >= kBigCopyBytes) != 0ull;
     return grp || (G.g_slots && (uint64_t)h.nslots * 4u >= kBigCopyBytes) ||
@@ -748,7 +748,7 @@ __global__ __launch_bounds__(256) void k_insert_plan(DevStore s, const DevEdge *
         need[POOL_BACK] += G.g_back ? G.cap_back : 0;
         need[POOL_HASH] += G.g_hash ? 2ull * G.cap_hash : 0;  // 16-byte entries
         need[POOL_MEM] += dpp_sum64(G.g_g ? (uint64_t)G.cap_g : 0ull);
-        nbigcopy += run_moves_big(s, h, gh, G) ? 1u : 0u;
+        nbigcopy += run_moves_big(h, gh, G) ? 1u : 0u;
     }
     if (ln == 0) {
         for (int p = 0; p < NPOOL; ++p)
@@ -2017,7 +2017,7 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
     int32_t acc_req[kServeWPL];
     uint64_t hp[kServeWPL];
     FoldState fst[kServeWPL];
-    uint32_t lookups = 0;
+    uint32_t lookups = 0;  // return-weight lookups (not reported by the served form)
 #pragma unroll
     for (int k = 0; k < kServeWPL; ++k) {
         rid[k] = acc_req[k] = -1;
