@@ -653,10 +653,10 @@ __device__ __forceinline__ bool run_too_big(const NodeHdr &h, const Growth &G)
 
 // ------------------------------------------------------------------------
 // Run loops are software-pipelined: while a wave works on one run it already
-// holds the next run's node header, group headers and first 64 arcs (or
-// delete slots), and the run table entry of the one after, so the dependent
-// round trips at the start of a run (table -> node -> records) overlap with
-// the previous run's work.  Runs name distinct nodes, so reading the next
+// holds the next runs' node headers, group headers and first 64 arcs (two
+// runs ahead in the insert kernels, one in the delete kernel), and the run
+// table entry of the one after, so the dependent round trips at the start
+// of a run (table -> node -> records) overlap with the previous runs' work.  Runs name distinct nodes, so reading the next
 // node's records early sees the same values as reading them later.
 // ------------------------------------------------------------------------
 struct RunMeta {
@@ -725,18 +725,21 @@ __global__ __launch_bounds__(256) void k_insert_plan(DevStore s, const DevEdge *
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     unsigned long long need[NPOOL] = {0, 0, 0, 0};
     uint32_t nbigcopy = 0;
-    RunMeta m0 = run_meta(s, r, wave, nruns), m1 = run_meta(s, r, wave + nwaves, nruns);
-    RunPre p0 = run_pre<false>(s, E, nullptr, m0, ln);
+    RunMeta m0 = run_meta(s, r, wave, nruns), m1 = run_meta(s, r, wave + nwaves, nruns),
+            m2 = run_meta(s, r, wave + 2 * nwaves, nruns);
+    RunPre p0 = run_pre<false>(s, E, nullptr, m0, ln), p1 = run_pre<false>(s, E, nullptr, m1, ln);
     for (uint32_t ri = wave; ri < nruns; ri += nwaves) {
-        const RunMeta m2 = run_meta(s, r, ri + 2 * nwaves, nruns);
-        const RunPre p1 = run_pre<false>(s, E, nullptr, m1, ln);
+        const RunMeta m3 = run_meta(s, r, ri + 3 * nwaves, nruns);
+        const RunPre p2 = run_pre<false>(s, E, nullptr, m2, ln);
         const RunMeta m = m0;
         const NodeHdr h = p0.h;
         const GroupHdr gh = p0.gh;
         const uint32_t w0 = p0.x.w;
         m0 = m1;
         m1 = m2;
+        m2 = m3;
         p0 = p1;
+        p1 = p2;
         if (!m.u) continue;  // invalid arcs were flagged already
         const uint32_t cnt = m.cnt;
         const Growth G = plan_run(s, E + m.start, cnt, h, gh, w0, ln);
@@ -990,15 +993,18 @@ __global__ __launch_bounds__(256) void k_insert_apply(DevStore s, const DevEdge 
     unsigned long long cur[NPOOL];
 #pragma unroll
     for (int p = 0; p < NPOOL; ++p) cur[p] = s.ctl->top[p] + wres[(size_t)wave * NPOOL + p];
-    RunMeta m0 = run_meta(s, r, wave, nruns), m1 = run_meta(s, r, wave + nwaves, nruns);
-    RunPre p0 = run_pre<false>(s, E, FR, m0, ln);
+    RunMeta m0 = run_meta(s, r, wave, nruns), m1 = run_meta(s, r, wave + nwaves, nruns),
+            m2 = run_meta(s, r, wave + 2 * nwaves, nruns);
+    RunPre p0 = run_pre<false>(s, E, FR, m0, ln), p1 = run_pre<false>(s, E, FR, m1, ln);
     for (uint32_t ri = wave; ri < nruns; ri += nwaves) {
-        const RunMeta m2 = run_meta(s, r, ri + 2 * nwaves, nruns);
-        const RunPre p1 = run_pre<false>(s, E, FR, m1, ln);
+        const RunMeta m3 = run_meta(s, r, ri + 3 * nwaves, nruns);
+        const RunPre p2 = run_pre<false>(s, E, FR, m2, ln);
         if (m0.u) insert_run(s, E + m0.start, FR ? FR + m0.start : nullptr, m0.cnt, m0.u, p0, cur, ln);
         m0 = m1;
         m1 = m2;
+        m2 = m3;
         p0 = p1;
+        p1 = p2;
     }
 }
 
@@ -1441,6 +1447,8 @@ __global__ __launch_bounds__(256) void k_delete_apply(DevStore s, const uint32_t
         if (m.cnt > kBigDelete) m.u = 0;
         return m;
     };
+    // one run ahead only: a second stage's registers would cost this
+    // kernel a wave per SIMD of occupancy (measured slower)
     RunMeta m0 = meta(task), m1 = meta(task + nwaves);
     RunPre p0 = run_pre<true>(s, nullptr, slots, m0, ln);
     for (; task - ngt < nruns; task += nwaves) {
