@@ -397,6 +397,42 @@ def test_walk_graph_replays_match_eager(orc, form):
     wgr.close()
 
 
+@pytest.mark.parametrize("P,Q", [(0.5, 2.0), (0.01, 1.0), (0.9, 1.0), (0.1, 0.25), (0.25, 4.0), (2.0, 0.5),
+                                 (1e-12, 1.0)])
+def test_node2vec_parameters_every_form(orc, P, Q):
+    """node2vec across the return/in-out parameters -- folded return regions
+    large (p = 0.01), small (p = 0.9) and beside an envelope above 1 (q < 1),
+    the plain envelope (p >= q) and a fold factor too large to fold
+    (p = 1e-12) -- on a graph of multi-arcs and symmetric pairs (returns are
+    common and carry several arcs' weight), in the plain, persistent and
+    served forms, against the oracle."""
+    rng = random.Random(int(P * 1000) + int(Q * 10))
+    n = 500
+    e = _rand_edges(rng, n, 6000, hub=3)
+    back = e.copy()
+    back["u"], back["v"] = e["v"], e["u"]
+    back["w"] = [max(1, rng.getrandbits(rng.randint(1, 6))) for _ in range(len(e))]
+    e = np.concatenate([e, back, e[:500]])  # reverse arcs and repeated arcs
+    g, o = _pair(orc, n, e)
+    starts = np.array([rng.randint(1, n) for _ in range(4000)], np.int32)
+    op = orc.walk_params(seed=21, p=P, q=Q, max_len=32)
+    want, steps = o.walk(B.NODE2VEC, op, starts=starts)
+    for form, knobs in ((B.FORM_PLAIN, {}), (B.FORM_PERSISTENT, dict(block=128)),
+                        (B.FORM_PERSISTENT, dict(block=256, walkers_per_lane=4, lds_depth=16))):
+        gp = B.walk_params(seed=21, p=P, q=Q, max_len=32, form=form)
+        for k, v in knobs.items():
+            setattr(gp, k, v)
+        pa, sa = g.walk(B.NODE2VEC, gp, starts=starts)
+        assert np.array_equal(pa, want) and sa == steps, (P, Q, form, knobs)
+    srv = B.Server(g, arena_bytes=1 << 24)
+    t = srv.submit_walk(B.NODE2VEC, B.walk_params(seed=21, p=P, q=Q, max_len=32), starts=starts)
+    out, st, _ = srv.wait(t)
+    srv.close()
+    assert np.array_equal(out, want) and st == steps, (P, Q, "served")
+    g.close()
+    o.close()
+
+
 def test_gather_probe_reports_rates():
     g = B.Graph(neighbor_index=False)
     g.load(1 << 12, B.gen_rmat(12, 50_000, B.W_LOGUNI32, seed=1))
