@@ -439,7 +439,11 @@ def main():
     for width in (32, 8, 32, 8):  # second pass counts
         nr, ms = g.gather_probe(width)
         rates[width] = nr / (ms / 1e3)
-    ceiling_s = attempts / K * (1 / rates[32] + 1 / rates[8]) + (probes + folds) / K / rates[8]
+    # a node2vec or float-weight draw reads the whole 32-byte slot (two
+    # 16-byte loads); other draws read its first 16 bytes (one load, priced
+    # as an 8-byte read) and rarely the second locator
+    slot_rate = rates[32] if (app == 1 or cfg.get("float")) else rates[8]
+    ceiling_s = attempts / K * (1 / slot_rate + 1 / rates[8]) + (probes + folds) / K / rates[8]
     log(f"random reads: {rates[32] / 1e9:.1f} G/s (32 B), {rates[8] / 1e9:.1f} G/s (8 B); "
         f"walk at {ceiling_s / (kern_ms / K / 1e3):.2f} of that ceiling")
 

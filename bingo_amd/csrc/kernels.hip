@@ -465,8 +465,10 @@ __device__ uint64_t alias_rebuild(const DevStore &s, int32_t u, uint32_t size, u
     if (own) {
         AliasEnt ae;
         ae.T = T;
-        ae.stay = off | ((uint64_t)size << kOffBits);
-        ae.alias = off_a | ((uint64_t)size_a << kOffBits);
+        const uint64_t stay = off | ((uint64_t)size << kOffBits), alias = off_a | ((uint64_t)size_a << kOffBits);
+        const bool stay_first = (T >> 63) != 0;  // the likelier outcome first
+        ae.first = stay_first ? stay : alias;
+        ae.second = stay_first ? alias : stay;
         ae.meta = (uint64_t)ali | (hmeta << 5) | (thr_hi << 47);
         s.alias[(size_t)u * kGroups + ln] = ae;
     }
@@ -507,8 +509,8 @@ __global__ void k_init_nodes(DevStore s)
     const int j = (int)(i & (kGroups - 1));
     AliasEnt e;
     e.T = ~0ull;
-    e.stay = 0;
-    e.alias = 0;
+    e.first = 0;
+    e.second = 0;
     e.meta = (uint64_t)j;
     s.alias[i] = e;
     GroupHdr g = {0, 0, 0};
@@ -1514,7 +1516,8 @@ __device__ __forceinline__ int32_t slot_member(const uint64_t *__restrict__ mem,
 {
     const uint64_t T = ((uint64_t)q0.y << 32) | q0.x;
     const uint64_t coin = ((uint64_t)r.y << 32) | r.z;
-    const uint64_t loc = coin < T ? (((uint64_t)q0.w << 32) | q0.z) : (((uint64_t)q1.y << 32) | q1.x);
+    const bool first = ((T >> 63) != 0) == (coin < T);  // the draw's outcome is the slot's first locator
+    const uint64_t loc = first ? (((uint64_t)q0.w << 32) | q0.z) : (((uint64_t)q1.y << 32) | q1.x);
     hm = (((uint64_t)q1.w << 32) | q1.z) >> 5;  // locator | the 17-bit field << 42
     const uint32_t sz = (uint32_t)(loc >> kOffBits);
     if (sz == 0u) return -1;
@@ -1527,6 +1530,31 @@ __device__ __forceinline__ int32_t sample_step(const AliasEnt *__restrict__ alia
     uint4 q0, q1;
     alias_slot(alias, cur, r, q0, q1);
     return slot_member(mem, r, q0, q1, hm);
+}
+
+// One draw on `cur` that needs no neighbour-table locator (integer weights,
+// apps other than node2vec): the slot's first 16 bytes (T and the likelier
+// outcome's locator) in one load, the second locator only when the less
+// likely outcome comes up.  Random gathers here are bound by the lane
+// addresses the memory pipeline handles, so one 16-byte load instead of two
+// per draw is the saving.
+__device__ __forceinline__ int32_t sample_lite(const AliasEnt *__restrict__ alias, const uint64_t *__restrict__ mem,
+                                               int32_t cur, uint4 r)
+{
+    const u32x4 *p = reinterpret_cast<const u32x4 *>(alias + (size_t)cur * kGroups + (r.x >> 27));
+    u32x4 a = p[0];
+    asm volatile("" : "+v"(a));
+    const uint64_t T = ((uint64_t)a.y << 32) | a.x;
+    const uint64_t coin = ((uint64_t)r.y << 32) | r.z;
+    uint64_t loc = ((uint64_t)a.w << 32) | a.z;
+    if (((T >> 63) != 0) != (coin < T)) {
+        const uint2 b = *reinterpret_cast<const uint2 *>(p + 1);
+        loc = ((uint64_t)b.y << 32) | b.x;
+    }
+    const uint32_t sz = (uint32_t)(loc >> kOffBits);
+    if (sz == 0u) return -1;
+    const uint32_t pos = __umulhi(r.w, sz);
+    return (int32_t)reinterpret_cast<const uint32_t *>(mem + (loc & kOffMask36) + pos)[1];
 }
 
 // One draw on `cur` with float weights (NodeData::sample's wtype::Float
@@ -1634,7 +1662,10 @@ __global__ __launch_bounds__(256) void k_walk(DevStore s, WalkArgs a)
                     }
                 } else {
                     r = philox(w0, w1, (uint32_t)j, 0u, k0, k1);
-                    nx = draw<FW>(s, cur, r, hcur, w0, w1, (uint32_t)j, 0u, k0, k1);
+                    if (!FW && APP != 1)
+                        nx = sample_lite(s.alias, s.mem, cur, r);
+                    else
+                        nx = draw<FW>(s, cur, r, hcur, w0, w1, (uint32_t)j, 0u, k0, k1);
                     ++draws;
                 }
                 if (nx < 0) break;
@@ -1854,8 +1885,9 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
             const int64_t w = wid[k];
             const uint32_t w0 = (uint32_t)w, w1 = (uint32_t)((uint64_t)w >> 32);
             const uint4 r = philox(w0, w1, (uint32_t)jj[k], att[k], k0, k1);
-            uint64_t hcur;
-            int32_t nx = draw<FW>(s, cur[k], r, hcur, w0, w1, (uint32_t)jj[k], att[k], k0, k1);
+            uint64_t hcur = 0;
+            int32_t nx = (!FW && APP != 1) ? sample_lite(s.alias, s.mem, cur[k], r)
+                                            : draw<FW>(s, cur[k], r, hcur, w0, w1, (uint32_t)jj[k], att[k], k0, k1);
             ++draws;
             bool taken = false;  // the fold coin took the return arc
             if (fold && jj[k] >= 2) {
@@ -2113,8 +2145,9 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
             const uint32_t k0 = (uint32_t)q.seed, k1 = (uint32_t)(q.seed >> 32);
             const uint4 r = philox(w0, w1, (uint32_t)jj[k], att[k], k0, k1);
             const bool fold = !FW && q.app == 1 && q.fold_r && jj[k] >= 2;
-            uint64_t hcur;
-            int32_t nx = draw<FW>(s, cur[k], r, hcur, w0, w1, (uint32_t)jj[k], att[k], k0, k1);
+            uint64_t hcur = 0;
+            int32_t nx = (!FW && q.app != 1) ? sample_lite(s.alias, s.mem, cur[k], r)
+                                              : draw<FW>(s, cur[k], r, hcur, w0, w1, (uint32_t)jj[k], att[k], k0, k1);
             bool taken = false;  // the fold coin took the return arc
             if (fold) {
                 if (att[k] == 0) fst[k].known = false;

@@ -63,9 +63,12 @@ constexpr uint32_t kThrHiMax = (1u << 17) - 1;  // the alias slot's 17-bit copy 
 constexpr uint64_t kHmetaMask = (1ull << 42) - 1;
 
 // One 32-byte alias slot i of a node:
-//   T     stay iff coin64 < T
-//   stay  locator of group i:        member offset (36 bits) | size << 36
-//   alias locator of the partner:    member offset (36 bits) | size << 36
+//   T      stay (group i) iff coin64 < T, else the partner
+//   first  locator of the likelier outcome: group i if T >= 2^63, else the
+//          partner (member offset (36 bits) | size << 36)
+//   second locator of the other outcome
+// (the first 16 bytes decide most draws alone: a draw reads the second
+// locator only when the less likely outcome comes up)
 //   meta  partner index (5 bits) | the node's neighbour-table locator << 5
 //         (42 bits: (log2 cap + 1) | table offset << 6, 0: no table)
 //         | << 47, 17 bits: with float weights, the node's fractional-part
@@ -79,8 +82,8 @@ constexpr uint64_t kHmetaMask = (1ull << 42) - 1;
 // single read.
 struct AliasEnt {
     uint64_t T;
-    uint64_t stay;
-    uint64_t alias;
+    uint64_t first;
+    uint64_t second;
     uint64_t meta;
 };
 constexpr int kOffBits = 36;
