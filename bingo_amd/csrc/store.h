@@ -103,7 +103,7 @@ struct Ctl {
     unsigned long long steps;        // last walk: appended vertices
     unsigned long long attempts;     // last walk: sample draws (node2vec rejections included)
     unsigned long long probes;       // last walk: neighbour-table membership lookups
-    unsigned long long folds;        // last walk: node2vec return-region lookups (mass + entry)
+    unsigned long long folds;        // last walk: node2vec return-weight lookups (neighbour entries)
     unsigned long long cursor;       // persistent walk: next unclaimed walker
     unsigned int nbigcopy;           // insert plan: runs that move a region of kBigCopyBytes or more
     unsigned int pad;

@@ -90,8 +90,9 @@ typedef struct {
     uint64_t walk_steps;    /* appended vertices of the last walk */
     uint64_t walk_attempts; /* sample draws of the last walk (node2vec rejections included) */
     uint64_t walk_probes;   /* neighbour-table membership lookups of the last walk (node2vec) */
-    uint64_t walk_folds;    /* node2vec steps that read the node's mass and the return arc's
-                               weight sum (folded return region) */
+    uint64_t walk_folds;    /* node2vec steps that looked up the return arc's weight sum
+                               (folded return region; only steps with a fold coin below
+                               the bound every threshold obeys) */
     uint64_t arcs;          /* live arcs */
     uint64_t bytes_reserved;/* HBM held by the store */
     uint64_t pool_used[4];  /* slot, back-index, member, neighbour pools (elements) */
