@@ -62,7 +62,7 @@ CONFIGS = {
     # at load; sliding-window churn: 10^7 new arcs in, the 10^7 oldest out
     "c5": dict(workload="rmat25_a30_loguni32_deepwalk_w32e6_churn1e7", scale=25, edges=1 << 30, wkind=2, app=0,
                max_len=80, batch=10_000_000, p=0.5, q=2.0, tp=0.0125, walkers=32_000_000, lens=None, graph=False,
-               directed=True, fit=True, churn="expire", cpu_sample_edges=1 << 26),
+               directed=True, fit=True, churn="expire", cpu_sample_edges=1 << 26, grow_in_place=True),
 }
 
 
@@ -306,6 +306,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--app", choices=APP_NAMES, default=None, help="walk app instead of the configuration's")
     ap.add_argument("--float-weights", action="store_true", help="float32 weights w / 256 (wtype::Float)")
+    ap.add_argument("--grow-in-place", type=int, choices=(0, 1), default=None,
+                    help="pools grow by mapping more HBM (1) or by copying (0); default: the config's choice")
     ap.add_argument("--schedule", default="", metavar="BLOCK,WPL,DEPTH",
                     help="fixed persistent walk knobs instead of autotuning (e.g. 512,2,16)")
     ap.add_argument("--seed", type=int, default=2026)
@@ -342,7 +344,9 @@ def main():
     base = B.gen_rmat(scale, m, cfg["wkind"], seed=args.seed, undirected=not cfg.get("directed"))
     log(f"generated {len(base)} arcs in {time.perf_counter() - t0:.1f}s")
     fit = dict(fit_load=True, fit_headroom=1.0 / 16, pool_slack=0.03) if cfg.get("fit") else {}
-    g = B.Graph(device=0, neighbor_index=app == B.NODE2VEC, float_weights=bool(cfg.get("float")), **fit)
+    gip = cfg.get("grow_in_place") if args.grow_in_place is None else bool(args.grow_in_place)
+    g = B.Graph(device=0, neighbor_index=app == B.NODE2VEC, float_weights=bool(cfg.get("float")),
+                grow_in_place=gip, **fit)
     d_base = B.DeviceArray(g, to_float_bits(base) if cfg.get("float") else base)
     g.load_device(n, d_base, len(base))
     g.sync()
@@ -472,6 +476,7 @@ def main():
                    "hip_graph": cfg["graph"], "directed": bool(cfg.get("directed")),
                    "deletes": "oldest arcs (sliding window)" if cfg.get("churn") == "expire" else "uniform live arcs",
                    "store": "fitted at load" if cfg.get("fit") else "power-of-two regions",
+                   "pools": "grown in place" if gip else "grown by copy",
                    "weights": "float32 w/256" if cfg.get("float") else "u32"},
         "walk_steps_per_s": walk_rate,
         "updates_per_s": nupd / ((ins_ms + del_ms) / 1e3),

@@ -47,6 +47,7 @@ class Config(C.Structure):
         ("fit_load", C.c_int32),
         ("fit_headroom", C.c_double),
         ("float_weights", C.c_int32),
+        ("grow_in_place", C.c_int32),
     ]
 
 
@@ -429,12 +430,14 @@ class Graph:
     (the reference's wtype::Float), see float_bits()."""
 
     def __init__(self, device=0, neighbor_index=True, hbm_budget=0, pool_slack=0.25, min_slots=16, fit_load=False,
-                 fit_headroom=1.0 / 16, float_weights=False):
+                 fit_headroom=1.0 / 16, float_weights=False, grow_in_place=None):
         cfg = Config()
         lib().bingo_config_default(C.byref(cfg))
         cfg.device, cfg.neighbor_index, cfg.hbm_budget = device, int(neighbor_index), hbm_budget
         cfg.pool_slack, cfg.min_slots = pool_slack, min_slots
         cfg.fit_load, cfg.fit_headroom = int(fit_load), fit_headroom
+        if grow_in_place is not None:
+            cfg.grow_in_place = int(grow_in_place)
         cfg.float_weights = int(float_weights)
         h = C.c_void_p()
         _check(lib().bingo_create(C.byref(cfg), C.byref(h)))

@@ -107,9 +107,89 @@ static int read_ctl(bingo_graph *g)
     return BINGO_OK;
 }
 
-// Grow pools so that top + need fits; a grown pool is a new allocation with
-// the old contents copied (region offsets stay valid).  Refuses growth past
-// the HBM budget with BINGO_ENOMEM before anything is modified.
+// grow_in_place: pool p is a reserved address range (the HBM budget, at
+// most 2^36 elements) mapped from its start in 1 GiB chunks; growing maps
+// more chunks of physical HBM behind the mapped part.  Returns BINGO_OK, a negative error,
+// or 1 when the device refuses the reservation (the caller copies instead).
+static int grow_mapped(bingo_graph *g, int p, uint64_t cap_elems)
+{
+    auto &v = g->vp[p];
+    hipMemAllocationProp prop{};
+    prop.type = hipMemAllocationTypePinned;
+    prop.location.type = hipMemLocationTypeDevice;
+    prop.location.id = g->device;
+    size_t gran = 0;
+    if (hipMemGetAllocationGranularity(&gran, &prop, hipMemAllocationGranularityRecommended) != hipSuccess || !gran) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    // the range and every chunk in whole 1 GiB units at 1 GiB offsets (chunks
+    // of other sizes after the first were refused on this part; the reported
+    // granularity is a 4 KiB page)
+    gran = std::max<size_t>(gran, 1ull << 30);
+    auto up = [gran](uint64_t x) { return (x + gran - 1) / gran * gran; };
+    if (!v.va) {
+        const uint64_t limit = std::min<uint64_t>(g->budget, (1ull << kOffBits) * kElem[p]);
+        if (hipMemAddressReserve(&v.va, up(limit), gran, nullptr, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            v.va = nullptr;
+            return 1;
+        }
+        v.reserved = up(limit);
+    }
+    const uint64_t want = up(cap_elems * kElem[p]);
+    if (want > v.reserved) return fail(BINGO_ENOMEM, "pool would exceed its reserved address range");
+    while (want > v.mapped) {
+        const size_t delta = gran;
+        char *at = static_cast<char *>(v.va) + v.mapped;
+        hipMemGenericAllocationHandle_t h{};
+        if (hipMemCreate(&h, delta, &prop, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(BINGO_ENOMEM, "pool growth of " + std::to_string(delta) + " bytes failed");
+        }
+        hipMemAccessDesc acc{};
+        acc.location = prop.location;
+        acc.flags = hipMemAccessFlagsProtReadWrite;
+        if (hipMemMap(at, delta, 0, h, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            (void)hipMemRelease(h);
+            return fail(BINGO_EDEVICE, "pool mapping failed");
+        }
+        const hipError_t ea = hipMemSetAccess(at, delta, &acc, 1);
+        if (ea != hipSuccess) {
+            (void)hipGetLastError();
+            (void)hipMemUnmap(at, delta);
+            (void)hipMemRelease(h);
+            return fail(BINGO_EDEVICE, std::string("pool access setup failed (") + hipGetErrorString(ea) + ", pool " +
+                                           std::to_string(p) + ", mapped " + std::to_string(v.mapped) + " + " +
+                                           std::to_string(delta) + " bytes, " + std::to_string(v.chunks.size()) +
+                                           " chunks)");
+        }
+        v.chunks.emplace_back(h, delta);
+        v.mapped += delta;
+    }
+    return BINGO_OK;
+}
+
+static void release_mapped(bingo_graph *g)
+{
+    for (auto &v : g->vp) {
+        if (!v.va) continue;
+        size_t at = 0;
+        for (auto &c : v.chunks) {
+            (void)hipMemUnmap(static_cast<char *>(v.va) + at, c.second);
+            (void)hipMemRelease(c.first);
+            at += c.second;
+        }
+        (void)hipMemAddressFree(v.va, v.reserved);
+        v = bingo_graph::VPool{};
+    }
+}
+
+// Grow pools so that top + need fits; a grown pool is either mapped further
+// in place (grow_in_place) or a new allocation with the old contents copied
+// (region offsets stay valid either way).  Refuses growth past the HBM
+// budget with BINGO_ENOMEM before anything is modified.
 static int ensure_pools(bingo_graph *g)
 {
     const Ctl &c = *g->ctl_host;
@@ -119,8 +199,11 @@ static int ensure_pools(bingo_graph *g)
         const uint64_t want = c.top[p] + c.need[p];
         newcap[p] = g->s.cap[p];
         if (want > g->s.cap[p]) {
+            // a copied pool grows by at least half (amortised copies); a
+            // mapped one by at least an eighth (fewer mapping calls)
             const uint64_t slack = (uint64_t)((double)want * g->cfg.pool_slack);
-            newcap[p] = std::max<uint64_t>(want + slack + 1024, g->s.cap[p] + g->s.cap[p] / 2);
+            const uint64_t step = g->vp[p].va ? g->s.cap[p] / 8 : g->s.cap[p] / 2;
+            newcap[p] = std::max<uint64_t>(want + slack + 1024, g->s.cap[p] + step);
         }
         total += newcap[p] * kElem[p];
     }
@@ -141,15 +224,28 @@ static int ensure_pools(bingo_graph *g)
     }
     for (int p = 0; p < NPOOL; ++p) {
         if (newcap[p] == g->s.cap[p]) continue;
+        void **slotp = p == POOL_SLOT ? (void **)&g->s.slot
+                       : p == POOL_BACK ? (void **)&g->s.back
+                       : p == POOL_MEM  ? (void **)&g->s.mem
+                                        : (void **)&g->s.hash;
+        // in place: a pool that is mapped already, or a first allocation
+        if (g->cfg.grow_in_place && !g->vmm_refused && (g->vp[p].va || !*slotp)) {
+            const int rc = grow_mapped(g, p, newcap[p]);
+            if (rc < 0) return rc;
+            if (rc == 0) {
+                const bool first = !*slotp;
+                *slotp = g->vp[p].va;
+                g->s.cap[p] = std::min<uint64_t>(g->vp[p].mapped / kElem[p], 1ull << kOffBits);
+                if (first) ++g->store_gen;
+                continue;
+            }
+            g->vmm_refused = true;  // refused: this and later pools are copied
+        }
         void *fresh = nullptr;
         if (hipMalloc(&fresh, newcap[p] * kElem[p]) != hipSuccess) {
             (void)hipGetLastError();
             return fail(BINGO_ENOMEM, "pool allocation of " + std::to_string(newcap[p] * kElem[p]) + " bytes failed");
         }
-        void **slotp = p == POOL_SLOT ? (void **)&g->s.slot
-                       : p == POOL_BACK ? (void **)&g->s.back
-                       : p == POOL_MEM  ? (void **)&g->s.mem
-                                        : (void **)&g->s.hash;
         if (*slotp) {
             if (c.top[p])
                 HIPCHK(hipMemcpyAsync(fresh, *slotp, c.top[p] * kElem[p], hipMemcpyDeviceToDevice, g->st));
@@ -410,6 +506,7 @@ void bingo_config_default(bingo_config *cfg)
     cfg->min_slots = 16;  // ref ORIGINAL (include/api.h:13)
     cfg->fit_load = 0;
     cfg->fit_headroom = 1.0 / 16;
+    cfg->grow_in_place = 0;
     cfg->float_weights = 0;
 }
 
@@ -490,10 +587,13 @@ void bingo_destroy(bingo_graph *g)
 {
     if (!g) return;
     if (g->st) (void)hipStreamSynchronize(g->st);
-    void *bufs[] = {g->s.hdr, g->s.grp, g->s.alias, g->s.fnode, g->s.nmass, g->s.slot, g->s.back, g->s.mem,
-                    g->s.hash, g->s.ctl, g->s.tasks, g->d_steps};
+    void *bufs[] = {g->s.hdr, g->s.grp, g->s.alias, g->s.fnode, g->s.nmass, g->s.ctl, g->s.tasks, g->d_steps};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    void *pools[NPOOL] = {g->s.slot, g->s.back, g->s.mem, g->s.hash};
+    for (int p = 0; p < NPOOL; ++p)
+        if (pools[p] && pools[p] != g->vp[p].va) (void)hipFree(pools[p]);
+    release_mapped(g);
     for (int i = 0; i < NSCRATCH; ++i)
         if (g->sc[i]) (void)hipFree(g->sc[i]);
     for (int i = 0; i < kEvents; ++i)

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "../../include/bingo.h"
 #include "store.h"
@@ -76,6 +78,13 @@ struct bingo_graph {
     unsigned long long *d_steps = nullptr;
     int num_cu = 0;
     uint64_t store_gen = 0;  // bumped whenever a pool moves
+    // grow_in_place pools: a reserved address range per pool, mapped chunk by chunk
+    struct VPool {
+        void *va = nullptr;
+        size_t reserved = 0, mapped = 0;
+        std::vector<std::pair<hipMemGenericAllocationHandle_t, size_t>> chunks;
+    } vp[bingo::NPOOL];
+    bool vmm_refused = false;  // the device refused a reservation: pools are copied instead
     struct Tuned {
         bool valid = false;
         int32_t max_len = 0;

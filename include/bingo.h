@@ -62,6 +62,11 @@ typedef struct {
     double fit_headroom;    /* fit_load's spare room per region, as a fraction (1/16) */
     int32_t float_weights;  /* 1: edge weights are float bits (ref wtype::Float, Bingo.cuh:65-69,
                                254-264); integer part through the bit groups, fraction per slot */
+    int32_t grow_in_place;  /* 1: each pool is a reserved address range that grows by mapping
+                               more HBM in 1 GiB chunks (no copy, no second copy in HBM while
+                               growing, pool addresses stable; at least 1 GiB per pool, so for
+                               large stores); 0: a growing pool is copied to a larger
+                               allocation.  Falls back to copying where mapping is refused. */
 } bingo_config;
 
 typedef struct {

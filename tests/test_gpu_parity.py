@@ -96,6 +96,48 @@ def test_update_rounds_match_oracle(orc, nbr, fit):
     check_against_brute(g.export(), bg)
 
 
+def test_pools_grown_in_place_match_oracle(orc):
+    """Pools that grow by mapping more HBM behind a reserved address range
+    (grow_in_place, no pool slack: every batch grows some pool) keep the
+    store equal to the oracle's, and a captured walk graph replays across
+    the growth without being captured again (the pool addresses stay)."""
+    rng = random.Random(12)
+    n = 800
+    e = _rand_edges(rng, n, 3000, hub=11)
+    print(f"[parity] load n={n} m={len(e)} grow_in_place", flush=True)
+    g = B.Graph(neighbor_index=True, pool_slack=0.0, grow_in_place=True)
+    g.load(n, e)
+    o = orc.OracleGraph(n, True)
+    assert o.load(e) == 0
+    _same_state(g, o, "load")
+    nw, L = 1500, 16
+    dp = B.DeviceArray(g, nbytes=nw * L * 4)
+    ds = B.DeviceArray(g, np.zeros(1, np.uint64))
+    wg = g.walk_graph(B.NODE2VEC, B.walk_params(seed=8, max_len=L), nw, dp, ds)
+    for rnd in range(6):
+        ins = _rand_edges(rng, n, 2000 * (rnd + 1), hub=11)
+        g.insert(ins)
+        assert o.insert(ins) == 0
+        d = o.export()
+        so = np.concatenate([[0], np.cumsum(d["nslots"].astype(np.int64))])
+        dels = []
+        for u in rng.sample(range(1, n + 1), 60):
+            live = np.flatnonzero(d["slot_w"][so[u]:so[u] + int(d["nslots"][u])] != 0)
+            dels += [(u, int(x)) for x in live[: rng.randint(0, 3)]]
+        dels = np.array(dels, B.DEL_DT)
+        g.delete(dels)
+        assert o.delete(dels) == 0
+        _same_state(g, o, f"round {rnd}")
+        wg.launch()
+        g.sync()
+        want, _ = o.walk(B.NODE2VEC, orc.walk_params(seed=8, max_len=L), nwalkers=nw)
+        assert np.array_equal(dp.to_host(np.int32, (nw, L)), want), rnd
+    wg.close()
+    for a in (dp, ds):
+        a.free()
+    g.close()
+
+
 def test_fitted_store_under_sliding_window(orc):
     """A store fitted to the graph at load (regions sized to their contents
     plus 1/16) holds the same state as the oracle, is smaller than the
