@@ -121,7 +121,9 @@ static int grow_mapped(bingo_graph *g, int p, uint64_t cap_elems)
     size_t gran = 0;
     if (hipMemGetAllocationGranularity(&gran, &prop, hipMemAllocationGranularityRecommended) != hipSuccess || !gran) {
         (void)hipGetLastError();
-        return 1;
+        // refused before anything was mapped: the caller copies instead; a
+        // mapped pool cannot move to a copied allocation
+        return v.va ? fail(BINGO_EDEVICE, "pool mapping granularity query failed") : 1;
     }
     // the range and every chunk in whole 1 GiB units at 1 GiB offsets (chunks
     // of other sizes after the first were refused on this part; the reported
