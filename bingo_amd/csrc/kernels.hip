@@ -1992,11 +1992,15 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
 // [base_r, base_r + nwalkers_r)); lanes claim indices with the same
 // wave-aggregated refill as k_walk_persist, map them to (request, local
 // walker) by binary search over the request table held in LDS, and run the
-// request's own app, parameters and seed.  Requests of different apps and
-// sizes therefore share every wave, and a request's paths do not depend on
-// what it was batched with.
+// request's own app, parameters and seed.  Requests of different apps,
+// sizes, seeds and parameters share every wave, and a request's paths do
+// not depend on what it was batched with.  CLS is the launch's class:
+// kServeOther (DeepWalk, PPR, sampling only) and kServeN2V (node2vec only)
+// keep only their own state in registers; kServeMixed runs any app.
 // ------------------------------------------------------------------------
-constexpr int kServeWPL = 2;
+constexpr int kServeOther = 0, kServeN2V = 1, kServeMixed = 2;
+// walker slots per lane: one where node2vec state is carried only
+__host__ __device__ constexpr int serve_wpl(int cls) { return cls == kServeN2V ? 1 : 2; }
 constexpr int kServeD = 8;  // path positions staged per slot in LDS
 
 // A served walker's row: staged kServeD positions at a time in LDS and
@@ -2034,32 +2038,34 @@ struct ServeRow {
     }
 };
 
-template <bool FW>
+template <bool FW, int CLS>
 __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *__restrict__ reqs, int nreq,
                                                     unsigned long long total, unsigned long long *cursor)
 {
+    constexpr int WPL = serve_wpl(CLS);
+    constexpr bool MIX = CLS == kServeMixed;
     __shared__ ServeReq rq[kMaxServeReqs];
-    __shared__ int32_t stage[256 * kServeWPL * (kServeD + 1)];  // odd pitch: no bank conflicts
+    __shared__ int32_t stage[256 * WPL * (kServeD + 1)];  // odd pitch: no bank conflicts
     for (int i = threadIdx.x; i < nreq; i += blockDim.x) rq[i] = reqs[i];
     __syncthreads();
     const int ln = lane_id();
     const uint64_t lt = (1ull << ln) - 1ull;
-    ServeRow out[kServeWPL];
-    bool stg[kServeWPL];
+    ServeRow out[WPL];
+    bool stg[WPL];
 #pragma unroll
-    for (int k = 0; k < kServeWPL; ++k) {
-        out[k].st = stage + ((size_t)threadIdx.x * kServeWPL + k) * (kServeD + 1);
+    for (int k = 0; k < WPL; ++k) {
+        out[k].st = stage + ((size_t)threadIdx.x * WPL + k) * (kServeD + 1);
         stg[k] = false;
     }
-    int32_t rid[kServeWPL], cur[kServeWPL], prev[kServeWPL], jj[kServeWPL], len[kServeWPL];
-    int64_t wid[kServeWPL];
-    uint32_t att[kServeWPL], acc[kServeWPL];
-    int32_t acc_req[kServeWPL];
-    uint64_t hp[kServeWPL];
-    FoldState fst[kServeWPL];
+    int32_t rid[WPL], cur[WPL], prev[WPL], jj[WPL], len[WPL];
+    int64_t wid[WPL];
+    uint32_t att[WPL], acc[WPL];
+    int32_t acc_req[WPL];
+    uint64_t hp[WPL];
+    FoldState fst[WPL];
     uint32_t lookups = 0;  // return-weight lookups (not reported by the served form)
 #pragma unroll
-    for (int k = 0; k < kServeWPL; ++k) {
+    for (int k = 0; k < WPL; ++k) {
         rid[k] = acc_req[k] = -1;
         cur[k] = prev[k] = jj[k] = len[k] = 0;
         wid[k] = 0;
@@ -2070,10 +2076,10 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
     bool more = true;
     for (;;) {
         if (more) {
-            uint64_t emp[kServeWPL];
+            uint64_t emp[WPL];
             uint32_t need = 0;
 #pragma unroll
-            for (int k = 0; k < kServeWPL; ++k) {
+            for (int k = 0; k < WPL; ++k) {
                 emp[k] = __ballot(rid[k] < 0);
                 need += (uint32_t)__popcll(emp[k]);
             }
@@ -2083,7 +2089,7 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
                 base = __shfl(base, 0, 64);
                 uint32_t off = 0;
 #pragma unroll
-                for (int k = 0; k < kServeWPL; ++k) {
+                for (int k = 0; k < WPL; ++k) {
                     if (rid[k] < 0) {
                         const unsigned long long c = base + off + (uint32_t)__popcll(emp[k] & lt);
                         if (c < total) {
@@ -2103,7 +2109,7 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
                             int32_t L = q.lens ? q.lens[w] : q.max_len;
                             if (q.app == 3 && L > 2) L = 2;
                             int32_t *row = q.paths + w * (int64_t)q.max_len;
-                            if (st < 1 || st > s.n || L < 1 || L > q.max_len) {
+                            if (st < 1 || st > s.n || L < 1 || L > q.max_len || (!MIX && (q.app == 1) != (CLS == kServeN2V))) {
                                 atomicOr(&s.ctl->error, (unsigned)ERR_INVAL);
                                 for (int32_t t = 0; t < q.max_len; ++t) row[t] = -1;
                             } else {
@@ -2130,13 +2136,13 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
         }
         bool mine = false;
 #pragma unroll
-        for (int k = 0; k < kServeWPL; ++k) mine |= rid[k] >= 0;
+        for (int k = 0; k < WPL; ++k) mine |= rid[k] >= 0;
         if (!__any(mine)) {
             if (!more) break;
             continue;
         }
 #pragma unroll
-        for (int k = 0; k < kServeWPL; ++k) {
+        for (int k = 0; k < WPL; ++k) {
             if (rid[k] < 0) continue;
             const ServeReq &q = rq[rid[k]];
             const int64_t w = wid[k];
@@ -2144,9 +2150,10 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
             const uint32_t w0 = (uint32_t)w, w1 = (uint32_t)((uint64_t)w >> 32);
             const uint32_t k0 = (uint32_t)q.seed, k1 = (uint32_t)(q.seed >> 32);
             const uint4 r = philox(w0, w1, (uint32_t)jj[k], att[k], k0, k1);
-            const bool fold = !FW && q.app == 1 && q.fold_r && jj[k] >= 2;
+            const bool n2v = MIX ? q.app == 1 : CLS == kServeN2V;
+            const bool fold = !FW && n2v && q.fold_r && jj[k] >= 2;
             uint64_t hcur = 0;
-            int32_t nx = (!FW && q.app != 1) ? sample_lite(s.alias, s.mem, cur[k], r)
+            int32_t nx = (!FW && !n2v) ? sample_lite(s.alias, s.mem, cur[k], r)
                                               : draw<FW>(s, cur[k], r, hcur, w0, w1, (uint32_t)jj[k], att[k], k0, k1);
             bool taken = false;  // the fold coin took the return arc
             if (fold) {
@@ -2160,7 +2167,7 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
             bool done = nx < 0;
             if (!done) {
                 const uint32_t aux = r.x & kAuxMask;
-                if (q.app == 1 && !taken && jj[k] >= 2 && (int64_t)att[k] + 1 < q.max_attempts) {
+                if (n2v && !taken && jj[k] >= 2 && (int64_t)att[k] + 1 < q.max_attempts) {
                     const uint32_t tlo = min(q.thr_neighbor, q.thr_other), thi = max(q.thr_neighbor, q.thr_other);
                     bool ok;
                     if (nx == prev[k]) ok = aux < q.thr_return;
@@ -2188,7 +2195,7 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
         }
     }
 #pragma unroll
-    for (int k = 0; k < kServeWPL; ++k)
+    for (int k = 0; k < WPL; ++k)
         if (acc_req[k] >= 0 && acc[k]) atomicAdd(rq[acc_req[k]].steps, (unsigned long long)acc[k]);
 }
 
@@ -2521,11 +2528,20 @@ void launch_walk(DevStore s, int app, const WalkArgs &a, int block, hipStream_t 
     }
 }
 
-int serve_max_blocks_per_cu(int fw)
+int serve_walkers_per_lane(int cls) { return serve_wpl(cls); }
+
+template <bool FW>
+static const void *serve_fn(int cls)
+{
+    return cls == kServeN2V     ? reinterpret_cast<const void *>(&k_walk_serve<FW, kServeN2V>)
+           : cls == kServeOther ? reinterpret_cast<const void *>(&k_walk_serve<FW, kServeOther>)
+                                : reinterpret_cast<const void *>(&k_walk_serve<FW, kServeMixed>);
+}
+
+int serve_max_blocks_per_cu(int fw, int cls)
 {
     int nb = 0;
-    const void *fn = fw ? reinterpret_cast<const void *>(&k_walk_serve<true>)
-                        : reinterpret_cast<const void *>(&k_walk_serve<false>);
+    const void *fn = fw ? serve_fn<true>(cls) : serve_fn<false>(cls);
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, 0) != hipSuccess) {
         (void)hipGetLastError();
         return 1;
@@ -2533,13 +2549,26 @@ int serve_max_blocks_per_cu(int fw)
     return nb > 0 ? nb : 1;
 }
 
+template <bool FW>
+static void launch_serve_cls(DevStore s, const ServeReq *reqs, int nreq, unsigned long long total,
+                             unsigned long long *cursor, int cls, int blocks, hipStream_t st)
+{
+    const dim3 g(blocks), b(256);
+    if (cls == kServeN2V)
+        hipLaunchKernelGGL((k_walk_serve<FW, kServeN2V>), g, b, 0, st, s, reqs, nreq, total, cursor);
+    else if (cls == kServeOther)
+        hipLaunchKernelGGL((k_walk_serve<FW, kServeOther>), g, b, 0, st, s, reqs, nreq, total, cursor);
+    else
+        hipLaunchKernelGGL((k_walk_serve<FW, kServeMixed>), g, b, 0, st, s, reqs, nreq, total, cursor);
+}
+
 void launch_walk_serve(DevStore s, const ServeReq *reqs, int nreq, unsigned long long total,
-                       unsigned long long *cursor, int blocks, hipStream_t st)
+                       unsigned long long *cursor, int cls, int blocks, hipStream_t st)
 {
     if (s.fw)
-        hipLaunchKernelGGL(k_walk_serve<true>, dim3(blocks), dim3(256), 0, st, s, reqs, nreq, total, cursor);
+        launch_serve_cls<true>(s, reqs, nreq, total, cursor, cls, blocks, st);
     else
-        hipLaunchKernelGGL(k_walk_serve<false>, dim3(blocks), dim3(256), 0, st, s, reqs, nreq, total, cursor);
+        launch_serve_cls<false>(s, reqs, nreq, total, cursor, cls, blocks, st);
 }
 
 size_t persist_lds_bytes(int block, int wpl, int depth)

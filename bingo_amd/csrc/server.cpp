@@ -129,7 +129,7 @@ struct bingo_server {
     int nstreams = 2;
     std::vector<double> lat;
     bingo_server_stats st{};
-    int blocks_per_cu = 1;
+    int blocks_per_cu[3] = {1, 1, 1};  // per launch class: other apps, node2vec, mixed
     // scheduler-local
     std::deque<std::shared_ptr<Req>> order;  // unfinished requests, submission order
     double busy_since = -1;
@@ -250,7 +250,9 @@ static bool admit(bingo_server *sv, Req &r)
 // Pack the schedulable walks (those before the first pending update) into
 // launch L in submission order: whole requests while the launch's walker
 // budget (max_launch_walkers) and the arena allow, a request larger than the
-// budget in chunks.  The launch is enqueued while the previous one still
+// budget in chunks.  A launch whose requests are all node2vec, or hold no
+// node2vec request, runs the kernel variant of that class (fewer registers,
+// more resident waves); a mixed launch runs the variant for any app.  The launch is enqueued while the previous one still
 // runs, so under load launches are large and back to back, and a request
 // that arrives while the device is busy goes into the next one.  False if
 // there was nothing to run.
@@ -295,17 +297,20 @@ static bool schedule(bingo_server *sv, Launch &L)
     }
     if (L.parts.empty()) return false;
     L.total = total;
+    int n2v = 0;
+    for (const Part &pt : L.parts) n2v += pt.r->app == BINGO_NODE2VEC;
+    const int cls = n2v == 0 ? 0 : n2v == (int)L.parts.size() ? 1 : 2;
     // inputs staged here may be read by chunks on any stream from now on
     hipError_t e = staged ? hipStreamSynchronize(sv->copy_st) : hipSuccess;
     bingo_graph *g = sv->g;
     const int n = (int)L.parts.size();
-    const int64_t per_block = 256 * 2;
-    const int64_t full = (int64_t)g->num_cu * sv->blocks_per_cu;
+    const int64_t per_block = 256 * serve_walkers_per_lane(cls);
+    const int64_t full = (int64_t)g->num_cu * sv->blocks_per_cu[cls];
     const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(full, (total + per_block - 1) / per_block));
     if (e == hipSuccess) e = hipMemcpyAsync(L.d_table, L.h_table, n * sizeof(ServeReq), hipMemcpyHostToDevice, L.st);
     if (e == hipSuccess) e = hipMemsetAsync(L.d_counts, 0, (kMaxServeReqs + 1) * sizeof(unsigned long long), L.st);
     if (e == hipSuccess) {
-        launch_walk_serve(g->s, L.d_table, n, (unsigned long long)total, L.d_counts + kMaxServeReqs, (int)blocks,
+        launch_walk_serve(g->s, L.d_table, n, (unsigned long long)total, L.d_counts + kMaxServeReqs, cls, (int)blocks,
                           L.st);
         e = hipGetLastError();
     }
@@ -441,8 +446,10 @@ int bingo_server_create(bingo_graph *g, const bingo_server_config *c, bingo_serv
         return fail(BINGO_EINVAL, "server arena, walker budget or stream count out of range");
     }
     sv->nstreams = sv->cfg.streams ? sv->cfg.streams : 2;
-    const int occ = serve_max_blocks_per_cu(g->s.fw);
-    sv->blocks_per_cu = sv->cfg.blocks_per_cu > 0 ? std::min(sv->cfg.blocks_per_cu, occ) : occ;
+    for (int c = 0; c < 3; ++c) {
+        const int occ = serve_max_blocks_per_cu(g->s.fw, c);
+        sv->blocks_per_cu[c] = sv->cfg.blocks_per_cu > 0 ? std::min(sv->cfg.blocks_per_cu, occ) : occ;
+    }
     (void)hipSetDevice(g->device);
     bool ok = hipMalloc((void **)&sv->arena.base, sv->cfg.arena_bytes) == hipSuccess &&
               hipStreamCreateWithFlags(&sv->copy_st, hipStreamNonBlocking) == hipSuccess;

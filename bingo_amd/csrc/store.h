@@ -246,9 +246,12 @@ struct PersistCfg {
 size_t persist_lds_bytes(int block, int wpl, int depth);
 int persist_max_blocks_per_cu(int app, int wpl, int depth, int block, int fw);
 void launch_walk_persist(DevStore s, int app, const WalkArgs &a, const PersistCfg &c, hipStream_t st);
+// cls: 0 = the launch holds no node2vec request, 1 = only node2vec
+// requests, 2 = any mix (a kernel variant per class)
 void launch_walk_serve(DevStore s, const ServeReq *reqs, int nreq, unsigned long long total,
-                       unsigned long long *cursor, int blocks, hipStream_t st);
-int serve_max_blocks_per_cu(int fw);
+                       unsigned long long *cursor, int cls, int blocks, hipStream_t st);
+int serve_max_blocks_per_cu(int fw, int cls);
+int serve_walkers_per_lane(int cls);
 // fmass/fthr (export_a) and slot_f (export_b) are null unless fw
 void launch_export_a(DevStore s, uint32_t *nslots, uint64_t *mass, uint32_t *bsize, uint64_t *T, uint8_t *alias,
                      uint64_t *fmass, uint64_t *fthr, hipStream_t st);
