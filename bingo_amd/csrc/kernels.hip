@@ -2000,7 +2000,7 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
 // ------------------------------------------------------------------------
 constexpr int kServeOther = 0, kServeN2V = 1, kServeMixed = 2;
 // walker slots per lane: one where node2vec state is carried only
-__host__ __device__ constexpr int serve_wpl(int cls) { return cls == kServeN2V ? 1 : 2; }
+__host__ __device__ constexpr int serve_wpl(int cls) { return cls == kServeOther ? 2 : 1; }
 constexpr int kServeD = 8;  // path positions staged per slot in LDS
 
 // A served walker's row: staged kServeD positions at a time in LDS and
