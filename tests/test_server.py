@@ -119,3 +119,41 @@ def test_server_device_outputs_and_admission(orc):
     srv.close()
     dp.free()
     g.close()
+
+
+@pytest.mark.parametrize("apps", [(B.NODE2VEC,), (B.DEEPWALK, B.PPR, B.SAMPLING)], ids=["node2vec", "no_node2vec"])
+def test_single_class_launches_equal_alone(orc, apps):
+    """Launches holding only node2vec requests, or none, run kernel variants
+    of their own; their paths must equal each request's stand-alone walk and
+    the oracle's."""
+    rng = random.Random(11)
+    scale = 11
+    n = 1 << scale
+    base = B.gen_rmat(scale, 12000, B.W_LOGUNI32, seed=21)
+    g = _load(n, base)
+    o = orc.OracleGraph(n)
+    o.load(base)
+    reqs = []
+    for i in range(10):
+        app = apps[i % len(apps)]
+        p = B.walk_params(seed=500 + i, max_len=rng.choice([8, 24]), tp=0.1, p=rng.choice([0.5, 2.0]),
+                          q=rng.choice([0.5, 2.0]))
+        nw = rng.choice([3, 300, 2500])
+        starts = np.array([rng.randint(1, n) for _ in range(nw)], np.int32) if i % 2 else None
+        reqs.append((app, p, nw, starts))
+    alone = []
+    for app, p, nw, starts in reqs:
+        paths, steps = g.walk(app, p, starts=starts, nwalkers=nw)
+        op = orc.walk_params(seed=p.seed, max_len=p.max_len, tp=p.tp, p=p.p, q=p.q)
+        pb, sb = o.walk(app, op, starts=starts, nwalkers=nw)
+        assert np.array_equal(paths, pb) and steps == sb
+        alone.append((paths, steps))
+    srv = B.Server(g, arena_bytes=1 << 28, max_launch_walkers=4000)
+    tickets = [srv.submit_walk(app, p, nwalkers=nw, starts=starts) for app, p, nw, starts in reqs]
+    for i, t in enumerate(tickets):
+        out, steps, _ = srv.wait(t)
+        assert np.array_equal(out, alone[i][0]), i
+        assert steps == alone[i][1], i
+    assert srv.stats()["walk_requests"] == len(reqs)
+    srv.close()
+    g.close()
