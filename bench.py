@@ -70,18 +70,24 @@ def log(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
-def measured_traffic(workload):
-    """HBM bytes per walk launch from the rocprofv3 FETCH_SIZE + WRITE_SIZE
-    runs summarised under profiles/ (copy-calibrated), or None."""
+def profile_figure(name, workload):
+    """A figure `name[workload] = value` recorded in a rocprofv3 summary
+    under profiles/, or None."""
     import glob
     import re
 
     for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         for line in open(f):
-            m = re.match(r"traffic_bytes_per_launch\[(.+)\] = (\S+)", line.strip())
-            if m and m.group(1) == workload:
-                return float(m.group(2))
+            m = re.match(r"(\w+)\[(.+)\] = (\S+)", line.strip())
+            if m and m.group(1) == name and m.group(2) == workload:
+                return float(m.group(3))
     return None
+
+
+def measured_traffic(workload):
+    """HBM bytes per walk launch from the rocprofv3 FETCH_SIZE + WRITE_SIZE
+    runs summarised under profiles/ (copy-calibrated), or None."""
+    return profile_figure("traffic_bytes_per_launch", workload)
 
 
 def walker_lens(cfg, nw, seed):
@@ -493,6 +499,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_GBps": traffic / kern_s / 1e9 if traffic else None,
+                     "l2_hit_rate": profile_figure("l2_hit_rate", cfg["workload"]),
                      "kernel": "k_walk_persist" if tuned.get("form") == B.FORM_PERSISTENT else "k_walk",
                      "kernel_ms": kern_ms / K, "algorithmic_bytes_per_launch": bytes_per_launch},
         "gather_ceiling": {"reads32_per_s": rates[32], "reads8_per_s": rates[8],
