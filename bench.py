@@ -62,7 +62,8 @@ CONFIGS = {
     # at load; sliding-window churn: 10^7 new arcs in, the 10^7 oldest out
     "c5": dict(workload="rmat25_a30_loguni32_deepwalk_w32e6_churn1e7", scale=25, edges=1 << 30, wkind=2, app=0,
                max_len=80, batch=10_000_000, p=0.5, q=2.0, tp=0.0125, walkers=32_000_000, lens=None, graph=False,
-               directed=True, fit=True, churn="expire", cpu_sample_edges=1 << 26, grow_in_place=True),
+               directed=True, fit=True, churn="expire", cpu_sample_edges=1 << 26, grow_in_place=True,
+               compact=1 / 32),
 }
 
 
@@ -372,6 +373,8 @@ def main():
         if cfg.get("float"):
             ins = to_float_bits(ins)
         dels = ug.expire(kb) if cfg.get("churn") == "expire" else ug.deletes(kb)
+        if cfg.get("compact"):
+            ug.compact(cfg["compact"])  # the store compacts at the same point of the stream
         batches.append((B.DeviceArray(g, ins), B.DeviceArray(g, dels), kb))
     del ug
     log(f"staged {W + K} update batches, sizes {sizes}")
@@ -397,33 +400,41 @@ def main():
         g.event(1)
         g.delete_device(dd, kb)
         g.event(2)
+        if cfg.get("compact"):
+            g.compact(cfg["compact"], counts=False)
+        g.event(6)
         if wgraph:
             wgraph.launch()
         else:
             g.walk_device(app, wp, nw, d_paths, d_steps, d_lens=d_lens)
         g.event(3)
         st = g.stats()
-        return g.elapsed_ms(0, 1), g.elapsed_ms(1, 2), g.elapsed_ms(2, 3), st
+        st["compact_ms"] = g.elapsed_ms(2, 6)
+        return g.elapsed_ms(0, 1), g.elapsed_ms(1, 2), g.elapsed_ms(6, 3), st
 
     for i in range(W):
         step(i)
         log(f"warmup step {i} done")
     g.sync()
     B._check(B.lib().bingo_memcpy_h2d(g._h, d_steps.ptr, np.zeros(1, np.uint64).ctypes.data, 8))
-    ins_ms = del_ms = walk_ms = kern_ms = 0.0
+    ins_ms = del_ms = walk_ms = kern_ms = comp_ms = 0.0
+    store_trace = []
     attempts = probes = folds = 0
     t0 = time.perf_counter()
     for i in range(K):
         a, b, c, st = step(W + i)
         ins_ms += a
         del_ms += b
+        comp_ms += st["compact_ms"]
+        store_trace.append(st["bytes_reserved"])
         walk_ms += c
         kern_ms += st["walk_kernel_ms"]
         attempts += st["walk_attempts"]
         probes += st["walk_probes"]
         folds += st["walk_folds"]
-        log(f"step {i}: insert {batches[W + i][2]} in {a:.2f} ms, delete {batches[W + i][2]} in {b:.2f} ms, "
-            f"walk {c:.2f} ms")
+        log(f"step {i}: insert {batches[W + i][2]} in {a:.2f} ms, delete {batches[W + i][2]} in {b:.2f} ms"
+            f"{' (+ compaction %.2f ms)' % st['compact_ms'] if cfg.get('compact') else ''}, walk {c:.2f} ms, "
+            f"store {st['bytes_reserved'] / 1e9:.1f} GB")
     g.sync()
     wall = time.perf_counter() - t0
     steps = int(d_steps.to_host(np.uint64, (1,))[0])
@@ -485,9 +496,11 @@ def main():
                    "pools": "grown in place" if gip else "grown by copy",
                    "weights": "float32 w/256" if cfg.get("float") else "u32"},
         "walk_steps_per_s": walk_rate,
-        "updates_per_s": nupd / ((ins_ms + del_ms) / 1e3),
+        "updates_per_s": nupd / ((ins_ms + del_ms + comp_ms) / 1e3),
         "insert_ms": ins_ms / K,
         "delete_ms": del_ms / K,
+        "compact_ms": comp_ms / K if cfg.get("compact") else None,
+        "store_bytes_first_last": [store_trace[0], store_trace[-1]] if store_trace else None,
         "walk_ms": walk_ms / K,
         "load_ms": load_ms,
         "walk_draws_per_step": attempts / max(steps, 1),

@@ -82,6 +82,7 @@ class Stats(C.Structure):
         ("arcs", C.c_uint64),
         ("bytes_reserved", C.c_uint64),
         ("pool_used", C.c_uint64 * 4),
+        ("compact_ms", C.c_double),
     ]
 
 
@@ -205,6 +206,8 @@ SIGNATURES = {
     "bingo_updgen_deletes": (_int, [_vp, _i64, _vp]),
     "bingo_updgen_expire": (_int, [_vp, _i64, _vp]),
     "bingo_updgen_live": (_i64, [_vp]),
+    "bingo_updgen_compact": (_int, [_vp, C.c_double]),
+    "bingo_compact": (_int, [_vp, C.c_double, C.POINTER(_u64), C.POINTER(_u64)]),
 }
 
 _lib = None
@@ -309,10 +312,23 @@ class UpdateGen:
     def live(self):
         return lib().bingo_updgen_live(self._h)
 
+    def compact(self, min_dead_frac):
+        """Renumber slots as Graph.compact(min_dead_frac) renumbers the store's."""
+        _check(lib().bingo_updgen_compact(self._h, min_dead_frac))
+
     def __del__(self):
         if getattr(self, "_h", None):
             lib().bingo_updgen_destroy(self._h)
             self._h = None
+
+
+def _need(arr, nbytes, what):
+    """A device buffer the kernels will touch must hold nbytes: an undersized
+    one would be written out of bounds on the GPU."""
+    if arr is None:
+        return
+    if arr.ptr is None or arr.nbytes < nbytes:
+        raise BingoError(-1, f"{what}: device buffer of {arr.nbytes} bytes, the call touches {nbytes}")
 
 
 class DeviceArray:
@@ -386,6 +402,8 @@ class Server:
         if lens is not None:
             lens = np.ascontiguousarray(lens, np.int32)
         on_dev = isinstance(out, DeviceArray)
+        if on_dev:
+            _need(out, nwalkers * params.max_len * 4, "served walk paths")
         if out is None:
             out = np.empty((nwalkers, params.max_len), np.int32)
         ptr = out.ptr if on_dev else _ptr(out)
@@ -421,6 +439,13 @@ class Server:
             self._h = None
 
     __del__ = close
+
+
+def _walk_buffers(params, nwalkers, d_paths, d_steps, d_starts, d_lens):
+    _need(d_paths, nwalkers * params.max_len * 4, "walk paths")
+    _need(d_steps, 8, "walk step counter")
+    _need(d_starts, nwalkers * 4, "walk starts")
+    _need(d_lens, nwalkers * 4, "walk lengths")
 
 
 class Graph:
@@ -465,6 +490,7 @@ class Graph:
         self.n = n
 
     def load_device(self, n, d_edges: DeviceArray, m):
+        _need(d_edges, m * EDGE_DT.itemsize, "load edges")
         _check(lib().bingo_load_device(self._h, n, d_edges.ptr, m))
         self.n = n
 
@@ -473,6 +499,7 @@ class Graph:
         _check(lib().bingo_insert(self._h, _ptr(e), len(e)))
 
     def insert_device(self, d_edges: DeviceArray, k):
+        _need(d_edges, k * EDGE_DT.itemsize, "insert edges")
         _check(lib().bingo_insert_device(self._h, d_edges.ptr, k))
 
     def delete(self, dels):
@@ -480,7 +507,20 @@ class Graph:
         _check(lib().bingo_delete(self._h, _ptr(d), len(d)))
 
     def delete_device(self, d_dels: DeviceArray, k):
+        _need(d_dels, k * DEL_DT.itemsize, "deletes")
         _check(lib().bingo_delete_device(self._h, d_dels.ptr, k))
+
+    def compact(self, min_dead_frac, counts=True):
+        """Reclaim tombstoned slots of nodes with dead >= 1 and dead / nslots
+        >= min_dead_frac (16-bit fixed point); live slots are renumbered by
+        rank.  Returns (nodes, slots) reclaimed, or None with counts=False
+        (then the call does not wait for the device)."""
+        if not counts:
+            _check(lib().bingo_compact(self._h, min_dead_frac, None, None))
+            return None
+        nodes, slots = C.c_uint64(0), C.c_uint64(0)
+        _check(lib().bingo_compact(self._h, min_dead_frac, C.byref(nodes), C.byref(slots)))
+        return nodes.value, slots.value
 
     # -- walks --
     def walk(self, app, params: WalkParams, starts=None, lens=None, nwalkers=None):
@@ -499,12 +539,14 @@ class Graph:
 
     def walk_device(self, app, params: WalkParams, nwalkers, d_paths: DeviceArray, d_steps: DeviceArray,
                     d_starts: DeviceArray | None = None, d_lens: DeviceArray | None = None):
+        _walk_buffers(params, nwalkers, d_paths, d_steps, d_starts, d_lens)
         _check(lib().bingo_walk_device(self._h, app, C.byref(params), d_starts.ptr if d_starts else None,
                                        d_lens.ptr if d_lens else None, nwalkers, d_paths.ptr, d_steps.ptr))
 
     def walk_graph(self, app, params: WalkParams, nwalkers, d_paths: DeviceArray, d_steps: DeviceArray,
                    d_starts: DeviceArray | None = None, d_lens: DeviceArray | None = None):
         """Capture one walk launch as a HIP graph; call .launch() to replay it."""
+        _walk_buffers(params, nwalkers, d_paths, d_steps, d_starts, d_lens)
         return WalkGraph(self, app, params, nwalkers, d_paths, d_steps, d_starts, d_lens)
 
     def tune(self, app, params: WalkParams, sample):

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <initializer_list>
 #include <string>
@@ -131,6 +132,7 @@ static int grow_mapped(bingo_graph *g, int p, uint64_t cap_elems)
     gran = std::max<size_t>(gran, 1ull << 30);
     auto up = [gran](uint64_t x) { return (x + gran - 1) / gran * gran; };
     if (!v.va) {
+        if (g->refuse_vmm & (1u << p)) return 1;  // test hook (BINGO_REFUSE_VMM)
         const uint64_t limit = std::min<uint64_t>(g->budget, (1ull << kOffBits) * kElem[p]);
         if (hipMemAddressReserve(&v.va, up(limit), gran, nullptr, 0) != hipSuccess) {
             (void)hipGetLastError();
@@ -188,15 +190,45 @@ static void release_mapped(bingo_graph *g)
     }
 }
 
+// Whether pool p grows by mapping (a reserved range that is mapped already,
+// or a first allocation of a grow_in_place handle whose device has not
+// refused a reservation).  A pool that is mapped stays mapped: it can never
+// move to a copied allocation.
+static bool pool_mapped(const bingo_graph *g, int p, const void *cur)
+{
+    return g->vp[p].va || (g->cfg.grow_in_place && !g->vmm_refused && !cur);
+}
+
+// HBM a pool of cap elements holds: mapped pools take whole 1 GiB chunks
+static uint64_t pool_footprint(const bingo_graph *g, int p, const void *cur, uint64_t cap)
+{
+    const uint64_t b = cap * kElem[p];
+    if (!pool_mapped(g, p, cur)) return b;
+    const uint64_t gib = 1ull << 30;
+    return (b + gib - 1) / gib * gib;
+}
+
+static void **pool_slot(bingo_graph *g, int p)
+{
+    return p == POOL_SLOT   ? (void **)&g->s.slot
+           : p == POOL_BACK ? (void **)&g->s.back
+           : p == POOL_MEM  ? (void **)&g->s.mem
+                            : (void **)&g->s.hash;
+}
+
 // Grow pools so that top + need fits; a grown pool is either mapped further
 // in place (grow_in_place) or a new allocation with the old contents copied
 // (region offsets stay valid either way).  Refuses growth past the HBM
-// budget with BINGO_ENOMEM before anything is modified.
+// budget with BINGO_ENOMEM before anything is modified; a batch that grows
+// no pool is never refused.
 static int ensure_pools(bingo_graph *g)
 {
     const Ctl &c = *g->ctl_host;
+    for (int p = 0; p < NPOOL; ++p)  // alias slots address members and tables with 36 bits
+        if (c.top[p] + c.need[p] > (1ull << kOffBits))
+            return fail(BINGO_ENOMEM, "a pool would exceed 2^36 elements");
     uint64_t newcap[NPOOL];
-    uint64_t total = g->node_bytes;
+    bool grows = false;
     for (int p = 0; p < NPOOL; ++p) {
         const uint64_t want = c.top[p] + c.need[p];
         newcap[p] = g->s.cap[p];
@@ -205,33 +237,32 @@ static int ensure_pools(bingo_graph *g)
             // mapped one by at least an eighth (fewer mapping calls)
             const uint64_t slack = (uint64_t)((double)want * g->cfg.pool_slack);
             const uint64_t step = g->vp[p].va ? g->s.cap[p] / 8 : g->s.cap[p] / 2;
-            newcap[p] = std::max<uint64_t>(want + slack + 1024, g->s.cap[p] + step);
+            newcap[p] = std::min<uint64_t>(std::max<uint64_t>(want + slack + 1024, g->s.cap[p] + step),
+                                           1ull << kOffBits);
+            grows = true;
         }
-        total += newcap[p] * kElem[p];
     }
-    for (int p = 0; p < NPOOL; ++p)  // alias slots address members and tables with 36 bits
-        if (c.top[p] + c.need[p] > (1ull << kOffBits))
-            return fail(BINGO_ENOMEM, "a pool would exceed 2^36 elements");
-    for (int p = 0; p < NPOOL; ++p) newcap[p] = std::min<uint64_t>(newcap[p], 1ull << kOffBits);
+    if (!grows) return BINGO_OK;
+    auto total_of = [&](const uint64_t *cap) {
+        uint64_t t = g->node_bytes;
+        for (int p = 0; p < NPOOL; ++p) t += pool_footprint(g, p, *pool_slot(g, p), cap[p]);
+        return t;
+    };
+    uint64_t total = total_of(newcap);
     if (total > g->budget) {  // retry without slack
-        total = g->node_bytes;
         for (int p = 0; p < NPOOL; ++p) {
             const uint64_t want = c.top[p] + c.need[p];
             if (want > g->s.cap[p]) newcap[p] = want;
-            total += newcap[p] * kElem[p];
         }
+        total = total_of(newcap);
         if (total > g->budget)
             return fail(BINGO_ENOMEM, "store would need " + std::to_string(total) + " bytes, budget is " +
                                           std::to_string(g->budget));
     }
     for (int p = 0; p < NPOOL; ++p) {
         if (newcap[p] == g->s.cap[p]) continue;
-        void **slotp = p == POOL_SLOT ? (void **)&g->s.slot
-                       : p == POOL_BACK ? (void **)&g->s.back
-                       : p == POOL_MEM  ? (void **)&g->s.mem
-                                        : (void **)&g->s.hash;
-        // in place: a pool that is mapped already, or a first allocation
-        if (g->cfg.grow_in_place && !g->vmm_refused && (g->vp[p].va || !*slotp)) {
+        void **slotp = pool_slot(g, p);
+        if (pool_mapped(g, p, *slotp)) {
             const int rc = grow_mapped(g, p, newcap[p]);
             if (rc < 0) return rc;
             if (rc == 0) {
@@ -241,7 +272,7 @@ static int ensure_pools(bingo_graph *g)
                 if (first) ++g->store_gen;
                 continue;
             }
-            g->vmm_refused = true;  // refused: this and later pools are copied
+            g->vmm_refused = true;  // refused before mapping: this and later new pools are copied
         }
         void *fresh = nullptr;
         if (hipMalloc(&fresh, newcap[p] * kElem[p]) != hipSuccess) {
@@ -307,11 +338,37 @@ static int ensure_batch_scratch(bingo_graph *g, int64_t k, bool del)
     } else {
         if ((rc = ensure_scratch(g, SC_KEYS, k4)) || (rc = ensure_scratch(g, SC_VALS, k4)) ||
             (rc = ensure_scratch(g, SC_EDGES, (size_t)k * sizeof(DevEdge))) ||
-            (g->s.fw && (rc = ensure_scratch(g, SC_FRAC, k4))) ||
-            (rc = ensure_scratch(g, SC_WRES, insert_wres_bytes(k))))
+            (g->s.fw && (rc = ensure_scratch(g, SC_FRAC, k4))))
             return rc;
     }
     return BINGO_OK;
+}
+
+// the next free lists (s.fl2) hold at least `entries`; their old content is
+// not kept (the merge rewrites them whole)
+static int ensure_fl2(bingo_graph *g, uint64_t entries)
+{
+    if (entries <= g->flcap[1] && g->flbuf[1]) return BINGO_OK;
+    if (g->flbuf[1]) {
+        HIPCHK(hipStreamSynchronize(g->st));
+        HIPCHK(hipFree(g->flbuf[1]));
+        g->flbuf[1] = nullptr;
+        g->flcap[1] = 0;
+    }
+    const uint64_t want = std::max<uint64_t>(entries + entries / 4, 1u << 16);
+    if (hipMalloc((void **)&g->flbuf[1], want * sizeof(uint64_t)) != hipSuccess) {
+        (void)hipGetLastError();
+        g->flbuf[1] = nullptr;
+        return fail(BINGO_ENOMEM, "free-list allocation failed");
+    }
+    g->flcap[1] = want;
+    return BINGO_OK;
+}
+
+static void set_lists(bingo_graph *g)
+{
+    g->s.fl = g->flbuf[0];
+    g->s.fl2 = g->flbuf[1];
 }
 
 static int do_insert(bingo_graph *g, const DevEdge *d_e, int64_t k)
@@ -320,6 +377,9 @@ static int do_insert(bingo_graph *g, const DevEdge *d_e, int64_t k)
     if (k < 0 || k >= (1ll << 31)) return fail(BINGO_EINVAL, "batch size out of range");
     if (k == 0) return BINGO_OK;
     if ((rc = ensure_batch_scratch(g, k, false))) return rc;
+    if ((rc = ensure_scratch(g, SC_WCLS, insert_wcls_bytes(k)))) return rc;
+    g->s.wcls = scp<uint32_t>(g, SC_WCLS);
+    launch_alloc_reset(g->s.fm, g->st);
     launch_reset_ctl(g->s.ctl, g->st);
     launch_insert_keys(d_e, k, g->s.n, g->s.fw, scp<uint32_t>(g, SC_KEYS), scp<uint32_t>(g, SC_VALS), g->s.ctl, g->st);
     const int bits = bits_for((uint64_t)g->s.n);
@@ -333,8 +393,8 @@ static int do_insert(bingo_graph *g, const DevEdge *d_e, int64_t k)
     launch_gather_edges(d_e, second ? v1 : v0, k, scp<DevEdge>(g, SC_EDGES), frac, g->st);
     Runs r;
     if ((rc = encode_runs(g, second ? k1 : k0, k, &r))) return rc;
-    unsigned long long *wres = scp<unsigned long long>(g, SC_WRES);
-    launch_insert_plan(g->s, scp<DevEdge>(g, SC_EDGES), r, wres, k, g->st);
+    launch_insert_plan(g->s, scp<DevEdge>(g, SC_EDGES), r, k, g->st);
+    launch_alloc_resolve(g->s, g->s.fw, g->st);
     HIPCHK(hipGetLastError());
     if ((rc = read_ctl(g))) return rc;
     if (g->ctl_host->error & ERR_INVAL)
@@ -343,13 +403,25 @@ static int do_insert(bingo_graph *g, const DevEdge *d_e, int64_t k)
                                           : "insert batch has an arc with u or v outside 1..n or w == 0");
     if (g->ctl_host->error & ERR_NOMEM)
         return fail(BINGO_ENOMEM, "a node would reach the per-node limits (2^28 slots, integer weight 2^59)");
+    if (g->ctl_host->error & ERR_OVERRUN)
+        return fail(BINGO_EDEVICE, "an earlier batch overran a pool; the store is no longer consistent");
     if (g->ctl_host->error) return fail(BINGO_EDEVICE, "store reported an internal error");
     if ((rc = ensure_pools(g))) return rc;
-    launch_insert_apply(g->s, scp<DevEdge>(g, SC_EDGES), frac, r, wres, k, g->st);
+    if ((rc = ensure_scratch(g, SC_PF, g->ctl_host->pend_total * sizeof(uint64_t))) ||
+        (rc = ensure_fl2(g, g->ctl_host->fl_total)))
+        return rc;
+    g->s.pf = scp<uint64_t>(g, SC_PF);
+    set_lists(g);
+    launch_insert_apply(g->s, scp<DevEdge>(g, SC_EDGES), frac, r, k, g->st);
     // moves of large regions (hubs) by the whole grid, before the neighbour
     // tables take the batch's arcs
     if (g->ctl_host->nbigcopy) launch_copy_tasks(g->s, g->st);
     if (g->s.nbr) launch_hash_insert(g->s, scp<DevEdge>(g, SC_EDGES), k, g->st);
+    // the batch's frees join the lists (after the copies that read them)
+    launch_freelist_merge(g->s, g->st);
+    std::swap(g->flbuf[0], g->flbuf[1]);
+    std::swap(g->flcap[0], g->flcap[1]);
+    set_lists(g);
     HIPCHK(hipGetLastError());
     g->arcs += k;
     return BINGO_OK;
@@ -379,6 +451,8 @@ static int do_delete(bingo_graph *g, const int32_t *d_pairs, int64_t k)
     if ((rc = read_ctl(g))) return rc;
     if (g->ctl_host->error & ERR_INVAL)
         return fail(BINGO_EINVAL, "delete batch names a node outside 1..n, a slot that is not live, or a slot twice");
+    if (g->ctl_host->error & ERR_OVERRUN)
+        return fail(BINGO_EDEVICE, "an earlier batch overran a pool; the store is no longer consistent");
     if (g->ctl_host->error) return fail(BINGO_EDEVICE, "store reported an internal error");
     // neighbour entries first: they subtract each deleted slot's weight,
     // which the apply kernel then tombstones
@@ -548,6 +622,9 @@ int bingo_create(const bingo_config *cfg, bingo_graph **out)
         return fail(BINGO_EINVAL, "fit_headroom must be in [0, 1]");
     }
     g->device = g->cfg.device;
+    // test hook: treat the address reservation of the pools in this bit mask
+    // as refused (exercises the copy fallback next to mapped pools)
+    if (const char *e = std::getenv("BINGO_REFUSE_VMM")) g->refuse_vmm = (unsigned)std::strtoul(e, nullptr, 0);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
         delete g;
@@ -569,6 +646,7 @@ int bingo_create(const bingo_config *cfg, bingo_graph **out)
     bool ok = hipHostMalloc((void **)&g->ctl_host, sizeof(Ctl), hipHostMallocDefault) == hipSuccess &&
               hipMalloc((void **)&g->s.ctl, sizeof(Ctl)) == hipSuccess &&
               hipMalloc((void **)&g->s.tasks, kMaxCopyTasks * sizeof(CopyTask)) == hipSuccess &&
+              hipMalloc((void **)&g->s.fm, sizeof(FreeMeta)) == hipSuccess &&
               hipMalloc((void **)&g->d_steps, sizeof(unsigned long long)) == hipSuccess;
     for (int i = 0; ok && i < kEvents; ++i) ok = hipEventCreate(&g->ev[i]) == hipSuccess;
     for (int p = 0; ok && p < NPHASE; ++p)
@@ -578,6 +656,7 @@ int bingo_create(const bingo_config *cfg, bingo_graph **out)
         return fail(BINGO_EDEVICE, "handle allocation failed");
     }
     (void)hipMemset(g->s.ctl, 0, sizeof(Ctl));
+    (void)hipMemset(g->s.fm, 0, sizeof(FreeMeta));
     g->s.min_slots = g->cfg.min_slots;
     g->s.nbr = g->cfg.neighbor_index ? 1 : 0;
     g->s.fw = g->cfg.float_weights ? 1 : 0;
@@ -589,7 +668,9 @@ void bingo_destroy(bingo_graph *g)
 {
     if (!g) return;
     if (g->st) (void)hipStreamSynchronize(g->st);
-    void *bufs[] = {g->s.hdr, g->s.grp, g->s.alias, g->s.fnode, g->s.nmass, g->s.ctl, g->s.tasks, g->d_steps};
+    void *bufs[] = {g->s.hdr,  g->s.grp,    g->s.alias,   g->s.fnode,   g->s.nmass,
+                    g->s.ctl,  g->s.tasks,  g->d_steps,   g->s.fm,      g->flbuf[0],
+                    g->flbuf[1]};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     void *pools[NPOOL] = {g->s.slot, g->s.back, g->s.mem, g->s.hash};
@@ -636,7 +717,6 @@ int bingo_load_device(bingo_graph *g, int32_t n, const bingo_edge *d_edges, int6
     g->loaded = true;
     if (g->cfg.fit_load) g->s.fit = 1u + (uint32_t)std::llround(g->cfg.fit_headroom * 65536.0);
     int rc = do_insert(g, reinterpret_cast<const DevEdge *>(d_edges), m);
-    g->s.fit = 0;  // later inserts grow regions by doubling
     phase_end(g, PH_LOAD);
     if (rc) {
         g->loaded = false;  // leave a handle that can be destroyed, not reused
@@ -955,6 +1035,27 @@ int bingo_walk(bingo_graph *g, int app, const bingo_walk_params *params, const i
     return BINGO_OK;
 }
 
+int bingo_compact(bingo_graph *g, double min_dead_frac, uint64_t *nodes_out, uint64_t *slots_out)
+{
+    if (!g || !g->loaded) return fail(BINGO_ESTATE, "graph not loaded");
+    if (!(min_dead_frac >= 0 && min_dead_frac <= 1)) return fail(BINGO_EINVAL, "min_dead_frac must be in [0, 1]");
+    const uint32_t thr16 = (uint32_t)std::floor(min_dead_frac * 65536.0);
+    const size_t n1 = (size_t)g->s.n + 1;
+    int rc;
+    if ((rc = ensure_scratch(g, SC_CLIST, 2 * n1 * sizeof(uint32_t)))) return rc;
+    uint32_t *lists = scp<uint32_t>(g, SC_CLIST);
+    phase_begin(g, PH_COMPACT);
+    launch_compact(g->s, thr16, lists, lists + n1, g->st);
+    phase_end(g, PH_COMPACT);
+    HIPCHK(hipGetLastError());
+    if (nodes_out || slots_out) {
+        if ((rc = read_ctl(g))) return rc;
+        if (nodes_out) *nodes_out = (uint64_t)g->ctl_host->ncomp[0] + g->ctl_host->ncomp[1];
+        if (slots_out) *slots_out = g->ctl_host->comp_freed;
+    }
+    return BINGO_OK;
+}
+
 int32_t bingo_num_nodes(const bingo_graph *g) { return g && g->loaded ? g->s.n : 0; }
 
 int bingo_sync(bingo_graph *g)
@@ -965,6 +1066,11 @@ int bingo_sync(bingo_graph *g)
         int rc = read_ctl(g);
         if (rc) return rc;
         if (g->ctl_host->error & ERR_OVERRUN) return fail(BINGO_EDEVICE, "pool overrun detected");
+        // device-side walk inputs are checked by the walk kernels: a walker
+        // with a start outside 1..n or a length outside 1..max_len gets a row
+        // of -1 and is reported here (until the next walk starts)
+        if (g->ctl_host->werr & ERR_INVAL)
+            return fail(BINGO_EINVAL, "a walk had a start node outside 1..n or a length outside 1..max_len");
     }
     return BINGO_OK;
 }
@@ -981,6 +1087,7 @@ int bingo_stats_get(bingo_graph *g, bingo_stats *out)
     out->insert_ms = g->ph_ms[PH_INSERT];
     out->delete_ms = g->ph_ms[PH_DELETE];
     out->walk_ms = g->ph_ms[PH_WALK];
+    out->compact_ms = g->ph_ms[PH_COMPACT];
     out->walk_kernel_ms = g->ph_ms[PH_WALK];
     out->walk_steps = g->last_walk_steps;
     out->arcs = (uint64_t)g->arcs;

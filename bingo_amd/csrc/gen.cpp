@@ -12,6 +12,7 @@
 //   k-th delete:    Philox(seed; k_lo, k_hi, 0, 'DELS') picks a live arc
 #include <cmath>
 #include <cstdint>
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -101,14 +102,31 @@ bool spec_ok(int scale, double a, double b, double c, int wkind)
 
 }  // namespace
 
+// Slots are kept as raw numbers (assigned per node in arrival order, never
+// reused); a node's current slot number is raw - base[u].  Compaction (the
+// rule of bingo_compact) renumbers a node's live slots by rank: under expiry
+// the dead slots of a node are always its lowest raw numbers, so base[u]
+// grows by the dead count; after random deletes the live arcs of compacted
+// nodes are renumbered one by one.
 struct bingo_updgen {
     RmatSpec sp;
     int32_t n;
     uint64_t next_edge, draws;
-    std::vector<uint32_t> deg;
-    std::vector<uint64_t> live;  // (u << 32) | slot; live[head..] in arrival order until a random delete
-    size_t head = 0;             // arcs expired so far (sliding window)
-    int mode = 0;                // 0 no deletes yet, 1 random deletes, 2 expiry
+    std::vector<uint32_t> nslots;  // current slot count per node (tombstones included)
+    std::vector<uint32_t> dead;    // tombstones per node since its last compaction
+    std::vector<uint32_t> base;    // raw - current slot number
+    std::vector<uint64_t> live;    // (u << 32) | raw slot; live[head..] in arrival order until a random delete
+    std::vector<uint64_t> dels;    // random deletes since the last compaction: (u << 32) | current slot
+    size_t head = 0;               // arcs expired so far (sliding window)
+    int mode = 0;                  // 0 no deletes yet, 1 random deletes, 2 expiry
+
+    uint32_t cur(uint64_t e) const { return (uint32_t)e - base[e >> 32]; }
+    bingo_del del_of(uint64_t e) const { return bingo_del{(int32_t)(e >> 32), (int32_t)cur(e)}; }
+    void push(int32_t u)
+    {
+        live.push_back(((uint64_t)(uint32_t)u << 32) | (nslots[u] + base[u]));
+        ++nslots[u];
+    }
 };
 
 extern "C" {
@@ -144,7 +162,9 @@ int bingo_updgen_create(int32_t n, int scale, const bingo_edge *base, int64_t m,
     s->n = n;
     s->next_edge = (uint64_t)edge_begin;
     s->draws = 0;
-    s->deg.assign((size_t)n + 1, 0);
+    s->nslots.assign((size_t)n + 1, 0);
+    s->dead.assign((size_t)n + 1, 0);
+    s->base.assign((size_t)n + 1, 0);
     s->live.reserve((size_t)m);
     for (int64_t i = 0; i < m; ++i) {
         const int32_t u = base[i].u;
@@ -152,7 +172,7 @@ int bingo_updgen_create(int32_t n, int scale, const bingo_edge *base, int64_t m,
             delete s;
             return BINGO_EINVAL;
         }
-        s->live.push_back(((uint64_t)(uint32_t)u << 32) | s->deg[u]++);
+        s->push(u);
     }
     *out = s;
     return BINGO_OK;
@@ -171,7 +191,7 @@ int bingo_updgen_inserts(bingo_updgen *s, int64_t k, bingo_edge *out)
         rmat_arc(s->sp, e, &u, &v);
         if (u > s->n || v > s->n) return BINGO_EINVAL;
         out[i] = bingo_edge{u, v, weight_of(s->sp.wkind, s->sp.seed, e)};
-        s->live.push_back(((uint64_t)(uint32_t)u << 32) | s->deg[u]++);
+        s->push(u);
     }
     return BINGO_OK;
 }
@@ -192,7 +212,9 @@ int bingo_updgen_deletes(bingo_updgen *s, int64_t k, bingo_del *out)
         const uint64_t pick = s->live[idx];
         s->live[idx] = s->live.back();
         s->live.pop_back();
-        out[i] = bingo_del{(int32_t)(pick >> 32), (int32_t)(uint32_t)pick};
+        out[i] = s->del_of(pick);
+        ++s->dead[pick >> 32];
+        s->dels.push_back(((pick >> 32) << 32) | s->cur(pick));
     }
     return BINGO_OK;
 }
@@ -205,11 +227,58 @@ int bingo_updgen_expire(bingo_updgen *s, int64_t k, bingo_del *out)
     if (k) s->mode = 2;
     for (int64_t i = 0; i < k; ++i) {
         const uint64_t pick = s->live[s->head++];
-        out[i] = bingo_del{(int32_t)(pick >> 32), (int32_t)(uint32_t)pick};
+        out[i] = s->del_of(pick);
+        ++s->dead[pick >> 32];
     }
     return BINGO_OK;
 }
 
 int64_t bingo_updgen_live(const bingo_updgen *s) { return s ? (int64_t)(s->live.size() - s->head) : 0; }
+
+// The selection rule of bingo_compact: dead >= 1 and dead 2^16 >= thr16 nslots.
+int bingo_updgen_compact(bingo_updgen *s, double min_dead_frac)
+{
+    if (!s || !(min_dead_frac >= 0 && min_dead_frac <= 1)) return BINGO_EINVAL;
+    const uint64_t thr16 = (uint64_t)std::floor(min_dead_frac * 65536.0);
+    const int32_t n = s->n;
+    std::vector<uint8_t> pick((size_t)n + 1, 0);
+    bool any = false;
+    for (int32_t u = 1; u <= n; ++u)
+        if (s->dead[u] && (uint64_t)s->dead[u] * 65536u >= thr16 * s->nslots[u]) {
+            pick[u] = 1;
+            any = true;
+        }
+    if (!any) return BINGO_OK;
+    if (s->mode == 1) {
+        // random deletes: a live arc's new number is its current one minus the
+        // node's tombstones below it
+        std::sort(s->dels.begin(), s->dels.end());
+        std::vector<uint64_t> first((size_t)n + 2, 0);  // dels of node u: [first[u], first[u + 1])
+        for (uint64_t d : s->dels) ++first[(d >> 32) + 1];
+        for (int32_t u = 0; u <= n; ++u) first[u + 1] += first[u];
+        const std::vector<uint64_t> &D = s->dels;
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < (int64_t)s->live.size(); ++i) {
+            const uint64_t e = s->live[i];
+            const uint32_t u = (uint32_t)(e >> 32);
+            if (!pick[u]) continue;
+            const uint32_t c = s->cur(e);
+            const auto lo = D.begin() + (int64_t)first[u], hi = D.begin() + (int64_t)first[u + 1];
+            const uint32_t below = (uint32_t)(std::lower_bound(lo, hi, ((uint64_t)u << 32) | c) - lo);
+            s->live[i] = e - below;
+        }
+        std::vector<uint64_t> keep;
+        for (uint64_t d : s->dels)
+            if (!pick[d >> 32]) keep.push_back(d);
+        s->dels.swap(keep);
+    }
+    for (int32_t u = 1; u <= n; ++u)
+        if (pick[u]) {
+            if (s->mode == 2) s->base[u] += s->dead[u];  // expiry: the dead are the lowest raw numbers
+            s->nslots[u] -= s->dead[u];
+            s->dead[u] = 0;
+        }
+    return BINGO_OK;
+}
 
 }  // extern "C"

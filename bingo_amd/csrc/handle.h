@@ -47,14 +47,16 @@ enum Scratch {
     SC_AUX3,
     SC_BIG,   // delete batch: long runs
     SC_FRAC,  // insert batch with float weights: fractions in node order
-    SC_WRES,  // insert batch: each plan wave's pool reservation
+    SC_CLIST, // compaction: the listed nodes (two lists of up to n + 1)
+    SC_WCLS,  // insert batch: each plan workgroup's class index ranges
+    SC_PF,    // insert batch: the regions it frees
     NSCRATCH
 };
 
 constexpr size_t kElem[NPOOL] = {4, 4, 8, 8};
 constexpr int kEvents = 16;
 
-enum Phase { PH_LOAD, PH_INSERT, PH_DELETE, PH_WALK, NPHASE };
+enum Phase { PH_LOAD, PH_INSERT, PH_DELETE, PH_WALK, PH_COMPACT, NPHASE };
 
 }  // namespace bingo
 
@@ -84,7 +86,10 @@ struct bingo_graph {
         size_t reserved = 0, mapped = 0;
         std::vector<std::pair<hipMemGenericAllocationHandle_t, size_t>> chunks;
     } vp[bingo::NPOOL];
+    uint64_t *flbuf[2] = {nullptr, nullptr};  // free lists: current (s.fl) and next (s.fl2)
+    size_t flcap[2] = {0, 0};                  // their capacities (entries)
     bool vmm_refused = false;  // the device refused a reservation: pools are copied instead
+    unsigned refuse_vmm = 0;   // test hook: pools (bit p) whose reservation is treated as refused
     struct Tuned {
         bool valid = false;
         int32_t max_len = 0;

@@ -483,7 +483,7 @@ __global__ void k_reset_ctl(Ctl *ctl)
     const int t = threadIdx.x;
     if (t < NPOOL) ctl->need[t] = 0;
     if (t == 0) {
-        ctl->error = 0;
+        ctl->error &= (unsigned)ERR_OVERRUN;  // sticky: a batch overran a pool
         ctl->nbig = 0;
         ctl->nbigcopy = 0;
         ctl->taskctr = 0;
@@ -498,6 +498,7 @@ __global__ void k_reset_walk(Ctl *ctl)
         ctl->probes = 0;
         ctl->folds = 0;
         ctl->cursor = 0;
+        ctl->werr = 0;
     }
 }
 
@@ -516,7 +517,7 @@ __global__ void k_init_nodes(DevStore s)
     GroupHdr g = {0, 0, 0};
     s.grp[i] = g;
     if (j == 0) {
-        NodeHdr h = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        NodeHdr h = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         s.hdr[i >> 5] = h;
         if (s.fw) {
             FNode f = {0, 0};
@@ -573,7 +574,7 @@ struct Growth {
 // capacity of a region that has to grow to hold `need` elements (DevStore::fit)
 __device__ __forceinline__ uint32_t grow_cap(const DevStore &s, uint32_t need, uint32_t least)
 {
-    if (s.fit) return need + (uint32_t)(((uint64_t)need * (s.fit - 1u) + 0xFFFFu) >> 16);
+    if (s.fit) return class_size(class_up(need + (uint32_t)(((uint64_t)need * (s.fit - 1u) + 0xFFFFu) >> 16)));
     return pow2_at_least(max(need, least));
 }
 
@@ -692,7 +693,7 @@ __device__ __forceinline__ RunPre run_pre(const DevStore &s, const DevEdge *__re
                                           const uint32_t *__restrict__ aux, const RunMeta &m, int ln)
 {
     RunPre p;
-    p.h = NodeHdr{0, 0, 0, 0, 0, 0, 0, 0, 0};
+    p.h = NodeHdr{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     p.gh = GroupHdr{0, 0, 0};
     p.x = DevEdge{0, 0, 0u};
     p.f = 0;
@@ -711,21 +712,29 @@ __device__ __forceinline__ RunPre run_pre(const DevStore &s, const DevEdge *__re
     return p;
 }
 
-// The plan also reserves each wave's pool space: a wave's growth over all its
-// runs, added to the batch total with one atomic per pool per wave, gives the
-// wave's offset (wres[wave][pool]) into the batch's new space.  The apply
-// kernel runs with the same grid, so every wave sees the same runs in the same
-// order, makes the same growth decisions, and carves its regions from its own
-// reservation with a wave-local cursor -- no shared counter (one contended
-// 64-bit atomic costs ~12 ns on this part; a batch makes one per growing run).
-__global__ __launch_bounds__(256) void k_insert_plan(DevStore s, const DevEdge *__restrict__ E, Runs r,
-                                                     unsigned long long *__restrict__ wres)
+// The plan also reserves each workgroup's regions: the apply kernel runs with
+// the same grid, so every wave sees the same runs in the same order and makes
+// the same growth decisions.  Per workgroup, one global atomic per class used
+// (counted in LDS first) instead of one per growing region: a single
+// contended 64-bit atomic costs ~12 ns on this part.
+// class index of a region of capacity cap (its size class, or for a free
+// region of any capacity the largest class it holds)
+__device__ __forceinline__ uint32_t cls_of(int pool, uint32_t cap) { return (uint32_t)pool * kClasses + class_down(cap); }
+
+// The plan counts each workgroup's region allocations and frees per class in
+// LDS and reserves the workgroup's index ranges with one atomic per used
+// class (s.wcls); the apply kernel, with the same grid, hands out indices
+// from those ranges (any order within a workgroup: the regions of a class
+// are interchangeable).
+__global__ __launch_bounds__(256) void k_insert_plan(DevStore s, const DevEdge *__restrict__ E, Runs r)
 {
+    __shared__ uint32_t lc[2 * kPC];  // allocations, then frees, per class
+    for (int i = threadIdx.x; i < 2 * kPC; i += blockDim.x) lc[i] = 0u;
+    __syncthreads();
     const int ln = lane_id();
     const uint32_t nruns = *r.nruns;
     const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    unsigned long long need[NPOOL] = {0, 0, 0, 0};
     uint32_t nbigcopy = 0;
     RunMeta m0 = run_meta(s, r, wave, nruns), m1 = run_meta(s, r, wave + nwaves, nruns),
             m2 = run_meta(s, r, wave + 2 * nwaves, nruns);
@@ -749,32 +758,127 @@ __global__ __launch_bounds__(256) void k_insert_plan(DevStore s, const DevEdge *
             if (ln == 0) atomicOr(&s.ctl->error, (unsigned)ERR_NOMEM);
             continue;
         }
-        need[POOL_SLOT] += G.g_slots ? (uint64_t)slot_arrays(s.fw) * G.cap_slots : 0;
-        need[POOL_BACK] += G.g_back ? G.cap_back : 0;
-        need[POOL_HASH] += G.g_hash ? 2ull * G.cap_hash : 0;  // 16-byte entries
-        need[POOL_MEM] += dpp_sum64(G.g_g ? (uint64_t)G.cap_g : 0ull);
         nbigcopy += run_moves_big(h, gh, G) ? 1u : 0u;
+        if (ln == 0) {
+            if (G.g_slots) {
+                atomicAdd(&lc[cls_of(POOL_SLOT, G.cap_slots)], 1u);
+                if (h.slot_cap) atomicAdd(&lc[kPC + cls_of(POOL_SLOT, h.slot_cap)], 1u);
+            }
+            if (G.g_back) {
+                atomicAdd(&lc[cls_of(POOL_BACK, G.cap_back)], 1u);
+                if (h.back_cap) atomicAdd(&lc[kPC + cls_of(POOL_BACK, h.back_cap)], 1u);
+            }
+            if (G.g_hash) {
+                atomicAdd(&lc[cls_of(POOL_HASH, G.cap_hash)], 1u);
+                if (h.hash_cap) atomicAdd(&lc[kPC + cls_of(POOL_HASH, h.hash_cap)], 1u);
+            }
+        }
+        if (G.g_g) {
+            atomicAdd(&lc[cls_of(POOL_MEM, G.cap_g)], 1u);
+            if (gh.cap) atomicAdd(&lc[kPC + cls_of(POOL_MEM, gh.cap)], 1u);
+        }
     }
-    if (ln == 0) {
-        for (int p = 0; p < NPOOL; ++p)
-            wres[(size_t)wave * NPOOL + p] = need[p] ? atomicAdd(&s.ctl->need[p], need[p]) : 0ull;
-        if (nbigcopy) atomicAdd(&s.ctl->nbigcopy, nbigcopy);
+    if (ln == 0 && nbigcopy) atomicAdd(&s.ctl->nbigcopy, nbigcopy);
+    __syncthreads();
+    for (int i = threadIdx.x; i < 2 * kPC; i += blockDim.x) {
+        const uint32_t c = lc[i];
+        const int t = i % kPC;
+        s.wcls[(size_t)blockIdx.x * 2 * kPC + i] = c ? atomicAdd(i < kPC ? &s.fm->take[t] : &s.fm->freed[t], c) : 0u;
     }
 }
 
-// Region of n elements from a pool at the wave's cursor (wave-uniform); ~0
-// if the pool (sized by the host from the plan) would overflow, in which case
-// the caller writes nothing.
-__device__ __forceinline__ uint64_t bump(const DevStore &s, unsigned long long (&cur)[NPOOL], int pool, uint64_t n)
+// After the plan of a batch: per class, the allocations served by free
+// regions and the bump space past them (classes of a pool in order after the
+// pool's top), the pending segments of the batch's frees, and the lists'
+// layout after the batch.  One workgroup of kPC threads; pool p is waves
+// 4p .. 4p + 3.
+__global__ __launch_bounds__(kPC) void k_alloc_resolve(DevStore s, int fw)
 {
-    const uint64_t at = cur[pool];
-    cur[pool] += n;
-    if (at + n > s.cap[pool]) {
-        if (lane_id() == 0) atomicOr(&s.ctl->error, (unsigned)ERR_OVERRUN);
-        return ~0ull;
+    constexpr int kW = kPC / 64, kWP = kClasses / 64;  // waves, waves per pool
+    __shared__ uint64_t part[3][kW];
+    const int t = threadIdx.x, p = t / kClasses, k = t % kClasses, ln = lane_id(), w = t >> 6;
+    FreeMeta *fm = s.fm;
+    const uint32_t av = fm->avail[t], tk = fm->take[t], fr = fm->freed[t];
+    const uint32_t from = min(tk, av);
+    const uint64_t mult = p == POOL_SLOT ? slot_arrays(fw) : p == POOL_HASH ? 2u : 1u;
+    const uint64_t ex = (uint64_t)(tk - from) * mult * class_size((uint32_t)k);
+    const uint64_t nv = (uint64_t)(av - from) + fr;
+    const uint64_t einc = dpp_incl64(ex), pinc = dpp_incl64((uint64_t)fr), ninc = dpp_incl64(nv);
+    if (ln == 63) {
+        part[0][w] = einc;
+        part[1][w] = pinc;
+        part[2][w] = ninc;
     }
-    return at;
+    __syncthreads();
+    uint64_t eb = 0, etot = 0, pb = 0, ptot = 0, nb = 0, ntot = 0;
+    for (int i = 0; i < kW; ++i) {
+        if (i / kWP == p) {  // the excess prefix stays inside the pool
+            eb += i < w ? part[0][i] : 0ull;
+            etot += part[0][i];
+        }
+        pb += i < w ? part[1][i] : 0ull;
+        ptot += part[1][i];
+        nb += i < w ? part[2][i] : 0ull;
+        ntot += part[2][i];
+    }
+    fm->bump[t] = s.ctl->top[p] + eb + einc - ex;
+    if (k == 0) s.ctl->need[p] = etot;
+    fm->pseg[t] = pb + pinc - fr;
+    fm->nfbase[t] = nb + ninc - nv;
+    fm->nav[t] = (uint32_t)nv;
+    if (t == 0) {
+        s.ctl->pend_total = ptot;
+        s.ctl->fl_total = ntot;
+    }
 }
+
+// After a batch (and its copy tasks): class t's list
+// becomes its unused free regions followed by the batch's frees, in s.fl2.
+__global__ __launch_bounds__(256) void k_freelist_merge(DevStore s)
+{
+    const int t = blockIdx.x;
+    FreeMeta *fm = s.fm;
+    const uint32_t av = fm->avail[t], from = min(fm->take[t], av), rem = av - from, fr = fm->freed[t];
+    const uint64_t nb = fm->nfbase[t], ob = fm->fbase[t] + from, pb = fm->pseg[t];
+    for (uint32_t i = threadIdx.x; i < rem; i += blockDim.x) s.fl2[nb + i] = s.fl[ob + i];
+    for (uint32_t i = threadIdx.x; i < fr; i += blockDim.x) s.fl2[nb + rem + i] = s.pf[pb + i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        fm->avail[t] = fm->nav[t];
+        fm->fbase[t] = nb;
+        fm->take[t] = 0u;
+        fm->freed[t] = 0u;
+    }
+}
+
+// A workgroup's class allocator in the apply kernel:
+// the plan reserved index ranges [wcls[t], wcls[t] + count) per class for
+// allocations and frees; li hands out indices inside them (LDS counters).
+struct ClassAlloc {
+    const uint32_t *base;  // this workgroup's 2 kPC range starts
+    uint32_t *li;          // 2 kPC LDS counters
+    // a region of class `cap` of pool p (any lane that calls); ~0 on overrun
+    __device__ __forceinline__ uint64_t take(const DevStore &s, int p, uint32_t cap) const
+    {
+        const uint32_t t = cls_of(p, cap);
+        const uint32_t g = base[t] + atomicAdd(&li[t], 1u);
+        const FreeMeta *fm = s.fm;
+        const uint64_t elems = (uint64_t)(p == POOL_SLOT ? slot_arrays(s.fw) : p == POOL_HASH ? 2u : 1u) * cap;
+        const uint32_t av = fm->avail[t];
+        const uint64_t off = g < av ? s.fl[fm->fbase[t] + g] : fm->bump[t] + (uint64_t)(g - av) * elems;
+        if (off == ~0ull || off + elems > s.cap[p]) {
+            atomicOr(&s.ctl->error, (unsigned)ERR_OVERRUN);
+            return ~0ull;
+        }
+        return off;
+    }
+    // the region of capacity `cap` at `off` of pool p, free from the next batch on
+    __device__ __forceinline__ void give(const DevStore &s, int p, uint32_t cap, uint64_t off) const
+    {
+        const uint32_t t = cls_of(p, cap);
+        s.pf[s.fm->pseg[t] + base[kPC + t] + atomicAdd(&li[kPC + t], 1u)] = off;
+    }
+};
 
 // the batch's reservations are used: the pool tops move past them
 __global__ void k_commit_tops(Ctl *ctl)
@@ -813,25 +917,41 @@ __device__ __forceinline__ void region_copy(const DevStore &s, T *dst, const T *
 // `fr` (float weights only) holds the run's fractions, arc for arc; `p` the
 // node's records and this lane's first arc, loaded by the run loop.
 __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, const uint32_t *__restrict__ fr,
-                           uint32_t cnt, int32_t u, const RunPre &p, unsigned long long (&cur)[NPOOL], int ln)
+                           uint32_t cnt, int32_t u, const RunPre &p, const ClassAlloc &ca, int ln)
 {
     NodeHdr h = p.h;
     const GroupHdr gh = p.gh;
     const Growth G = plan_run(s, e, cnt, h, gh, p.x.w, ln);
     if (run_too_big(h, G)) return;  // rejected by the plan
 
-    // ---- regions from the wave's reservation (in plan order) ----
+    // ---- regions: each from its class; the region it replaces is freed ----
     uint64_t a_slot = h.slot_off, a_back = h.back_off, a_hash = h.hash_off;
-    const uint64_t gneed = G.g_g ? (uint64_t)G.cap_g : 0ull;
-    const uint64_t gincl = dpp_incl64(gneed);
-    const uint64_t gtotal = rl64(gincl, 63);
-    uint64_t gbase = 0;
-    if (G.g_slots) a_slot = bump(s, cur, POOL_SLOT, (uint64_t)slot_arrays(s.fw) * G.cap_slots);
-    if (G.g_back) a_back = bump(s, cur, POOL_BACK, G.cap_back);
-    if (G.g_hash) a_hash = bump(s, cur, POOL_HASH, 2ull * G.cap_hash);
-    if (gtotal) gbase = bump(s, cur, POOL_MEM, gtotal);
-    if (a_slot == ~0ull || a_back == ~0ull || a_hash == ~0ull || gbase == ~0ull) return;  // node left as it was
-    const uint64_t goff = G.g_g ? gbase + gincl - gneed : gh.off;
+    uint64_t goff = gh.off;
+    if (ln == 0) {
+        if (G.g_slots) {
+            a_slot = ca.take(s, POOL_SLOT, G.cap_slots);
+            if (h.slot_cap) ca.give(s, POOL_SLOT, h.slot_cap, h.slot_off);
+        }
+        if (G.g_back) {
+            a_back = ca.take(s, POOL_BACK, G.cap_back);
+            if (h.back_cap) ca.give(s, POOL_BACK, h.back_cap, h.back_off);
+        }
+        if (G.g_hash) {
+            a_hash = ca.take(s, POOL_HASH, G.cap_hash);
+            if (h.hash_cap) ca.give(s, POOL_HASH, h.hash_cap, h.hash_off);
+        }
+    }
+    a_slot = rl64(a_slot, 0);
+    a_back = rl64(a_back, 0);
+    a_hash = rl64(a_hash, 0);
+    if (G.g_g) {
+        goff = ca.take(s, POOL_MEM, G.cap_g);
+        if (gh.cap) ca.give(s, POOL_MEM, gh.cap, gh.off);
+    }
+    // an overrun (never, with the pools sized from the plan) leaves the store
+    // inconsistent and is reported sticky
+    const bool fail = a_slot == ~0ull || a_back == ~0ull || a_hash == ~0ull || __any(goff == ~0ull);
+    if (fail) return;  // node left as it was
 
     const uint32_t n0 = h.nslots;
     uint32_t *nv = s.slot + a_slot;
@@ -968,6 +1088,7 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, con
         h.slot_off = a_slot;
         h.slot_cap = G.cap_slots;
         h.nslots = G.new_n;
+        h.nlive += cnt;
         h.back_off = a_back;
         h.back_cap = G.cap_back;
         h.back_used = G.new_back;
@@ -985,23 +1106,23 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, con
 // Runs by grid stride in node order (neighbouring runs touch neighbouring
 // node records; a shared work counter measured slower).
 __global__ __launch_bounds__(256) void k_insert_apply(DevStore s, const DevEdge *__restrict__ E,
-                                                      const uint32_t *__restrict__ FR, Runs r,
-                                                      const unsigned long long *__restrict__ wres)
+                                                      const uint32_t *__restrict__ FR, Runs r)
 {
+    __shared__ uint32_t li[2 * kPC];
+    for (int i = threadIdx.x; i < 2 * kPC; i += blockDim.x) li[i] = 0u;
+    __syncthreads();
+    const ClassAlloc ca{s.wcls + (size_t)blockIdx.x * 2 * kPC, li};
     const int ln = lane_id();
     const uint32_t nruns = *r.nruns;
     const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    unsigned long long cur[NPOOL];
-#pragma unroll
-    for (int p = 0; p < NPOOL; ++p) cur[p] = s.ctl->top[p] + wres[(size_t)wave * NPOOL + p];
     RunMeta m0 = run_meta(s, r, wave, nruns), m1 = run_meta(s, r, wave + nwaves, nruns),
             m2 = run_meta(s, r, wave + 2 * nwaves, nruns);
     RunPre p0 = run_pre<false>(s, E, FR, m0, ln), p1 = run_pre<false>(s, E, FR, m1, ln);
     for (uint32_t ri = wave; ri < nruns; ri += nwaves) {
         const RunMeta m3 = run_meta(s, r, ri + 3 * nwaves, nruns);
         const RunPre p2 = run_pre<false>(s, E, FR, m2, ln);
-        if (m0.u) insert_run(s, E + m0.start, FR ? FR + m0.start : nullptr, m0.cnt, m0.u, p0, cur, ln);
+        if (m0.u) insert_run(s, E + m0.start, FR ? FR + m0.start : nullptr, m0.cnt, m0.u, p0, ca, ln);
         m0 = m1;
         m1 = m2;
         m2 = m3;
@@ -1167,6 +1288,7 @@ __device__ void delete_run(const DevStore &s, const uint32_t *__restrict__ slots
     }
     if (bad) atomicOr(&s.ctl->error, (unsigned)ERR_OVERRUN);
     if (kj) s.grp[(size_t)u * kGroups + ln].size = S2;
+    if (ln == 0) s.hdr[u].nlive = h.nlive - cnt;
     const uint64_t F = s.fw ? s.fnode[u].F - dpp_sum64(fsub) : 0ull;
     alias_rebuild(s, u, own ? S2 : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, F, ln);
 }
@@ -1270,6 +1392,7 @@ __device__ void delete_small(const DevStore &s, uint32_t cnt, int32_t u, const R
     if (bad) atomicOr(&s.ctl->error, (unsigned)ERR_OVERRUN);
     if (kj) s.grp[(size_t)u * kGroups + ln].size = S2;
     uint64_t fsub = 0;
+    if (ln == 0) s.hdr[u].nlive = h.nlive - cnt;
     if (in) {
         W[t_sl] = 0u;
         if (s.fw) {
@@ -1361,6 +1484,7 @@ __device__ void delete_finish(const DevStore &s, const uint32_t *__restrict__ sl
             FR[sl] = 0u;
         }
     }
+    if (ln == 0) s.hdr[u].nlive = h.nlive - cnt;
     GroupHdr gh = {0, 0, 0};
     if (ln < kGroups) gh = s.grp[(size_t)u * kGroups + ln];
     const uint64_t F = s.fw ? s.fnode[u].F - dpp_sum64(fsub) : 0ull;
@@ -1471,6 +1595,201 @@ __global__ __launch_bounds__(256) void k_delete_apply(DevStore s, const uint32_t
         m0 = m1;
         m1 = m2;
         p0 = p1;
+    }
+}
+
+// ------------------------------------------------------------------------
+// compaction: tombstoned slots are reclaimed in place.  A listed node's live
+// slots are renumbered by rank (the k-th live slot becomes slot k), their
+// slot records and back-index entries are packed to the front of the node's
+// regions, and each member's slot field takes the new number; bucket order,
+// masses and the alias table stay as they are.  The neighbour table is
+// rebuilt from the live slots (its dead keys go).  The reference tombstones
+// and never reuses a slot (include/operation.cuh:209-210, Bingo.cuh:199-218).
+//
+// In place: a live slot's new position and new back offset are at or below
+// its old ones, so a chunk of slots is read completely (records and back
+// entries) before any of its writes, and later chunks read only above what
+// earlier chunks write.
+// ------------------------------------------------------------------------
+__global__ void k_compact_reset(Ctl *ctl)
+{
+    if (threadIdx.x == 0) {
+        ctl->ncomp[0] = ctl->ncomp[1] = 0;
+        ctl->comp_freed = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_compact_select(DevStore s, uint32_t thr16, uint32_t *__restrict__ small_list,
+                                                        uint32_t *__restrict__ big_list)
+{
+    const int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    bool pick = false, big = false;
+    uint32_t dead = 0;
+    if (u >= 1 && u <= s.n) {
+        const NodeHdr h = s.hdr[u];
+        dead = h.nslots - h.nlive;
+        pick = dead >= 1u && (uint64_t)dead * 65536u >= (uint64_t)thr16 * h.nslots;
+        big = h.nslots > kBigCompact;
+    }
+    const int ln = lane_id();
+    const uint64_t lt = (1ull << ln) - 1ull;
+    const uint64_t ms = __ballot(pick && !big), mb = __ballot(pick && big);
+    uint32_t bs = 0, bb = 0;
+    if (ln == 0) {
+        if (ms) bs = atomicAdd(&s.ctl->ncomp[0], (unsigned)__popcll(ms));
+        if (mb) bb = atomicAdd(&s.ctl->ncomp[1], (unsigned)__popcll(mb));
+    }
+    bs = __shfl(bs, 0, 64);
+    bb = __shfl(bb, 0, 64);
+    if (pick && !big) small_list[bs + (uint32_t)__popcll(ms & lt)] = (uint32_t)u;
+    if (pick && big) big_list[bb + (uint32_t)__popcll(mb & lt)] = (uint32_t)u;
+    const uint32_t fr = wave_sum32(pick ? dead : 0u);
+    if (ln == 0 && fr) atomicAdd(&s.ctl->comp_freed, (unsigned long long)fr);
+}
+
+// A team of NT threads working on one node: one wave (NT = 64) or one
+// workgroup (NT = 1024, scan partials in LDS).
+template <int NT>
+struct Team {
+    uint32_t *part;  // NT / 64 words of LDS (NT > 64)
+    int t;
+    __device__ __forceinline__ void sync() const
+    {
+        if (NT > 64) __syncthreads();
+        else __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    // exclusive team prefix sum of v (threads in t order) and its total
+    __device__ __forceinline__ uint32_t excl(uint32_t v, uint32_t &total) const
+    {
+        const uint32_t inc = dpp_incl_add(v);
+        if (NT == 64) {
+            total = rl32(inc, 63);
+            return inc - v;
+        }
+        const int w = t >> 6;
+        if ((t & 63) == 63) part[w] = inc;
+        __syncthreads();
+        uint32_t before = 0, tot = 0;
+#pragma unroll
+        for (int i = 0; i < NT / 64; ++i) {
+            const uint32_t x = part[i];
+            before += i < w ? x : 0u;
+            tot += x;
+        }
+        __syncthreads();  // part is reused by the next scan
+        total = tot;
+        return before + inc - v;
+    }
+};
+
+// Every read of a chunk before any write of it: a workgroup barrier (which
+// waits for each wave's loads), or for one wave a wait on its own loads.
+template <int NT>
+__device__ __forceinline__ void reads_done()
+{
+    if (NT > 64) __syncthreads();
+    else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+template <int NT>
+__device__ void compact_node(const DevStore &s, int32_t u, const Team<NT> &tm, uint64_t *goff)
+{
+    const int t = tm.t;
+    const NodeHdr h = s.hdr[u];
+    uint32_t *V = s.slot + h.slot_off;
+    uint32_t *W = V + h.slot_cap;
+    uint32_t *B = V + 2ull * h.slot_cap;
+    uint32_t *F = V + 3ull * h.slot_cap;  // float weights only
+    uint32_t *back = s.back + h.back_off;
+    if (t < kGroups) goff[t] = s.grp[(size_t)u * kGroups + t].off;
+    tm.sync();
+    uint32_t nid = 0, nb = 0;  // live slots and their back entries so far
+    for (uint32_t c0 = 0; c0 < h.nslots; c0 += NT) {
+        const uint32_t i = c0 + (uint32_t)t;
+        const bool in = i < h.nslots;
+        const uint32_t v = in ? V[i] : 0u, w = in ? W[i] : 0u, b = in ? B[i] : 0u;
+        const uint32_t f = (s.fw && in) ? F[i] : 0u;
+        const bool live = in && (w || f);
+        const uint32_t pc = live ? (uint32_t)__popc(w) : 0u;
+        uint32_t tot;
+        // live count (high half) and back entries (low half, at most 32 NT)
+        const uint32_t ex = tm.excl((live ? 1u << 16 : 0u) | pc, tot);
+        const uint32_t id = nid + (ex >> 16), nbk = nb + (ex & 0xFFFFu);
+        const bool mslot = live && id != i, mback = live && nbk != b;
+        uint32_t e[kGroups];
+#pragma unroll
+        for (int r = 0; r < kGroups; ++r) e[r] = (mslot || mback) && (uint32_t)r < pc ? back[b + r] : 0u;
+        reads_done<NT>();
+        if (mslot) {
+            V[id] = v;
+            W[id] = w;
+            if (s.fw) F[id] = f;
+        }
+        if (mslot || mback) B[id] = nbk;
+        if (mback) {
+#pragma unroll
+            for (int r = 0; r < kGroups; ++r)
+                if ((uint32_t)r < pc) back[nbk + r] = e[r];
+        }
+        if (mslot) {
+            uint32_t bits = w;
+#pragma unroll
+            for (int r = 0; r < kGroups; ++r)
+                if ((uint32_t)r < pc) {
+                    const int j = __ffs(bits) - 1;
+                    bits &= bits - 1u;
+                    reinterpret_cast<uint32_t *>(s.mem + goff[j] + e[r])[0] = id;
+                }
+        }
+        nid += tot >> 16;
+        nb += tot & 0xFFFFu;
+    }
+    // the neighbour table from the live slots: counts and weight sums are
+    // unchanged, keys whose count fell to zero go
+    uint32_t hused = h.hash_used;
+    if (s.nbr && h.hash_cap) {
+        __threadfence_block();
+        tm.sync();
+        uint64_t *tab = s.hash + h.hash_off;
+        for (uint32_t q = (uint32_t)t; q < 2u * h.hash_cap; q += NT) tab[q] = 0ull;
+        __threadfence_block();
+        tm.sync();
+        uint32_t fresh = 0;
+        for (uint32_t i = (uint32_t)t; i < nid; i += NT) fresh += hash_add(tab, h.hash_cap, V[i], 1u, W[i]);
+        tm.excl(fresh, hused);
+    }
+    if (t == 0) {
+        NodeHdr nh = h;
+        nh.nslots = nid;
+        nh.nlive = nid;
+        nh.back_used = nb;
+        nh.hash_used = hused;
+        s.hdr[u] = nh;
+    }
+}
+
+// listed nodes of at most kBigCompact slots: one wave each
+__global__ __launch_bounds__(256) void k_compact_small(DevStore s, const uint32_t *__restrict__ list)
+{
+    __shared__ uint64_t goff[4][kGroups];
+    const Team<64> tm{nullptr, lane_id()};
+    const uint32_t cnt = s.ctl->ncomp[0];
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t k = wave; k < cnt; k += nwaves) compact_node<64>(s, (int32_t)list[k], tm, goff[threadIdx.x >> 6]);
+}
+
+// larger listed nodes: one workgroup of 1024 threads each
+__global__ __launch_bounds__(1024) void k_compact_big(DevStore s, const uint32_t *__restrict__ list)
+{
+    __shared__ uint64_t goff[kGroups];
+    __shared__ uint32_t part[16];
+    const Team<1024> tm{part, (int)threadIdx.x};
+    const uint32_t cnt = s.ctl->ncomp[1];
+    for (uint32_t k = blockIdx.x; k < cnt; k += gridDim.x) {
+        compact_node<1024>(s, (int32_t)list[k], tm, goff);
+        __syncthreads();  // goff and the header are the next node's
     }
 }
 
@@ -1624,7 +1943,7 @@ __global__ __launch_bounds__(256) void k_walk(DevStore s, WalkArgs a)
         int32_t j = 1;
         if (start < 1 || start > s.n || L < 1 || L > a.max_len) {
             j = 0;
-            atomicOr(&s.ctl->error, (unsigned)ERR_INVAL);
+            atomicOr(&s.ctl->werr, (unsigned)ERR_INVAL);
         } else {
             const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
             const uint32_t w0 = (uint32_t)wid, w1 = (uint32_t)((uint64_t)wid >> 32);
@@ -1849,7 +2168,7 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
                             int32_t L = a.lens ? a.lens[w] : a.max_len;
                             if (APP == 3 && L > 2) L = 2;
                             if (st < 1 || st > s.n || L < 1 || L > a.max_len) {
-                                atomicOr(&s.ctl->error, (unsigned)ERR_INVAL);
+                                atomicOr(&s.ctl->werr, (unsigned)ERR_INVAL);
                                 out.finish(a, k, w, 0);
                             } else {
                                 wid[k] = w;
@@ -2110,7 +2429,7 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
                             if (q.app == 3 && L > 2) L = 2;
                             int32_t *row = q.paths + w * (int64_t)q.max_len;
                             if (st < 1 || st > s.n || L < 1 || L > q.max_len || (!MIX && (q.app == 1) != (CLS == kServeN2V))) {
-                                atomicOr(&s.ctl->error, (unsigned)ERR_INVAL);
+                                atomicOr(&s.ctl->werr, (unsigned)ERR_INVAL);
                                 for (int32_t t = 0; t < q.max_len; ++t) row[t] = -1;
                             } else {
                                 stg[k] = q.max_len % kServeD == 0 && ((uintptr_t)q.paths & 15u) == 0;
@@ -2457,19 +2776,38 @@ void launch_gather_edges(const DevEdge *e, const uint32_t *order, int64_t k, Dev
     hipLaunchKernelGGL(k_gather_edges, dim3(blocks_for(k, 256)), dim3(256), 0, st, e, order, k, out, frac);
 }
 
-size_t insert_wres_bytes(int64_t k) { return (size_t)run_blocks(k) * 4 * NPOOL * sizeof(unsigned long long); }
+size_t insert_wcls_bytes(int64_t k) { return (size_t)run_blocks(k) * 2 * kPC * sizeof(uint32_t); }
 
-void launch_insert_plan(DevStore s, const DevEdge *sorted, Runs r, unsigned long long *wres, int64_t k,
-                        hipStream_t st)
+__global__ void k_alloc_reset(FreeMeta *fm)
 {
-    hipLaunchKernelGGL(k_insert_plan, dim3(run_blocks(k)), dim3(256), 0, st, s, sorted, r, wres);
+    const int t = threadIdx.x;
+    if (t < kPC) {
+        fm->take[t] = 0u;
+        fm->freed[t] = 0u;
+    }
 }
 
-void launch_insert_apply(DevStore s, const DevEdge *sorted, const uint32_t *frac, Runs r,
-                         const unsigned long long *wres, int64_t k, hipStream_t st)
+void launch_alloc_reset(FreeMeta *fm, hipStream_t st) { hipLaunchKernelGGL(k_alloc_reset, dim3(1), dim3(kPC), 0, st, fm); }
+
+void launch_alloc_resolve(DevStore s, int fw, hipStream_t st)
 {
-    // same grid as the plan: wave w uses the reservation wave w made
-    hipLaunchKernelGGL(k_insert_apply, dim3(run_blocks(k)), dim3(256), 0, st, s, sorted, frac, r, wres);
+    hipLaunchKernelGGL(k_alloc_resolve, dim3(1), dim3(kPC), 0, st, s, fw);
+}
+
+void launch_freelist_merge(DevStore s, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_freelist_merge, dim3(kPC), dim3(256), 0, st, s);
+}
+
+void launch_insert_plan(DevStore s, const DevEdge *sorted, Runs r, int64_t k, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_insert_plan, dim3(run_blocks(k)), dim3(256), 0, st, s, sorted, r);
+}
+
+void launch_insert_apply(DevStore s, const DevEdge *sorted, const uint32_t *frac, Runs r, int64_t k, hipStream_t st)
+{
+    // same grid as the plan: workgroup b takes the index ranges workgroup b reserved
+    hipLaunchKernelGGL(k_insert_apply, dim3(run_blocks(k)), dim3(256), 0, st, s, sorted, frac, r);
     hipLaunchKernelGGL(k_commit_tops, dim3(1), dim3(64), 0, st, s.ctl);
 }
 
@@ -2648,6 +2986,19 @@ void launch_export_b(DevStore s, const uint64_t *slot_out_off, const uint64_t *m
     const int64_t waves = (int64_t)s.n + 1;
     hipLaunchKernelGGL(k_export_b, dim3(blocks_for(waves * 64, 256)), dim3(256), 0, st, s, slot_out_off, mem_out_off,
                        slot_v, slot_w, members, backpos, slot_f);
+}
+
+void launch_compact(DevStore s, uint32_t thr16, uint32_t *small_list, uint32_t *big_list, hipStream_t st)
+{
+    int dev = 0, ncu = 1;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    hipLaunchKernelGGL(k_compact_reset, dim3(1), dim3(64), 0, st, s.ctl);
+    const int64_t n1 = (int64_t)s.n + 1;
+    hipLaunchKernelGGL(k_compact_select, dim3(blocks_for(n1, 256)), dim3(256), 0, st, s, thr16, small_list, big_list);
+    const unsigned sb = (unsigned)std::min<int64_t>((n1 + 3) / 4, (int64_t)ncu * 8);
+    hipLaunchKernelGGL(k_compact_small, dim3(sb), dim3(256), 0, st, s, small_list);
+    hipLaunchKernelGGL(k_compact_big, dim3((unsigned)ncu), dim3(1024), 0, st, s, big_list);
 }
 
 void launch_edge_count(DevStore s, const int32_t *us, const int32_t *vs, int64_t k, uint32_t *counts, hipStream_t st)

@@ -35,11 +35,13 @@ enum ErrFlag { ERR_INVAL = 1, ERR_NOMEM = 2, ERR_OVERRUN = 4 };
 
 struct NodeHdr {
     uint64_t slot_off;
-    uint32_t nslots, slot_cap;
+    uint32_t nslots, slot_cap;  // slots ever assigned (tombstones included), capacity
     uint64_t back_off;
     uint32_t back_used, back_cap;
     uint64_t hash_off;
     uint32_t hash_cap, hash_used;
+    uint32_t nlive, spare;       // live slots (nslots - nlive tombstones, reclaimed by compaction)
+    uint64_t spare2;
 };
 
 struct GroupHdr {
@@ -90,7 +92,7 @@ constexpr int kOffBits = 36;
 constexpr uint64_t kOffMask36 = (1ull << kOffBits) - 1ull;
 constexpr uint32_t kMaxGroupSize = 1u << 28;  // sizes live in 28 bits
 static_assert(sizeof(AliasEnt) == 32, "alias entry is one 32-byte read");
-static_assert(sizeof(NodeHdr) == 48, "node header layout");
+static_assert(sizeof(NodeHdr) == 64, "node header layout");
 
 // slot arrays per node region: v, w, back offset (and fraction)
 __host__ __device__ inline uint32_t slot_arrays(int32_t fw) { return fw ? 4u : 3u; }
@@ -106,8 +108,52 @@ struct Ctl {
     unsigned long long folds;        // last walk: node2vec return-weight lookups (neighbour entries)
     unsigned long long cursor;       // persistent walk: next unclaimed walker
     unsigned int nbigcopy;           // insert plan: runs that move a region of kBigCopyBytes or more
-    unsigned int pad;
+    unsigned int werr;               // walks since the last walk reset: ERR_INVAL (a start or length out of range)
     unsigned long long taskctr;      // insert apply: region moves left to k_copy_tasks << 32 | their chunks
+    unsigned int ncomp[2];           // compaction: nodes listed for the wave path and the workgroup path
+    unsigned long long comp_freed;   // compaction: tombstoned slots reclaimed
+    unsigned long long fl_total;     // insert batch: free-list entries after the batch
+    unsigned long long pend_total;   // insert batch: regions the batch frees
+};
+
+// Region sizes and recycling.  Every region of pool p has a capacity of a
+// size class: class k = 8 q + r (r < 8) holds S(k) = floor((8 + r) 2^q / 8)
+// units -- 8 classes per power of two, neighbours at most 1/8 apart -- and a
+// unit is kMult[p] elements (slot regions: one per slot array; neighbour
+// tables: 2 pool words per entry).  A region that moves out is freed to the
+// list of its class.  An insert batch's plan counts, per workgroup, its
+// allocations and frees per (pool, class); the batch's allocations of a
+// class take the class's free regions first and bump the pool past them;
+// its frees become available to the next batch (a moved region is still
+// read by the batch's deferred copies).  Index t = pool * kClasses + class.
+constexpr int kClasses = 256;
+constexpr int kPC = NPOOL * kClasses;
+__host__ __device__ constexpr uint32_t class_size(uint32_t k)
+{
+    return (uint32_t)((((uint64_t)8u + (k & 7u)) << (k >> 3)) >> 3);
+}
+// the largest class whose size is at most c (c >= 1)
+__host__ __device__ inline uint32_t class_down(uint32_t c)
+{
+    const uint32_t q = 31u - (uint32_t)__builtin_clz(c);
+    const uint32_t r = q >= 3 ? (c >> (q - 3)) - 8u : (c << (3 - q)) - 8u;
+    return 8u * q + r;
+}
+// the smallest class whose size is at least c (1 <= c <= S(255) = 15 2^28)
+__host__ __device__ inline uint32_t class_up(uint32_t c)
+{
+    const uint32_t k = class_down(c);
+    return class_size(k) == c ? k : k + 1u;
+}
+struct FreeMeta {
+    uint32_t avail[kPC];   // free regions of the class: list entries [fbase, fbase + avail)
+    uint32_t take[kPC];    // this batch: allocations of the class (plan)
+    uint32_t freed[kPC];   // this batch: regions of the class freed
+    uint32_t nav[kPC];     // this batch: free regions after it
+    uint64_t fbase[kPC];   // list segment of the class
+    uint64_t nfbase[kPC];  // list segment after the batch
+    uint64_t bump[kPC];    // this batch: pool offset of the class's allocations past avail
+    uint64_t pseg[kPC];    // this batch: pending-list segment of the class's frees
 };
 
 // A region move of an insert batch that is too large for the run's own wave:
@@ -141,10 +187,10 @@ struct DevStore {
     int32_t n;
     int32_t nbr;
     int32_t min_slots;
-    // Growth rule of the call in flight.  0: a region that must grow gets the
-    // next power of two (slots: at least min_slots).  Otherwise regions are
-    // sized to their contents plus ceil(size * fit / 2^16) (a load that fits
-    // the store to the graph); neighbour tables stay powers of two.
+    // Growth rule.  0: a region that must grow gets the next power of two
+    // (slots: at least min_slots).  Otherwise (a fitted store) it gets the
+    // smallest size class holding its contents plus ceil(size * (fit - 1) /
+    // 2^16); neighbour tables stay powers of two.
     uint32_t fit;
     int32_t fw;      // weights are float bits: slot regions carry a fraction array
     int32_t pad;
@@ -160,6 +206,11 @@ struct DevStore {
     unsigned long long cap[NPOOL];
     Ctl *ctl;
     CopyTask *tasks;  // [kMaxCopyTasks]
+    FreeMeta *fm;
+    uint64_t *fl;     // free regions, per class segments (FreeMeta::fbase)
+    uint64_t *fl2;    // the lists after the current batch
+    uint64_t *pf;     // regions freed by the current batch, per class segments (FreeMeta::pseg)
+    uint32_t *wcls;   // insert batch: per workgroup, its first allocation and free index per class (2 kPC)
 };
 
 struct WalkArgs {
@@ -217,14 +268,18 @@ void launch_insert_keys(const DevEdge *e, int64_t k, int32_t n, int fw, uint32_t
                         hipStream_t st);
 void launch_gather_edges(const DevEdge *e, const uint32_t *order, int64_t k, DevEdge *out, uint32_t *frac,
                          hipStream_t st);
-// The plan reserves each wave's pool space (wres: insert_wres_bytes(k)); the
-// apply, launched for the same k, allocates from it and then moves the pool
-// tops past the batch's total (Ctl::need).
-size_t insert_wres_bytes(int64_t k);
-void launch_insert_plan(DevStore s, const DevEdge *sorted, Runs r, unsigned long long *wres, int64_t k,
-                        hipStream_t st);
-void launch_insert_apply(DevStore s, const DevEdge *sorted, const uint32_t *frac, Runs r,
-                         const unsigned long long *wres, int64_t k, hipStream_t st);
+// The plan counts each workgroup's region allocations and frees per class
+// (s.wcls: insert_wcls_bytes(k)); launch_alloc_resolve then sets Ctl::need
+// (the batch's bump space per pool), fl_total and pend_total; the apply,
+// launched for the same k, takes its regions and moves the pool tops past
+// the bump space; after it (and its copy tasks) launch_freelist_merge writes
+// the free lists to s.fl2 (the caller swaps fl and fl2).
+size_t insert_wcls_bytes(int64_t k);
+void launch_alloc_reset(FreeMeta *fm, hipStream_t st);  // before a batch's plan
+void launch_alloc_resolve(DevStore s, int fw, hipStream_t st);
+void launch_freelist_merge(DevStore s, hipStream_t st);
+void launch_insert_plan(DevStore s, const DevEdge *sorted, Runs r, int64_t k, hipStream_t st);
+void launch_insert_apply(DevStore s, const DevEdge *sorted, const uint32_t *frac, Runs r, int64_t k, hipStream_t st);
 // neighbour multisets (nbr stores): the batch's arcs, one thread each, after
 // the insert apply kernel; the batch's deletes before the delete apply kernel
 // (their slot weights are read before the tombstones)
@@ -257,6 +312,12 @@ void launch_export_a(DevStore s, uint32_t *nslots, uint64_t *mass, uint32_t *bsi
                      uint64_t *fmass, uint64_t *fthr, hipStream_t st);
 void launch_export_b(DevStore s, const uint64_t *slot_out_off, const uint64_t *mem_out_off, uint32_t *slot_v,
                      uint32_t *slot_w, uint64_t *members, uint32_t *backpos, uint32_t *slot_f, hipStream_t st);
+// Compaction of tombstoned slots: nodes with dead = nslots - nlive >= 1 and
+// dead 2^16 >= thr16 nslots are listed (small: one wave each, big: one
+// workgroup each, by slot count; room for n entries in each list) and
+// rewritten with their live slots renumbered by rank.
+constexpr uint32_t kBigCompact = 8192;
+void launch_compact(DevStore s, uint32_t thr16, uint32_t *small_list, uint32_t *big_list, hipStream_t st);
 void launch_edge_count(DevStore s, const int32_t *us, const int32_t *vs, int64_t k, uint32_t *counts,
                        hipStream_t st);
 

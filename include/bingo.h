@@ -101,6 +101,7 @@ typedef struct {
     uint64_t arcs;          /* live arcs */
     uint64_t bytes_reserved;/* HBM held by the store */
     uint64_t pool_used[4];  /* slot, back-index, member, neighbour pools (elements) */
+    double compact_ms;      /* HIP-event time of the last bingo_compact */
 } bingo_stats;
 
 /* Flat export for bit-exact diffs (node-major; see bingo_export). */
@@ -147,6 +148,20 @@ int bingo_insert_device(bingo_graph *g, const bingo_edge *d_edges, int64_t k);
 int bingo_delete(bingo_graph *g, const bingo_del *dels, int64_t k);
 int bingo_delete_device(bingo_graph *g, const bingo_del *d_dels, int64_t k);
 
+/* ---- compaction of tombstoned slots (no reference counterpart: the
+ * reference tombstones a deleted slot and never reuses it,
+ * include/operation.cuh:209-210, so a node's slot array only grows) ----
+ * Every node u with dead = nslots(u) - live(u) >= 1 and
+ *     dead * 2^16 >= floor(min_dead_frac * 2^16) * nslots(u)
+ * is compacted in place: its k-th live slot (in slot order) becomes slot k,
+ * and nslots(u) becomes live(u).  Bucket order, masses and walk laws are
+ * unchanged; deletes after the call name the new slot numbers, and new arcs
+ * take slots from live(u) on.  A caller that tracks slot numbers applies the
+ * same rule (bingo_updgen_compact does).  nodes_out / slots_out (may be
+ * NULL; either makes the call wait for the device) receive the number of
+ * nodes compacted and of slots reclaimed. */
+int bingo_compact(bingo_graph *g, double min_dead_frac, uint64_t *nodes_out, uint64_t *slots_out);
+
 /* ---- walks: ref randomWalk() (graph.cuh:93-150) and the kernels of
  * operation.cuh:252-368 (DeepWalk, node2vec, PPR) and 407-430 (sampling).
  * paths is [nwalkers][params->max_len] int32, -1 padded.  starts NULL:
@@ -155,7 +170,11 @@ int bingo_delete_device(bingo_graph *g, const bingo_del *d_dels, int64_t k);
 int bingo_walk(bingo_graph *g, int app, const bingo_walk_params *params, const int32_t *starts,
                const int32_t *lens, int64_t nwalkers, int32_t *paths, uint64_t *steps_out);
 /* device-pointer form, enqueued on the handle's stream; *d_steps (device
- * uint64) is incremented by the steps.  Returns without waiting. */
+ * uint64) is incremented by the steps.  Returns without waiting.  Starts and
+ * lengths are checked on the device: a walker whose start is outside 1..n or
+ * whose length is outside 1..max_len gets a row of -1, and the next
+ * bingo_sync returns BINGO_EINVAL (the flag clears when the next walk starts;
+ * walk graph launches report the same way). */
 int bingo_walk_device(bingo_graph *g, int app, const bingo_walk_params *params,
                       const int32_t *d_starts, const int32_t *d_lens, int64_t nwalkers,
                       int32_t *d_paths, uint64_t *d_steps);
@@ -183,6 +202,8 @@ int bingo_walk_tune(bingo_graph *g, int app, const bingo_walk_params *params, in
 /* ---- inspection ---- */
 int32_t bingo_num_nodes(const bingo_graph *g);
 int bingo_stats_get(bingo_graph *g, bingo_stats *out);
+/* waits for the handle's stream; BINGO_EINVAL if the last walk had an
+ * invalid device-side start or length, BINGO_EDEVICE if a batch overran a pool */
 int bingo_sync(bingo_graph *g);
 void *bingo_stream(bingo_graph *g);      /* the handle's hipStream_t */
 /* multiplicities of live arcs us[i] -> vs[i] (host arrays) */
@@ -296,6 +317,9 @@ int bingo_updgen_deletes(bingo_updgen *s, int64_t k, bingo_del *out);
  * generator uses either this or bingo_updgen_deletes */
 int bingo_updgen_expire(bingo_updgen *s, int64_t k, bingo_del *out);
 int64_t bingo_updgen_live(const bingo_updgen *s);
+/* renumber the generator's slots as bingo_compact(g, min_dead_frac) renumbers
+ * the store's (call both at the same point of the update stream) */
+int bingo_updgen_compact(bingo_updgen *s, double min_dead_frac);
 
 #ifdef __cplusplus
 }

@@ -457,6 +457,55 @@ int orc_delete(orc_graph *g, const orc_del *d, int64_t k)
     return ORC_OK;
 }
 
+/* Compaction of tombstoned slots (the reference never reclaims them,
+ * operation.cuh:209-210).  Node u is compacted when dead = nslots - live >= 1
+ * and dead * 2^16 >= floor(min_dead_frac * 2^16) * nslots: its live slots
+ * keep their order and take the numbers 0..live-1, their back-index entries
+ * are packed in the same order, and every member's slot field takes the new
+ * number.  Bucket order, masses and the alias table do not change.  Returns
+ * the number of nodes compacted (ORC_EINVAL for a fraction outside [0, 1]). */
+int64_t orc_compact(orc_graph *g, double min_dead_frac, uint64_t *slots_freed)
+{
+    if (!(min_dead_frac >= 0 && min_dead_frac <= 1)) return ORC_EINVAL;
+    const uint64_t thr16 = (uint64_t)floor(min_dead_frac * 65536.0);
+    int64_t nodes = 0;
+    uint64_t freed = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : nodes, freed)
+    for (int32_t u = 1; u <= g->n; ++u) {
+        node_t *x = &g->nodes[u];
+        uint32_t live = 0;
+        for (uint32_t s = 0; s < x->nslots; ++s) live += (x->w[s] || (x->fr && x->fr[s])) ? 1u : 0u;
+        const uint32_t dead = x->nslots - live;
+        if (!dead || (uint64_t)dead * 65536u < thr16 * x->nslots) continue;
+        uint32_t ns = 0, nb = 0;
+        for (uint32_t s = 0; s < x->nslots; ++s) {
+            const uint32_t w = x->w[s];
+            if (!w && !(x->fr && x->fr[s])) continue;
+            const uint32_t b = x->boff[s], pc = (uint32_t)popc32(w);
+            uint32_t bits = w;
+            for (uint32_t r = 0; r < pc; ++r) {  /* nb <= b: a forward copy */
+                const int j = __builtin_ctz(bits);
+                bits &= bits - 1;
+                const uint32_t pos = x->back[b + r];
+                x->back[nb + r] = pos;
+                x->mem[j][pos] = (x->mem[j][pos] & 0xFFFFFFFF00000000ull) | ns;
+            }
+            x->v[ns] = x->v[s];
+            x->w[ns] = w;
+            if (x->fr) x->fr[ns] = x->fr[s];
+            x->boff[ns] = nb;
+            nb += pc;
+            ++ns;
+        }
+        x->nslots = ns;
+        x->back_used = nb;
+        nodes += 1;
+        freed += dead;
+    }
+    if (slots_freed) *slots_freed = freed;
+    return nodes;
+}
+
 uint32_t orc_edge_count(const orc_graph *g, int32_t u, int32_t v)
 {
     if (u < 1 || u > g->n || v < 1 || v > g->n) return 0;

@@ -85,6 +85,8 @@ def lib():
         L.orc_fold_threshold.restype = C.c_uint64
         L.orc_fold_threshold.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
         L.orc_export.argtypes = [vp, C.POINTER(Dump)]
+        L.orc_compact.restype = i64
+        L.orc_compact.argtypes = [vp, C.c_double, C.POINTER(u64)]
         L.orc_edge_count.restype = C.c_uint32
         L.orc_edge_count.argtypes = [vp, i32, i32]
         L.orc_alias_build.argtypes = [vp, vp, vp]
@@ -221,6 +223,15 @@ class OracleGraph:
     def delete(self, dels):
         d = np.ascontiguousarray(dels, DEL_DT)
         return lib().orc_delete(self._h, _ptr(d), len(d))
+
+    def compact(self, min_dead_frac):
+        """Renumber live slots of nodes past the tombstone threshold; returns
+        (nodes compacted, slots reclaimed)."""
+        freed = C.c_uint64(0)
+        nodes = lib().orc_compact(self._h, min_dead_frac, C.byref(freed))
+        if nodes < 0:
+            raise ValueError("min_dead_frac outside [0, 1]")
+        return nodes, freed.value
 
     def walk(self, app, params: WalkParams, starts=None, lens=None, nwalkers=None, nthreads=8):
         if starts is not None:
