@@ -3,13 +3,16 @@
 A step is one round of the dynamic-graph workload: one insert batch, one
 delete batch, then one walk of every walker, all on HBM-resident inputs.
 
-Default workload `c2` (R-MAT 2^22 nodes / 2^26 undirected edges, u32 weights
-log-uniform in [1, 2^32), node2vec p=0.5 q=2, one walker per node of length
-80, 10^6 inserts + 10^6 deletes per step).  `value` is whole-job walk steps/s
-(walk steps / wall time of the K steps, updates included); the line also
-carries walk-only steps/s, updates/s, the walk kernel's roofline, the plain
-per-launch form on the same graph, and the CPU oracle timed on this host's
-cores on a bounded sample.  `c3` is the batched workload (R-MAT 2^23 / 2^27,
+Default: the served form (`c4`) line, then the headline line of the scale
+workload `c5` (R-MAT 2^25 nodes / 2^30 arcs, u32 weights log-uniform in
+[1, 2^32), 3.2e7 DeepWalk walkers of length 80, a sliding window of 10^7
+inserts + 10^7 expiries of the oldest arcs and a compaction per step).
+`value` is whole-job walk steps/s (walk steps / wall time of the K steps,
+updates and compaction included); the line also carries walk-only steps/s,
+updates/s, the walk kernel's roofline, the plain per-launch form on the same
+graph, and the CPU oracle (built -march=native) timed on this host's cores:
+load, update rounds and walks of a bounded sample, medians of 3.  `c2` is the
+node2vec workload (R-MAT 2^22 / 2^26 undirected edges, 10^6 + 10^6 updates).  `c3` is the batched workload (R-MAT 2^23 / 2^27,
 Zipf weights, 10^7 PPR walkers with lengths in [5, 200], update batches of
 log-uniform size), replayed from a captured HIP graph; `c1` the small one.
 
@@ -63,7 +66,7 @@ CONFIGS = {
     "c5": dict(workload="rmat25_a30_loguni32_deepwalk_w32e6_churn1e7", scale=25, edges=1 << 30, wkind=2, app=0,
                max_len=80, batch=10_000_000, p=0.5, q=2.0, tp=0.0125, walkers=32_000_000, lens=None, graph=False,
                directed=True, fit=True, churn="expire", cpu_sample_edges=1 << 26, grow_in_place=True,
-               compact=1 / 32),
+               compact=1 / 16),
 }
 
 
@@ -115,43 +118,86 @@ def to_float_bits(edges):
     return e
 
 
-def cpu_baseline(base, n, cfg, seed, budget_s=15.0):
-    """The CPU oracle (a C/OpenMP restatement of the same sampler) on this
-    host's cores: full graph (or, where the configuration names a sample,
-    the first arcs of the same generator stream), a bounded sample of
-    walkers."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(B, base, n, cfg, seed, walk_budget_s=4.0, runs=3):
+    """The CPU oracle (a C/OpenMP restatement of the same store, updates and
+    sampler), built -march=native on this host and run on all of its cores:
+    the graph load, `runs` update rounds (insert batch, delete batch and, where
+    the configuration compacts, the compaction) and `runs` walks of a bounded
+    walker sample, each timed alone; the medians are reported.  The graph is
+    the configuration's own or, where the configuration names a sample, the
+    first arcs of the same generator stream, with update batches scaled by
+    the sample's share of the arcs."""
     from oracle import oracle as orc
 
+    flags = orc.use_native()
     threads = len(os.sched_getaffinity(0))
     graph_note = "the same graph"
+    batch = cfg["batch"] if isinstance(cfg["batch"], int) else int(round(np.sqrt(cfg["batch"][0] * cfg["batch"][1])))
+    m_full = cfg["edges"] * (1 if cfg.get("directed") else 2)
     if cfg.get("cpu_sample_edges"):
         k = cfg["cpu_sample_edges"]
-        base = orc.gen_rmat(cfg["scale"], k, cfg["wkind"], seed=seed, undirected=not cfg.get("directed"),
-                            nthreads=threads)
+        base = B.gen_rmat(cfg["scale"], k, cfg["wkind"], seed=seed, undirected=not cfg.get("directed"))
+        batch = max(1, int(batch * len(base) / m_full))
         graph_note = (f"the first {len(base)} arcs of the same R-MAT stream and node space (the full graph's "
-                      f"CPU store would not fit host memory)")
+                      f"CPU store would not fit host memory), update batches scaled to {batch}")
     if cfg.get("float"):
         base = to_float_bits(base)
     t0 = time.perf_counter()
     o = orc.OracleGraph(n, neighbor_index=cfg["app"] == 1, float_weights=bool(cfg.get("float")))
     assert o.load(base) == 0
     load_s = time.perf_counter() - t0
-    log(f"cpu oracle loaded in {load_s:.1f}s ({threads} threads)")
+    log(f"cpu oracle ({flags}) loaded {len(base)} arcs in {load_s:.1f}s ({threads} threads)")
+    ug = B.UpdateGen(n, cfg["scale"], base, len(base) // (1 if cfg.get("directed") else 2), cfg["wkind"], seed=seed)
+    upd_s = []
+    for r in range(runs):
+        ins = ug.inserts(batch)
+        if cfg.get("float"):
+            ins = to_float_bits(ins)
+        dels = ug.expire(batch) if cfg.get("churn") == "expire" else ug.deletes(batch)
+        if cfg.get("compact"):
+            ug.compact(cfg["compact"])
+        t0 = time.perf_counter()
+        assert o.insert(ins) == 0 and o.delete(dels) == 0
+        if cfg.get("compact"):
+            o.compact(cfg["compact"])
+        upd_s.append(time.perf_counter() - t0)
+    del ug
     wp = orc.walk_params(seed=seed, p=cfg["p"], q=cfg["q"], tp=cfg["tp"], max_len=cfg["max_len"],
                          float_weights=bool(cfg.get("float")))
     nw = 4096
-    while True:
+    while True:  # calibrate the walker sample to about walk_budget_s
         starts = (np.arange(nw, dtype=np.int64) * (n // nw) % n + 1).astype(np.int32)
         t0 = time.perf_counter()
-        _, steps = o.walk(cfg["app"], wp, starts=starts, lens=walker_lens(cfg, nw, seed), nthreads=threads)
+        o.walk(cfg["app"], wp, starts=starts, lens=walker_lens(cfg, nw, seed), nthreads=threads)
         dt = time.perf_counter() - t0
-        if dt > budget_s / 4 or nw >= n:
+        if dt > walk_budget_s / 4 or nw >= n:
             break
         nw = min(n, nw * 4)
+    walk_rates = []
+    for r in range(runs):
+        t0 = time.perf_counter()
+        _, steps = o.walk(cfg["app"], wp, starts=starts, lens=walker_lens(cfg, nw, seed), nthreads=threads)
+        walk_rates.append(steps / (time.perf_counter() - t0))
     o.close()
-    return {"value": steps / dt, "unit": "walk steps/s", "cores": threads, "kind": "port",
-            "sample": f"{nw} {APP_NAMES[cfg['app']]} walkers (max length {cfg['max_len']}) spread over "
-                      f"{graph_note}, {steps} steps in {dt:.2f}s on {threads} threads (load {load_s:.1f}s not counted)"}
+    walk = float(np.median(walk_rates))
+    upd = 2 * batch / float(np.median(upd_s))
+    return {"value": walk, "unit": "walk steps/s", "cores": threads, "kind": "port",
+            "updates_per_s": upd, "load_arcs_per_s": len(base) / load_s,
+            "cpu_model": cpu_model(), "build": flags, "runs": runs, "statistic": "median",
+            "sample": f"{nw} {APP_NAMES[cfg['app']]} walkers (max length {cfg['max_len']}) over {graph_note}; "
+                      f"{steps} steps per walk run, update rounds of {batch} inserts + {batch} "
+                      f"{'expiries' if cfg.get('churn') == 'expire' else 'deletes'}"
+                      f"{' + compaction' if cfg.get('compact') else ''}; {threads} threads"}
 
 
 def served(B, args):
@@ -307,7 +353,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c4"])
+    ap.add_argument("--config", default=None, choices=sorted(CONFIGS) + ["c4"],
+                    help="one workload; default: the served form (c4) line, then the scale workload (c5) line")
     ap.add_argument("--rate", type=float, default=2000.0, help="c4: offered requests per second")
     ap.add_argument("--requests", type=int, default=300, help="c4: number of requests")
     ap.add_argument("--no-cpu", action="store_true")
@@ -315,6 +362,8 @@ def main():
     ap.add_argument("--float-weights", action="store_true", help="float32 weights w / 256 (wtype::Float)")
     ap.add_argument("--grow-in-place", type=int, choices=(0, 1), default=None,
                     help="pools grow by mapping more HBM (1) or by copying (0); default: the config's choice")
+    ap.add_argument("--compact", type=float, default=None, metavar="FRAC",
+                    help="compaction threshold (tombstones per slot) after every step; default: the config's")
     ap.add_argument("--schedule", default="", metavar="BLOCK,WPL,DEPTH",
                     help="fixed persistent walk knobs instead of autotuning (e.g. 512,2,16)")
     ap.add_argument("--seed", type=int, default=2026)
@@ -333,11 +382,16 @@ def main():
     if args.config == "c4":
         served(B, args)
         return
+    if args.config is None:  # the served line first, the headline (scale workload) last
+        served(B, args)
+        args.config = "c5"
 
     cfg = dict(CONFIGS[args.config])
     if args.app and APP_NAMES.index(args.app) != cfg["app"]:
         cfg["app"] = APP_NAMES.index(args.app)
         cfg["workload"] += f"_app_{args.app}"
+    if args.compact is not None:
+        cfg["compact"] = args.compact or None
     if args.float_weights:
         cfg["float"] = True
         cfg["workload"] += "_float32w"
@@ -527,8 +581,9 @@ def main():
     d_paths.free()
     g.close()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(base, n, cfg, args.seed)
+        out["cpu_baseline"] = cpu_baseline(B, base, n, cfg, args.seed)
         out["gpu_vs_cpu_walk"] = out["walk_steps_per_s"] / out["cpu_baseline"]["value"]
+        out["gpu_vs_cpu_updates"] = out["updates_per_s"] / out["cpu_baseline"]["updates_per_s"]
     print(json.dumps(out), flush=True)
 
 

@@ -1045,14 +1045,35 @@ int bingo_compact(bingo_graph *g, double min_dead_frac, uint64_t *nodes_out, uin
     if ((rc = ensure_scratch(g, SC_CLIST, 2 * n1 * sizeof(uint32_t)))) return rc;
     uint32_t *lists = scp<uint32_t>(g, SC_CLIST);
     phase_begin(g, PH_COMPACT);
-    launch_compact(g->s, thr16, lists, lists + n1, g->st);
+    launch_compact_select(g->s, thr16, lists, lists + n1, g->st);
+    HIPCHK(hipGetLastError());
+    // the listed counts size the big nodes' staging scratch
+    if ((rc = read_ctl(g))) return rc;
+    const Ctl c = *g->ctl_host;
+    const uint32_t nsmall = c.ncomp[0], nbig = c.ncomp[1];
+    CompactBig cb{};
+    if (nbig) {
+        auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+        const size_t sz[] = {al((nbig + 1) * 4ull), al(c.comp_nseg * 4), al(c.comp_nseg * 4), al(nbig * 8ull),
+                             al(nbig * 8ull),       al(nbig * 4ull),    al(nbig * 4ull),    al(c.comp_bigslots * 16),
+                             al(c.comp_bigback * 4)};
+        size_t tot = 0;
+        for (size_t x : sz) tot += x;
+        if ((rc = ensure_scratch(g, SC_CBIG, tot))) return rc;
+        char *p = static_cast<char *>(g->sc[SC_CBIG]);
+        void **dst[] = {(void **)&cb.segbase, (void **)&cb.seg_live, (void **)&cb.seg_pc, (void **)&cb.trec,
+                        (void **)&cb.tback,   (void **)&cb.nlive,    (void **)&cb.nback,  (void **)&cb.rec,
+                        (void **)&cb.back};
+        for (int i = 0; i < 9; ++i) {
+            *dst[i] = p;
+            p += sz[i];
+        }
+    }
+    launch_compact_apply(g->s, lists, lists + n1, nsmall, nbig, c.comp_nseg, cb, g->st);
     phase_end(g, PH_COMPACT);
     HIPCHK(hipGetLastError());
-    if (nodes_out || slots_out) {
-        if ((rc = read_ctl(g))) return rc;
-        if (nodes_out) *nodes_out = (uint64_t)g->ctl_host->ncomp[0] + g->ctl_host->ncomp[1];
-        if (slots_out) *slots_out = g->ctl_host->comp_freed;
-    }
+    if (nodes_out) *nodes_out = (uint64_t)nsmall + nbig;
+    if (slots_out) *slots_out = c.comp_freed;
     return BINGO_OK;
 }
 

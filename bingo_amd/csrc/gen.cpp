@@ -185,14 +185,19 @@ void bingo_updgen_destroy(bingo_updgen *s) { delete s; }
 int bingo_updgen_inserts(bingo_updgen *s, int64_t k, bingo_edge *out)
 {
     if (!s || k < 0 || (k && !out)) return BINGO_EINVAL;
-    for (int64_t i = 0; i < k; ++i) {
-        const uint64_t e = s->next_edge++;
+    const uint64_t e0 = s->next_edge;
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+    for (int64_t i = 0; i < k; ++i) {  // arcs are pure functions of their edge index
+        const uint64_t e = e0 + (uint64_t)i;
         int32_t u, v;
         rmat_arc(s->sp, e, &u, &v);
-        if (u > s->n || v > s->n) return BINGO_EINVAL;
+        bad |= u > s->n || v > s->n;
         out[i] = bingo_edge{u, v, weight_of(s->sp.wkind, s->sp.seed, e)};
-        s->push(u);
     }
+    if (bad) return BINGO_EINVAL;
+    s->next_edge = e0 + (uint64_t)k;
+    for (int64_t i = 0; i < k; ++i) s->push(out[i].u);  // slots in arrival order
     return BINGO_OK;
 }
 

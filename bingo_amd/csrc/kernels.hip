@@ -1617,6 +1617,7 @@ __global__ void k_compact_reset(Ctl *ctl)
     if (threadIdx.x == 0) {
         ctl->ncomp[0] = ctl->ncomp[1] = 0;
         ctl->comp_freed = 0;
+        ctl->comp_bigslots = ctl->comp_bigback = ctl->comp_nseg = 0;
     }
 }
 
@@ -1625,12 +1626,14 @@ __global__ __launch_bounds__(256) void k_compact_select(DevStore s, uint32_t thr
 {
     const int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     bool pick = false, big = false;
-    uint32_t dead = 0;
+    uint32_t dead = 0, ns = 0, nbk = 0;
     if (u >= 1 && u <= s.n) {
         const NodeHdr h = s.hdr[u];
         dead = h.nslots - h.nlive;
         pick = dead >= 1u && (uint64_t)dead * 65536u >= (uint64_t)thr16 * h.nslots;
         big = h.nslots > kBigCompact;
+        ns = h.nslots;
+        nbk = h.back_used;
     }
     const int ln = lane_id();
     const uint64_t lt = (1ull << ln) - 1ull;
@@ -1646,6 +1649,16 @@ __global__ __launch_bounds__(256) void k_compact_select(DevStore s, uint32_t thr
     if (pick && big) big_list[bb + (uint32_t)__popcll(mb & lt)] = (uint32_t)u;
     const uint32_t fr = wave_sum32(pick ? dead : 0u);
     if (ln == 0 && fr) atomicAdd(&s.ctl->comp_freed, (unsigned long long)fr);
+    if (mb) {  // big nodes: slots, back entries and segments, for the staging scratch
+        const bool b = pick && big;
+        const uint32_t sl = wave_sum32(b ? ns : 0u), bk = wave_sum32(b ? nbk : 0u),
+                       sg = wave_sum32(b ? (ns + kCompactSeg - 1) / kCompactSeg : 0u);
+        if (ln == 0) {
+            atomicAdd(&s.ctl->comp_bigslots, (unsigned long long)sl);
+            atomicAdd(&s.ctl->comp_bigback, (unsigned long long)bk);
+            atomicAdd(&s.ctl->comp_nseg, (unsigned long long)sg);
+        }
+    }
 }
 
 // A team of NT threads working on one node: one wave (NT = 64) or one
@@ -1780,16 +1793,226 @@ __global__ __launch_bounds__(256) void k_compact_small(DevStore s, const uint32_
     for (uint32_t k = wave; k < cnt; k += nwaves) compact_node<64>(s, (int32_t)list[k], tm, goff[threadIdx.x >> 6]);
 }
 
-// larger listed nodes: one workgroup of 1024 threads each
-__global__ __launch_bounds__(1024) void k_compact_big(DevStore s, const uint32_t *__restrict__ list)
+// ---- big nodes: segments of kCompactSeg slots over the whole grid ----
+// Renumbering needs each segment's live count and back entries before it;
+// the in-place moves of one node would collide across workgroups, so the
+// live records and back entries are staged out (with the members' slot
+// fields patched on the way) and written back by a second kernel.
+
+// the big node a segment belongs to (largest i with segbase[i] <= seg)
+__device__ __forceinline__ uint32_t seg_node(const uint32_t *segbase, uint32_t nbig, uint32_t seg)
 {
-    __shared__ uint64_t goff[kGroups];
+    uint32_t lo = 0, hi = nbig - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (segbase[mid] <= seg) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// one workgroup: segbase = exclusive prefix of each big node's segment count
+__global__ __launch_bounds__(1024) void k_compact_segbase(DevStore s, const uint32_t *__restrict__ big, uint32_t nbig,
+                                                          CompactBig cb)
+{
     __shared__ uint32_t part[16];
     const Team<1024> tm{part, (int)threadIdx.x};
-    const uint32_t cnt = s.ctl->ncomp[1];
-    for (uint32_t k = blockIdx.x; k < cnt; k += gridDim.x) {
-        compact_node<1024>(s, (int32_t)list[k], tm, goff);
-        __syncthreads();  // goff and the header are the next node's
+    uint32_t run = 0;
+    for (uint32_t c0 = 0; c0 < nbig; c0 += 1024) {
+        const uint32_t i = c0 + threadIdx.x;
+        const uint32_t ns = i < nbig ? s.hdr[big[i]].nslots : 0u;
+        uint32_t tot;
+        const uint32_t ex = tm.excl((ns + kCompactSeg - 1) / kCompactSeg, tot);
+        if (i < nbig) cb.segbase[i] = run + ex;
+        run += tot;
+    }
+    if (threadIdx.x == 0) cb.segbase[nbig] = run;
+}
+
+// live slots and their back entries per segment (workgroups of 256)
+__global__ __launch_bounds__(256) void k_compact_segcount(DevStore s, const uint32_t *__restrict__ big, uint32_t nbig,
+                                                          uint64_t nseg, CompactBig cb)
+{
+    __shared__ uint32_t part[4];
+    const Team<256> tm{part, (int)threadIdx.x};
+    for (uint64_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
+        const uint32_t i = seg_node(cb.segbase, nbig, (uint32_t)sg);
+        const NodeHdr h = s.hdr[big[i]];
+        const uint32_t *V = s.slot + h.slot_off;
+        const uint32_t *W = V + h.slot_cap, *F = V + 3ull * h.slot_cap;
+        const uint32_t a = ((uint32_t)sg - cb.segbase[i]) * kCompactSeg, b = min(a + kCompactSeg, h.nslots);
+        uint32_t live = 0, pc = 0;
+        for (uint32_t j = a + threadIdx.x; j < b; j += 256) {
+            const uint32_t w = W[j];
+            if (w || (s.fw && F[j])) {
+                ++live;
+                pc += (uint32_t)__popc(w);
+            }
+        }
+        uint32_t tl, tp;
+        tm.excl(live, tl);
+        tm.excl(pc, tp);
+        if (threadIdx.x == 0) {
+            cb.seg_live[sg] = tl;
+            cb.seg_pc[sg] = tp;
+        }
+    }
+}
+
+// per big node (one wave each): its segments' first new id and back offset
+// (in place of the counts) and its totals
+__global__ __launch_bounds__(256) void k_compact_segscan(uint32_t nbig, CompactBig cb)
+{
+    const int ln = lane_id();
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t i = wave; i < nbig; i += nwaves) {
+        const uint32_t s0 = cb.segbase[i], s1 = cb.segbase[i + 1];
+        uint32_t rl = 0, rp = 0;
+        for (uint32_t c0 = s0; c0 < s1; c0 += 64) {
+            const uint32_t sg = c0 + (uint32_t)ln;
+            const bool in = sg < s1;
+            const uint32_t l = in ? cb.seg_live[sg] : 0u, p = in ? cb.seg_pc[sg] : 0u;
+            const uint32_t il = dpp_incl_add(l), ip = dpp_incl_add(p);
+            if (in) {
+                cb.seg_live[sg] = rl + il - l;
+                cb.seg_pc[sg] = rp + ip - p;
+            }
+            rl += rl32(il, 63);
+            rp += rl32(ip, 63);
+        }
+        if (ln == 0) {
+            cb.nlive[i] = rl;
+            cb.nback[i] = rp;
+        }
+    }
+}
+
+// one workgroup: the big nodes' staging offsets (prefixes of their totals)
+__global__ __launch_bounds__(1024) void k_compact_nodescan(uint32_t nbig, CompactBig cb)
+{
+    __shared__ uint32_t part[16];
+    const Team<1024> tm{part, (int)threadIdx.x};
+    uint64_t trec = 0, tback = 0;
+    for (uint32_t c0 = 0; c0 < nbig; c0 += 1024) {
+        const uint32_t i = c0 + threadIdx.x;
+        const bool in = i < nbig;
+        // 32-bit scans of the high and low 16 bits (1024 values each)
+        const uint32_t nl = in ? cb.nlive[i] : 0u, nk = in ? cb.nback[i] : 0u;
+        uint32_t a, b, c, d;
+        const uint32_t lh = tm.excl(nl >> 16, a), ll = tm.excl(nl & 0xFFFFu, b);
+        const uint32_t kh = tm.excl(nk >> 16, c), kl = tm.excl(nk & 0xFFFFu, d);
+        if (in) {
+            cb.trec[i] = trec + ((uint64_t)lh << 16) + ll;
+            cb.tback[i] = tback + ((uint64_t)kh << 16) + kl;
+        }
+        trec += ((uint64_t)a << 16) + b;
+        tback += ((uint64_t)c << 16) + d;
+    }
+}
+
+// stage each live slot's record (new back offset) and back entries, and
+// patch its members' slot fields with its new number
+__global__ __launch_bounds__(256) void k_compact_stage(DevStore s, const uint32_t *__restrict__ big, uint32_t nbig,
+                                                       uint64_t nseg, CompactBig cb)
+{
+    __shared__ uint32_t part[4];
+    __shared__ uint64_t goff[kGroups];
+    const Team<256> tm{part, (int)threadIdx.x};
+    for (uint64_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
+        const uint32_t i = seg_node(cb.segbase, nbig, (uint32_t)sg);
+        const int32_t u = (int32_t)big[i];
+        const NodeHdr h = s.hdr[u];
+        if (threadIdx.x < kGroups) goff[threadIdx.x] = s.grp[(size_t)u * kGroups + threadIdx.x].off;
+        __syncthreads();
+        const uint32_t *V = s.slot + h.slot_off;
+        const uint32_t *W = V + h.slot_cap, *B = V + 2ull * h.slot_cap, *F = V + 3ull * h.slot_cap;
+        const uint32_t *back = s.back + h.back_off;
+        uint4 *rec = cb.rec + cb.trec[i];
+        uint32_t *tb = cb.back + cb.tback[i];
+        const uint32_t a = ((uint32_t)sg - cb.segbase[i]) * kCompactSeg, b = min(a + kCompactSeg, h.nslots);
+        uint32_t nid = cb.seg_live[sg], nb = cb.seg_pc[sg];
+        for (uint32_t c0 = a; c0 < b; c0 += 256) {
+            const uint32_t j = c0 + threadIdx.x;
+            const bool in = j < b;
+            const uint32_t w = in ? W[j] : 0u, f = (s.fw && in) ? F[j] : 0u;
+            const bool live = in && (w || f);
+            const uint32_t pc = live ? (uint32_t)__popc(w) : 0u;
+            uint32_t tot;
+            const uint32_t ex = tm.excl((live ? 1u << 16 : 0u) | pc, tot);
+            if (live) {
+                const uint32_t id = nid + (ex >> 16), nbk = nb + (ex & 0xFFFFu), ob = B[j];
+                rec[id] = make_uint4(V[j], w, f, nbk);
+                uint32_t bits = w;
+                for (uint32_t r = 0; r < pc; ++r) {
+                    const int jj = __ffs(bits) - 1;
+                    bits &= bits - 1u;
+                    const uint32_t pos = back[ob + r];
+                    tb[nbk + r] = pos;
+                    reinterpret_cast<uint32_t *>(s.mem + goff[jj] + pos)[0] = id;
+                }
+            }
+            nid += tot >> 16;
+            nb += tot & 0xFFFFu;
+        }
+        __syncthreads();  // goff is the next segment's
+    }
+}
+
+// write the staged records and back entries to the front of the node's regions
+__global__ __launch_bounds__(256) void k_compact_unstage(DevStore s, const uint32_t *__restrict__ big, uint32_t nbig,
+                                                         uint64_t nseg, CompactBig cb)
+{
+    for (uint64_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
+        const uint32_t i = seg_node(cb.segbase, nbig, (uint32_t)sg);
+        const NodeHdr h = s.hdr[big[i]];
+        uint32_t *V = s.slot + h.slot_off;
+        uint32_t *W = V + h.slot_cap, *B = V + 2ull * h.slot_cap, *F = V + 3ull * h.slot_cap;
+        uint32_t *back = s.back + h.back_off;
+        const bool last = (uint32_t)sg + 1 == cb.segbase[i + 1];
+        const uint32_t id0 = cb.seg_live[sg], id1 = last ? cb.nlive[i] : cb.seg_live[sg + 1];
+        const uint32_t b0 = cb.seg_pc[sg], b1 = last ? cb.nback[i] : cb.seg_pc[sg + 1];
+        const uint4 *rec = cb.rec + cb.trec[i];
+        const uint32_t *tb = cb.back + cb.tback[i];
+        for (uint32_t id = id0 + threadIdx.x; id < id1; id += 256) {
+            const uint4 r = rec[id];
+            V[id] = r.x;
+            W[id] = r.y;
+            if (s.fw) F[id] = r.z;
+            B[id] = r.w;
+        }
+        for (uint32_t q = b0 + threadIdx.x; q < b1; q += 256) back[q] = tb[q];
+    }
+}
+
+// headers of the big nodes, and their neighbour tables from the live slots
+__global__ __launch_bounds__(256) void k_compact_bigfinish(DevStore s, const uint32_t *__restrict__ big, uint32_t nbig,
+                                                           CompactBig cb)
+{
+    __shared__ uint32_t part[4];
+    const Team<256> tm{part, (int)threadIdx.x};
+    for (uint32_t i = blockIdx.x; i < nbig; i += gridDim.x) {
+        const int32_t u = (int32_t)big[i];
+        const NodeHdr h = s.hdr[u];
+        const uint32_t nid = cb.nlive[i];
+        uint32_t hused = h.hash_used;
+        if (s.nbr && h.hash_cap) {
+            uint64_t *tab = s.hash + h.hash_off;
+            const uint32_t *V = s.slot + h.slot_off, *W = V + h.slot_cap;
+            for (uint32_t q = threadIdx.x; q < 2u * h.hash_cap; q += 256) tab[q] = 0ull;
+            __syncthreads();
+            uint32_t fresh = 0;
+            for (uint32_t j = threadIdx.x; j < nid; j += 256) fresh += hash_add(tab, h.hash_cap, V[j], 1u, W[j]);
+            tm.excl(fresh, hused);
+        }
+        if (threadIdx.x == 0) {
+            NodeHdr nh = h;
+            nh.nslots = nid;
+            nh.nlive = nid;
+            nh.back_used = cb.nback[i];
+            nh.hash_used = hused;
+            s.hdr[u] = nh;
+        }
+        __syncthreads();
     }
 }
 
@@ -2988,17 +3211,33 @@ void launch_export_b(DevStore s, const uint64_t *slot_out_off, const uint64_t *m
                        slot_v, slot_w, members, backpos, slot_f);
 }
 
-void launch_compact(DevStore s, uint32_t thr16, uint32_t *small_list, uint32_t *big_list, hipStream_t st)
+void launch_compact_select(DevStore s, uint32_t thr16, uint32_t *small_list, uint32_t *big_list, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_compact_reset, dim3(1), dim3(64), 0, st, s.ctl);
+    const int64_t n1 = (int64_t)s.n + 1;
+    hipLaunchKernelGGL(k_compact_select, dim3(blocks_for(n1, 256)), dim3(256), 0, st, s, thr16, small_list, big_list);
+}
+
+void launch_compact_apply(DevStore s, const uint32_t *small_list, const uint32_t *big_list, uint32_t nsmall,
+                          uint32_t nbig, uint64_t nseg, const CompactBig &cb, hipStream_t st)
 {
     int dev = 0, ncu = 1;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    hipLaunchKernelGGL(k_compact_reset, dim3(1), dim3(64), 0, st, s.ctl);
-    const int64_t n1 = (int64_t)s.n + 1;
-    hipLaunchKernelGGL(k_compact_select, dim3(blocks_for(n1, 256)), dim3(256), 0, st, s, thr16, small_list, big_list);
-    const unsigned sb = (unsigned)std::min<int64_t>((n1 + 3) / 4, (int64_t)ncu * 8);
-    hipLaunchKernelGGL(k_compact_small, dim3(sb), dim3(256), 0, st, s, small_list);
-    hipLaunchKernelGGL(k_compact_big, dim3((unsigned)ncu), dim3(1024), 0, st, s, big_list);
+    if (nsmall) {
+        const unsigned sb = (unsigned)std::min<int64_t>((nsmall + 3) / 4, (int64_t)ncu * 8);
+        hipLaunchKernelGGL(k_compact_small, dim3(sb), dim3(256), 0, st, s, small_list);
+    }
+    if (!nbig) return;
+    const unsigned gb = (unsigned)std::min<uint64_t>(nseg, (uint64_t)ncu * 8);
+    hipLaunchKernelGGL(k_compact_segbase, dim3(1), dim3(1024), 0, st, s, big_list, nbig, cb);
+    hipLaunchKernelGGL(k_compact_segcount, dim3(gb), dim3(256), 0, st, s, big_list, nbig, nseg, cb);
+    hipLaunchKernelGGL(k_compact_segscan, dim3((nbig + 3) / 4), dim3(256), 0, st, nbig, cb);
+    hipLaunchKernelGGL(k_compact_nodescan, dim3(1), dim3(1024), 0, st, nbig, cb);
+    hipLaunchKernelGGL(k_compact_stage, dim3(gb), dim3(256), 0, st, s, big_list, nbig, nseg, cb);
+    hipLaunchKernelGGL(k_compact_unstage, dim3(gb), dim3(256), 0, st, s, big_list, nbig, nseg, cb);
+    hipLaunchKernelGGL(k_compact_bigfinish, dim3((unsigned)std::min<uint32_t>(nbig, (uint32_t)ncu * 4)), dim3(256), 0,
+                       st, s, big_list, nbig, cb);
 }
 
 void launch_edge_count(DevStore s, const int32_t *us, const int32_t *vs, int64_t k, uint32_t *counts, hipStream_t st)

@@ -158,8 +158,8 @@ int bingo_delete_device(bingo_graph *g, const bingo_del *d_dels, int64_t k);
  * unchanged; deletes after the call name the new slot numbers, and new arcs
  * take slots from live(u) on.  A caller that tracks slot numbers applies the
  * same rule (bingo_updgen_compact does).  nodes_out / slots_out (may be
- * NULL; either makes the call wait for the device) receive the number of
- * nodes compacted and of slots reclaimed. */
+ * NULL) receive the number of nodes compacted and of slots reclaimed.  The
+ * call waits for the device once (the selection sizes its scratch). */
 int bingo_compact(bingo_graph *g, double min_dead_frac, uint64_t *nodes_out, uint64_t *slots_out);
 
 /* ---- walks: ref randomWalk() (graph.cuh:93-150) and the kernels of

@@ -58,6 +58,22 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+NATIVE_PATH = os.path.join(HERE, "_native", "liboracle.so")
+NATIVE_FLAGS = "-O3 -march=native -mtune=native -fopenmp"
+
+
+def use_native() -> str:
+    """Build the oracle for this machine's CPU (-march=native, into _native/)
+    and load that build instead of the portable one: the CPU baseline is
+    timed with it.  Must run before the first lib() call; returns the flags."""
+    global LIB_PATH
+    if _lib is not None:
+        raise RuntimeError("the portable oracle is loaded already")
+    subprocess.run(["make", "-s", "-C", HERE, "native"], check=True)
+    LIB_PATH = NATIVE_PATH
+    return NATIVE_FLAGS
+
+
 _lib = None
 
 
