@@ -1621,46 +1621,6 @@ __global__ void k_compact_reset(Ctl *ctl)
     }
 }
 
-__global__ __launch_bounds__(256) void k_compact_select(DevStore s, uint32_t thr16, uint32_t *__restrict__ small_list,
-                                                        uint32_t *__restrict__ big_list)
-{
-    const int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    bool pick = false, big = false;
-    uint32_t dead = 0, ns = 0, nbk = 0;
-    if (u >= 1 && u <= s.n) {
-        const NodeHdr h = s.hdr[u];
-        dead = h.nslots - h.nlive;
-        pick = dead >= 1u && (uint64_t)dead * 65536u >= (uint64_t)thr16 * h.nslots;
-        big = h.nslots > kBigCompact;
-        ns = h.nslots;
-        nbk = h.back_used;
-    }
-    const int ln = lane_id();
-    const uint64_t lt = (1ull << ln) - 1ull;
-    const uint64_t ms = __ballot(pick && !big), mb = __ballot(pick && big);
-    uint32_t bs = 0, bb = 0;
-    if (ln == 0) {
-        if (ms) bs = atomicAdd(&s.ctl->ncomp[0], (unsigned)__popcll(ms));
-        if (mb) bb = atomicAdd(&s.ctl->ncomp[1], (unsigned)__popcll(mb));
-    }
-    bs = __shfl(bs, 0, 64);
-    bb = __shfl(bb, 0, 64);
-    if (pick && !big) small_list[bs + (uint32_t)__popcll(ms & lt)] = (uint32_t)u;
-    if (pick && big) big_list[bb + (uint32_t)__popcll(mb & lt)] = (uint32_t)u;
-    const uint32_t fr = wave_sum32(pick ? dead : 0u);
-    if (ln == 0 && fr) atomicAdd(&s.ctl->comp_freed, (unsigned long long)fr);
-    if (mb) {  // big nodes: slots, back entries and segments, for the staging scratch
-        const bool b = pick && big;
-        const uint32_t sl = wave_sum32(b ? ns : 0u), bk = wave_sum32(b ? nbk : 0u),
-                       sg = wave_sum32(b ? (ns + kCompactSeg - 1) / kCompactSeg : 0u);
-        if (ln == 0) {
-            atomicAdd(&s.ctl->comp_bigslots, (unsigned long long)sl);
-            atomicAdd(&s.ctl->comp_bigback, (unsigned long long)bk);
-            atomicAdd(&s.ctl->comp_nseg, (unsigned long long)sg);
-        }
-    }
-}
-
 // A team of NT threads working on one node: one wave (NT = 64) or one
 // workgroup (NT = 1024, scan partials in LDS).
 template <int NT>
@@ -1695,6 +1655,71 @@ struct Team {
         return before + inc - v;
     }
 };
+
+// Selection in tiles of 2048 nodes per workgroup (8 per thread, strided so
+// header reads coalesce): list positions come from one atomic per list per
+// tile, the totals from one atomic per workgroup (a single contended 64-bit
+// atomic costs ~12 ns on this part; one per wave made this kernel 6.5 ms on
+// 2^25 nodes).
+constexpr uint32_t kSelPer = 8;
+__global__ __launch_bounds__(256) void k_compact_select(DevStore s, uint32_t thr16, uint32_t *__restrict__ small_list,
+                                                        uint32_t *__restrict__ big_list)
+{
+    __shared__ uint32_t part[4];
+    __shared__ uint32_t base[2];
+    const Team<256> tm{part, (int)threadIdx.x};
+    const int64_t n1 = (int64_t)s.n + 1;
+    const int64_t tile = 256 * kSelPer;
+    uint64_t freed = 0, bslots = 0, bback = 0, bseg = 0;
+    for (int64_t t0 = (int64_t)blockIdx.x * tile; t0 < n1; t0 += (int64_t)gridDim.x * tile) {
+        uint32_t ps = 0, pb = 0;  // this thread's picks (bit k: node t0 + k 256 + tid)
+#pragma unroll
+        for (uint32_t k = 0; k < kSelPer; ++k) {
+            const int64_t u = t0 + k * 256 + threadIdx.x;
+            if (u < 1 || u >= n1) continue;
+            const NodeHdr h = s.hdr[u];
+            const uint32_t dead = h.nslots - h.nlive;
+            if (!(dead >= 1u && (uint64_t)dead * 65536u >= (uint64_t)thr16 * h.nslots)) continue;
+            freed += dead;
+            if (h.nslots > kBigCompact) {
+                pb |= 1u << k;
+                bslots += h.nslots;
+                bback += h.back_used;
+                bseg += (h.nslots + kCompactSeg - 1) / kCompactSeg;
+            } else {
+                ps |= 1u << k;
+            }
+        }
+        uint32_t ts, tb;
+        const uint32_t es = tm.excl((uint32_t)__popc(ps), ts), eb = tm.excl((uint32_t)__popc(pb), tb);
+        if (threadIdx.x == 0) {
+            base[0] = ts ? atomicAdd(&s.ctl->ncomp[0], ts) : 0u;
+            base[1] = tb ? atomicAdd(&s.ctl->ncomp[1], tb) : 0u;
+        }
+        __syncthreads();
+        uint32_t os = base[0] + es, ob = base[1] + eb;
+#pragma unroll
+        for (uint32_t k = 0; k < kSelPer; ++k) {
+            const uint32_t u = (uint32_t)(t0 + k * 256 + threadIdx.x);
+            if (ps & (1u << k)) small_list[os++] = u;
+            if (pb & (1u << k)) big_list[ob++] = u;
+        }
+        __syncthreads();  // base is the next tile's
+    }
+    // workgroup totals from 32-bit sums of 32-, 16- and 16-bit pieces (exact)
+    const uint64_t vals[4] = {freed, bslots, bback, bseg};
+    unsigned long long *dst[4] = {&s.ctl->comp_freed, &s.ctl->comp_bigslots, &s.ctl->comp_bigback, &s.ctl->comp_nseg};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint64_t x = vals[q];
+        uint32_t hi, lo16, hi16;
+        tm.excl((uint32_t)(x >> 32), hi);
+        tm.excl((uint32_t)(x & 0xFFFFu), lo16);
+        tm.excl((uint32_t)((x >> 16) & 0xFFFFu), hi16);
+        const uint64_t sum = ((uint64_t)hi << 32) + ((uint64_t)hi16 << 16) + lo16;
+        if (threadIdx.x == 0 && sum) atomicAdd(dst[q], (unsigned long long)sum);
+    }
+}
 
 // Every read of a chunk before any write of it: a workgroup barrier (which
 // waits for each wave's loads), or for one wave a wait on its own loads.
@@ -3215,7 +3240,11 @@ void launch_compact_select(DevStore s, uint32_t thr16, uint32_t *small_list, uin
 {
     hipLaunchKernelGGL(k_compact_reset, dim3(1), dim3(64), 0, st, s.ctl);
     const int64_t n1 = (int64_t)s.n + 1;
-    hipLaunchKernelGGL(k_compact_select, dim3(blocks_for(n1, 256)), dim3(256), 0, st, s, thr16, small_list, big_list);
+    int dev = 0, ncu = 1;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    const unsigned gb = (unsigned)std::min<int64_t>((n1 + 256 * kSelPer - 1) / (256 * kSelPer), (int64_t)ncu * 8);
+    hipLaunchKernelGGL(k_compact_select, dim3(gb), dim3(256), 0, st, s, thr16, small_list, big_list);
 }
 
 void launch_compact_apply(DevStore s, const uint32_t *small_list, const uint32_t *big_list, uint32_t nsmall,

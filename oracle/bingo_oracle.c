@@ -685,6 +685,13 @@ int orc_walk(const orc_graph *g, int app, const orc_walk_params *p, const int32_
              const int32_t *lens, int64_t nwalkers, int32_t *paths, uint64_t *steps_out,
              int nthreads)
 {
+    return orc_walk_ids(g, app, p, starts, lens, NULL, nwalkers, paths, steps_out, nthreads);
+}
+
+int orc_walk_ids(const orc_graph *g, int app, const orc_walk_params *p, const int32_t *starts,
+                 const int32_t *lens, const int64_t *wids, int64_t nwalkers, int32_t *paths,
+                 uint64_t *steps_out, int nthreads)
+{
     if (!p || p->max_len < 1 || nwalkers < 0 || (nwalkers && !paths) || app < 0 || app > 3)
         return ORC_EINVAL;
     if (app == ORC_NODE2VEC && p->max_attempts < 1) return ORC_EINVAL;
@@ -698,10 +705,11 @@ int orc_walk(const orc_graph *g, int app, const orc_walk_params *p, const int32_
     if (nthreads < 1) nthreads = 1;
 #pragma omp parallel for schedule(dynamic, 256) reduction(+ : total) num_threads(nthreads)
     for (int64_t i = 0; i < nwalkers; ++i) {
-        int32_t s = starts ? starts[i] : (int32_t)(i % g->n) + 1;
+        const int64_t w = wids ? wids[i] : i;
+        int32_t s = starts ? starts[i] : (int32_t)(w % g->n) + 1;
         int32_t L = lens ? lens[i] : p->max_len;
         if (app == ORC_SAMPLING && L > 2) L = 2;
-        total += walk_one(g, app, p, (uint64_t)i, s, L, paths + i * (int64_t)p->max_len);
+        total += walk_one(g, app, p, (uint64_t)w, s, L, paths + i * (int64_t)p->max_len);
     }
     if (steps_out) *steps_out = total;
     return ORC_OK;

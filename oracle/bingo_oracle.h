@@ -108,6 +108,13 @@ int orc_walk(const orc_graph *g, int app, const orc_walk_params *p, const int32_
              const int32_t *lens, int64_t nwalkers, int32_t *paths, uint64_t *steps_out,
              int nthreads);
 
+/* the same for a subset of walkers: row i is walker wids[i] (its Philox
+ * stream), starting at starts[i] with length lens[i] (starts NULL: node
+ * (wids[i] % n) + 1) */
+int orc_walk_ids(const orc_graph *g, int app, const orc_walk_params *p, const int32_t *starts,
+                 const int32_t *lens, const int64_t *wids, int64_t nwalkers, int32_t *paths,
+                 uint64_t *steps_out, int nthreads);
+
 /* quantise float walk parameters exactly as the product does; fold = 0
  * keeps the plain envelope max(1/p, 1, 1/q) (float-weight graphs) */
 void orc_walk_thresholds(double p, double q, double tp, orc_walk_params *out);

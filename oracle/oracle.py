@@ -96,6 +96,7 @@ def lib():
         L.orc_insert.argtypes = [vp, vp, i64]
         L.orc_delete.argtypes = [vp, vp, i64]
         L.orc_walk.argtypes = [vp, C.c_int, C.POINTER(WalkParams), vp, vp, i64, vp, C.POINTER(u64), C.c_int]
+        L.orc_walk_ids.argtypes = [vp, C.c_int, C.POINTER(WalkParams), vp, vp, vp, i64, vp, C.POINTER(u64), C.c_int]
         L.orc_walk_thresholds.argtypes = [C.c_double, C.c_double, C.c_double, C.POINTER(WalkParams)]
         L.orc_walk_thresholds_ex.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, C.POINTER(WalkParams)]
         L.orc_fold_threshold.restype = C.c_uint64
@@ -263,6 +264,22 @@ class OracleGraph:
                             C.byref(steps), nthreads)
         if rc:
             raise ValueError(f"orc_walk rc={rc}")
+        return paths, steps.value
+
+    def walk_ids(self, app, params: WalkParams, wids, starts=None, lens=None, nthreads=8):
+        """Rows for walkers wids (their own Philox streams); starts/lens are
+        the rows' own (None: node (wid % n) + 1, max_len)."""
+        wids = np.ascontiguousarray(wids, np.int64)
+        if starts is not None:
+            starts = np.ascontiguousarray(starts, np.int32)
+        if lens is not None:
+            lens = np.ascontiguousarray(lens, np.int32)
+        paths = np.empty((len(wids), params.max_len), np.int32)
+        steps = C.c_uint64(0)
+        rc = lib().orc_walk_ids(self._h, app, C.byref(params), _ptr(starts), _ptr(lens), _ptr(wids), len(wids),
+                                _ptr(paths), C.byref(steps), nthreads)
+        if rc:
+            raise ValueError(f"orc_walk_ids rc={rc}")
         return paths, steps.value
 
     def edge_count(self, u, v):

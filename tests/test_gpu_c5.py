@@ -3,12 +3,12 @@ weights) loaded into a store fitted to the graph, one sliding-window step
 (10^7 inserts, then the 10^7 oldest arcs expired), then size-independent
 properties of the whole store against numpy, alias tables of sampled nodes
 against the oracle's construction, walk steps out of probe nodes checked
-against their live arcs, and the hub's transition law by chi-square.
+against their live arcs, and the hub's transition law by chi-square; then a
+compaction of every node with a tombstone, after which slot counts equal
+live counts and masses, buckets and walks are unchanged.
 
-Needs about 215 GB of HBM, 60 GB of host memory and a few minutes, so it
-runs only when BINGO_FULL_SCALE=1."""
+Needs about 215 GB of HBM, 60 GB of host memory and about 90 s."""
 import ctypes as C
-import os
 
 import numpy as np
 import pytest
@@ -16,11 +16,7 @@ from scipy import stats
 
 import bingo_amd as B
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skipif(os.environ.get("BINGO_FULL_SCALE") != "1",
-                       reason="2^30-arc store: ~215 GB HBM and minutes; set BINGO_FULL_SCALE=1"),
-]
+pytestmark = pytest.mark.gpu
 
 
 def _per_node_tables(g):
@@ -41,7 +37,7 @@ def test_scale_store_window_and_walks(orc):
     n = 1 << scale
     print("[c5] generating", flush=True)
     base = B.gen_rmat(scale, m, B.W_LOGUNI32, seed=2026, undirected=False)
-    g = B.Graph(neighbor_index=False, fit_load=True, pool_slack=0.03)
+    g = B.Graph(neighbor_index=False, fit_load=True, pool_slack=0.03, grow_in_place=True)
     d_base = B.DeviceArray(g, base)
     g.load_device(n, d_base, m)
     d_base.free()
@@ -116,3 +112,13 @@ def test_scale_store_window_and_walks(orc):
     got = np.bincount((p2[:, 1].astype(np.int64) * 2654435761) % bins, minlength=bins)
     assert got.sum() == N
     assert stats.chisquare(got, exp / exp.sum() * N).pvalue > 0.01
+
+    print("[c5] compaction", flush=True)
+    nodes, freed = g.compact(0.0)
+    assert freed == k and nodes == len(np.unique(dels["u"]))
+    st2 = _per_node_tables(g)
+    assert np.array_equal(st2["nslots"], np.bincount(lu, minlength=n + 1).astype(np.uint32))
+    for f in ("mass", "bsize", "T", "alias"):
+        assert np.array_equal(st2[f], st[f]), f
+    p3, _ = g.walk(B.SAMPLING, B.walk_params(seed=11, max_len=2), starts=np.full(N, hub, np.int32))
+    assert np.array_equal(p3, p2)  # same buckets in the same order: the same draws
