@@ -168,6 +168,7 @@ SIGNATURES = {
     "bingo_insert_device": (_int, [_vp, _vp, _i64]),
     "bingo_delete": (_int, [_vp, _vp, _i64]),
     "bingo_delete_device": (_int, [_vp, _vp, _i64]),
+    "bingo_delete_noalloc": (_int, [_vp, _vp, _i64]),
     "bingo_walk": (_int, [_vp, _int, C.POINTER(WalkParams), _vp, _vp, _i64, _vp, C.POINTER(_u64)]),
     "bingo_walk_device": (_int, [_vp, _int, C.POINTER(WalkParams), _vp, _vp, _i64, _vp, _vp]),
     "bingo_walk_tune": (_int, [_vp, _int, C.POINTER(WalkParams), _i64, C.POINTER(WalkParams)]),
@@ -505,6 +506,12 @@ class Graph:
     def delete(self, dels):
         d = np.ascontiguousarray(dels, DEL_DT)
         _check(lib().bingo_delete(self._h, _ptr(d), len(d)))
+
+    def delete_noalloc(self, dels):
+        """bingo_delete with every batch buffer in the library's static device
+        scratch (at most BINGO_NOALLOC_MAX_DELETES deletes)."""
+        d = np.ascontiguousarray(dels, DEL_DT)
+        _check(lib().bingo_delete_noalloc(self._h, _ptr(d), len(d)))
 
     def delete_device(self, d_dels: DeviceArray, k):
         _need(d_dels, k * DEL_DT.itemsize, "deletes")

@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <initializer_list>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -303,25 +304,37 @@ static int bits_for(uint64_t maxval)
 // min(k, n + 1) runs.
 static int64_t max_runs(const bingo_graph *g, int64_t k) { return std::min<int64_t>(k, (int64_t)g->s.n + 1); }
 
-// run-length encode sorted node keys into the Runs arrays (node, start, count)
+// rocPRIM temporary storage for run-length encoding k sorted keys into at
+// most kr runs and scanning the run counts (sizes only: nothing is touched)
+static size_t encode_runs_tmp(int64_t k, int64_t kr, hipStream_t st)
+{
+    return std::max(run_length_encode_u32(nullptr, 0, nullptr, k, nullptr, nullptr, nullptr, st),
+                    exclusive_scan_u32(nullptr, 0, nullptr, nullptr, kr, st));
+}
+
+// run-length encode sorted node keys into the Runs arrays (node, start,
+// count), with rocPRIM's temporary storage in tmp (encode_runs_tmp bytes)
+static int encode_runs_in(bingo_graph *g, const uint32_t *keys_sorted, int64_t k, uint32_t *uniq, uint32_t *counts,
+                          uint32_t *starts, uint32_t *nruns, void *tmp, size_t tmp_bytes, Runs *r)
+{
+    const int64_t kr = max_runs(g, k);
+    run_length_encode_u32(tmp, tmp_bytes, keys_sorted, k, uniq, counts, nruns, g->st);
+    exclusive_scan_u32(tmp, tmp_bytes, counts, starts, kr, g->st);
+    HIPCHK(hipGetLastError());
+    r->node = uniq;
+    r->start = starts;
+    r->count = counts;
+    r->nruns = nruns;
+    return BINGO_OK;
+}
+
 static int encode_runs(bingo_graph *g, const uint32_t *keys_sorted, int64_t k, Runs *r)
 {
     int rc;
-    const int64_t kr = max_runs(g, k);
-    size_t tb = run_length_encode_u32(nullptr, 0, keys_sorted, k, scp<uint32_t>(g, SC_UNIQ),
-                                      scp<uint32_t>(g, SC_COUNTS), scp<uint32_t>(g, SC_NRUNS), g->st);
-    tb = std::max(tb, exclusive_scan_u32(nullptr, 0, scp<uint32_t>(g, SC_COUNTS), scp<uint32_t>(g, SC_STARTS), kr, g->st));
-    if ((rc = ensure_scratch(g, SC_TMP, tb))) return rc;
-    run_length_encode_u32(g->sc[SC_TMP], g->sc_bytes[SC_TMP], keys_sorted, k, scp<uint32_t>(g, SC_UNIQ),
-                          scp<uint32_t>(g, SC_COUNTS), scp<uint32_t>(g, SC_NRUNS), g->st);
-    exclusive_scan_u32(g->sc[SC_TMP], g->sc_bytes[SC_TMP], scp<uint32_t>(g, SC_COUNTS), scp<uint32_t>(g, SC_STARTS), kr,
-                       g->st);
-    HIPCHK(hipGetLastError());
-    r->node = scp<uint32_t>(g, SC_UNIQ);
-    r->start = scp<uint32_t>(g, SC_STARTS);
-    r->count = scp<uint32_t>(g, SC_COUNTS);
-    r->nruns = scp<uint32_t>(g, SC_NRUNS);
-    return BINGO_OK;
+    if ((rc = ensure_scratch(g, SC_TMP, encode_runs_tmp(k, max_runs(g, k), g->st)))) return rc;
+    return encode_runs_in(g, keys_sorted, k, scp<uint32_t>(g, SC_UNIQ), scp<uint32_t>(g, SC_COUNTS),
+                          scp<uint32_t>(g, SC_STARTS), scp<uint32_t>(g, SC_NRUNS), g->sc[SC_TMP], g->sc_bytes[SC_TMP],
+                          r);
 }
 
 // scratch of one insert (del false) or delete batch of k entries
@@ -427,27 +440,48 @@ static int do_insert(bingo_graph *g, const DevEdge *d_e, int64_t k)
     return BINGO_OK;
 }
 
-static int do_delete(bingo_graph *g, const int32_t *d_pairs, int64_t k)
+// Device buffers of one delete batch of k entries.
+struct DelWork {
+    unsigned long long *ka, *kb;  // keys (u << 32 | slot), then sorted
+    uint32_t *us, *slots;         // checked node and slot of each sorted delete
+    uint32_t *uniq, *counts, *starts, *nruns;  // runs per node
+    uint32_t *big;                // long runs
+    void *tmp;                    // rocPRIM temporary storage
+    size_t tmp_bytes;
+};
+
+static size_t del_tmp_bytes(bingo_graph *g, int64_t k)
+{
+    const int bits = 32 + bits_for((uint64_t)g->s.n);
+    return std::max(sort_keys_u64(nullptr, 0, nullptr, nullptr, k, bits, g->st), encode_runs_tmp(k, max_runs(g, k), g->st));
+}
+
+// the buffers of a delete batch from the handle's scratch (allocated as needed)
+static int del_work_scratch(bingo_graph *g, int64_t k, DelWork *w)
 {
     int rc;
-    if (k < 0 || k >= (1ll << 31)) return fail(BINGO_EINVAL, "batch size out of range");
-    if (k == 0) return BINGO_OK;
-    if ((rc = ensure_batch_scratch(g, k, true))) return rc;
+    if ((rc = ensure_batch_scratch(g, k, true)) || (rc = ensure_scratch(g, SC_TMP, del_tmp_bytes(g, k))) ||
+        (rc = ensure_scratch(g, SC_BIG, ((size_t)k / kBigDelete + 1) * sizeof(uint32_t))))
+        return rc;
+    *w = DelWork{scp<unsigned long long>(g, SC_U64A), scp<unsigned long long>(g, SC_U64B), scp<uint32_t>(g, SC_KEYS2),
+                 scp<uint32_t>(g, SC_VALS2),          scp<uint32_t>(g, SC_UNIQ),             scp<uint32_t>(g, SC_COUNTS),
+                 scp<uint32_t>(g, SC_STARTS),         scp<uint32_t>(g, SC_NRUNS),            scp<uint32_t>(g, SC_BIG),
+                 g->sc[SC_TMP],                       g->sc_bytes[SC_TMP]};
+    return BINGO_OK;
+}
+
+static int do_delete(bingo_graph *g, const int32_t *d_pairs, int64_t k, const DelWork &w)
+{
+    int rc;
     launch_reset_ctl(g->s.ctl, g->st);
-    unsigned long long *ka = scp<unsigned long long>(g, SC_U64A), *kb = scp<unsigned long long>(g, SC_U64B);
-    launch_delete_keys(d_pairs, k, ka, g->st);
+    launch_delete_keys(d_pairs, k, w.ka, g->st);
     const int bits = 32 + bits_for((uint64_t)g->s.n);
-    size_t tb = sort_keys_u64(nullptr, 0, ka, kb, k, bits, g->st);
-    if ((rc = ensure_scratch(g, SC_TMP, tb))) return rc;
-    sort_keys_u64(g->sc[SC_TMP], g->sc_bytes[SC_TMP], ka, kb, k, bits, g->st);
-    launch_delete_check(g->s, kb, k, scp<uint32_t>(g, SC_KEYS2), scp<uint32_t>(g, SC_VALS2), g->st);
+    sort_keys_u64(w.tmp, w.tmp_bytes, w.ka, w.kb, k, bits, g->st);
+    launch_delete_check(g->s, w.kb, k, w.us, w.slots, g->st);
     Runs r;
-    if ((rc = encode_runs(g, scp<uint32_t>(g, SC_KEYS2), k, &r))) return rc;
+    if ((rc = encode_runs_in(g, w.us, k, w.uniq, w.counts, w.starts, w.nruns, w.tmp, w.tmp_bytes, &r))) return rc;
     // long runs (more than kBigDelete deletes)
-    const size_t nbig_max = (size_t)k / kBigDelete + 1;
-    if ((rc = ensure_scratch(g, SC_BIG, nbig_max * sizeof(uint32_t)))) return rc;
-    uint32_t *big = scp<uint32_t>(g, SC_BIG);
-    launch_delete_big_list(r, k, big, g->s.ctl, g->st);
+    launch_delete_big_list(r, k, w.big, g->s.ctl, g->st);
     if ((rc = read_ctl(g))) return rc;
     if (g->ctl_host->error & ERR_INVAL)
         return fail(BINGO_EINVAL, "delete batch names a node outside 1..n, a slot that is not live, or a slot twice");
@@ -456,10 +490,81 @@ static int do_delete(bingo_graph *g, const int32_t *d_pairs, int64_t k)
     if (g->ctl_host->error) return fail(BINGO_EDEVICE, "store reported an internal error");
     // neighbour entries first: they subtract each deleted slot's weight,
     // which the apply kernel then tombstones
-    if (g->s.nbr) launch_hash_delete(g->s, scp<uint32_t>(g, SC_KEYS2), scp<uint32_t>(g, SC_VALS2), k, g->st);
-    launch_delete_apply(g->s, scp<uint32_t>(g, SC_VALS2), r, big, k, g->st);
+    if (g->s.nbr) launch_hash_delete(g->s, w.us, w.slots, k, g->st);
+    launch_delete_apply(g->s, w.slots, r, w.big, k, g->st);
     HIPCHK(hipGetLastError());
     g->arcs -= k;
+    return BINGO_OK;
+}
+
+static int do_delete(bingo_graph *g, const int32_t *d_pairs, int64_t k)
+{
+    if (k < 0 || k >= (1ll << 31)) return fail(BINGO_EINVAL, "batch size out of range");
+    if (k == 0) return BINGO_OK;
+    DelWork w;
+    int rc = del_work_scratch(g, k, &w);
+    return rc ? rc : do_delete(g, d_pairs, k, w);
+}
+
+// ------------------------------------------------------------------------
+// Allocation-free deletes: the batch's buffers live in `scratch`, one
+// statically sized device array per GPU (each device holds its own copy at
+// its own address, looked up once per device by hipGetSymbolAddress when the
+// first handle on that device is created).  The host-side name `scratch` is
+// only a stand-in for the symbol: host code passes the looked-up address.
+// Every call on a device shares that device's array: calls that overlap on
+// one GPU (two handles on the same device, from two threads) race.  Nothing
+// in it needs to start at zero (the batch's counters live in the handle).
+// ------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[BINGO_NOALLOC_SCRATCH_BYTES];
+constexpr int kMaxDevices = 64;
+static void *g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+// the device's scratch address (the device current); looked up once per device
+static int scratch_setup(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(BINGO_EINVAL, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return BINGO_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(BINGO_EDEVICE, std::string("scratch symbol lookup failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = ptr;
+    return BINGO_OK;
+}
+
+// the buffers of a delete batch carved from the device's scratch; k must fit
+static int del_work_static(bingo_graph *g, int64_t k, DelWork *w, void **stage)
+{
+    unsigned char *base = static_cast<unsigned char *>(g_scratch_ptr[g->device]);
+    if (!base) return fail(BINGO_ESTATE, "no scratch on this device");
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t k4 = al((size_t)k * 4), k8 = al((size_t)k * 8), r4 = al((size_t)max_runs(g, k) * 4);
+    const size_t tmp = del_tmp_bytes(g, k);
+    const size_t need = 3 * k8 + 2 * k4 + 3 * r4 + 256 + al(((size_t)k / kBigDelete + 1) * 4) + al(tmp);
+    if (need > BINGO_NOALLOC_SCRATCH_BYTES)
+        return fail(BINGO_EINVAL, "delete batch needs " + std::to_string(need) + " bytes of scratch, more than the " +
+                                      std::to_string(BINGO_NOALLOC_SCRATCH_BYTES) + " of the allocation-free path");
+    unsigned char *p = base;
+    auto take = [&](size_t b) {
+        unsigned char *q = p;
+        p += b;
+        return q;
+    };
+    *stage = take(k8);
+    w->ka = reinterpret_cast<unsigned long long *>(take(k8));
+    w->kb = reinterpret_cast<unsigned long long *>(take(k8));
+    w->us = reinterpret_cast<uint32_t *>(take(k4));
+    w->slots = reinterpret_cast<uint32_t *>(take(k4));
+    w->uniq = reinterpret_cast<uint32_t *>(take(r4));
+    w->counts = reinterpret_cast<uint32_t *>(take(r4));
+    w->starts = reinterpret_cast<uint32_t *>(take(r4));
+    w->nruns = reinterpret_cast<uint32_t *>(take(256));
+    w->big = reinterpret_cast<uint32_t *>(take(al(((size_t)k / kBigDelete + 1) * 4)));
+    w->tmp = take(al(tmp));
+    w->tmp_bytes = al(tmp);
     return BINGO_OK;
 }
 
@@ -655,6 +760,10 @@ int bingo_create(const bingo_config *cfg, bingo_graph **out)
         bingo_destroy(g);
         return fail(BINGO_EDEVICE, "handle allocation failed");
     }
+    if (int rc = scratch_setup(g->device)) {
+        bingo_destroy(g);
+        return rc;
+    }
     (void)hipMemset(g->s.ctl, 0, sizeof(Ctl));
     (void)hipMemset(g->s.fm, 0, sizeof(FreeMeta));
     g->s.min_slots = g->cfg.min_slots;
@@ -780,6 +889,27 @@ int bingo_delete(bingo_graph *g, const bingo_del *dels, int64_t k)
     if ((rc = ensure_scratch(g, SC_STAGE, (size_t)k * sizeof(bingo_del)))) return rc;
     if (k) HIPCHK(hipMemcpyAsync(g->sc[SC_STAGE], dels, (size_t)k * sizeof(bingo_del), hipMemcpyHostToDevice, g->st));
     rc = bingo_delete_device(g, static_cast<const bingo_del *>(g->sc[SC_STAGE]), k);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(g->st));
+    return BINGO_OK;
+}
+
+int bingo_delete_noalloc(bingo_graph *g, const bingo_del *dels, int64_t k)
+{
+    if (!g || !g->loaded) return fail(BINGO_ESTATE, "graph not loaded");
+    if (k < 0 || (k > 0 && !dels)) return fail(BINGO_EINVAL, "bad delete array");
+    if (k > BINGO_NOALLOC_MAX_DELETES)
+        return fail(BINGO_EINVAL, "batch of " + std::to_string(k) + " deletes above the allocation-free limit of " +
+                                      std::to_string(BINGO_NOALLOC_MAX_DELETES));
+    if (k == 0) return BINGO_OK;
+    DelWork w;
+    void *stage = nullptr;
+    int rc = del_work_static(g, k, &w, &stage);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(stage, dels, (size_t)k * sizeof(bingo_del), hipMemcpyHostToDevice, g->st));
+    phase_begin(g, PH_DELETE);
+    rc = do_delete(g, static_cast<const int32_t *>(stage), k, w);
+    phase_end(g, PH_DELETE);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(g->st));
     return BINGO_OK;

@@ -162,6 +162,18 @@ int bingo_delete_device(bingo_graph *g, const bingo_del *d_dels, int64_t k);
  * call waits for the device once (the selection sizes its scratch). */
 int bingo_compact(bingo_graph *g, double min_dead_frac, uint64_t *nodes_out, uint64_t *slots_out);
 
+/* ---- allocation-free delete (no reference counterpart) ----
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): the same
+ * batch and result as bingo_delete, with every buffer of the batch in a
+ * statically sized device array of BINGO_NOALLOC_SCRATCH_BYTES per GPU (set
+ * up when the first handle on that GPU is created).  Batches of more than
+ * BINGO_NOALLOC_MAX_DELETES deletes are rejected with BINGO_EINVAL.  All
+ * calls on one GPU share its array: calls that overlap on one GPU race. */
+#define BINGO_NOALLOC_MAX_DELETES 65536
+#define BINGO_NOALLOC_SCRATCH_BYTES (16u << 20)
+int bingo_delete_noalloc(bingo_graph *g, const bingo_del *dels, int64_t k);
+
 /* ---- walks: ref randomWalk() (graph.cuh:93-150) and the kernels of
  * operation.cuh:252-368 (DeepWalk, node2vec, PPR) and 407-430 (sampling).
  * paths is [nwalkers][params->max_len] int32, -1 padded.  starts NULL:
