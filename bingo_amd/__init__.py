@@ -139,6 +139,7 @@ class InstanceOpts(C.Structure):
         ("walk_len", C.c_int32),
         ("device", C.c_int32),
         ("print", C.c_int32),
+        ("keep_graph", C.c_int32),
     ]
 
 
@@ -150,6 +151,7 @@ class InstanceResult(C.Structure):
         ("walk_steps", C.c_uint64),
         ("arcs", C.c_int64),
         ("n", C.c_int32),
+        ("graph", C.c_void_p),
     ]
 
 
@@ -207,6 +209,10 @@ SIGNATURES = {
     "bingo_updgen_deletes": (_int, [_vp, _i64, _vp]),
     "bingo_updgen_expire": (_int, [_vp, _i64, _vp]),
     "bingo_updgen_live": (_i64, [_vp]),
+    "bingo_instgen_create": (_int, [_vp, _i64, _i32, _int, _i64, _u64, C.POINTER(_vp)]),
+    "bingo_instgen_round": (_int, [_vp, _vp, C.POINTER(_i64), _vp, C.POINTER(_i64)]),
+    "bingo_instgen_live": (_i64, [_vp]),
+    "bingo_instgen_destroy": (None, [_vp]),
     "bingo_updgen_compact": (_int, [_vp, C.c_double]),
     "bingo_compact": (_int, [_vp, C.c_double, C.POINTER(_u64), C.POINTER(_u64)]),
 }
@@ -272,14 +278,41 @@ def read_edgelist(path):
     return n.value, out
 
 
-def instance(app, utype, dataset, rounds=10, batch=100000, seed=1, walk_len=80, device=0, verbose=True):
-    """The reference's instance(app, utype, dataset) (include/apps.h:4)."""
+def instance(app, utype, dataset, rounds=10, batch=100000, seed=1, walk_len=80, device=0, verbose=True,
+             keep_graph=False):
+    """The reference's instance(app, utype, dataset) (include/apps.h:4).
+    keep_graph: the result's "graph" is a Graph holding the state after the rounds."""
     o = InstanceOpts()
     lib().bingo_instance_opts_default(C.byref(o))
     o.rounds, o.batch, o.seed, o.walk_len, o.device, o.print = rounds, batch, seed, walk_len, device, int(verbose)
+    o.keep_graph = int(keep_graph)
     r = InstanceResult()
     _check(lib().bingo_instance(app, utype, dataset.encode(), C.byref(o), C.byref(r)))
-    return {k: getattr(r, k) for k, _ in InstanceResult._fields_}
+    out = {k: getattr(r, k) for k, _ in InstanceResult._fields_}
+    out["graph"] = Graph.adopt(r.graph, r.n) if r.graph else None
+    return out
+
+
+class InstanceGen:
+    """The instance() driver's update stream (host only)."""
+
+    def __init__(self, base_edges, n, utype, batch, seed):
+        base = np.ascontiguousarray(base_edges, EDGE_DT)
+        h = C.c_void_p()
+        _check(lib().bingo_instgen_create(_ptr(base), len(base), n, utype, batch, seed, C.byref(h)))
+        self._h = h
+        self.per = batch // 2 if utype == UPD_MIXED else batch
+
+    def round(self):
+        ins, dels = np.zeros(self.per, EDGE_DT), np.zeros(self.per, DEL_DT)
+        ni, nd = C.c_int64(0), C.c_int64(0)
+        _check(lib().bingo_instgen_round(self._h, _ptr(ins), C.byref(ni), _ptr(dels), C.byref(nd)))
+        return ins[:ni.value], dels[:nd.value]
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().bingo_instgen_destroy(self._h)
+            self._h = None
 
 
 UPD_INSERT, UPD_DELETE, UPD_MIXED = 0, 1, 2
@@ -477,6 +510,15 @@ class Graph:
             self._h = None
 
     __del__ = close
+
+    @classmethod
+    def adopt(cls, handle, n, float_weights=False):
+        """A Graph owning an existing loaded handle (e.g. instance(keep_graph=True))."""
+        g = cls.__new__(cls)
+        g._h = C.c_void_p(handle)
+        g.n = n
+        g.float_weights = float_weights
+        return g
 
     def __enter__(self):
         return self

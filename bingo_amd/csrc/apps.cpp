@@ -73,7 +73,88 @@ double now_s()
 
 }  // namespace
 
+// The driver's update stream (ref insertGraph/deleteGraph, graph.cuh:58-92):
+// a host mirror of every arc's slot; an insert duplicates a random existing
+// arc as the source node's next slot with weight d[v] (clamped to >= 1), a
+// delete picks a random live arc.  Mixed mode halves the batch of each kind
+// (apps.cu:22-23).
+struct bingo_instgen {
+    struct Arc {
+        int32_t u, v, slot;
+    };
+    std::vector<Arc> arcs;
+    std::vector<uint32_t> d;
+    int utype = 0;
+    int64_t batch = 0;
+    Stream rs{0, 0, 0x494E5354u};
+};
+
 extern "C" {
+
+int bingo_instgen_create(const bingo_edge *base, int64_t m, int32_t n, int utype, int64_t batch, uint64_t seed,
+                         bingo_instgen **out)
+{
+    if (!out || m < 0 || (m && !base) || n < 1 || utype < 0 || utype > 2 || batch < 0) return BINGO_EINVAL;
+    auto *s = new bingo_instgen();
+    s->utype = utype;
+    s->batch = utype == BINGO_UPD_MIXED ? batch >> 1 : batch;
+    s->rs = Stream{seed, 0, 0x494E5354u};
+    s->d.assign((size_t)n + 1, 0);
+    s->arcs.resize((size_t)m);
+    for (int64_t i = 0; i < m; ++i) {
+        if (base[i].u < 1 || base[i].u > n) {
+            delete s;
+            return BINGO_EINVAL;
+        }
+        s->arcs[i] = bingo_instgen::Arc{base[i].u, base[i].v, (int32_t)s->d[base[i].u]++};
+    }
+    *out = s;
+    return BINGO_OK;
+}
+
+void bingo_instgen_destroy(bingo_instgen *s) { delete s; }
+
+int bingo_instgen_round(bingo_instgen *s, bingo_edge *ins, int64_t *ni, bingo_del *dels, int64_t *nd)
+{
+    if (!s || !ni || !nd) return BINGO_EINVAL;
+    const int64_t batch = s->batch;
+    *ni = *nd = 0;
+    if (s->utype == BINGO_UPD_MIXED || s->utype == BINGO_UPD_INSERT) {
+        if (batch && !ins) return BINGO_EINVAL;
+        const int64_t last = (int64_t)s->arcs.size();
+        if (batch && !last) return BINGO_EINVAL;
+        for (int64_t i = 0; i < batch; ++i) {
+            const bingo_instgen::Arc a = s->arcs[s->rs.below((uint64_t)last)];
+            s->arcs.push_back(bingo_instgen::Arc{a.u, a.v, (int32_t)s->d[a.u]++});
+            const uint32_t w = s->d[a.v];  // graph.cuh:70 passes bias(d[v]) = d[v], without the +1
+            ins[i] = bingo_edge{a.u, a.v, w ? w : 1u};
+        }
+        *ni = batch;
+    }
+    if (s->utype == BINGO_UPD_MIXED || s->utype == BINGO_UPD_DELETE) {
+        if (batch && !dels) return BINGO_EINVAL;
+        int64_t live = 0;
+        for (const auto &a : s->arcs) live += a.slot >= 0;
+        if (live < batch) return BINGO_EINVAL;
+        for (int64_t i = 0; i < batch; ++i) {
+            uint64_t pos;
+            do pos = s->rs.below(s->arcs.size());
+            while (s->arcs[pos].slot < 0);
+            dels[i] = bingo_del{s->arcs[pos].u, s->arcs[pos].slot};
+            s->arcs[pos].slot = -1;
+        }
+        *nd = batch;
+    }
+    return BINGO_OK;
+}
+
+int64_t bingo_instgen_live(const bingo_instgen *s)
+{
+    if (!s) return 0;
+    int64_t live = 0;
+    for (const auto &a : s->arcs) live += a.slot >= 0;
+    return live;
+}
 
 void bingo_instance_opts_default(bingo_instance_opts *o)
 {
@@ -84,6 +165,7 @@ void bingo_instance_opts_default(bingo_instance_opts *o)
     o->walk_len = 80;      // LEN (api.h:12)
     o->device = 0;
     o->print = 1;
+    o->keep_graph = 0;
 }
 
 int bingo_read_edgelist(const char *path, int32_t *n_out, bingo_edge *edges, int64_t cap, int64_t *m_out)
@@ -144,58 +226,34 @@ int bingo_instance(int app, int utype, const char *dataset, const bingo_instance
         bingo_destroy(g);
         return rc;
     }
-    // host mirror of arc slots for the update generators (EdgeData, graph.cuh:12)
-    struct Arc {
-        int32_t u, v, slot;
-    };
-    std::vector<Arc> arcs((size_t)m);
-    std::vector<uint32_t> d((size_t)n + 1, 0);
-    for (int64_t i = 0; i < m; ++i) arcs[i] = Arc{base[i].u, base[i].v, (int32_t)d[base[i].u]++};
+    // the update stream, over a host mirror of every arc's slot (EdgeData, graph.cuh:12)
+    bingo_instgen *gen = nullptr;
+    if ((rc = bingo_instgen_create(base.data(), m, n, utype, o.batch, o.seed, &gen))) {
+        bingo_destroy(g);
+        return rc;
+    }
     base.clear();
     base.shrink_to_fit();
-    int64_t batch = o.batch;
-    if (utype == BINGO_UPD_MIXED) batch >>= 1;
-    Stream rs{o.seed, 0, 0x494E5354u};
     bingo_walk_params wp;
     bingo_walk_params_default(&wp);
     wp.seed = o.seed;
     wp.max_len = o.walk_len;
     void *d_paths = nullptr, *d_steps = nullptr;
     if ((rc = bingo_dev_alloc(g, (size_t)n * o.walk_len * 4, &d_paths)) || (rc = bingo_dev_alloc(g, 8, &d_steps))) {
+        bingo_instgen_destroy(gen);
         bingo_destroy(g);
         return rc;
     }
     uint64_t zero = 0, steps_total = 0;
     double walk_s = 0;
     const double t_eval = now_s();
-    std::vector<bingo_edge> ins;
-    std::vector<bingo_del> del;
+    std::vector<bingo_edge> ins((size_t)gen->batch);
+    std::vector<bingo_del> del((size_t)gen->batch);
     for (int r = 0; r < o.rounds && !rc; ++r) {
-        if (utype == BINGO_UPD_MIXED || utype == BINGO_UPD_INSERT) {
-            const int64_t last = (int64_t)arcs.size();
-            ins.resize((size_t)batch);
-            for (int64_t i = 0; i < batch; ++i) {
-                const Arc a = arcs[rs.below((uint64_t)last)];
-                arcs.push_back(Arc{a.u, a.v, (int32_t)d[a.u]++});
-                const uint32_t w = d[a.v];  // graph.cuh:70 passes bias(d[v]) = d[v], without the +1
-                ins[i] = bingo_edge{a.u, a.v, w ? w : 1u};
-            }
-            rc = bingo_insert(g, ins.data(), batch);
-        }
-        if (!rc && (utype == BINGO_UPD_MIXED || utype == BINGO_UPD_DELETE)) {
-            del.resize((size_t)batch);
-            int64_t live = 0;
-            for (const Arc &a : arcs) live += a.slot >= 0;
-            if (live < batch) rc = BINGO_EINVAL;
-            for (int64_t i = 0; i < batch && !rc; ++i) {
-                uint64_t pos;
-                do pos = rs.below(arcs.size());
-                while (arcs[pos].slot < 0);
-                del[i] = bingo_del{arcs[pos].u, arcs[pos].slot};
-                arcs[pos].slot = -1;
-            }
-            if (!rc) rc = bingo_delete(g, del.data(), batch);
-        }
+        int64_t ni = 0, nd = 0;
+        rc = bingo_instgen_round(gen, ins.data(), &ni, del.data(), &nd);
+        if (!rc && ni) rc = bingo_insert(g, ins.data(), ni);
+        if (!rc && nd) rc = bingo_delete(g, del.data(), nd);
         if (rc) break;
         bingo_memcpy_h2d(g, d_steps, &zero, 8);
         const double tw = now_s();
@@ -222,13 +280,14 @@ int bingo_instance(int app, int utype, const char *dataset, const bingo_instance
         res->memory_gb = mem_gb;
         res->walk_steps = steps_total;
         res->n = n;
-        int64_t live = 0;
-        for (const Arc &a : arcs) live += a.slot >= 0;
-        res->arcs = live;
+        res->arcs = bingo_instgen_live(gen);
+        res->graph = nullptr;
     }
+    bingo_instgen_destroy(gen);
     bingo_dev_free(g, d_paths);
     bingo_dev_free(g, d_steps);
-    bingo_destroy(g);
+    if (!rc && o.keep_graph && res) res->graph = g;  // the caller destroys it
+    else bingo_destroy(g);
     return rc;
 }
 

@@ -299,6 +299,7 @@ typedef struct {
     int32_t walk_len;   /* ref LEN: 80 */
     int32_t device;
     int32_t print;
+    int32_t keep_graph; /* 1: the loaded and updated graph is returned in res->graph (caller destroys) */
 } bingo_instance_opts;
 typedef struct {
     double walk_s;      /* "Random walk in ... s." */
@@ -307,12 +308,25 @@ typedef struct {
     uint64_t walk_steps;
     int64_t arcs;       /* live arcs at the end */
     int32_t n;
+    bingo_graph *graph; /* opts->keep_graph: the graph after the rounds, else NULL */
 } bingo_instance_result;
 void bingo_instance_opts_default(bingo_instance_opts *o);
 /* parse + weight an edge list; edges NULL: only *n_out and *m_out are set */
 int bingo_read_edgelist(const char *path, int32_t *n_out, bingo_edge *edges, int64_t cap, int64_t *m_out);
 int bingo_instance(int app, int utype, const char *dataset, const bingo_instance_opts *opts,
                    bingo_instance_result *res);
+/* The driver's update stream on its own (host only): over a loaded arc list,
+ * round after round an insert batch that duplicates random existing arcs
+ * (utype I or M; ref insertGraph, graph.cuh:58-77) and a delete batch of
+ * random live arcs (D or M; ref deleteGraph, graph.cuh:78-92); mixed mode
+ * halves `batch`.  bingo_instance draws its rounds from this stream. */
+typedef struct bingo_instgen bingo_instgen;
+int bingo_instgen_create(const bingo_edge *base, int64_t m, int32_t n, int utype, int64_t batch, uint64_t seed,
+                         bingo_instgen **out);
+/* ins / dels: room for one round's batch each; *ni / *nd receive the counts */
+int bingo_instgen_round(bingo_instgen *s, bingo_edge *ins, int64_t *ni, bingo_del *dels, int64_t *nd);
+int64_t bingo_instgen_live(const bingo_instgen *s);
+void bingo_instgen_destroy(bingo_instgen *s);
 
 /* ---- workload generators (host, deterministic; ref graph.cuh:16-92) ----
  * R-MAT arcs of edges [edge_begin, edge_begin + nedges) at the given scale,
