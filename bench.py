@@ -17,12 +17,15 @@ Zipf weights, 10^7 PPR walkers with lengths in [5, 200], update batches of
 log-uniform size), replayed from a captured HIP graph; `c1` the small one.
 
     python bench.py [--steps K] [--warmup W] [--config c1|c2|c3|c4|c5] [--no-cpu]
-                    [--app deepwalk|node2vec|ppr|sampling] [--float-weights]
+                    [--app deepwalk|node2vec|ppr|sampling] [--float-weights] [--dense-rate R]
 
 `--app` runs a configuration's graph and schedule with another walk app;
 `--float-weights` turns its weights into float32 values w / 256 (the
 reference's wtype::Float: integer part through the bit groups, fraction by
-rejection), for graph, updates and CPU baseline alike.
+rejection), for graph, updates and CPU baseline alike.  `--dense-rate R`
+stores with the reference's sparse/dense bit groups (RATE): a group of a
+node holding at least R of the node's first arcs keeps only its count and is
+drawn from by rejection over the node's slots.
 
 `c4` is the served form: a Poisson stream of walk queries and update
 batches through the serving loop, with p50/p99 latency.  `c5` is the scale
@@ -51,6 +54,9 @@ HBM_PEAK_GBS = 8000.0
 # node2vec fold lookup 16 (the return arc's neighbour entry; the node's mass
 # comes coded in the alias slot, its exact value read about once in 10^4)
 B_STEP, B_PROBE, B_FOLD = 36, 8, 16
+# a dense-group rejection trial reads one slot weight (8 B per trial: the
+# weight's line share and the slot index)
+B_DENSE = 8
 APP_NAMES = ["deepwalk", "node2vec", "ppr", "sampling"]
 
 CONFIGS = {
@@ -153,7 +159,8 @@ def cpu_baseline(B, base, n, cfg, seed, walk_budget_s=4.0, runs=3):
     if cfg.get("float"):
         base = to_float_bits(base)
     t0 = time.perf_counter()
-    o = orc.OracleGraph(n, neighbor_index=cfg["app"] == 1, float_weights=bool(cfg.get("float")))
+    o = orc.OracleGraph(n, neighbor_index=cfg["app"] == 1, float_weights=bool(cfg.get("float")),
+                        dense_rate=cfg.get("dense_rate", 0.0))
     assert o.load(base) == 0
     load_s = time.perf_counter() - t0
     log(f"cpu oracle ({flags}) loaded {len(base)} arcs in {load_s:.1f}s ({threads} threads)")
@@ -360,6 +367,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--app", choices=APP_NAMES, default=None, help="walk app instead of the configuration's")
     ap.add_argument("--float-weights", action="store_true", help="float32 weights w / 256 (wtype::Float)")
+    ap.add_argument("--dense-rate", type=float, default=0.0, metavar="R",
+                    help="sparse/dense bit groups at rate R (the reference's RATE); 0: every group listed")
     ap.add_argument("--grow-in-place", type=int, choices=(0, 1), default=None,
                     help="pools grow by mapping more HBM (1) or by copying (0); default: the config's choice")
     ap.add_argument("--compact", type=float, default=None, metavar="FRAC",
@@ -395,6 +404,9 @@ def main():
     if args.float_weights:
         cfg["float"] = True
         cfg["workload"] += "_float32w"
+    if args.dense_rate:
+        cfg["dense_rate"] = args.dense_rate
+        cfg["workload"] += f"_rate{args.dense_rate:g}"
     scale, m = cfg["scale"], cfg["edges"]
     n = 1 << scale
     K, W = args.steps, args.warmup
@@ -407,7 +419,7 @@ def main():
     fit = dict(fit_load=True, fit_headroom=1.0 / 16, pool_slack=0.03) if cfg.get("fit") else {}
     gip = cfg.get("grow_in_place") if args.grow_in_place is None else bool(args.grow_in_place)
     g = B.Graph(device=0, neighbor_index=app == B.NODE2VEC, float_weights=bool(cfg.get("float")),
-                grow_in_place=gip, **fit)
+                grow_in_place=gip, dense_rate=cfg.get("dense_rate", 0.0), **fit)
     d_base = B.DeviceArray(g, to_float_bits(base) if cfg.get("float") else base)
     g.load_device(n, d_base, len(base))
     g.sync()
@@ -473,7 +485,7 @@ def main():
     B._check(B.lib().bingo_memcpy_h2d(g._h, d_steps.ptr, np.zeros(1, np.uint64).ctypes.data, 8))
     ins_ms = del_ms = walk_ms = kern_ms = comp_ms = 0.0
     store_trace = []
-    attempts = probes = folds = 0
+    attempts = probes = folds = dtrials = 0
     t0 = time.perf_counter()
     for i in range(K):
         a, b, c, st = step(W + i)
@@ -486,6 +498,7 @@ def main():
         attempts += st["walk_attempts"]
         probes += st["walk_probes"]
         folds += st["walk_folds"]
+        dtrials += st["walk_dense_trials"]
         log(f"step {i}: insert {batches[W + i][2]} in {a:.2f} ms, delete {batches[W + i][2]} in {b:.2f} ms"
             f"{' (+ compaction %.2f ms)' % st['compact_ms'] if cfg.get('compact') else ''}, walk {c:.2f} ms, "
             f"store {st['bytes_reserved'] / 1e9:.1f} GB")
@@ -518,11 +531,11 @@ def main():
     # 16-byte loads); other draws read its first 16 bytes (one load, priced
     # as an 8-byte read) and rarely the second locator
     slot_rate = rates[32] if (app == 1 or cfg.get("float")) else rates[8]
-    ceiling_s = attempts / K * (1 / slot_rate + 1 / rates[8]) + (probes + folds) / K / rates[8]
+    ceiling_s = attempts / K * (1 / slot_rate + 1 / rates[8]) + (probes + folds + dtrials) / K / rates[8]
     log(f"random reads: {rates[32] / 1e9:.1f} G/s (32 B), {rates[8] / 1e9:.1f} G/s (8 B); "
         f"walk at {ceiling_s / (kern_ms / K / 1e3):.2f} of that ceiling")
 
-    bytes_per_launch = (B_STEP * attempts + B_PROBE * probes + B_FOLD * folds) / K
+    bytes_per_launch = (B_STEP * attempts + B_PROBE * probes + B_FOLD * folds + B_DENSE * dtrials) / K
     kern_s = kern_ms / K / 1e3
     achieved = bytes_per_launch / kern_s / 1e9
     traffic = measured_traffic(cfg["workload"])
@@ -548,7 +561,8 @@ def main():
                    "deletes": "oldest arcs (sliding window)" if cfg.get("churn") == "expire" else "uniform live arcs",
                    "store": "fitted at load" if cfg.get("fit") else "power-of-two regions",
                    "pools": "grown in place" if gip else "grown by copy",
-                   "weights": "float32 w/256" if cfg.get("float") else "u32"},
+                   "weights": "float32 w/256" if cfg.get("float") else "u32",
+                   "dense_rate": cfg.get("dense_rate", 0.0)},
         "walk_steps_per_s": walk_rate,
         "updates_per_s": nupd / ((ins_ms + del_ms + comp_ms) / 1e3),
         "insert_ms": ins_ms / K,
@@ -559,6 +573,7 @@ def main():
         "load_ms": load_ms,
         "walk_draws_per_step": attempts / max(steps, 1),
         "walk_fold_lookups_per_step": folds / max(steps, 1),
+        "walk_dense_trials_per_step": dtrials / max(steps, 1),
         "walk_schedule": tuned,
         "plain_form_walk_steps_per_s": plain_rate,
         "batched_vs_plain": walk_rate / plain_rate,

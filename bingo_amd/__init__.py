@@ -48,6 +48,7 @@ class Config(C.Structure):
         ("fit_headroom", C.c_double),
         ("float_weights", C.c_int32),
         ("grow_in_place", C.c_int32),
+        ("dense_rate", C.c_double),
     ]
 
 
@@ -83,6 +84,7 @@ class Stats(C.Structure):
         ("bytes_reserved", C.c_uint64),
         ("pool_used", C.c_uint64 * 4),
         ("compact_ms", C.c_double),
+        ("walk_dense_trials", C.c_uint64),
     ]
 
 
@@ -102,6 +104,7 @@ class Dump(C.Structure):
         ("slot_f", C.c_void_p),
         ("fmass", C.c_void_p),
         ("fthr", C.c_void_p),
+        ("dmask", C.c_void_p),
     ]
 
 
@@ -486,10 +489,11 @@ class Graph:
     """One HBM-resident dynamic graph on one GPU (ids 1..n).
 
     float_weights=True: edge weights are the IEEE-754 bits of float32 values
-    (the reference's wtype::Float), see float_bits()."""
+    (the reference's wtype::Float), see float_bits().  dense_rate: the
+    reference's RATE (0: every group keeps a member list)."""
 
     def __init__(self, device=0, neighbor_index=True, hbm_budget=0, pool_slack=0.25, min_slots=16, fit_load=False,
-                 fit_headroom=1.0 / 16, float_weights=False, grow_in_place=None):
+                 fit_headroom=1.0 / 16, float_weights=False, grow_in_place=None, dense_rate=0.0):
         cfg = Config()
         lib().bingo_config_default(C.byref(cfg))
         cfg.device, cfg.neighbor_index, cfg.hbm_budget = device, int(neighbor_index), hbm_budget
@@ -498,6 +502,7 @@ class Graph:
         if grow_in_place is not None:
             cfg.grow_in_place = int(grow_in_place)
         cfg.float_weights = int(float_weights)
+        cfg.dense_rate = float(dense_rate)
         h = C.c_void_p()
         _check(lib().bingo_create(C.byref(cfg), C.byref(h)))
         self._h = h
@@ -661,6 +666,7 @@ class Graph:
             slot_w=np.zeros(d.total_slots, np.uint32),
             members=np.zeros(d.total_members, np.uint64),
             backpos=np.zeros(d.total_members, np.uint32),
+            dmask=np.zeros(n1, np.uint32),
         )
         if self.float_weights:
             out.update(slot_f=np.zeros(d.total_slots, np.uint32), fmass=np.zeros(n1, np.uint64),

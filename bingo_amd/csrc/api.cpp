@@ -606,7 +606,7 @@ static int launch_choice(bingo_graph *g, int app, const WalkChoice &c, const Wal
     if (c.form == BINGO_FORM_PLAIN) {
         launch_walk(g->s, app, a, c.block, g->st);
     } else {
-        int nb = persist_max_blocks_per_cu(app, c.wpl, c.depth, c.block, g->s.fw);
+        int nb = persist_max_blocks_per_cu(app, c.wpl, c.depth, c.block, g->s.fw, g->s.rate16 != 0);
         if (nb < 1) return fail(BINGO_EINVAL, "persistent walk configuration does not fit a CU");
         if (blocks_per_cu > 0 && blocks_per_cu < nb) nb = blocks_per_cu;
         const int64_t per_block = (int64_t)c.block * c.wpl;
@@ -626,7 +626,7 @@ static uint32_t thr27(double prob)
     return (uint32_t)std::floor(prob * 134217728.0);
 }
 
-int bingo::fill_walk_args(const bingo_walk_params *p, int app, WalkArgs *a, int fw)
+int bingo::fill_walk_args(const bingo_walk_params *p, int app, WalkArgs *a, int fw, int dense)
 {
     if (!p) return fail(BINGO_EINVAL, "null walk params");
     if (app < 0 || app > 3) return fail(BINGO_EINVAL, "unknown app");
@@ -636,6 +636,9 @@ int bingo::fill_walk_args(const bingo_walk_params *p, int app, WalkArgs *a, int 
     // the fractional stream's counter holds the attempt in 16 bits
     if (fw && p->max_attempts > kMaxAttemptsFloat)
         return fail(BINGO_EINVAL, "max_attempts above 65536 with float weights");
+    // so does the dense-group rejection stream's
+    if (dense && p->max_attempts > kMaxAttemptsDense)
+        return fail(BINGO_EINVAL, "max_attempts above 65536 with dense groups (dense_rate)");
     if (app == BINGO_PPR && !(p->tp >= 0 && p->tp <= 1)) return fail(BINGO_EINVAL, "tp must be in [0, 1]");
     // node2vec: alpha = 1/p (return), 1 (distance 1), 1/q (distance 2)
     // (ref operation.cuh:291,302-325; lim there is max(1/p, 1/q)).  A proposal
@@ -689,6 +692,7 @@ void bingo_config_default(bingo_config *cfg)
     cfg->fit_headroom = 1.0 / 16;
     cfg->grow_in_place = 0;
     cfg->float_weights = 0;
+    cfg->dense_rate = 0.0;  // every non-empty group keeps a member list (ref RATE > 1)
 }
 
 void bingo_walk_params_default(bingo_walk_params *p)
@@ -725,6 +729,10 @@ int bingo_create(const bingo_config *cfg, bingo_graph **out)
     if (g->cfg.fit_load && !(g->cfg.fit_headroom >= 0 && g->cfg.fit_headroom <= 1)) {
         delete g;
         return fail(BINGO_EINVAL, "fit_headroom must be in [0, 1]");
+    }
+    if (!(g->cfg.dense_rate >= 0)) {
+        delete g;
+        return fail(BINGO_EINVAL, "dense_rate must be >= 0");
     }
     g->device = g->cfg.device;
     // test hook: treat the address reservation of the pools in this bit mask
@@ -769,6 +777,10 @@ int bingo_create(const bingo_config *cfg, bingo_graph **out)
     g->s.min_slots = g->cfg.min_slots;
     g->s.nbr = g->cfg.neighbor_index ? 1 : 0;
     g->s.fw = g->cfg.float_weights ? 1 : 0;
+    // the rate in units of 2^-16; 0 (off) for 0 and for 1 or more, where
+    // every non-empty group keeps a list (a count never exceeds its run)
+    const double r = g->cfg.dense_rate;
+    g->s.rate16 = (r > 0 && r < 1) ? (uint32_t)std::min(std::max((long long)(r * 65536.0 + 0.5), 1ll), 65535ll) : 0u;
     *out = g;
     return BINGO_OK;
 }
@@ -922,7 +934,7 @@ int bingo_walk_device(bingo_graph *g, int app, const bingo_walk_params *params, 
     if (nwalkers < 0 || (nwalkers > 0 && (!d_paths || !d_steps))) return fail(BINGO_EINVAL, "bad walk buffers");
     if (app == BINGO_NODE2VEC && !g->s.nbr) return fail(BINGO_EUNSUPPORTED, "node2vec needs neighbor_index=1");
     WalkArgs a{};
-    int rc = fill_walk_args(params, app, &a, g->s.fw);
+    int rc = fill_walk_args(params, app, &a, g->s.fw, g->s.rate16 != 0);
     if (rc) return rc;
     if (nwalkers == 0) return BINGO_OK;
     a.starts = d_starts;
@@ -1001,7 +1013,7 @@ int bingo_walk_graph_create(bingo_graph *g, int app, const bingo_walk_params *pa
     bingo_walk_graph *wg = new bingo_walk_graph();
     wg->g = g;
     wg->app = app;
-    int rc = fill_walk_args(params, app, &wg->args, g->s.fw);
+    int rc = fill_walk_args(params, app, &wg->args, g->s.fw, g->s.rate16 != 0);
     if (!rc) rc = resolve_walk(g, app, params, &wg->choice);
     if (rc) {
         delete wg;
@@ -1050,7 +1062,7 @@ int bingo_walk_tune(bingo_graph *g, int app, const bingo_walk_params *params, in
     if (!g || !g->loaded) return fail(BINGO_ESTATE, "graph not loaded");
     if (app == BINGO_NODE2VEC && !g->s.nbr) return fail(BINGO_EUNSUPPORTED, "node2vec needs neighbor_index=1");
     WalkArgs a{};
-    int rc = fill_walk_args(params, app, &a, g->s.fw);
+    int rc = fill_walk_args(params, app, &a, g->s.fw, g->s.rate16 != 0);
     if (rc) return rc;
     if (sample < 1) return fail(BINGO_EINVAL, "sample must be >= 1");
     const int32_t n = g->s.n;
@@ -1081,7 +1093,7 @@ int bingo_walk_tune(bingo_graph *g, int app, const bingo_walk_params *params, in
         for (int w : {1, 2, 4})
             for (int d : {0, 4, 16}) {
                 WalkChoice c{BINGO_FORM_PERSISTENT, b, w, d};
-                if (choice_ok(c, params->max_len) && persist_max_blocks_per_cu(app, w, d, b, g->s.fw) > 0)
+                if (choice_ok(c, params->max_len) && persist_max_blocks_per_cu(app, w, d, b, g->s.fw, g->s.rate16 != 0) > 0)
                     cand.push_back(c);
             }
     float best = 1e30f;
@@ -1251,6 +1263,7 @@ int bingo_stats_get(bingo_graph *g, bingo_stats *out)
         out->walk_attempts = g->ctl_host->attempts;
         out->walk_probes = g->ctl_host->probes;
         out->walk_folds = g->ctl_host->folds;
+        out->walk_dense_trials = g->ctl_host->dtrials;
     }
     return BINGO_OK;
 }
@@ -1278,15 +1291,15 @@ int bingo_export(bingo_graph *g, bingo_dump *d)
     if (!g || !g->loaded) return fail(BINGO_ESTATE, "graph not loaded");
     if (!d) return fail(BINGO_EINVAL, "null dump");
     const size_t n1 = (size_t)g->s.n + 1, ng = n1 * kGroups;
-    std::vector<uint32_t> nsl(n1), bsz(ng);
+    std::vector<uint32_t> nsl(n1), bsz(ng), dmk(n1);
     void *dn = nullptr, *dm = nullptr, *db = nullptr, *dt = nullptr, *da = nullptr;
-    void *dfm = nullptr, *dft = nullptr;
+    void *dfm = nullptr, *dft = nullptr, *ddm = nullptr;
     auto cleanup = [&]() {
-        for (void *p : {dn, dm, db, dt, da, dfm, dft})
+        for (void *p : {dn, dm, db, dt, da, dfm, dft, ddm})
             if (p) (void)hipFree(p);
     };
     if (hipMalloc(&dn, n1 * 4) || hipMalloc(&dm, n1 * 8) || hipMalloc(&db, ng * 4) || hipMalloc(&dt, ng * 8) ||
-        hipMalloc(&da, ng)) {
+        hipMalloc(&da, ng) || hipMalloc(&ddm, n1 * 4)) {
         cleanup();
         return fail(BINGO_ENOMEM, "export buffers");
     }
@@ -1295,9 +1308,10 @@ int bingo_export(bingo_graph *g, bingo_dump *d)
         return fail(BINGO_ENOMEM, "export buffers");
     }
     launch_export_a(g->s, (uint32_t *)dn, (uint64_t *)dm, (uint32_t *)db, (uint64_t *)dt, (uint8_t *)da,
-                    (uint64_t *)dfm, (uint64_t *)dft, g->st);
+                    (uint64_t *)dfm, (uint64_t *)dft, (uint32_t *)ddm, g->st);
     (void)hipMemcpyAsync(nsl.data(), dn, n1 * 4, hipMemcpyDeviceToHost, g->st);
     (void)hipMemcpyAsync(bsz.data(), db, ng * 4, hipMemcpyDeviceToHost, g->st);
+    (void)hipMemcpyAsync(dmk.data(), ddm, n1 * 4, hipMemcpyDeviceToHost, g->st);
     if (hipStreamSynchronize(g->st) != hipSuccess) {
         cleanup();
         return fail(BINGO_EDEVICE, "export");
@@ -1308,7 +1322,7 @@ int bingo_export(bingo_graph *g, bingo_dump *d)
         soff[u] = ts;
         moff[u] = tm;
         ts += nsl[u];
-        for (int j = 0; j < kGroups; ++j) tm += bsz[u * kGroups + j];
+        for (int j = 0; j < kGroups; ++j) tm += ((dmk[u] >> j) & 1u) ? 0u : bsz[u * kGroups + j];  // listed groups
     }
     d->total_slots = ts;
     d->total_members = tm;
@@ -1316,6 +1330,7 @@ int bingo_export(bingo_graph *g, bingo_dump *d)
     // member arrays only when all four of them are given
     if (d->nslots) std::memcpy(d->nslots, nsl.data(), n1 * 4);
     if (d->bsize) std::memcpy(d->bsize, bsz.data(), ng * 4);
+    if (d->dmask) std::memcpy(d->dmask, dmk.data(), n1 * 4);
     if (d->mass) (void)hipMemcpyAsync(d->mass, dm, n1 * 8, hipMemcpyDeviceToHost, g->st);
     if (d->T) (void)hipMemcpyAsync(d->T, dt, ng * 8, hipMemcpyDeviceToHost, g->st);
     if (d->alias) (void)hipMemcpyAsync(d->alias, da, ng, hipMemcpyDeviceToHost, g->st);

@@ -14,8 +14,9 @@ namespace bingo {
 
 // records msg as the calling thread's last error and returns code
 int fail(int code, const std::string &msg);
-// host-side quantisation of walk parameters into kernel thresholds
-int fill_walk_args(const bingo_walk_params *p, int app, WalkArgs *a, int fw);
+// host-side quantisation of walk parameters into kernel thresholds (fw:
+// float weights; dense: the store has dense groups)
+int fill_walk_args(const bingo_walk_params *p, int app, WalkArgs *a, int fw, int dense);
 
 }  // namespace bingo
 

@@ -409,9 +409,10 @@ __device__ __forceinline__ bool fold_step(const DevStore &s, int32_t cur, int32_
 
 // Returns the node's integer mass S (all lanes).  All 64 lanes active.  With
 // float weights, F is the node's new fraction mass: the node's FNode is
-// written and every slot carries the threshold's 17-bit ceiling.
+// written and every slot carries the threshold's 17-bit ceiling.  dm: the
+// node's dense groups, whose locators are kDenseLoc | j.
 __device__ uint64_t alias_rebuild(const DevStore &s, int32_t u, uint32_t size, uint64_t off, uint64_t hmeta,
-                                  uint64_t F, int ln)
+                                  uint64_t F, uint32_t dm, int ln)
 {
     const bool own = ln < kGroups;
     const uint64_t M = own ? ((uint64_t)size << ln) : 0ull;
@@ -460,12 +461,15 @@ __device__ uint64_t alias_rebuild(const DevStore &s, int32_t u, uint32_t size, u
     }
     if (!s.fw) thr_hi = mass_code(S);
     if (s.nmass && ln == 0) s.nmass[u] = S;
-    const uint32_t size_a = __shfl(size, ali, 64);
-    const uint64_t off_a = __shfl(off, ali, 64);
+    // this lane's group as an outcome: its members, or a dense group's index
+    // (0 when empty: such an outcome is never drawn)
+    const uint64_t stay = !size ? 0ull
+                          : ((dm >> (ln & 31)) & 1u) ? kDenseLoc | (uint64_t)ln
+                                                     : off | ((uint64_t)size << kOffBits);
+    const uint64_t alias = __shfl(stay, ali, 64);
     if (own) {
         AliasEnt ae;
         ae.T = T;
-        const uint64_t stay = off | ((uint64_t)size << kOffBits), alias = off_a | ((uint64_t)size_a << kOffBits);
         const bool stay_first = (T >> 63) != 0;  // the likelier outcome first
         ae.first = stay_first ? stay : alias;
         ae.second = stay_first ? alias : stay;
@@ -497,6 +501,7 @@ __global__ void k_reset_walk(Ctl *ctl)
         ctl->attempts = 0;
         ctl->probes = 0;
         ctl->folds = 0;
+        ctl->dtrials = 0;
         ctl->cursor = 0;
         ctl->werr = 0;
     }
@@ -563,6 +568,7 @@ __global__ void k_gather_edges(const DevEdge *__restrict__ e, const uint32_t *__
 // ------------------------------------------------------------------------
 struct Growth {
     uint32_t cnt, new_n, new_back, pcsum;
+    uint32_t dm;      // the node's dense groups after the run
     bool heavy_node;  // the node's integer weight would reach kMaxNodeMass
     uint32_t cap_slots, cap_back, cap_hash;
     bool g_slots, g_back, g_hash;
@@ -585,18 +591,20 @@ __device__ __forceinline__ uint32_t grow_cap(const DevStore &s, uint32_t need, u
 constexpr uint32_t kSmallRun = 4;
 
 // counts of the run and the growth decisions; identical in plan and apply.
-// w0: this lane's arc of the first chunk (already loaded; 0 past the run)
+// w0: this lane's arc of the first chunk (already loaded; 0 past the run).
+// A node without slots takes its dense groups from the run (DevStore::rate16);
+// back entries and member regions are for its listed groups only.
+template <bool DN>
 __device__ Growth plan_run(const DevStore &s, const DevEdge *__restrict__ e, uint32_t cnt, const NodeHdr &h,
                            const GroupHdr &gh, uint32_t w0, int ln)
 {
     Growth G;
-    uint32_t addj = 0, pcsum = 0;
+    uint32_t addj = 0;
     uint64_t wtot = 0;
     if (cnt <= kSmallRun) {  // lane j tests bit j of each arc in turn
         for (uint32_t i = 0; i < cnt; ++i) {
             const uint32_t wi = rl32(w0, (int)i);
             if (ln < kGroups) addj += (wi >> ln) & 1u;
-            pcsum += (uint32_t)__popc(wi);
             wtot += wi;
         }
     } else {  // 64 arcs at a time, one ballot per set bit
@@ -612,10 +620,18 @@ __device__ Growth plan_run(const DevStore &s, const DevEdge *__restrict__ e, uin
                 const uint64_t m = __ballot((w >> j) & 1u);
                 if (ln == j) addj += (uint32_t)__popcll(m);
             }
-            pcsum += dpp_sum32((uint32_t)__popc(w));
         }
         wtot = dpp_sum64(wsum);
     }
+    uint32_t dm = 0;
+    if (DN) {
+        dm = (uint32_t)__builtin_amdgcn_readfirstlane((int)h.dmask);
+        if (h.nslots == 0)  // the node's first slots: its classes (ref build, operation.cuh:57-72)
+            dm = (uint32_t)__ballot(ln < kGroups && addj && ((uint64_t)addj << 16) >= (uint64_t)s.rate16 * cnt);
+    }
+    const bool dense = ln < kGroups && ((dm >> ln) & 1u);
+    const uint32_t pcsum = dpp_sum32(ln < kGroups && !dense ? addj : 0u);  // back entries of the run
+    G.dm = dm;
     G.cnt = cnt;
     G.addj = addj;
     G.pcsum = pcsum;
@@ -629,7 +645,7 @@ __device__ Growth plan_run(const DevStore &s, const DevEdge *__restrict__ e, uin
     G.g_back = G.new_back > h.back_cap;
     G.cap_back = G.g_back ? grow_cap(s, G.new_back, 16u) : h.back_cap;
     const uint32_t need_g = gh.size + addj;
-    G.g_g = ln < kGroups && addj && need_g > gh.cap;
+    G.g_g = ln < kGroups && addj && !dense && need_g > gh.cap;
     G.cap_g = G.g_g ? grow_cap(s, need_g, 4u) : gh.cap;
     const uint32_t hneed = 2u * (h.hash_used + cnt);
     G.g_hash = s.nbr && hneed > h.hash_cap;
@@ -726,6 +742,7 @@ __device__ __forceinline__ uint32_t cls_of(int pool, uint32_t cap) { return (uin
 // class (s.wcls); the apply kernel, with the same grid, hands out indices
 // from those ranges (any order within a workgroup: the regions of a class
 // are interchangeable).
+template <bool DN>
 __global__ __launch_bounds__(256) void k_insert_plan(DevStore s, const DevEdge *__restrict__ E, Runs r)
 {
     __shared__ uint32_t lc[2 * kPC];  // allocations, then frees, per class
@@ -753,7 +770,7 @@ __global__ __launch_bounds__(256) void k_insert_plan(DevStore s, const DevEdge *
         p1 = p2;
         if (!m.u) continue;  // invalid arcs were flagged already
         const uint32_t cnt = m.cnt;
-        const Growth G = plan_run(s, E + m.start, cnt, h, gh, w0, ln);
+        const Growth G = plan_run<DN>(s, E + m.start, cnt, h, gh, w0, ln);
         if (run_too_big(h, G)) {
             if (ln == 0) atomicOr(&s.ctl->error, (unsigned)ERR_NOMEM);
             continue;
@@ -916,12 +933,13 @@ __device__ __forceinline__ void region_copy(const DevStore &s, T *dst, const T *
 
 // `fr` (float weights only) holds the run's fractions, arc for arc; `p` the
 // node's records and this lane's first arc, loaded by the run loop.
+template <bool DN>
 __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, const uint32_t *__restrict__ fr,
                            uint32_t cnt, int32_t u, const RunPre &p, const ClassAlloc &ca, int ln)
 {
     NodeHdr h = p.h;
     const GroupHdr gh = p.gh;
-    const Growth G = plan_run(s, e, cnt, h, gh, p.x.w, ln);
+    const Growth G = plan_run<DN>(s, e, cnt, h, gh, p.x.w, ln);
     if (run_too_big(h, G)) return;  // rejected by the plan
 
     // ---- regions: each from its class; the region it replaces is freed ----
@@ -1012,7 +1030,8 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, con
         // arc by arc (arc i's back entries start at boff_i, bits ascending)
         const DevEdge x = p.x;  // zero past the run
         const bool act = (uint32_t)ln < cnt;
-        const uint32_t pc = (uint32_t)__popc(x.w);
+        const uint32_t sm = ~G.dm;  // listed groups
+        const uint32_t pc = (uint32_t)__popc(x.w & sm);
         const uint32_t my_boff = boff_run + dpp_incl_add(pc) - pc;
         if (act) {
             nv[n0 + ln] = (uint32_t)x.v;
@@ -1026,8 +1045,10 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, con
         for (uint32_t i = 0; i < cnt; ++i) {
             const uint32_t wi = rl32(x.w, (int)i), vi = rl32((uint32_t)x.v, (int)i), bi = rl32(my_boff, (int)i);
             if (ln < kGroups && ((wi >> ln) & 1u)) {
-                s.mem[goff + szj] = ((uint64_t)vi << 32) | (n0 + i);
-                back[bi + __popc(wi & ((1u << ln) - 1u))] = szj;
+                if ((sm >> ln) & 1u) {  // a dense group only counts
+                    s.mem[goff + szj] = ((uint64_t)vi << 32) | (n0 + i);
+                    back[bi + __popc(wi & sm & ((1u << ln) - 1u))] = szj;
+                }
                 ++szj;
             }
         }
@@ -1039,7 +1060,7 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, con
             if (c == 0) x = p.x;
             else if (act) x = e[i];
             const uint32_t slot = n0 + i;
-            const uint32_t pc = (uint32_t)__popc(x.w);
+            const uint32_t pc = (uint32_t)__popc(x.w & ~G.dm);
             const uint32_t pincl = dpp_incl_add(pc);
             const uint32_t my_boff = boff_run + pincl - pc;
             if (act) {
@@ -1060,13 +1081,15 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, con
                 b &= b - 1u;
                 const bool has = (x.w >> j) & 1u;
                 const uint64_t m = __ballot(has);
-                const uint32_t base = rl32(szj, j);
-                const uint64_t go = rl64(goff, j);
-                if (has) {
-                    const uint32_t pos = base + (uint32_t)__popcll(m & lt);
-                    s.mem[go + pos] = entry;
-                    back[my_boff + rank] = pos;
-                    ++rank;
+                if (!((G.dm >> j) & 1u)) {  // a dense group only counts
+                    const uint32_t base = rl32(szj, j);
+                    const uint64_t go = rl64(goff, j);
+                    if (has) {
+                        const uint32_t pos = base + (uint32_t)__popcll(m & lt);
+                        s.mem[go + pos] = entry;
+                        back[my_boff + rank] = pos;
+                        ++rank;
+                    }
                 }
                 if (ln == j) szj += (uint32_t)__popcll(m);
             }
@@ -1092,6 +1115,7 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, con
         h.back_off = a_back;
         h.back_cap = G.cap_back;
         h.back_used = G.new_back;
+        h.dmask = G.dm;
         if (s.nbr) {
             h.hash_off = a_hash;
             h.hash_cap = G.cap_hash;
@@ -1100,11 +1124,12 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, con
         s.hdr[u] = h;
     }
     const uint64_t F = s.fw ? s.fnode[u].F + dpp_sum64(fsum) : 0ull;
-    alias_rebuild(s, u, ln < kGroups ? szj : 0u, goff, s.nbr ? make_hmeta(a_hash, G.cap_hash) : 0ull, F, ln);
+    alias_rebuild(s, u, ln < kGroups ? szj : 0u, goff, s.nbr ? make_hmeta(a_hash, G.cap_hash) : 0ull, F, G.dm, ln);
 }
 
 // Runs by grid stride in node order (neighbouring runs touch neighbouring
 // node records; a shared work counter measured slower).
+template <bool DN>
 __global__ __launch_bounds__(256) void k_insert_apply(DevStore s, const DevEdge *__restrict__ E,
                                                       const uint32_t *__restrict__ FR, Runs r)
 {
@@ -1122,7 +1147,7 @@ __global__ __launch_bounds__(256) void k_insert_apply(DevStore s, const DevEdge 
     for (uint32_t ri = wave; ri < nruns; ri += nwaves) {
         const RunMeta m3 = run_meta(s, r, ri + 3 * nwaves, nruns);
         const RunPre p2 = run_pre<false>(s, E, FR, m2, ln);
-        if (m0.u) insert_run(s, E + m0.start, FR ? FR + m0.start : nullptr, m0.cnt, m0.u, p0, ca, ln);
+        if (m0.u) insert_run<DN>(s, E + m0.start, FR ? FR + m0.start : nullptr, m0.cnt, m0.u, p0, ca, ln);
         m0 = m1;
         m1 = m2;
         m2 = m3;
@@ -1177,6 +1202,7 @@ __global__ void k_delete_check(DevStore s, const unsigned long long *__restrict_
 // [S2, S2 + holes)), which lets the k-th hole read its survivor at S2 + k.
 constexpr int kDB = 8;
 
+template <bool DN>
 __device__ void delete_run(const DevStore &s, const uint32_t *__restrict__ slots, uint32_t cnt, int32_t u,
                            const RunPre &p, int ln)
 {
@@ -1186,7 +1212,10 @@ __device__ void delete_run(const DevStore &s, const uint32_t *__restrict__ slots
     const uint32_t *B = V + 2ull * h.slot_cap;
     uint32_t *back = s.back + h.back_off;
     const bool own = ln < kGroups;
-    const uint32_t bit = own ? 1u << ln : 0u, below = bit - 1u;
+    // bit counts this lane's group; lbit marks list work (0 for a dense
+    // group, which only counts); a slot's back entries are its listed bits'
+    const uint32_t dm = DN ? (uint32_t)__builtin_amdgcn_readfirstlane((int)h.dmask) : 0u, sm = ~dm;  // (uniform)
+    const uint32_t bit = own ? 1u << ln : 0u, lbit = bit & sm, below = (bit - 1u) & sm;
     const GroupHdr gh = p.gh;
     uint64_t *m = s.mem + gh.off;
     const bool single = cnt <= 64u;  // one staged chunk serves both passes
@@ -1204,7 +1233,7 @@ __device__ void delete_run(const DevStore &s, const uint32_t *__restrict__ slots
 #pragma unroll
         for (int q = 0; q < kDB; ++q) {
             const uint32_t w = rl32(cw, (int)q0 + q), bo = rl32(cbo, (int)q0 + q);
-            pos[q] = (w & bit) ? back[bo + __popc(w & below)] : 0u;
+            pos[q] = (w & lbit) ? back[bo + __popc(w & below)] : 0u;
         }
     };
 
@@ -1220,21 +1249,21 @@ __device__ void delete_run(const DevStore &s, const uint32_t *__restrict__ slots
 #pragma unroll
             for (int q = 0; q < kDB; ++q) {
                 keep[q] = pos[q];
-                if (rl32(cw, (int)q0 + q) & bit) {
-                    ++kj;
-                    reinterpret_cast<uint32_t *>(m + pos[q])[0] = rl32(csl, (int)q0 + q) | kMark;
-                }
+                const uint32_t w = rl32(cw, (int)q0 + q);
+                kj += (w & bit) ? 1u : 0u;
+                if (w & lbit) reinterpret_cast<uint32_t *>(m + pos[q])[0] = rl32(csl, (int)q0 + q) | kMark;
             }
         }
     }
 
     // compact the tail's survivors to [S2, S2 + nsurv), in position order
-    const uint32_t S = gh.size, S2 = S - kj;
+    // (a dense group has no list: nothing to compact, no holes)
+    const uint32_t S = gh.size, S2 = S - kj, Slist = lbit ? S : S2;
     uint32_t nsurv = 0;
-    for (uint32_t p0 = S2; p0 < S; p0 += kDB) {
+    for (uint32_t p0 = S2; p0 < Slist; p0 += kDB) {
         uint64_t e[kDB];
 #pragma unroll
-        for (int q = 0; q < kDB; ++q) e[q] = p0 + q < S ? m[p0 + q] : (uint64_t)kMark;
+        for (int q = 0; q < kDB; ++q) e[q] = p0 + q < Slist ? m[p0 + q] : (uint64_t)kMark;
 #pragma unroll
         for (int q = 0; q < kDB; ++q)
             if (!((uint32_t)e[q] & kMark)) m[S2 + nsurv++] = e[q];
@@ -1259,7 +1288,7 @@ __device__ void delete_run(const DevStore &s, const uint32_t *__restrict__ slots
             uint64_t mv[kDB];
 #pragma unroll
             for (int q = 0; q < kDB; ++q) {
-                const bool hole = (rl32(cw, (int)q0 + q) & bit) && pos[q] < S2;
+                const bool hole = (rl32(cw, (int)q0 + q) & lbit) && pos[q] < S2;
                 rk[q] = hole ? hr : ~0u;
                 hr += hole;
                 bad |= hole && rk[q] >= nsurv;  // no survivor left: the store is inconsistent
@@ -1290,7 +1319,7 @@ __device__ void delete_run(const DevStore &s, const uint32_t *__restrict__ slots
     if (kj) s.grp[(size_t)u * kGroups + ln].size = S2;
     if (ln == 0) s.hdr[u].nlive = h.nlive - cnt;
     const uint64_t F = s.fw ? s.fnode[u].F - dpp_sum64(fsub) : 0ull;
-    alias_rebuild(s, u, own ? S2 : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, F, ln);
+    alias_rebuild(s, u, own ? S2 : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, F, dm, ln);
 }
 
 // A run of at most K (<= kDB) deletes: the same result as delete_run in fewer
@@ -1300,7 +1329,7 @@ __device__ void delete_run(const DevStore &s, const uint32_t *__restrict__ slots
 // pairs the k-th hole with the k-th survivor there, and only gathers the
 // moved members' back offsets afterwards.  Instantiated for K = 1, 2, 4, 8
 // (the loops are unrolled over K; most runs of a batch delete one arc).
-template <int K>
+template <int K, bool DN>
 __device__ void delete_small(const DevStore &s, uint32_t cnt, int32_t u, const RunPre &p, int ln)
 {
     const NodeHdr h = p.h;
@@ -1312,12 +1341,15 @@ __device__ void delete_small(const DevStore &s, uint32_t cnt, int32_t u, const R
     uint32_t *W = V + h.slot_cap;
     const uint32_t *B = V + 2ull * h.slot_cap;
     uint32_t *back = s.back + h.back_off;
-    const uint32_t bit = own ? 1u << ln : 0u, below = bit - 1u;
+    // lbit: list work for this lane's group (0 for a dense group, which only
+    // counts); a slot's back entries are its listed bits'
+    const uint32_t dm = DN ? (uint32_t)__builtin_amdgcn_readfirstlane((int)h.dmask) : 0u, sm = ~dm;  // (uniform)
+    const uint32_t bit = own ? 1u << ln : 0u, lbit = bit & sm, below = (bit - 1u) & sm;
     uint64_t *m = s.mem + gh.off;
 
     // the deletes' slot records; this group's last c members
     const uint32_t cw = in ? W[t_sl] : 0u, cbo = in ? B[t_sl] : 0u;
-    const uint32_t S = gh.size, c = min(cnt, S);
+    const uint32_t S = gh.size, c = lbit ? min(cnt, S) : 0u;
     uint64_t e[K];
 #pragma unroll
     for (int i = 0; i < K; ++i) e[i] = (uint32_t)i < c ? m[S - c + i] : 0ull;
@@ -1329,10 +1361,11 @@ __device__ void delete_small(const DevStore &s, uint32_t cnt, int32_t u, const R
     for (int q = 0; q < K; ++q) {
         wq[q] = rl32(cw, q);  // 0 for q >= cnt
         sq[q] = rl32(t_sl, q);
-        pos[q] = (wq[q] & bit) ? back[rl32(cbo, q) + __popc(wq[q] & below)] : 0u;
+        pos[q] = (wq[q] & lbit) ? back[rl32(cbo, q) + __popc(wq[q] & below)] : 0u;
         kj += (wq[q] & bit) ? 1u : 0u;
     }
-    const uint32_t S2 = S - kj;  // kj <= c: the tail [S2, S) is prefetched
+    // a listed group: kj <= c, the tail [S2, S) is prefetched
+    const uint32_t S2 = S - kj;
     // slot records of the tail [S2, S), read together with the positions
     // above: the survivors among them are the members that move
     uint32_t eb[K], ew[K];
@@ -1351,7 +1384,7 @@ __device__ void delete_small(const DevStore &s, uint32_t cnt, int32_t u, const R
     for (int i = 0; i < K; ++i) {
         bool del = false;
 #pragma unroll
-        for (int q = 0; q < K; ++q) del |= (wq[q] & bit) && (uint32_t)e[i] == sq[q];
+        for (int q = 0; q < K; ++q) del |= (wq[q] & lbit) && (uint32_t)e[i] == sq[q];
         surv[i] = (uint32_t)i < c && (uint32_t)i + kj >= c && !del;
         srank[i] = ns;
         ns += surv[i] ? 1u : 0u;
@@ -1364,7 +1397,7 @@ __device__ void delete_small(const DevStore &s, uint32_t cnt, int32_t u, const R
     uint32_t hr = 0;
 #pragma unroll
     for (int q = 0; q < K; ++q) {
-        const bool h0 = (wq[q] & bit) && pos[q] < S2;
+        const bool h0 = (wq[q] & lbit) && pos[q] < S2;
         uint64_t x = 0;
         uint32_t xb = 0, xw = 0;
         bool found = false;
@@ -1402,7 +1435,7 @@ __device__ void delete_small(const DevStore &s, uint32_t cnt, int32_t u, const R
         }
     }
     const uint64_t F = s.fw ? s.fnode[u].F - dpp_sum64(fsub) : 0ull;
-    alias_rebuild(s, u, own ? S2 : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, F, ln);
+    alias_rebuild(s, u, own ? S2 : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, F, dm, ln);
 }
 
 // One bit group j of a long delete run, by one wave with lanes across the
@@ -1411,6 +1444,7 @@ __device__ void delete_small(const DevStore &s, uint32_t cnt, int32_t u, const R
 // member below S2) with the compacted survivors in order.  Groups of one node
 // touch disjoint members and back entries and only read the slot arrays, so
 // the 32 tasks of a run run concurrently; tombstones wait for the last one.
+template <bool DN>
 __device__ void delete_group(const DevStore &s, const uint32_t *__restrict__ slots, uint32_t cnt, int32_t u, int j,
                              int ln)
 {
@@ -1419,10 +1453,20 @@ __device__ void delete_group(const DevStore &s, const uint32_t *__restrict__ slo
     const uint32_t *B = W + h.slot_cap;
     uint32_t *back = s.back + h.back_off;
     const GroupHdr gh = s.grp[(size_t)u * kGroups + j];
-    const uint32_t bit = 1u << j, below = bit - 1u;
+    const uint32_t dm = DN ? h.dmask : 0u;
+    const uint32_t bit = 1u << j, below = (bit - 1u) & ~dm;
     uint64_t *m = s.mem + gh.off;
     const uint64_t lt = (1ull << ln) - 1ull;
     uint32_t kj = 0;
+    if ((dm >> j) & 1u) {  // a dense group only counts
+        for (uint32_t c0 = 0; c0 < cnt; c0 += 64) {
+            const uint32_t t = c0 + (uint32_t)ln;
+            const uint32_t w = t < cnt ? W[slots[t]] : 0u;
+            kj += (uint32_t)__popcll(__ballot((w & bit) != 0u));
+        }
+        if (kj && ln == 0) s.grp[(size_t)u * kGroups + j].size = gh.size - kj;
+        return;
+    }
     for (uint32_t c0 = 0; c0 < cnt; c0 += 64) {  // mark
         const uint32_t t = c0 + (uint32_t)ln;
         const uint32_t sl = t < cnt ? slots[t] : 0u;
@@ -1469,6 +1513,7 @@ __device__ void delete_group(const DevStore &s, const uint32_t *__restrict__ slo
 
 // After the last group task of a long run: tombstones and the node's alias
 // table.
+template <bool DN>
 __device__ void delete_finish(const DevStore &s, const uint32_t *__restrict__ slots, uint32_t cnt, int32_t u, int ln)
 {
     const NodeHdr h = s.hdr[u];
@@ -1489,7 +1534,7 @@ __device__ void delete_finish(const DevStore &s, const uint32_t *__restrict__ sl
     if (ln < kGroups) gh = s.grp[(size_t)u * kGroups + ln];
     const uint64_t F = s.fw ? s.fnode[u].F - dpp_sum64(fsub) : 0ull;
     alias_rebuild(s, u, ln < kGroups ? gh.size : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, F,
-                  ln);
+                  DN ? h.dmask : 0u, ln);
 }
 
 // ------------------------------------------------------------------------
@@ -1523,6 +1568,7 @@ __global__ void k_hash_delete(DevStore s, const uint32_t *__restrict__ us, const
 }
 
 // one wave per long delete run: tombstones, fractions and alias table
+template <bool DN>
 __global__ __launch_bounds__(256) void k_delete_finish(DevStore s, const uint32_t *__restrict__ slots, Runs r,
                                                        const uint32_t *__restrict__ big)
 {
@@ -1533,7 +1579,7 @@ __global__ __launch_bounds__(256) void k_delete_finish(DevStore s, const uint32_
     for (uint32_t bi = wave; bi < nbig; bi += nwaves) {
         const uint32_t ri = big[bi];
         const int32_t u = (int32_t)r.node[ri];
-        if (u >= 1 && u <= s.n) delete_finish(s, slots + r.start[ri], r.count[ri], u, ln);
+        if (u >= 1 && u <= s.n) delete_finish<DN>(s, slots + r.start[ri], r.count[ri], u, ln);
     }
 }
 
@@ -1549,6 +1595,7 @@ __global__ void k_delete_big_list(Runs r, uint32_t *__restrict__ big, Ctl *ctl)
 // runs are finished by k_delete_finish after this kernel: the kernel boundary
 // orders every group task's writes before the finish, where a device-scope
 // fence inside the kernel would write back and invalidate an XCD's L2.
+template <bool DN>
 __global__ __launch_bounds__(256) void k_delete_apply(DevStore s, const uint32_t *__restrict__ slots, Runs r,
                                                       const uint32_t *__restrict__ big)
 {
@@ -1564,7 +1611,7 @@ __global__ __launch_bounds__(256) void k_delete_apply(DevStore s, const uint32_t
             const uint32_t bi = task / kGroups, ri = big[bi];
             const int32_t u = (int32_t)r.node[ri];
             if (u < 1 || u > s.n) continue;
-            delete_group(s, slots + r.start[ri], r.count[ri], u, (int)(task % kGroups), ln);
+            delete_group<DN>(s, slots + r.start[ri], r.count[ri], u, (int)(task % kGroups), ln);
         }
     }
     // the other runs, continuing this wave's stride (long runs were done above)
@@ -1582,15 +1629,15 @@ __global__ __launch_bounds__(256) void k_delete_apply(DevStore s, const uint32_t
         const RunPre p1 = run_pre<true>(s, nullptr, slots, m1, ln);
         if (m0.u) {
             if (m0.cnt == 1)
-                delete_small<1>(s, m0.cnt, m0.u, p0, ln);
+                delete_small<1, DN>(s, m0.cnt, m0.u, p0, ln);
             else if (m0.cnt == 2)
-                delete_small<2>(s, m0.cnt, m0.u, p0, ln);
+                delete_small<2, DN>(s, m0.cnt, m0.u, p0, ln);
             else if (m0.cnt <= 4)
-                delete_small<4>(s, m0.cnt, m0.u, p0, ln);
+                delete_small<4, DN>(s, m0.cnt, m0.u, p0, ln);
             else if (m0.cnt <= (uint32_t)kDB)
-                delete_small<kDB>(s, m0.cnt, m0.u, p0, ln);
+                delete_small<kDB, DN>(s, m0.cnt, m0.u, p0, ln);
             else
-                delete_run(s, slots + m0.start, m0.cnt, m0.u, p0, ln);
+                delete_run<DN>(s, slots + m0.start, m0.cnt, m0.u, p0, ln);
         }
         m0 = m1;
         m1 = m2;
@@ -1749,7 +1796,7 @@ __device__ void compact_node(const DevStore &s, int32_t u, const Team<NT> &tm, u
         const uint32_t v = in ? V[i] : 0u, w = in ? W[i] : 0u, b = in ? B[i] : 0u;
         const uint32_t f = (s.fw && in) ? F[i] : 0u;
         const bool live = in && (w || f);
-        const uint32_t pc = live ? (uint32_t)__popc(w) : 0u;
+        const uint32_t pc = live ? (uint32_t)__popc(w & ~h.dmask) : 0u;  // back entries: listed bits
         uint32_t tot;
         // live count (high half) and back entries (low half, at most 32 NT)
         const uint32_t ex = tm.excl((live ? 1u << 16 : 0u) | pc, tot);
@@ -1771,7 +1818,7 @@ __device__ void compact_node(const DevStore &s, int32_t u, const Team<NT> &tm, u
                 if ((uint32_t)r < pc) back[nbk + r] = e[r];
         }
         if (mslot) {
-            uint32_t bits = w;
+            uint32_t bits = w & ~h.dmask;
 #pragma unroll
             for (int r = 0; r < kGroups; ++r)
                 if ((uint32_t)r < pc) {
@@ -1870,7 +1917,7 @@ __global__ __launch_bounds__(256) void k_compact_segcount(DevStore s, const uint
             const uint32_t w = W[j];
             if (w || (s.fw && F[j])) {
                 ++live;
-                pc += (uint32_t)__popc(w);
+                pc += (uint32_t)__popc(w & ~h.dmask);
             }
         }
         uint32_t tl, tp;
@@ -1961,13 +2008,13 @@ __global__ __launch_bounds__(256) void k_compact_stage(DevStore s, const uint32_
             const bool in = j < b;
             const uint32_t w = in ? W[j] : 0u, f = (s.fw && in) ? F[j] : 0u;
             const bool live = in && (w || f);
-            const uint32_t pc = live ? (uint32_t)__popc(w) : 0u;
+            const uint32_t pc = live ? (uint32_t)__popc(w & ~h.dmask) : 0u;
             uint32_t tot;
             const uint32_t ex = tm.excl((live ? 1u << 16 : 0u) | pc, tot);
             if (live) {
                 const uint32_t id = nid + (ex >> 16), nbk = nb + (ex & 0xFFFFu), ob = B[j];
                 rec[id] = make_uint4(V[j], w, f, nbk);
-                uint32_t bits = w;
+                uint32_t bits = w & ~h.dmask;
                 for (uint32_t r = 0; r < pc; ++r) {
                     const int jj = __ffs(bits) - 1;
                     bits &= bits - 1u;
@@ -2047,6 +2094,8 @@ __global__ __launch_bounds__(256) void k_compact_bigfinish(DevStore s, const uin
 // One draw on node `cur` (NodeData::sample, include/Bingo.cuh:220-267):
 // slot i = top 5 bits of r.x; the 64-bit coin (r.y:r.z) against T_i picks
 // group i or its partner; member floor(r.w * size / 2^32) of that group.
+// A dense group j (no member list) returns -2 - j, for dense_draw; -1 is a
+// node without live arcs.
 // `hm` receives the slot's meta word without the partner index: cur's
 // neighbour-table locator (kHmetaMask bits) and the 17-bit field above it
 // (mass code or float threshold ceiling).  Split in
@@ -2087,7 +2136,7 @@ __device__ __forceinline__ int32_t slot_member(const uint64_t *__restrict__ mem,
     const uint64_t loc = first ? (((uint64_t)q0.w << 32) | q0.z) : (((uint64_t)q1.y << 32) | q1.x);
     hm = (((uint64_t)q1.w << 32) | q1.z) >> 5;  // locator | the 17-bit field << 42
     const uint32_t sz = (uint32_t)(loc >> kOffBits);
-    if (sz == 0u) return -1;
+    if (sz == 0u) return (loc & kDenseLoc) ? -2 - (int32_t)(loc & 31u) : -1;
     const uint32_t pos = __umulhi(r.w, sz);
     return (int32_t)reinterpret_cast<const uint32_t *>(mem + (loc & kOffMask36) + pos)[1];
 }
@@ -2119,7 +2168,7 @@ __device__ __forceinline__ int32_t sample_lite(const AliasEnt *__restrict__ alia
         loc = ((uint64_t)b.y << 32) | b.x;
     }
     const uint32_t sz = (uint32_t)(loc >> kOffBits);
-    if (sz == 0u) return -1;
+    if (sz == 0u) return (loc & kDenseLoc) ? -2 - (int32_t)(loc & 31u) : -1;
     const uint32_t pos = __umulhi(r.w, sz);
     return (int32_t)reinterpret_cast<const uint32_t *>(mem + (loc & kOffMask36) + pos)[1];
 }
@@ -2178,11 +2227,48 @@ __device__ __forceinline__ int32_t draw(const DevStore &s, int32_t cur, uint4 r,
     return slot_member(s.mem, r, q0, q1, hm);
 }
 
-template <int APP, bool FW>
+// A draw from dense group j of node `cur`, which keeps no member list: the
+// reference's rejection for groups classed dense (include/Bingo.cuh:244-252),
+// a uniform slot until its weight has bit j (tombstones have no bits).
+// Trials come four per Philox block (counter word 3 = kDenseTag | it << 16 |
+// attempt), words x, y, z, w in order, slot floor(word nslots / 2^32); after
+// kDenseBlocks blocks without a hit (probability below (1 - count /
+// nslots)^4096), the first slot with the bit after the last trial's slot,
+// cyclically.
+// `trials` counts the slot weights read.
+__device__ __forceinline__ int32_t dense_draw(const DevStore &s, int32_t cur, int j, uint32_t w0, uint32_t w1,
+                                           uint32_t step, uint32_t att, uint32_t k0, uint32_t k1, uint32_t &trials)
+{
+    const NodeHdr h = s.hdr[cur];
+    const uint32_t n = h.nslots, bit = 1u << j;
+    if (!n) return -1;
+    const uint32_t *V = s.slot + h.slot_off, *W = V + h.slot_cap;
+    uint32_t sl = 0;
+    for (uint32_t it = 0; it < kDenseBlocks; ++it) {
+        const uint4 b = philox(w0, w1, step, kDenseTag | (it << 16) | att, k0, k1);
+        const uint32_t x[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            sl = __umulhi(x[q], n);
+            ++trials;
+            if (W[sl] & bit) return (int32_t)V[sl];
+        }
+    }
+    for (uint32_t t = 1; t <= n; ++t) {
+        ++trials;
+        const uint32_t s2 = sl + t >= n ? sl + t - n : sl + t;
+        if (W[s2] & bit) return (int32_t)V[s2];
+    }
+    return -1;
+}
+
+// DN: the store has dense groups (DevStore::rate16); the other instances
+// carry no rejection code (its registers would cost them occupancy).
+template <int APP, bool FW, bool DN>
 __global__ __launch_bounds__(256) void k_walk(DevStore s, WalkArgs a)
 {
     const int64_t wid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    uint32_t steps = 0, draws = 0, probes = 0, folds = 0;
+    uint32_t steps = 0, draws = 0, probes = 0, folds = 0, dtr = 0;
     if (wid < a.nwalkers) {
         const int32_t start = a.starts ? a.starts[wid] : (int32_t)(wid % s.n) + 1;
         int32_t L = a.lens ? a.lens[wid] : a.max_len;
@@ -2215,6 +2301,7 @@ __global__ __launch_bounds__(256) void k_walk(DevStore s, WalkArgs a)
                             nx = prev;
                             break;
                         }
+                        if (DN && nx < -1) nx = dense_draw(s, cur, -2 - nx, w0, w1, (uint32_t)j, at, k0, k1, dtr);
                         if (nx < 0 || (int64_t)at + 1 >= a.max_attempts) break;
                         const uint32_t aux = r.x & kAuxMask;
                         bool acc;
@@ -2233,6 +2320,7 @@ __global__ __launch_bounds__(256) void k_walk(DevStore s, WalkArgs a)
                         nx = sample_lite(s.alias, s.mem, cur, r);
                     else
                         nx = draw<FW>(s, cur, r, hcur, w0, w1, (uint32_t)j, 0u, k0, k1);
+                    if (DN && nx < -1) nx = dense_draw(s, cur, -2 - nx, w0, w1, (uint32_t)j, 0u, k0, k1, dtr);
                     ++draws;
                 }
                 if (nx < 0) break;
@@ -2250,7 +2338,7 @@ __global__ __launch_bounds__(256) void k_walk(DevStore s, WalkArgs a)
         for (; j < a.max_len; ++j) row[j] = -1;
     }
     const uint32_t tot = wave_sum32(steps), tdraw = wave_sum32(draws), tprobe = wave_sum32(probes),
-                   tfold = wave_sum32(folds);
+                   tfold = wave_sum32(folds), tdtr = DN ? wave_sum32(dtr) : 0u;
     if (lane_id() == 0) {
         if (tot) {
             atomicAdd(a.steps, (unsigned long long)tot);
@@ -2259,6 +2347,7 @@ __global__ __launch_bounds__(256) void k_walk(DevStore s, WalkArgs a)
         if (tdraw) atomicAdd(&s.ctl->attempts, (unsigned long long)tdraw);
         if (tprobe) atomicAdd(&s.ctl->probes, (unsigned long long)tprobe);
         if (tfold) atomicAdd(&s.ctl->folds, (unsigned long long)tfold);
+        if (tdtr) atomicAdd(&s.ctl->dtrials, (unsigned long long)tdtr);
     }
 }
 
@@ -2344,7 +2433,7 @@ struct PathOut {
     }
 };
 
-template <int APP, int WPL, int D, bool FW>
+template <int APP, int WPL, int D, bool FW, bool DN>
 __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
 {
     extern __shared__ int32_t lds_stage[];
@@ -2371,7 +2460,7 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
         padw[k] = -1;
         padj[k] = 0;
     }
-    uint32_t steps = 0, draws = 0, probes = 0, folds = 0;
+    uint32_t steps = 0, draws = 0, probes = 0, folds = 0, dtr = 0;
     bool more = true;
     // walker ids are claimed from the global cursor a chunk at a time into a
     // wave-local range [lp, le) and handed to empty slots from there: one
@@ -2465,6 +2554,7 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
                     taken = true;
                 }
             }
+            if (DN && nx < -1) nx = dense_draw(s, cur[k], -2 - nx, w0, w1, (uint32_t)jj[k], att[k], k0, k1, dtr);
             if (nx < 0) {
                 padj[k] = out.close(a, k, w, jj[k]);
                 padw[k] = w;
@@ -2541,7 +2631,7 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
         }
     }
     const uint32_t tot = wave_sum32(steps), tdraw = wave_sum32(draws), tprobe = wave_sum32(probes),
-                   tfold = wave_sum32(folds);
+                   tfold = wave_sum32(folds), tdtr = DN ? wave_sum32(dtr) : 0u;
     if (ln == 0) {
         if (tot) {
             atomicAdd(a.steps, (unsigned long long)tot);
@@ -2550,6 +2640,7 @@ __global__ __launch_bounds__(512) void k_walk_persist(DevStore s, WalkArgs a)
         if (tdraw) atomicAdd(&s.ctl->attempts, (unsigned long long)tdraw);
         if (tprobe) atomicAdd(&s.ctl->probes, (unsigned long long)tprobe);
         if (tfold) atomicAdd(&s.ctl->folds, (unsigned long long)tfold);
+        if (tdtr) atomicAdd(&s.ctl->dtrials, (unsigned long long)tdtr);
     }
 }
 
@@ -2605,7 +2696,7 @@ struct ServeRow {
     }
 };
 
-template <bool FW, int CLS>
+template <bool FW, int CLS, bool DN>
 __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *__restrict__ reqs, int nreq,
                                                     unsigned long long total, unsigned long long *cursor)
 {
@@ -2630,7 +2721,7 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
     int32_t acc_req[WPL];
     uint64_t hp[WPL];
     FoldState fst[WPL];
-    uint32_t lookups = 0;  // return-weight lookups (not reported by the served form)
+    uint32_t lookups = 0, dtr = 0;  // return-weight lookups, dense-group trials (not reported by the served form)
 #pragma unroll
     for (int k = 0; k < WPL; ++k) {
         rid[k] = acc_req[k] = -1;
@@ -2731,6 +2822,7 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
                     taken = true;
                 }
             }
+            if (DN && nx < -1) nx = dense_draw(s, cur[k], -2 - nx, w0, w1, (uint32_t)jj[k], att[k], k0, k1, dtr);
             bool done = nx < 0;
             if (!done) {
                 const uint32_t aux = r.x & kAuxMask;
@@ -2770,7 +2862,7 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
 // export and point queries
 // ------------------------------------------------------------------------
 __global__ void k_export_a(DevStore s, uint32_t *nslots, uint64_t *mass, uint32_t *bsize, uint64_t *T, uint8_t *alias,
-                           uint64_t *fmass, uint64_t *fthr)
+                           uint64_t *fmass, uint64_t *fthr, uint32_t *dmask)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int64_t total = ((int64_t)s.n + 1) * kGroups;
@@ -2784,6 +2876,7 @@ __global__ void k_export_a(DevStore s, uint32_t *nslots, uint64_t *mass, uint32_
     alias[i] = (uint8_t)(e.meta & 31u);
     if (j == 0) {
         nslots[u] = s.hdr[u].nslots;
+        dmask[u] = s.hdr[u].dmask;
         uint64_t acc = 0;
         for (int g = 0; g < kGroups; ++g) acc += (uint64_t)s.grp[u * kGroups + g].size << g;
         mass[u] = acc;
@@ -2812,7 +2905,8 @@ __global__ __launch_bounds__(256) void k_export_b(DevStore s, const uint64_t *sl
         if (slot_f) slot_f[so + i] = V[3ull * h.slot_cap + i];
     }
     uint64_t mo = mem_out_off[u];
-    for (int j = 0; j < kGroups; ++j) {
+    for (int j = 0; j < kGroups; ++j) {  // member lists (dense groups keep none)
+        if ((h.dmask >> j) & 1u) continue;
         const GroupHdr g = s.grp[u * kGroups + j];
         for (uint32_t i = ln; i < g.size; i += 64) members[mo + i] = s.mem[g.off + i];
         mo += g.size;
@@ -2820,7 +2914,7 @@ __global__ __launch_bounds__(256) void k_export_b(DevStore s, const uint64_t *sl
     uint64_t bo = mem_out_off[u];
     for (uint32_t c = 0; c < h.nslots; c += 64) {
         const uint32_t i = c + ln;
-        const uint32_t w = i < h.nslots ? W[i] : 0u;
+        const uint32_t w = i < h.nslots ? W[i] & ~h.dmask : 0u;
         const uint32_t pc = (uint32_t)__popc(w);
         const uint32_t incl = wave_incl32(pc);
         for (uint32_t k = 0; k < pc; ++k) backpos[bo + incl - pc + k] = s.back[h.back_off + B[i] + k];
@@ -3049,13 +3143,19 @@ void launch_freelist_merge(DevStore s, hipStream_t st)
 
 void launch_insert_plan(DevStore s, const DevEdge *sorted, Runs r, int64_t k, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_insert_plan, dim3(run_blocks(k)), dim3(256), 0, st, s, sorted, r);
+    if (s.rate16)
+        hipLaunchKernelGGL(k_insert_plan<true>, dim3(run_blocks(k)), dim3(256), 0, st, s, sorted, r);
+    else
+        hipLaunchKernelGGL(k_insert_plan<false>, dim3(run_blocks(k)), dim3(256), 0, st, s, sorted, r);
 }
 
 void launch_insert_apply(DevStore s, const DevEdge *sorted, const uint32_t *frac, Runs r, int64_t k, hipStream_t st)
 {
     // same grid as the plan: workgroup b takes the index ranges workgroup b reserved
-    hipLaunchKernelGGL(k_insert_apply, dim3(run_blocks(k)), dim3(256), 0, st, s, sorted, frac, r);
+    if (s.rate16)
+        hipLaunchKernelGGL(k_insert_apply<true>, dim3(run_blocks(k)), dim3(256), 0, st, s, sorted, frac, r);
+    else
+        hipLaunchKernelGGL(k_insert_apply<false>, dim3(run_blocks(k)), dim3(256), 0, st, s, sorted, frac, r);
     hipLaunchKernelGGL(k_commit_tops, dim3(1), dim3(64), 0, st, s.ctl);
 }
 
@@ -3087,47 +3187,54 @@ void launch_delete_big_list(Runs r, int64_t k, uint32_t *big, Ctl *ctl, hipStrea
 
 void launch_delete_apply(DevStore s, const uint32_t *slots, Runs r, const uint32_t *big, int64_t k, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_delete_apply, dim3(run_blocks(k)), dim3(256), 0, st, s, slots, r, big);
     const int64_t nbig_max = k / kBigDelete + 1;  // waves loop over the list: at most 1024 blocks
-    hipLaunchKernelGGL(k_delete_finish, dim3(std::min<unsigned>(blocks_for(nbig_max * 64, 256), 1024u)), dim3(256), 0,
-                       st, s, slots, r, big);
+    const dim3 fin(std::min<unsigned>(blocks_for(nbig_max * 64, 256), 1024u));
+    if (s.rate16) {
+        hipLaunchKernelGGL(k_delete_apply<true>, dim3(run_blocks(k)), dim3(256), 0, st, s, slots, r, big);
+        hipLaunchKernelGGL(k_delete_finish<true>, fin, dim3(256), 0, st, s, slots, r, big);
+    } else {
+        hipLaunchKernelGGL(k_delete_apply<false>, dim3(run_blocks(k)), dim3(256), 0, st, s, slots, r, big);
+        hipLaunchKernelGGL(k_delete_finish<false>, fin, dim3(256), 0, st, s, slots, r, big);
+    }
+}
+
+template <bool FW, bool DN>
+static void launch_walk_fd(DevStore s, int app, const WalkArgs &a, dim3 grid, dim3 blk, hipStream_t st)
+{
+    switch (app) {
+    case 0: hipLaunchKernelGGL((k_walk<0, FW, DN>), grid, blk, 0, st, s, a); break;
+    case 1: hipLaunchKernelGGL((k_walk<1, FW, DN>), grid, blk, 0, st, s, a); break;
+    case 2: hipLaunchKernelGGL((k_walk<2, FW, DN>), grid, blk, 0, st, s, a); break;
+    default: hipLaunchKernelGGL((k_walk<3, FW, DN>), grid, blk, 0, st, s, a); break;
+    }
 }
 
 void launch_walk(DevStore s, int app, const WalkArgs &a, int block, hipStream_t st)
 {
     if (block != 64 && block != 128 && block != 256) block = 256;
     const dim3 grid(blocks_for(a.nwalkers, block)), blk(block);
-    if (s.fw) {
-        switch (app) {
-        case 0: hipLaunchKernelGGL((k_walk<0, true>), grid, blk, 0, st, s, a); break;
-        case 1: hipLaunchKernelGGL((k_walk<1, true>), grid, blk, 0, st, s, a); break;
-        case 2: hipLaunchKernelGGL((k_walk<2, true>), grid, blk, 0, st, s, a); break;
-        default: hipLaunchKernelGGL((k_walk<3, true>), grid, blk, 0, st, s, a); break;
-        }
-        return;
-    }
-    switch (app) {
-    case 0: hipLaunchKernelGGL((k_walk<0, false>), grid, blk, 0, st, s, a); break;
-    case 1: hipLaunchKernelGGL((k_walk<1, false>), grid, blk, 0, st, s, a); break;
-    case 2: hipLaunchKernelGGL((k_walk<2, false>), grid, blk, 0, st, s, a); break;
-    default: hipLaunchKernelGGL((k_walk<3, false>), grid, blk, 0, st, s, a); break;
-    }
+    if (s.fw)
+        s.rate16 ? launch_walk_fd<true, true>(s, app, a, grid, blk, st) : launch_walk_fd<true, false>(s, app, a, grid, blk, st);
+    else
+        s.rate16 ? launch_walk_fd<false, true>(s, app, a, grid, blk, st)
+                 : launch_walk_fd<false, false>(s, app, a, grid, blk, st);
 }
 
 int serve_walkers_per_lane(int cls) { return serve_wpl(cls); }
 
-template <bool FW>
+template <bool FW, bool DN>
 static const void *serve_fn(int cls)
 {
-    return cls == kServeN2V     ? reinterpret_cast<const void *>(&k_walk_serve<FW, kServeN2V>)
-           : cls == kServeOther ? reinterpret_cast<const void *>(&k_walk_serve<FW, kServeOther>)
-                                : reinterpret_cast<const void *>(&k_walk_serve<FW, kServeMixed>);
+    return cls == kServeN2V     ? reinterpret_cast<const void *>(&k_walk_serve<FW, kServeN2V, DN>)
+           : cls == kServeOther ? reinterpret_cast<const void *>(&k_walk_serve<FW, kServeOther, DN>)
+                                : reinterpret_cast<const void *>(&k_walk_serve<FW, kServeMixed, DN>);
 }
 
-int serve_max_blocks_per_cu(int fw, int cls)
+int serve_max_blocks_per_cu(int fw, int cls, int dense)
 {
     int nb = 0;
-    const void *fn = fw ? serve_fn<true>(cls) : serve_fn<false>(cls);
+    const void *fn = fw ? (dense ? serve_fn<true, true>(cls) : serve_fn<true, false>(cls))
+                        : (dense ? serve_fn<false, true>(cls) : serve_fn<false, false>(cls));
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, 0) != hipSuccess) {
         (void)hipGetLastError();
         return 1;
@@ -3135,26 +3242,28 @@ int serve_max_blocks_per_cu(int fw, int cls)
     return nb > 0 ? nb : 1;
 }
 
-template <bool FW>
+template <bool FW, bool DN>
 static void launch_serve_cls(DevStore s, const ServeReq *reqs, int nreq, unsigned long long total,
                              unsigned long long *cursor, int cls, int blocks, hipStream_t st)
 {
     const dim3 g(blocks), b(256);
     if (cls == kServeN2V)
-        hipLaunchKernelGGL((k_walk_serve<FW, kServeN2V>), g, b, 0, st, s, reqs, nreq, total, cursor);
+        hipLaunchKernelGGL((k_walk_serve<FW, kServeN2V, DN>), g, b, 0, st, s, reqs, nreq, total, cursor);
     else if (cls == kServeOther)
-        hipLaunchKernelGGL((k_walk_serve<FW, kServeOther>), g, b, 0, st, s, reqs, nreq, total, cursor);
+        hipLaunchKernelGGL((k_walk_serve<FW, kServeOther, DN>), g, b, 0, st, s, reqs, nreq, total, cursor);
     else
-        hipLaunchKernelGGL((k_walk_serve<FW, kServeMixed>), g, b, 0, st, s, reqs, nreq, total, cursor);
+        hipLaunchKernelGGL((k_walk_serve<FW, kServeMixed, DN>), g, b, 0, st, s, reqs, nreq, total, cursor);
 }
 
 void launch_walk_serve(DevStore s, const ServeReq *reqs, int nreq, unsigned long long total,
                        unsigned long long *cursor, int cls, int blocks, hipStream_t st)
 {
     if (s.fw)
-        launch_serve_cls<true>(s, reqs, nreq, total, cursor, cls, blocks, st);
+        s.rate16 ? launch_serve_cls<true, true>(s, reqs, nreq, total, cursor, cls, blocks, st)
+                 : launch_serve_cls<true, false>(s, reqs, nreq, total, cursor, cls, blocks, st);
     else
-        launch_serve_cls<false>(s, reqs, nreq, total, cursor, cls, blocks, st);
+        s.rate16 ? launch_serve_cls<false, true>(s, reqs, nreq, total, cursor, cls, blocks, st)
+                 : launch_serve_cls<false, false>(s, reqs, nreq, total, cursor, cls, blocks, st);
 }
 
 size_t persist_lds_bytes(int block, int wpl, int depth)
@@ -3162,10 +3271,10 @@ size_t persist_lds_bytes(int block, int wpl, int depth)
     return depth ? (size_t)block * wpl * (depth + 1) * sizeof(int32_t) : 0;
 }
 
-template <int APP, bool FW>
+template <int APP, bool FW, bool DN>
 static const void *persist_fn(int wpl, int depth)
 {
-#define BINGO_PF(W, D) reinterpret_cast<const void *>(&k_walk_persist<APP, W, D, FW>)
+#define BINGO_PF(W, D) reinterpret_cast<const void *>(&k_walk_persist<APP, W, D, FW, DN>)
     switch (wpl * 100 + depth) {
     case 100: return BINGO_PF(1, 0);
     case 104: return BINGO_PF(1, 4);
@@ -3181,27 +3290,26 @@ static const void *persist_fn(int wpl, int depth)
 #undef BINGO_PF
 }
 
-static const void *persist_kernel(int app, int wpl, int depth, int fw)
+template <bool FW, bool DN>
+static const void *persist_kernel_fd(int app, int wpl, int depth)
 {
-    if (fw) {
-        switch (app) {
-        case 0: return persist_fn<0, true>(wpl, depth);
-        case 1: return persist_fn<1, true>(wpl, depth);
-        case 2: return persist_fn<2, true>(wpl, depth);
-        default: return persist_fn<3, true>(wpl, depth);
-        }
-    }
     switch (app) {
-    case 0: return persist_fn<0, false>(wpl, depth);
-    case 1: return persist_fn<1, false>(wpl, depth);
-    case 2: return persist_fn<2, false>(wpl, depth);
-    default: return persist_fn<3, false>(wpl, depth);
+    case 0: return persist_fn<0, FW, DN>(wpl, depth);
+    case 1: return persist_fn<1, FW, DN>(wpl, depth);
+    case 2: return persist_fn<2, FW, DN>(wpl, depth);
+    default: return persist_fn<3, FW, DN>(wpl, depth);
     }
 }
 
-int persist_max_blocks_per_cu(int app, int wpl, int depth, int block, int fw)
+static const void *persist_kernel(int app, int wpl, int depth, int fw, int dense)
 {
-    const void *fn = persist_kernel(app, wpl, depth, fw);
+    if (fw) return dense ? persist_kernel_fd<true, true>(app, wpl, depth) : persist_kernel_fd<true, false>(app, wpl, depth);
+    return dense ? persist_kernel_fd<false, true>(app, wpl, depth) : persist_kernel_fd<false, false>(app, wpl, depth);
+}
+
+int persist_max_blocks_per_cu(int app, int wpl, int depth, int block, int fw, int dense)
+{
+    const void *fn = persist_kernel(app, wpl, depth, fw, dense);
     if (!fn) return 0;
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, persist_lds_bytes(block, wpl, depth)) !=
@@ -3214,18 +3322,18 @@ int persist_max_blocks_per_cu(int app, int wpl, int depth, int block, int fw)
 
 void launch_walk_persist(DevStore s, int app, const WalkArgs &a, const PersistCfg &c, hipStream_t st)
 {
-    const void *fn = persist_kernel(app, c.wpl, c.depth, s.fw);
+    const void *fn = persist_kernel(app, c.wpl, c.depth, s.fw, s.rate16 != 0);
     if (!fn) return;
     void *args[] = {&s, const_cast<WalkArgs *>(&a)};
     (void)hipLaunchKernel(fn, dim3(c.blocks), dim3(c.block), args, persist_lds_bytes(c.block, c.wpl, c.depth), st);
 }
 
 void launch_export_a(DevStore s, uint32_t *nslots, uint64_t *mass, uint32_t *bsize, uint64_t *T, uint8_t *alias,
-                     uint64_t *fmass, uint64_t *fthr, hipStream_t st)
+                     uint64_t *fmass, uint64_t *fthr, uint32_t *dmask, hipStream_t st)
 {
     const int64_t total = ((int64_t)s.n + 1) * kGroups;
     hipLaunchKernelGGL(k_export_a, dim3(blocks_for(total, 256)), dim3(256), 0, st, s, nslots, mass, bsize, T, alias,
-                       fmass, fthr);
+                       fmass, fthr, dmask);
 }
 
 void launch_export_b(DevStore s, const uint64_t *slot_out_off, const uint64_t *mem_out_off, uint32_t *slot_v,

@@ -447,7 +447,7 @@ int bingo_server_create(bingo_graph *g, const bingo_server_config *c, bingo_serv
     }
     sv->nstreams = sv->cfg.streams ? sv->cfg.streams : 2;
     for (int c = 0; c < 3; ++c) {
-        const int occ = serve_max_blocks_per_cu(g->s.fw, c);
+        const int occ = serve_max_blocks_per_cu(g->s.fw, c, g->s.rate16 != 0);
         sv->blocks_per_cu[c] = sv->cfg.blocks_per_cu > 0 ? std::min(sv->cfg.blocks_per_cu, occ) : occ;
     }
     (void)hipSetDevice(g->device);
@@ -481,7 +481,7 @@ int bingo_server_submit_walk(bingo_server *sv, int app, const bingo_walk_params 
     if (!sv) return fail(BINGO_EINVAL, "null server");
     bingo_graph *g = sv->g;
     auto r = std::make_shared<Req>();
-    int rc = fill_walk_args(params, app, &r->args, g->s.fw);
+    int rc = fill_walk_args(params, app, &r->args, g->s.fw, g->s.rate16 != 0);
     if (rc) return rc;
     if (nwalkers < 1 || !paths) return fail(BINGO_EINVAL, "bad walk request");
     if (app == BINGO_NODE2VEC && !g->s.nbr) return fail(BINGO_EUNSUPPORTED, "node2vec needs neighbor_index=1");

@@ -40,7 +40,8 @@ struct NodeHdr {
     uint32_t back_used, back_cap;
     uint64_t hash_off;
     uint32_t hash_cap, hash_used;
-    uint32_t nlive, spare;       // live slots (nslots - nlive tombstones, reclaimed by compaction)
+    uint32_t nlive;              // live slots (nslots - nlive tombstones, reclaimed by compaction)
+    uint32_t dmask;              // dense bit groups (count only, no member list; DevStore::rate16)
     uint64_t spare2;
 };
 
@@ -63,11 +64,15 @@ constexpr uint32_t kFracIters = 4096u;         // fractional rejection rounds pe
 constexpr int32_t kMaxAttemptsFloat = 65536;   // node2vec cap with float weights (attempt in 16 bits)
 constexpr uint32_t kThrHiMax = (1u << 17) - 1;  // the alias slot's 17-bit copy of thr saturates here
 constexpr uint64_t kHmetaMask = (1ull << 42) - 1;
+constexpr uint32_t kDenseTag = 0x20000000u;      // Philox counter word 3 of the dense-group rejection stream
+constexpr uint32_t kDenseBlocks = 1024u;         // its Philox blocks per draw (four trials each)
+constexpr int32_t kMaxAttemptsDense = 65536;     // node2vec cap with dense groups (attempt in 16 bits)
 
 // One 32-byte alias slot i of a node:
 //   T      stay (group i) iff coin64 < T, else the partner
 //   first  locator of the likelier outcome: group i if T >= 2^63, else the
-//          partner (member offset (36 bits) | size << 36)
+//          partner (member offset (36 bits) | size << 36; for a dense group
+//          kDenseLoc | its index, size field 0; 0 for an empty group)
 //   second locator of the other outcome
 // (the first 16 bytes decide most draws alone: a draw reads the second
 // locator only when the less likely outcome comes up)
@@ -90,6 +95,7 @@ struct AliasEnt {
 };
 constexpr int kOffBits = 36;
 constexpr uint64_t kOffMask36 = (1ull << kOffBits) - 1ull;
+constexpr uint64_t kDenseLoc = 1ull << 35;  // a locator with size field 0 and this bit: a dense group
 constexpr uint32_t kMaxGroupSize = 1u << 28;  // sizes live in 28 bits
 static_assert(sizeof(AliasEnt) == 32, "alias entry is one 32-byte read");
 static_assert(sizeof(NodeHdr) == 64, "node header layout");
@@ -106,6 +112,7 @@ struct Ctl {
     unsigned long long attempts;     // last walk: sample draws (node2vec rejections included)
     unsigned long long probes;       // last walk: neighbour-table membership lookups
     unsigned long long folds;        // last walk: node2vec return-weight lookups (neighbour entries)
+    unsigned long long dtrials;      // last walk: slot weights read by dense-group rejection draws
     unsigned long long cursor;       // persistent walk: next unclaimed walker
     unsigned int nbigcopy;           // insert plan: runs that move a region of kBigCopyBytes or more
     unsigned int werr;               // walks since the last walk reset: ERR_INVAL (a start or length out of range)
@@ -196,7 +203,13 @@ struct DevStore {
     // 2^16); neighbour tables stay powers of two.
     uint32_t fit;
     int32_t fw;      // weights are float bits: slot regions carry a fraction array
-    int32_t pad;
+    // Sparse/dense groups (ref RATE, include/api.h:25, operation.cuh:72): 0,
+    // or the rate x 2^16.  The batch that gives a node its first slots sets
+    // its dense groups: j with count_j >= 1 and count_j 2^16 >= rate16 len
+    // over that batch's arcs of the node.  A dense group keeps only its count
+    // (GroupHdr::size, cap 0) and no back entries; draws from it reject over
+    // the node's slots (dense_draw).
+    uint32_t rate16;
     FNode *fnode;    // [n+1] when fw
     uint64_t *nmass; // [n+1] when nbr: each node's integer weight S (node2vec's folded return region)
     NodeHdr *hdr;
@@ -302,17 +315,17 @@ struct PersistCfg {
     int block, wpl, depth, blocks;  // blocks = grid size
 };
 size_t persist_lds_bytes(int block, int wpl, int depth);
-int persist_max_blocks_per_cu(int app, int wpl, int depth, int block, int fw);
+int persist_max_blocks_per_cu(int app, int wpl, int depth, int block, int fw, int dense);
 void launch_walk_persist(DevStore s, int app, const WalkArgs &a, const PersistCfg &c, hipStream_t st);
 // cls: 0 = the launch holds no node2vec request, 1 = only node2vec
 // requests, 2 = any mix (a kernel variant per class)
 void launch_walk_serve(DevStore s, const ServeReq *reqs, int nreq, unsigned long long total,
                        unsigned long long *cursor, int cls, int blocks, hipStream_t st);
-int serve_max_blocks_per_cu(int fw, int cls);
+int serve_max_blocks_per_cu(int fw, int cls, int dense);
 int serve_walkers_per_lane(int cls);
 // fmass/fthr (export_a) and slot_f (export_b) are null unless fw
 void launch_export_a(DevStore s, uint32_t *nslots, uint64_t *mass, uint32_t *bsize, uint64_t *T, uint8_t *alias,
-                     uint64_t *fmass, uint64_t *fthr, hipStream_t st);
+                     uint64_t *fmass, uint64_t *fthr, uint32_t *dmask, hipStream_t st);
 void launch_export_b(DevStore s, const uint64_t *slot_out_off, const uint64_t *mem_out_off, uint32_t *slot_v,
                      uint32_t *slot_w, uint64_t *members, uint32_t *backpos, uint32_t *slot_f, hipStream_t st);
 // Compaction of tombstoned slots: nodes with dead = nslots - nlive >= 1 and

@@ -67,6 +67,14 @@ typedef struct {
                                growing, pool addresses stable; at least 1 GiB per pool, so for
                                large stores); 0: a growing pool is copied to a larger
                                allocation.  Falls back to copying where mapping is refused. */
+    double dense_rate;      /* ref RATE (api.h:25, operation.cuh:72): 0 (default) or >= 1: every
+                               non-empty bit group keeps a member list; in (0, 1): the batch that
+                               gives a node its first slots classes its groups, and group j is
+                               dense when count_j >= 1 and count_j >= rate * (that batch's arcs of
+                               the node), rate taken to 2^-16.  A dense group keeps only its count;
+                               walks draw from it by rejection over the node's slots (ref
+                               Bingo.cuh:244-252).  Less HBM, costlier draws from dense groups;
+                               with it, max_attempts <= 65536. */
 } bingo_config;
 
 typedef struct {
@@ -75,7 +83,7 @@ typedef struct {
     double tp;              /* PPR stop probability per step (ref TP=0.0125) */
     int32_t max_len;        /* path columns, vertices per walk incl. start (ref LEN=80) */
     int32_t max_attempts;   /* node2vec rejection cap per step; last draw accepted (<= 65536
-                               with float weights) */
+                               with float weights or dense groups) */
     int32_t form;           /* BINGO_FORM_*: how walkers are scheduled (paths are the same) */
     int32_t block;          /* threads per block, 64..512; 0 = tuned or default */
     int32_t walkers_per_lane; /* persistent form: 1, 2 or 4 walker slots per lane; 0 = tuned */
@@ -102,6 +110,8 @@ typedef struct {
     uint64_t bytes_reserved;/* HBM held by the store */
     uint64_t pool_used[4];  /* slot, back-index, member, neighbour pools (elements) */
     double compact_ms;      /* HIP-event time of the last bingo_compact */
+    uint64_t walk_dense_trials; /* slot weights read by the last walk's draws from dense groups
+                               (rejection trials; dense_rate handles) */
 } bingo_stats;
 
 /* Flat export for bit-exact diffs (node-major; see bingo_export). */
@@ -115,12 +125,14 @@ typedef struct {
     uint8_t *alias;     /* [(n+1)*32] alias partners */
     uint32_t *slot_v;   /* [total_slots] */
     uint32_t *slot_w;   /* [total_slots] 0 = deleted */
-    uint64_t *members;  /* [total_members] (v << 32) | slot, per node, per group */
-    uint32_t *backpos;  /* [total_members] per live slot, its set bits ascending */
+    uint64_t *members;  /* [total_members] (v << 32) | slot, per node, per listed group */
+    uint32_t *backpos;  /* [total_members] per live slot, its listed set bits ascending */
     /* float_weights handles only (NULL to skip; slot_f with the slot arrays): */
     uint32_t *slot_f;   /* [total_slots] fraction of each slot's weight, x 2^32 (slot_w: integer part) */
     uint64_t *fmass;    /* [n+1] sum of the live fractions, x 2^32 */
     uint64_t *fthr;     /* [n+1] fractional-part threshold floor(2^64 F / (S 2^32 + F)) */
+    uint32_t *dmask;    /* [n+1] dense groups of each node, bit j (NULL to skip); a dense group
+                           has no members or back positions in the arrays above */
 } bingo_dump;
 
 /* ---- lifetime ---- */

@@ -46,12 +46,14 @@ typedef struct {
     uint8_t alias[ORC_GROUPS];
     uint32_t *fr;                    /* float weights: per slot fraction x 2^32 */
     uint64_t fmass, fthr;            /* sum of fractions; fractional-part threshold */
+    uint32_t dmask;                  /* dense groups: count only, no member list */
 } node_t;
 
 struct orc_graph {
     int32_t n;
     int nbr;
     int fw;                          /* weights are float bits */
+    uint32_t rate16;                 /* sparse/dense rate x 2^16 (0: every group listed) */
     node_t *nodes;
 };
 
@@ -72,10 +74,11 @@ static uint32_t grow_cap(uint32_t need, uint32_t minc)
 static inline int popc32(uint32_t x) { return __builtin_popcount(x); }
 static inline int hibit32(uint32_t x) { return 31 - __builtin_clz(x); }
 
-/* position of slot s inside bucket j (s must have bit j set) */
+/* position of slot s inside bucket j (s must have bit j set, j listed):
+ * a slot's back entries are those of its listed set bits, ascending */
 static inline uint32_t *backref(node_t *x, uint32_t s, int j)
 {
-    return &x->back[x->boff[s] + popc32(x->w[s] & ((1u << j) - 1u))];
+    return &x->back[x->boff[s] + popc32(x->w[s] & ~x->dmask & ((1u << j) - 1u))];
 }
 
 /* ---------------- neighbour multiset (node2vec membership) ---------------- */
@@ -225,15 +228,21 @@ static void node_alias(node_t *x)
 
 /* ---------------- graph ---------------- */
 
-orc_graph *orc_create(int32_t n, int neighbor_index) { return orc_create_ex(n, neighbor_index, 0); }
+orc_graph *orc_create(int32_t n, int neighbor_index) { return orc_create_rate(n, neighbor_index, 0, 0); }
 
 orc_graph *orc_create_ex(int32_t n, int neighbor_index, int float_weights)
+{
+    return orc_create_rate(n, neighbor_index, float_weights, 0);
+}
+
+orc_graph *orc_create_rate(int32_t n, int neighbor_index, int float_weights, uint32_t dense_rate16)
 {
     if (n < 1) return NULL;
     orc_graph *g = (orc_graph *)calloc(1, sizeof(orc_graph));
     g->n = n;
     g->nbr = neighbor_index;
     g->fw = float_weights != 0;
+    g->rate16 = dense_rate16;
     g->nodes = (node_t *)calloc((size_t)n + 1, sizeof(node_t));
     if (!g->nodes) abort();
     for (int32_t u = 0; u <= n; ++u) node_alias(&g->nodes[u]);
@@ -256,7 +265,8 @@ int32_t orc_num_nodes(const orc_graph *g) { return g->n; }
 
 /* Append one arc as a new slot: slot = nslots++, and for each set bit j in
  * ascending order the pair (v, slot) is appended to bucket j
- * (Bingo.cuh:182-198 insert; Bingo.cuh:159-181 build uses the same order). */
+ * (Bingo.cuh:182-198 insert; Bingo.cuh:159-181 build uses the same order);
+ * a dense group only counts the arc (no member, no back entry). */
 static void node_append(node_t *x, uint32_t v, uint32_t w, uint32_t f, int fw, int nbr)
 {
     uint32_t s = x->nslots;
@@ -274,13 +284,17 @@ static void node_append(node_t *x, uint32_t v, uint32_t w, uint32_t f, int fw, i
         x->fmass += f;
     }
     x->boff[s] = x->back_used;
-    uint32_t pc = (uint32_t)popc32(w);
+    uint32_t pc = (uint32_t)popc32(w & ~x->dmask);
     if (x->back_used + pc > x->back_cap) {
         x->back_cap = grow_cap(x->back_used + pc, 16);
         x->back = (uint32_t *)xrealloc(x->back, (size_t)x->back_cap * 4);
     }
     for (uint32_t bits = w; bits; bits &= bits - 1) {
         int j = __builtin_ctz(bits);
+        if (x->dmask >> j & 1u) {
+            x->size[j]++;
+            continue;
+        }
         if (x->size[j] == x->cap[j]) {
             x->cap[j] = grow_cap(x->size[j] + 1, 4);
             x->mem[j] = (uint64_t *)xrealloc(x->mem[j], (size_t)x->cap[j] * 8);
@@ -357,6 +371,21 @@ int orc_insert(orc_graph *g, const orc_edge *e, int64_t k)
 #pragma omp parallel for schedule(dynamic, 64)
     for (int64_t r = 0; r < G.nruns; ++r) {
         node_t *x = &g->nodes[G.run_node[r]];
+        if (!x->nslots) {
+            /* the node's first slots: its group classes, from this run
+             * (ref build, operation.cuh:57-72): dense iff count_j >= 1 and
+             * count_j 2^16 >= rate16 len */
+            uint32_t cnt[32] = {0};
+            const uint64_t len = (uint64_t)(G.runs[r + 1] - G.runs[r]);
+            for (int64_t t = G.runs[r]; t < G.runs[r + 1]; ++t) {
+                uint32_t w = e[G.order[t]].w, f;
+                if (g->fw) orc_split_float(e[G.order[t]].w, &w, &f);
+                for (uint32_t b = w; b; b &= b - 1) cnt[__builtin_ctz(b)]++;
+            }
+            x->dmask = 0;
+            for (int j = 0; g->rate16 && j < 32; ++j)
+                if (cnt[j] && ((uint64_t)cnt[j] << 16) >= (uint64_t)g->rate16 * len) x->dmask |= 1u << j;
+        }
         for (int64_t t = G.runs[r]; t < G.runs[r + 1]; ++t) {
             const orc_edge *a = &e[G.order[t]];
             uint32_t w = a->w, f = 0;
@@ -388,6 +417,10 @@ static void node_delete_run(node_t *x, const uint32_t *slots, int64_t cnt, int n
         uint32_t bit = 1u << j, kj = 0;
         for (int64_t t = 0; t < cnt; ++t) kj += (x->w[slots[t]] & bit) != 0;
         if (!kj) continue;
+        if (x->dmask & bit) {  /* a dense group only counts */
+            x->size[j] -= kj;
+            continue;
+        }
         uint32_t S = x->size[j], S2 = S - kj;
         uint64_t *m = x->mem[j];
         for (int64_t t = 0; t < cnt; ++t)
@@ -481,8 +514,8 @@ int64_t orc_compact(orc_graph *g, double min_dead_frac, uint64_t *slots_freed)
         for (uint32_t s = 0; s < x->nslots; ++s) {
             const uint32_t w = x->w[s];
             if (!w && !(x->fr && x->fr[s])) continue;
-            const uint32_t b = x->boff[s], pc = (uint32_t)popc32(w);
-            uint32_t bits = w;
+            const uint32_t b = x->boff[s], pc = (uint32_t)popc32(w & ~x->dmask);
+            uint32_t bits = w & ~x->dmask;
             for (uint32_t r = 0; r < pc; ++r) {  /* nb <= b: a forward copy */
                 const int j = __builtin_ctz(bits);
                 bits &= bits - 1;
@@ -518,17 +551,49 @@ uint32_t orc_edge_count(const orc_graph *g, int32_t u, int32_t v)
 
 /* ---------------- sampling ---------------- */
 
+/* A draw from dense group g of node x (no member list): rejection over the
+ * node's slots, as the reference samples a group classed dense
+ * (Bingo.cuh:244-252: a uniform slot until its weight has bit g).  Trials
+ * come four per Philox block (wid; step, DENSE_TAG | it << 16 | attempt),
+ * words 0..3 in order, slot floor(word * nslots / 2^32); tombstones have no
+ * bits.  After ORC_DENSE_BLOCKS blocks without a hit (probability below
+ * (1 - count/nslots)^4096), the first slot with bit g after the last
+ * trial's slot, cyclically. */
+static int32_t node_dense_draw(const node_t *x, int g, const uint32_t key[2], uint64_t wid, uint32_t step,
+                               uint32_t att)
+{
+    uint32_t b[4], ctr[4] = {(uint32_t)wid, (uint32_t)(wid >> 32), step, 0};
+    uint32_t sl = 0;
+    for (uint32_t it = 0; it < ORC_DENSE_BLOCKS; ++it) {
+        ctr[3] = ORC_DENSE_TAG | (it << 16) | att;
+        orc_philox4x32_10(ctr, key, b);
+        for (int q = 0; q < 4; ++q) {
+            sl = (uint32_t)(((uint64_t)b[q] * x->nslots) >> 32);
+            if (x->w[sl] >> g & 1u) return (int32_t)x->v[sl];
+        }
+    }
+    for (uint32_t t = 1; t <= x->nslots; ++t) {
+        uint32_t s2 = sl + t;
+        if (s2 >= x->nslots) s2 -= x->nslots;
+        if (x->w[s2] >> g & 1u) return (int32_t)x->v[s2];
+    }
+    return -1;
+}
+
 /* One draw (Bingo.cuh:220-267): slot i = top 5 bits of x0, 64-bit coin
  * (x1:x2) against T_i picks group i or its alias, then member
- * floor(x3 * size / 2^32) of that group.  Returns -1 on a node without live
- * arcs (the reference's aliasTableSZ == 0 case). */
-static inline int32_t node_sample(const node_t *x, const uint32_t r[4])
+ * floor(x3 * size / 2^32) of that group, or for a dense group the rejection
+ * draw of node_dense_draw.  Returns -1 on a node without live arcs (the
+ * reference's aliasTableSZ == 0 case). */
+static inline int32_t node_sample(const node_t *x, const uint32_t r[4], const uint32_t key[2], uint64_t wid,
+                                  uint32_t step, uint32_t att)
 {
     uint32_t i = r[0] >> 27;
     uint64_t c = ((uint64_t)r[1] << 32) | r[2];
     uint32_t gsel = (c < x->T[i]) ? i : x->alias[i];
     uint32_t sz = x->size[gsel];
     if (!sz) return -1;
+    if (x->dmask >> gsel & 1u) return node_dense_draw(x, (int)gsel, key, wid, step, att);
     uint32_t pos = (uint32_t)(((uint64_t)r[3] * sz) >> 32);
     return (int32_t)(x->mem[gsel][pos] >> 32);
 }
@@ -561,13 +626,15 @@ static int32_t node_draw(const orc_graph *g, const node_t *x, const uint32_t r[4
             }
         }
     }
-    return node_sample(x, r);
+    return node_sample(x, r, key, wid, step, att);
 }
 
+/* dense groups draw from the stream of walker 0, step 0, attempt 0, key 0 */
 int32_t orc_sample_words(const orc_graph *g, int32_t u, const uint32_t x[4])
 {
+    static const uint32_t key0[2] = {0, 0};
     if (u < 1 || u > g->n) return -1;
-    return node_sample(&g->nodes[u], x);
+    return node_sample(&g->nodes[u], x, key0, 0, 0, 0);
 }
 
 static uint32_t thr27(double prob)
@@ -696,6 +763,7 @@ int orc_walk_ids(const orc_graph *g, int app, const orc_walk_params *p, const in
         return ORC_EINVAL;
     if (app == ORC_NODE2VEC && p->max_attempts < 1) return ORC_EINVAL;
     if (g->fw && p->max_attempts > ORC_MAX_ATTEMPTS_FLOAT) return ORC_EINVAL;
+    if (g->rate16 && p->max_attempts > ORC_MAX_ATTEMPTS_DENSE) return ORC_EINVAL;
     if (app == ORC_NODE2VEC && p->fold_r && (g->fw || !g->nbr)) return ORC_EINVAL;  /* folding needs weight sums */
     for (int64_t i = 0; starts && i < nwalkers; ++i)
         if (starts[i] < 1 || starts[i] > g->n) return ORC_EINVAL;
@@ -723,7 +791,7 @@ int orc_export(const orc_graph *g, orc_dump *d)
     for (int32_t u = 0; u <= g->n; ++u) {
         const node_t *x = &g->nodes[u];
         ts += x->nslots;
-        for (int j = 0; j < 32; ++j) tm += x->size[j];
+        for (int j = 0; j < 32; ++j) tm += (x->dmask >> j & 1u) ? 0 : x->size[j];
     }
     d->total_slots = ts;
     d->total_members = tm;
@@ -737,16 +805,18 @@ int orc_export(const orc_graph *g, orc_dump *d)
             d->bsize[(size_t)u * 32 + j] = x->size[j];
             d->T[(size_t)u * 32 + j] = x->T[j];
             d->alias[(size_t)u * 32 + j] = x->alias[j];
+            if (x->dmask >> j & 1u) continue;
             for (uint32_t t = 0; t < x->size[j]; ++t) d->members[mo++] = x->mem[j][t];
         }
         if (d->fmass) d->fmass[u] = x->fmass;
         if (d->fthr) d->fthr[u] = x->fthr;
+        if (d->dmask) d->dmask[u] = x->dmask;
         for (uint32_t s = 0; s < x->nslots; ++s) {
             d->slot_v[so] = x->v[s];
             d->slot_w[so] = x->w[s];
             if (d->slot_f) d->slot_f[so] = x->fr ? x->fr[s] : 0u;
             so++;
-            uint32_t w = x->w[s];
+            uint32_t w = x->w[s] & ~x->dmask;
             for (uint32_t k = 0; k < (uint32_t)popc32(w); ++k) d->backpos[bo++] = x->back[x->boff[s] + k];
         }
     }

@@ -18,7 +18,17 @@
  *
  * Deliberate deviations from the reference (each decided, each tested):
  *   - all 32 bit groups of a uint32 weight are kept (ref: LOGT/MAXLOG=16 cap);
- *   - every non-empty bit group keeps a member list (ref RATE>1 mode);
+ *   - sparse/dense groups (ref RATE, api.h:25, operation.cuh:72) as a policy
+ *     flag: off (every non-empty group keeps a member list, the reference's
+ *     RATE > 1) or a rate r in (0, 1) in units of 2^-16.  A node's classes
+ *     are decided by the batch that gives it its first slots (ref: at
+ *     build): group j is dense iff count_j >= 1 and count_j 2^16 >= r len
+ *     over that batch's arcs of the node (ref: dense also when empty; here
+ *     an empty group keeps a list).  A dense group keeps only its count; a
+ *     draw from it is rejection over the node's slots (ref Bingo.cuh:244-252)
+ *     from stream blocks (wid; step, DENSE_TAG | it << 16 | attempt), four
+ *     trials per block, up to ORC_DENSE_BLOCKS blocks, then the first slot
+ *     with the bit after the last trial (cyclic); see node_dense_draw;
  *   - members store (slot, target) so a walk step needs no edge-record read;
  *   - alias table: 32 fixed slots, integer-exact sweep construction with a
  *     64-bit stay threshold (ref: float/EPS table, Bingo.cuh:111-158);
@@ -55,6 +65,9 @@ extern "C" {
 #define ORC_MAX_SLOTS (1u << 28)      /* slots per node (the store's 28-bit group sizes) */
 #define ORC_MAX_MASS (1ull << 59)     /* integer weight per node (alias prefix sums stay in 64 bits) */
 #define ORC_MIX_BIT 0x80000000u       /* Philox counter word 2 (step) of the node2vec fold stream */
+#define ORC_DENSE_TAG 0x20000000u     /* Philox counter word 3 of the dense-group rejection stream */
+#define ORC_DENSE_BLOCKS 1024u        /* its Philox blocks per draw (four trials each) */
+#define ORC_MAX_ATTEMPTS_DENSE 65536  /* node2vec cap with dense groups (attempt fits 16 bits) */
 
 enum { ORC_DEEPWALK = 0, ORC_NODE2VEC = 1, ORC_PPR = 2, ORC_SAMPLING = 3 };
 enum { ORC_W_CONST1 = 0, ORC_W_UNIFORM16 = 1, ORC_W_LOGUNI32 = 2, ORC_W_ZIPF24 = 3 };
@@ -88,6 +101,9 @@ typedef struct {
 /* graph lifetime; float_weights != 0: edge weights are float bits */
 orc_graph *orc_create(int32_t n, int neighbor_index);
 orc_graph *orc_create_ex(int32_t n, int neighbor_index, int float_weights);
+/* dense_rate16: 0 = every non-empty group keeps a member list; otherwise
+ * the sparse/dense rate in units of 2^-16 (ref RATE 0.4 ~ 26214) */
+orc_graph *orc_create_rate(int32_t n, int neighbor_index, int float_weights, uint32_t dense_rate16);
 void orc_destroy(orc_graph *g);
 int32_t orc_num_nodes(const orc_graph *g);
 
@@ -125,7 +141,7 @@ uint64_t orc_fold_threshold(uint64_t S, uint64_t wret, uint64_t fold_r);
 /* state export for bit-exact diffs (flat, node-major) */
 typedef struct {
     uint64_t total_slots;    /* out: sum of nslots over nodes 0..n */
-    uint64_t total_members;  /* out: sum of bucket sizes */
+    uint64_t total_members;  /* out: sum of the sizes of groups with member lists */
     uint32_t *nslots;        /* [n+1] */
     uint64_t *mass;          /* [n+1] sum of live weights */
     uint32_t *bsize;         /* [(n+1)*32] */
@@ -133,12 +149,13 @@ typedef struct {
     uint8_t *alias;          /* [(n+1)*32] alias partners */
     uint32_t *slot_v;        /* [total_slots] */
     uint32_t *slot_w;        /* [total_slots] 0 = tombstone */
-    uint64_t *members;       /* [total_members] (v<<32)|slot, node-major, group-major */
-    uint32_t *backpos;       /* [total_members] live slots in order, set bits ascending */
+    uint64_t *members;       /* [total_members] (v<<32)|slot, node-major, group-major (lists only) */
+    uint32_t *backpos;       /* [total_members] live slots in order, set bits of listed groups ascending */
     /* float weights only (may be NULL): */
     uint32_t *slot_f;        /* [total_slots] fraction of each slot's weight, x 2^32 */
     uint64_t *fmass;         /* [n+1] sum of the fractions, x 2^32 */
     uint64_t *fthr;          /* [n+1] fractional-part threshold against a 64-bit coin */
+    uint32_t *dmask;         /* [n+1] dense groups of each node (bit j); may be NULL */
 } orc_dump;
 /* pass all pointers NULL to learn the totals, then again with buffers */
 int orc_export(const orc_graph *g, orc_dump *d);

@@ -51,6 +51,7 @@ class Dump(C.Structure):
         ("slot_f", C.c_void_p),
         ("fmass", C.c_void_p),
         ("fthr", C.c_void_p),
+        ("dmask", C.c_void_p),
     ]
 
 
@@ -88,6 +89,8 @@ def lib():
         L.orc_create.argtypes = [i32, C.c_int]
         L.orc_create_ex.restype = vp
         L.orc_create_ex.argtypes = [i32, C.c_int, C.c_int]
+        L.orc_create_rate.restype = vp
+        L.orc_create_rate.argtypes = [i32, C.c_int, C.c_int, C.c_uint32]
         L.orc_frac_threshold.restype = u64
         L.orc_frac_threshold.argtypes = [u64, u64]
         L.orc_split_float.restype = C.c_int
@@ -213,14 +216,25 @@ def split_float(bits):
     return ip.value, fr.value
 
 
+def dense_rate16(rate):
+    """The sparse/dense rate in units of 2^-16, as the product quantises its
+    configuration's dense_rate: 0 (off) for rate <= 0 or >= 1, else
+    round(rate 2^16) within [1, 65535]."""
+    if not (rate > 0.0) or rate >= 1.0:
+        return 0
+    return min(max(int(rate * 65536.0 + 0.5), 1), 65535)
+
+
 class OracleGraph:
     """Sequential CPU model of the store (n nodes, ids 1..n).  With
-    float_weights, the `w` field of edges carries IEEE-754 float bits."""
+    float_weights, the `w` field of edges carries IEEE-754 float bits;
+    dense_rate (ref RATE) classes groups sparse or dense (0: all listed)."""
 
-    def __init__(self, n, neighbor_index=True, float_weights=False):
+    def __init__(self, n, neighbor_index=True, float_weights=False, dense_rate=0.0):
         self.n = n
         self.float_weights = float_weights
-        self._h = lib().orc_create_ex(n, int(neighbor_index), int(float_weights))
+        self.rate16 = dense_rate16(dense_rate)
+        self._h = lib().orc_create_rate(n, int(neighbor_index), int(float_weights), self.rate16)
         if not self._h:
             raise ValueError("bad n")
 
@@ -303,6 +317,7 @@ class OracleGraph:
             slot_w=np.zeros(d.total_slots, np.uint32),
             members=np.zeros(d.total_members, np.uint64),
             backpos=np.zeros(d.total_members, np.uint32),
+            dmask=np.zeros(n1, np.uint32),
         )
         if self.float_weights:
             out.update(slot_f=np.zeros(d.total_slots, np.uint32), fmass=np.zeros(n1, np.uint64),

@@ -28,10 +28,19 @@ class BruteGraph:
         return [(s, v, w) for s, (v, w) in enumerate(self.slots[u]) if w]
 
 
+def dense_of(d, n):
+    """Dense-group masks of a dump, [n + 1] (zeros if it has none)."""
+    dm = d.get("dmask")
+    return np.zeros(n + 1, np.uint32) if dm is None else np.asarray(dm, np.uint32)
+
+
 def split_dump(d, n):
-    """Per-node views of a flat dump: slot arrays and per-group member lists."""
+    """Per-node views of a flat dump: slot arrays and per-group member lists
+    (empty for dense groups, which keep no list)."""
     so = np.concatenate([[0], np.cumsum(d["nslots"].astype(np.int64))])
-    msz = d["bsize"].astype(np.int64)
+    dm = dense_of(d, n)
+    dense = ((dm[:, None] >> np.arange(32, dtype=np.uint32)[None, :]) & 1).astype(bool)
+    msz = np.where(dense, 0, d["bsize"].astype(np.int64))
     mo = np.concatenate([[0], np.cumsum(msz.reshape(-1))])
     nodes = []
     for u in range(n + 1):
@@ -48,9 +57,11 @@ def split_dump(d, n):
 def check_against_brute(d, bg: BruteGraph):
     """Bucket j of node u must hold exactly the live slots with bit j set
     (as (v << 32) | slot), every live slot's back positions must point at its
-    own member entries, masses are count_j * 2^j summed."""
+    own member entries, masses are count_j * 2^j summed.  A dense group
+    (dmask bit) holds no list: its size is the count of those slots."""
     n = bg.n
     nodes = split_dump(d, n)
+    dm = dense_of(d, n)
     bp = 0
     for u in range(n + 1):
         sv, sw, groups = nodes[u]
@@ -61,13 +72,16 @@ def check_against_brute(d, bg: BruteGraph):
         mass = 0
         for j in range(32):
             want = sorted((v << 32) | s for s, (v, w) in enumerate(ref) if w >> j & 1)
-            got = sorted(int(x) for x in groups[j])
-            assert got == want, (u, j)
+            if int(dm[u]) >> j & 1:
+                assert len(groups[j]) == 0 and int(np.ravel(d["bsize"])[u * 32 + j]) == len(want), (u, j)
+            else:
+                got = sorted(int(x) for x in groups[j])
+                assert got == want, (u, j)
             mass += len(want) << j
         assert int(d["mass"][u]) == mass, u
         for s, (v, w) in enumerate(ref):
             for j in range(32):
-                if w >> j & 1:
+                if w >> j & 1 and not int(dm[u]) >> j & 1:
                     pos = int(d["backpos"][bp])
                     bp += 1
                     assert int(groups[j][pos]) == (v << 32) | s, (u, s, j)

@@ -1,6 +1,6 @@
 // Host code under AddressSanitizer and UndefinedBehaviorSanitizer: the
 // oracle (store, batched updates, compaction, every walk app, export, float
-// weights) and the library's host-side generators (R-MAT arcs, update
+// weights, dense groups) and the library's host-side generators (R-MAT arcs, update
 // streams with deletes, expiry and compaction renumbering).
 // Built and run by tests/test_sanitize_cpu.py; any report aborts the run.
 #include <cstdio>
@@ -28,6 +28,8 @@ static void export_all(orc_graph *g)
         slot_w(d.total_slots + 1), backpos(d.total_members + 1);
     std::vector<uint64_t> mass(n + 1), T((size_t)(n + 1) * 32), members(d.total_members + 1);
     std::vector<uint8_t> alias((size_t)(n + 1) * 32);
+    std::vector<uint32_t> dmask(n + 1);
+    d.dmask = dmask.data();
     d.nslots = nslots.data();
     d.mass = mass.data();
     d.bsize = bsize.data();
@@ -61,7 +63,9 @@ int main()
     const int32_t n = 1 << scale;
     uint32_t thr[3];
     orc_rmat_thresholds(0.57, 0.19, 0.19, thr);
-    for (int fw = 0; fw < 2; ++fw) {
+    // integer weights, float weights, integer weights with dense groups
+    for (int mode = 0; mode < 3; ++mode) {
+        const int fw = mode == 1;
         const int64_t m = 6000;
         std::vector<orc_edge> base((size_t)m * 2);
         orc_gen_rmat(scale, 0, m, thr, ORC_W_LOGUNI32, 5, 1, base.data(), 2);
@@ -70,7 +74,7 @@ int main()
                 const float x = (float)e.w / 256.0f;
                 std::memcpy(&e.w, &x, 4);
             }
-        orc_graph *g = orc_create_ex(n, 1, fw);
+        orc_graph *g = orc_create_rate(n, 1, fw, mode == 2 ? 26214u : 0u);
         CHECK(orc_insert(g, base.data(), (int64_t)base.size()) == ORC_OK);
         // the library's generator drives the updates (expiry, then compaction)
         bingo_updgen *ug = nullptr;
