@@ -68,11 +68,12 @@ CONFIGS = {
                max_len=200, batch=(1_000, 1_000_000), p=0.5, q=2.0, tp=0.15, walkers=10_000_000, lens=(5, 200),
                graph=True),
     # 2^30 R-MAT arcs (one arc per generated edge), store fitted to the graph
-    # at load; sliding-window churn: 10^7 new arcs in, the 10^7 oldest out
+    # at load, sparse/dense groups at the reference's default RATE 0.4;
+    # sliding-window churn: 10^7 new arcs in, the 10^7 oldest out
     "c5": dict(workload="rmat25_a30_loguni32_deepwalk_w32e6_churn1e7", scale=25, edges=1 << 30, wkind=2, app=0,
                max_len=80, batch=10_000_000, p=0.5, q=2.0, tp=0.0125, walkers=32_000_000, lens=None, graph=False,
                directed=True, fit=True, churn="expire", cpu_sample_edges=1 << 26, grow_in_place=True,
-               compact=1 / 16),
+               compact=1 / 16, dense_rate=0.4),
 }
 
 
@@ -367,8 +368,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--app", choices=APP_NAMES, default=None, help="walk app instead of the configuration's")
     ap.add_argument("--float-weights", action="store_true", help="float32 weights w / 256 (wtype::Float)")
-    ap.add_argument("--dense-rate", type=float, default=0.0, metavar="R",
-                    help="sparse/dense bit groups at rate R (the reference's RATE); 0: every group listed")
+    ap.add_argument("--dense-rate", type=float, default=None, metavar="R",
+                    help="sparse/dense bit groups at rate R (the reference's RATE); 0: every group listed; "
+                         "default: the config's")
+    ap.add_argument("--symmetric", action="store_true", help="emit both arcs of every generated edge")
+    ap.add_argument("--capture-updates", type=int, choices=(0, 1), default=None,
+                    help="update batches replayed from captured HIP graphs (1) or called one by one (0); "
+                         "default: the config's (captured where its walks are)")
     ap.add_argument("--grow-in-place", type=int, choices=(0, 1), default=None,
                     help="pools grow by mapping more HBM (1) or by copying (0); default: the config's choice")
     ap.add_argument("--compact", type=float, default=None, metavar="FRAC",
@@ -404,9 +410,13 @@ def main():
     if args.float_weights:
         cfg["float"] = True
         cfg["workload"] += "_float32w"
-    if args.dense_rate:
+    if args.dense_rate is not None:
         cfg["dense_rate"] = args.dense_rate
-        cfg["workload"] += f"_rate{args.dense_rate:g}"
+    if cfg.get("dense_rate"):
+        cfg["workload"] += f"_rate{cfg['dense_rate']:g}"
+    if args.symmetric and cfg.get("directed"):
+        cfg["directed"] = False
+        cfg["workload"] = cfg["workload"].replace("_a30_", "_e30_sym_")
     scale, m = cfg["scale"], cfg["edges"]
     n = 1 << scale
     K, W = args.steps, args.warmup
@@ -458,14 +468,28 @@ def main():
         tuned = g.tune(app, wp, sample=nw)  # at the workload's own walker count
     log(f"walk schedule: {tuned}")
     wgraph = g.walk_graph(app, wp, nw, d_paths, d_steps, d_lens=d_lens) if cfg["graph"] else None
+    cap_upd = bool(cfg["graph"]) if args.capture_updates is None else bool(args.capture_updates)
+    ugraph = None
+    if cap_upd:  # one captured insert + delete per step, inputs copied into its buffers
+        kmax = max(sizes)
+        u_ins, u_dels = B.DeviceArray(g, nbytes=kmax * 12), B.DeviceArray(g, nbytes=kmax * 8)
+        ugraph = g.update_graph(u_ins, kmax, u_dels, kmax)
 
     def step(i):
         di, dd, kb = batches[i]
-        g.event(0)
-        g.insert_device(di, kb)
-        g.event(1)
-        g.delete_device(dd, kb)
-        g.event(2)
+        if ugraph:
+            g.event(0)
+            B._check(B.lib().bingo_memcpy_d2d(g._h, u_ins.ptr, di.ptr, kb * 12))
+            B._check(B.lib().bingo_memcpy_d2d(g._h, u_dels.ptr, dd.ptr, kb * 8))
+            ugraph.launch(kb, kb)
+            g.event(2)
+            g.event(1)
+        else:
+            g.event(0)
+            g.insert_device(di, kb)
+            g.event(1)
+            g.delete_device(dd, kb)
+            g.event(2)
         if cfg.get("compact"):
             g.compact(cfg["compact"], counts=False)
         g.event(6)
@@ -476,6 +500,8 @@ def main():
         g.event(3)
         st = g.stats()
         st["compact_ms"] = g.elapsed_ms(2, 6)
+        if ugraph:  # the replay's own insert and delete phases
+            return st["insert_ms"], st["delete_ms"], g.elapsed_ms(6, 3), st
         return g.elapsed_ms(0, 1), g.elapsed_ms(1, 2), g.elapsed_ms(6, 3), st
 
     for i in range(W):
@@ -557,7 +583,7 @@ def main():
         "config": {"workload": cfg["workload"], "nodes": n, "arcs": int(narcs), "walkers": nw,
                    "walk_len": cfg["lens"] or cfg["max_len"], "app": APP_NAMES[app],
                    "update_batch": cfg["batch"], "p": cfg["p"], "q": cfg["q"], "tp": cfg["tp"],
-                   "hip_graph": cfg["graph"], "directed": bool(cfg.get("directed")),
+                   "hip_graph": cfg["graph"], "captured_updates": cap_upd, "directed": bool(cfg.get("directed")),
                    "deletes": "oldest arcs (sliding window)" if cfg.get("churn") == "expire" else "uniform live arcs",
                    "store": "fitted at load" if cfg.get("fit") else "power-of-two regions",
                    "pools": "grown in place" if gip else "grown by copy",
@@ -590,6 +616,10 @@ def main():
     }
     if wgraph:
         wgraph.close()
+    if ugraph:
+        ugraph.close()
+        u_ins.free()
+        u_dels.free()
     for di, dd, _ in batches:
         di.free()
         dd.free()

@@ -29,6 +29,7 @@ EDGE_DT = np.dtype([("u", np.int32), ("v", np.int32), ("w", np.uint32)])
 DEL_DT = np.dtype([("u", np.int32), ("slot", np.int32)])
 
 ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "ESTATE", -5: "EUNSUPPORTED"}
+EINVAL, ENOMEM, EDEVICE, ESTATE, EUNSUPPORTED = -1, -2, -3, -4, -5
 
 
 class BingoError(RuntimeError):
@@ -180,6 +181,9 @@ SIGNATURES = {
     "bingo_walk_graph_create": (_int, [_vp, _int, C.POINTER(WalkParams), _vp, _vp, _i64, _vp, _vp, C.POINTER(_vp)]),
     "bingo_walk_graph_launch": (_int, [_vp]),
     "bingo_walk_graph_destroy": (None, [_vp]),
+    "bingo_update_graph_create": (_int, [_vp, _vp, _i64, _vp, _i64, _u64, C.POINTER(_vp)]),
+    "bingo_update_graph_launch": (_int, [_vp, _i64, _i64]),
+    "bingo_update_graph_destroy": (None, [_vp]),
     "bingo_num_nodes": (_i32, [_vp]),
     "bingo_stats_get": (_int, [_vp, C.POINTER(Stats)]),
     "bingo_sync": (_int, [_vp]),
@@ -380,6 +384,14 @@ class DeviceArray:
         if host is not None and self.nbytes:
             _check(lib().bingo_memcpy_h2d(graph._h, self.ptr, _ptr(np.ascontiguousarray(host)), self.nbytes))
 
+    def write(self, host: np.ndarray):
+        """Copy host data to the start of the buffer (ordered on the graph's stream)."""
+        h = np.ascontiguousarray(host)
+        if h.nbytes > self.nbytes:
+            raise BingoError(-1, f"write of {h.nbytes} bytes into a device buffer of {self.nbytes}")
+        if h.nbytes:
+            _check(lib().bingo_memcpy_h2d(self.g._h, self.ptr, _ptr(h), h.nbytes))
+
     def to_host(self, dtype, shape):
         out = np.empty(shape, dtype)
         assert out.nbytes <= self.nbytes
@@ -409,6 +421,33 @@ class WalkGraph:
     def close(self):
         if getattr(self, "_h", None):
             lib().bingo_walk_graph_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+
+class UpdateGraph:
+    """Captured update batches (insert then delete) replayed with hipGraphLaunch:
+    launch(nins, ndels) applies d_ins[:nins] and d_dels[:ndels] as they are
+    when the launch runs, without waiting for the host."""
+
+    def __init__(self, graph, d_ins, max_ins, d_dels, max_dels, room_bytes=0):
+        if max_ins:
+            _need(d_ins, max_ins * EDGE_DT.itemsize, "update graph inserts")
+        if max_dels:
+            _need(d_dels, max_dels * DEL_DT.itemsize, "update graph deletes")
+        h = C.c_void_p()
+        _check(lib().bingo_update_graph_create(graph._h, d_ins.ptr if d_ins else None, max_ins,
+                                               d_dels.ptr if d_dels else None, max_dels, room_bytes, C.byref(h)))
+        self._h = h
+        self.g = graph
+
+    def launch(self, nins, ndels):
+        _check(lib().bingo_update_graph_launch(self._h, nins, ndels))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().bingo_update_graph_destroy(self._h)
             self._h = None
 
     __del__ = close
@@ -602,6 +641,11 @@ class Graph:
         """Capture one walk launch as a HIP graph; call .launch() to replay it."""
         _walk_buffers(params, nwalkers, d_paths, d_steps, d_starts, d_lens)
         return WalkGraph(self, app, params, nwalkers, d_paths, d_steps, d_starts, d_lens)
+
+    def update_graph(self, d_ins: DeviceArray | None, max_ins, d_dels: DeviceArray | None, max_dels, room_bytes=0):
+        """Capture insert + delete batches of up to max_ins / max_dels entries as
+        HIP graphs; .launch(nins, ndels) replays them."""
+        return UpdateGraph(self, d_ins, max_ins, d_dels, max_dels, room_bytes)
 
     def tune(self, app, params: WalkParams, sample):
         """Autotune the walk schedule for `app`; returns the chosen knobs."""

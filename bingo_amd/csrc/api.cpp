@@ -109,6 +109,21 @@ static int read_ctl(bingo_graph *g)
     return BINGO_OK;
 }
 
+// A captured update batch that applied nothing (read by read_ctl) is reported
+// once: the status is cleared and its error returned.
+static int take_ustatus(bingo_graph *g)
+{
+    const unsigned u = g->ctl_host->ustatus;
+    if (!u) return BINGO_OK;
+    launch_clear_ustatus(g->s.ctl, g->st);
+    g->ctl_host->ustatus = 0;
+    HIPCHK(hipStreamSynchronize(g->st));
+    const std::string rest = "; it and the captured batches after it were not applied (an insert or delete that "
+                             "reports this is not applied either)";
+    if (u & UST_INVAL) return fail(BINGO_EINVAL, "a captured update batch was invalid" + rest);
+    return fail(BINGO_ENOMEM, "a captured update batch did not fit the pools' committed room" + rest);
+}
+
 // grow_in_place: pool p is a reserved address range (the HBM budget, at
 // most 2^36 elements) mapped from its start in 1 GiB chunks; growing maps
 // more chunks of physical HBM behind the mapped part.  Returns BINGO_OK, a negative error,
@@ -384,17 +399,23 @@ static void set_lists(bingo_graph *g)
     g->s.fl2 = g->flbuf[1];
 }
 
+// arcs or deletes per batch (a batch indexes its entries in 31 bits); arcs
+// per batch of a load
+constexpr int64_t kMaxBatch = (1ll << 31) - 1;
+constexpr int64_t kLoadChunk = 1ll << 30;
+
 static int do_insert(bingo_graph *g, const DevEdge *d_e, int64_t k)
 {
     int rc;
-    if (k < 0 || k >= (1ll << 31)) return fail(BINGO_EINVAL, "batch size out of range");
+    if (k < 0 || k > kMaxBatch) return fail(BINGO_EINVAL, "batch size out of range");
     if (k == 0) return BINGO_OK;
     if ((rc = ensure_batch_scratch(g, k, false))) return rc;
     if ((rc = ensure_scratch(g, SC_WCLS, insert_wcls_bytes(k)))) return rc;
     g->s.wcls = scp<uint32_t>(g, SC_WCLS);
     launch_alloc_reset(g->s.fm, g->st);
     launch_reset_ctl(g->s.ctl, g->st);
-    launch_insert_keys(d_e, k, g->s.n, g->s.fw, scp<uint32_t>(g, SC_KEYS), scp<uint32_t>(g, SC_VALS), g->s.ctl, g->st);
+    launch_insert_keys(d_e, k, nullptr, g->s.n, g->s.fw, scp<uint32_t>(g, SC_KEYS), scp<uint32_t>(g, SC_VALS), g->s.ctl,
+                       g->st);
     const int bits = bits_for((uint64_t)g->s.n);
     uint32_t *k0 = scp<uint32_t>(g, SC_KEYS), *k1 = scp<uint32_t>(g, SC_KEYS2);
     uint32_t *v0 = scp<uint32_t>(g, SC_VALS), *v1 = scp<uint32_t>(g, SC_VALS2);
@@ -403,13 +424,13 @@ static int do_insert(bingo_graph *g, const DevEdge *d_e, int64_t k)
     bool second = false;
     sort_pairs_u32(g->sc[SC_TMP], g->sc_bytes[SC_TMP], k0, k1, v0, v1, k, bits, g->st, &second);
     uint32_t *frac = g->s.fw ? scp<uint32_t>(g, SC_FRAC) : nullptr;
-    launch_gather_edges(d_e, second ? v1 : v0, k, scp<DevEdge>(g, SC_EDGES), frac, g->st);
+    launch_gather_edges(d_e, second ? v1 : v0, k, nullptr, scp<DevEdge>(g, SC_EDGES), frac, g->st);
     Runs r;
     if ((rc = encode_runs(g, second ? k1 : k0, k, &r))) return rc;
     launch_insert_plan(g->s, scp<DevEdge>(g, SC_EDGES), r, k, g->st);
     launch_alloc_resolve(g->s, g->s.fw, g->st);
     HIPCHK(hipGetLastError());
-    if ((rc = read_ctl(g))) return rc;
+    if ((rc = read_ctl(g)) || (rc = take_ustatus(g))) return rc;
     if (g->ctl_host->error & ERR_INVAL)
         return fail(BINGO_EINVAL, g->s.fw ? "insert batch has an arc with u or v outside 1..n or a float weight that is "
                                             "not finite, not in (0, 2^32) or below 2^-32"
@@ -474,15 +495,15 @@ static int do_delete(bingo_graph *g, const int32_t *d_pairs, int64_t k, const De
 {
     int rc;
     launch_reset_ctl(g->s.ctl, g->st);
-    launch_delete_keys(d_pairs, k, w.ka, g->st);
+    launch_delete_keys(d_pairs, k, nullptr, g->s.n, g->s.ctl, w.ka, g->st);
     const int bits = 32 + bits_for((uint64_t)g->s.n);
     sort_keys_u64(w.tmp, w.tmp_bytes, w.ka, w.kb, k, bits, g->st);
-    launch_delete_check(g->s, w.kb, k, w.us, w.slots, g->st);
+    launch_delete_check(g->s, w.kb, k, 0, w.us, w.slots, g->st);
     Runs r;
     if ((rc = encode_runs_in(g, w.us, k, w.uniq, w.counts, w.starts, w.nruns, w.tmp, w.tmp_bytes, &r))) return rc;
     // long runs (more than kBigDelete deletes)
     launch_delete_big_list(r, k, w.big, g->s.ctl, g->st);
-    if ((rc = read_ctl(g))) return rc;
+    if ((rc = read_ctl(g)) || (rc = take_ustatus(g))) return rc;
     if (g->ctl_host->error & ERR_INVAL)
         return fail(BINGO_EINVAL, "delete batch names a node outside 1..n, a slot that is not live, or a slot twice");
     if (g->ctl_host->error & ERR_OVERRUN)
@@ -499,7 +520,7 @@ static int do_delete(bingo_graph *g, const int32_t *d_pairs, int64_t k, const De
 
 static int do_delete(bingo_graph *g, const int32_t *d_pairs, int64_t k)
 {
-    if (k < 0 || k >= (1ll << 31)) return fail(BINGO_EINVAL, "batch size out of range");
+    if (k < 0 || k > kMaxBatch) return fail(BINGO_EINVAL, "batch size out of range");
     if (k == 0) return BINGO_OK;
     DelWork w;
     int rc = del_work_scratch(g, k, &w);
@@ -738,6 +759,12 @@ int bingo_create(const bingo_config *cfg, bingo_graph **out)
     // test hook: treat the address reservation of the pools in this bit mask
     // as refused (exercises the copy fallback next to mapped pools)
     if (const char *e = std::getenv("BINGO_REFUSE_VMM")) g->refuse_vmm = (unsigned)std::strtoul(e, nullptr, 0);
+    // test hook: a smaller load batch (exercises the batched load on small graphs)
+    g->load_chunk = kLoadChunk;
+    if (const char *e = std::getenv("BINGO_LOAD_CHUNK")) {
+        const long long c = std::strtoll(e, nullptr, 0);
+        if (c >= 1 && c <= kLoadChunk) g->load_chunk = c;
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
         delete g;
@@ -837,7 +864,12 @@ int bingo_load_device(bingo_graph *g, int32_t n, const bingo_edge *d_edges, int6
     HIPCHK(hipGetLastError());
     g->loaded = true;
     if (g->cfg.fit_load) g->s.fit = 1u + (uint32_t)std::llround(g->cfg.fit_headroom * 65536.0);
-    int rc = do_insert(g, reinterpret_cast<const DevEdge *>(d_edges), m);
+    // larger loads go in consecutive batches of kLoadChunk arcs (same slots:
+    // arcs keep array order), which bounds the load's sort scratch (28 bytes
+    // per arc of a batch)
+    const DevEdge *de = reinterpret_cast<const DevEdge *>(d_edges);
+    int rc = m ? BINGO_OK : do_insert(g, de, 0);
+    for (int64_t at = 0; !rc && at < m; at += g->load_chunk) rc = do_insert(g, de + at, std::min(g->load_chunk, m - at));
     phase_end(g, PH_LOAD);
     if (rc) {
         g->loaded = false;  // leave a handle that can be destroyed, not reused
@@ -1056,6 +1088,340 @@ void bingo_walk_graph_destroy(bingo_walk_graph *wg)
     delete wg;
 }
 
+}  // extern "C"
+
+// ------------------------------------------------------------------------
+// Captured update batches.  One HIP graph per power-of-two batch size
+// (captured on first use) holds a whole insert batch -- keys, stable merge
+// sort, gather, runs, plan, resolve, apply, region moves, neighbour tables,
+// free-list merge -- and one per delete size the whole delete batch.  Batch
+// sizes are set on the device before the replays (entries past them are
+// padding, ignored), so one graph serves every size up to its bucket.
+// Nothing waits for the host: pools grow on the device inside the room
+// committed when the update graph was created, and a batch that does not
+// fit that room, or is invalid, applies nothing and raises Ctl::ustatus,
+// which makes every later captured batch apply nothing either until a call
+// that waits for the device reports it.  No captured node is a memset
+// (checked after every capture).
+// ------------------------------------------------------------------------
+struct bingo_update_graph {
+    bingo_graph *g = nullptr;
+    const DevEdge *ins = nullptr;
+    const int32_t *dels = nullptr;
+    int64_t max_ins = 0, max_dels = 0, max_k = 0;
+    void *buf = nullptr;  // the captured batches' own scratch, carved below
+    uint32_t *keys = nullptr, *keys2 = nullptr, *vals = nullptr, *vals2 = nullptr, *frac = nullptr;
+    uint32_t *uniq = nullptr, *counts = nullptr, *starts = nullptr, *nruns = nullptr, *us = nullptr, *slots = nullptr;
+    uint32_t *big = nullptr, *wcls = nullptr;
+    unsigned long long *ka = nullptr, *kb = nullptr;
+    DevEdge *edges = nullptr;
+    uint64_t *pf = nullptr, *fl2 = nullptr;
+    uint64_t pfcap = 0, flcap = 0;
+    void *tmp = nullptr;
+    size_t tmp_bytes = 0;
+    struct Bucket {
+        int64_t size = 0;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<Bucket> ib, db;  // insert and delete buckets
+    DevStore key{};               // the store as the captured graphs saw it
+};
+
+// a free-list buffer of the handle's first list slot holding at least
+// `entries`, its contents kept (the live lists)
+static int ensure_fl0(bingo_graph *g, uint64_t entries)
+{
+    if (entries <= g->flcap[0] && g->flbuf[0]) return BINGO_OK;
+    uint64_t *fresh = nullptr;
+    if (hipMalloc((void **)&fresh, entries * sizeof(uint64_t)) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(BINGO_ENOMEM, "free-list allocation failed");
+    }
+    HIPCHK(hipStreamSynchronize(g->st));
+    if (g->flbuf[0]) {
+        if (g->flcap[0]) HIPCHK(hipMemcpy(fresh, g->flbuf[0], g->flcap[0] * sizeof(uint64_t), hipMemcpyDeviceToDevice));
+        HIPCHK(hipFree(g->flbuf[0]));
+    }
+    g->flbuf[0] = fresh;
+    g->flcap[0] = entries;
+    set_lists(g);
+    return BINGO_OK;
+}
+
+static int64_t ug_bucket(const bingo_update_graph *ug, int64_t k, int64_t max)
+{
+    int64_t b = 1024;
+    while (b < k) b <<= 1;
+    return std::min(b, max);
+}
+
+// the store view of the captured batches: the handle's store with the update
+// graph's own scratch and second free list
+static DevStore ug_store(const bingo_update_graph *ug)
+{
+    DevStore cs = ug->g->s;
+    cs.wcls = ug->wcls;
+    cs.pf = ug->pf;
+    cs.fl = ug->g->flbuf[0];
+    cs.fl2 = ug->fl2;
+    return cs;
+}
+
+static void ug_release(std::vector<bingo_update_graph::Bucket> &v)
+{
+    for (auto &b : v) {
+        if (b.exec) (void)hipGraphExecDestroy(b.exec);
+        if (b.graph) (void)hipGraphDestroy(b.graph);
+    }
+    v.clear();
+}
+
+// no memset node may be replayed (such nodes have given wrong results here
+// on replays after the first)
+static int check_no_memset(hipGraph_t graph)
+{
+    size_t n = 0;
+    if (hipGraphGetNodes(graph, nullptr, &n) != hipSuccess) return fail(BINGO_EDEVICE, "graph node query failed");
+    std::vector<hipGraphNode_t> nodes(n);
+    if (n && hipGraphGetNodes(graph, nodes.data(), &n) != hipSuccess)
+        return fail(BINGO_EDEVICE, "graph node query failed");
+    for (auto nd : nodes) {
+        hipGraphNodeType t;
+        if (hipGraphNodeGetType(nd, &t) != hipSuccess) return fail(BINGO_EDEVICE, "graph node query failed");
+        if (t == hipGraphNodeTypeMemset)
+            return fail(BINGO_EUNSUPPORTED, "a captured update batch recorded a memset node");
+    }
+    return BINGO_OK;
+}
+
+static int ug_capture(bingo_update_graph *ug, bool del, int64_t B, bingo_update_graph::Bucket *out)
+{
+    bingo_graph *g = ug->g;
+    const DevStore cs = ug_store(ug);
+    Ctl *ctl = g->s.ctl;
+    const int64_t kr = std::min<int64_t>(B, (int64_t)cs.n + 1);
+    Runs r{ug->uniq, ug->starts, ug->counts, ug->nruns};
+    HIPCHK(hipStreamBeginCapture(g->st, hipStreamCaptureModeThreadLocal));
+    if (!del) {
+        launch_alloc_reset(cs.fm, g->st);
+        launch_reset_ctl(ctl, g->st);
+        launch_insert_keys(ug->ins, B, &ctl->bk[0], cs.n, cs.fw, ug->keys, ug->vals, ctl, g->st);
+        merge_sort_pairs_u32(ug->tmp, ug->tmp_bytes, ug->keys, ug->keys2, ug->vals, ug->vals2, B, g->st);
+        launch_gather_edges(ug->ins, ug->vals2, B, &ctl->bk[0], ug->edges, cs.fw ? ug->frac : nullptr, g->st);
+        run_length_encode_u32(ug->tmp, ug->tmp_bytes, ug->keys2, B, ug->uniq, ug->counts, ug->nruns, g->st);
+        exclusive_scan_u32(ug->tmp, ug->tmp_bytes, ug->counts, ug->starts, kr, g->st);
+        launch_insert_plan(cs, ug->edges, r, B, g->st);
+        launch_alloc_resolve(cs, cs.fw, g->st, 1, ug->flcap, ug->pfcap);
+        launch_insert_apply(cs, ug->edges, cs.fw ? ug->frac : nullptr, r, B, g->st);
+        launch_copy_tasks(cs, g->st);
+        if (cs.nbr) launch_hash_insert(cs, ug->edges, B, g->st);
+        launch_freelist_merge(cs, g->st);
+        launch_freelist_copyback(cs, g->st);
+    } else {
+        launch_reset_ctl(ctl, g->st);
+        launch_delete_keys(ug->dels, B, &ctl->bk[1], cs.n, ctl, ug->ka, g->st);
+        merge_sort_keys_u64(ug->tmp, ug->tmp_bytes, ug->ka, ug->kb, B, g->st);
+        launch_delete_check(cs, ug->kb, B, 1, ug->us, ug->slots, g->st);
+        run_length_encode_u32(ug->tmp, ug->tmp_bytes, ug->us, B, ug->uniq, ug->counts, ug->nruns, g->st);
+        exclusive_scan_u32(ug->tmp, ug->tmp_bytes, ug->counts, ug->starts, kr, g->st);
+        launch_delete_big_list(r, B, ug->big, ctl, g->st);
+        launch_delete_guard(ctl, g->st);
+        if (cs.nbr) launch_hash_delete(cs, ug->us, ug->slots, B, g->st);
+        launch_delete_apply(cs, ug->slots, r, ug->big, B, g->st);
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(g->st, &graph);
+    if (e != hipSuccess || !graph) {
+        (void)hipGetLastError();
+        if (graph) (void)hipGraphDestroy(graph);
+        return fail(BINGO_EDEVICE, std::string("update batch capture: ") + hipGetErrorString(e));
+    }
+    int rc = check_no_memset(graph);
+    hipGraphExec_t exec = nullptr;
+    if (!rc && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        rc = fail(BINGO_EDEVICE, "update graph instantiate failed");
+    }
+    if (rc) {
+        (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    *out = bingo_update_graph::Bucket{B, graph, exec};
+    return BINGO_OK;
+}
+
+// the bucket's replayable graph (captured on first use, or again after the
+// store moved, grew or swapped its free lists)
+static int ug_exec(bingo_update_graph *ug, bool del, int64_t k, hipGraphExec_t *out)
+{
+    DevStore cs = ug_store(ug);
+    if (std::memcmp(&cs, &ug->key, sizeof(DevStore)) != 0) {
+        // an uncaptured insert may have swapped in a smaller list buffer:
+        // the captured merge copies its lists back into it
+        int rc = ensure_fl0(ug->g, ug->flcap);
+        if (rc) return rc;
+        ug_release(ug->ib);
+        ug_release(ug->db);
+        cs = ug_store(ug);
+        ug->key = cs;
+    }
+    auto &v = del ? ug->db : ug->ib;
+    const int64_t B = ug_bucket(ug, k, del ? ug->max_dels : ug->max_ins);
+    for (auto &b : v)
+        if (b.size == B) {
+            *out = b.exec;
+            return BINGO_OK;
+        }
+    bingo_update_graph::Bucket nb;
+    int rc = ug_capture(ug, del, B, &nb);
+    if (rc) return rc;
+    v.push_back(nb);
+    *out = nb.exec;
+    return BINGO_OK;
+}
+
+static void ug_free(bingo_update_graph *ug)
+{
+    ug_release(ug->ib);
+    ug_release(ug->db);
+    if (ug->buf) (void)hipFree(ug->buf);
+    ug->buf = nullptr;
+}
+
+extern "C" {
+
+int bingo_update_graph_create(bingo_graph *g, const bingo_edge *d_ins, int64_t max_ins, const bingo_del *d_dels,
+                              int64_t max_dels, uint64_t room_bytes, bingo_update_graph **out)
+{
+    if (!out) return fail(BINGO_EINVAL, "null output");
+    *out = nullptr;
+    if (!g || !g->loaded) return fail(BINGO_ESTATE, "graph not loaded");
+    if (max_ins < 0 || max_dels < 0 || max_ins > kMaxBatch || max_dels > kMaxBatch || (max_ins && !d_ins) ||
+        (max_dels && !d_dels) || (!max_ins && !max_dels))
+        return fail(BINGO_EINVAL, "bad update graph buffers or sizes");
+    int rc;
+    if ((rc = read_ctl(g)) || (rc = take_ustatus(g))) return rc;
+    // the committed room: `room_bytes` (0: the store's pool bytes, at least
+    // 1 GiB, at most half the budget left) spread over the pools by size
+    uint64_t pool_bytes = 0, w[NPOOL];
+    for (int p = 0; p < NPOOL; ++p) {
+        w[p] = std::max<uint64_t>(g->ctl_host->top[p] * kElem[p], 1ull << 20);
+        pool_bytes += g->ctl_host->top[p] * kElem[p];
+    }
+    const uint64_t used = store_bytes(g);
+    uint64_t room = room_bytes;
+    if (!room) {
+        const uint64_t left = g->budget > used ? g->budget - used : 0;
+        room = std::min<uint64_t>(std::max<uint64_t>(pool_bytes, 1ull << 30), left / 2);
+    }
+    uint64_t wsum = 0;
+    for (int p = 0; p < NPOOL; ++p) wsum += w[p];
+    for (int p = 0; p < NPOOL; ++p) {
+        const uint64_t share = (uint64_t)((double)room * (double)w[p] / (double)wsum);
+        const uint64_t free_elems = g->s.cap[p] > g->ctl_host->top[p] ? g->s.cap[p] - g->ctl_host->top[p] : 0;
+        const uint64_t want = share / kElem[p];
+        g->ctl_host->need[p] = want > free_elems ? want : 0;
+    }
+    if ((rc = ensure_pools(g))) return rc;
+    bingo_update_graph *ug = new bingo_update_graph();
+    ug->g = g;
+    ug->ins = reinterpret_cast<const DevEdge *>(d_ins);
+    ug->dels = reinterpret_cast<const int32_t *>(d_dels);
+    ug->max_ins = max_ins;
+    ug->max_dels = max_dels;
+    const int64_t K = std::max(max_ins, max_dels);
+    ug->max_k = K;
+    const int64_t kr = std::min<int64_t>(K, (int64_t)g->s.n + 1);
+    // regions a batch frees: at most the slot, back and table regions and 32
+    // member lists of each run (node); free lists: room for the current ones
+    // and a good many batches' frees
+    ug->pfcap = std::max<uint64_t>(35ull * (uint64_t)std::min<int64_t>(max_ins, (int64_t)g->s.n + 1), 1024);
+    ug->flcap = std::max<uint64_t>({4ull * g->ctl_host->fl_total, 1ull << 22, 2 * ug->pfcap});
+    if ((rc = ensure_fl0(g, ug->flcap))) {
+        delete ug;
+        return rc;
+    }
+    const size_t tb = std::max({merge_sort_pairs_u32(nullptr, 0, nullptr, nullptr, nullptr, nullptr, K, g->st),
+                                merge_sort_keys_u64(nullptr, 0, nullptr, nullptr, K, g->st),
+                                run_length_encode_u32(nullptr, 0, nullptr, K, nullptr, nullptr, nullptr, g->st),
+                                exclusive_scan_u32(nullptr, 0, nullptr, nullptr, kr, g->st)});
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t k4 = al((size_t)K * 4), k8 = al((size_t)K * 8), r4 = al((size_t)kr * 4);
+    const size_t sizes[] = {k4,  k4, k4, k4, g->s.fw ? k4 : 0, r4, r4, r4, 256, k4, k4, al(((size_t)K / kBigDelete + 1) * 4),
+                            al(insert_wcls_bytes(K)), k8, k8, al((size_t)K * sizeof(DevEdge)),
+                            al(ug->pfcap * 8), al(ug->flcap * 8), al(tb)};
+    size_t total = 0;
+    for (size_t b : sizes) total += b;
+    if (hipMalloc(&ug->buf, total) != hipSuccess) {
+        (void)hipGetLastError();
+        delete ug;
+        return fail(BINGO_ENOMEM, "update graph scratch allocation failed");
+    }
+    unsigned char *p = static_cast<unsigned char *>(ug->buf);
+    void *parts[19];
+    for (int i = 0; i < 19; ++i) {
+        parts[i] = sizes[i] ? p : nullptr;
+        p += sizes[i];
+    }
+    ug->keys = (uint32_t *)parts[0];
+    ug->keys2 = (uint32_t *)parts[1];
+    ug->vals = (uint32_t *)parts[2];
+    ug->vals2 = (uint32_t *)parts[3];
+    ug->frac = (uint32_t *)parts[4];
+    ug->uniq = (uint32_t *)parts[5];
+    ug->counts = (uint32_t *)parts[6];
+    ug->starts = (uint32_t *)parts[7];
+    ug->nruns = (uint32_t *)parts[8];
+    ug->us = (uint32_t *)parts[9];
+    ug->slots = (uint32_t *)parts[10];
+    ug->big = (uint32_t *)parts[11];
+    ug->wcls = (uint32_t *)parts[12];
+    ug->ka = (unsigned long long *)parts[13];
+    ug->kb = (unsigned long long *)parts[14];
+    ug->edges = (DevEdge *)parts[15];
+    ug->pf = (uint64_t *)parts[16];
+    ug->fl2 = (uint64_t *)parts[17];
+    ug->tmp = parts[18];
+    ug->tmp_bytes = sizes[18];
+    ug->key = ug_store(ug);
+    *out = ug;
+    return BINGO_OK;
+}
+
+int bingo_update_graph_launch(bingo_update_graph *ug, int64_t nins, int64_t ndels)
+{
+    if (!ug || !ug->g || !ug->g->loaded) return fail(BINGO_ESTATE, "no update graph");
+    if (nins < 0 || nins > ug->max_ins || ndels < 0 || ndels > ug->max_dels)
+        return fail(BINGO_EINVAL, "batch sizes outside the update graph's maxima");
+    bingo_graph *g = ug->g;
+    hipGraphExec_t ei = nullptr, ed = nullptr;
+    int rc;
+    if ((nins && (rc = ug_exec(ug, false, nins, &ei))) || (ndels && (rc = ug_exec(ug, true, ndels, &ed)))) return rc;
+    launch_set_batch(g->s.ctl, nins, ndels, g->st);
+    if (ei) {
+        phase_begin(g, PH_INSERT);
+        HIPCHK(hipGraphLaunch(ei, g->st));
+        phase_end(g, PH_INSERT);
+    }
+    if (ed) {
+        phase_begin(g, PH_DELETE);
+        HIPCHK(hipGraphLaunch(ed, g->st));
+        phase_end(g, PH_DELETE);
+    }
+    g->arcs += nins - ndels;
+    return BINGO_OK;
+}
+
+void bingo_update_graph_destroy(bingo_update_graph *ug)
+{
+    if (!ug) return;
+    if (ug->g && ug->g->st) (void)hipStreamSynchronize(ug->g->st);
+    ug_free(ug);
+    delete ug;
+}
+
+
 int bingo_walk_tune(bingo_graph *g, int app, const bingo_walk_params *params, int64_t sample,
                     bingo_walk_params *chosen)
 {
@@ -1229,6 +1595,7 @@ int bingo_sync(bingo_graph *g)
         int rc = read_ctl(g);
         if (rc) return rc;
         if (g->ctl_host->error & ERR_OVERRUN) return fail(BINGO_EDEVICE, "pool overrun detected");
+        if ((rc = take_ustatus(g))) return rc;
         // device-side walk inputs are checked by the walk kernels: a walker
         // with a start outside 1..n or a length outside 1..max_len gets a row
         // of -1 and is reported here (until the next walk starts)

@@ -92,6 +92,7 @@ struct bingo_graph {
     size_t flcap[2] = {0, 0};                  // their capacities (entries)
     bool vmm_refused = false;  // the device refused a reservation: pools are copied instead
     unsigned refuse_vmm = 0;   // test hook: pools (bit p) whose reservation is treated as refused
+    int64_t load_chunk = 0;    // arcs per batch of a load (kLoadChunk; smaller as a test hook)
     struct Tuned {
         bool valid = false;
         int32_t max_len = 0;

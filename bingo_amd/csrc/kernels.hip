@@ -532,11 +532,16 @@ __global__ void k_init_nodes(DevStore s)
     }
 }
 
-__global__ void k_insert_keys(const DevEdge *__restrict__ e, int64_t k, int32_t n, int fw, uint32_t *__restrict__ keys,
-                              uint32_t *__restrict__ vals, Ctl *ctl)
+__global__ void k_insert_keys(const DevEdge *__restrict__ e, int64_t k, const unsigned long long *__restrict__ dk,
+                              int32_t n, int fw, uint32_t *__restrict__ keys, uint32_t *__restrict__ vals, Ctl *ctl)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= k) return;
+    if (dk && (uint64_t)i >= *dk) {  // padding of a captured batch
+        keys[i] = 0u;
+        vals[i] = (uint32_t)i;
+        return;
+    }
     const DevEdge x = e[i];
     uint32_t ip, fr;
     const bool wbad = fw ? !split_float(x.w, ip, fr) : x.w == 0u;
@@ -549,11 +554,13 @@ __global__ void k_insert_keys(const DevEdge *__restrict__ e, int64_t k, int32_t 
 // Arcs in node order.  With float weights (frac != null) the gathered arc
 // carries the integer part as its weight and frac[i] its fraction.
 __global__ void k_gather_edges(const DevEdge *__restrict__ e, const uint32_t *__restrict__ order, int64_t k,
-                               DevEdge *__restrict__ out, uint32_t *__restrict__ frac)
+                               const unsigned long long *__restrict__ dk, DevEdge *__restrict__ out,
+                               uint32_t *__restrict__ frac)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= k) return;
-    DevEdge x = e[order[i]];
+    const uint32_t o = order[i];
+    DevEdge x = (dk && o >= *dk) ? DevEdge{0, 0, 0u} : e[o];  // padding: an empty arc (node 0)
     if (frac) {
         uint32_t ip, fr;
         split_float(x.w, ip, fr);
@@ -809,7 +816,7 @@ __global__ __launch_bounds__(256) void k_insert_plan(DevStore s, const DevEdge *
 // pool's top), the pending segments of the batch's frees, and the lists'
 // layout after the batch.  One workgroup of kPC threads; pool p is waves
 // 4p .. 4p + 3.
-__global__ __launch_bounds__(kPC) void k_alloc_resolve(DevStore s, int fw)
+__global__ __launch_bounds__(kPC) void k_alloc_resolve(DevStore s, int fw, int captured, uint64_t flcap, uint64_t pfcap)
 {
     constexpr int kW = kPC / 64, kWP = kClasses / 64;  // waves, waves per pool
     __shared__ uint64_t part[3][kW];
@@ -840,12 +847,20 @@ __global__ __launch_bounds__(kPC) void k_alloc_resolve(DevStore s, int fw)
     }
     fm->bump[t] = s.ctl->top[p] + eb + einc - ex;
     if (k == 0) s.ctl->need[p] = etot;
+    // a captured batch grows nothing from the host: it applies only if the
+    // pools, lists and pending frees have room (and the batch is valid)
+    if (captured && k == 0 && s.ctl->top[p] + etot > s.cap[p]) atomicOr(&s.ctl->ustatus, (unsigned)UST_NOROOM);
     fm->pseg[t] = pb + pinc - fr;
     fm->nfbase[t] = nb + ninc - nv;
     fm->nav[t] = (uint32_t)nv;
     if (t == 0) {
         s.ctl->pend_total = ptot;
         s.ctl->fl_total = ntot;
+        if (captured) {
+            const unsigned e = s.ctl->error;
+            if (e & (ERR_INVAL | ERR_OVERRUN)) atomicOr(&s.ctl->ustatus, (unsigned)UST_INVAL);
+            if ((e & ERR_NOMEM) || ntot > flcap || ptot > pfcap) atomicOr(&s.ctl->ustatus, (unsigned)UST_NOROOM);
+        }
     }
 }
 
@@ -853,6 +868,7 @@ __global__ __launch_bounds__(kPC) void k_alloc_resolve(DevStore s, int fw)
 // becomes its unused free regions followed by the batch's frees, in s.fl2.
 __global__ __launch_bounds__(256) void k_freelist_merge(DevStore s)
 {
+    if (s.ctl->ustatus) return;  // a captured batch that applied nothing
     const int t = blockIdx.x;
     FreeMeta *fm = s.fm;
     const uint32_t av = fm->avail[t], from = min(fm->take[t], av), rem = av - from, fr = fm->freed[t];
@@ -900,6 +916,7 @@ struct ClassAlloc {
 // the batch's reservations are used: the pool tops move past them
 __global__ void k_commit_tops(Ctl *ctl)
 {
+    if (ctl->ustatus) return;
     const int t = threadIdx.x;
     if (t < NPOOL) ctl->top[t] += ctl->need[t];
 }
@@ -1133,6 +1150,7 @@ template <bool DN>
 __global__ __launch_bounds__(256) void k_insert_apply(DevStore s, const DevEdge *__restrict__ E,
                                                       const uint32_t *__restrict__ FR, Runs r)
 {
+    if (s.ctl->ustatus) return;  // a captured batch that does not apply
     __shared__ uint32_t li[2 * kPC];
     for (int i = threadIdx.x; i < 2 * kPC; i += blockDim.x) li[i] = 0u;
     __syncthreads();
@@ -1159,19 +1177,32 @@ __global__ __launch_bounds__(256) void k_insert_apply(DevStore s, const DevEdge 
 // ------------------------------------------------------------------------
 // delete: check (all-or-nothing) and apply
 // ------------------------------------------------------------------------
-__global__ void k_delete_keys(const int32_t *__restrict__ d, int64_t k, unsigned long long *__restrict__ keys)
+__global__ void k_delete_keys(const int32_t *__restrict__ d, int64_t k, const unsigned long long *__restrict__ dk,
+                              int32_t n, Ctl *ctl, unsigned long long *__restrict__ keys)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= k) return;
+    if (dk) {  // captured: key 0 marks padding, so a real node-0 delete is flagged here
+        if ((uint64_t)i >= *dk) {
+            keys[i] = 0ull;
+            return;
+        }
+        if (d[2 * i] < 1 || d[2 * i] > n) atomicOr(&ctl->error, (unsigned)ERR_INVAL);
+    }
     keys[i] = ((unsigned long long)(uint32_t)d[2 * i] << 32) | (uint32_t)d[2 * i + 1];
 }
 
-__global__ void k_delete_check(DevStore s, const unsigned long long *__restrict__ keys, int64_t k,
+__global__ void k_delete_check(DevStore s, const unsigned long long *__restrict__ keys, int64_t k, int pads,
                                uint32_t *__restrict__ us, uint32_t *__restrict__ slots)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= k) return;
     const unsigned long long key = keys[i];
+    if (pads && key == 0ull) {  // padding of a captured batch
+        us[i] = 0u;
+        slots[i] = 0u;
+        return;
+    }
     const int32_t u = (int32_t)(key >> 32);
     const uint32_t sl = (uint32_t)key;
     bool bad = u < 1 || u > s.n || (i > 0 && keys[i - 1] == key);
@@ -1545,6 +1576,7 @@ __device__ void delete_finish(const DevStore &s, const uint32_t *__restrict__ sl
 // ------------------------------------------------------------------------
 __global__ void k_hash_insert(DevStore s, const DevEdge *__restrict__ E, int64_t k)
 {
+    if (s.ctl->ustatus) return;
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= k) return;
     const DevEdge x = E[i];
@@ -1557,6 +1589,7 @@ __global__ void k_hash_insert(DevStore s, const DevEdge *__restrict__ E, int64_t
 __global__ void k_hash_delete(DevStore s, const uint32_t *__restrict__ us, const uint32_t *__restrict__ slots,
                               int64_t k)
 {
+    if (s.ctl->ustatus) return;
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= k) return;
     const int32_t u = (int32_t)us[i];
@@ -1572,6 +1605,7 @@ template <bool DN>
 __global__ __launch_bounds__(256) void k_delete_finish(DevStore s, const uint32_t *__restrict__ slots, Runs r,
                                                        const uint32_t *__restrict__ big)
 {
+    if (s.ctl->ustatus) return;
     const int ln = lane_id();
     const uint32_t nbig = s.ctl->nbig;
     const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
@@ -1596,9 +1630,10 @@ __global__ void k_delete_big_list(Runs r, uint32_t *__restrict__ big, Ctl *ctl)
 // orders every group task's writes before the finish, where a device-scope
 // fence inside the kernel would write back and invalidate an XCD's L2.
 template <bool DN>
-__global__ __launch_bounds__(256) void k_delete_apply(DevStore s, const uint32_t *__restrict__ slots, Runs r,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_delete_apply(DevStore s, const uint32_t *__restrict__ slots, Runs r,
                                                       const uint32_t *__restrict__ big)
 {
+    if (s.ctl->ustatus) return;  // a captured batch that does not apply
     const int ln = lane_id();
     const uint32_t nruns = *r.nruns;
     const uint32_t nbig = s.ctl->nbig;
@@ -3013,7 +3048,7 @@ __global__ __launch_bounds__(256) void k_copy_tasks(DevStore s)
 {
     __shared__ uint32_t cb[kMaxCopyTasks];
     const uint32_t nt = min((uint32_t)(s.ctl->taskctr >> 32), kMaxCopyTasks);
-    if (nt == 0) return;
+    if (nt == 0 || s.ctl->ustatus) return;
     for (uint32_t i = threadIdx.x; i < nt; i += blockDim.x) cb[i] = s.tasks[i].cbase;
     __syncthreads();
     const uint32_t nchunks = cb[nt - 1] + s.tasks[nt - 1].nch;
@@ -3106,16 +3141,57 @@ void launch_init_nodes(DevStore s, hipStream_t st)
     hipLaunchKernelGGL(k_init_nodes, dim3(blocks_for(total, 256)), dim3(256), 0, st, s);
 }
 
-void launch_insert_keys(const DevEdge *e, int64_t k, int32_t n, int fw, uint32_t *keys, uint32_t *vals, Ctl *ctl,
-                        hipStream_t st)
+void launch_insert_keys(const DevEdge *e, int64_t k, const unsigned long long *dk, int32_t n, int fw, uint32_t *keys,
+                        uint32_t *vals, Ctl *ctl, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_insert_keys, dim3(blocks_for(k, 256)), dim3(256), 0, st, e, k, n, fw, keys, vals, ctl);
+    hipLaunchKernelGGL(k_insert_keys, dim3(blocks_for(k, 256)), dim3(256), 0, st, e, k, dk, n, fw, keys, vals, ctl);
 }
 
-void launch_gather_edges(const DevEdge *e, const uint32_t *order, int64_t k, DevEdge *out, uint32_t *frac,
-                         hipStream_t st)
+void launch_gather_edges(const DevEdge *e, const uint32_t *order, int64_t k, const unsigned long long *dk, DevEdge *out,
+                         uint32_t *frac, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_gather_edges, dim3(blocks_for(k, 256)), dim3(256), 0, st, e, order, k, out, frac);
+    hipLaunchKernelGGL(k_gather_edges, dim3(blocks_for(k, 256)), dim3(256), 0, st, e, order, k, dk, out, frac);
+}
+
+__global__ void k_set_batch(Ctl *ctl, int64_t kins, int64_t kdel)
+{
+    if (threadIdx.x == 0) {
+        ctl->bk[0] = (unsigned long long)kins;
+        ctl->bk[1] = (unsigned long long)kdel;
+    }
+}
+
+void launch_set_batch(Ctl *ctl, int64_t kins, int64_t kdel, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_set_batch, dim3(1), dim3(64), 0, st, ctl, kins, kdel);
+}
+
+__global__ void k_delete_guard(Ctl *ctl)
+{
+    if (threadIdx.x == 0 && (ctl->error & (ERR_INVAL | ERR_OVERRUN))) atomicOr(&ctl->ustatus, (unsigned)UST_INVAL);
+}
+
+void launch_delete_guard(Ctl *ctl, hipStream_t st) { hipLaunchKernelGGL(k_delete_guard, dim3(1), dim3(64), 0, st, ctl); }
+
+__global__ void k_clear_ustatus(Ctl *ctl)
+{
+    if (threadIdx.x == 0) ctl->ustatus = 0u;
+}
+
+void launch_clear_ustatus(Ctl *ctl, hipStream_t st) { hipLaunchKernelGGL(k_clear_ustatus, dim3(1), dim3(64), 0, st, ctl); }
+
+// the merged free lists back into s.fl (captured batches keep one list buffer)
+__global__ __launch_bounds__(256) void k_freelist_copyback(DevStore s)
+{
+    if (s.ctl->ustatus) return;
+    const uint64_t n = s.ctl->fl_total;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        s.fl[i] = s.fl2[i];
+}
+
+void launch_freelist_copyback(DevStore s, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_freelist_copyback, dim3(1024), dim3(256), 0, st, s);
 }
 
 size_t insert_wcls_bytes(int64_t k) { return (size_t)run_blocks(k) * 2 * kPC * sizeof(uint32_t); }
@@ -3131,9 +3207,9 @@ __global__ void k_alloc_reset(FreeMeta *fm)
 
 void launch_alloc_reset(FreeMeta *fm, hipStream_t st) { hipLaunchKernelGGL(k_alloc_reset, dim3(1), dim3(kPC), 0, st, fm); }
 
-void launch_alloc_resolve(DevStore s, int fw, hipStream_t st)
+void launch_alloc_resolve(DevStore s, int fw, hipStream_t st, int captured, uint64_t flcap, uint64_t pfcap)
 {
-    hipLaunchKernelGGL(k_alloc_resolve, dim3(1), dim3(kPC), 0, st, s, fw);
+    hipLaunchKernelGGL(k_alloc_resolve, dim3(1), dim3(kPC), 0, st, s, fw, captured, flcap, pfcap);
 }
 
 void launch_freelist_merge(DevStore s, hipStream_t st)
@@ -3169,15 +3245,16 @@ void launch_hash_delete(DevStore s, const uint32_t *us, const uint32_t *slots, i
     hipLaunchKernelGGL(k_hash_delete, dim3(blocks_for(k, 256)), dim3(256), 0, st, s, us, slots, k);
 }
 
-void launch_delete_keys(const int32_t *dels, int64_t k, unsigned long long *keys, hipStream_t st)
+void launch_delete_keys(const int32_t *dels, int64_t k, const unsigned long long *dk, int32_t n, Ctl *ctl,
+                        unsigned long long *keys, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_delete_keys, dim3(blocks_for(k, 256)), dim3(256), 0, st, dels, k, keys);
+    hipLaunchKernelGGL(k_delete_keys, dim3(blocks_for(k, 256)), dim3(256), 0, st, dels, k, dk, n, ctl, keys);
 }
 
-void launch_delete_check(DevStore s, const unsigned long long *keys, int64_t k, uint32_t *us, uint32_t *slots,
+void launch_delete_check(DevStore s, const unsigned long long *keys, int64_t k, int pads, uint32_t *us, uint32_t *slots,
                          hipStream_t st)
 {
-    hipLaunchKernelGGL(k_delete_check, dim3(blocks_for(k, 256)), dim3(256), 0, st, s, keys, k, us, slots);
+    hipLaunchKernelGGL(k_delete_check, dim3(blocks_for(k, 256)), dim3(256), 0, st, s, keys, k, pads, us, slots);
 }
 
 void launch_delete_big_list(Runs r, int64_t k, uint32_t *big, Ctl *ctl, hipStream_t st)
@@ -3422,6 +3499,22 @@ size_t exclusive_scan_u32(void *tmp, size_t tmp_bytes, const uint32_t *in, uint3
 {
     size_t b = tmp_bytes;
     (void)rocprim::exclusive_scan(tmp, b, in, out, 0u, (size_t)k, rocprim::plus<uint32_t>(), st);
+    return b;
+}
+
+size_t merge_sort_pairs_u32(void *tmp, size_t tmp_bytes, const uint32_t *kin, uint32_t *kout, const uint32_t *vin,
+                            uint32_t *vout, int64_t k, hipStream_t st)
+{
+    size_t b = tmp_bytes;
+    (void)rocprim::merge_sort(tmp, b, kin, kout, vin, vout, (size_t)k, rocprim::less<uint32_t>(), st);
+    return b;
+}
+
+size_t merge_sort_keys_u64(void *tmp, size_t tmp_bytes, const unsigned long long *kin, unsigned long long *kout,
+                           int64_t k, hipStream_t st)
+{
+    size_t b = tmp_bytes;
+    (void)rocprim::merge_sort(tmp, b, kin, kout, (size_t)k, rocprim::less<unsigned long long>(), st);
     return b;
 }
 

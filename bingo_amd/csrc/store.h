@@ -124,7 +124,16 @@ struct Ctl {
     unsigned long long comp_nseg;      // compaction: segments of the listed big nodes
     unsigned long long fl_total;     // insert batch: free-list entries after the batch
     unsigned long long pend_total;   // insert batch: regions the batch frees
+    // Captured update batches (update graphs).  bk: the replay's insert and
+    // delete counts (set by a kernel before the graph); entries past them are
+    // padding.  ustatus (sticky until the host reads it): a captured batch
+    // could not apply (UST_*); every apply-side kernel returns at once while it
+    // is set, so later captured batches apply nothing either.
+    unsigned long long bk[2];
+    unsigned int ustatus;
+    unsigned int pad_ustatus;
 };
+enum UStatus { UST_INVAL = 1, UST_NOROOM = 2 };
 
 // Region sizes and recycling.  Every region of pool p has a capacity of a
 // size class: class k = 8 q + r (r < 8) holds S(k) = floor((8 + r) 2^q / 8)
@@ -280,10 +289,12 @@ void launch_reset_walk(Ctl *ctl, hipStream_t st);
 void launch_init_nodes(DevStore s, hipStream_t st);
 // fw: weights are float bits (validated as such; the gather then writes the
 // integer part as the arc's weight and the fraction to frac[i])
-void launch_insert_keys(const DevEdge *e, int64_t k, int32_t n, int fw, uint32_t *keys, uint32_t *vals, Ctl *ctl,
-                        hipStream_t st);
-void launch_gather_edges(const DevEdge *e, const uint32_t *order, int64_t k, DevEdge *out, uint32_t *frac,
-                         hipStream_t st);
+// dk (captured batches): the batch's true size on the device; entries at
+// or past it are padding (key 0, no error; gathered as an empty arc)
+void launch_insert_keys(const DevEdge *e, int64_t k, const unsigned long long *dk, int32_t n, int fw, uint32_t *keys,
+                        uint32_t *vals, Ctl *ctl, hipStream_t st);
+void launch_gather_edges(const DevEdge *e, const uint32_t *order, int64_t k, const unsigned long long *dk, DevEdge *out,
+                         uint32_t *frac, hipStream_t st);
 // The plan counts each workgroup's region allocations and frees per class
 // (s.wcls: insert_wcls_bytes(k)); launch_alloc_resolve then sets Ctl::need
 // (the batch's bump space per pool), fl_total and pend_total; the apply,
@@ -292,7 +303,18 @@ void launch_gather_edges(const DevEdge *e, const uint32_t *order, int64_t k, Dev
 // the free lists to s.fl2 (the caller swaps fl and fl2).
 size_t insert_wcls_bytes(int64_t k);
 void launch_alloc_reset(FreeMeta *fm, hipStream_t st);  // before a batch's plan
-void launch_alloc_resolve(DevStore s, int fw, hipStream_t st);
+// captured != 0: a batch whose growth does not fit the pools' capacity, whose
+// free lists would pass flcap entries or whose frees pfcap, or that is
+// invalid, sets Ctl::ustatus instead of asking the host
+void launch_alloc_resolve(DevStore s, int fw, hipStream_t st, int captured = 0, uint64_t flcap = 0,
+                          uint64_t pfcap = 0);
+// captured batches: the merged lists (s.fl2) copied back over s.fl
+void launch_freelist_copyback(DevStore s, hipStream_t st);
+void launch_set_batch(Ctl *ctl, int64_t kins, int64_t kdel, hipStream_t st);
+// after a captured delete batch's check: an invalid batch sets Ctl::ustatus
+void launch_delete_guard(Ctl *ctl, hipStream_t st);
+// host clears a reported status (stream-ordered)
+void launch_clear_ustatus(Ctl *ctl, hipStream_t st);
 void launch_freelist_merge(DevStore s, hipStream_t st);
 void launch_insert_plan(DevStore s, const DevEdge *sorted, Runs r, int64_t k, hipStream_t st);
 void launch_insert_apply(DevStore s, const DevEdge *sorted, const uint32_t *frac, Runs r, int64_t k, hipStream_t st);
@@ -301,8 +323,10 @@ void launch_insert_apply(DevStore s, const DevEdge *sorted, const uint32_t *frac
 // (their slot weights are read before the tombstones)
 void launch_hash_insert(DevStore s, const DevEdge *sorted, int64_t k, hipStream_t st);
 void launch_hash_delete(DevStore s, const uint32_t *us, const uint32_t *slots, int64_t k, hipStream_t st);
-void launch_delete_keys(const int32_t *dels, int64_t k, unsigned long long *keys, hipStream_t st);
-void launch_delete_check(DevStore s, const unsigned long long *keys, int64_t k, uint32_t *us, uint32_t *slots,
+void launch_delete_keys(const int32_t *dels, int64_t k, const unsigned long long *dk, int32_t n, Ctl *ctl,
+                        unsigned long long *keys, hipStream_t st);
+// pads: key 0 is padding (captured batches), not an invalid delete
+void launch_delete_check(DevStore s, const unsigned long long *keys, int64_t k, int pads, uint32_t *us, uint32_t *slots,
                          hipStream_t st);
 // Delete runs longer than kBigDelete are listed (big: run indices, room for
 // k / kBigDelete + 1 entries), handled by one wave per bit group and then
@@ -371,5 +395,11 @@ size_t run_length_encode_u32(void *tmp, size_t tmp_bytes, const uint32_t *keys, 
                              uint32_t *counts, uint32_t *nruns, hipStream_t st);
 size_t exclusive_scan_u32(void *tmp, size_t tmp_bytes, const uint32_t *in, uint32_t *out, int64_t k,
                           hipStream_t st);
+// stable merge sorts for captured batches (no memset is recorded: the radix
+// sort's hipMemsetAsync has given wrong results on graph replays)
+size_t merge_sort_pairs_u32(void *tmp, size_t tmp_bytes, const uint32_t *kin, uint32_t *kout, const uint32_t *vin,
+                            uint32_t *vout, int64_t k, hipStream_t st);
+size_t merge_sort_keys_u64(void *tmp, size_t tmp_bytes, const unsigned long long *kin, unsigned long long *kout,
+                           int64_t k, hipStream_t st);
 
 }  // namespace bingo

@@ -146,7 +146,10 @@ const char *bingo_version(void);
 /* ---- graph load: ref build() (operation.cuh:33-121) ----
  * Arcs are taken in array order; the k-th arc of a node gets slot k (as
  * graph.cuh:38-42 numbers nodeIdu).  u, v in 1..n; w >= 1 (float weights:
- * finite, 2^-32 <= w < 2^32). */
+ * finite, 2^-32 <= w < 2^32).  A load of more than 2^30 arcs is applied as
+ * consecutive batches of 2^30 arcs (with dense_rate, a node's classes come
+ * from the first of them that holds its arcs); insert and delete batches
+ * take at most 2^31 - 1 entries. */
 int bingo_load(bingo_graph *g, int32_t n, const bingo_edge *edges, int64_t m);
 int bingo_load_device(bingo_graph *g, int32_t n, const bingo_edge *d_edges, int64_t m);
 
@@ -214,6 +217,31 @@ int bingo_walk_graph_create(bingo_graph *g, int app, const bingo_walk_params *pa
                             bingo_walk_graph **out);
 int bingo_walk_graph_launch(bingo_walk_graph *wg);
 void bingo_walk_graph_destroy(bingo_walk_graph *wg);
+
+/* ---- captured update batches (batched form; no reference counterpart: the
+ * reference runs insert() and deleteE() as host functions that wait for
+ * the device mid-batch, include/operation.cuh:144-251) ----
+ * An insert batch and a delete batch captured as HIP graphs, one per
+ * power-of-two batch size (captured on first use), replayed with the sizes
+ * of each launch: d_ins holds nins <= max_ins arcs and d_dels ndels <=
+ * max_dels deletes when the launch runs (device buffers given here; a
+ * launch reads them in stream order, so writing the next batch with stream-
+ * ordered copies after the launch is safe).  A launch applies the inserts,
+ * then the deletes, with the same results as bingo_insert_device and
+ * bingo_delete_device, and returns without waiting: nothing in it waits for
+ * the host.  Pools grow on the device inside room committed at creation
+ * (room_bytes, 0 = the store's pool bytes, at least 1 GiB, at most half of
+ * the budget left, spread over the pools by size).  A batch that does not
+ * fit that room, or is invalid, applies nothing, and neither do the
+ * captured batches after it until the next call that waits for the device
+ * (bingo_sync, an insert or delete) reports it: BINGO_ENOMEM (room) or
+ * BINGO_EINVAL (invalid).  The store then holds every batch before the
+ * failed one. */
+typedef struct bingo_update_graph bingo_update_graph;
+int bingo_update_graph_create(bingo_graph *g, const bingo_edge *d_ins, int64_t max_ins, const bingo_del *d_dels,
+                              int64_t max_dels, uint64_t room_bytes, bingo_update_graph **out);
+int bingo_update_graph_launch(bingo_update_graph *ug, int64_t nins, int64_t ndels);
+void bingo_update_graph_destroy(bingo_update_graph *ug);
 
 /* Autotuner: times the persistent form over block size, walkers per lane
  * and LDS staging depth (and the plain form) on up to `sample` walkers with

@@ -232,3 +232,28 @@ def test_dense_limits():
     g.close()
     with pytest.raises(B.BingoError):
         B.Graph(dense_rate=-0.5)
+
+
+@pytest.mark.parametrize("rate", [0.0, 0.4])
+def test_load_in_batches(orc, monkeypatch, rate):
+    """A load larger than the load batch (2^30 arcs; 7000 here through the
+    test hook) is applied as consecutive batches in array order: the store
+    equals the oracle's fed the same batches, and with dense groups a node's
+    classes come from the first batch that holds its arcs."""
+    rng = random.Random(17)
+    n = 400
+    e = _rand_edges(rng, n, 30000, hub=3, wbits=rng.choice([16, 32]))
+    monkeypatch.setenv("BINGO_LOAD_CHUNK", "7000")
+    g = B.Graph(neighbor_index=True, dense_rate=rate)
+    monkeypatch.delenv("BINGO_LOAD_CHUNK")
+    g.load(n, e)
+    o = orc.OracleGraph(n, True, dense_rate=rate)
+    for at in range(0, len(e), 7000):
+        assert o.insert(e[at:at + 7000]) == 0
+    _same_state(g, o, "batched load")
+    one = orc.OracleGraph(n, True, dense_rate=rate)
+    assert one.load(e) == 0
+    if rate:  # one batch decides classes from all arcs of a node: a different store
+        assert not np.array_equal(one.export()["dmask"], o.export()["dmask"])
+    _same_walks(g, o, orc, B.NODE2VEC, 3, 32, FORMS[1], nwalkers=n)
+    g.close()
