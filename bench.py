@@ -247,7 +247,7 @@ def served(B, args):
     # served form
     g = fresh_graph()
     outs = [B.DeviceArray(g, nbytes=len(r[2]) * 80 * 4) if r[0] == "walk" else None for r in reqs]
-    srv = B.Server(g, arena_bytes=1 << 30, max_launch_walkers=1 << 22, streams=2)
+    srv = B.Server(g, arena_bytes=1 << 30, max_launch_walkers=args.launch_walkers, streams=2)
     log(f"served: {nreq} requests at {args.rate} req/s")
     tickets, t_sub = [], []
     t0 = time.perf_counter()
@@ -320,7 +320,8 @@ def served(B, args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
         "config": {"workload": "served_rmat22_e26_loguni32_mixed_poisson", "nodes": n, "arcs": int(len(base)),
                    "requests": nreq, "offered_rate_per_s": args.rate, "walkers_per_query": [1000, 1000000],
-                   "update_batch": [1000, 100000], "walk_len": 80, "apps": ["deepwalk", "node2vec"]},
+                   "update_batch": [1000, 100000], "walk_len": 80, "apps": ["deepwalk", "node2vec"],
+                   "launch_walkers": args.launch_walkers},
         "achieved_rate_per_s": nreq / wall,
         "submit_span_s": submit_span,
         "p50_ms": st["p50_ms"], "p90_ms": st["p90_ms"], "p99_ms": st["p99_ms"], "max_ms": st["max_ms"],
@@ -365,6 +366,8 @@ def main():
                     help="one workload; default: the served form (c4) line, then the scale workload (c5) line")
     ap.add_argument("--rate", type=float, default=2000.0, help="c4: offered requests per second")
     ap.add_argument("--requests", type=int, default=300, help="c4: number of requests")
+    ap.add_argument("--launch-walkers", type=int, default=1 << 22,
+                    help="c4: the serving loop's walkers per launch (larger requests are chunked)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--app", choices=APP_NAMES, default=None, help="walk app instead of the configuration's")
     ap.add_argument("--float-weights", action="store_true", help="float32 weights w / 256 (wtype::Float)")
