@@ -1562,17 +1562,16 @@ int bingo_compact(bingo_graph *g, double min_dead_frac, uint64_t *nodes_out, uin
     CompactBig cb{};
     if (nbig) {
         auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-        const size_t sz[] = {al((nbig + 1) * 4ull), al(c.comp_nseg * 4), al(c.comp_nseg * 4), al(nbig * 8ull),
-                             al(nbig * 8ull),       al(nbig * 4ull),    al(nbig * 4ull),    al(c.comp_bigslots * 16),
-                             al(c.comp_bigback * 4)};
+        const size_t sz[] = {al((nbig + 1) * 4ull),          al(c.comp_nseg * 4), al(c.comp_nseg * 4),
+                             al(c.comp_nseg * 4 * kGroups),    al(nbig * 8ull),     al(nbig * 4ull),
+                             al(nbig * 4ull),                  al(c.comp_bigslots * 16)};
         size_t tot = 0;
         for (size_t x : sz) tot += x;
         if ((rc = ensure_scratch(g, SC_CBIG, tot))) return rc;
         char *p = static_cast<char *>(g->sc[SC_CBIG]);
-        void **dst[] = {(void **)&cb.segbase, (void **)&cb.seg_live, (void **)&cb.seg_pc, (void **)&cb.trec,
-                        (void **)&cb.tback,   (void **)&cb.nlive,    (void **)&cb.nback,  (void **)&cb.rec,
-                        (void **)&cb.back};
-        for (int i = 0; i < 9; ++i) {
+        void **dst[] = {(void **)&cb.segbase, (void **)&cb.seg_live, (void **)&cb.seg_pc, (void **)&cb.seg_gc,
+                        (void **)&cb.trec,    (void **)&cb.nlive,    (void **)&cb.nback,  (void **)&cb.rec};
+        for (int i = 0; i < 8; ++i) {
             *dst[i] = p;
             p += sz[i];
         }

@@ -1682,24 +1682,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 
 // ------------------------------------------------------------------------
 // compaction: tombstoned slots are reclaimed in place.  A listed node's live
-// slots are renumbered by rank (the k-th live slot becomes slot k), their
-// slot records and back-index entries are packed to the front of the node's
-// regions, and each member's slot field takes the new number; bucket order,
-// masses and the alias table stay as they are.  The neighbour table is
-// rebuilt from the live slots (its dead keys go).  The reference tombstones
-// and never reuses a slot (include/operation.cuh:209-210, Bingo.cuh:199-218).
-//
-// In place: a live slot's new position and new back offset are at or below
-// its old ones, so a chunk of slots is read completely (records and back
-// entries) before any of its writes, and later chunks read only above what
-// earlier chunks write.
+// slots are renumbered by rank (the k-th live slot becomes slot k) and their
+// records packed to the front of the node's slot arrays; its listed groups'
+// member lists and its back entries are rebuilt from the slots in slot order
+// (the k-th member of group j is the k-th live slot with bit j).  Masses and
+// the alias table stay as they are.  The neighbour table is rebuilt from the
+// live slots (its dead keys go).  The reference tombstones and never reuses a
+// slot (include/operation.cuh:209-210, Bingo.cuh:199-218).
 // ------------------------------------------------------------------------
 __global__ void k_compact_reset(Ctl *ctl)
 {
     if (threadIdx.x == 0) {
         ctl->ncomp[0] = ctl->ncomp[1] = 0;
         ctl->comp_freed = 0;
-        ctl->comp_bigslots = ctl->comp_bigback = ctl->comp_nseg = 0;
+        ctl->comp_bigslots = ctl->comp_nseg = 0;
     }
 }
 
@@ -1752,7 +1748,7 @@ __global__ __launch_bounds__(256) void k_compact_select(DevStore s, uint32_t thr
     const Team<256> tm{part, (int)threadIdx.x};
     const int64_t n1 = (int64_t)s.n + 1;
     const int64_t tile = 256 * kSelPer;
-    uint64_t freed = 0, bslots = 0, bback = 0, bseg = 0;
+    uint64_t freed = 0, bslots = 0, bseg = 0;
     for (int64_t t0 = (int64_t)blockIdx.x * tile; t0 < n1; t0 += (int64_t)gridDim.x * tile) {
         uint32_t ps = 0, pb = 0;  // this thread's picks (bit k: node t0 + k 256 + tid)
 #pragma unroll
@@ -1766,33 +1762,40 @@ __global__ __launch_bounds__(256) void k_compact_select(DevStore s, uint32_t thr
             if (h.nslots > kBigCompact) {
                 pb |= 1u << k;
                 bslots += h.nslots;
-                bback += h.back_used;
                 bseg += (h.nslots + kCompactSeg - 1) / kCompactSeg;
             } else {
                 ps |= 1u << k;
             }
         }
-        uint32_t ts, tb;
-        const uint32_t es = tm.excl((uint32_t)__popc(ps), ts), eb = tm.excl((uint32_t)__popc(pb), tb);
+        // the wave-path list in node order (one scan per k), so that a run
+        // of list entries is a run of neighbouring headers and regions
+        uint32_t es[kSelPer], tk[kSelPer], ts = 0, tb;
+#pragma unroll
+        for (uint32_t k = 0; k < kSelPer; ++k) {
+            es[k] = tm.excl((ps >> k) & 1u, tk[k]);
+            ts += tk[k];
+        }
+        const uint32_t eb = tm.excl((uint32_t)__popc(pb), tb);
         if (threadIdx.x == 0) {
             base[0] = ts ? atomicAdd(&s.ctl->ncomp[0], ts) : 0u;
             base[1] = tb ? atomicAdd(&s.ctl->ncomp[1], tb) : 0u;
         }
         __syncthreads();
-        uint32_t os = base[0] + es, ob = base[1] + eb;
+        uint32_t os = base[0], ob = base[1] + eb;
 #pragma unroll
         for (uint32_t k = 0; k < kSelPer; ++k) {
             const uint32_t u = (uint32_t)(t0 + k * 256 + threadIdx.x);
-            if (ps & (1u << k)) small_list[os++] = u;
+            if (ps & (1u << k)) small_list[os + es[k]] = u;
+            os += tk[k];
             if (pb & (1u << k)) big_list[ob++] = u;
         }
         __syncthreads();  // base is the next tile's
     }
     // workgroup totals from 32-bit sums of 32-, 16- and 16-bit pieces (exact)
-    const uint64_t vals[4] = {freed, bslots, bback, bseg};
-    unsigned long long *dst[4] = {&s.ctl->comp_freed, &s.ctl->comp_bigslots, &s.ctl->comp_bigback, &s.ctl->comp_nseg};
+    const uint64_t vals[3] = {freed, bslots, bseg};
+    unsigned long long *dst[3] = {&s.ctl->comp_freed, &s.ctl->comp_bigslots, &s.ctl->comp_nseg};
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < 3; ++q) {
         const uint64_t x = vals[q];
         uint32_t hi, lo16, hi16;
         tm.excl((uint32_t)(x >> 32), hi);
@@ -1803,108 +1806,248 @@ __global__ __launch_bounds__(256) void k_compact_select(DevStore s, uint32_t thr
     }
 }
 
-// Every read of a chunk before any write of it: a workgroup barrier (which
-// waits for each wave's loads), or for one wave a wait on its own loads.
-template <int NT>
-__device__ __forceinline__ void reads_done()
+// One chunk of a node's slots, [c0, min(c0 + 64, end)), by one wave: lane l
+// holds slot c0 + l as read (ChunkIn; a run of chunks is read before any of
+// them is rebuilt).
+struct ChunkIn {
+    uint32_t v, w, f;
+    bool in;
+};
+// Every lane reads (a lane past end reads slot c0 again, c0 < end), so that
+// a run's reads are a fixed number of instructions with no branches between
+// them.
+__device__ __forceinline__ ChunkIn chunk_load(const DevStore &s, const NodeHdr &h, uint32_t c0, uint32_t end, int ln)
 {
-    if (NT > 64) __syncthreads();
-    else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    const uint32_t *V = s.slot + h.slot_off;
+    const uint32_t i = c0 + (uint32_t)ln;
+    ChunkIn x;
+    x.in = i < end;
+    const uint32_t k = x.in ? i : c0;
+    x.v = V[k];
+    x.w = V[h.slot_cap + k];
+    x.f = s.fw ? V[3ull * h.slot_cap + k] : 0u;
+    return x;
 }
 
-template <int NT>
-__device__ void compact_node(const DevStore &s, int32_t u, const Team<NT> &tm, uint64_t *goff)
+// The chunk rebuilt: live slots take the ids nid, nid + 1, ... and back
+// offsets nb, ... in slot order; the members of each listed group are
+// written in slot order from position gc_j of the group's list (lane j < 32
+// holds gc_j and the list's offset go_j in the member pool), each with its
+// back entry.  Neither the old member lists nor the old back entries are
+// read: the lists follow from the slots.  STAGE (big nodes, whose segments
+// other waves move concurrently): each live record goes to rec[id] as {v, w,
+// fraction, back offset}; otherwise it moves down to slot id in place.  In
+// place is safe because a record's id is at most its position: the chunk's
+// writes wait for its reads (they store what was read; vector memory
+// operations complete in issue order), and chunks read later lie above every
+// position written so far.
+// A chunk's members of each listed group, in slot order from position gc_j;
+// each slot's back entries (one per listed bit, in bit order) at bdst[b0..]
+// (lw: the lane's listed bits, ent: its member word).
+template <typename BT>
+__device__ __forceinline__ void chunk_members(const DevStore &s, uint32_t lw, uint64_t ent, uint32_t b0, uint64_t go,
+                                              uint32_t &gc, BT *bdst, int ln)
 {
-    const int t = tm.t;
-    const NodeHdr h = s.hdr[u];
+    const uint64_t lt = (1ull << ln) - 1ull;
+    for (uint32_t any = dpp_or32(lw); any; any &= any - 1u) {
+        const int j = __ffs(any) - 1;
+        const bool has = (lw >> j) & 1u;
+        const uint64_t m = __ballot(has);
+        const uint32_t base = rl32(gc, j);
+        const uint64_t gj = rl64(go, j);
+        if (has) {
+            const uint32_t pos = base + (uint32_t)__popcll(m & lt);
+            s.mem[gj + pos] = ent;
+            bdst[b0 + (uint32_t)__popc(lw & ((1u << j) - 1u))] = pos;
+        }
+        if (ln == j) gc += (uint32_t)__popcll(m);
+    }
+}
+
+// Back entries are gathered in LDS (bt: kBackTab words per wave) and
+// written in order, four 64-byte lines per store, rather than scattered
+// across the chunk's back range once per group; a chunk with more than
+// kBackTab of them writes them directly.
+constexpr uint32_t kBackTab = 1024;
+template <bool STAGE>
+__device__ __forceinline__ void rebuild_chunk(const DevStore &s, const NodeHdr &h, const ChunkIn &x, uint32_t c0,
+                                              uint64_t go, uint32_t &gc, uint32_t &nid, uint32_t &nb, uint4 *rec,
+                                              uint32_t *bt, int ln)
+{
     uint32_t *V = s.slot + h.slot_off;
-    uint32_t *W = V + h.slot_cap;
-    uint32_t *B = V + 2ull * h.slot_cap;
-    uint32_t *F = V + 3ull * h.slot_cap;  // float weights only
     uint32_t *back = s.back + h.back_off;
-    if (t < kGroups) goff[t] = s.grp[(size_t)u * kGroups + t].off;
-    tm.sync();
-    uint32_t nid = 0, nb = 0;  // live slots and their back entries so far
-    for (uint32_t c0 = 0; c0 < h.nslots; c0 += NT) {
-        const uint32_t i = c0 + (uint32_t)t;
-        const bool in = i < h.nslots;
-        const uint32_t v = in ? V[i] : 0u, w = in ? W[i] : 0u, b = in ? B[i] : 0u;
-        const uint32_t f = (s.fw && in) ? F[i] : 0u;
-        const bool live = in && (w || f);
-        const uint32_t pc = live ? (uint32_t)__popc(w & ~h.dmask) : 0u;  // back entries: listed bits
-        uint32_t tot;
-        // live count (high half) and back entries (low half, at most 32 NT)
-        const uint32_t ex = tm.excl((live ? 1u << 16 : 0u) | pc, tot);
-        const uint32_t id = nid + (ex >> 16), nbk = nb + (ex & 0xFFFFu);
-        const bool mslot = live && id != i, mback = live && nbk != b;
-        uint32_t e[kGroups];
-#pragma unroll
-        for (int r = 0; r < kGroups; ++r) e[r] = (mslot || mback) && (uint32_t)r < pc ? back[b + r] : 0u;
-        reads_done<NT>();
-        if (mslot) {
-            V[id] = v;
-            W[id] = w;
-            if (s.fw) F[id] = f;
+    const uint32_t i = c0 + (uint32_t)ln;
+    const bool live = x.in && (x.w || x.f);
+    const uint32_t lw = live ? x.w & ~h.dmask : 0u;  // listed bits: one member and one back entry each
+    const uint64_t lt = (1ull << ln) - 1ull;
+    const uint64_t lv = __ballot(live);
+    const uint32_t id = nid + (uint32_t)__popcll(lv & lt);
+    const uint32_t pc = (uint32_t)__popc(lw), pinc = dpp_incl_add(pc);
+    const uint32_t nbk = nb + pinc - pc;
+    if (STAGE) {
+        if (live) rec[id] = make_uint4(x.v, x.w, x.f, nbk);
+    } else if (live) {
+        if (id != i) {
+            V[id] = x.v;
+            V[h.slot_cap + id] = x.w;
+            if (s.fw) V[3ull * h.slot_cap + id] = x.f;
         }
-        if (mslot || mback) B[id] = nbk;
-        if (mback) {
-#pragma unroll
-            for (int r = 0; r < kGroups; ++r)
-                if ((uint32_t)r < pc) back[nbk + r] = e[r];
-        }
-        if (mslot) {
-            uint32_t bits = w & ~h.dmask;
-#pragma unroll
-            for (int r = 0; r < kGroups; ++r)
-                if ((uint32_t)r < pc) {
-                    const int j = __ffs(bits) - 1;
-                    bits &= bits - 1u;
-                    reinterpret_cast<uint32_t *>(s.mem + goff[j] + e[r])[0] = id;
-                }
-        }
-        nid += tot >> 16;
-        nb += tot & 0xFFFFu;
+        V[2ull * h.slot_cap + id] = nbk;
     }
-    // the neighbour table from the live slots: counts and weight sums are
-    // unchanged, keys whose count fell to zero go
-    uint32_t hused = h.hash_used;
-    if (s.nbr && h.hash_cap) {
-        __threadfence_block();
-        tm.sync();
-        uint64_t *tab = s.hash + h.hash_off;
-        for (uint32_t q = (uint32_t)t; q < 2u * h.hash_cap; q += NT) tab[q] = 0ull;
-        __threadfence_block();
-        tm.sync();
-        uint32_t fresh = 0;
-        for (uint32_t i = (uint32_t)t; i < nid; i += NT) fresh += hash_add(tab, h.hash_cap, V[i], 1u, W[i]);
-        tm.excl(fresh, hused);
+    const uint64_t ent = ((uint64_t)x.v << 32) | id;
+    const uint32_t P = rl32(pinc, 63);  // the chunk's back entries
+    if (P <= kBackTab) {  // entry nb + q goes to bt[q]
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");  // the previous chunk's table reads first
+        chunk_members(s, lw, ent, nbk - nb, go, gc, bt, ln);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        for (uint32_t q = (uint32_t)ln; q < P; q += 64) back[nb + q] = bt[q];
+    } else {
+        chunk_members(s, lw, ent, nbk, go, gc, back, ln);
     }
-    if (t == 0) {
+    nid += (uint32_t)__popcll(lv);
+    nb += P;
+}
+
+// a compacted node's header (its neighbour table, if any, is rebuilt by
+// k_compact_hash)
+__device__ __forceinline__ void compact_finish(const DevStore &s, int32_t u, const NodeHdr &h, uint32_t nid,
+                                               uint32_t nb, int ln)
+{
+    if (ln == 0) {
         NodeHdr nh = h;
         nh.nslots = nid;
         nh.nlive = nid;
         nh.back_used = nb;
-        nh.hash_used = hused;
         s.hdr[u] = nh;
     }
 }
 
-// listed nodes of at most kBigCompact slots: one wave each
-__global__ __launch_bounds__(256) void k_compact_small(DevStore s, const uint32_t *__restrict__ list)
+// lane l's header as seen by every lane
+__device__ __forceinline__ NodeHdr hdr_of_lane(const NodeHdr &x, int l)
 {
-    __shared__ uint64_t goff[4][kGroups];
-    const Team<64> tm{nullptr, lane_id()};
+    static_assert(sizeof(NodeHdr) == 64, "NodeHdr is 16 words");
+    uint32_t a[16], b[16];
+    __builtin_memcpy(a, &x, 64);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) b[q] = rl32(a[q], l);
+    NodeHdr r;
+    __builtin_memcpy(&r, b, 64);
+    return r;
+}
+
+// lane j (and j + 32) holds the member-pool offset of group j's list
+__device__ __forceinline__ uint64_t group_offsets(const DevStore &s, int32_t u, int ln)
+{
+    return s.grp[(size_t)u * kGroups + (ln & (kGroups - 1))].off;
+}
+
+// the value of x once its load has completed (a use that the compiler must
+// wait for here, so that later uses carry no pending load)
+__device__ __forceinline__ void settle(uint32_t &x) { asm volatile("" : "+v"(x)); }
+
+// listed nodes of at most kBigCompact slots, one wave each.  Batches of 64
+// consecutive list entries go round robin over the waves, and a batch's 64
+// headers are read at once, one per lane.  Its chunks of slots are then
+// taken kChunkRun at a time: all their reads (and the group offsets of each
+// node they start) are issued before any of their writes.  Vector memory
+// operations complete in issue order and a wait for a read also waits for
+// every write before it, so the wave waits once per run, not per chunk.
+// Eight waves per SIMD: the grid (8 workgroups per CU) is resident at once.
+constexpr int kChunkRun = 4;
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_compact_small(
+    DevStore s, const uint32_t *__restrict__ list)
+{
+    __shared__ uint32_t btab[4][kBackTab];
+    const int ln = lane_id();
+    uint32_t *bt = btab[threadIdx.x >> 6];
+    const uint64_t cnt = s.ctl->ncomp[0];
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const uint64_t step = (uint64_t)((gridDim.x * blockDim.x) >> 6) * 64;
+    for (uint64_t k0 = (uint64_t)wave * 64; k0 < cnt; k0 += step) {
+        uint32_t uk = k0 + ln < cnt ? list[k0 + ln] : 0u;
+        NodeHdr hk{};
+        if (k0 + ln < cnt) hk = s.hdr[uk];
+        {
+            uint32_t a[16];
+            __builtin_memcpy(a, &hk, 64);
+            settle(uk);
+#pragma unroll
+            for (int q = 0; q < 16; ++q) settle(a[q]);
+            __builtin_memcpy(&hk, a, 64);
+        }
+        const int nq = (int)min<uint64_t>(64, cnt - k0);
+        int q = 0;       // the next chunk to read: node q of the batch, from slot c0
+        uint32_t c0 = 0;
+        uint64_t go = 0;  // the node being rebuilt: group offsets, members per group, ids and back entries so far
+        uint32_t gc = 0, nid = 0, nb = 0;
+        while (q < nq) {
+            ChunkIn X[kChunkRun];
+            uint64_t G[kChunkRun];
+            int Q[kChunkRun];
+            uint32_t C[kChunkRun];
+#pragma unroll
+            for (int r = 0; r < kChunkRun; ++r) {  // past the batch's end: node 0's first chunk, unused
+                const bool ok = q < nq;
+                const int qr = ok ? q : 0;
+                Q[r] = q;
+                C[r] = ok ? c0 : 0u;
+                const NodeHdr h = hdr_of_lane(hk, qr);
+                X[r] = chunk_load(s, h, C[r], h.nslots, ln);
+                G[r] = group_offsets(s, (int32_t)rl32(uk, qr), ln);
+                if (ok && c0 + 64 < h.nslots) {
+                    c0 += 64;
+                } else if (ok) {
+                    ++q;
+                    c0 = 0;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < kChunkRun; ++r) {
+                if (Q[r] >= nq) break;
+                const NodeHdr h = hdr_of_lane(hk, Q[r]);
+                if (C[r] == 0) {
+                    go = G[r];
+                    gc = nid = nb = 0;
+                }
+                rebuild_chunk<false>(s, h, X[r], C[r], go, gc, nid, nb, nullptr, bt, ln);
+                if (C[r] + 64 >= h.nslots) compact_finish(s, (int32_t)rl32(uk, Q[r]), h, nid, nb, ln);
+            }
+        }
+    }
+}
+
+// neighbour tables of the compacted small nodes (stores with them), one wave
+// per node, from the live slots: counts and weight sums are unchanged, keys
+// whose count fell to zero go
+__global__ __launch_bounds__(256) void k_compact_hash(DevStore s, const uint32_t *__restrict__ list)
+{
+    const int ln = lane_id();
+    const Team<64> tm{nullptr, ln};
     const uint32_t cnt = s.ctl->ncomp[0];
     const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (uint32_t k = wave; k < cnt; k += nwaves) compact_node<64>(s, (int32_t)list[k], tm, goff[threadIdx.x >> 6]);
+    for (uint32_t k = wave; k < cnt; k += nwaves) {
+        const int32_t u = (int32_t)list[k];
+        const NodeHdr h = s.hdr[u];
+        if (!h.hash_cap) continue;
+        uint64_t *tab = s.hash + h.hash_off;
+        const uint32_t *V = s.slot + h.slot_off;
+        for (uint32_t q = (uint32_t)ln; q < 2u * h.hash_cap; q += 64) tab[q] = 0ull;
+        __threadfence_block();
+        tm.sync();
+        uint32_t fresh = 0, hused;
+        for (uint32_t i = (uint32_t)ln; i < h.nslots; i += 64) fresh += hash_add(tab, h.hash_cap, V[i], 1u, V[h.slot_cap + i]);
+        tm.excl(fresh, hused);
+        if (ln == 0) s.hdr[u].hash_used = hused;
+    }
 }
 
 // ---- big nodes: segments of kCompactSeg slots over the whole grid ----
-// Renumbering needs each segment's live count and back entries before it;
-// the in-place moves of one node would collide across workgroups, so the
-// live records and back entries are staged out (with the members' slot
-// fields patched on the way) and written back by a second kernel.
+// Renumbering needs each segment's live count, back entries and per-group
+// member counts before it; the in-place moves of one node's records would
+// collide across waves, so the live records are staged out and written back
+// by a second kernel.  Members and back entries go straight to their final
+// places, which no other segment writes.
 
 // the big node a segment belongs to (largest i with segbase[i] <= seg)
 __device__ __forceinline__ uint32_t seg_node(const uint32_t *segbase, uint32_t nbig, uint32_t seg)
@@ -1935,38 +2078,47 @@ __global__ __launch_bounds__(1024) void k_compact_segbase(DevStore s, const uint
     if (threadIdx.x == 0) cb.segbase[nbig] = run;
 }
 
-// live slots and their back entries per segment (workgroups of 256)
+// per segment (one wave each): live slots, back entries and members of each
+// listed group
 __global__ __launch_bounds__(256) void k_compact_segcount(DevStore s, const uint32_t *__restrict__ big, uint32_t nbig,
                                                           uint64_t nseg, CompactBig cb)
 {
-    __shared__ uint32_t part[4];
-    const Team<256> tm{part, (int)threadIdx.x};
-    for (uint64_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
+    const int ln = lane_id();
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint64_t sg = wave; sg < nseg; sg += nwaves) {
         const uint32_t i = seg_node(cb.segbase, nbig, (uint32_t)sg);
         const NodeHdr h = s.hdr[big[i]];
         const uint32_t *V = s.slot + h.slot_off;
         const uint32_t *W = V + h.slot_cap, *F = V + 3ull * h.slot_cap;
         const uint32_t a = ((uint32_t)sg - cb.segbase[i]) * kCompactSeg, b = min(a + kCompactSeg, h.nslots);
-        uint32_t live = 0, pc = 0;
-        for (uint32_t j = a + threadIdx.x; j < b; j += 256) {
-            const uint32_t w = W[j];
-            if (w || (s.fw && F[j])) {
-                ++live;
-                pc += (uint32_t)__popc(w & ~h.dmask);
+        uint32_t live = 0, pc = 0, gc = 0;
+        for (uint32_t c0 = a; c0 < b; c0 += 64) {
+            const uint32_t j = c0 + (uint32_t)ln;
+            const bool in = j < b;
+            const uint32_t w = in ? W[j] : 0u, f = (s.fw && in) ? F[j] : 0u;
+            const bool lv = in && (w || f);
+            const uint32_t lw = lv ? w & ~h.dmask : 0u;
+            live += lv ? 1u : 0u;
+            pc += (uint32_t)__popc(lw);
+            for (uint32_t any = dpp_or32(lw); any; any &= any - 1u) {
+                const int g = __ffs(any) - 1;
+                const uint64_t m = __ballot((lw >> g) & 1u);
+                if (ln == g) gc += (uint32_t)__popcll(m);
             }
         }
-        uint32_t tl, tp;
-        tm.excl(live, tl);
-        tm.excl(pc, tp);
-        if (threadIdx.x == 0) {
-            cb.seg_live[sg] = tl;
-            cb.seg_pc[sg] = tp;
+        live = dpp_sum32(live);
+        pc = dpp_sum32(pc);
+        if (ln < kGroups) cb.seg_gc[sg * kGroups + ln] = gc;
+        if (ln == 0) {
+            cb.seg_live[sg] = live;
+            cb.seg_pc[sg] = pc;
         }
     }
 }
 
-// per big node (one wave each): its segments' first new id and back offset
-// (in place of the counts) and its totals
+// per big node (one wave each): in place of its segments' counts, the first
+// new id, back offset and per-group member position of each; and its totals
 __global__ __launch_bounds__(256) void k_compact_segscan(uint32_t nbig, CompactBig cb)
 {
     const int ln = lane_id();
@@ -1974,7 +2126,7 @@ __global__ __launch_bounds__(256) void k_compact_segscan(uint32_t nbig, CompactB
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     for (uint32_t i = wave; i < nbig; i += nwaves) {
         const uint32_t s0 = cb.segbase[i], s1 = cb.segbase[i + 1];
-        uint32_t rl = 0, rp = 0;
+        uint32_t rl = 0, rp = 0, rg = 0;  // rg: lane j's running member count of group j
         for (uint32_t c0 = s0; c0 < s1; c0 += 64) {
             const uint32_t sg = c0 + (uint32_t)ln;
             const bool in = sg < s1;
@@ -1986,6 +2138,16 @@ __global__ __launch_bounds__(256) void k_compact_segscan(uint32_t nbig, CompactB
             }
             rl += rl32(il, 63);
             rp += rl32(ip, 63);
+            uint32_t x[kGroups];
+#pragma unroll
+            for (int g = 0; g < kGroups; ++g) x[g] = in ? cb.seg_gc[(uint64_t)sg * kGroups + g] : 0u;
+#pragma unroll
+            for (int g = 0; g < kGroups; ++g) {
+                const uint32_t ig = dpp_incl_add(x[g]);
+                if (in) cb.seg_gc[(uint64_t)sg * kGroups + g] = rl32(rg, g) + ig - x[g];
+                const uint32_t tg = rl32(ig, 63);
+                if (ln == g) rg += tg;
+            }
         }
         if (ln == 0) {
             cb.nlive[i] = rl;
@@ -1994,78 +2156,55 @@ __global__ __launch_bounds__(256) void k_compact_segscan(uint32_t nbig, CompactB
     }
 }
 
-// one workgroup: the big nodes' staging offsets (prefixes of their totals)
+// one workgroup: the big nodes' record staging offsets (prefix of their live counts)
 __global__ __launch_bounds__(1024) void k_compact_nodescan(uint32_t nbig, CompactBig cb)
 {
     __shared__ uint32_t part[16];
     const Team<1024> tm{part, (int)threadIdx.x};
-    uint64_t trec = 0, tback = 0;
+    uint64_t trec = 0;
     for (uint32_t c0 = 0; c0 < nbig; c0 += 1024) {
         const uint32_t i = c0 + threadIdx.x;
         const bool in = i < nbig;
         // 32-bit scans of the high and low 16 bits (1024 values each)
-        const uint32_t nl = in ? cb.nlive[i] : 0u, nk = in ? cb.nback[i] : 0u;
-        uint32_t a, b, c, d;
+        const uint32_t nl = in ? cb.nlive[i] : 0u;
+        uint32_t a, b;
         const uint32_t lh = tm.excl(nl >> 16, a), ll = tm.excl(nl & 0xFFFFu, b);
-        const uint32_t kh = tm.excl(nk >> 16, c), kl = tm.excl(nk & 0xFFFFu, d);
-        if (in) {
-            cb.trec[i] = trec + ((uint64_t)lh << 16) + ll;
-            cb.tback[i] = tback + ((uint64_t)kh << 16) + kl;
-        }
+        if (in) cb.trec[i] = trec + ((uint64_t)lh << 16) + ll;
         trec += ((uint64_t)a << 16) + b;
-        tback += ((uint64_t)c << 16) + d;
     }
 }
 
-// stage each live slot's record (new back offset) and back entries, and
-// patch its members' slot fields with its new number
+// per segment (one wave each): stage the live records, write the members
+// and back entries in their final places
 __global__ __launch_bounds__(256) void k_compact_stage(DevStore s, const uint32_t *__restrict__ big, uint32_t nbig,
                                                        uint64_t nseg, CompactBig cb)
 {
-    __shared__ uint32_t part[4];
-    __shared__ uint64_t goff[kGroups];
-    const Team<256> tm{part, (int)threadIdx.x};
-    for (uint64_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
+    __shared__ uint32_t btab[4][kBackTab];
+    const int ln = lane_id();
+    uint32_t *bt = btab[threadIdx.x >> 6];
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint64_t sg = wave; sg < nseg; sg += nwaves) {
         const uint32_t i = seg_node(cb.segbase, nbig, (uint32_t)sg);
         const int32_t u = (int32_t)big[i];
         const NodeHdr h = s.hdr[u];
-        if (threadIdx.x < kGroups) goff[threadIdx.x] = s.grp[(size_t)u * kGroups + threadIdx.x].off;
-        __syncthreads();
-        const uint32_t *V = s.slot + h.slot_off;
-        const uint32_t *W = V + h.slot_cap, *B = V + 2ull * h.slot_cap, *F = V + 3ull * h.slot_cap;
-        const uint32_t *back = s.back + h.back_off;
-        uint4 *rec = cb.rec + cb.trec[i];
-        uint32_t *tb = cb.back + cb.tback[i];
-        const uint32_t a = ((uint32_t)sg - cb.segbase[i]) * kCompactSeg, b = min(a + kCompactSeg, h.nslots);
+        const uint64_t go = group_offsets(s, u, ln);
+        uint32_t gc = ln < kGroups ? cb.seg_gc[sg * kGroups + ln] : 0u;
         uint32_t nid = cb.seg_live[sg], nb = cb.seg_pc[sg];
-        for (uint32_t c0 = a; c0 < b; c0 += 256) {
-            const uint32_t j = c0 + threadIdx.x;
-            const bool in = j < b;
-            const uint32_t w = in ? W[j] : 0u, f = (s.fw && in) ? F[j] : 0u;
-            const bool live = in && (w || f);
-            const uint32_t pc = live ? (uint32_t)__popc(w & ~h.dmask) : 0u;
-            uint32_t tot;
-            const uint32_t ex = tm.excl((live ? 1u << 16 : 0u) | pc, tot);
-            if (live) {
-                const uint32_t id = nid + (ex >> 16), nbk = nb + (ex & 0xFFFFu), ob = B[j];
-                rec[id] = make_uint4(V[j], w, f, nbk);
-                uint32_t bits = w & ~h.dmask;
-                for (uint32_t r = 0; r < pc; ++r) {
-                    const int jj = __ffs(bits) - 1;
-                    bits &= bits - 1u;
-                    const uint32_t pos = back[ob + r];
-                    tb[nbk + r] = pos;
-                    reinterpret_cast<uint32_t *>(s.mem + goff[jj] + pos)[0] = id;
-                }
-            }
-            nid += tot >> 16;
-            nb += tot & 0xFFFFu;
+        uint4 *rec = cb.rec + cb.trec[i];
+        const uint32_t a = ((uint32_t)sg - cb.segbase[i]) * kCompactSeg, b = min(a + kCompactSeg, h.nslots);
+        for (uint32_t c0 = a; c0 < b; c0 += kChunkRun * 64) {  // runs of chunks, as in k_compact_small
+            ChunkIn X[kChunkRun];
+#pragma unroll
+            for (int r = 0; r < kChunkRun; ++r) X[r] = chunk_load(s, h, c0 + r * 64 < b ? c0 + r * 64 : c0, b, ln);
+#pragma unroll
+            for (int r = 0; r < kChunkRun; ++r)
+                if (c0 + r * 64 < b) rebuild_chunk<true>(s, h, X[r], c0 + r * 64, go, gc, nid, nb, rec, bt, ln);
         }
-        __syncthreads();  // goff is the next segment's
     }
 }
 
-// write the staged records and back entries to the front of the node's regions
+// write the staged records to the front of the node's slot arrays
 __global__ __launch_bounds__(256) void k_compact_unstage(DevStore s, const uint32_t *__restrict__ big, uint32_t nbig,
                                                          uint64_t nseg, CompactBig cb)
 {
@@ -2074,12 +2213,9 @@ __global__ __launch_bounds__(256) void k_compact_unstage(DevStore s, const uint3
         const NodeHdr h = s.hdr[big[i]];
         uint32_t *V = s.slot + h.slot_off;
         uint32_t *W = V + h.slot_cap, *B = V + 2ull * h.slot_cap, *F = V + 3ull * h.slot_cap;
-        uint32_t *back = s.back + h.back_off;
         const bool last = (uint32_t)sg + 1 == cb.segbase[i + 1];
         const uint32_t id0 = cb.seg_live[sg], id1 = last ? cb.nlive[i] : cb.seg_live[sg + 1];
-        const uint32_t b0 = cb.seg_pc[sg], b1 = last ? cb.nback[i] : cb.seg_pc[sg + 1];
         const uint4 *rec = cb.rec + cb.trec[i];
-        const uint32_t *tb = cb.back + cb.tback[i];
         for (uint32_t id = id0 + threadIdx.x; id < id1; id += 256) {
             const uint4 r = rec[id];
             V[id] = r.x;
@@ -2087,7 +2223,6 @@ __global__ __launch_bounds__(256) void k_compact_unstage(DevStore s, const uint3
             if (s.fw) F[id] = r.z;
             B[id] = r.w;
         }
-        for (uint32_t q = b0 + threadIdx.x; q < b1; q += 256) back[q] = tb[q];
     }
 }
 
@@ -3439,16 +3574,18 @@ void launch_compact_apply(DevStore s, const uint32_t *small_list, const uint32_t
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     if (nsmall) {
-        const unsigned sb = (unsigned)std::min<int64_t>((nsmall + 3) / 4, (int64_t)ncu * 8);
+        const unsigned sb = (unsigned)std::min<int64_t>((nsmall + 255) / 256, (int64_t)ncu * 8);
         hipLaunchKernelGGL(k_compact_small, dim3(sb), dim3(256), 0, st, s, small_list);
+        if (s.nbr) hipLaunchKernelGGL(k_compact_hash, dim3(sb), dim3(256), 0, st, s, small_list);
     }
     if (!nbig) return;
+    const unsigned gw = (unsigned)std::min<uint64_t>((nseg + 3) / 4, (uint64_t)ncu * 8);  // a wave per segment
     const unsigned gb = (unsigned)std::min<uint64_t>(nseg, (uint64_t)ncu * 8);
     hipLaunchKernelGGL(k_compact_segbase, dim3(1), dim3(1024), 0, st, s, big_list, nbig, cb);
-    hipLaunchKernelGGL(k_compact_segcount, dim3(gb), dim3(256), 0, st, s, big_list, nbig, nseg, cb);
+    hipLaunchKernelGGL(k_compact_segcount, dim3(gw), dim3(256), 0, st, s, big_list, nbig, nseg, cb);
     hipLaunchKernelGGL(k_compact_segscan, dim3((nbig + 3) / 4), dim3(256), 0, st, nbig, cb);
     hipLaunchKernelGGL(k_compact_nodescan, dim3(1), dim3(1024), 0, st, nbig, cb);
-    hipLaunchKernelGGL(k_compact_stage, dim3(gb), dim3(256), 0, st, s, big_list, nbig, nseg, cb);
+    hipLaunchKernelGGL(k_compact_stage, dim3(gw), dim3(256), 0, st, s, big_list, nbig, nseg, cb);
     hipLaunchKernelGGL(k_compact_unstage, dim3(gb), dim3(256), 0, st, s, big_list, nbig, nseg, cb);
     hipLaunchKernelGGL(k_compact_bigfinish, dim3((unsigned)std::min<uint32_t>(nbig, (uint32_t)ncu * 4)), dim3(256), 0,
                        st, s, big_list, nbig, cb);

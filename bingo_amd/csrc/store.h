@@ -120,7 +120,6 @@ struct Ctl {
     unsigned int ncomp[2];           // compaction: nodes listed for the wave path and the workgroup path
     unsigned long long comp_freed;   // compaction: tombstoned slots reclaimed
     unsigned long long comp_bigslots;  // compaction: slots of the listed big nodes
-    unsigned long long comp_bigback;   // compaction: back entries of the listed big nodes
     unsigned long long comp_nseg;      // compaction: segments of the listed big nodes
     unsigned long long fl_total;     // insert batch: free-list entries after the batch
     unsigned long long pend_total;   // insert batch: regions the batch frees
@@ -354,23 +353,22 @@ void launch_export_b(DevStore s, const uint64_t *slot_out_off, const uint64_t *m
                      uint32_t *slot_w, uint64_t *members, uint32_t *backpos, uint32_t *slot_f, hipStream_t st);
 // Compaction of tombstoned slots: nodes with dead = nslots - nlive >= 1 and
 // dead 2^16 >= thr16 nslots are listed (room for n + 1 entries in each list)
-// by launch_compact_select; Ctl::ncomp, comp_bigslots, comp_bigback and
-// comp_nseg then size the big-node scratch (CompactBig) for
-// launch_compact_apply.  Nodes of at most kBigCompact slots are rewritten
-// by one wave each; larger ones in segments of kCompactSeg slots spread
-// over the grid, staged through the scratch.
+// by launch_compact_select; Ctl::ncomp, comp_bigslots and comp_nseg then
+// size the big-node scratch (CompactBig) for launch_compact_apply.  Nodes of
+// at most kBigCompact slots are rewritten by one wave each; larger ones in
+// segments of kCompactSeg slots spread over the grid (a wave each), their
+// records staged through the scratch.
 constexpr uint32_t kBigCompact = 8192;
 constexpr uint32_t kCompactSeg = 2048;
 struct CompactBig {
     uint32_t *segbase;  // [nbig + 1] first segment of each big node
     uint32_t *seg_live; // [nseg] live slots per segment, then (in place) the new id of its first
     uint32_t *seg_pc;   // [nseg] back entries per segment, then the new offset of its first
+    uint32_t *seg_gc;   // [nseg][32] members of each listed group per segment, then its first position
     uint64_t *trec;     // [nbig] staging offset of the node's records
-    uint64_t *tback;    // [nbig] staging offset of the node's back entries
     uint32_t *nlive;    // [nbig] live slots after compaction
     uint32_t *nback;    // [nbig] back entries after compaction
     uint4 *rec;         // staged records {v, w, fraction, back offset} (comp_bigslots)
-    uint32_t *back;     // staged back entries (comp_bigback)
 };
 void launch_compact_select(DevStore s, uint32_t thr16, uint32_t *small_list, uint32_t *big_list, hipStream_t st);
 void launch_compact_apply(DevStore s, const uint32_t *small_list, const uint32_t *big_list, uint32_t nsmall,

@@ -493,10 +493,11 @@ int orc_delete(orc_graph *g, const orc_del *d, int64_t k)
 /* Compaction of tombstoned slots (the reference never reclaims them,
  * operation.cuh:209-210).  Node u is compacted when dead = nslots - live >= 1
  * and dead * 2^16 >= floor(min_dead_frac * 2^16) * nslots: its live slots
- * keep their order and take the numbers 0..live-1, their back-index entries
- * are packed in the same order, and every member's slot field takes the new
- * number.  Bucket order, masses and the alias table do not change.  Returns
- * the number of nodes compacted (ORC_EINVAL for a fraction outside [0, 1]). */
+ * keep their order and take the numbers 0..live-1, and each of its listed
+ * groups' member lists is rebuilt in slot order (the k-th member of group j
+ * is the k-th live slot with bit j), with the back entries to match.  Masses
+ * and the alias table do not change.  Returns the number of nodes compacted
+ * (ORC_EINVAL for a fraction outside [0, 1]). */
 int64_t orc_compact(orc_graph *g, double min_dead_frac, uint64_t *slots_freed)
 {
     if (!(min_dead_frac >= 0 && min_dead_frac <= 1)) return ORC_EINVAL;
@@ -510,24 +511,19 @@ int64_t orc_compact(orc_graph *g, double min_dead_frac, uint64_t *slots_freed)
         for (uint32_t s = 0; s < x->nslots; ++s) live += (x->w[s] || (x->fr && x->fr[s])) ? 1u : 0u;
         const uint32_t dead = x->nslots - live;
         if (!dead || (uint64_t)dead * 65536u < thr16 * x->nslots) continue;
-        uint32_t ns = 0, nb = 0;
+        uint32_t ns = 0, nb = 0, pos[32] = {0};
         for (uint32_t s = 0; s < x->nslots; ++s) {
             const uint32_t w = x->w[s];
             if (!w && !(x->fr && x->fr[s])) continue;
-            const uint32_t b = x->boff[s], pc = (uint32_t)popc32(w & ~x->dmask);
-            uint32_t bits = w & ~x->dmask;
-            for (uint32_t r = 0; r < pc; ++r) {  /* nb <= b: a forward copy */
-                const int j = __builtin_ctz(bits);
-                bits &= bits - 1;
-                const uint32_t pos = x->back[b + r];
-                x->back[nb + r] = pos;
-                x->mem[j][pos] = (x->mem[j][pos] & 0xFFFFFFFF00000000ull) | ns;
-            }
             x->v[ns] = x->v[s];
             x->w[ns] = w;
             if (x->fr) x->fr[ns] = x->fr[s];
             x->boff[ns] = nb;
-            nb += pc;
+            for (uint32_t bits = w & ~x->dmask; bits; bits &= bits - 1) {  /* ns <= s, nb <= old offset */
+                const int j = __builtin_ctz(bits);
+                x->mem[j][pos[j]] = ((uint64_t)x->v[ns] << 32) | ns;
+                x->back[nb++] = pos[j]++;
+            }
             ++ns;
         }
         x->nslots = ns;

@@ -1,6 +1,6 @@
 """GPU parity of compaction (bingo_compact): the store after compaction
 rounds bit-exact against the oracle's, with small nodes (one wave each) and
-hubs of more than 8192 slots (one workgroup each), integer and float
+hubs of more than 8192 slots (segments over the grid), integer and float
 weights, with and without neighbour tables; walks and edge counts after
 compaction; and a sliding window whose generator renumbers with the store."""
 import random
@@ -69,6 +69,30 @@ def test_compaction_rounds_match_oracle(orc, nbr):
         us = np.array([rng.randint(1, n) for _ in range(2000)], np.int32)
         vs = np.array([rng.randint(1, n) for _ in range(2000)], np.int32)
         assert (g.edge_count(us, vs) == np.array([o.edge_count(int(a), int(b)) for a, b in zip(us, vs)])).all()
+    g.close()
+
+
+def test_compaction_many_bits_match_oracle(orc):
+    """Weights with 12 to 32 bits set: 64-slot chunks carry from about 800 to
+    2048 back entries, on both sides of the 1024 a wave gathers in LDS before
+    writing them in order (above it they are written directly), in small
+    nodes and in a hub's segments."""
+    rng = random.Random(17)
+    n = 300
+    e = _edges(rng, n, 32000, hubs=(5,))  # node 5: about 9600 slots (segments)
+    e["w"] = [(1 << 32) - 1 - sum(1 << b for b in rng.sample(range(32), rng.randint(0, 20))) for _ in range(len(e))]
+    print("[compact] many bits", flush=True)
+    g = B.Graph(neighbor_index=False)
+    g.load(n, e)
+    o = orc.OracleGraph(n, False)
+    assert o.load(e) == 0
+    for rnd, frac in enumerate([0.0, 0.2]):
+        d = _dels(rng, o, 5000)
+        g.delete(d)
+        assert o.delete(d) == 0
+        assert g.compact(frac) == o.compact(frac)
+        _same_state(g, o, f"many-bits compact {rnd}")
+        _same_walks(g, o, orc, B.DEEPWALK, seed=60 + rnd, max_len=16, nwalkers=2 * n)
     g.close()
 
 

@@ -1,6 +1,6 @@
 """Compaction of tombstoned slots, on the CPU: the oracle against a brute-force
 restatement (drop the dead slots of the selected nodes, keep the order), the
-bucket order kept position by position, the selection rule, and the update
+member lists rebuilt in slot order, the selection rule, and the update
 generator's renumbering against the oracle's store."""
 import random
 
@@ -53,42 +53,48 @@ def test_compaction_matches_brute_force(orc, seed):
                 assert g.edge_count(u, v) == sum(1 for s, vv, w in bg.live_arcs(u) if vv == v)
 
 
-def test_compaction_keeps_bucket_order(orc):
-    """Members stay at their positions; only the slot field changes, to the
-    slot's rank among the node's live slots."""
+def test_compaction_rebuilds_lists_in_slot_order(orc):
+    """A compacted node's member lists are its live slots with bit j in slot
+    order, under the new slot numbers; nodes left alone keep their lists
+    entry for entry.  The one-draw law of a node moves by at most the
+    multiply-shift's 2^-32 steps."""
+    from brute import transition_law
     rng = random.Random(7)
-    n = 4
+    n = 6
     g = orc.OracleGraph(n)
     bg = BruteGraph(n)
     e = _rand_edges(rng, n, 500, wmax_bits=8, hub=1)
     g.load(e)
     bg.insert(e)
     d = _rand_dels(rng, bg, 200)
+    d = d[d["u"] != 6]  # node 6 keeps every slot, so it is not compacted
     g.delete(d)
     bg.delete(d)
-    before = split_dump(g.export(), n)
-    rank = [{} for _ in range(n + 1)]
-    for u in range(1, n + 1):
-        k = 0
-        for s, (v, w) in enumerate(bg.slots[u]):
-            if w:
-                rank[u][s] = k
-                k += 1
-    assert g.compact(0.0)[0] == sum(1 for u in range(1, n + 1) if any(w == 0 for v, w in bg.slots[u]))
-    after = split_dump(g.export(), n)
+    d0 = g.export()
+    before = split_dump(d0, n)
+    compacted = {u for u in range(1, n + 1) if any(w == 0 for v, w in bg.slots[u])}
+    law0 = {u: transition_law(d0["T"][u], d0["alias"][u], before[u][2])
+            for u in range(1, n + 1) if int(d0["mass"][u])}
+    assert 6 not in compacted and len(compacted) >= 3
+    assert g.compact(0.0)[0] == len(compacted)
+    _brute_compact(bg, 0.0)
+    d1 = g.export()
+    check_against_brute(d1, bg)
+    after = split_dump(d1, n)
     for u in range(1, n + 1):
         for j in range(32):
-            B0 = [int(x) for x in before[u][2][j]]
             A0 = [int(x) for x in after[u][2][j]]
-            assert A0 == [(m >> 32) << 32 | rank[u][m & 0xFFFFFFFF] for m in B0], (u, j)
-    # walks see the same buckets in the same order: one draw per node is unchanged
-    wp = orc.walk_params(seed=5, max_len=2)
-    pa, _ = g.walk(orc.SAMPLING, wp, starts=np.arange(1, n + 1, dtype=np.int32))
-    g2 = orc.OracleGraph(n)
-    g2.load(e)
-    g2.delete(d)
-    pb, _ = g2.walk(orc.SAMPLING, wp, starts=np.arange(1, n + 1, dtype=np.int32))
-    assert np.array_equal(pa, pb)
+            if u in compacted:
+                want = [(v << 32) | s for s, (v, w) in enumerate(bg.slots[u]) if w >> j & 1]
+            else:
+                want = [int(x) for x in before[u][2][j]]
+            assert A0 == want, (u, j)
+        if u in law0:  # members move, so each may gain or lose one 2^-32 step
+            law1 = transition_law(d1["T"][u], d1["alias"][u], after[u][2])
+            assert law1.keys() == law0[u].keys(), u
+            assert max(abs(law1[v] - law0[u][v]) for v in law1) < 2.0 ** -26, u
+            if u not in compacted:
+                assert law1 == law0[u], u
 
 
 def test_compaction_rule_and_bad_fraction(orc):
