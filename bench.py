@@ -8,7 +8,8 @@ workload `c5` (R-MAT 2^25 nodes / 2^30 arcs, u32 weights log-uniform in
 [1, 2^32), 3.2e7 DeepWalk walkers of length 80, a sliding window of 10^7
 inserts + 10^7 expiries of the oldest arcs and a compaction per step).
 `value` is whole-job walk steps/s (walk steps / wall time of the K steps,
-updates and compaction included); the line also carries walk-only steps/s,
+updates and compaction included; the default K = 25 spans four compaction
+cycles, whose cost comes in bursts); the line also carries walk-only steps/s,
 updates/s, the walk kernel's roofline, the plain per-launch form on the same
 graph, and the CPU oracle (built -march=native) timed on this host's cores:
 load, update rounds and walks of a bounded sample, medians of 3.  `c2` is the
@@ -360,8 +361,10 @@ def copy_calibration(B, mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    # 25 steps hold four compaction cycles of the scale workload: its nodes
+    # reach the 1/16 threshold together, every 6 to 7 window steps
+    ap.add_argument("--steps", type=int, default=25)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default=None, choices=sorted(CONFIGS) + ["c4"],
                     help="one workload; default: the served form (c4) line, then the scale workload (c5) line")
     ap.add_argument("--rate", type=float, default=2000.0, help="c4: offered requests per second")

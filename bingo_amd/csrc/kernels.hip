@@ -864,21 +864,40 @@ __global__ __launch_bounds__(kPC) void k_alloc_resolve(DevStore s, int fw, int c
     }
 }
 
-// After a batch (and its copy tasks): class t's list
-// becomes its unused free regions followed by the batch's frees, in s.fl2.
+// After a batch (and its copy tasks): class t's list becomes its unused free
+// regions followed by the batch's frees, at [nfbase[t], nfbase[t] + nav[t])
+// of s.fl2.  The grid strides over all the new lists' entries (one class may
+// hold millions of free regions), each finding its class by a search of
+// nfbase (ascending in t; an empty class shares its base with the next).
 __global__ __launch_bounds__(256) void k_freelist_merge(DevStore s)
 {
+    __shared__ uint64_t nfb[kPC];
     if (s.ctl->ustatus) return;  // a captured batch that applied nothing
-    const int t = blockIdx.x;
-    FreeMeta *fm = s.fm;
-    const uint32_t av = fm->avail[t], from = min(fm->take[t], av), rem = av - from, fr = fm->freed[t];
-    const uint64_t nb = fm->nfbase[t], ob = fm->fbase[t] + from, pb = fm->pseg[t];
-    for (uint32_t i = threadIdx.x; i < rem; i += blockDim.x) s.fl2[nb + i] = s.fl[ob + i];
-    for (uint32_t i = threadIdx.x; i < fr; i += blockDim.x) s.fl2[nb + rem + i] = s.pf[pb + i];
+    const FreeMeta *fm = s.fm;
+    for (int t = threadIdx.x; t < kPC; t += blockDim.x) nfb[t] = fm->nfbase[t];
     __syncthreads();
-    if (threadIdx.x == 0) {
+    const uint64_t total = s.ctl->fl_total;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = kPC - 1;  // the largest t with nfb[t] <= i
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (nfb[mid] <= i) lo = mid; else hi = mid - 1;
+        }
+        const uint64_t q = i - nfb[lo];
+        const uint32_t av = fm->avail[lo], from = min(fm->take[lo], av), rem = av - from;
+        s.fl2[i] = q < rem ? s.fl[fm->fbase[lo] + from + q] : s.pf[fm->pseg[lo] + (q - rem)];
+    }
+}
+
+// then the lists' bookkeeping (after every read of the old one)
+__global__ void k_freelist_commit(DevStore s)
+{
+    if (s.ctl->ustatus) return;
+    FreeMeta *fm = s.fm;
+    const int t = threadIdx.x;
+    if (t < kPC) {
         fm->avail[t] = fm->nav[t];
-        fm->fbase[t] = nb;
+        fm->fbase[t] = fm->nfbase[t];
         fm->take[t] = 0u;
         fm->freed[t] = 0u;
     }
@@ -3349,7 +3368,8 @@ void launch_alloc_resolve(DevStore s, int fw, hipStream_t st, int captured, uint
 
 void launch_freelist_merge(DevStore s, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_freelist_merge, dim3(kPC), dim3(256), 0, st, s);
+    hipLaunchKernelGGL(k_freelist_merge, dim3(512), dim3(256), 0, st, s);
+    hipLaunchKernelGGL(k_freelist_commit, dim3(1), dim3(kPC), 0, st, s);
 }
 
 void launch_insert_plan(DevStore s, const DevEdge *sorted, Runs r, int64_t k, hipStream_t st)
