@@ -227,7 +227,7 @@ def served(B, args):
     nreq = args.requests
     reqs = []
     for i in range(nreq):
-        if rng.random() < 0.1:
+        if rng.random() < args.update_share:
             k = int(round(np.exp(rng.uniform(np.log(1e3), np.log(1e5)))))
             reqs.append(("update", ug.inserts(k // 2), ug.deletes(k - k // 2)))
         else:
@@ -321,7 +321,8 @@ def served(B, args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
         "config": {"workload": "served_rmat22_e26_loguni32_mixed_poisson", "nodes": n, "arcs": int(len(base)),
                    "requests": nreq, "offered_rate_per_s": args.rate, "walkers_per_query": [1000, 1000000],
-                   "update_batch": [1000, 100000], "walk_len": 80, "apps": ["deepwalk", "node2vec"],
+                   "update_batch": [1000, 100000], "update_share": args.update_share, "walk_len": 80,
+                   "apps": ["deepwalk", "node2vec"],
                    "launch_walkers": args.launch_walkers},
         "achieved_rate_per_s": nreq / wall,
         "submit_span_s": submit_span,
@@ -369,6 +370,8 @@ def main():
                     help="one workload; default: the served form (c4) line, then the scale workload (c5) line")
     ap.add_argument("--rate", type=float, default=2000.0, help="c4: offered requests per second")
     ap.add_argument("--requests", type=int, default=300, help="c4: number of requests")
+    ap.add_argument("--update-share", type=float, default=0.1, metavar="F",
+                    help="c4: share of the requests that are update batches")
     ap.add_argument("--launch-walkers", type=int, default=1 << 22,
                     help="c4: the serving loop's walkers per launch (larger requests are chunked)")
     ap.add_argument("--no-cpu", action="store_true")
