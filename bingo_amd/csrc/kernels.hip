@@ -1848,18 +1848,6 @@ __device__ __forceinline__ ChunkIn chunk_load(const DevStore &s, const NodeHdr &
     return x;
 }
 
-// The chunk rebuilt: live slots take the ids nid, nid + 1, ... and back
-// offsets nb, ... in slot order; the members of each listed group are
-// written in slot order from position gc_j of the group's list (lane j < 32
-// holds gc_j and the list's offset go_j in the member pool), each with its
-// back entry.  Neither the old member lists nor the old back entries are
-// read: the lists follow from the slots.  STAGE (big nodes, whose segments
-// other waves move concurrently): each live record goes to rec[id] as {v, w,
-// fraction, back offset}; otherwise it moves down to slot id in place.  In
-// place is safe because a record's id is at most its position: the chunk's
-// writes wait for its reads (they store what was read; vector memory
-// operations complete in issue order), and chunks read later lie above every
-// position written so far.
 // A chunk's members of each listed group, in slot order from position gc_j;
 // each slot's back entries (one per listed bit, in bit order) at bdst[b0..]
 // (lw: the lane's listed bits, ent: its member word).
@@ -1883,11 +1871,23 @@ __device__ __forceinline__ void chunk_members(const DevStore &s, uint32_t lw, ui
     }
 }
 
+constexpr uint32_t kBackTab = 1024;
+// The chunk rebuilt: live slots take the ids nid, nid + 1, ... and back
+// offsets nb, ... in slot order; the members of each listed group are
+// written in slot order from position gc_j of the group's list (lane j < 32
+// holds gc_j and the list's offset go_j in the member pool), each with its
+// back entry.  Neither the old member lists nor the old back entries are
+// read: the lists follow from the slots.  STAGE (big nodes, whose segments
+// other waves move concurrently): each live record goes to rec[id] as {v, w,
+// fraction, back offset}; otherwise it moves down to slot id in place.  In
+// place is safe because a record's id is at most its position: the chunk's
+// writes wait for its reads (they store what was read; vector memory
+// operations complete in issue order), and chunks read later lie above every
+// position written so far.
 // Back entries are gathered in LDS (bt: kBackTab words per wave) and
 // written in order, four 64-byte lines per store, rather than scattered
 // across the chunk's back range once per group; a chunk with more than
 // kBackTab of them writes them directly.
-constexpr uint32_t kBackTab = 1024;
 template <bool STAGE>
 __device__ __forceinline__ void rebuild_chunk(const DevStore &s, const NodeHdr &h, const ChunkIn &x, uint32_t c0,
                                               uint64_t go, uint32_t &gc, uint32_t &nid, uint32_t &nb, uint4 *rec,
