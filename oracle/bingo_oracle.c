@@ -18,6 +18,12 @@
  *   float weights .... exact law (1 - F/(S+F)) I/S + (F/(S+F)) f/F = w/sum(w)
  *                      from the exported state, chi-square of draws,
  *                      threshold against 128-bit division, float splitting
+ *   dense groups ..... class rule restated from each node's first batch,
+ *                      brute-force store, exact law of a rejection draw by
+ *                      enumeration, chi-square, the trial fallback
+ *   compaction ....... brute-force drop of the dead slots; member lists equal
+ *                      the live slots with bit j in slot order; the one-draw
+ *                      law moves by less than 2^-26
  */
 #include "bingo_oracle.h"
 #include "philox.h"
