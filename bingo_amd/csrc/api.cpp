@@ -1550,15 +1550,15 @@ int bingo_compact(bingo_graph *g, double min_dead_frac, uint64_t *nodes_out, uin
     const uint32_t thr16 = (uint32_t)std::floor(min_dead_frac * 65536.0);
     const size_t n1 = (size_t)g->s.n + 1;
     int rc;
-    if ((rc = ensure_scratch(g, SC_CLIST, 2 * n1 * sizeof(uint32_t)))) return rc;
+    if ((rc = ensure_scratch(g, SC_CLIST, 3 * n1 * sizeof(uint32_t)))) return rc;
     uint32_t *lists = scp<uint32_t>(g, SC_CLIST);
     phase_begin(g, PH_COMPACT);
-    launch_compact_select(g->s, thr16, lists, lists + n1, g->st);
+    launch_compact_select(g->s, thr16, lists, lists + n1, lists + 2 * n1, g->st);
     HIPCHK(hipGetLastError());
     // the listed counts size the big nodes' staging scratch
     if ((rc = read_ctl(g))) return rc;
     const Ctl c = *g->ctl_host;
-    const uint32_t nsmall = c.ncomp[0], nbig = c.ncomp[1];
+    const uint32_t nsmall = c.ncomp[0], nbig = c.ncomp[1], nzs = c.ncomp[2], nzb = c.ncomp[3];
     CompactBig cb{};
     if (nbig) {
         auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
@@ -1576,10 +1576,10 @@ int bingo_compact(bingo_graph *g, double min_dead_frac, uint64_t *nodes_out, uin
             p += sz[i];
         }
     }
-    launch_compact_apply(g->s, lists, lists + n1, nsmall, nbig, c.comp_nseg, cb, g->st);
+    launch_compact_apply(g->s, lists, lists + n1, lists + 2 * n1, nsmall, nbig, nzs, nzb, c.comp_nseg, cb, g->st);
     phase_end(g, PH_COMPACT);
     HIPCHK(hipGetLastError());
-    if (nodes_out) *nodes_out = (uint64_t)nsmall + nbig;
+    if (nodes_out) *nodes_out = (uint64_t)nsmall + nbig + nzs + nzb;
     if (slots_out) *slots_out = c.comp_freed;
     return BINGO_OK;
 }

@@ -48,7 +48,7 @@ enum Scratch {
     SC_AUX3,
     SC_BIG,   // delete batch: long runs
     SC_FRAC,  // insert batch with float weights: fractions in node order
-    SC_CLIST, // compaction: the listed nodes (two lists of up to n + 1)
+    SC_CLIST, // compaction: the listed nodes (three lists of up to n + 1)
     SC_CBIG,  // compaction: big nodes' segment tables and staging
     SC_WCLS,  // insert batch: each plan workgroup's class index ranges
     SC_PF,    // insert batch: the regions it frees

@@ -1082,7 +1082,7 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, con
             const uint32_t wi = rl32(x.w, (int)i), vi = rl32((uint32_t)x.v, (int)i), bi = rl32(my_boff, (int)i);
             if (ln < kGroups && ((wi >> ln) & 1u)) {
                 if ((sm >> ln) & 1u) {  // a dense group only counts
-                    s.mem[goff + szj] = ((uint64_t)vi << 32) | (n0 + i);
+                    s.mem[goff + szj] = ((uint64_t)vi << 32) | (n0 + i + h.sbase);
                     back[bi + __popc(wi & sm & ((1u << ln) - 1u))] = szj;
                 }
                 ++szj;
@@ -1111,7 +1111,7 @@ __device__ void insert_run(const DevStore &s, const DevEdge *__restrict__ e, con
             }
             const uint64_t lt = (1ull << ln) - 1ull;
             uint32_t rank = 0;
-            const uint64_t entry = ((uint64_t)(uint32_t)x.v << 32) | slot;
+            const uint64_t entry = ((uint64_t)(uint32_t)x.v << 32) | (slot + h.sbase);
             for (uint32_t b = dpp_or32(x.w); b;) {
                 const int j = __ffs(b) - 1;
                 b &= b - 1u;
@@ -1347,7 +1347,7 @@ __device__ void delete_run(const DevStore &s, const uint32_t *__restrict__ slots
             uint32_t mb[kDB], mw[kDB];
 #pragma unroll
             for (int q = 0; q < kDB; ++q) {
-                const uint32_t ms = (uint32_t)mv[q];
+                const uint32_t ms = (uint32_t)mv[q] - h.sbase;
                 mb[q] = rk[q] < nsurv ? B[ms] : 0u;
                 mw[q] = rk[q] < nsurv ? W[ms] : 0u;
                 if (rk[q] < nsurv) m[pos[q]] = mv[q];
@@ -1367,7 +1367,10 @@ __device__ void delete_run(const DevStore &s, const uint32_t *__restrict__ slots
     }
     if (bad) atomicOr(&s.ctl->error, (unsigned)ERR_OVERRUN);
     if (kj) s.grp[(size_t)u * kGroups + ln].size = S2;
-    if (ln == 0) s.hdr[u].nlive = h.nlive - cnt;
+    if (ln == 0) {
+        s.hdr[u].nlive = h.nlive - cnt;
+        s.hdr[u].dend = max(h.dend, slots[cnt - 1] + 1u);  // a run's slots ascend
+    }
     const uint64_t F = s.fw ? s.fnode[u].F - dpp_sum64(fsub) : 0ull;
     alias_rebuild(s, u, own ? S2 : 0u, gh.off, s.nbr ? make_hmeta(h.hash_off, h.hash_cap) : 0ull, F, dm, ln);
 }
@@ -1418,13 +1421,13 @@ __device__ void delete_small(const DevStore &s, uint32_t cnt, int32_t u, const R
     const uint32_t S2 = S - kj;
     // slot records of the tail [S2, S), read together with the positions
     // above: the survivors among them are the members that move
-    uint32_t eb[K], ew[K];
+    uint32_t eb[K], ew[K], es[K];  // es: the tail members' slots
 #pragma unroll
     for (int i = 0; i < K; ++i) {
         const bool cand = (uint32_t)i < c && (uint32_t)i + kj >= c;
-        const uint32_t ms = (uint32_t)e[i];
-        eb[i] = cand ? B[ms] : 0u;
-        ew[i] = cand ? W[ms] : 0u;
+        es[i] = (uint32_t)e[i] - h.sbase;
+        eb[i] = cand ? B[es[i]] : 0u;
+        ew[i] = cand ? W[es[i]] : 0u;
     }
 
     // survivors: tail members whose slot is not one of this group's deletes
@@ -1434,7 +1437,7 @@ __device__ void delete_small(const DevStore &s, uint32_t cnt, int32_t u, const R
     for (int i = 0; i < K; ++i) {
         bool del = false;
 #pragma unroll
-        for (int q = 0; q < K; ++q) del |= (wq[q] & lbit) && (uint32_t)e[i] == sq[q];
+        for (int q = 0; q < K; ++q) del |= (wq[q] & lbit) && es[i] == sq[q];
         surv[i] = (uint32_t)i < c && (uint32_t)i + kj >= c && !del;
         srank[i] = ns;
         ns += surv[i] ? 1u : 0u;
@@ -1475,7 +1478,10 @@ __device__ void delete_small(const DevStore &s, uint32_t cnt, int32_t u, const R
     if (bad) atomicOr(&s.ctl->error, (unsigned)ERR_OVERRUN);
     if (kj) s.grp[(size_t)u * kGroups + ln].size = S2;
     uint64_t fsub = 0;
-    if (ln == 0) s.hdr[u].nlive = h.nlive - cnt;
+    if (ln == 0) {
+        s.hdr[u].nlive = h.nlive - cnt;
+        s.hdr[u].dend = max(h.dend, rl32(t_sl, (int)cnt - 1) + 1u);  // a run's slots ascend
+    }
     if (in) {
         W[t_sl] = 0u;
         if (s.fw) {
@@ -1552,7 +1558,7 @@ __device__ void delete_group(const DevStore &s, const uint32_t *__restrict__ slo
         bad |= hole && !ok;
         const uint64_t mv = ok ? m[S2 + rank] : 0ull;
         if (ok) m[pos] = mv;
-        const uint32_t ms = (uint32_t)mv;
+        const uint32_t ms = (uint32_t)mv - h.sbase;
         const uint32_t mb = ok ? B[ms] : 0u, mw = ok ? W[ms] : 0u;
         if (ok) back[mb + __popc(mw & below)] = pos;
         hr += (uint32_t)__popcll(hb);
@@ -1579,7 +1585,10 @@ __device__ void delete_finish(const DevStore &s, const uint32_t *__restrict__ sl
             FR[sl] = 0u;
         }
     }
-    if (ln == 0) s.hdr[u].nlive = h.nlive - cnt;
+    if (ln == 0) {
+        s.hdr[u].nlive = h.nlive - cnt;
+        s.hdr[u].dend = max(h.dend, slots[cnt - 1] + 1u);  // a run's slots ascend
+    }
     GroupHdr gh = {0, 0, 0};
     if (ln < kGroups) gh = s.grp[(size_t)u * kGroups + ln];
     const uint64_t F = s.fw ? s.fnode[u].F - dpp_sum64(fsub) : 0ull;
@@ -1712,7 +1721,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 __global__ void k_compact_reset(Ctl *ctl)
 {
     if (threadIdx.x == 0) {
-        ctl->ncomp[0] = ctl->ncomp[1] = 0;
+        ctl->ncomp[0] = ctl->ncomp[1] = ctl->ncomp[2] = ctl->ncomp[3] = 0;
         ctl->comp_freed = 0;
         ctl->comp_bigslots = ctl->comp_nseg = 0;
     }
@@ -1760,16 +1769,20 @@ struct Team {
 // 2^25 nodes).
 constexpr uint32_t kSelPer = 8;
 __global__ __launch_bounds__(256) void k_compact_select(DevStore s, uint32_t thr16, uint32_t *__restrict__ small_list,
-                                                        uint32_t *__restrict__ big_list)
+                                                        uint32_t *__restrict__ big_list,
+                                                        uint32_t *__restrict__ shift_list)
 {
     __shared__ uint32_t part[4];
-    __shared__ uint32_t base[2];
+    __shared__ uint32_t base[4];
     const Team<256> tm{part, (int)threadIdx.x};
     const int64_t n1 = (int64_t)s.n + 1;
     const int64_t tile = 256 * kSelPer;
     uint64_t freed = 0, bslots = 0, bseg = 0;
     for (int64_t t0 = (int64_t)blockIdx.x * tile; t0 < n1; t0 += (int64_t)gridDim.x * tile) {
-        uint32_t ps = 0, pb = 0;  // this thread's picks (bit k: node t0 + k 256 + tid)
+        // this thread's picks (bit k: node t0 + k 256 + tid): rebuilt on the
+        // wave path (ps) or in segments (pb); a dead prefix dropped by a wave
+        // (pz) or a workgroup (py)
+        uint32_t ps = 0, pb = 0, pz = 0, py = 0;
 #pragma unroll
         for (uint32_t k = 0; k < kSelPer; ++k) {
             const int64_t u = t0 + k * 256 + threadIdx.x;
@@ -1778,7 +1791,10 @@ __global__ __launch_bounds__(256) void k_compact_select(DevStore s, uint32_t thr
             const uint32_t dead = h.nslots - h.nlive;
             if (!(dead >= 1u && (uint64_t)dead * 65536u >= (uint64_t)thr16 * h.nslots)) continue;
             freed += dead;
-            if (h.nslots > kBigCompact) {
+            const bool big = h.nslots > kBigCompact;
+            if (h.dend == dead && (uint64_t)h.sbase + h.nslots <= kSbaseMax) {
+                (big ? py : pz) |= 1u << k;
+            } else if (big) {
                 pb |= 1u << k;
                 bslots += h.nslots;
                 bseg += (h.nslots + kCompactSeg - 1) / kCompactSeg;
@@ -1786,27 +1802,34 @@ __global__ __launch_bounds__(256) void k_compact_select(DevStore s, uint32_t thr
                 ps |= 1u << k;
             }
         }
-        // the wave-path list in node order (one scan per k), so that a run
-        // of list entries is a run of neighbouring headers and regions
-        uint32_t es[kSelPer], tk[kSelPer], ts = 0, tb;
+        // the wave-path lists in node order (one scan per k, both lists in
+        // one word), so that a run of list entries is a run of neighbouring
+        // headers and regions; the workgroup-path ones in thread order
+        uint32_t es[kSelPer], tk[kSelPer], ts = 0, tz = 0, tbz;
 #pragma unroll
         for (uint32_t k = 0; k < kSelPer; ++k) {
-            es[k] = tm.excl((ps >> k) & 1u, tk[k]);
-            ts += tk[k];
+            es[k] = tm.excl(((ps >> k) & 1u) | ((pz >> k) & 1u) << 16, tk[k]);
+            ts += tk[k] & 0xFFFFu;
+            tz += tk[k] >> 16;
         }
-        const uint32_t eb = tm.excl((uint32_t)__popc(pb), tb);
+        const uint32_t ebz = tm.excl((uint32_t)__popc(pb) | (uint32_t)__popc(py) << 16, tbz);
         if (threadIdx.x == 0) {
             base[0] = ts ? atomicAdd(&s.ctl->ncomp[0], ts) : 0u;
-            base[1] = tb ? atomicAdd(&s.ctl->ncomp[1], tb) : 0u;
+            base[1] = (tbz & 0xFFFFu) ? atomicAdd(&s.ctl->ncomp[1], tbz & 0xFFFFu) : 0u;
+            base[2] = tz ? atomicAdd(&s.ctl->ncomp[2], tz) : 0u;
+            base[3] = (tbz >> 16) ? atomicAdd(&s.ctl->ncomp[3], tbz >> 16) : 0u;
         }
         __syncthreads();
-        uint32_t os = base[0], ob = base[1] + eb;
+        uint32_t os = base[0], oz = base[2], ob = base[1] + (ebz & 0xFFFFu), oy = base[3] + (ebz >> 16);
 #pragma unroll
         for (uint32_t k = 0; k < kSelPer; ++k) {
             const uint32_t u = (uint32_t)(t0 + k * 256 + threadIdx.x);
-            if (ps & (1u << k)) small_list[os + es[k]] = u;
-            os += tk[k];
+            if (ps & (1u << k)) small_list[os + (es[k] & 0xFFFFu)] = u;
+            if (pz & (1u << k)) shift_list[oz + (es[k] >> 16)] = u;
+            os += tk[k] & 0xFFFFu;
+            oz += tk[k] >> 16;
             if (pb & (1u << k)) big_list[ob++] = u;
+            if (py & (1u << k)) shift_list[n1 - 1 - oy++] = u;  // from the list's end
         }
         __syncthreads();  // base is the next tile's
     }
@@ -1937,6 +1960,8 @@ __device__ __forceinline__ void compact_finish(const DevStore &s, int32_t u, con
         nh.nslots = nid;
         nh.nlive = nid;
         nh.back_used = nb;
+        nh.sbase = 0;  // member words were rewritten with the new slot numbers
+        nh.dend = 0;
         s.hdr[u] = nh;
     }
 }
@@ -2035,14 +2060,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     }
 }
 
-// neighbour tables of the compacted small nodes (stores with them), one wave
-// per node, from the live slots: counts and weight sums are unchanged, keys
-// whose count fell to zero go
-__global__ __launch_bounds__(256) void k_compact_hash(DevStore s, const uint32_t *__restrict__ list)
+// neighbour tables of compacted nodes (stores with them), one wave per node,
+// from the live slots: counts and weight sums are unchanged, keys whose
+// count fell to zero go.  list[0..ncomp[c]), or with from_end the last
+// ncomp[c] entries of a list of n1.
+__global__ __launch_bounds__(256) void k_compact_hash(DevStore s, const uint32_t *__restrict__ list, int c,
+                                                      int from_end, uint32_t n1)
 {
     const int ln = lane_id();
     const Team<64> tm{nullptr, ln};
-    const uint32_t cnt = s.ctl->ncomp[0];
+    const uint32_t cnt = s.ctl->ncomp[c];
+    if (from_end) list += n1 - cnt;
     const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     for (uint32_t k = wave; k < cnt; k += nwaves) {
@@ -2058,6 +2086,119 @@ __global__ __launch_bounds__(256) void k_compact_hash(DevStore s, const uint32_t
         for (uint32_t i = (uint32_t)ln; i < h.nslots; i += 64) fresh += hash_add(tab, h.hash_cap, V[i], 1u, V[h.slot_cap + i]);
         tm.excl(fresh, hused);
         if (ln == 0) s.hdr[u].hash_used = hused;
+    }
+}
+
+// ---- nodes whose dead slots are their first d ones (a sliding window's
+// expiries): the live records move down by d (back offsets lowered by D,
+// the first live slot's) and the back entries down by D; member words keep
+// their place and their slot fields, the node's sbase grows by d.  Moving
+// down in place is safe as in rebuild_chunk: a run of chunks is read before
+// any of it is written, and later runs read above every position written.
+constexpr int kShiftRun = 4;
+
+// [c0, min(c0 + 64 kShiftRun, n)) of a node's records or back entries moved
+// from index + d to index by nt threads (t: this thread), lowering B values
+// by D
+template <int NT>
+__device__ __forceinline__ void shift_records(const DevStore &s, const NodeHdr &h, uint32_t d, uint32_t D, uint32_t n,
+                                              int t)
+{
+    uint32_t *V = s.slot + h.slot_off;
+    for (uint32_t c0 = 0; c0 < n; c0 += NT * kShiftRun) {
+        uint32_t v[kShiftRun], w[kShiftRun], b[kShiftRun], f[kShiftRun];
+#pragma unroll
+        for (int r = 0; r < kShiftRun; ++r) {
+            const uint32_t i = c0 + (uint32_t)(r * NT + t), k = (i < n ? i : 0u) + d;  // (n >= 1)
+            v[r] = V[k];
+            w[r] = V[h.slot_cap + k];
+            b[r] = V[2ull * h.slot_cap + k];
+            f[r] = s.fw ? V[3ull * h.slot_cap + k] : 0u;
+        }
+        if (NT > 64) __syncthreads();  // every wave's reads before any wave's writes
+#pragma unroll
+        for (int r = 0; r < kShiftRun; ++r) {
+            const uint32_t i = c0 + (uint32_t)(r * NT + t);
+            if (i < n) {
+                V[i] = v[r];
+                V[h.slot_cap + i] = w[r];
+                V[2ull * h.slot_cap + i] = b[r] - D;
+                if (s.fw) V[3ull * h.slot_cap + i] = f[r];
+            }
+        }
+    }
+}
+template <int NT>
+__device__ __forceinline__ void shift_back(const DevStore &s, const NodeHdr &h, uint32_t D, uint32_t n, int t)
+{
+    uint32_t *back = s.back + h.back_off;
+    for (uint32_t c0 = 0; c0 < n; c0 += NT * kShiftRun * 2) {
+        uint32_t x[2 * kShiftRun];
+#pragma unroll
+        for (int r = 0; r < 2 * kShiftRun; ++r) {
+            const uint32_t i = c0 + (uint32_t)(r * NT + t);
+            x[r] = back[(i < n ? i : 0u) + D];
+        }
+        if (NT > 64) __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 2 * kShiftRun; ++r) {
+            const uint32_t i = c0 + (uint32_t)(r * NT + t);
+            if (i < n) back[i] = x[r];
+        }
+    }
+}
+
+// the node's header after the shift
+__device__ __forceinline__ void shift_header(const DevStore &s, int32_t u, NodeHdr h, uint32_t d, uint32_t D)
+{
+    h.nslots -= d;
+    h.back_used -= D;
+    h.sbase += d;
+    h.dend = 0;
+    s.hdr[u] = h;
+}
+
+// nodes of at most kBigCompact slots (shift_list[0..ncomp[2])), one wave
+// each; batches of 64 list entries with their headers read at once
+__global__ __launch_bounds__(256) void k_compact_shift(DevStore s, const uint32_t *__restrict__ list)
+{
+    const int ln = lane_id();
+    const uint64_t cnt = s.ctl->ncomp[2];
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const uint64_t step = (uint64_t)((gridDim.x * blockDim.x) >> 6) * 64;
+    for (uint64_t k0 = (uint64_t)wave * 64; k0 < cnt; k0 += step) {
+        uint32_t uk = k0 + ln < cnt ? list[k0 + ln] : 0u;
+        NodeHdr hk{};
+        uint32_t Dk = 0;
+        if (k0 + ln < cnt) {
+            hk = s.hdr[uk];
+            const uint32_t d = hk.nslots - hk.nlive;
+            Dk = hk.nlive ? s.slot[hk.slot_off + 2ull * hk.slot_cap + d] : hk.back_used;
+        }
+        const int nq = (int)min<uint64_t>(64, cnt - k0);
+        for (int q = 0; q < nq; ++q) {
+            const NodeHdr h = hdr_of_lane(hk, q);
+            const uint32_t d = h.nslots - h.nlive, D = rl32(Dk, q);
+            if (h.nlive) shift_records<64>(s, h, d, D, h.nlive, ln);
+            if (h.back_used > D) shift_back<64>(s, h, D, h.back_used - D, ln);
+            if (ln == 0) shift_header(s, (int32_t)rl32(uk, q), h, d, D);
+        }
+    }
+}
+
+// larger nodes (the last ncomp[3] entries of shift_list), a workgroup each
+__global__ __launch_bounds__(256) void k_compact_shift_big(DevStore s, const uint32_t *__restrict__ list, uint32_t n1)
+{
+    const uint32_t cnt = s.ctl->ncomp[3];
+    for (uint32_t k = blockIdx.x; k < cnt; k += gridDim.x) {
+        const int32_t u = (int32_t)list[n1 - cnt + k];
+        const NodeHdr h = s.hdr[u];
+        const uint32_t d = h.nslots - h.nlive;
+        const uint32_t D = h.nlive ? s.slot[h.slot_off + 2ull * h.slot_cap + d] : h.back_used;
+        if (h.nlive) shift_records<256>(s, h, d, D, h.nlive, (int)threadIdx.x);
+        if (h.back_used > D) shift_back<256>(s, h, D, h.back_used - D, (int)threadIdx.x);
+        if (threadIdx.x == 0) shift_header(s, u, h, d, D);
+        __syncthreads();  // the header read of the next node by every thread after this one's write
     }
 }
 
@@ -2271,6 +2412,8 @@ __global__ __launch_bounds__(256) void k_compact_bigfinish(DevStore s, const uin
             nh.nlive = nid;
             nh.back_used = cb.nback[i];
             nh.hash_used = hused;
+            nh.sbase = 0;
+            nh.dend = 0;
             s.hdr[u] = nh;
         }
         __syncthreads();
@@ -3097,7 +3240,7 @@ __global__ __launch_bounds__(256) void k_export_b(DevStore s, const uint64_t *sl
     for (int j = 0; j < kGroups; ++j) {  // member lists (dense groups keep none)
         if ((h.dmask >> j) & 1u) continue;
         const GroupHdr g = s.grp[u * kGroups + j];
-        for (uint32_t i = ln; i < g.size; i += 64) members[mo + i] = s.mem[g.off + i];
+        for (uint32_t i = ln; i < g.size; i += 64) members[mo + i] = s.mem[g.off + i] - h.sbase;  // slot fields
         mo += g.size;
     }
     uint64_t bo = mem_out_off[u];
@@ -3576,7 +3719,8 @@ void launch_export_b(DevStore s, const uint64_t *slot_out_off, const uint64_t *m
                        slot_v, slot_w, members, backpos, slot_f);
 }
 
-void launch_compact_select(DevStore s, uint32_t thr16, uint32_t *small_list, uint32_t *big_list, hipStream_t st)
+void launch_compact_select(DevStore s, uint32_t thr16, uint32_t *small_list, uint32_t *big_list, uint32_t *shift_list,
+                           hipStream_t st)
 {
     hipLaunchKernelGGL(k_compact_reset, dim3(1), dim3(64), 0, st, s.ctl);
     const int64_t n1 = (int64_t)s.n + 1;
@@ -3584,19 +3728,32 @@ void launch_compact_select(DevStore s, uint32_t thr16, uint32_t *small_list, uin
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     const unsigned gb = (unsigned)std::min<int64_t>((n1 + 256 * kSelPer - 1) / (256 * kSelPer), (int64_t)ncu * 8);
-    hipLaunchKernelGGL(k_compact_select, dim3(gb), dim3(256), 0, st, s, thr16, small_list, big_list);
+    hipLaunchKernelGGL(k_compact_select, dim3(gb), dim3(256), 0, st, s, thr16, small_list, big_list, shift_list);
 }
 
-void launch_compact_apply(DevStore s, const uint32_t *small_list, const uint32_t *big_list, uint32_t nsmall,
-                          uint32_t nbig, uint64_t nseg, const CompactBig &cb, hipStream_t st)
+void launch_compact_apply(DevStore s, const uint32_t *small_list, const uint32_t *big_list, const uint32_t *shift_list,
+                          uint32_t nsmall, uint32_t nbig, uint32_t nzs, uint32_t nzb, uint64_t nseg,
+                          const CompactBig &cb, hipStream_t st)
 {
     int dev = 0, ncu = 1;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    const uint32_t n1 = (uint32_t)s.n + 1;
+    if (nzs) {
+        const unsigned zb = (unsigned)std::min<int64_t>((nzs + 255) / 256, (int64_t)ncu * 8);
+        hipLaunchKernelGGL(k_compact_shift, dim3(zb), dim3(256), 0, st, s, shift_list);
+        if (s.nbr) hipLaunchKernelGGL(k_compact_hash, dim3(zb), dim3(256), 0, st, s, shift_list, 2, 0, n1);
+    }
+    if (nzb) {
+        hipLaunchKernelGGL(k_compact_shift_big, dim3(std::min<uint32_t>(nzb, (uint32_t)ncu * 4)), dim3(256), 0, st, s,
+                           shift_list, n1);
+        if (s.nbr)
+            hipLaunchKernelGGL(k_compact_hash, dim3((nzb + 3) / 4), dim3(256), 0, st, s, shift_list, 3, 1, n1);
+    }
     if (nsmall) {
         const unsigned sb = (unsigned)std::min<int64_t>((nsmall + 255) / 256, (int64_t)ncu * 8);
         hipLaunchKernelGGL(k_compact_small, dim3(sb), dim3(256), 0, st, s, small_list);
-        if (s.nbr) hipLaunchKernelGGL(k_compact_hash, dim3(sb), dim3(256), 0, st, s, small_list);
+        if (s.nbr) hipLaunchKernelGGL(k_compact_hash, dim3(sb), dim3(256), 0, st, s, small_list, 0, 0, n1);
     }
     if (!nbig) return;
     const unsigned gw = (unsigned)std::min<uint64_t>((nseg + 3) / 4, (uint64_t)ncu * 8);  // a wave per segment

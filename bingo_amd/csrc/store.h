@@ -42,7 +42,10 @@ struct NodeHdr {
     uint32_t hash_cap, hash_used;
     uint32_t nlive;              // live slots (nslots - nlive tombstones, reclaimed by compaction)
     uint32_t dmask;              // dense bit groups (count only, no member list; DevStore::rate16)
-    uint64_t spare2;
+    uint32_t sbase;              // a member word's slot field is its slot + sbase: a compaction that
+                                 // drops a dead prefix of d slots adds d here instead of rewriting them
+    uint32_t dend;               // one past the highest slot deleted since the last compaction: the
+                                 // dead slots are exactly the first ones iff dend = nslots - nlive
 };
 
 struct GroupHdr {
@@ -117,7 +120,8 @@ struct Ctl {
     unsigned int nbigcopy;           // insert plan: runs that move a region of kBigCopyBytes or more
     unsigned int werr;               // walks since the last walk reset: ERR_INVAL (a start or length out of range)
     unsigned long long taskctr;      // insert apply: region moves left to k_copy_tasks << 32 | their chunks
-    unsigned int ncomp[2];           // compaction: nodes listed for the wave path and the workgroup path
+    unsigned int ncomp[4];           // compaction: nodes listed to be rebuilt (wave path, segment path) and to
+                                     // drop a dead prefix (wave path, workgroup path)
     unsigned long long comp_freed;   // compaction: tombstoned slots reclaimed
     unsigned long long comp_bigslots;  // compaction: slots of the listed big nodes
     unsigned long long comp_nseg;      // compaction: segments of the listed big nodes
@@ -359,6 +363,10 @@ void launch_export_b(DevStore s, const uint64_t *slot_out_off, const uint64_t *m
 // segments of kCompactSeg slots spread over the grid (a wave each), their
 // records staged through the scratch.
 constexpr uint32_t kBigCompact = 8192;
+// a dead prefix is dropped without rewriting member words only while the
+// slot fields stay below this (they must stay clear of the delete kernels'
+// kMark bit); past it the node is rebuilt and sbase returns to 0
+constexpr uint64_t kSbaseMax = 1ull << 30;
 constexpr uint32_t kCompactSeg = 2048;
 struct CompactBig {
     uint32_t *segbase;  // [nbig + 1] first segment of each big node
@@ -370,9 +378,14 @@ struct CompactBig {
     uint32_t *nback;    // [nbig] back entries after compaction
     uint4 *rec;         // staged records {v, w, fraction, back offset} (comp_bigslots)
 };
-void launch_compact_select(DevStore s, uint32_t thr16, uint32_t *small_list, uint32_t *big_list, hipStream_t st);
-void launch_compact_apply(DevStore s, const uint32_t *small_list, const uint32_t *big_list, uint32_t nsmall,
-                          uint32_t nbig, uint64_t nseg, const CompactBig &cb, hipStream_t st);
+// shift_list (n + 1 entries) holds the nodes whose dead slots are their
+// first ones: those of at most kBigCompact slots from its start, larger ones
+// from its end (Ctl::ncomp[2], ncomp[3])
+void launch_compact_select(DevStore s, uint32_t thr16, uint32_t *small_list, uint32_t *big_list, uint32_t *shift_list,
+                           hipStream_t st);
+void launch_compact_apply(DevStore s, const uint32_t *small_list, const uint32_t *big_list, const uint32_t *shift_list,
+                          uint32_t nsmall, uint32_t nbig, uint32_t nzs, uint32_t nzb, uint64_t nseg,
+                          const CompactBig &cb, hipStream_t st);
 void launch_edge_count(DevStore s, const int32_t *us, const int32_t *vs, int64_t k, uint32_t *counts,
                        hipStream_t st);
 

@@ -21,9 +21,10 @@
  *   dense groups ..... class rule restated from each node's first batch,
  *                      brute-force store, exact law of a rejection draw by
  *                      enumeration, chi-square, the trial fallback
- *   compaction ....... brute-force drop of the dead slots; member lists equal
- *                      the live slots with bit j in slot order; the one-draw
- *                      law moves by less than 2^-26
+ *   compaction ....... brute-force drop of the dead slots; a dead prefix
+ *                      keeps the lists' order (slot fields lowered), other
+ *                      nodes' lists equal the live slots with bit j in slot
+ *                      order; the one-draw law moves by less than 2^-26
  */
 #include "bingo_oracle.h"
 #include "philox.h"
@@ -499,11 +500,14 @@ int orc_delete(orc_graph *g, const orc_del *d, int64_t k)
 /* Compaction of tombstoned slots (the reference never reclaims them,
  * operation.cuh:209-210).  Node u is compacted when dead = nslots - live >= 1
  * and dead * 2^16 >= floor(min_dead_frac * 2^16) * nslots: its live slots
- * keep their order and take the numbers 0..live-1, and each of its listed
- * groups' member lists is rebuilt in slot order (the k-th member of group j
- * is the k-th live slot with bit j), with the back entries to match.  Masses
- * and the alias table do not change.  Returns the number of nodes compacted
- * (ORC_EINVAL for a fraction outside [0, 1]). */
+ * keep their order and take the numbers 0..live-1.  When its dead slots are
+ * exactly its first `dead` (a sliding window's expiries), the live slots
+ * move down by dead with their back entries, and the member lists keep their
+ * order, each member's slot field lowered by dead.  Otherwise each of its
+ * listed groups' member lists is rebuilt in slot order (the k-th member of
+ * group j is the k-th live slot with bit j), with the back entries to match.
+ * Masses and the alias table do not change.  Returns the number of nodes
+ * compacted (ORC_EINVAL for a fraction outside [0, 1]). */
 int64_t orc_compact(orc_graph *g, double min_dead_frac, uint64_t *slots_freed)
 {
     if (!(min_dead_frac >= 0 && min_dead_frac <= 1)) return ORC_EINVAL;
@@ -513,27 +517,44 @@ int64_t orc_compact(orc_graph *g, double min_dead_frac, uint64_t *slots_freed)
 #pragma omp parallel for schedule(dynamic, 256) reduction(+ : nodes, freed)
     for (int32_t u = 1; u <= g->n; ++u) {
         node_t *x = &g->nodes[u];
-        uint32_t live = 0;
+        uint32_t live = 0, lead = 0;
         for (uint32_t s = 0; s < x->nslots; ++s) live += (x->w[s] || (x->fr && x->fr[s])) ? 1u : 0u;
+        while (lead < x->nslots && !x->w[lead] && !(x->fr && x->fr[lead])) ++lead;
         const uint32_t dead = x->nslots - live;
         if (!dead || (uint64_t)dead * 65536u < thr16 * x->nslots) continue;
-        uint32_t ns = 0, nb = 0, pos[32] = {0};
-        for (uint32_t s = 0; s < x->nslots; ++s) {
-            const uint32_t w = x->w[s];
-            if (!w && !(x->fr && x->fr[s])) continue;
-            x->v[ns] = x->v[s];
-            x->w[ns] = w;
-            if (x->fr) x->fr[ns] = x->fr[s];
-            x->boff[ns] = nb;
-            for (uint32_t bits = w & ~x->dmask; bits; bits &= bits - 1) {  /* ns <= s, nb <= old offset */
-                const int j = __builtin_ctz(bits);
-                x->mem[j][pos[j]] = ((uint64_t)x->v[ns] << 32) | ns;
-                x->back[nb++] = pos[j]++;
+        if (lead == dead) {  /* a dead prefix */
+            const uint32_t D = lead < x->nslots ? x->boff[lead] : x->back_used;
+            for (uint32_t s = lead; s < x->nslots; ++s) {
+                x->v[s - lead] = x->v[s];
+                x->w[s - lead] = x->w[s];
+                if (x->fr) x->fr[s - lead] = x->fr[s];
+                x->boff[s - lead] = x->boff[s] - D;
             }
-            ++ns;
+            if (x->back_used > D) memmove(x->back, x->back + D, (size_t)(x->back_used - D) * sizeof(uint32_t));
+            x->back_used -= D;
+            for (int j = 0; j < ORC_GROUPS; ++j)
+                if (!((x->dmask >> j) & 1u))
+                    for (uint32_t k = 0; k < x->size[j]; ++k) x->mem[j][k] -= lead;
+            x->nslots -= lead;
+        } else {
+            uint32_t ns = 0, nb = 0, pos[32] = {0};
+            for (uint32_t s = 0; s < x->nslots; ++s) {
+                const uint32_t w = x->w[s];
+                if (!w && !(x->fr && x->fr[s])) continue;
+                x->v[ns] = x->v[s];
+                x->w[ns] = w;
+                if (x->fr) x->fr[ns] = x->fr[s];
+                x->boff[ns] = nb;
+                for (uint32_t bits = w & ~x->dmask; bits; bits &= bits - 1) {  /* ns <= s, nb <= old offset */
+                    const int j = __builtin_ctz(bits);
+                    x->mem[j][pos[j]] = ((uint64_t)x->v[ns] << 32) | ns;
+                    x->back[nb++] = pos[j]++;
+                }
+                ++ns;
+            }
+            x->nslots = ns;
+            x->back_used = nb;
         }
-        x->nslots = ns;
-        x->back_used = nb;
         nodes += 1;
         freed += dead;
     }

@@ -114,8 +114,9 @@ int orc_delete(orc_graph *g, const orc_del *d, int64_t k);
 
 /* compaction of tombstoned slots (mirror of the product's bingo_compact):
  * nodes with dead >= 1 and dead 2^16 >= floor(min_dead_frac 2^16) nslots
- * renumber their live slots by rank and rebuild their member lists in slot
- * order; returns the number of nodes compacted */
+ * renumber their live slots by rank; a node whose dead slots are its first
+ * ones keeps its member lists' order, any other rebuilds them in slot order;
+ * returns the number of nodes compacted */
 int64_t orc_compact(orc_graph *g, double min_dead_frac, uint64_t *slots_freed);
 
 /* walks: paths is [nwalkers][params->max_len] int32, -1 padded.

@@ -4,10 +4,9 @@ weights) loaded into a store fitted to the graph, one sliding-window step
 properties of the whole store against numpy, alias tables of sampled nodes
 against the oracle's construction, walk steps out of probe nodes checked
 against their live arcs, and the hub's transition law by chi-square; then a
-compaction of every node with a tombstone, after which slot counts equal
-live counts, masses and buckets are unchanged, and the hub's draws equal the
-oracle's on its live arcs in slot order (the member order compaction
-rebuilds).
+compaction of every node with a tombstone (each one's oldest slots: a dead
+prefix), after which slot counts equal live counts and masses, buckets and
+the hub's draws are unchanged.
 
 Needs about 215 GB of HBM, 60 GB of host memory and about 90 s."""
 import ctypes as C
@@ -122,15 +121,7 @@ def test_scale_store_window_and_walks(orc):
     assert np.array_equal(st2["nslots"], np.bincount(lu, minlength=n + 1).astype(np.uint32))
     for f in ("mass", "bsize", "T", "alias"):
         assert np.array_equal(st2[f], st[f]), f
-    # the hub's member lists are rebuilt in slot order: its draws equal those
-    # of the oracle holding only its live arcs, loaded in slot order (each
-    # target renamed to its arc's rank; the walk stream depends on walker
-    # and step, not on node ids)
+    # every node's dead slots are its oldest, a prefix: compaction drops them
+    # without touching member lists, so the hub's draws are the same
     p3, _ = g.walk(B.SAMPLING, B.walk_params(seed=11, max_len=2), starts=np.full(N, hub, np.int32))
-    L = len(tgt)
-    e = np.zeros(L, orc.EDGE_DT)
-    e["u"], e["v"], e["w"] = 1, np.arange(2, L + 2), lw[hv]
-    o = orc.OracleGraph(L + 1)
-    assert o.load(e) == 0
-    po, _ = o.walk(orc.SAMPLING, orc.walk_params(seed=11, max_len=2), starts=np.ones(N, np.int32))
-    assert np.array_equal(p3[:, 1], tgt[po[:, 1] - 2].astype(np.int32))
+    assert np.array_equal(p3, p2)

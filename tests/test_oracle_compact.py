@@ -1,6 +1,7 @@
 """Compaction of tombstoned slots, on the CPU: the oracle against a brute-force
 restatement (drop the dead slots of the selected nodes, keep the order), the
-member lists rebuilt in slot order, the selection rule, and the update
+member lists rebuilt in slot order or, after a dead prefix, kept with slot
+numbers lowered, the selection rule, and the update
 generator's renumbering against the oracle's store."""
 import random
 
@@ -54,10 +55,12 @@ def test_compaction_matches_brute_force(orc, seed):
 
 
 def test_compaction_rebuilds_lists_in_slot_order(orc):
-    """A compacted node's member lists are its live slots with bit j in slot
-    order, under the new slot numbers; nodes left alone keep their lists
-    entry for entry.  The one-draw law of a node moves by at most the
-    multiply-shift's 2^-32 steps."""
+    """A compacted node whose dead slots are not its first ones gets member
+    lists that are its live slots with bit j in slot order, under the new
+    slot numbers; a dead prefix keeps the lists' order with slot fields
+    lowered by its length; nodes left alone keep their lists entry for
+    entry.  The one-draw law of a node moves by at most the multiply-shift's
+    2^-32 steps."""
     from brute import transition_law
     rng = random.Random(7)
     n = 6
@@ -73,6 +76,8 @@ def test_compaction_rebuilds_lists_in_slot_order(orc):
     d0 = g.export()
     before = split_dump(d0, n)
     compacted = {u for u in range(1, n + 1) if any(w == 0 for v, w in bg.slots[u])}
+    lead = {u: next((s for s, (v, w) in enumerate(bg.slots[u]) if w), len(bg.slots[u])) for u in compacted}
+    prefix = {u for u in compacted if lead[u] == sum(1 for v, w in bg.slots[u] if not w)}
     law0 = {u: transition_law(d0["T"][u], d0["alias"][u], before[u][2])
             for u in range(1, n + 1) if int(d0["mass"][u])}
     assert 6 not in compacted and len(compacted) >= 3
@@ -84,7 +89,9 @@ def test_compaction_rebuilds_lists_in_slot_order(orc):
     for u in range(1, n + 1):
         for j in range(32):
             A0 = [int(x) for x in after[u][2][j]]
-            if u in compacted:
+            if u in prefix:
+                want = [int(x) - lead[u] for x in before[u][2][j]]
+            elif u in compacted:
                 want = [(v << 32) | s for s, (v, w) in enumerate(bg.slots[u]) if w >> j & 1]
             else:
                 want = [int(x) for x in before[u][2][j]]
@@ -93,8 +100,49 @@ def test_compaction_rebuilds_lists_in_slot_order(orc):
             law1 = transition_law(d1["T"][u], d1["alias"][u], after[u][2])
             assert law1.keys() == law0[u].keys(), u
             assert max(abs(law1[v] - law0[u][v]) for v in law1) < 2.0 ** -26, u
-            if u not in compacted:
+            if u not in compacted or u in prefix:
                 assert law1 == law0[u], u
+
+
+def test_dead_prefix_keeps_lists_and_walks(orc):
+    """A sliding window's expiries leave each node a dead prefix: compaction
+    drops it and lowers every slot number by its length, member lists keep
+    their order, and walks (and the exact one-draw law) are unchanged."""
+    from brute import transition_law
+    rng = random.Random(11)
+    n = 40
+    e = _rand_edges(rng, n, 2000, wmax_bits=32, hub=1)
+    g = orc.OracleGraph(n)
+    bg = BruteGraph(n)
+    assert g.load(e) == 0
+    bg.insert(e)
+    more = _rand_edges(rng, n, 600, wmax_bits=16, hub=1)  # later arcs: deletes below move members
+    assert g.insert(more) == 0
+    bg.insert(more)
+    dels = []
+    for u in range(1, n + 1):  # each node's oldest arcs, up to a third of them
+        k = rng.randint(0, len(bg.slots[u]) // 3)
+        dels += [(u, s) for s in range(k)]
+    d = np.array(dels, dtype=orc.DEL_DT)
+    assert g.delete(d) == 0
+    bg.delete(d)
+    d0 = g.export()
+    before = split_dump(d0, n)
+    lead = [next((s for s, (v, w) in enumerate(bg.slots[u]) if w), len(bg.slots[u])) for u in range(n + 1)]
+    wp = orc.walk_params(seed=4, max_len=30)
+    pa, sa = g.walk(orc.DEEPWALK, wp, nwalkers=4 * n)
+    assert g.compact(0.0) == _brute_compact(bg, 0.0)
+    d1 = g.export()
+    check_against_brute(d1, bg)
+    after = split_dump(d1, n)
+    for u in range(1, n + 1):
+        for j in range(32):
+            assert [int(x) for x in after[u][2][j]] == [int(x) - lead[u] for x in before[u][2][j]], (u, j)
+        if int(d0["mass"][u]):
+            assert (transition_law(d1["T"][u], d1["alias"][u], after[u][2])
+                    == transition_law(d0["T"][u], d0["alias"][u], before[u][2])), u
+    pb, sb = g.walk(orc.DEEPWALK, wp, nwalkers=4 * n)
+    assert np.array_equal(pa, pb) and sa == sb
 
 
 def test_compaction_rule_and_bad_fraction(orc):
