@@ -2186,8 +2186,10 @@ __global__ __launch_bounds__(256) void k_compact_shift(DevStore s, const uint32_
     }
 }
 
-// larger nodes (the last ncomp[3] entries of shift_list), a workgroup each
-__global__ __launch_bounds__(256) void k_compact_shift_big(DevStore s, const uint32_t *__restrict__ list, uint32_t n1)
+// larger nodes (the last ncomp[3] entries of shift_list), a workgroup of
+// 1024 threads each: a hub's move is one workgroup's chain of round trips,
+// so it takes as many loads in flight as one workgroup holds
+__global__ __launch_bounds__(1024) void k_compact_shift_big(DevStore s, const uint32_t *__restrict__ list, uint32_t n1)
 {
     const uint32_t cnt = s.ctl->ncomp[3];
     for (uint32_t k = blockIdx.x; k < cnt; k += gridDim.x) {
@@ -2195,10 +2197,9 @@ __global__ __launch_bounds__(256) void k_compact_shift_big(DevStore s, const uin
         const NodeHdr h = s.hdr[u];
         const uint32_t d = h.nslots - h.nlive;
         const uint32_t D = h.nlive ? s.slot[h.slot_off + 2ull * h.slot_cap + d] : h.back_used;
-        if (h.nlive) shift_records<256>(s, h, d, D, h.nlive, (int)threadIdx.x);
-        if (h.back_used > D) shift_back<256>(s, h, D, h.back_used - D, (int)threadIdx.x);
+        if (h.nlive) shift_records<1024>(s, h, d, D, h.nlive, (int)threadIdx.x);
+        if (h.back_used > D) shift_back<1024>(s, h, D, h.back_used - D, (int)threadIdx.x);
         if (threadIdx.x == 0) shift_header(s, u, h, d, D);
-        __syncthreads();  // the header read of the next node by every thread after this one's write
     }
 }
 
@@ -3745,7 +3746,7 @@ void launch_compact_apply(DevStore s, const uint32_t *small_list, const uint32_t
         if (s.nbr) hipLaunchKernelGGL(k_compact_hash, dim3(zb), dim3(256), 0, st, s, shift_list, 2, 0, n1);
     }
     if (nzb) {
-        hipLaunchKernelGGL(k_compact_shift_big, dim3(std::min<uint32_t>(nzb, (uint32_t)ncu * 4)), dim3(256), 0, st, s,
+        hipLaunchKernelGGL(k_compact_shift_big, dim3(std::min<uint32_t>(nzb, (uint32_t)ncu * 2)), dim3(1024), 0, st, s,
                            shift_list, n1);
         if (s.nbr)
             hipLaunchKernelGGL(k_compact_hash, dim3((nzb + 3) / 4), dim3(256), 0, st, s, shift_list, 3, 1, n1);

@@ -96,6 +96,46 @@ def test_compaction_many_bits_match_oracle(orc):
     g.close()
 
 
+@pytest.mark.parametrize("nbr", [False, True])
+def test_dead_prefix_shift_matches_oracle(orc, nbr):
+    """Nodes whose dead slots are their oldest (a sliding window) take the
+    shift path: records and back entries move down, member words keep their
+    place and slot fields (the node's offset grows); small nodes one wave
+    each, a hub of more than 8192 slots a workgroup.  State, deletes and
+    inserts after it, and walks, against the oracle."""
+    rng = random.Random(23 if nbr else 24)
+    n = 300
+    e = _edges(rng, n, 40000, hubs=(5,))  # node 5: about 12000 slots
+    print(f"[compact] dead prefix nbr={nbr}", flush=True)
+    g = B.Graph(neighbor_index=nbr)
+    g.load(n, e)
+    o = orc.OracleGraph(n, nbr)
+    assert o.load(e) == 0
+
+    def oldest(frac):  # each node's live slots among its first ones
+        ex = o.export()
+        so = np.concatenate([[0], np.cumsum(ex["nslots"].astype(np.int64))])
+        dl = [(u, s) for u in range(1, n + 1) for s in range(int(ex["nslots"][u] * frac * rng.random()))
+              if ex["slot_w"][so[u] + s]]
+        return np.array(dl, dtype=B.DEL_DT)
+
+    for rnd in range(3):
+        d = oldest(0.3)
+        g.delete(d)
+        assert o.delete(d) == 0
+        assert g.compact(0.0) == o.compact(0.0), rnd
+        _same_state(g, o, f"shift round {rnd}")
+        ins = _edges(rng, n, 4000, hubs=(5, 9))
+        g.insert(ins)
+        assert o.insert(ins) == 0
+        d = _dels(rng, o, 3000)  # any live slots, under the new numbers: the next round mixes both paths
+        g.delete(d)
+        assert o.delete(d) == 0
+        _same_state(g, o, f"after shift round {rnd}")
+        _same_walks(g, o, orc, B.NODE2VEC if nbr else B.DEEPWALK, seed=80 + rnd, max_len=20, nwalkers=2 * n)
+    g.close()
+
+
 def test_compaction_float_weights_match_oracle(orc):
     rng = random.Random(5)
     n = 200
