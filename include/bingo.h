@@ -168,13 +168,15 @@ int bingo_delete_device(bingo_graph *g, const bingo_del *d_dels, int64_t k);
  * include/operation.cuh:209-210, so a node's slot array only grows) ----
  * Every node u with dead = nslots(u) - live(u) >= 1 and
  *     dead * 2^16 >= floor(min_dead_frac * 2^16) * nslots(u)
- * is compacted in place: its k-th live slot (in slot order) becomes slot k,
- * nslots(u) becomes live(u), and each of its listed groups' member lists is
- * rebuilt in slot order (the k-th member of group j is the k-th live slot
- * with bit j).  Masses and alias tables are unchanged, and so is the law of
- * a draw up to the multiply-shift's 2^-32 steps (a member may change
- * position); deletes after the call name the new slot numbers, and new arcs
- * take slots from live(u) on.  A caller that tracks slot numbers applies the
+ * is compacted in place: its k-th live slot (in slot order) becomes slot k
+ * and nslots(u) becomes live(u).  When u's dead slots are exactly its first
+ * ones (the oldest arcs expired, as in a sliding window), its member lists
+ * keep their order and walks draw the same targets as before; otherwise
+ * each of its listed groups' member lists is rebuilt in slot order (the
+ * k-th member of group j is the k-th live slot with bit j) and the law of a
+ * draw is unchanged up to the multiply-shift's 2^-32 steps.  Masses and
+ * alias tables are unchanged; deletes after the call name the new slot
+ * numbers, and new arcs take slots from live(u) on.  A caller that tracks slot numbers applies the
  * same rule (bingo_updgen_compact does).  nodes_out / slots_out (may be
  * NULL) receive the number of nodes compacted and of slots reclaimed.  The
  * call waits for the device once (the selection sizes its scratch). */
