@@ -759,6 +759,13 @@ int bingo_create(const bingo_config *cfg, bingo_graph **out)
     // test hook: treat the address reservation of the pools in this bit mask
     // as refused (exercises the copy fallback next to mapped pools)
     if (const char *e = std::getenv("BINGO_REFUSE_VMM")) g->refuse_vmm = (unsigned)std::strtoul(e, nullptr, 0);
+    // test hook: a smaller sbase limit (exercises the lowering of member slot
+    // fields by dead-prefix compactions on small graphs)
+    g->s.sbase_max = kSbaseMax;
+    if (const char *e = std::getenv("BINGO_SBASE_MAX")) {
+        const long long c = std::strtoll(e, nullptr, 0);
+        if (c >= 0 && c <= (long long)kSbaseMax) g->s.sbase_max = (uint32_t)c;
+    }
     // test hook: a smaller load batch (exercises the batched load on small graphs)
     g->load_chunk = kLoadChunk;
     if (const char *e = std::getenv("BINGO_LOAD_CHUNK")) {

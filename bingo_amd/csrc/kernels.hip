@@ -1792,7 +1792,7 @@ __global__ __launch_bounds__(256) void k_compact_select(DevStore s, uint32_t thr
             if (!(dead >= 1u && (uint64_t)dead * 65536u >= (uint64_t)thr16 * h.nslots)) continue;
             freed += dead;
             const bool big = h.nslots > kBigCompact;
-            if (h.dend == dead && (uint64_t)h.sbase + h.nslots <= kSbaseMax) {
+            if (h.dend == dead) {
                 (big ? py : pz) |= 1u << k;
             } else if (big) {
                 pb |= 1u << k;
@@ -2148,12 +2148,29 @@ __device__ __forceinline__ void shift_back(const DevStore &s, const NodeHdr &h, 
     }
 }
 
+// When the shift would take sbase past s.sbase_max, the node's member slot
+// fields (slot + sbase, each at least sbase + d: its slot was live) are
+// lowered by sbase + d instead, in place and in order, and sbase returns to 0
+template <int NT>
+__device__ __forceinline__ bool shift_renorm(const DevStore &s, int32_t u, const NodeHdr &h, uint32_t d, int t)
+{
+    if ((uint64_t)h.sbase + d <= s.sbase_max) return false;
+    const uint64_t sub = (uint64_t)h.sbase + d;
+    for (int j = 0; j < kGroups; ++j) {
+        if ((h.dmask >> j) & 1u) continue;  // a dense group keeps no list
+        const GroupHdr g = s.grp[(size_t)u * kGroups + j];
+        for (uint32_t i = (uint32_t)t; i < g.size; i += NT) s.mem[g.off + i] -= sub;
+    }
+    return true;
+}
+
 // the node's header after the shift
-__device__ __forceinline__ void shift_header(const DevStore &s, int32_t u, NodeHdr h, uint32_t d, uint32_t D)
+__device__ __forceinline__ void shift_header(const DevStore &s, int32_t u, NodeHdr h, uint32_t d, uint32_t D,
+                                             bool renorm)
 {
     h.nslots -= d;
     h.back_used -= D;
-    h.sbase += d;
+    h.sbase = renorm ? 0u : h.sbase + d;
     h.dend = 0;
     s.hdr[u] = h;
 }
@@ -2179,9 +2196,11 @@ __global__ __launch_bounds__(256) void k_compact_shift(DevStore s, const uint32_
         for (int q = 0; q < nq; ++q) {
             const NodeHdr h = hdr_of_lane(hk, q);
             const uint32_t d = h.nslots - h.nlive, D = rl32(Dk, q);
+            const int32_t u = (int32_t)rl32(uk, q);
             if (h.nlive) shift_records<64>(s, h, d, D, h.nlive, ln);
             if (h.back_used > D) shift_back<64>(s, h, D, h.back_used - D, ln);
-            if (ln == 0) shift_header(s, (int32_t)rl32(uk, q), h, d, D);
+            const bool renorm = shift_renorm<64>(s, u, h, d, ln);
+            if (ln == 0) shift_header(s, u, h, d, D, renorm);
         }
     }
 }
@@ -2199,7 +2218,8 @@ __global__ __launch_bounds__(1024) void k_compact_shift_big(DevStore s, const ui
         const uint32_t D = h.nlive ? s.slot[h.slot_off + 2ull * h.slot_cap + d] : h.back_used;
         if (h.nlive) shift_records<1024>(s, h, d, D, h.nlive, (int)threadIdx.x);
         if (h.back_used > D) shift_back<1024>(s, h, D, h.back_used - D, (int)threadIdx.x);
-        if (threadIdx.x == 0) shift_header(s, u, h, d, D);
+        const bool renorm = shift_renorm<1024>(s, u, h, d, (int)threadIdx.x);
+        if (threadIdx.x == 0) shift_header(s, u, h, d, D, renorm);
     }
 }
 

@@ -222,6 +222,10 @@ struct DevStore {
     // (GroupHdr::size, cap 0) and no back entries; draws from it reject over
     // the node's slots (dense_draw).
     uint32_t rate16;
+    // a dead-prefix compaction that would take a node's sbase past this
+    // lowers the node's member slot fields instead and returns sbase to 0
+    // (kSbaseMax; smaller only as a test hook)
+    uint32_t sbase_max;
     FNode *fnode;    // [n+1] when fw
     uint64_t *nmass; // [n+1] when nbr: each node's integer weight S (node2vec's folded return region)
     NodeHdr *hdr;
@@ -363,10 +367,11 @@ void launch_export_b(DevStore s, const uint64_t *slot_out_off, const uint64_t *m
 // segments of kCompactSeg slots spread over the grid (a wave each), their
 // records staged through the scratch.
 constexpr uint32_t kBigCompact = 8192;
-// a dead prefix is dropped without rewriting member words only while the
-// slot fields stay below this (they must stay clear of the delete kernels'
-// kMark bit); past it the node is rebuilt and sbase returns to 0
-constexpr uint64_t kSbaseMax = 1ull << 30;
+// a dead prefix is dropped without rewriting member words while the node's
+// sbase stays at most this (member slot fields, slot + sbase < 2^30 + 2^28,
+// must stay clear of the delete kernels' kMark bit); past it the member slot
+// fields are lowered in place and sbase returns to 0
+constexpr uint32_t kSbaseMax = 1u << 30;
 constexpr uint32_t kCompactSeg = 2048;
 struct CompactBig {
     uint32_t *segbase;  // [nbig + 1] first segment of each big node

@@ -96,18 +96,23 @@ def test_compaction_many_bits_match_oracle(orc):
     g.close()
 
 
-@pytest.mark.parametrize("nbr", [False, True])
-def test_dead_prefix_shift_matches_oracle(orc, nbr):
+@pytest.mark.parametrize("nbr,sbase_max", [(False, None), (True, None), (False, 40)])
+def test_dead_prefix_shift_matches_oracle(orc, monkeypatch, nbr, sbase_max):
     """Nodes whose dead slots are their oldest (a sliding window) take the
     shift path: records and back entries move down, member words keep their
     place and slot fields (the node's offset grows); small nodes one wave
-    each, a hub of more than 8192 slots a workgroup.  State, deletes and
-    inserts after it, and walks, against the oracle."""
-    rng = random.Random(23 if nbr else 24)
+    each, a hub of more than 8192 slots a workgroup.  With a small offset
+    limit (test hook) the slot fields are lowered in place instead.  State,
+    deletes and inserts after it, and walks, against the oracle."""
+    rng = random.Random(23 if nbr else 24 if sbase_max is None else 25)
     n = 300
     e = _edges(rng, n, 40000, hubs=(5,))  # node 5: about 12000 slots
-    print(f"[compact] dead prefix nbr={nbr}", flush=True)
+    print(f"[compact] dead prefix nbr={nbr} sbase_max={sbase_max}", flush=True)
+    if sbase_max is not None:
+        monkeypatch.setenv("BINGO_SBASE_MAX", str(sbase_max))
     g = B.Graph(neighbor_index=nbr)
+    if sbase_max is not None:
+        monkeypatch.delenv("BINGO_SBASE_MAX")
     g.load(n, e)
     o = orc.OracleGraph(n, nbr)
     assert o.load(e) == 0
