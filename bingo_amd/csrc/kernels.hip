@@ -2097,53 +2097,56 @@ __global__ __launch_bounds__(256) void k_compact_hash(DevStore s, const uint32_t
 // any of it is written, and later runs read above every position written.
 constexpr int kShiftRun = 4;
 
-// [c0, min(c0 + 64 kShiftRun, n)) of a node's records or back entries moved
-// from index + d to index by nt threads (t: this thread), lowering B values
-// by D
+// A node's n live records moved from index + d to index (back offsets
+// lowered by D) and its nb back entries from index + D to index, by NT
+// threads (t: this thread), in runs: each run's records and back entries are
+// read together before any of them is written.
 template <int NT>
-__device__ __forceinline__ void shift_records(const DevStore &s, const NodeHdr &h, uint32_t d, uint32_t D, uint32_t n,
-                                              int t)
+__device__ __forceinline__ void shift_node(const DevStore &s, const NodeHdr &h, uint32_t d, uint32_t D, uint32_t n,
+                                           uint32_t nb, int t)
 {
     uint32_t *V = s.slot + h.slot_off;
-    for (uint32_t c0 = 0; c0 < n; c0 += NT * kShiftRun) {
-        uint32_t v[kShiftRun], w[kShiftRun], b[kShiftRun], f[kShiftRun];
+    uint32_t *back = s.back + h.back_off;
+    constexpr uint32_t RS = NT * kShiftRun, RB = 2 * NT * kShiftRun;  // records, back entries per run
+    for (uint32_t k = 0; k * RS < n || k * RB < nb; ++k) {
+        const uint32_t c0 = k * RS, b0 = k * RB;
+        uint32_t v[kShiftRun], w[kShiftRun], b[kShiftRun], f[kShiftRun], x[2 * kShiftRun];
+        if (c0 < n) {
 #pragma unroll
-        for (int r = 0; r < kShiftRun; ++r) {
-            const uint32_t i = c0 + (uint32_t)(r * NT + t), k = (i < n ? i : 0u) + d;  // (n >= 1)
-            v[r] = V[k];
-            w[r] = V[h.slot_cap + k];
-            b[r] = V[2ull * h.slot_cap + k];
-            f[r] = s.fw ? V[3ull * h.slot_cap + k] : 0u;
-        }
-        if (NT > 64) __syncthreads();  // every wave's reads before any wave's writes
-#pragma unroll
-        for (int r = 0; r < kShiftRun; ++r) {
-            const uint32_t i = c0 + (uint32_t)(r * NT + t);
-            if (i < n) {
-                V[i] = v[r];
-                V[h.slot_cap + i] = w[r];
-                V[2ull * h.slot_cap + i] = b[r] - D;
-                if (s.fw) V[3ull * h.slot_cap + i] = f[r];
+            for (int r = 0; r < kShiftRun; ++r) {
+                const uint32_t i = c0 + (uint32_t)(r * NT + t), q = (i < n ? i : c0) + d;
+                v[r] = V[q];
+                w[r] = V[h.slot_cap + q];
+                b[r] = V[2ull * h.slot_cap + q];
+                f[r] = s.fw ? V[3ull * h.slot_cap + q] : 0u;
             }
         }
-    }
-}
-template <int NT>
-__device__ __forceinline__ void shift_back(const DevStore &s, const NodeHdr &h, uint32_t D, uint32_t n, int t)
-{
-    uint32_t *back = s.back + h.back_off;
-    for (uint32_t c0 = 0; c0 < n; c0 += NT * kShiftRun * 2) {
-        uint32_t x[2 * kShiftRun];
+        if (b0 < nb) {
 #pragma unroll
-        for (int r = 0; r < 2 * kShiftRun; ++r) {
-            const uint32_t i = c0 + (uint32_t)(r * NT + t);
-            x[r] = back[(i < n ? i : 0u) + D];
+            for (int r = 0; r < 2 * kShiftRun; ++r) {
+                const uint32_t i = b0 + (uint32_t)(r * NT + t);
+                x[r] = back[(i < nb ? i : b0) + D];
+            }
         }
-        if (NT > 64) __syncthreads();
+        if (NT > 64) __syncthreads();  // every wave's reads before any wave's writes
+        if (c0 < n) {
 #pragma unroll
-        for (int r = 0; r < 2 * kShiftRun; ++r) {
-            const uint32_t i = c0 + (uint32_t)(r * NT + t);
-            if (i < n) back[i] = x[r];
+            for (int r = 0; r < kShiftRun; ++r) {
+                const uint32_t i = c0 + (uint32_t)(r * NT + t);
+                if (i < n) {
+                    V[i] = v[r];
+                    V[h.slot_cap + i] = w[r];
+                    V[2ull * h.slot_cap + i] = b[r] - D;
+                    if (s.fw) V[3ull * h.slot_cap + i] = f[r];
+                }
+            }
+        }
+        if (b0 < nb) {
+#pragma unroll
+            for (int r = 0; r < 2 * kShiftRun; ++r) {
+                const uint32_t i = b0 + (uint32_t)(r * NT + t);
+                if (i < nb) back[i] = x[r];
+            }
         }
     }
 }
@@ -2197,8 +2200,7 @@ __global__ __launch_bounds__(256) void k_compact_shift(DevStore s, const uint32_
             const NodeHdr h = hdr_of_lane(hk, q);
             const uint32_t d = h.nslots - h.nlive, D = rl32(Dk, q);
             const int32_t u = (int32_t)rl32(uk, q);
-            if (h.nlive) shift_records<64>(s, h, d, D, h.nlive, ln);
-            if (h.back_used > D) shift_back<64>(s, h, D, h.back_used - D, ln);
+            shift_node<64>(s, h, d, D, h.nlive, h.back_used - D, ln);
             const bool renorm = shift_renorm<64>(s, u, h, d, ln);
             if (ln == 0) shift_header(s, u, h, d, D, renorm);
         }
@@ -2216,8 +2218,7 @@ __global__ __launch_bounds__(1024) void k_compact_shift_big(DevStore s, const ui
         const NodeHdr h = s.hdr[u];
         const uint32_t d = h.nslots - h.nlive;
         const uint32_t D = h.nlive ? s.slot[h.slot_off + 2ull * h.slot_cap + d] : h.back_used;
-        if (h.nlive) shift_records<1024>(s, h, d, D, h.nlive, (int)threadIdx.x);
-        if (h.back_used > D) shift_back<1024>(s, h, D, h.back_used - D, (int)threadIdx.x);
+        shift_node<1024>(s, h, d, D, h.nlive, h.back_used - D, (int)threadIdx.x);
         const bool renorm = shift_renorm<1024>(s, u, h, d, (int)threadIdx.x);
         if (threadIdx.x == 0) shift_header(s, u, h, d, D, renorm);
     }
