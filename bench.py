@@ -569,6 +569,12 @@ def main():
     ceiling_s = attempts / K * (1 / slot_rate + 1 / rates[8]) + (probes + folds + dtrials) / K / rates[8]
     log(f"random reads: {rates[32] / 1e9:.1f} G/s (32 B), {rates[8] / 1e9:.1f} G/s (8 B); "
         f"walk at {ceiling_s / (kern_ms / K / 1e3):.2f} of that ceiling")
+    # the update kernels' ceiling: independent random read-modify-writes of
+    # 8-byte words (each a line read and a 32-byte write request)
+    for _ in range(2):
+        nr, ms = g.gather_probe(0)
+        rates[0] = nr / (ms / 1e3)
+    log(f"random read-modify-writes: {rates[0] / 1e9:.1f} G/s (8 B)")
 
     bytes_per_launch = (B_STEP * attempts + B_PROBE * probes + B_FOLD * folds + B_DENSE * dtrials) / K
     kern_s = kern_ms / K / 1e3
@@ -619,7 +625,7 @@ def main():
                      "l2_hit_rate": profile_figure("l2_hit_rate", cfg["workload"]),
                      "kernel": "k_walk_persist" if tuned.get("form") == B.FORM_PERSISTENT else "k_walk",
                      "kernel_ms": kern_ms / K, "algorithmic_bytes_per_launch": bytes_per_launch},
-        "gather_ceiling": {"reads32_per_s": rates[32], "reads8_per_s": rates[8],
+        "gather_ceiling": {"reads32_per_s": rates[32], "reads8_per_s": rates[8], "rmw8_per_s": rates[0],
                            "kernel_ms_at_ceiling": ceiling_s * 1e3, "frac": ceiling_s / kern_s},
         "cpu_baseline": None,
     }

@@ -673,7 +673,8 @@ class Graph:
         return ms.value
 
     def gather_probe(self, width):
-        """(reads, kernel ms) of independent random `width`-byte reads over the member pool"""
+        """(reads, kernel ms) of independent random `width`-byte reads over the member pool;
+        width 0: (read-modify-writes, kernel ms) of 8-byte words stored back unchanged"""
         n, ms = C.c_uint64(0), C.c_float(0)
         _check(lib().bingo_gather_probe(self._h, width, C.byref(n), C.byref(ms)))
         return n.value, ms.value

@@ -1796,11 +1796,12 @@ int bingo_memcpy_d2d(bingo_graph *g, void *d_dst, const void *d_src, size_t byte
 int bingo_gather_probe(bingo_graph *g, int width, uint64_t *reads, float *ms)
 {
     if (!g || !g->loaded) return fail(BINGO_ESTATE, "graph not loaded");
-    if ((width != 8 && width != 32) || !reads || !ms) return fail(BINGO_EINVAL, "width must be 8 or 32");
+    if ((width != 0 && width != 8 && width != 32) || !reads || !ms)
+        return fail(BINGO_EINVAL, "width must be 0, 8 or 32");
     int rc;
     if ((rc = read_ctl(g))) return rc;
     const uint64_t bytes = g->ctl_host->top[POOL_MEM] * kElem[POOL_MEM];
-    const uint64_t units = bytes / (uint64_t)width;
+    const uint64_t units = bytes / (uint64_t)(width ? width : 8);
     if (units < 1) return fail(BINGO_EINVAL, "member pool is empty");
     if ((rc = ensure_scratch(g, SC_AUX1, 64))) return rc;
     hipEvent_t e0 = g->ev[kEvents - 2], e1 = g->ev[kEvents - 1];

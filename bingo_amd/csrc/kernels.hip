@@ -3356,6 +3356,29 @@ __global__ __launch_bounds__(256) void k_gather_probe(const uint8_t *__restrict_
     if (acc == 0x5eed5eed5eedull) *sink = acc;  // keeps the loads; practically never taken
 }
 
+// Random read-modify-write probe (the ceiling for the update kernels'
+// scattered partial-line writes): rounds of 8 independent 8-byte words at
+// hashed positions in [base, base + units * 8), each read and stored back
+// unchanged, so the store it runs over keeps its contents.  A random 8-byte
+// store is one 32-byte write request to memory on this part.
+__global__ __launch_bounds__(256) void k_rmw_probe(uint64_t *__restrict__ base, uint64_t units, uint32_t iters)
+{
+    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    for (uint32_t it = 0; it < iters; ++it) {
+        uint64_t i[8], v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            i[k] = probe_mix(t * 0x9E3779B97F4A7C15ull + (uint64_t)it * 8 + k) % units;
+            v[k] = base[i[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            asm volatile("" : "+v"(v[k]));  // the store is not provably the loaded value: kept
+            base[i[k]] = v[k];
+        }
+    }
+}
+
 // ------------------------------------------------------------------------
 // deferred region moves of an insert batch (see CopyTask): phase 0 copies
 // regions and zeroes new neighbour tables, phase 1 re-inserts the old tables'
@@ -3424,11 +3447,14 @@ static inline unsigned run_blocks(int64_t k)
 
 void launch_reset_ctl(Ctl *ctl, hipStream_t st) { hipLaunchKernelGGL(k_reset_ctl, dim3(1), dim3(64), 0, st, ctl); }
 
-uint64_t launch_gather_probe(const void *base, uint64_t units, int width, unsigned long long *sink, hipStream_t st)
+uint64_t launch_gather_probe(void *base, uint64_t units, int width, unsigned long long *sink, hipStream_t st)
 {
     const unsigned blocks = 4096, threads = 256;
     const uint32_t iters = 64;
-    if (width == 32)
+    if (width == 0)
+        hipLaunchKernelGGL(k_rmw_probe, dim3(blocks), dim3(threads), 0, st,
+                           static_cast<uint64_t *>(base), units, iters);
+    else if (width == 32)
         hipLaunchKernelGGL(k_gather_probe<32>, dim3(blocks), dim3(threads), 0, st,
                            static_cast<const uint8_t *>(base), units, iters, sink);
     else

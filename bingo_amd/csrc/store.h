@@ -398,7 +398,7 @@ void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st);
 void launch_copy_tasks(const DevStore &s, hipStream_t st);
 // random reads of `width` (8 or 32) bytes over units * width bytes at base;
 // returns the number of reads
-uint64_t launch_gather_probe(const void *base, uint64_t units, int width, unsigned long long *sink, hipStream_t st);
+uint64_t launch_gather_probe(void *base, uint64_t units, int width, unsigned long long *sink, hipStream_t st);
 
 // rocPRIM wrappers (temp storage sized by a null call)
 // Pairs sort over double buffers: input in (k0, v0), (k1, v1) is the other

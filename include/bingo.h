@@ -282,7 +282,10 @@ int bingo_memcpy_d2d(bingo_graph *g, void *d_dst, const void *d_src, size_t byte
  * bytes (8 = a member, 32 = an alias slot) at hashed positions over the
  * graph's member pool, independent of each other; *reads receives their
  * number and *ms the kernel time.  The rate is the ceiling for a walk's
- * dependent gathers on this store.  Overwrites event slots 14 and 15. */
+ * dependent gathers on this store.  width 0: random read-modify-writes of
+ * 8-byte words instead, each stored back unchanged (the pool keeps its
+ * contents), the ceiling for the update kernels' scattered writes; *reads
+ * receives their number.  Overwrites event slots 14 and 15. */
 int bingo_gather_probe(bingo_graph *g, int width, uint64_t *reads, float *ms);
 /* HIP events on the handle's stream: record into slot (0..15), elapsed ms
  * (bingo_walk_tune and bingo_gather_probe overwrite slots 14 and 15) */

@@ -481,6 +481,13 @@ def test_gather_probe_reports_rates():
     for width in (8, 32):
         n, ms = g.gather_probe(width)
         assert n > 0 and ms > 0
+    before = g.export()
+    n, ms = g.gather_probe(0)  # read-modify-writes leave the store as it was
+    assert n > 0 and ms > 0
+    after = g.export()
+    assert before.keys() == after.keys()
+    for k in before:
+        assert np.array_equal(before[k], after[k]), k
     with pytest.raises(B.BingoError):
         g.gather_probe(16)
     g.close()
