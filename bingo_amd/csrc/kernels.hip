@@ -3058,7 +3058,16 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
     constexpr bool MIX = CLS == kServeMixed;
     __shared__ ServeReq rq[kMaxServeReqs];
     __shared__ int32_t stage[256 * WPL * (kServeD + 1)];  // odd pitch: no bank conflicts
-    for (int i = threadIdx.x; i < nreq; i += blockDim.x) rq[i] = reqs[i];
+    // Steps per request are counted per workgroup: the LDS copy's `steps`
+    // word holds this workgroup's count (the global counter's address stays
+    // in reqs), added to the global counter once at the end.  Lanes counting
+    // straight into the global counters (adjacent words, so one L2 channel)
+    // serialised ~10^6 atomics per launch.
+    auto count = [&](int i) { return reinterpret_cast<unsigned long long *>(&rq[i].steps); };
+    for (int i = threadIdx.x; i < nreq; i += blockDim.x) {
+        rq[i] = reqs[i];
+        *count(i) = 0ull;
+    }
     __syncthreads();
     const int ln = lane_id();
     const uint64_t lt = (1ull << ln) - 1ull;
@@ -3086,6 +3095,14 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
         fst[k] = FoldState{0, 0, 0, false};
     }
     bool more = true;
+    // indices are claimed from the launch cursor a chunk at a time into a
+    // wave-local range [lp, le), as in k_walk_persist: walkers that finish
+    // at different iterations (node2vec rejections, PPR stops) would
+    // otherwise cost one atomic on the one cursor address per wave and
+    // iteration
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint32_t chunk = (uint32_t)min<uint64_t>(4u * 64u * WPL, (uint64_t)total / (8u * nwaves));
+    uint64_t lp = 0, le = 0;
     for (;;) {
         if (more) {
             uint64_t emp[WPL];
@@ -3096,14 +3113,25 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
                 need += (uint32_t)__popcll(emp[k]);
             }
             if (need) {
-                unsigned long long base = 0;
-                if (ln == 0) base = atomicAdd(cursor, (unsigned long long)need);
-                base = __shfl(base, 0, 64);
+                // ranks below n0 take the rest of the local range, the others
+                // a fresh claim of at least `need` indices
+                const uint64_t r0 = lp, n0 = le - lp;
+                unsigned long long b = 0;
+                if (n0 < need) {
+                    const uint32_t claim = max(chunk, need);
+                    if (ln == 0) b = atomicAdd(cursor, (unsigned long long)claim);
+                    b = __shfl(b, 0, 64);
+                    lp = b + (need - n0);
+                    le = b + claim;
+                } else {
+                    lp += need;
+                }
                 uint32_t off = 0;
 #pragma unroll
                 for (int k = 0; k < WPL; ++k) {
                     if (rid[k] < 0) {
-                        const unsigned long long c = base + off + (uint32_t)__popcll(emp[k] & lt);
+                        const uint64_t rk = off + (uint32_t)__popcll(emp[k] & lt);
+                        const unsigned long long c = rk < n0 ? r0 + rk : b + (rk - n0);
                         if (c < total) {
                             int lo = 0, hi = nreq - 1;  // last request with base <= c
                             while (lo < hi) {
@@ -3113,7 +3141,7 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
                             const ServeReq &q = rq[lo];
                             const int64_t w = q.first + (int64_t)(c - q.base);
                             if (acc_req[k] != lo) {
-                                if (acc_req[k] >= 0 && acc[k]) atomicAdd(rq[acc_req[k]].steps, (unsigned long long)acc[k]);
+                                if (acc_req[k] >= 0 && acc[k]) atomicAdd(count(acc_req[k]), (unsigned long long)acc[k]);
                                 acc_req[k] = lo;
                                 acc[k] = 0;
                             }
@@ -3143,7 +3171,8 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
                     }
                     off += (uint32_t)__popcll(emp[k]);
                 }
-                if (base + need >= total) more = false;
+                // nothing valid left locally; the cursor is past the end too
+                if (lp >= (uint64_t)total) more = false;
             }
         }
         bool mine = false;
@@ -3209,7 +3238,12 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
     }
 #pragma unroll
     for (int k = 0; k < WPL; ++k)
-        if (acc_req[k] >= 0 && acc[k]) atomicAdd(rq[acc_req[k]].steps, (unsigned long long)acc[k]);
+        if (acc_req[k] >= 0 && acc[k]) atomicAdd(count(acc_req[k]), (unsigned long long)acc[k]);
+    __syncthreads();  // every wave leaves the loop above
+    for (int i = threadIdx.x; i < nreq; i += blockDim.x) {
+        const unsigned long long c = *count(i);
+        if (c) atomicAdd(reqs[i].steps, c);
+    }
 }
 
 // ------------------------------------------------------------------------
