@@ -248,7 +248,7 @@ def served(B, args):
     # served form
     g = fresh_graph()
     outs = [B.DeviceArray(g, nbytes=len(r[2]) * 80 * 4) if r[0] == "walk" else None for r in reqs]
-    srv = B.Server(g, arena_bytes=1 << 30, max_launch_walkers=args.launch_walkers, streams=2)
+    srv = B.Server(g, arena_bytes=1 << 30, max_launch_walkers=args.launch_walkers, streams=args.streams)
     log(f"served: {nreq} requests at {args.rate} req/s")
     tickets, t_sub = [], []
     t0 = time.perf_counter()
@@ -323,7 +323,7 @@ def served(B, args):
                    "requests": nreq, "offered_rate_per_s": args.rate, "walkers_per_query": [1000, 1000000],
                    "update_batch": [1000, 100000], "update_share": args.update_share, "walk_len": 80,
                    "apps": ["deepwalk", "node2vec"],
-                   "launch_walkers": args.launch_walkers},
+                   "launch_walkers": args.launch_walkers, "streams": args.streams},
         "achieved_rate_per_s": nreq / wall,
         "submit_span_s": submit_span,
         "p50_ms": st["p50_ms"], "p90_ms": st["p90_ms"], "p99_ms": st["p99_ms"], "max_ms": st["max_ms"],
@@ -372,6 +372,7 @@ def main():
     ap.add_argument("--requests", type=int, default=300, help="c4: number of requests")
     ap.add_argument("--update-share", type=float, default=0.1, metavar="F",
                     help="c4: share of the requests that are update batches")
+    ap.add_argument("--streams", type=int, default=2, help="c4: server launches in flight at once")
     ap.add_argument("--launch-walkers", type=int, default=1 << 22,
                     help="c4: the serving loop's walkers per launch (larger requests are chunked)")
     ap.add_argument("--no-cpu", action="store_true")
