@@ -29,7 +29,10 @@ node holding at least R of the node's first arcs keeps only its count and is
 drawn from by rejection over the node's slots.
 
 `c4` is the served form: a Poisson stream of walk queries and update
-batches through the serving loop, with p50/p99 latency.  `c5` is the scale
+batches through the serving loop, with p50/p99 latency, and a load curve
+(the same stream at further rates, `--curve-rates`) giving the highest rate
+at which the served form and a one-at-a-time plain server keep p99 of
+1M-step queries within 10 ms.  `c5` is the scale
 workload: 2^30 R-MAT arcs over 2^25 nodes (log-uniform u32 weights) in a
 store fitted to the graph at load, 3.2e7 DeepWalk walkers of length 80, and
 a sliding window of 10^7 inserts + 10^7 expiries of the oldest arcs per step;
@@ -245,40 +248,48 @@ def served(B, args):
         d_base.free()
         return g
 
+    def serve_pass(rate):
+        """one pass of the stream through a server at `rate` requests/s (the
+        same arrivals, time-scaled), on a freshly loaded graph"""
+        g = fresh_graph()
+        outs = [B.DeviceArray(g, nbytes=len(r[2]) * 80 * 4) if r[0] == "walk" else None for r in reqs]
+        srv = B.Server(g, arena_bytes=1 << 30, max_launch_walkers=args.launch_walkers, streams=args.streams)
+        log(f"served: {nreq} requests at {rate} req/s")
+        tickets = []
+        t0 = time.perf_counter()
+        due = t0
+        for r, gap, out in zip(reqs, gaps * (args.rate / rate), outs):
+            due += gap
+            while time.perf_counter() < due:
+                pass
+            if r[0] == "walk":
+                tickets.append(srv.submit_walk(r[1], B.walk_params(seed=r[3], max_len=80), starts=r[2], out=out))
+            else:
+                tickets.append(srv.submit_update(r[1], r[2]))
+        submit_span = time.perf_counter() - t0
+        res = {"lat_per_mstep": [], "steps_total": 0, "lat_1m": [], "req_steps": [0] * nreq,
+               "submit_span": submit_span}
+        for i, (r, t) in enumerate(zip(reqs, tickets)):
+            _, steps, lat = srv.wait(t)
+            if r[0] == "walk":
+                res["req_steps"][i] = steps
+                res["steps_total"] += steps
+                res["lat_per_mstep"].append(lat / max(steps, 1) * 1e6)
+                if 0.5e6 <= steps <= 2e6:  # queries of about one million steps
+                    res["lat_1m"].append(lat)
+        res["wall"] = time.perf_counter() - t0
+        res["st"] = srv.stats()
+        srv.close()
+        for o in outs:
+            if o:
+                o.free()
+        g.close()
+        return res
+
     # served form
-    g = fresh_graph()
-    outs = [B.DeviceArray(g, nbytes=len(r[2]) * 80 * 4) if r[0] == "walk" else None for r in reqs]
-    srv = B.Server(g, arena_bytes=1 << 30, max_launch_walkers=args.launch_walkers, streams=args.streams)
-    log(f"served: {nreq} requests at {args.rate} req/s")
-    tickets, t_sub = [], []
-    t0 = time.perf_counter()
-    due = t0
-    for r, gap, out in zip(reqs, gaps, outs):
-        due += gap
-        while time.perf_counter() < due:
-            pass
-        if r[0] == "walk":
-            tickets.append(srv.submit_walk(r[1], B.walk_params(seed=r[3], max_len=80), starts=r[2], out=out))
-        else:
-            tickets.append(srv.submit_update(r[1], r[2]))
-    submit_span = time.perf_counter() - t0
-    lat_per_mstep, steps_total, lat_1m = [], 0, []
-    req_steps = [0] * nreq
-    for i, (r, t) in enumerate(zip(reqs, tickets)):
-        _, steps, lat = srv.wait(t)
-        if r[0] == "walk":
-            req_steps[i] = steps
-            steps_total += steps
-            lat_per_mstep.append(lat / max(steps, 1) * 1e6)
-            if 0.5e6 <= steps <= 2e6:  # queries of about one million steps
-                lat_1m.append(lat)
-    wall = time.perf_counter() - t0
-    st = srv.stats()
-    srv.close()
-    for o in outs:
-        if o:
-            o.free()
-    g.close()
+    main = serve_pass(args.rate)
+    lat_per_mstep, steps_total, lat_1m = main["lat_per_mstep"], main["steps_total"], main["lat_1m"]
+    req_steps, submit_span, wall, st = main["req_steps"], main["submit_span"], main["wall"], main["st"]
 
     # plain form: the same requests one at a time, one plain launch each
     g = fresh_graph()
@@ -301,16 +312,45 @@ def served(B, args):
     plain_wall = time.perf_counter() - t1
     # the plain form as a server: one request at a time in arrival order, each
     # taking its measured service time, under the same Poisson arrivals
-    arrive = np.cumsum(gaps) * 1e3
-    free_at, plain_lat, plain_1m = 0.0, [], []
-    for i, r in enumerate(reqs):
-        free_at = max(free_at, arrive[i]) + svc[i]
-        if r[0] == "walk":
-            plain_lat.append(free_at - arrive[i])
-            if 0.5e6 <= req_steps[i] <= 2e6:
-                plain_1m.append(free_at - arrive[i])
+    def plain_server(rate):
+        arrive = np.cumsum(gaps * (args.rate / rate)) * 1e3
+        free_at, lat, lat1m = 0.0, [], []
+        for i, r in enumerate(reqs):
+            free_at = max(free_at, arrive[i]) + svc[i]
+            if r[0] == "walk":
+                lat.append(free_at - arrive[i])
+                if 0.5e6 <= req_steps[i] <= 2e6:
+                    lat1m.append(free_at - arrive[i])
+        return lat, lat1m
+
+    plain_lat, plain_1m = plain_server(args.rate)
     plain_steps = int(dsteps.to_host(np.uint64, (1,))[0])
     g.close()
+
+    # load curve: the same stream at other rates through the server (a pass
+    # each) and through the plain server (from the measured service times);
+    # the highest rate at which each keeps p99 of 1M-step queries <= 10 ms
+    def p99(x):
+        return float(np.percentile(x, 99)) if x else None
+
+    curve = []
+    for r in sorted(set(args.curve_rates) | {args.rate}):
+        res = main if r == args.rate else serve_pass(r)
+        curve.append({"rate_per_s": r, "achieved_rate_per_s": nreq / res["wall"],
+                      "served_walk_steps_per_s": res["steps_total"] / res["wall"],
+                      "p50_ms_1M_step_queries": float(np.percentile(res["lat_1m"], 50)) if res["lat_1m"] else None,
+                      "p99_ms_1M_step_queries": p99(res["lat_1m"]),
+                      "plain_server_p99_ms_1M_step_queries": p99(plain_server(r)[1])})
+    served_ok = [c["rate_per_s"] for c in curve if c["p99_ms_1M_step_queries"] is not None
+                 and c["p99_ms_1M_step_queries"] <= 10.0 and c["achieved_rate_per_s"] >= 0.95 * c["rate_per_s"]]
+    plain_ok = 0.0
+    for r in np.arange(25.0, 4000.0 + 1e-9, 25.0):  # the plain server's p99 grows with the rate
+        q = p99(plain_server(float(r))[1])
+        if q is None or q > 10.0:
+            break
+        plain_ok = float(r)
+    log(f"p99 of 1M-step queries <= 10 ms up to {max(served_ok, default=0)} req/s served (of the rates run), "
+        f"{plain_ok} req/s plain server")
     lp = np.percentile(lat_per_mstep, [50, 99]) if lat_per_mstep else [None, None]
     nupd = sum(len(r[1]) + len(r[2]) for r in reqs if r[0] == "update")
     out = {
@@ -338,6 +378,8 @@ def served(B, args):
         "plain_server_p50_ms": float(np.percentile(plain_lat, 50)),
         "plain_server_p99_ms": float(np.percentile(plain_lat, 99)),
         "plain_server_p99_ms_1M_step_queries": float(np.percentile(plain_1m, 99)) if plain_1m else None,
+        "load_curve": curve,
+        "max_rate_p99_1M_step_le_10ms": {"served": max(served_ok, default=0), "plain_server": plain_ok},
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
@@ -372,6 +414,8 @@ def main():
     ap.add_argument("--requests", type=int, default=300, help="c4: number of requests")
     ap.add_argument("--update-share", type=float, default=0.1, metavar="F",
                     help="c4: share of the requests that are update batches")
+    ap.add_argument("--curve-rates", type=lambda x: [float(v) for v in x.split(",") if v], default=[500.0, 1000.0, 1500.0],
+                    help="c4: further offered rates (req/s, comma-separated) for the load curve")
     ap.add_argument("--streams", type=int, default=2, help="c4: server launches in flight at once")
     ap.add_argument("--launch-walkers", type=int, default=1 << 22,
                     help="c4: the serving loop's walkers per launch (larger requests are chunked)")
