@@ -3058,15 +3058,15 @@ __global__ __launch_bounds__(256) void k_walk_serve(DevStore s, const ServeReq *
     constexpr bool MIX = CLS == kServeMixed;
     __shared__ ServeReq rq[kMaxServeReqs];
     __shared__ int32_t stage[256 * WPL * (kServeD + 1)];  // odd pitch: no bank conflicts
-    // Steps per request are counted per workgroup: the LDS copy's `steps`
-    // word holds this workgroup's count (the global counter's address stays
-    // in reqs), added to the global counter once at the end.  Lanes counting
-    // straight into the global counters (adjacent words, so one L2 channel)
-    // serialised ~10^6 atomics per launch.
-    auto count = [&](int i) { return reinterpret_cast<unsigned long long *>(&rq[i].steps); };
+    // Steps per request are counted per workgroup: the LDS copy of an entry
+    // holds this workgroup's count in place of the counter's address (which
+    // stays in reqs), added to the global counter once at the end.  Lanes
+    // counting straight into the global counters (adjacent words, so one L2
+    // channel) serialised ~10^6 atomics per launch.
+    auto count = [&](int i) { return &rq[i].count; };
     for (int i = threadIdx.x; i < nreq; i += blockDim.x) {
         rq[i] = reqs[i];
-        *count(i) = 0ull;
+        rq[i].count = 0ull;
     }
     __syncthreads();
     const int ln = lane_id();

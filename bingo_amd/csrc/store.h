@@ -271,7 +271,10 @@ struct ServeReq {
     const int32_t *starts;
     const int32_t *lens;
     int32_t *paths;
-    unsigned long long *steps;
+    union {
+        unsigned long long *steps;  // the request's step counter (device memory), as passed
+        unsigned long long count;   // a workgroup's LDS copy of the entry: its steps for the request
+    };
     unsigned long long base;
     int64_t first;
     int64_t nwalkers;
