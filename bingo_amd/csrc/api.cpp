@@ -124,6 +124,16 @@ static int take_ustatus(bingo_graph *g)
     return fail(BINGO_ENOMEM, "a captured update batch did not fit the pools' committed room" + rest);
 }
 
+namespace bingo {
+int update_wait(bingo_graph *g)
+{
+    int rc = read_ctl(g);
+    if (rc) return rc;
+    if (g->ctl_host->error & ERR_OVERRUN) return fail(BINGO_EDEVICE, "pool overrun detected");
+    return take_ustatus(g);
+}
+}  // namespace bingo
+
 // grow_in_place: pool p is a reserved address range (the HBM budget, at
 // most 2^36 elements) mapped from its start in 1 GiB chunks; growing maps
 // more chunks of physical HBM behind the mapped part.  Returns BINGO_OK, a negative error,

@@ -17,6 +17,11 @@ int fail(int code, const std::string &msg);
 // host-side quantisation of walk parameters into kernel thresholds (fw:
 // float weights; dense: the store has dense groups)
 int fill_walk_args(const bingo_walk_params *p, int app, WalkArgs *a, int fw, int dense);
+// waits for the handle's stream and reports what the update batches since
+// the last wait left: a pool overrun, or a captured batch that applied
+// nothing (invalid, or past the committed room); unlike bingo_sync it does
+// not report walk inputs
+int update_wait(bingo_graph *g);
 
 }  // namespace bingo
 

@@ -45,6 +45,12 @@ double now_ms()
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 constexpr int kMaxStreams = 4;
+// update batches of up to kCapturedMax arcs each way replay captured graphs
+// (bingo_update_graph_*), whose pools grow inside kCapturedRoom committed at
+// creation; larger batches, and a batch past the room, take bingo_insert /
+// bingo_delete (which grow the pools)
+constexpr int64_t kCapturedMax = 1 << 17;
+constexpr uint64_t kCapturedRoom = 1ull << 30;
 
 struct Req {
     uint64_t ticket = 0;
@@ -130,6 +136,9 @@ struct bingo_server {
     std::vector<double> lat;
     bingo_server_stats st{};
     int blocks_per_cu[3] = {1, 1, 1};  // per launch class: other apps, node2vec, mixed
+    bingo_update_graph *ug = nullptr;  // captured update batches (null: not available)
+    bingo_edge *d_ins = nullptr;       // their staging buffers
+    bingo_del *d_dels = nullptr;
     // scheduler-local
     std::deque<std::shared_ptr<Req>> order;  // unfinished requests, submission order
     double busy_since = -1;
@@ -330,12 +339,47 @@ static bool schedule(bingo_server *sv, Launch &L)
     return true;
 }
 
+// One side of an update batch (inserts, or deletes) replayed from the
+// captured graphs: staged on the handle's stream, launched, and waited for;
+// BINGO_ENOMEM if it did not fit the committed room (then it applied
+// nothing), BINGO_EINVAL if it was invalid (likewise).
+static int captured_side(bingo_server *sv, bool del, const void *host, int64_t k)
+{
+    bingo_graph *g = sv->g;
+    void *dst = del ? static_cast<void *>(sv->d_dels) : static_cast<void *>(sv->d_ins);
+    const size_t bytes = (size_t)k * (del ? sizeof(bingo_del) : sizeof(bingo_edge));
+    if (hipMemcpyAsync(dst, host, bytes, hipMemcpyHostToDevice, g->st) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(BINGO_EDEVICE, "staging an update batch failed");
+    }
+    int rc = bingo_update_graph_launch(sv->ug, del ? 0 : k, del ? k : 0);
+    if (rc) return rc;
+    if ((rc = update_wait(g))) g->arcs += del ? k : -k;  // the launch counted arcs it did not apply
+    return rc;
+}
+
+// An update batch: its inserts, then its deletes, each side applied whole or
+// not at all; a side that fails leaves the deletes unapplied too (as
+// bingo_insert then bingo_delete).  Each side replays the captured graphs
+// when it fits them (no host round trip inside the batch), else -- or when
+// the captured side ran past the committed room -- it is applied by
+// bingo_insert / bingo_delete, which grow the pools.
 static void run_update(bingo_server *sv, const std::shared_ptr<Req> &u)
 {
     const double t0 = now_ms();
-    int rc = bingo_insert(sv->g, u->ins.data(), (int64_t)u->ins.size());
-    if (!rc) rc = bingo_delete(sv->g, u->dels.data(), (int64_t)u->dels.size());
-    if (hipStreamSynchronize(sv->g->st) != hipSuccess && !rc) rc = fail(BINGO_EDEVICE, "update failed on the device");
+    bingo_graph *g = sv->g;
+    const int64_t ni = (int64_t)u->ins.size(), nd = (int64_t)u->dels.size();
+    auto side = [&](bool del, int64_t k) {
+        if (!k) return (int)BINGO_OK;
+        int rc = BINGO_ENOMEM;
+        if (sv->ug && k <= kCapturedMax)
+            rc = captured_side(sv, del, del ? static_cast<const void *>(u->dels.data()) : u->ins.data(), k);
+        if (rc == BINGO_ENOMEM) rc = del ? bingo_delete(g, u->dels.data(), k) : bingo_insert(g, u->ins.data(), k);
+        return rc;
+    };
+    int rc = side(false, ni);
+    if (!rc) rc = side(true, nd);
+    if (hipStreamSynchronize(g->st) != hipSuccess && !rc) rc = fail(BINGO_EDEVICE, "update failed on the device");
     {
         std::lock_guard<std::mutex> lk(sv->mu);
         sv->st.busy_s += (now_ms() - t0) / 1e3;
@@ -356,6 +400,16 @@ static void worker_main(bingo_server *sv)
             }
             if (sv->stopping && sv->order.empty() && !busy_launches(sv)) return;
         }
+        // stage the inputs of arrived walks (submission order, as the arena
+        // allows) ahead of packing, while launches run: packing a launch then
+        // waits for no copies
+        bool staged = false;
+        for (auto &r : sv->order) {
+            if (!r->walk || r->admitted) continue;
+            if (!admit(sv, *r)) break;
+            staged = true;
+        }
+        if (staged) (void)hipStreamSynchronize(sv->copy_st);
         bool did = false;
         for (int i = 0; i < sv->nstreams; ++i) {  // completed launches
             Launch &L = sv->ln[i];
@@ -416,6 +470,9 @@ static void release_server(bingo_server *sv)
         if (L.h_table) (void)hipHostFree(L.h_table);
         if (L.h_counts) (void)hipHostFree(L.h_counts);
     }
+    if (sv->ug) bingo_update_graph_destroy(sv->ug);
+    if (sv->d_ins) (void)hipFree(sv->d_ins);
+    if (sv->d_dels) (void)hipFree(sv->d_dels);
     if (sv->copy_st) (void)hipStreamDestroy(sv->copy_st);
     if (sv->arena.base) (void)hipFree(sv->arena.base);
     delete sv;
@@ -469,6 +526,12 @@ int bingo_server_create(bingo_graph *g, const bingo_server_config *c, bingo_serv
         release_server(sv);
         return fail(BINGO_ENOMEM, "server arena, stream or table allocation failed");
     }
+    if (hipMalloc((void **)&sv->d_ins, kCapturedMax * sizeof(bingo_edge)) == hipSuccess &&
+        hipMalloc((void **)&sv->d_dels, kCapturedMax * sizeof(bingo_del)) == hipSuccess &&
+        bingo_update_graph_create(g, sv->d_ins, kCapturedMax, sv->d_dels, kCapturedMax, kCapturedRoom, &sv->ug) !=
+            BINGO_OK)
+        sv->ug = nullptr;  // update batches then all take bingo_insert / bingo_delete
+    (void)hipGetLastError();
     sv->worker = std::thread(worker_main, sv);
     *out = sv;
     return BINGO_OK;
