@@ -13,8 +13,11 @@
 // completion is observed through its event: the next launch is packed from
 // whatever arrived and enqueued while the current one runs, so the device
 // goes from one to the next without a gap and a request that arrives while
-// others run starts in the next launch.  Every request records submit ->
-// completion latency.
+// others run starts in the next launch.  While launches run, the scheduler
+// stages the inputs of arrived walks, so packing a launch waits for no
+// copies, and an update batch replays captured graphs (no host round trip
+// inside it) when it fits them.  Every request records submit -> completion
+// latency.
 //
 // There is no counterpart in the reference, which launches one walk kernel at
 // a time and synchronises after it (include/graph.cuh:93-150).
@@ -94,7 +97,6 @@ struct Launch {
     std::vector<Part> parts;
     int64_t total = 0;
     bool busy = false;
-    uint64_t seq = 0;  // launch order
 };
 
 // first-fit placement of request staging in the device arena
@@ -142,7 +144,6 @@ struct bingo_server {
     // scheduler-local
     std::deque<std::shared_ptr<Req>> order;  // unfinished requests, submission order
     double busy_since = -1;
-    uint64_t launch_seq = 0;
 };
 
 static void complete(bingo_server *sv, const std::shared_ptr<Req> &r, int rc)
@@ -328,7 +329,6 @@ static bool schedule(bingo_server *sv, Launch &L)
                            hipMemcpyDeviceToHost, L.st);
     if (e == hipSuccess) e = hipEventRecord(L.ev, L.st);
     L.busy = true;
-    L.seq = ++sv->launch_seq;
     if (e != hipSuccess) {
         (void)hipStreamSynchronize(L.st);
         harvest(sv, L, e);
@@ -400,16 +400,6 @@ static void worker_main(bingo_server *sv)
             }
             if (sv->stopping && sv->order.empty() && !busy_launches(sv)) return;
         }
-        // stage the inputs of arrived walks (submission order, as the arena
-        // allows) ahead of packing, while launches run: packing a launch then
-        // waits for no copies
-        bool staged = false;
-        for (auto &r : sv->order) {
-            if (!r->walk || r->admitted) continue;
-            if (!admit(sv, *r)) break;
-            staged = true;
-        }
-        if (staged) (void)hipStreamSynchronize(sv->copy_st);
         bool did = false;
         for (int i = 0; i < sv->nstreams; ++i) {  // completed launches
             Launch &L = sv->ln[i];
@@ -430,16 +420,22 @@ static void worker_main(bingo_server *sv)
         }
         busy_clock(sv);
         if (did) continue;
-        const int busy = busy_launches(sv);
-        if (busy == sv->nstreams) {  // nothing can start: sleep on the oldest launch
-            int old = 0;
-            for (int i = 1; i < sv->nstreams; ++i)
-                if (sv->ln[i].seq < sv->ln[old].seq) old = i;
-            (void)hipEventSynchronize(sv->ln[old].ev);
+        // nothing to start now: stage the inputs of the walks that have
+        // arrived (submission order, as the arena allows) while launches
+        // run, so that packing their launch waits for no copies
+        bool staged = false;
+        for (auto &r : sv->order) {
+            if (!r->walk || r->admitted) continue;
+            if (!admit(sv, *r)) break;
+            staged = true;
+        }
+        if (staged) {
+            (void)hipStreamSynchronize(sv->copy_st);
             continue;
         }
+        const int busy = busy_launches(sv);
         std::unique_lock<std::mutex> lk(sv->mu);
-        if (busy)  // poll completions while waiting for new requests
+        if (busy)  // poll completions, waking at once for a new request (to stage it)
             sv->cv_work.wait_for(lk, std::chrono::microseconds(50), [&] { return !sv->inbox.empty(); });
         else
             sv->cv_work.wait(lk, [&] { return sv->stopping || !sv->inbox.empty(); });
