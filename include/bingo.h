@@ -305,8 +305,13 @@ int bingo_event_elapsed(bingo_graph *g, int slot_a, int slot_b, float *ms);
  * the current one runs, so work that arrives while others run starts in the
  * next launch and the device never idles between launches.  A walk's paths are those it
  * would get alone (walker i of a request draws from Philox(seed; i, step,
- * attempt), whatever chunk runs it).  There is no reference counterpart: the
- * reference runs one walk kernel at a time (include/graph.cuh:93-150). */
+ * attempt), whatever chunk runs it).  An update batch of up to 2^17 inserts
+ * and 2^17 deletes is applied from captured graphs (bingo_update_graph_*),
+ * for which creating the server commits 1 GiB of pool room (when the budget
+ * allows; otherwise, and for larger batches or once that room is used up,
+ * batches take bingo_insert / bingo_delete); the results are the same.
+ * There is no reference counterpart: the reference runs one walk kernel at a
+ * time (include/graph.cuh:93-150). */
 typedef struct bingo_server bingo_server;
 typedef struct {
     uint64_t arena_bytes;      /* device arena for request staging (HBM admission budget) */
