@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -53,7 +54,7 @@ constexpr int kMaxStreams = 4;
 // creation; larger batches, and a batch past the room, take bingo_insert /
 // bingo_delete (which grow the pools)
 constexpr int64_t kCapturedMax = 1 << 17;
-constexpr uint64_t kCapturedRoom = 1ull << 30;
+constexpr uint64_t kCapturedRoom = 1ull << 30;  // (test hook: BINGO_SERVE_ROOM bytes)
 
 struct Req {
     uint64_t ticket = 0;
@@ -522,10 +523,11 @@ int bingo_server_create(bingo_graph *g, const bingo_server_config *c, bingo_serv
         release_server(sv);
         return fail(BINGO_ENOMEM, "server arena, stream or table allocation failed");
     }
+    uint64_t room = kCapturedRoom;
+    if (const char *e = std::getenv("BINGO_SERVE_ROOM")) room = std::max<uint64_t>(1, std::strtoull(e, nullptr, 0));
     if (hipMalloc((void **)&sv->d_ins, kCapturedMax * sizeof(bingo_edge)) == hipSuccess &&
         hipMalloc((void **)&sv->d_dels, kCapturedMax * sizeof(bingo_del)) == hipSuccess &&
-        bingo_update_graph_create(g, sv->d_ins, kCapturedMax, sv->d_dels, kCapturedMax, kCapturedRoom, &sv->ug) !=
-            BINGO_OK)
+        bingo_update_graph_create(g, sv->d_ins, kCapturedMax, sv->d_dels, kCapturedMax, room, &sv->ug) != BINGO_OK)
         sv->ug = nullptr;  // update batches then all take bingo_insert / bingo_delete
     (void)hipGetLastError();
     sv->worker = std::thread(worker_main, sv);

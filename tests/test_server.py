@@ -157,3 +157,77 @@ def test_single_class_launches_equal_alone(orc, apps):
     assert srv.stats()["walk_requests"] == len(reqs)
     srv.close()
     g.close()
+
+
+def test_served_update_paths_equal_alone(orc, monkeypatch):
+    """Served update batches replay the captured graphs when they fit, and
+    take bingo_insert / bingo_delete past the captured size (2^17 arcs a
+    side) or past the committed room (4 MiB here, through the server's test
+    hook); an invalid batch fails its own ticket only.  Every walk returns
+    what it returns alone after the same updates, and the final store equals
+    the oracle's."""
+    monkeypatch.setenv("BINGO_SERVE_ROOM", str(4 << 20))
+    scale = 10
+    n = 1 << scale
+    base = B.gen_rmat(scale, 4000, B.W_LOGUNI32, seed=21)
+    gen = B.UpdateGen(n, scale, base, 4000, B.W_LOGUNI32, seed=21)
+    ups = {
+        "small": (gen.inserts(300), gen.deletes(200)),
+        "past_room": (gen.inserts(100_000), gen.deletes(500)),  # ~13 MB of new slots and members
+        "invalid": (np.zeros(0, B.EDGE_DT), np.array([(0, 0)], B.DEL_DT)),  # node 0
+        "past_size": (gen.inserts((1 << 17) + 5), gen.deletes(100)),
+        "after": (gen.inserts(200), gen.deletes(300)),
+    }
+    seq = []
+    for i, name in enumerate(ups):
+        for app in (B.DEEPWALK, B.NODE2VEC):
+            seq.append(("walk", app, B.walk_params(seed=50 + 2 * i + app, max_len=20), 500))
+        seq.append(("update", name))
+    seq.append(("walk", B.NODE2VEC, B.walk_params(seed=99, max_len=20), 500))
+
+    def graph():
+        g = B.Graph(neighbor_index=True, pool_slack=0.0)
+        g.load(n, base)
+        return g
+
+    g1 = graph()  # each request alone, in order
+    o = orc.OracleGraph(n, True)
+    assert o.load(base) == 0
+    alone = []
+    for r in seq:
+        if r[0] == "walk":
+            alone.append(g1.walk(r[1], r[2], nwalkers=r[3]))
+            continue
+        ins, dels = ups[r[1]]
+        if r[1] == "invalid":
+            with pytest.raises(B.BingoError):
+                g1.delete(dels)
+        else:
+            g1.insert(ins)
+            g1.delete(dels)
+            assert o.insert(ins) == 0 and o.delete(dels) == 0
+        alone.append(None)
+    g1.close()
+
+    g2 = graph()
+    srv = B.Server(g2, max_launch_walkers=4000)
+    tickets = []
+    for r in seq:
+        if r[0] == "walk":
+            tickets.append(srv.submit_walk(r[1], r[2], nwalkers=r[3]))
+        else:
+            tickets.append(srv.submit_update(*ups[r[1]]))
+    for i, (r, t) in enumerate(zip(seq, tickets)):
+        if r[0] == "update" and r[1] == "invalid":
+            with pytest.raises(B.BingoError) as ei:
+                srv.wait(t)
+            assert ei.value.code == B.EINVAL
+            continue
+        out, steps, _ = srv.wait(t)
+        if r[0] == "walk":
+            assert np.array_equal(out, alone[i][0]) and steps == alone[i][1], i
+    srv.close()
+    got, want = g2.export(), o.export()
+    for k in want:
+        assert np.array_equal(got[k], want[k]), k
+    g2.close()
