@@ -73,13 +73,14 @@ CONFIGS = {
                graph=True),
     # 2^30 R-MAT arcs (one arc per generated edge), store fitted to the graph
     # at load, sparse/dense groups at the reference's default RATE 0.4;
-    # sliding-window churn: 10^7 new arcs in, the 10^7 oldest out; pools as
-    # whole allocations with 6 % room beyond the load (the window's update
-    # kernels run faster on them than on pools mapped for growth in place)
+    # sliding-window churn: 10^7 new arcs in, the 10^7 oldest out; each node
+    # fitted with 1/8 room for its arcs, pools as whole allocations with 6 %
+    # room beyond the load: the window then runs without any pool growth (and
+    # its update kernels run faster than on pools mapped for growth in place)
     "c5": dict(workload="rmat25_a30_loguni32_deepwalk_w32e6_churn1e7", scale=25, edges=1 << 30, wkind=2, app=0,
                max_len=80, batch=10_000_000, p=0.5, q=2.0, tp=0.0125, walkers=32_000_000, lens=None, graph=False,
                directed=True, fit=True, churn="expire", cpu_sample_edges=1 << 26, grow_in_place=False,
-               pool_slack=0.06, compact=1 / 16, dense_rate=0.4),
+               fit_headroom=1 / 8, pool_slack=0.06, compact=1 / 16, dense_rate=0.4),
 }
 
 
@@ -433,6 +434,8 @@ def main():
                          "default: the config's (captured where its walks are)")
     ap.add_argument("--pool-slack", type=float, default=None,
                     help="pool capacity beyond the load's need (fraction; default: the workload's)")
+    ap.add_argument("--fit-headroom", type=float, default=None,
+                    help="fitted stores: per-node room at load (fraction of its arcs; default: the workload's)")
     ap.add_argument("--grow-in-place", type=int, choices=(0, 1), default=None,
                     help="pools grow by mapping more HBM (1) or by copying (0); default: the config's choice")
     ap.add_argument("--compact", type=float, default=None, metavar="FRAC",
@@ -484,9 +487,12 @@ def main():
     t0 = time.perf_counter()
     base = B.gen_rmat(scale, m, cfg["wkind"], seed=args.seed, undirected=not cfg.get("directed"))
     log(f"generated {len(base)} arcs in {time.perf_counter() - t0:.1f}s")
-    fit = dict(fit_load=True, fit_headroom=1.0 / 16, pool_slack=cfg.get("pool_slack", 0.03)) if cfg.get("fit") else {}
+    fit = dict(fit_load=True, fit_headroom=cfg.get("fit_headroom", 1.0 / 16),
+               pool_slack=cfg.get("pool_slack", 0.03)) if cfg.get("fit") else {}
     if args.pool_slack is not None:
         fit["pool_slack"] = args.pool_slack
+    if args.fit_headroom is not None and fit:
+        fit["fit_headroom"] = args.fit_headroom
     gip = cfg.get("grow_in_place") if args.grow_in_place is None else bool(args.grow_in_place)
     g = B.Graph(device=0, neighbor_index=app == B.NODE2VEC, float_weights=bool(cfg.get("float")),
                 grow_in_place=gip, dense_rate=cfg.get("dense_rate", 0.0), **fit)
@@ -653,7 +659,7 @@ def main():
                    "deletes": "oldest arcs (sliding window)" if cfg.get("churn") == "expire" else "uniform live arcs",
                    "store": "fitted at load" if cfg.get("fit") else "power-of-two regions",
                    "pools": "grown in place" if gip else "grown by copy",
-                   "pool_slack": fit.get("pool_slack"),
+                   "pool_slack": fit.get("pool_slack"), "fit_headroom": fit.get("fit_headroom"),
                    "weights": "float32 w/256" if cfg.get("float") else "u32",
                    "dense_rate": cfg.get("dense_rate", 0.0)},
         "walk_steps_per_s": walk_rate,
